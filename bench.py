@@ -1,0 +1,301 @@
+"""Benchmark of the local DVC step: 12-DOF IC-GN throughput on a 512^3 bead pair.
+
+Workload (the batched IC-GN configuration at scale): a 512^3 float32 pair of 250,000 Gaussian
+beads (sigma 1.5 voxel, amplitude U[0.5, 1], centres uniform, seed 0) plus N(0, 0.01) noise; the
+deformed volume re-renders the beads at the centres moved by 2 % uniaxial strain along z and a
+5 degree rotation about z through the volume centre. Subsets 32^3 (33^3 = 35,937 sampled voxels),
+step 8: 61^3 = 226,981 nodes; cubic interpolation, icgn_tol 1e-2, dp_tol 1e-3, max_iter 100,
+patience 5, predictive stop on; each node starts from round(u_imposed) with F = 0. Volumes,
+gradients and the per-node precompute are resident in HBM before the timed region; one step is
+one IC-GN solve of every node.
+
+Prints ONE JSON line (rank 0, N = 1): nodes/s of the fused kernel, the plain (reference-design)
+kernel's nodes/s beside it, the HBM roofline of the fused kernel measured with HIP events on its
+stream, and the float64 CPU restatement timed on the host cores on a bounded node sample.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+METRIC = "IC-GN nodes/s at 512³ (batched vs plain GPU, ≤0.01 vox err); HBM GB/s"
+HBM_PEAK_GBS = 8000.0
+SHAPE = (512, 512, 512)
+N_BEADS = 250_000
+SIGMA = 1.5
+NOISE = 0.01
+WINSIZE = 32
+STEP = 8
+HALF = WINSIZE // 2
+TRAFFIC_FILE = os.path.join(HERE, "profiles", "icgn12_fused_512_traffic.json")
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def render_torch(shape, centres, amps, sigma, dev, chunk=16384):
+    """Render Gaussian beads on the GPU (float64 accumulation, float32 result)."""
+    import torch
+
+    nz, ny, nx = shape
+    r = int(math.ceil(3.0 * sigma))
+    off = torch.arange(-r, r + 1, device=dev)
+    oz, oy, ox = torch.meshgrid(off, off, off, indexing="ij")
+    oz, oy, ox = oz.reshape(1, -1), oy.reshape(1, -1), ox.reshape(1, -1)
+    vol = torch.zeros(nz * ny * nx, dtype=torch.float64, device=dev)
+    inv = 1.0 / (2.0 * sigma * sigma)
+    C = torch.from_numpy(centres).to(dev)
+    A = torch.from_numpy(amps).to(dev)
+    for s in range(0, C.shape[0], chunk):
+        c = C[s : s + chunk]
+        a = A[s : s + chunk]
+        b = torch.round(c).long()
+        bx, by, bz = b[:, 0:1] + ox, b[:, 1:2] + oy, b[:, 2:3] + oz
+        d2 = (bx - c[:, 0:1]) ** 2 + (by - c[:, 1:2]) ** 2 + (bz - c[:, 2:3]) ** 2
+        val = a[:, None] * torch.exp(-d2 * inv)
+        ok = (bx >= 0) & (bx < nx) & (by >= 0) & (by < ny) & (bz >= 0) & (bz < nz)
+        lin = (bz * ny + by) * nx + bx
+        vol.index_add_(0, lin[ok], val[ok])
+    return vol.reshape(shape).float()
+
+
+def normalize_torch(v):
+    import torch
+
+    m = v.double().mean()
+    s = v.double().std(unbiased=False)
+    s = s if float(s) >= 1e-12 else torch.ones_like(s)
+    return ((v.double() - m) / s).float().contiguous()
+
+
+def make_workload(dev):
+    import torch
+
+    from pyaldvc_amd import kernels as K
+    from pyaldvc_amd import synthetic as syn
+
+    t0 = time.time()
+    beads = syn.random_beads(SHAPE, N_BEADS, SIGMA, seed=0)
+    centre = tuple((s - 1) / 2 for s in SHAPE[::-1])
+    F = syn.strain_rotation_F(0.02, 5.0)
+    disp = syn.affine(F, centre)
+    moved = syn.moved(beads, disp)
+    gen = torch.Generator(device=dev)
+    f = render_torch(SHAPE, beads.centres, beads.amps, SIGMA, dev)
+    gen.manual_seed(1)
+    f += NOISE * torch.randn(SHAPE, device=dev, generator=gen)
+    g = render_torch(SHAPE, moved.centres, moved.amps, SIGMA, dev)
+    gen.manual_seed(2)
+    g += NOISE * torch.randn(SHAPE, device=dev, generator=gen)
+    f = normalize_torch(f)
+    g = normalize_torch(g)
+    gx, gy, gz = K.compute_gradients(f)
+    coords = syn.node_lattice(SHAPE, (WINSIZE,) * 3, (STEP,) * 3)
+    u, v, w = disp(coords[:, 0].astype(float), coords[:, 1].astype(float), coords[:, 2].astype(float))
+    U_true = np.column_stack([u, v, w])
+    torch.cuda.synchronize()
+    log(f"volumes rendered and gradients computed in {time.time() - t0:.1f}s ({coords.shape[0]} nodes)")
+    return {"f": f, "g": g, "gx": gx, "gy": gy, "gz": gz, "coords": coords, "U_true": U_true, "F": F}
+
+
+def timed_steps(fn, steps, warmup):
+    """Per-step times (s) from HIP events on the stream the kernels are launched on."""
+    import torch
+
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    s = torch.cuda.current_stream()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    t0 = time.perf_counter()
+    for a, b in evs:
+        a.record(s)
+        out = fn()
+        b.record(s)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return [a.elapsed_time(b) / 1e3 for a, b in evs], wall, out
+
+
+def cpu_baseline(W, pre, P0, target_s=15.0):
+    """The float64 CPU restatement (OpenMP, every host core) on a bounded sample of valid nodes."""
+    import torch
+
+    from oracle import cpu_ref
+
+    f = W["f"].cpu().numpy()
+    g = W["g"].cpu().numpy()
+    gx, gy, gz = (t.cpu().numpy() for t in (W["gx"], W["gy"], W["gz"]))
+    valid = pre["valid"].cpu().numpy()
+    idx = np.flatnonzero(valid)
+    rng = np.random.default_rng(0)
+    nomask = np.ones((1, 1, 1), np.uint8)
+    threads = cpu_ref.num_threads()
+
+    def run(sample):
+        c = W["coords"][sample]
+        H, L, mf, bf, nv, vd = cpu_ref.precompute_nodes(c, HALF, HALF, HALF, f, gx, gy, gz, nomask, 0.5, 1e12)
+        t0 = time.perf_counter()
+        out = cpu_ref.icgn_12dof_parallel(c, P0[sample], HALF, HALF, HALF, f, gx, gy, gz, nomask, g, 1, L, mf, bf, vd,
+                                          1e-2, 1e-3, 100, 5)  # fmt: skip
+        return time.perf_counter() - t0, int(vd.sum()), out
+
+    probe = np.sort(rng.choice(idx, size=min(idx.size, 4 * threads), replace=False))
+    t_probe, n_probe, _ = run(probe)
+    per_node = t_probe / max(n_probe, 1)
+    n = int(min(idx.size, max(n_probe, target_s / max(per_node, 1e-6))))
+    sample = np.sort(rng.choice(idx, size=n, replace=False))
+    t, nv, out = run(sample)
+    del torch
+    return {
+        "value": nv / t,
+        "unit": "nodes/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{nv} valid nodes drawn at random (seed 0) from the same 226,981-node workload, {t:.1f}s",
+        "cpu": _cpu_model(),
+        "sample_nodes": sample,
+        "sample_out": out,
+    }
+
+
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--plain-steps", type=int, default=2, help="timed steps of the plain kernel (0 skips it)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: the local step shards by node blocks with no exchange, "
+              f"and multi-GPU runs are not part of this build (got --gpus {args.gpus})", file=sys.stderr)  # fmt: skip
+        sys.exit(2)
+
+    import torch
+
+    from pyaldvc_amd import kernels as K
+
+    dev = K.device()
+    W = make_workload(dev)
+    cd = torch.from_numpy(W["coords"]).to(dev)
+    N = int(cd.shape[0])
+    nomask = np.ones((1, 1, 1), np.uint8)
+    t0 = time.time()
+    H, L, mf, bf, nv, valid = K.precompute_nodes(cd, HALF, HALF, HALF, W["f"], W["gx"], W["gy"], W["gz"], nomask, 0.5, 1e12)
+    torch.cuda.synchronize()
+    pre = {"L": L, "meanf": mf, "bottomf": bf, "valid": valid, "n_valid": nv}
+    n_valid = int(valid.sum())
+    log(f"precompute {time.time() - t0:.2f}s, {n_valid}/{N} valid nodes")
+    P0 = np.zeros((N, 12))
+    P0[:, 9:] = np.round(W["U_true"])
+    P0d = torch.from_numpy(P0).to(dev)
+
+    def step(variant):
+        return K.icgn_12dof_parallel(cd, P0d, HALF, HALF, HALF, W["f"], W["gx"], W["gy"], W["gz"], nomask, W["g"], 1,
+                                     L, mf, bf, valid, 1e-2, 1e-3, 100, 5, variant=variant)  # fmt: skip
+
+    times, wall, out = timed_steps(lambda: step("fused"), args.steps, args.warmup)
+    t_step = float(np.mean(times))
+    P, n_iter, status, zncc = (x.cpu().numpy() for x in out)
+    conv = status == 0
+    err = np.linalg.norm(P[conv, 9:] - W["U_true"][conv], axis=1)
+    S = (2 * HALF + 1) ** 3
+    iters = int(n_iter[valid.cpu().numpy()].sum())
+    bytes_alg = 20.0 * S * iters  # f + 3 gradients + 1 deformed sample per sampled voxel-iteration
+    achieved = bytes_alg / t_step / 1e9
+    log(f"fused: {t_step * 1e3:.1f} ms/step, {n_valid / t_step:,.0f} nodes/s, {int(conv.sum())} converged, "
+        f"mean iterations {iters / max(n_valid, 1):.2f}, median |err| {np.median(err):.4f} vox")  # fmt: skip
+
+    plain = None
+    if args.plain_steps > 0:
+        tp, _, outp = timed_steps(lambda: step("plain"), args.plain_steps, 1)
+        t_plain = float(np.mean(tp))
+        stp = outp[2].cpu().numpy()
+        plain = {"ms_per_step": t_plain * 1e3, "nodes_per_s": n_valid / t_plain, "status_equal_fused": float(np.mean(stp == status))}
+        log(f"plain: {t_plain * 1e3:.1f} ms/step, {n_valid / t_plain:,.0f} nodes/s")
+
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as fh:
+            traffic = json.load(fh).get("hbm_bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu:
+        t0 = time.time()
+        cpu = cpu_baseline(W, pre, P0, args.cpu_seconds)
+        sample = cpu.pop("sample_nodes")
+        Pc, itc, stc, zc = cpu.pop("sample_out")
+        agree = float(np.mean(stc == status[sample]))
+        okc = (stc == 0) & (status[sample] == 0)
+        du = float(np.max(np.abs(Pc[okc, 9:] - P[sample][okc, 9:]))) if okc.any() else float("nan")
+        cpu["gpu_agreement"] = {"status_equal": agree, "max_du_vox": du}
+        log(f"cpu baseline: {cpu['value']:.1f} nodes/s on {cpu['cores']} threads ({time.time() - t0:.1f}s); "
+            f"status agreement {agree:.4f}, max |du| {du:.2e}")  # fmt: skip
+
+    result = {
+        "metric": METRIC,
+        "value": n_valid / t_step,
+        "unit": "nodes/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_step * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "512^3 f32 bead pair (250k beads sigma 1.5, noise 0.01), 2% z strain + 5 deg z rotation; "
+            "subset 32^3, step 8, 61^3 nodes; fused 12-DOF IC-GN, cubic",
+            "nodes": N,
+            "nodes_valid": n_valid,
+            "nodes_converged": int(conv.sum()),
+            "mean_iterations": iters / max(n_valid, 1),
+            "median_err_vox": float(np.median(err)) if err.size else None,
+            "p95_err_vox": float(np.percentile(err, 95)) if err.size else None,
+            "plain_kernel": plain,
+            "speedup_vs_plain": (plain["ms_per_step"] / (t_step * 1e3)) if plain else None,
+            "speedup_vs_cpu": ((n_valid / t_step) / cpu["value"]) if cpu else None,
+            "wall_s_timed": wall,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "algorithmic_bytes_per_launch": bytes_alg,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,126 @@
+/*
+ * aldvc_hip.h -- C ABI of libaldvc_hip.so, the gfx950 (MI355X) local DVC step.
+ *
+ * Every entry point takes DEVICE pointers (caller-owned memory, e.g. torch tensors or
+ * hipMalloc'd buffers), plain sizes and a hipStream_t; it enqueues its kernels on that stream
+ * and returns 0, or a negative code with a message for aldvc_last_error(): -2 bad argument,
+ * -3 workspace too small, -4 a copy failed, -5 a launch failed. Nothing synchronises the
+ * stream. Per-node failures are status codes in the outputs, never errors.
+ *
+ * Layouts (reference al_dvc 1.2.1, core/data_structures.py:1-11, solver/numba_kernels.py:9-16):
+ *   volumes     (nz, ny, nx) C-order float32, all with the shape of f
+ *   coords      (N, 3) int64 node centres [x, y, z]
+ *   P           (N, 12) float64 = [F row-major (9), u, v, w]
+ *   H, L        (N, 12, 12) float64; pattern (12, 12) float64
+ *   status      int8 codes: 0 converged, 1 max_iter, 2 out_of_bounds, 3 invalid_subset,
+ *               4 singular, 5 nan, 6 skipped, 7 stalled (numba_kernels.py:29-36)
+ *   gx/gy/gz == NULL  gradients evaluated on the fly by the 7-point stencil
+ *   mask == NULL      every voxel counts (the reference's 1x1x1 placeholder)
+ *   split_index == NULL  whole subsets; else row r >= 0 of split_keep (keep_bytes per row,
+ *               bit i = sampled voxel i in (dz, dy, dx) order) restricts node n's subset
+ *   interp mode 0 linear, 1 Keys cubic (a = -0.5), 2 cubic B-spline on prefiltered coefficients
+ */
+#ifndef ALDVC_HIP_H
+#define ALDVC_HIP_H
+
+#include <hip/hip_runtime_api.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ABI version of this header (1). */
+int aldvc_abi_version(void);
+
+/* Message of the last failed call on this thread ("" after a successful one). */
+const char *aldvc_last_error(void);
+
+/* 7-point central-difference gradients, zero within `border` voxels of any face.
+ * Replaces io/volume_ops.py:124-197 (_gradient_stencil7 / compute_gradients). */
+int aldvc_gradients(const float *f, int64_t nz, int64_t ny, int64_t nx, int border, float *gx, float *gy,
+                    float *gz, hipStream_t stream);
+
+/* Interpolate `vol` at n points (float64 coordinates, float32 result); NaN outside the
+ * admissible domain 1 <= c <= n-3. Replaces solver/interp_kernels.py:139-152 (sample_points). */
+int aldvc_sample_points(const float *vol, int64_t nz, int64_t ny, int64_t nx, const double *zs,
+                        const double *ys, const double *xs, int64_t n, int mode, float *out,
+                        hipStream_t stream);
+
+/* Per-node Hessian, Cholesky factor, ZNSSD normalisation and validity.
+ * Replaces solver/numba_kernels.py:499-509 (precompute_nodes) and
+ * solver/cuda_kernels.py:1635-1710 (precompute_nodes_cuda). Outputs: H, L (N,12,12), meanf,
+ * bottomf (N) float64, n_valid (N) int64, valid (N) uint8. */
+int aldvc_precompute(const int64_t *coords, int64_t N, int hx, int hy, int hz, int stride, const float *f,
+                     const float *gx, const float *gy, const float *gz, const uint8_t *mask, int64_t nz,
+                     int64_t ny, int64_t nx, double min_valid_ratio, double cond_max,
+                     const int64_t *split_index, const uint8_t *split_keep, int64_t keep_bytes, double *H,
+                     double *L, double *meanf, double *bottomf, int64_t *n_valid, uint8_t *valid,
+                     hipStream_t stream);
+
+/* Workspace bytes aldvc_icgn12 needs for N nodes (variant 0 adds the 2048-node scratch rows). */
+size_t aldvc_icgn12_workspace_bytes(int64_t N, int hx, int hy, int hz, int stride, int variant);
+
+/* 12-DOF inverse-compositional Gauss-Newton on the ZNSSD criterion over all valid nodes; P is
+ * updated in place, n_iter/status/zncc are written for every node.
+ * Replaces solver/numba_kernels.py:1211-1278 (icgn_12dof_parallel) and
+ * solver/cuda_kernels.py:1397-1515 (icgn_12dof_cuda). H and pattern (noise-corrected Hessian,
+ * used when noise_gain > 0) may be NULL; meanf (N) is the accumulation pilot of variant 1 and
+ * may be NULL. variant 0 = plain (one 256-lane workgroup per node, two passes through scratch
+ * rows, the reference GPU kernel's design), 1 = fused one-pass kernel. */
+int aldvc_icgn12(const int64_t *coords, int64_t N, double *P, int hx, int hy, int hz, int stride,
+                 const float *f, const float *gx, const float *gy, const float *gz, const uint8_t *mask,
+                 const float *g, int64_t nz, int64_t ny, int64_t nx, int mode, const double *L,
+                 const double *H, const double *pattern, double noise_gain, const double *meanf,
+                 const uint8_t *valid, double tol, double dp_tol, int max_iter, int patience, int predictive,
+                 const int64_t *split_index, const uint8_t *split_keep, int64_t keep_bytes, int32_t *n_iter,
+                 int8_t *status, double *zncc, int variant, void *workspace, size_t workspace_bytes,
+                 hipStream_t stream);
+
+/* Workspace bytes aldvc_icgn3 needs for N nodes. */
+size_t aldvc_icgn3_workspace_bytes(int64_t N);
+
+/* ADMM subproblem 1: translation-only IC-GN with the proximal term mu (u - u_hat - v), F held
+ * fixed. U receives the result (U_old is copied into it first).
+ * Replaces solver/numba_kernels.py:1281-1346 (icgn_3dof_parallel) and
+ * solver/cuda_kernels.py:1518-1632 (icgn_3dof_cuda). U_old, U, vdual (N,3); F_fixed (N,3,3). */
+int aldvc_icgn3(const int64_t *coords, int64_t N, const double *U_old, double *U, const double *F_fixed,
+                const double *vdual, int hx, int hy, int hz, int stride, const float *f, const float *gx,
+                const float *gy, const float *gz, const uint8_t *mask, const float *g, int64_t nz, int64_t ny,
+                int64_t nx, int mode, const double *H, const double *meanf, const double *bottomf,
+                const uint8_t *valid, double mu, double tol, double dp_tol, int max_iter, int patience,
+                double noise_gain, int predictive, const int64_t *split_index, const uint8_t *split_keep,
+                int64_t keep_bytes, int32_t *n_iter, int8_t *status, double *zncc, void *workspace,
+                size_t workspace_bytes, hipStream_t stream);
+
+/* ZNCC of each valid node's subset under the warp P_all (NaN where not sampled).
+ * Replaces solver/numba_kernels.py:1354-1387 (evaluate_zncc_parallel). */
+int aldvc_evaluate_zncc(const int64_t *coords, int64_t N, const double *P_all, int hx, int hy, int hz,
+                        int stride, const float *f, const uint8_t *mask, const float *g, int64_t nz, int64_t ny,
+                        int64_t nx, int mode, const uint8_t *valid, double *out, hipStream_t stream);
+
+/* Workspace bytes aldvc_zncc_map needs (0 for the float32 engine). */
+size_t aldvc_zncc_map_workspace_bytes(int64_t n, int hx, int hy, int hz, int rx, int ry, int rz, int precision);
+
+/* ZNCC maps out (n, 2rz+1, 2ry+1, 2rx+1) float32 of the (2h+1)^3 template around each node
+ * against the search window starting at (wx0, wy0, wz0); -2 for t_ok == 0 or a flat template.
+ * Replaces solver/integer_search.py:44-151 (_zncc_direct) and solver/cuda_kernels.py:1713-1755
+ * (zncc_direct_cuda). precision 0 = float32. */
+int aldvc_zncc_map(const float *f, const float *g, int64_t nz, int64_t ny, int64_t nx, const int64_t *coords,
+                   int64_t n, int hx, int hy, int hz, int rx, int ry, int rz, const int64_t *wx0,
+                   const int64_t *wy0, const int64_t *wz0, const uint8_t *t_ok, float *out, int precision,
+                   void *workspace, size_t workspace_bytes, hipStream_t stream);
+
+/* Peak statistics of ZNCC maps: first-index argmax (C order), peak value, PCE and the 27-point
+ * quadratic sub-voxel offset (x, y, z) for interior peaks; pinv is the (10, 27) pseudo-inverse
+ * of the quadratic design matrix. Replaces the per-node peak logic of
+ * solver/integer_search.py:444-477 (with _subvoxel_peak, :297-321). */
+int aldvc_ncc_peaks(const float *maps, int64_t n, int rx, int ry, int rz, const double *pinv, int64_t *peak_idx,
+                    double *peak_val, double *pce, double *sub, hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ALDVC_HIP_H */

@@ -1,0 +1,113 @@
+"""ctypes binding of ``libaldvc_hip.so``, the C ABI of the HIP kernels.
+
+torch is imported first on purpose: it loads its HIP runtime (``libamdhip64.so.7``), and the
+library then resolves to that same runtime, so device pointers and streams are shared with
+torch tensors. The library is required: there is no CPU fallback, and a missing or stale
+build raises :class:`HipLibraryError`.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+import torch  # noqa: F401  (must precede the CDLL load, see the module docstring)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libaldvc_hip.so")
+ABI_VERSION = 1
+
+P = ctypes.c_void_p
+I64 = ctypes.c_int64
+I32 = ctypes.c_int
+F64 = ctypes.c_double
+SZ = ctypes.c_size_t
+
+# name -> (restype, argtypes); mirrors include/aldvc_hip.h
+SIGNATURES = {
+    "aldvc_abi_version": (I32, []),
+    "aldvc_last_error": (ctypes.c_char_p, []),
+    "aldvc_gradients": (I32, [P, I64, I64, I64, I32, P, P, P, P]),
+    "aldvc_sample_points": (I32, [P, I64, I64, I64, P, P, P, I64, I32, P, P]),
+    "aldvc_precompute": (
+        I32,
+        [P, I64, I32, I32, I32, I32, P, P, P, P, P, I64, I64, I64, F64, F64, P, P, I64, P, P, P, P, P, P, P],
+    ),
+    "aldvc_icgn12_workspace_bytes": (SZ, [I64, I32, I32, I32, I32, I32]),
+    "aldvc_icgn12": (
+        I32,
+        [
+            P, I64, P, I32, I32, I32, I32,  # coords, N, P, hx, hy, hz, stride
+            P, P, P, P, P, P, I64, I64, I64, I32,  # f, gx, gy, gz, mask, g, nz, ny, nx, mode
+            P, P, P, F64, P, P,  # L, H, pattern, noise_gain, meanf, valid
+            F64, F64, I32, I32, I32,  # tol, dp_tol, max_iter, patience, predictive
+            P, P, I64,  # split_index, split_keep, keep_bytes
+            P, P, P, I32, P, SZ, P,  # n_iter, status, zncc, variant, workspace, bytes, stream
+        ],
+    ),  # fmt: skip
+    "aldvc_icgn3_workspace_bytes": (SZ, [I64]),
+    "aldvc_icgn3": (
+        I32,
+        [
+            P, I64, P, P, P, P, I32, I32, I32, I32,  # coords, N, U_old, U, F, vdual, hx, hy, hz, stride
+            P, P, P, P, P, P, I64, I64, I64, I32,  # f, gx, gy, gz, mask, g, nz, ny, nx, mode
+            P, P, P, P,  # H, meanf, bottomf, valid
+            F64, F64, F64, I32, I32, F64, I32,  # mu, tol, dp_tol, max_iter, patience, noise_gain, predictive
+            P, P, I64,  # split
+            P, P, P, P, SZ, P,  # n_iter, status, zncc, workspace, bytes, stream
+        ],
+    ),  # fmt: skip
+    "aldvc_evaluate_zncc": (
+        I32,
+        [P, I64, P, I32, I32, I32, I32, P, P, P, I64, I64, I64, I32, P, P, P],
+    ),
+    "aldvc_zncc_map": (
+        I32,
+        [P, P, I64, I64, I64, P, I64, I32, I32, I32, I32, I32, I32, P, P, P, P, P, I32, P, SZ, P],
+    ),
+    "aldvc_zncc_map_workspace_bytes": (SZ, [I64, I32, I32, I32, I32, I32, I32, I32]),
+    "aldvc_ncc_peaks": (I32, [P, I64, I32, I32, I32, P, P, P, P, P, P]),
+}
+
+
+class HipLibraryError(RuntimeError):
+    """The HIP library is missing, stale, or a call into it failed."""
+
+
+_lib = None
+_lock = threading.Lock()
+
+
+def lib():
+    """The loaded library (raises :class:`HipLibraryError` when it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise HipLibraryError(
+                f"{LIB_PATH} is not built; run `python -m pyaldvc_amd.build` (or __graft_entry__.build())"
+            )
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name, None)
+            if fn is None:
+                raise HipLibraryError(f"{LIB_PATH} does not export {name}; rebuild it")
+            fn.restype = res
+            fn.argtypes = args
+        v = L.aldvc_abi_version()
+        if v != ABI_VERSION:
+            raise HipLibraryError(f"ABI version {v} of {LIB_PATH} != {ABI_VERSION} expected; rebuild it")
+        _lib = L
+    return _lib
+
+
+def check(rc: int, what: str) -> None:
+    """Raise with the library's message when a call returned a nonzero code."""
+    if rc != 0:
+        msg = lib().aldvc_last_error()
+        msg = msg.decode() if msg else ""
+        raise HipLibraryError(f"{what} failed ({rc}): {msg}")

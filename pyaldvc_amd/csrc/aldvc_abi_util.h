@@ -1,0 +1,37 @@
+// Error reporting shared by the C entry points: every call returns 0 or a negative code and
+// leaves a thread-local message for aldvc_last_error().
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#define ALDVC_ABI_VERSION 1
+
+namespace aldvc {
+
+inline std::string& err_slot() {
+    static thread_local std::string s;
+    return s;
+}
+
+inline void clear_error() { err_slot().clear(); }
+
+inline int fail(int code, const char* msg) {
+    err_slot() = msg;
+    return code;
+}
+
+inline const char* last_error_cstr() { return err_slot().c_str(); }
+
+// Status of the launches issued so far on this thread (does not synchronise).
+inline int launch_status() {
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        err_slot() = std::string("HIP launch failed: ") + hipGetErrorString(e);
+        return -5;
+    }
+    return 0;
+}
+
+}  // namespace aldvc

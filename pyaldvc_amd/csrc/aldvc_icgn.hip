@@ -1,0 +1,1267 @@
+// Local DVC step on gfx950: per-node precompute and 12-DOF / 3-DOF IC-GN subset solvers.
+//
+// Two 12-DOF kernels share the float64 control step:
+//   * icgn12_plain  -- the drop-in design of the reference GPU kernel (cuda_kernels.py:491-835):
+//                      one 256-lane workgroup per node, pass 1 samples the warped subset into a
+//                      global scratch row and reduces the ZNSSD statistics, pass 2 re-reads the
+//                      row for the residual and the 12-vector, lane 0 runs the control in float64.
+//   * icgn12_fused  -- one pass per iteration: every lane accumulates the 44 raw moments
+//                      (n, sums of f, f^2, g, g^2, fg and of SD, SD*f, SD*g) from which the
+//                      centred ZNSSD gradient follows in float64 after the reduction, so no
+//                      scratch row exists and each subset voxel is read once per iteration.
+//                      Nodes are mapped to workgroups so that the nodes in flight on one XCD
+//                      are neighbours in the node list (their subsets overlap in that XCD's L2).
+// Both keep the reference's float32 sampling / float64 control split (cuda_kernels.py:17-21).
+#include "aldvc_device.h"
+
+#include <algorithm>
+#include <mutex>
+#include <string>
+
+#include "aldvc_abi_util.h"
+
+using namespace aldvc;
+
+namespace {
+
+constexpr int BLOCK = 256;
+constexpr int NWAVES = BLOCK / 64;
+constexpr int64_t PLAIN_CHUNK = 2048;  // nodes per launch of the plain kernel (cuda_kernels.py:67)
+
+struct IcgnArgs {
+    Volumes V;
+    Subset sub;
+    const int64_t* coords;
+    double* P;             // (N, 12) in/out
+    const double* L_all;   // (N, 144)
+    const double* H_all;   // (N, 144) or nullptr
+    const double* pattern; // (144) or nullptr
+    const double* meanf;   // (N) precomputed reference mean: the accumulation pilot of the fused kernel
+    const int64_t* split_index;
+    const uint8_t* split_keep;
+    int64_t keep_bytes;
+    double noise_gain, tol, dp_tol;
+    int max_iter, patience, predictive;
+    int32_t* n_iter;
+    int8_t* status;
+    double* zncc;
+};
+
+// Per-node iteration state of the control lane (numba_kernels.py:663-677). It lives in LDS and
+// only lane 0 touches it: held in registers, the float64 control would set the register budget
+// of every lane of the sampling loops.
+struct Ctl12 {
+    double P[12], P_best[12], b[12], dP[12], y[12], work[36];
+    double H0[144], L0[144];
+    double norm_init, best_zncc, best_dp, last_dp, dp_prev, zncc;
+    int stall;
+};
+
+__device__ inline void ctl12_init(Ctl12& c, const double* P) {
+#pragma unroll 1
+    for (int k = 0; k < 12; ++k) c.P[k] = c.P_best[k] = P[k];
+    c.norm_init = -1.0;
+    c.best_zncc = -2.0;
+    c.best_dp = 1e300;
+    c.last_dp = 1e300;
+    c.dp_prev = -1.0;
+    c.zncc = NAN;
+    c.stall = 0;
+}
+
+// L L^T x = b with every array in memory and no unrolling (numba_kernels.py:106-119).
+__device__ inline void chol_solve12_mem(const double* L, const double* b, double* y, double* x) {
+#pragma unroll 1
+    for (int i = 0; i < 12; ++i) {
+        double s = b[i];
+#pragma unroll 1
+        for (int k = 0; k < i; ++k) s -= L[i * 12 + k] * y[k];
+        y[i] = s / L[i * 12 + i];
+    }
+#pragma unroll 1
+    for (int i = 11; i >= 0; --i) {
+        double s = y[i];
+#pragma unroll 1
+        for (int k = i + 1; k < 12; ++k) s -= L[k * 12 + i] * x[k];
+        x[i] = s / L[i * 12 + i];
+    }
+}
+
+// One IC-GN decision from the reduced gradient c.b (already scaled by bottomf_d) and the ZNCC:
+// numba_kernels.py:726-790. Returns -1 to continue, else the final status; c.P is updated.
+__device__ int ctl12_step(Ctl12& c, double zncc, double bottomf_d, double n_valid, const double* L, const double* H,
+                          const double* pattern, double noise_gain, double tol, double dp_tol, int patience,
+                          int predictive, double half_scale) {
+    double norm_abs = 0.0;
+#pragma unroll 1
+    for (int k = 0; k < 12; ++k) norm_abs += c.b[k] * c.b[k];
+    norm_abs = sqrt(norm_abs);
+    if (!isfinite(norm_abs)) {
+        c.zncc = NAN;
+        return ST_NAN;
+    }
+    if (c.norm_init < 0.0) c.norm_init = norm_abs;
+    const double norm_rel = c.norm_init > 1e-300 ? norm_abs / c.norm_init : 0.0;
+    c.zncc = zncc;
+    if (norm_rel < tol || norm_abs < ABS_TOL) return ST_CONVERGED;
+#pragma unroll 1
+    for (int k = 0; k < 12; ++k) c.b[k] = -c.b[k];
+    bool use_corr = false;
+    const double n_full = pattern ? pattern[9 * 12 + 9] : 0.0;
+    if (noise_gain > 0.0 && n_full > 0.0 && c.last_dp < NOISE_CORR_STEP && H) {
+        const double om = 1.0 - zncc;
+        const double s2 = (om > 0.0 ? om : 0.0) * bottomf_d * bottomf_d / n_valid;
+        const double corr = noise_gain * s2 * (n_valid / n_full);
+        use_corr = corrected_cholesky12(H, pattern, corr, c.H0, c.L0);
+    }
+    chol_solve12_mem(use_corr ? c.L0 : L, c.b, c.y, c.dP);
+    double dp_norm = 0.0;
+#pragma unroll 1
+    for (int k = 0; k < 12; ++k) {
+        const double v = k < 9 ? c.dP[k] * half_scale : c.dP[k];
+        dp_norm += v * v;
+    }
+    dp_norm = sqrt(dp_norm);
+    if (!isfinite(dp_norm)) {
+        c.zncc = NAN;
+        return ST_NAN;
+    }
+    c.last_dp = dp_norm;
+    if (dp_norm < dp_tol) return ST_CONVERGED;
+    if (predictive && c.dp_prev > 0.0 && dp_norm < PREDICT_CONTRACTION * c.dp_prev &&
+        dp_norm * dp_norm < dp_tol * c.dp_prev)
+        return compose_warp(c.P, c.dP, c.work) ? ST_CONVERGED : ST_SINGULAR;
+    c.dp_prev = dp_norm;
+    bool improved = zncc > c.best_zncc + STALL_ZNCC_EPS;
+    if (improved) {
+        c.best_zncc = zncc;
+#pragma unroll 1
+        for (int k = 0; k < 12; ++k) c.P_best[k] = c.P[k];
+    }
+    if (dp_norm < c.best_dp * STALL_STEP_DECAY) {
+        c.best_dp = dp_norm;
+        improved = true;
+    }
+    if (improved) {
+        c.stall = 0;
+    } else {
+        c.stall += 1;
+        if (patience > 0 && c.stall >= patience) {
+#pragma unroll 1
+            for (int k = 0; k < 12; ++k) c.P[k] = c.P_best[k];
+            c.zncc = c.best_zncc;
+            return ST_STALLED;
+        }
+    }
+    return compose_warp(c.P, c.dP, c.work) ? -1 : ST_SINGULAR;
+}
+
+// Warp coefficients broadcast to the sampling lanes: A = I + F (row-major) and the translation,
+// relative to the node centre (float32 stays accurate because the magnitudes are small).
+__device__ inline void put_warp(float* w, const double* P) {
+    w[0] = (float)(1.0 + P[0]);
+    w[1] = (float)P[1];
+    w[2] = (float)P[2];
+    w[3] = (float)P[3];
+    w[4] = (float)(1.0 + P[4]);
+    w[5] = (float)P[5];
+    w[6] = (float)P[6];
+    w[7] = (float)P[7];
+    w[8] = (float)(1.0 + P[8]);
+    w[9] = (float)P[9];
+    w[10] = (float)P[10];
+    w[11] = (float)P[11];
+}
+
+// Mixed-radix walk of the subset voxels v = tid, tid + BLOCK, ... in sampling order.
+struct VoxelWalk {
+    int ix, iy, iz, v;
+    int stx, sty, stz;
+    __device__ inline void start(const Subset& s, int v0) {
+        v = v0;
+        iz = v0 / (s.sy * s.sx);
+        const int rem = v0 - iz * s.sy * s.sx;
+        iy = rem / s.sx;
+        ix = rem - iy * s.sx;
+        stz = BLOCK / (s.sy * s.sx);
+        const int r2 = BLOCK - stz * s.sy * s.sx;
+        sty = r2 / s.sx;
+        stx = r2 - sty * s.sx;
+    }
+    __device__ inline void next(const Subset& s) {
+        v += BLOCK;
+        ix += stx;
+        int cy = 0;
+        if (ix >= s.sx) {
+            ix -= s.sx;
+            cy = 1;
+        }
+        iy += sty + cy;
+        int cz = 0;
+        if (iy >= s.sy) {
+            iy -= s.sy;
+            cz = 1;
+        }
+        iz += stz + cz;
+    }
+};
+
+__device__ inline bool subset_in_volume(const Volumes& V, const Subset& s, int64_t x0, int64_t y0, int64_t z0) {
+    return x0 - s.hx >= 0 && x0 + s.hx < V.nx && y0 - s.hy >= 0 && y0 + s.hy < V.ny && z0 - s.hz >= 0 &&
+           z0 + s.hz < V.nz;
+}
+
+__device__ inline const uint8_t* keep_row_of(const IcgnArgs& a, int64_t n) {
+    if (!a.split_index || !a.split_keep) return nullptr;
+    const int64_t r = a.split_index[n];
+    return r >= 0 ? a.split_keep + r * a.keep_bytes : nullptr;
+}
+
+// ---------------------------------------------------------------------------- plain kernel
+template <int MODE, bool STORED>
+__global__ __launch_bounds__(BLOCK) void k_icgn12_plain(IcgnArgs a, const int64_t* __restrict__ list,
+                                                        const int64_t* __restrict__ count, int64_t list_off,
+                                                        float* __restrict__ gbuf) {
+    const Volumes& V = a.V;
+    const Subset& s = a.sub;
+    const int tid = threadIdx.x;
+    if (list_off + (int64_t)blockIdx.x >= *count) return;
+    const int64_t n = list[list_off + blockIdx.x];
+    float* gb = gbuf + (int64_t)blockIdx.x * s.S;
+    const int64_t x0 = a.coords[3 * n], y0 = a.coords[3 * n + 1], z0 = a.coords[3 * n + 2];
+    const uint8_t* keep = keep_row_of(a, n);
+
+    __shared__ float sw[12];
+    __shared__ double st[8];
+    __shared__ int ctrl;
+    __shared__ double red[NWAVES * 13];
+    __shared__ double res[13];
+    __shared__ Ctl12 c;
+
+    if (tid == 0) {
+        bool finite = true;
+        ctl12_init(c, a.P + 12 * n);
+#pragma unroll 1
+        for (int k = 0; k < 12; ++k)
+            if (!isfinite(c.P[k])) finite = false;
+        ctrl = -1;
+        if (!finite)
+            ctrl = ST_NAN;
+        else if (!keep && !subset_in_volume(V, s, x0, y0, z0))
+            ctrl = ST_INVALID_SUBSET;  // a valid flag on a subset that leaves the volume: refuse it
+    }
+    __syncthreads();
+    const double half_scale = (double)max(s.hx, max(s.hy, s.hz));
+    const Domain dom = make_domain(V, (int)x0, (int)y0, (int)z0);
+    const float* gbase = V.g + lin(V, z0, y0, x0);
+    const int64_t obase = lin(V, z0, y0, x0);
+    const int rowp = V.nx, planep = V.nx * V.ny;
+    int it_done = 0;
+    int status = ST_MAX_ITER;
+    for (int it = 1; it <= a.max_iter; ++it) {
+        if (ctrl >= 0) break;
+        it_done = it;
+        if (tid == 0) put_warp(sw, c.P);
+        __syncthreads();
+        // ---------------- pass 1: warp, sample, statistics (cuda_kernels.py:577-674)
+        float p1[7] = {0, 0, 0, 0, 0, 0, 0};  // s1, s2, s1f, s2f, nv, nref, oob
+        VoxelWalk w;
+        w.start(s, tid);
+        for (; w.v < s.S; w.next(s)) {
+            const int dx = -s.hx + s.stride * w.ix, dy = -s.hy + s.stride * w.iy, dz = -s.hz + s.stride * w.iz;
+            const int64_t o = obase + (int64_t)dz * planep + (int64_t)dy * rowp + dx;
+            bool k;
+            if (keep) {
+                k = kept_bit(keep, w.v);
+                const int64_t xx = x0 + dx, yy = y0 + dy, zz = z0 + dz;
+                k = k && xx >= 0 && xx < V.nx && yy >= 0 && yy < V.ny && zz >= 0 && zz < V.nz;
+            } else {
+                k = !V.mask || V.mask[o] != 0;
+            }
+            if (!k) {
+                gb[w.v] = NAN;
+                continue;
+            }
+            p1[5] += 1.0f;
+            const float X = (float)dx, Y = (float)dy, Z = (float)dz;
+            const float xr = sw[0] * X + sw[1] * Y + sw[2] * Z + sw[9];
+            const float yr = sw[3] * X + sw[4] * Y + sw[5] * Z + sw[10];
+            const float zr = sw[6] * X + sw[7] * Y + sw[8] * Z + sw[11];
+            if (!inside(dom, xr, yr, zr)) {
+                p1[6] = 1.0f;
+                gb[w.v] = NAN;
+                continue;
+            }
+            const float val = sample_rel<MODE>(gbase, rowp, planep, xr, yr, zr, dom.kx, dom.ky, dom.kz);
+            gb[w.v] = val;
+            if (val != val) continue;
+            const float fv = V.f[o];
+            p1[4] += 1.0f;
+            p1[0] += val;
+            p1[1] += val * val;
+            p1[2] += fv;
+            p1[3] += fv * fv;
+        }
+        block_sum<7, NWAVES>(p1, res, red);
+        const double* r1 = res;
+        if (tid == 0) {
+            const double n_valid = r1[4], n_ref = r1[5];
+            if (r1[6] > 0.0) {
+                ctrl = ST_OUT_OF_BOUNDS;
+            } else if (n_valid < MIN_SUBSET_VOXELS || n_valid < MIN_VALID_FRACTION * n_ref) {
+                ctrl = ST_INVALID_SUBSET;
+            } else {
+                const double mg = r1[0] / n_valid;
+                double ssg = r1[1] - n_valid * mg * mg;
+                ssg = ssg < 0.0 ? 0.0 : ssg;
+                const double mf = r1[2] / n_valid;
+                double ssf = r1[3] - n_valid * mf * mf;
+                ssf = ssf < 0.0 ? 0.0 : ssf;
+                const double bg = sqrt(ssg > 1e-30 ? ssg : 1e-30), bf = sqrt(ssf > 1e-30 ? ssf : 1e-30);
+                st[0] = mf;
+                st[1] = bf;
+                st[2] = mg;
+                st[3] = bg;
+                st[4] = 1.0 / bf;
+                st[5] = 1.0 / bg;
+                st[6] = n_valid;
+            }
+        }
+        __syncthreads();
+        if (ctrl >= 0) {
+            if (tid == 0) c.zncc = NAN;
+            break;
+        }
+        // ---------------- pass 2: residual, 12-vector, ZNCC numerator (cuda_kernels.py:679-747)
+        const float mf32 = (float)st[0], mg32 = (float)st[2], ibf32 = (float)st[4], ibg32 = (float)st[5];
+        float p2[13] = {0};
+        w.start(s, tid);
+        for (; w.v < s.S; w.next(s)) {
+            const float gv = gb[w.v];
+            if (gv != gv) continue;
+            const int dx = -s.hx + s.stride * w.ix, dy = -s.hy + s.stride * w.iy, dz = -s.hz + s.stride * w.iz;
+            const int64_t o = obase + (int64_t)dz * planep + (int64_t)dy * rowp + dx;
+            const float fd = V.f[o] - mf32, gd = gv - mg32;
+            p2[12] += fd * gd;
+            const float res = fd * ibf32 - gd * ibg32;
+            float dfx, dfy, dfz;
+            grad_at<STORED>(V, o, (int)z0 + dz, (int)y0 + dy, (int)x0 + dx, dfx, dfy, dfz);
+            const float gxv = dfx * res, gyv = dfy * res, gzv = dfz * res;
+            const float X = (float)dx, Y = (float)dy, Z = (float)dz;
+            p2[0] += gxv * X;
+            p2[1] += gxv * Y;
+            p2[2] += gxv * Z;
+            p2[3] += gyv * X;
+            p2[4] += gyv * Y;
+            p2[5] += gyv * Z;
+            p2[6] += gzv * X;
+            p2[7] += gzv * Y;
+            p2[8] += gzv * Z;
+            p2[9] += gxv;
+            p2[10] += gyv;
+            p2[11] += gzv;
+        }
+        block_sum<13, NWAVES>(p2, res, red);
+        const double* r2 = res;
+        if (tid == 0) {
+#pragma unroll 1
+            for (int k = 0; k < 12; ++k) c.b[k] = r2[k] * st[1];
+            const double zncc = r2[12] * st[4] * st[5];
+            const double* Hn = a.H_all ? a.H_all + 144 * n : nullptr;
+            ctrl = ctl12_step(c, zncc, st[1], st[6], a.L_all + 144 * n, Hn, a.pattern, a.noise_gain, a.tol, a.dp_tol,
+                              a.patience, a.predictive, half_scale);
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        if (ctrl >= 0) status = ctrl;
+        for (int k = 0; k < 12; ++k) a.P[12 * n + k] = c.P[k];
+        a.n_iter[n] = it_done;
+        a.status[n] = (int8_t)status;
+        // the reference GPU kernel reports a ZNCC only for these outcomes (cuda_kernels.py:833-835)
+        a.zncc[n] = (status == ST_CONVERGED || status == ST_STALLED || status == ST_MAX_ITER) ? c.zncc : NAN;
+    }
+}
+
+// ---------------------------------------------------------------------------- fused kernel
+// Moment layout of the one-pass accumulation (per lane float32, reduced in float64):
+//   0 n_valid, 1 n_ref, 2 oob, 3 sum f', 4 sum f'^2, 5 sum g', 6 sum g'^2, 7 sum f'g',
+//   8 + 4c + j   : sum grad_c * q_j          (C)
+//   20 + 4c + j  : sum grad_c * q_j * f'     (A)
+//   32 + 4c + j  : sum grad_c * q_j * g'     (B)
+// with q = (X, Y, Z, 1) and f' = f - pf, g' = g - pg (pilot shifts that keep float32 sums centred).
+constexpr int NM = 44;
+
+template <int MODE, bool STORED>
+__global__ __launch_bounds__(BLOCK) void k_icgn12_fused(IcgnArgs a, const int64_t* __restrict__ list,
+                                                        const int64_t* __restrict__ count) {
+    const Volumes& V = a.V;
+    const Subset& s = a.sub;
+    const int tid = threadIdx.x;
+    // XCD-aware mapping: workgroup b runs on XCD b % 8; give each XCD a contiguous run of the
+    // (spatially ordered) node list so its concurrent nodes share subset voxels in its L2.
+    const int64_t M = *count;
+    const int64_t b = blockIdx.x;
+    const int64_t per = (M + 7) / 8;
+    if ((b >> 3) >= per) return;  // the grid is sized for all N nodes; M <= N are valid
+    const int64_t slot = (b & 7) * per + (b >> 3);
+    if (slot >= M) return;
+    const int64_t n = list[slot];
+    const int64_t x0 = a.coords[3 * n], y0 = a.coords[3 * n + 1], z0 = a.coords[3 * n + 2];
+    const uint8_t* keep = keep_row_of(a, n);
+
+    __shared__ float sw[14];  // warp (12), pilots pf, pg
+    __shared__ int ctrl;
+    __shared__ double red[NWAVES * NM];
+    __shared__ double res[NM];
+    __shared__ Ctl12 c;
+
+    double pg_next = 0.0;
+    if (tid == 0) {
+        bool finite = true;
+        ctl12_init(c, a.P + 12 * n);
+#pragma unroll 1
+        for (int k = 0; k < 12; ++k)
+            if (!isfinite(c.P[k])) finite = false;
+        ctrl = -1;
+        if (!finite)
+            ctrl = ST_NAN;
+        else if (!keep && !subset_in_volume(V, s, x0, y0, z0))
+            ctrl = ST_INVALID_SUBSET;
+        double pf = a.meanf ? a.meanf[n] : 0.0;
+        if (!isfinite(pf)) pf = 0.0;
+        sw[12] = (float)pf;
+        pg_next = pf;
+    }
+    __syncthreads();
+    const double half_scale = (double)max(s.hx, max(s.hy, s.hz));
+    const Domain dom = make_domain(V, (int)x0, (int)y0, (int)z0);
+    const float* gbase = V.g + lin(V, z0, y0, x0);
+    const int64_t obase = lin(V, z0, y0, x0);
+    const int rowp = V.nx, planep = V.nx * V.ny;
+    int it_done = 0;
+    int status = ST_MAX_ITER;
+    for (int it = 1; it <= a.max_iter; ++it) {
+        if (ctrl >= 0) break;
+        it_done = it;
+        if (tid == 0) {
+            put_warp(sw, c.P);
+            sw[13] = (float)pg_next;
+        }
+        __syncthreads();
+        const float pf = sw[12], pg = sw[13];
+        float m[NM];
+#pragma unroll
+        for (int q = 0; q < NM; ++q) m[q] = 0.0f;
+        VoxelWalk w;
+        w.start(s, tid);
+        for (; w.v < s.S; w.next(s)) {
+            const int dx = -s.hx + s.stride * w.ix, dy = -s.hy + s.stride * w.iy, dz = -s.hz + s.stride * w.iz;
+            const int64_t o = obase + (int64_t)dz * planep + (int64_t)dy * rowp + dx;
+            bool k;
+            if (keep) {
+                k = kept_bit(keep, w.v);
+                const int64_t xx = x0 + dx, yy = y0 + dy, zz = z0 + dz;
+                k = k && xx >= 0 && xx < V.nx && yy >= 0 && yy < V.ny && zz >= 0 && zz < V.nz;
+            } else {
+                k = !V.mask || V.mask[o] != 0;
+            }
+            if (!k) continue;
+            m[1] += 1.0f;
+            const float X = (float)dx, Y = (float)dy, Z = (float)dz;
+            const float xr = sw[0] * X + sw[1] * Y + sw[2] * Z + sw[9];
+            const float yr = sw[3] * X + sw[4] * Y + sw[5] * Z + sw[10];
+            const float zr = sw[6] * X + sw[7] * Y + sw[8] * Z + sw[11];
+            if (!inside(dom, xr, yr, zr)) {
+                m[2] = 1.0f;
+                continue;
+            }
+            const float val = sample_rel<MODE>(gbase, rowp, planep, xr, yr, zr, dom.kx, dom.ky, dom.kz);
+            if (val != val) continue;
+            const float fv = V.f[o] - pf, gv = val - pg;
+            float gr[3];
+            grad_at<STORED>(V, o, (int)z0 + dz, (int)y0 + dy, (int)x0 + dx, gr[0], gr[1], gr[2]);
+            m[0] += 1.0f;
+            m[3] += fv;
+            m[4] += fv * fv;
+            m[5] += gv;
+            m[6] += gv * gv;
+            m[7] += fv * gv;
+            const float q[4] = {X, Y, Z, 1.0f};
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc) {
+                const float t1 = gr[cc], tf = gr[cc] * fv, tg = gr[cc] * gv;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    m[8 + 4 * cc + j] += t1 * q[j];
+                    m[20 + 4 * cc + j] += tf * q[j];
+                    m[32 + 4 * cc + j] += tg * q[j];
+                }
+            }
+        }
+        block_sum<NM, NWAVES>(m, res, red);
+        const double* r = res;
+        if (tid == 0) {
+            const double n_valid = r[0], n_ref = r[1];
+            int st = -1;
+            if (r[2] > 0.0) {
+                st = ST_OUT_OF_BOUNDS;
+            } else if (n_valid < MIN_SUBSET_VOXELS || n_valid < MIN_VALID_FRACTION * n_ref) {
+                st = ST_INVALID_SUBSET;
+            }
+            if (st >= 0) {
+                c.zncc = NAN;
+                ctrl = st;
+            } else {
+                // centred statistics on the dynamic voxel set (numba_kernels.py:586-598)
+                const double mf = r[3] / n_valid, mg = r[5] / n_valid;
+                double ssf = r[4] - n_valid * mf * mf, ssg = r[6] - n_valid * mg * mg;
+                ssf = ssf < 0.0 ? 0.0 : ssf;
+                ssg = ssg < 0.0 ? 0.0 : ssg;
+                const double bf = sqrt(ssf > 1e-30 ? ssf : 1e-30), bg = sqrt(ssg > 1e-30 ? ssg : 1e-30);
+                const double scc = r[7] - n_valid * mf * mg;
+                const double zncc = scc / (bf * bg);
+                // b_k = bottomf * sum SD_k ((f - mf)/bf - (g - mg)/bg)   (numba_kernels.py:705-729)
+                const double ratio = bf / bg;
+#pragma unroll 1
+                for (int q = 0; q < 12; ++q) {
+                    const int cc = q >> 2, j = q & 3;
+                    const double C = r[8 + q], A = r[20 + q], B = r[32 + q];
+                    c.b[j < 3 ? 3 * cc + j : 9 + cc] = (A - mf * C) - ratio * (B - mg * C);
+                }
+                const double* Hn = a.H_all ? a.H_all + 144 * n : nullptr;
+                ctrl = ctl12_step(c, zncc, bf, n_valid, a.L_all + 144 * n, Hn, a.pattern, a.noise_gain, a.tol,
+                                  a.dp_tol, a.patience, a.predictive, half_scale);
+                pg_next = (double)pg + mg;
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        if (ctrl >= 0) status = ctrl;
+        for (int k = 0; k < 12; ++k) a.P[12 * n + k] = c.P[k];
+        a.n_iter[n] = it_done;
+        a.status[n] = (int8_t)status;
+        a.zncc[n] = (status == ST_CONVERGED || status == ST_STALLED || status == ST_MAX_ITER) ? c.zncc : NAN;
+    }
+}
+
+// ---------------------------------------------------------------------------- 3-DOF kernel
+struct Icgn3Args {
+    Volumes V;
+    Subset sub;
+    const int64_t* coords;
+    const double* U_old;  // (N, 3)
+    double* U;            // (N, 3) out
+    const double* F;      // (N, 9)
+    const double* vdual;  // (N, 3)
+    const double* H_all;
+    const double* bottomf;
+    const double* meanf;
+    const int64_t* split_index;
+    const uint8_t* split_keep;
+    int64_t keep_bytes;
+    double mu, tol, dp_tol, noise_gain;
+    int max_iter, patience, predictive;
+    int32_t* n_iter;
+    int8_t* status;
+    double* zncc;
+};
+
+// ADMM subproblem 1: translation-only IC-GN with a proximal term (numba_kernels.py:931-1082),
+// one pass per iteration with the moments of the fused 12-DOF kernel restricted to q = 1.
+template <int MODE, bool STORED>
+__global__ __launch_bounds__(BLOCK) void k_icgn3_fused(Icgn3Args a, const int64_t* __restrict__ list,
+                                                       const int64_t* __restrict__ count) {
+    const Volumes& V = a.V;
+    const Subset& s = a.sub;
+    const int tid = threadIdx.x;
+    const int64_t M = *count;
+    const int64_t b = blockIdx.x, per = (M + 7) / 8;
+    if ((b >> 3) >= per) return;
+    const int64_t slot = (b & 7) * per + (b >> 3);
+    if (slot >= M) return;
+    const int64_t n = list[slot];
+    const int64_t x0 = a.coords[3 * n], y0 = a.coords[3 * n + 1], z0 = a.coords[3 * n + 2];
+    const uint8_t* keep = (a.split_index && a.split_keep && a.split_index[n] >= 0)
+                              ? a.split_keep + a.split_index[n] * a.keep_bytes
+                              : nullptr;
+    constexpr int NQ = 17;  // n, n_ref, oob, f', f'^2, g', g'^2, f'g', C(3), A(3), B(3)
+    __shared__ float sw[14];
+    __shared__ int ctrl;
+    __shared__ double red[NWAVES * NQ];
+    __shared__ double res[NQ];
+    __shared__ double P[12], Hinv[9];  // control-lane state kept out of the sampling lanes' registers
+    double bf2 = 1.0, best_dn = 1e300, last_dn = 1e300, dn_prev = -1.0, norm_init = -1.0, zncc = NAN,
+           pg_next = 0.0;
+    int stall = 0;
+    const double* H = a.H_all + 144 * n;
+    if (tid == 0) {
+        bool finite = true;
+        for (int k = 0; k < 9; ++k) {
+            P[k] = a.F[9 * n + k];
+            if (!isfinite(P[k])) finite = false;
+        }
+        for (int k = 0; k < 3; ++k) {
+            P[9 + k] = a.U_old[3 * n + k];
+            if (!isfinite(P[9 + k])) finite = false;
+        }
+        ctrl = -1;
+        bf2 = a.bottomf[n] * a.bottomf[n];
+        if (bf2 < 1e-30) bf2 = 1e-30;
+        if (!finite)
+            ctrl = ST_NAN;
+        else if (!keep && !subset_in_volume(V, s, x0, y0, z0))
+            ctrl = ST_INVALID_SUBSET;
+        else if (!build_h3(H, bf2, a.mu, 0.0, Hinv))
+            ctrl = ST_SINGULAR;  // before any iteration: n_iter stays 0 (numba_kernels.py:980-982)
+        double pf = a.meanf ? a.meanf[n] : 0.0;
+        if (!isfinite(pf)) pf = 0.0;
+        sw[12] = (float)pf;
+        pg_next = pf;
+        put_warp(sw, P);
+    }
+    __syncthreads();
+    const Domain dom = make_domain(V, (int)x0, (int)y0, (int)z0);
+    const float* gbase = V.g + lin(V, z0, y0, x0);
+    const int64_t obase = lin(V, z0, y0, x0);
+    const int rowp = V.nx, planep = V.nx * V.ny;
+    int it_done = 0, status = ST_MAX_ITER;
+    for (int it = 1; it <= a.max_iter; ++it) {
+        if (ctrl >= 0) break;
+        it_done = it;
+        if (tid == 0) {
+            sw[9] = (float)P[9];
+            sw[10] = (float)P[10];
+            sw[11] = (float)P[11];
+            sw[13] = (float)pg_next;
+        }
+        __syncthreads();
+        const float pf = sw[12], pg = sw[13];
+        float m[NQ];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) m[q] = 0.0f;
+        VoxelWalk w;
+        w.start(s, tid);
+        for (; w.v < s.S; w.next(s)) {
+            const int dx = -s.hx + s.stride * w.ix, dy = -s.hy + s.stride * w.iy, dz = -s.hz + s.stride * w.iz;
+            const int64_t o = obase + (int64_t)dz * planep + (int64_t)dy * rowp + dx;
+            bool k;
+            if (keep) {
+                k = kept_bit(keep, w.v);
+                const int64_t xx = x0 + dx, yy = y0 + dy, zz = z0 + dz;
+                k = k && xx >= 0 && xx < V.nx && yy >= 0 && yy < V.ny && zz >= 0 && zz < V.nz;
+            } else {
+                k = !V.mask || V.mask[o] != 0;
+            }
+            if (!k) continue;
+            m[1] += 1.0f;
+            const float X = (float)dx, Y = (float)dy, Z = (float)dz;
+            const float xr = sw[0] * X + sw[1] * Y + sw[2] * Z + sw[9];
+            const float yr = sw[3] * X + sw[4] * Y + sw[5] * Z + sw[10];
+            const float zr = sw[6] * X + sw[7] * Y + sw[8] * Z + sw[11];
+            if (!inside(dom, xr, yr, zr)) {
+                m[2] = 1.0f;
+                continue;
+            }
+            const float val = sample_rel<MODE>(gbase, rowp, planep, xr, yr, zr, dom.kx, dom.ky, dom.kz);
+            if (val != val) continue;
+            const float fv = V.f[o] - pf, gv = val - pg;
+            float gr[3];
+            grad_at<STORED>(V, o, (int)z0 + dz, (int)y0 + dy, (int)x0 + dx, gr[0], gr[1], gr[2]);
+            m[0] += 1.0f;
+            m[3] += fv;
+            m[4] += fv * fv;
+            m[5] += gv;
+            m[6] += gv * gv;
+            m[7] += fv * gv;
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc) {
+                m[8 + cc] += gr[cc];
+                m[11 + cc] += gr[cc] * fv;
+                m[14 + cc] += gr[cc] * gv;
+            }
+        }
+        block_sum<NQ, NWAVES>(m, res, red);
+        const double* r = res;
+        if (tid == 0) {
+            const double n_valid = r[0], n_ref = r[1];
+            int st = -1;
+            if (r[2] > 0.0)
+                st = ST_OUT_OF_BOUNDS;
+            else if (n_valid < MIN_SUBSET_VOXELS || n_valid < MIN_VALID_FRACTION * n_ref)
+                st = ST_INVALID_SUBSET;
+            if (st >= 0) {
+                zncc = NAN;
+                ctrl = st;
+            } else {
+                const double mf = r[3] / n_valid, mg = r[5] / n_valid;
+                double ssf = r[4] - n_valid * mf * mf, ssg = r[6] - n_valid * mg * mg;
+                ssf = ssf < 0.0 ? 0.0 : ssf;
+                ssg = ssg < 0.0 ? 0.0 : ssg;
+                const double bf = sqrt(ssf > 1e-30 ? ssf : 1e-30), bg = sqrt(ssg > 1e-30 ? ssg : 1e-30);
+                const double ratio = bf / bg;
+                double tb[3], norm_abs = 0.0;
+                for (int k = 0; k < 3; ++k) {
+                    const double C = r[8 + k], A = r[11 + k], B = r[14 + k];
+                    const double bk = (A - mf * C) - ratio * (B - mg * C);
+                    tb[k] = bk * 2.0 / bf2 + a.mu * (P[9 + k] - a.U_old[3 * n + k] - a.vdual[3 * n + k]);
+                    norm_abs += tb[k] * tb[k];
+                }
+                norm_abs = sqrt(norm_abs);
+                pg_next = (double)pg + mg;
+                if (!isfinite(norm_abs)) {
+                    zncc = NAN;
+                    ctrl = ST_NAN;
+                } else {
+                    if (norm_init < 0.0) norm_init = norm_abs;
+                    const double norm_rel = norm_init > 1e-300 ? norm_abs / norm_init : 0.0;
+                    zncc = (r[7] - n_valid * mf * mg) / (bf * bg);
+                    if (norm_rel < a.tol || norm_abs < a.mu * 1e-4) {
+                        ctrl = ST_CONVERGED;
+                    } else {
+                        if (a.noise_gain > 0.0 && last_dn < NOISE_CORR_STEP) {
+                            const double om = 1.0 - zncc;
+                            const double s2 = (om > 0.0 ? om : 0.0) * bf * bf / n_valid;
+                            if (!build_h3(H, bf2, a.mu, a.noise_gain * s2 * n_valid, Hinv))
+                                build_h3(H, bf2, a.mu, 0.0, Hinv);
+                        }
+                        double dt[3];
+                        for (int i = 0; i < 3; ++i) {
+                            double sacc = 0.0;
+                            for (int j = 0; j < 3; ++j) sacc += Hinv[3 * i + j] * tb[j];
+                            dt[i] = -sacc;
+                        }
+                        const double dn = sqrt(dt[0] * dt[0] + dt[1] * dt[1] + dt[2] * dt[2]);
+                        if (!isfinite(dn)) {
+                            zncc = NAN;
+                            ctrl = ST_NAN;
+                        } else {
+                            last_dn = dn;
+                            const double a00 = 1.0 + P[0], a01 = P[1], a02 = P[2], a10 = P[3], a11 = 1.0 + P[4],
+                                         a12 = P[5], a20 = P[6], a21 = P[7], a22 = 1.0 + P[8];
+                            bool apply = true;
+                            if (dn < a.dp_tol) {
+                                ctrl = ST_CONVERGED;
+                                apply = false;
+                            } else if (a.predictive && dn_prev > 0.0 && dn < PREDICT_CONTRACTION * dn_prev &&
+                                       dn * dn < a.dp_tol * dn_prev) {
+                                ctrl = ST_CONVERGED;
+                            } else {
+                                dn_prev = dn;
+                                if (dn < best_dn * STALL_STEP_DECAY) {
+                                    best_dn = dn;
+                                    stall = 0;
+                                } else {
+                                    stall += 1;
+                                    if (a.patience > 0 && stall >= a.patience) {
+                                        ctrl = ST_STALLED;
+                                        apply = false;
+                                    }
+                                }
+                            }
+                            if (apply) {
+                                P[9] -= a00 * dt[0] + a01 * dt[1] + a02 * dt[2];
+                                P[10] -= a10 * dt[0] + a11 * dt[1] + a12 * dt[2];
+                                P[11] -= a20 * dt[0] + a21 * dt[1] + a22 * dt[2];
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        if (ctrl >= 0) status = ctrl;
+        for (int k = 0; k < 3; ++k) a.U[3 * n + k] = P[9 + k];
+        a.n_iter[n] = it_done;
+        a.status[n] = (int8_t)status;
+        a.zncc[n] = (status == ST_CONVERGED || status == ST_STALLED || status == ST_MAX_ITER) ? zncc : NAN;
+    }
+}
+
+// ---------------------------------------------------------------------------- precompute
+// Per node: H = sum SD SD^T over the kept voxels, sum f, sum f^2, n_valid (float32 lane partials,
+// float64 cross-lane sums), then meanf/bottomf, the float64 Cholesky factor and the validity
+// rules of numba_kernels.py:391-417. Subsets that leave the volume without a keep row are
+// rejected before any read (numba_kernels.py:342-343).
+template <bool STORED>
+__global__ __launch_bounds__(BLOCK) void k_precompute(Volumes V, Subset s, const int64_t* __restrict__ coords,
+                                                      int64_t N, double min_valid_ratio, double cond_max,
+                                                      const int64_t* split_index, const uint8_t* split_keep,
+                                                      int64_t keep_bytes, double* H_all, double* L_all,
+                                                      double* meanf, double* bottomf, int64_t* nvalid,
+                                                      uint8_t* valid) {
+    const int64_t n = blockIdx.x;
+    if (n >= N) return;
+    const int tid = threadIdx.x;
+    const int64_t x0 = coords[3 * n], y0 = coords[3 * n + 1], z0 = coords[3 * n + 2];
+    const uint8_t* keep =
+        (split_index && split_keep && split_index[n] >= 0) ? split_keep + split_index[n] * keep_bytes : nullptr;
+    double* H = H_all + 144 * n;
+    double* L = L_all + 144 * n;
+    if (!keep && !subset_in_volume(V, s, x0, y0, z0)) {
+        for (int i = tid; i < 144; i += BLOCK) {
+            H[i] = 0.0;
+            L[i] = 0.0;
+        }
+        if (tid == 0) {
+            meanf[n] = 0.0;
+            bottomf[n] = 1.0;
+            nvalid[n] = 0;
+            valid[n] = 0;
+        }
+        return;
+    }
+    constexpr int NQ = 81;
+    __shared__ double red[NWAVES * NQ];
+    __shared__ double res[NQ];
+    __shared__ double Hs[144];
+    float acc[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) acc[q] = 0.0f;
+    const int64_t obase = lin(V, z0, y0, x0);
+    const int rowp = V.nx, planep = V.nx * V.ny;
+    VoxelWalk w;
+    w.start(s, tid);
+    for (; w.v < s.S; w.next(s)) {
+        const int dx = -s.hx + s.stride * w.ix, dy = -s.hy + s.stride * w.iy, dz = -s.hz + s.stride * w.iz;
+        const int64_t o = obase + (int64_t)dz * planep + (int64_t)dy * rowp + dx;
+        bool k;
+        if (keep) {
+            k = kept_bit(keep, w.v);
+            const int64_t xx = x0 + dx, yy = y0 + dy, zz = z0 + dz;
+            k = k && xx >= 0 && xx < V.nx && yy >= 0 && yy < V.ny && zz >= 0 && zz < V.nz;
+        } else {
+            k = !V.mask || V.mask[o] != 0;
+        }
+        if (!k) continue;
+        const float fv = V.f[o];
+        float gxv, gyv, gzv;
+        grad_at<STORED>(V, o, (int)z0 + dz, (int)y0 + dy, (int)x0 + dx, gxv, gyv, gzv);
+        const float X = (float)dx, Y = (float)dy, Z = (float)dz;
+        const float sd[12] = {gxv * X, gxv * Y, gxv * Z, gyv * X, gyv * Y, gyv * Z,
+                              gzv * X, gzv * Y, gzv * Z, gxv, gyv, gzv};
+        int q = 0;
+#pragma unroll
+        for (int i = 0; i < 12; ++i)
+#pragma unroll
+            for (int j = i; j < 12; ++j) acc[q++] += sd[i] * sd[j];
+        acc[78] += fv;
+        acc[79] += fv * fv;
+        acc[80] += 1.0f;
+    }
+    block_sum<NQ, NWAVES>(acc, res, red);
+    const double* r = res;
+    if (tid == 0) {
+        int q = 0;
+        for (int i = 0; i < 12; ++i)
+            for (int j = i; j < 12; ++j) {
+                Hs[i * 12 + j] = r[q];
+                Hs[j * 12 + i] = r[q];
+                ++q;
+            }
+        const double nv = r[80];
+        const int64_t nvi = (int64_t)(nv + 0.5);
+        double mf = 0.0, bf = 1.0;
+        bool ok = true;
+        if (nvi < 27 || (double)nvi < min_valid_ratio * (double)s.S) {
+            ok = false;
+        } else {
+            mf = r[78] / nv;
+            double ssf = r[79] - nv * mf * mf;
+            ssf = ssf < 0.0 ? 0.0 : ssf;
+            bf = sqrt(ssf > 1e-30 ? ssf : 1e-30);
+            if (bf < 1e-10) ok = false;
+        }
+        nvalid[n] = nvi;
+        meanf[n] = mf;
+        bottomf[n] = bf;
+        if (ok) ok = cholesky12(Hs, L);
+        else
+            for (int i = 0; i < 144; ++i) L[i] = 0.0;
+        if (ok) {
+            double dmin = L[0], dmax = L[0];
+            for (int i = 1; i < 12; ++i) {
+                dmin = L[i * 13] < dmin ? L[i * 13] : dmin;
+                dmax = L[i * 13] > dmax ? L[i * 13] : dmax;
+            }
+            if (dmin <= 0.0 || (dmax / dmin) * (dmax / dmin) > cond_max) ok = false;
+        }
+        valid[n] = ok ? 1 : 0;
+    }
+    __syncthreads();
+    for (int i = tid; i < 144; i += BLOCK) H[i] = Hs[i];
+}
+
+// ---------------------------------------------------------------------------- ZNCC of given warps
+// numba_kernels.py:1354-1387: ZNCC of each valid node's subset under the warp P (NaN when the
+// warped subset leaves the domain or keeps too few voxels).
+template <int MODE>
+__global__ __launch_bounds__(BLOCK) void k_evaluate_zncc(Volumes V, Subset s, const int64_t* __restrict__ coords,
+                                                         int64_t N, const double* __restrict__ P_all,
+                                                         const uint8_t* __restrict__ valid, double* out) {
+    const int64_t n = blockIdx.x;
+    if (n >= N) return;
+    const int tid = threadIdx.x;
+    const int64_t x0 = coords[3 * n], y0 = coords[3 * n + 1], z0 = coords[3 * n + 2];
+    if (!valid[n] || !subset_in_volume(V, s, x0, y0, z0)) {
+        if (tid == 0) out[n] = NAN;
+        return;
+    }
+    constexpr int NQ = 8;  // n, n_ref, oob, f', f'^2, g', g'^2, f'g'
+    __shared__ double red[NWAVES * NQ];
+    __shared__ double res[NQ];
+    float sw[12];
+    put_warp(sw, P_all + 12 * n);
+    const Domain dom = make_domain(V, (int)x0, (int)y0, (int)z0);
+    const float* gbase = V.g + lin(V, z0, y0, x0);
+    const int64_t obase = lin(V, z0, y0, x0);
+    const int rowp = V.nx, planep = V.nx * V.ny;
+    const float pf = V.f[obase];
+    float m[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) m[q] = 0.0f;
+    VoxelWalk w;
+    w.start(s, tid);
+    for (; w.v < s.S; w.next(s)) {
+        const int dx = -s.hx + s.stride * w.ix, dy = -s.hy + s.stride * w.iy, dz = -s.hz + s.stride * w.iz;
+        const int64_t o = obase + (int64_t)dz * planep + (int64_t)dy * rowp + dx;
+        if (V.mask && V.mask[o] == 0) continue;
+        m[1] += 1.0f;
+        const float X = (float)dx, Y = (float)dy, Z = (float)dz;
+        const float xr = sw[0] * X + sw[1] * Y + sw[2] * Z + sw[9];
+        const float yr = sw[3] * X + sw[4] * Y + sw[5] * Z + sw[10];
+        const float zr = sw[6] * X + sw[7] * Y + sw[8] * Z + sw[11];
+        if (!inside(dom, xr, yr, zr)) {
+            m[2] = 1.0f;
+            continue;
+        }
+        const float val = sample_rel<MODE>(gbase, rowp, planep, xr, yr, zr, dom.kx, dom.ky, dom.kz);
+        if (val != val) continue;
+        const float fv = V.f[o] - pf, gv = val - pf;
+        m[0] += 1.0f;
+        m[3] += fv;
+        m[4] += fv * fv;
+        m[5] += gv;
+        m[6] += gv * gv;
+        m[7] += fv * gv;
+    }
+    block_sum<NQ, NWAVES>(m, res, red);
+    if (tid == 0) {
+        const double nv = res[0];
+        double z = NAN;
+        if (!(res[2] > 0.0) && nv >= MIN_SUBSET_VOXELS && nv >= MIN_VALID_FRACTION * res[1]) {
+            const double mf = res[3] / nv, mg = res[5] / nv;
+            double ssf = res[4] - nv * mf * mf, ssg = res[6] - nv * mg * mg;
+            ssf = ssf < 0.0 ? 0.0 : ssf;
+            ssg = ssg < 0.0 ? 0.0 : ssg;
+            const double bf = sqrt(ssf > 1e-30 ? ssf : 1e-30), bg = sqrt(ssg > 1e-30 ? ssg : 1e-30);
+            z = (res[7] - nv * mf * mg) / (bf * bg);
+        }
+        out[n] = z;
+    }
+}
+
+// ---------------------------------------------------------------------------- small kernels
+// 7-point central differences, float32 arithmetic, zero within `border` of a face
+// (io/volume_ops.py:124-155).
+__global__ void k_gradients(const float* __restrict__ f, int nz, int ny, int nx, int border, float* __restrict__ gx,
+                            float* __restrict__ gy, float* __restrict__ gz) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    const int z = blockIdx.z;
+    if (x >= nx) return;
+    const int64_t o = ((int64_t)z * ny + y) * nx + x;
+    const int64_t r = nx, p = (int64_t)nx * ny;
+    if (z >= border && z < nz - border && y >= border && y < ny - border && x >= border && x < nx - border) {
+        gx[o] = C1 * (f[o + 1] - f[o - 1]) + C2 * (f[o + 2] - f[o - 2]) + C3 * (f[o + 3] - f[o - 3]);
+        gy[o] = C1 * (f[o + r] - f[o - r]) + C2 * (f[o + 2 * r] - f[o - 2 * r]) + C3 * (f[o + 3 * r] - f[o - 3 * r]);
+        gz[o] = C1 * (f[o + p] - f[o - p]) + C2 * (f[o + 2 * p] - f[o - 2 * p]) + C3 * (f[o + 3 * p] - f[o - 3 * p]);
+    } else {
+        gx[o] = 0.0f;
+        gy[o] = 0.0f;
+        gz[o] = 0.0f;
+    }
+}
+
+// interp_kernels.py:139-152: NaN outside the admissible domain
+template <int MODE>
+__global__ void k_sample_points(const float* __restrict__ vol, int nz, int ny, int nx, const double* zs,
+                                const double* ys, const double* xs, int64_t n, float* out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Volumes V{};
+    V.nz = nz;
+    V.ny = ny;
+    V.nx = nx;
+    // anchor at the voxel below the point so the relative coordinates stay small
+    const double z = zs[i], y = ys[i], x = xs[i];
+    if (!(x >= 1.0 && x <= nx - 3.0 && y >= 1.0 && y <= ny - 3.0 && z >= 1.0 && z <= nz - 3.0)) {
+        out[i] = NAN;
+        return;
+    }
+    const int ax = (int)floor(x), ay = (int)floor(y), az = (int)floor(z);
+    const Domain d = make_domain(V, ax, ay, az);
+    const float* base = vol + (((int64_t)az * ny + ay) * nx + ax);
+    out[i] = sample_rel<MODE>(base, nx, nx * ny, (float)(x - ax), (float)(y - ay), (float)(z - az), d.kx, d.ky, d.kz);
+}
+
+// Indices of the valid nodes in order (single workgroup scan; N is at most a few million).
+__global__ __launch_bounds__(1024) void k_compact(const uint8_t* __restrict__ valid, int64_t N, int64_t* __restrict__ list,
+                                                  int64_t* __restrict__ count) {
+    __shared__ int64_t wsum[16];
+    __shared__ int64_t base;
+    if (threadIdx.x == 0) base = 0;
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int64_t start = 0; start < N; start += 1024) {
+        const int64_t i = start + threadIdx.x;
+        const int flag = (i < N && valid[i]) ? 1 : 0;
+        // inclusive scan within the wave
+        int64_t v = flag;
+        for (int off = 1; off < 64; off <<= 1) {
+            const int64_t t = __shfl_up(v, off, 64);
+            if (lane >= off) v += t;
+        }
+        if (lane == 63) wsum[wave] = v;
+        __syncthreads();
+        int64_t before = 0;
+        for (int k = 0; k < wave; ++k) before += wsum[k];
+        if (flag) list[base + before + v - 1] = i;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int64_t tot = 0;
+            for (int k = 0; k < 16; ++k) tot += wsum[k];
+            base += tot;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *count = base;
+}
+
+// Status/iteration/ZNCC defaults of the reference batch wrappers (numba_kernels.py:832-840).
+__global__ void k_init_outputs(const uint8_t* __restrict__ valid, int64_t N, int32_t* n_iter, int8_t* status,
+                               double* zncc) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    n_iter[i] = 0;
+    status[i] = valid[i] ? ST_SKIPPED : ST_INVALID_SUBSET;
+    zncc[i] = NAN;
+}
+
+// The valid-node count lives on the device (written by k_compact); grids are sized for all N
+// nodes and the workgroups past the count exit at once, so no host round trip is needed.
+template <int MODE, bool STORED>
+void launch_icgn12(int variant, const IcgnArgs& a, const int64_t* list, const int64_t* count, int64_t N, float* gbuf,
+                   hipStream_t st) {
+    if (variant == 0) {
+        for (int64_t off = 0; off < N; off += PLAIN_CHUNK) {
+            const int64_t cnt = std::min<int64_t>(PLAIN_CHUNK, N - off);
+            hipLaunchKernelGGL((k_icgn12_plain<MODE, STORED>), dim3((unsigned)cnt), dim3(BLOCK), 0, st, a, list, count,
+                               off, gbuf);
+        }
+    } else {
+        const int64_t nb = ((N + 7) / 8) * 8;
+        hipLaunchKernelGGL((k_icgn12_fused<MODE, STORED>), dim3((unsigned)nb), dim3(BLOCK), 0, st, a, list, count);
+    }
+}
+
+template <int MODE, bool STORED>
+void launch_icgn3(const Icgn3Args& a, const int64_t* list, const int64_t* count, int64_t N, hipStream_t st) {
+    const int64_t nb = ((N + 7) / 8) * 8;
+    hipLaunchKernelGGL((k_icgn3_fused<MODE, STORED>), dim3((unsigned)nb), dim3(BLOCK), 0, st, a, list, count);
+}
+
+int check_common(const Volumes& V, const Subset& s, int mode) {
+    if (V.nz < 4 || V.ny < 4 || V.nx < 4) return fail(-2, "volumes must be at least 4 voxels along every axis");
+    if ((int64_t)V.nx * V.ny * V.nz > INT32_MAX)
+        return fail(-2, "volumes beyond 2^31 voxels must be tiled (32-bit sample offsets)");
+    if (s.hx < 0 || s.hy < 0 || s.hz < 0 || s.stride < 1) return fail(-2, "bad subset half-widths or stride");
+    if (mode < 0 || mode > 2) return fail(-2, "interp mode must be 0 (linear), 1 (cubic) or 2 (bspline)");
+    if (!V.f || !V.g) return fail(-2, "f and g are required");
+    if ((V.gx == nullptr) != (V.gy == nullptr) || (V.gx == nullptr) != (V.gz == nullptr))
+        return fail(-2, "gx, gy, gz must be all given or all null");
+    return 0;
+}
+
+}  // namespace
+
+// ============================================================================ C ABI
+extern "C" {
+
+int aldvc_abi_version(void) { return ALDVC_ABI_VERSION; }
+
+const char* aldvc_last_error(void) { return last_error_cstr(); }
+
+int aldvc_gradients(const float* f, int64_t nz, int64_t ny, int64_t nx, int border, float* gx, float* gy, float* gz,
+                    hipStream_t stream) {
+    clear_error();
+    if (!f || !gx || !gy || !gz) return fail(-2, "null pointer");
+    if (nz < 1 || ny < 1 || nx < 1 || ny > 65535 || nz > 65535) return fail(-2, "bad volume shape");
+    dim3 grid((unsigned)((nx + 255) / 256), (unsigned)ny, (unsigned)nz);
+    hipLaunchKernelGGL(k_gradients, grid, dim3(256), 0, stream, f, (int)nz, (int)ny, (int)nx, border, gx, gy, gz);
+    return launch_status();
+}
+
+int aldvc_sample_points(const float* vol, int64_t nz, int64_t ny, int64_t nx, const double* zs, const double* ys,
+                        const double* xs, int64_t n, int mode, float* out, hipStream_t stream) {
+    clear_error();
+    if (!vol || !zs || !ys || !xs || !out) return fail(-2, "null pointer");
+    if (nz < 4 || ny < 4 || nx < 4) return fail(-2, "volume too small");
+    if (n <= 0) return 0;
+    const unsigned nb = (unsigned)((n + 255) / 256);
+    switch (mode) {
+        case 0: hipLaunchKernelGGL(k_sample_points<0>, dim3(nb), dim3(256), 0, stream, vol, (int)nz, (int)ny, (int)nx, zs, ys, xs, n, out); break;
+        case 1: hipLaunchKernelGGL(k_sample_points<1>, dim3(nb), dim3(256), 0, stream, vol, (int)nz, (int)ny, (int)nx, zs, ys, xs, n, out); break;
+        case 2: hipLaunchKernelGGL(k_sample_points<2>, dim3(nb), dim3(256), 0, stream, vol, (int)nz, (int)ny, (int)nx, zs, ys, xs, n, out); break;
+        default: return fail(-2, "bad interp mode");
+    }
+    return launch_status();
+}
+
+int aldvc_precompute(const int64_t* coords, int64_t N, int hx, int hy, int hz, int stride, const float* f,
+                     const float* gx, const float* gy, const float* gz, const uint8_t* mask, int64_t nz, int64_t ny,
+                     int64_t nx, double min_valid_ratio, double cond_max, const int64_t* split_index,
+                     const uint8_t* split_keep, int64_t keep_bytes, double* H_all, double* L_all, double* meanf,
+                     double* bottomf, int64_t* n_valid, uint8_t* valid, hipStream_t stream) {
+    clear_error();
+    Volumes V{f, gx, gy, gz, f, mask, (int)nz, (int)ny, (int)nx};
+    const Subset s = make_subset(hx, hy, hz, stride);
+    if (int e = check_common(V, s, 1)) return e;
+    if (N < 0) return fail(-2, "negative node count");
+    if (N == 0) return 0;
+    if (!coords || !H_all || !L_all || !meanf || !bottomf || !n_valid || !valid) return fail(-2, "null pointer");
+    if (split_keep && keep_bytes < (s.S + 7) / 8) return fail(-2, "keep rows shorter than ceil(S/8) bytes");
+    if (N > INT32_MAX) return fail(-2, "too many nodes for one launch");
+    if (gx)
+        hipLaunchKernelGGL(k_precompute<true>, dim3((unsigned)N), dim3(BLOCK), 0, stream, V, s, coords, N,
+                           min_valid_ratio, cond_max, split_index, split_keep, keep_bytes, H_all, L_all, meanf,
+                           bottomf, n_valid, valid);
+    else
+        hipLaunchKernelGGL(k_precompute<false>, dim3((unsigned)N), dim3(BLOCK), 0, stream, V, s, coords, N,
+                           min_valid_ratio, cond_max, split_index, split_keep, keep_bytes, H_all, L_all, meanf,
+                           bottomf, n_valid, valid);
+    return launch_status();
+}
+
+size_t aldvc_icgn12_workspace_bytes(int64_t N, int hx, int hy, int hz, int stride, int variant) {
+    const Subset s = make_subset(hx, hy, hz, stride);
+    size_t bytes = 256 + sizeof(int64_t) * (size_t)(N > 0 ? N : 1);  // count + node list
+    bytes = (bytes + 255) / 256 * 256;
+    if (variant == 0) bytes += sizeof(float) * (size_t)std::min<int64_t>(N, PLAIN_CHUNK) * (size_t)s.S;
+    return bytes;
+}
+
+int aldvc_icgn12(const int64_t* coords, int64_t N, double* P, int hx, int hy, int hz, int stride, const float* f,
+                 const float* gx, const float* gy, const float* gz, const uint8_t* mask, const float* g, int64_t nz,
+                 int64_t ny, int64_t nx, int mode, const double* L_all, const double* H_all, const double* pattern,
+                 double noise_gain, const double* meanf, const uint8_t* valid, double tol, double dp_tol,
+                 int max_iter, int patience, int predictive, const int64_t* split_index, const uint8_t* split_keep,
+                 int64_t keep_bytes, int32_t* n_iter, int8_t* status, double* zncc, int variant, void* workspace,
+                 size_t workspace_bytes, hipStream_t stream) {
+    clear_error();
+    Volumes V{f, gx, gy, gz, g, mask, (int)nz, (int)ny, (int)nx};
+    const Subset s = make_subset(hx, hy, hz, stride);
+    if (int e = check_common(V, s, mode)) return e;
+    if (variant != 0 && variant != 1) return fail(-2, "variant must be 0 (plain) or 1 (fused)");
+    if (N < 0) return fail(-2, "negative node count");
+    if (N == 0) return 0;
+    if (!coords || !P || !L_all || !valid || !n_iter || !status || !zncc) return fail(-2, "null pointer");
+    if (N > INT32_MAX) return fail(-2, "too many nodes for one launch");
+    if (split_keep && keep_bytes < (s.S + 7) / 8) return fail(-2, "keep rows shorter than ceil(S/8) bytes");
+    if (!(H_all && pattern && noise_gain > 0.0)) {
+        H_all = nullptr;
+        pattern = nullptr;
+        noise_gain = 0.0;
+    }
+    const size_t need = aldvc_icgn12_workspace_bytes(N, hx, hy, hz, stride, variant);
+    if (!workspace || workspace_bytes < need) return fail(-3, "workspace too small (see aldvc_icgn12_workspace_bytes)");
+    int64_t* count = (int64_t*)workspace;
+    int64_t* list = (int64_t*)((char*)workspace + 256);
+    size_t list_bytes = (256 + sizeof(int64_t) * (size_t)N + 255) / 256 * 256;
+    float* gbuf = (float*)((char*)workspace + list_bytes);
+    hipLaunchKernelGGL(k_init_outputs, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, stream, valid, N, n_iter,
+                       status, zncc);
+    hipLaunchKernelGGL(k_compact, dim3(1), dim3(1024), 0, stream, valid, N, list, count);
+    IcgnArgs a{V, s, coords, P, L_all, H_all, pattern, meanf, split_index, split_keep, keep_bytes,
+               noise_gain, tol, dp_tol, max_iter, patience, predictive, n_iter, status, zncc};
+    const bool stored = gx != nullptr;
+#define ALDVC_DISPATCH(MD)                                                            \
+    if (stored) launch_icgn12<MD, true>(variant, a, list, count, N, gbuf, stream);    \
+    else launch_icgn12<MD, false>(variant, a, list, count, N, gbuf, stream);
+    switch (mode) {
+        case 0: ALDVC_DISPATCH(0) break;
+        case 1: ALDVC_DISPATCH(1) break;
+        default: ALDVC_DISPATCH(2) break;
+    }
+#undef ALDVC_DISPATCH
+    return launch_status();
+}
+
+size_t aldvc_icgn3_workspace_bytes(int64_t N) { return aldvc_icgn12_workspace_bytes(N, 0, 0, 0, 1, 1); }
+
+int aldvc_icgn3(const int64_t* coords, int64_t N, const double* U_old, double* U, const double* F_fixed,
+                const double* vdual, int hx, int hy, int hz, int stride, const float* f, const float* gx,
+                const float* gy, const float* gz, const uint8_t* mask, const float* g, int64_t nz, int64_t ny,
+                int64_t nx, int mode, const double* H_all, const double* meanf, const double* bottomf,
+                const uint8_t* valid, double mu, double tol, double dp_tol, int max_iter, int patience,
+                double noise_gain, int predictive, const int64_t* split_index, const uint8_t* split_keep,
+                int64_t keep_bytes, int32_t* n_iter, int8_t* status, double* zncc, void* workspace,
+                size_t workspace_bytes, hipStream_t stream) {
+    clear_error();
+    Volumes V{f, gx, gy, gz, g, mask, (int)nz, (int)ny, (int)nx};
+    const Subset s = make_subset(hx, hy, hz, stride);
+    if (int e = check_common(V, s, mode)) return e;
+    if (N < 0) return fail(-2, "negative node count");
+    if (N == 0) return 0;
+    if (!coords || !U_old || !U || !F_fixed || !vdual || !H_all || !bottomf || !valid || !n_iter || !status || !zncc)
+        return fail(-2, "null pointer");
+    if (N > INT32_MAX) return fail(-2, "too many nodes for one launch");
+    if (split_keep && keep_bytes < (s.S + 7) / 8) return fail(-2, "keep rows shorter than ceil(S/8) bytes");
+    const size_t need = aldvc_icgn3_workspace_bytes(N);
+    if (!workspace || workspace_bytes < need) return fail(-3, "workspace too small (see aldvc_icgn3_workspace_bytes)");
+    int64_t* count = (int64_t*)workspace;
+    int64_t* list = (int64_t*)((char*)workspace + 256);
+    if (hipMemcpyAsync(U, U_old, sizeof(double) * 3 * (size_t)N, hipMemcpyDeviceToDevice, stream) != hipSuccess)
+        return fail(-4, "copy of U_old failed");
+    hipLaunchKernelGGL(k_init_outputs, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, stream, valid, N, n_iter,
+                       status, zncc);
+    hipLaunchKernelGGL(k_compact, dim3(1), dim3(1024), 0, stream, valid, N, list, count);
+    Icgn3Args a{V, s, coords, U_old, U, F_fixed, vdual, H_all, bottomf, meanf, split_index, split_keep, keep_bytes,
+                mu, tol, dp_tol, noise_gain > 0.0 ? noise_gain : 0.0, max_iter, patience, predictive, n_iter, status,
+                zncc};
+    const bool stored = gx != nullptr;
+#define ALDVC_DISPATCH3(MD)                                   \
+    if (stored) launch_icgn3<MD, true>(a, list, count, N, stream);  \
+    else launch_icgn3<MD, false>(a, list, count, N, stream);
+    switch (mode) {
+        case 0: ALDVC_DISPATCH3(0) break;
+        case 1: ALDVC_DISPATCH3(1) break;
+        default: ALDVC_DISPATCH3(2) break;
+    }
+#undef ALDVC_DISPATCH3
+    return launch_status();
+}
+
+int aldvc_evaluate_zncc(const int64_t* coords, int64_t N, const double* P_all, int hx, int hy, int hz, int stride,
+                        const float* f, const uint8_t* mask, const float* g, int64_t nz, int64_t ny, int64_t nx,
+                        int mode, const uint8_t* valid, double* out, hipStream_t stream) {
+    clear_error();
+    Volumes V{f, nullptr, nullptr, nullptr, g, mask, (int)nz, (int)ny, (int)nx};
+    const Subset s = make_subset(hx, hy, hz, stride);
+    if (int e = check_common(V, s, mode)) return e;
+    if (N < 0) return fail(-2, "negative node count");
+    if (N == 0) return 0;
+    if (!coords || !P_all || !valid || !out) return fail(-2, "null pointer");
+    if (N > INT32_MAX) return fail(-2, "too many nodes for one launch");
+    switch (mode) {
+        case 0: hipLaunchKernelGGL(k_evaluate_zncc<0>, dim3((unsigned)N), dim3(BLOCK), 0, stream, V, s, coords, N, P_all, valid, out); break;
+        case 1: hipLaunchKernelGGL(k_evaluate_zncc<1>, dim3((unsigned)N), dim3(BLOCK), 0, stream, V, s, coords, N, P_all, valid, out); break;
+        default: hipLaunchKernelGGL(k_evaluate_zncc<2>, dim3((unsigned)N), dim3(BLOCK), 0, stream, V, s, coords, N, P_all, valid, out); break;
+    }
+    return launch_status();
+}
+
+}  // extern "C"

@@ -1,0 +1,329 @@
+// Integer NCC search on gfx950: per-node ZNCC maps over the search window and the fused peak
+// epilogue (first-index argmax, PCE, 27-point quadratic sub-voxel fit).
+//
+// Map semantics (solver/integer_search.py:44-151, cuda_kernels.py:1296-1389): the template is
+// the (2h+1)^3 reference box around the node, centred by its mean; for every offset o of the
+// (2r+1)^3 search grid, ZNCC(o) = sum T~ g / sqrt(var_w * sum T~^2) with
+// var_w = sum g^2 - (sum g)^2 / n over the window at o; var_w < 1e-20 gives 0, values are
+// clipped to [-1, 1], and a flat template (sum T~^2 < 1e-20) or t_ok == 0 gives a map of -2.
+#include "aldvc_device.h"
+
+#include <algorithm>
+
+#include "aldvc_abi_util.h"
+
+using namespace aldvc;
+
+namespace {
+
+constexpr int BLOCK = 256;
+constexpr int NWAVES = BLOCK / 64;
+constexpr int RX = 8;  // consecutive x offsets per lane (register blocking of the window row)
+constexpr int LDS_FLOATS_MAX = 160 * 1024 / 4 - 1024;
+
+struct NccArgs {
+    const float* f;
+    const float* g;
+    int nz, ny, nx;
+    const int64_t* coords;
+    int64_t n;
+    int hx, hy, hz, rx, ry, rz;
+    const int64_t* wx0;
+    const int64_t* wy0;
+    const int64_t* wz0;
+    const uint8_t* t_ok;
+    float* out;
+};
+
+__device__ inline void fill_map(float* o, int64_t n_off, float v) {
+    for (int64_t i = threadIdx.x; i < n_off; i += BLOCK) o[i] = v;
+}
+
+// Direct float32 correlation: one workgroup per node, the centred template in LDS, every lane
+// owns RX consecutive x offsets of one (dz, dy) output row and slides along the window rows.
+__global__ __launch_bounds__(BLOCK) void k_zncc_map_f32(NccArgs a) {
+    extern __shared__ float tpl[];
+    __shared__ double red[NWAVES];
+    __shared__ double bc[2];
+    const int64_t node = blockIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int sx = 2 * a.hx + 1, sy = 2 * a.hy + 1, sz = 2 * a.hz + 1;
+    const int vx = 2 * a.rx + 1, vy = 2 * a.ry + 1, vz = 2 * a.rz + 1;
+    const int wx = sx + vx - 1;
+    const int n_tpl = sx * sy * sz;
+    const int64_t n_off = (int64_t)vx * vy * vz;
+    float* o = a.out + node * n_off;
+    const int64_t x0 = a.coords[3 * node], y0 = a.coords[3 * node + 1], z0 = a.coords[3 * node + 2];
+    const int64_t xb = a.wx0[node], yb = a.wy0[node], zb = a.wz0[node];
+    // a template or a search window that leaves the volume is treated like t_ok == 0
+    const bool tpl_in = x0 - a.hx >= 0 && x0 + a.hx < a.nx && y0 - a.hy >= 0 && y0 + a.hy < a.ny && z0 - a.hz >= 0 &&
+                        z0 + a.hz < a.nz;
+    const bool win_in = xb >= 0 && xb + wx <= a.nx && yb >= 0 && yb + sy + vy - 1 <= a.ny && zb >= 0 &&
+                        zb + sz + vz - 1 <= a.nz;
+    if (!a.t_ok[node] || !tpl_in || !win_in) {
+        fill_map(o, n_off, -2.0f);
+        return;
+    }
+    // template and its mean (float64 sums)
+    double s = 0.0;
+    for (int v = tid; v < n_tpl; v += BLOCK) {
+        const int k = v / (sy * sx), r = v - k * sy * sx, j = r / sx, i = r - j * sx;
+        const float val = a.f[((z0 - a.hz + k) * a.ny + (y0 - a.hy + j)) * (int64_t)a.nx + (x0 - a.hx + i)];
+        tpl[v] = val;
+        s += val;
+    }
+    s = wave_sum(s);
+    if (lane == 0) red[wave] = s;
+    __syncthreads();
+    if (tid == 0) {
+        double t = 0.0;
+        for (int w = 0; w < NWAVES; ++w) t += red[w];
+        bc[0] = t / n_tpl;
+    }
+    __syncthreads();
+    const float mf = (float)bc[0];
+    double ss = 0.0;
+    for (int v = tid; v < n_tpl; v += BLOCK) {
+        const float val = tpl[v] - mf;
+        tpl[v] = val;
+        ss += (double)val * val;
+    }
+    ss = wave_sum(ss);
+    __syncthreads();
+    if (lane == 0) red[wave] = ss;
+    __syncthreads();
+    if (tid == 0) {
+        double t = 0.0;
+        for (int w = 0; w < NWAVES; ++w) t += red[w];
+        bc[1] = t;
+    }
+    __syncthreads();
+    const double ssf = bc[1];
+    if (ssf < 1e-20) {
+        fill_map(o, n_off, -2.0f);
+        return;
+    }
+    const float inv_n = 1.0f / (float)n_tpl;
+    const float sqrt_ssf = (float)sqrt(ssf);
+    const int nxb = (vx + RX - 1) / RX;
+    const int64_t items = (int64_t)vz * vy * nxb;
+    const float c = mf;  // window pilot: sum T~ (g - c) = sum T~ g because sum T~ = 0
+    for (int64_t it = tid; it < items; it += BLOCK) {
+        const int oz = (int)(it / ((int64_t)vy * nxb));
+        const int rem = (int)(it - (int64_t)oz * vy * nxb);
+        const int oy = rem / nxb;
+        const int ox0 = (rem - oy * nxb) * RX;
+        float acc[RX], s1[RX], s2[RX];
+#pragma unroll
+        for (int r = 0; r < RX; ++r) acc[r] = s1[r] = s2[r] = 0.0f;
+        for (int tz = 0; tz < sz; ++tz) {
+            for (int ty = 0; ty < sy; ++ty) {
+                const float* grow = a.g + ((zb + oz + tz) * a.ny + (yb + oy + ty)) * (int64_t)a.nx + xb;
+                const float* trow = tpl + (tz * sy + ty) * sx;
+                float win[RX];
+#pragma unroll
+                for (int r = 0; r < RX - 1; ++r) win[r] = grow[min(ox0 + r, wx - 1)] - c;
+                for (int tx = 0; tx < sx; ++tx) {
+                    win[RX - 1] = grow[min(ox0 + tx + RX - 1, wx - 1)] - c;
+                    const float t = trow[tx];
+#pragma unroll
+                    for (int r = 0; r < RX; ++r) {
+                        acc[r] += t * win[r];
+                        s1[r] += win[r];
+                        s2[r] += win[r] * win[r];
+                    }
+#pragma unroll
+                    for (int r = 0; r < RX - 1; ++r) win[r] = win[r + 1];
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < RX; ++r) {
+            if (ox0 + r >= vx) break;
+            const float var = s2[r] - s1[r] * s1[r] * inv_n;
+            float val;
+            if (var < 1e-20f) {
+                val = 0.0f;
+            } else {
+                val = acc[r] / (sqrtf(var) * sqrt_ssf);
+                val = val > 1.0f ? 1.0f : (val < -1.0f ? -1.0f : val);
+            }
+            o[((int64_t)oz * vy + oy) * vx + ox0 + r] = val;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------- peak epilogue
+// Per node (integer_search.py:444-477): the first index of the maximum in C order (a NaN wins,
+// as in np.argmax), the minimum, PCE = (peak - min)^2 / mean((C - min)^2) (0 when the energy is
+// not above 1e-30), and, for a peak off the search border with every radius > 0, the quadratic
+// fit of its 27-neighbourhood (pinv of the fixed 27 x 10 design, integer_search.py:210-227,
+// :297-321). Map value -2 (flat template) reads as 0, as in ncc_search (:420-421).
+__device__ inline float map_val(float v) { return v < -1.0f ? 0.0f : v; }
+
+__device__ inline bool better(float a, int64_t ia, float b, int64_t ib) {
+    const bool na = a != a, nb = b != b;
+    if (na || nb) return na && (!nb || ia < ib);
+    return a > b || (a == b && ia < ib);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_ncc_peaks(const float* __restrict__ maps, int64_t n, int rx, int ry, int rz,
+                                                     const double* __restrict__ pinv, int64_t* peak_idx,
+                                                     double* peak_val, double* pce, double* sub) {
+    const int64_t node = blockIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int vx = 2 * rx + 1, vy = 2 * ry + 1, vz = 2 * rz + 1;
+    const int64_t n_off = (int64_t)vx * vy * vz;
+    const float* m = maps + node * n_off;
+    __shared__ float wbest[NWAVES], wmin[NWAVES];
+    __shared__ int64_t wbi[NWAVES];
+    __shared__ double wen[NWAVES];
+    __shared__ float sbest, smin;
+    __shared__ int64_t sbi;
+    float best = -INFINITY, mn = INFINITY;
+    int64_t bi = INT64_MAX;
+    bool any_nan = false;
+    for (int64_t i = tid; i < n_off; i += BLOCK) {
+        const float v = map_val(m[i]);
+        if (bi == INT64_MAX || better(v, i, best, bi)) {
+            best = v;
+            bi = i;
+        }
+        if (v != v) any_nan = true;
+        mn = v < mn ? v : mn;
+    }
+    if (any_nan) mn = NAN;
+    for (int off = 32; off > 0; off >>= 1) {
+        const float ob = __shfl_xor(best, off, 64);
+        const int64_t oi = __shfl_xor(bi, off, 64);
+        const float om = __shfl_xor(mn, off, 64);
+        if (oi != INT64_MAX && (bi == INT64_MAX || better(ob, oi, best, bi))) {
+            best = ob;
+            bi = oi;
+        }
+        mn = (om != om || mn != mn) ? NAN : (om < mn ? om : mn);
+    }
+    if (lane == 0) {
+        wbest[wave] = best;
+        wbi[wave] = bi;
+        wmin[wave] = mn;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        float b = wbest[0], mm = wmin[0];
+        int64_t ii = wbi[0];
+        for (int w = 1; w < NWAVES; ++w) {
+            if (wbi[w] != INT64_MAX && (ii == INT64_MAX || better(wbest[w], wbi[w], b, ii))) {
+                b = wbest[w];
+                ii = wbi[w];
+            }
+            mm = (wmin[w] != wmin[w] || mm != mm) ? NAN : (wmin[w] < mm ? wmin[w] : mm);
+        }
+        sbest = b;
+        sbi = ii;
+        smin = mm;
+    }
+    __syncthreads();
+    const double cmin = (double)smin;
+    double en = 0.0;
+    for (int64_t i = tid; i < n_off; i += BLOCK) {
+        const double d = (double)map_val(m[i]) - cmin;
+        en += d * d;
+    }
+    en = wave_sum(en);
+    if (lane == 0) wen[wave] = en;
+    __syncthreads();
+    if (tid != 0) return;
+    double energy = 0.0;
+    for (int w = 0; w < NWAVES; ++w) energy += wen[w];
+    energy /= (double)n_off;
+    const double pk = (double)sbest;
+    const double pkm = pk - cmin;
+    pce[node] = energy > 1e-30 ? pkm * pkm / energy : 0.0;
+    peak_idx[node] = sbi;
+    peak_val[node] = pk;
+    const int64_t b = sbi;
+    const int pz = (int)(b / ((int64_t)vy * vx)), rem = (int)(b - (int64_t)pz * vy * vx);
+    const int py = rem / vx, px = rem - py * vx;
+    double sx_ = 0.0, sy_ = 0.0, sz_ = 0.0;
+    const bool border = px == 0 || px == 2 * rx || py == 0 || py == 2 * ry || pz == 0 || pz == 2 * rz;
+    if (!border && rx > 0 && ry > 0 && rz > 0) {
+        double coef[10];
+        for (int q = 0; q < 10; ++q) coef[q] = 0.0;
+        int t = 0;
+        for (int dz = -1; dz <= 1; ++dz)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx, ++t) {
+                    const double v = (double)map_val(m[((int64_t)(pz + dz) * vy + (py + dy)) * vx + (px + dx)]);
+                    for (int q = 0; q < 10; ++q) coef[q] += pinv[q * 27 + t] * v;
+                }
+        // H x = -a with H = [[2c7, c4, c5], [c4, 2c8, c6], [c5, c6, 2c9]], a = (c1, c2, c3)
+        const double h00 = 2.0 * coef[7], h11 = 2.0 * coef[8], h22 = 2.0 * coef[9];
+        const double h01 = coef[4], h02 = coef[5], h12 = coef[6];
+        const double det = h00 * (h11 * h22 - h12 * h12) - h01 * (h01 * h22 - h12 * h02) + h02 * (h01 * h12 - h11 * h02);
+        if (fabs(det) > 1e-12) {
+            const double r0 = -coef[1], r1 = -coef[2], r2 = -coef[3];
+            const double x = (r0 * (h11 * h22 - h12 * h12) - h01 * (r1 * h22 - h12 * r2) + h02 * (r1 * h12 - h11 * r2)) / det;
+            const double y = (h00 * (r1 * h22 - h12 * r2) - r0 * (h01 * h22 - h12 * h02) + h02 * (h01 * r2 - r1 * h02)) / det;
+            const double z = (h00 * (h11 * r2 - r1 * h12) - h01 * (h01 * r2 - r1 * h02) + r0 * (h01 * h12 - h11 * h02)) / det;
+            if (isfinite(x) && isfinite(y) && isfinite(z) && fabs(x) <= 1.0 && fabs(y) <= 1.0 && fabs(z) <= 1.0) {
+                sx_ = x;
+                sy_ = y;
+                sz_ = z;
+            }
+        }
+    }
+    sub[3 * node] = sx_;
+    sub[3 * node + 1] = sy_;
+    sub[3 * node + 2] = sz_;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t aldvc_zncc_map_workspace_bytes(int64_t n, int hx, int hy, int hz, int rx, int ry, int rz, int precision) {
+    (void)n; (void)hx; (void)hy; (void)hz; (void)rx; (void)ry; (void)rz; (void)precision;
+    return 0;
+}
+
+int aldvc_zncc_map(const float* f, const float* g, int64_t nz, int64_t ny, int64_t nx, const int64_t* coords,
+                   int64_t n, int hx, int hy, int hz, int rx, int ry, int rz, const int64_t* wx0, const int64_t* wy0,
+                   const int64_t* wz0, const uint8_t* t_ok, float* out, int precision, void* workspace,
+                   size_t workspace_bytes, hipStream_t stream) {
+    clear_error();
+    (void)workspace;
+    (void)workspace_bytes;
+    if (n < 0) return fail(-2, "negative node count");
+    if (n == 0) return 0;
+    if (!f || !g || !coords || !wx0 || !wy0 || !wz0 || !t_ok || !out) return fail(-2, "null pointer");
+    if (hx < 0 || hy < 0 || hz < 0 || rx < 0 || ry < 0 || rz < 0) return fail(-2, "negative half-width or radius");
+    const int64_t sx = 2 * hx + 1, sy = 2 * hy + 1, sz = 2 * hz + 1;
+    if (sx + 2 * rx > nx || sy + 2 * ry > ny || sz + 2 * rz > nz) return fail(-2, "search window larger than the volume");
+    const int64_t n_tpl = sx * sy * sz;
+    if (n_tpl > LDS_FLOATS_MAX) return fail(-2, "template does not fit in LDS (at most ~40k voxels)");
+    if (n > INT32_MAX) return fail(-2, "too many nodes for one launch");
+    if (precision != 0) return fail(-2, "precision must be 0 (float32)");
+    NccArgs a{f, g, (int)nz, (int)ny, (int)nx, coords, n, hx, hy, hz, rx, ry, rz, wx0, wy0, wz0, t_ok, out};
+    const size_t lds = sizeof(float) * (size_t)n_tpl;
+    if (lds > 64 * 1024 &&
+        hipFuncSetAttribute((const void*)k_zncc_map_f32, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+        return fail(-5, "could not raise the LDS limit for the template");
+    hipLaunchKernelGGL(k_zncc_map_f32, dim3((unsigned)n), dim3(BLOCK), sizeof(float) * n_tpl, stream, a);
+    return launch_status();
+}
+
+int aldvc_ncc_peaks(const float* maps, int64_t n, int rx, int ry, int rz, const double* pinv, int64_t* peak_idx,
+                    double* peak_val, double* pce, double* sub, hipStream_t stream) {
+    clear_error();
+    if (n < 0) return fail(-2, "negative node count");
+    if (n == 0) return 0;
+    if (!maps || !pinv || !peak_idx || !peak_val || !pce || !sub) return fail(-2, "null pointer");
+    if (rx < 0 || ry < 0 || rz < 0) return fail(-2, "negative radius");
+    if (n > INT32_MAX) return fail(-2, "too many nodes for one launch");
+    hipLaunchKernelGGL(k_ncc_peaks, dim3((unsigned)n), dim3(BLOCK), 0, stream, maps, n, rx, ry, rz, pinv, peak_idx,
+                       peak_val, pce, sub);
+    return launch_status();
+}
+
+}  // extern "C"

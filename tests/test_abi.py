@@ -1,0 +1,86 @@
+"""The C ABI library loads and exports every entry point include/aldvc_hip.h declares.
+
+No compute call is made (this runs on CPU-only hosts too); only the size queries and the
+argument checks that return before any HIP call.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+import pytest
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(REPO, "include", "aldvc_hip.h")
+
+
+def declared():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(aldvc_\w+)\s*\(", text)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    from pyaldvc_amd import _lib
+
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.fail(f"{_lib.LIB_PATH} is not built (run __graft_entry__.build())")
+    return _lib.lib()
+
+
+def test_header_declares_the_binding_table():
+    from pyaldvc_amd import _lib
+
+    assert declared() == sorted(_lib.SIGNATURES)
+
+
+def test_library_exports_every_declared_symbol(lib):
+    for name in declared():
+        assert hasattr(lib, name), name
+
+
+def test_abi_version_and_sizes(lib):
+    from pyaldvc_amd import _lib
+
+    assert lib.aldvc_abi_version() == _lib.ABI_VERSION
+    # plain variant: node list + 2048 scratch rows of S floats
+    S = 33 ** 3
+    b0 = lib.aldvc_icgn12_workspace_bytes(10000, 16, 16, 16, 1, 0)
+    b1 = lib.aldvc_icgn12_workspace_bytes(10000, 16, 16, 16, 1, 1)
+    assert b0 - b1 == 4 * 2048 * S
+    assert b1 >= 8 * 10000
+    assert lib.aldvc_icgn3_workspace_bytes(10) >= 80
+
+
+def _default_args(name):
+    from pyaldvc_amd import _lib
+
+    out = []
+    for t in _lib.SIGNATURES[name][1]:
+        if t is ctypes.c_void_p:
+            out.append(None)
+        elif t is ctypes.c_double:
+            out.append(0.0)
+        else:
+            out.append(8)
+    return out
+
+
+def test_argument_errors_are_reported_without_a_gpu(lib):
+    from pyaldvc_amd import _lib
+
+    args = _default_args("aldvc_icgn12")
+    args[16] = 7  # interp mode: rejected before any HIP call
+    assert lib.aldvc_icgn12(*args) == -2
+    assert b"interp mode" in lib.aldvc_last_error()
+    args = _default_args("aldvc_zncc_map")  # null volumes
+    assert lib.aldvc_zncc_map(*args) == -2 and b"null" in lib.aldvc_last_error()
+    args = _default_args("aldvc_icgn3")
+    args[19] = 1
+    args[6] = -1  # negative half-width
+    assert lib.aldvc_icgn3(*args) == -2 and b"half-widths" in lib.aldvc_last_error()
+    with pytest.raises(_lib.HipLibraryError):
+        _lib.check(-2, "probe")

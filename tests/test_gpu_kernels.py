@@ -1,0 +1,216 @@
+"""HIP kernels against the oracle, through the C ABI (needs an MI355X).
+
+Criteria: the reference's own GPU-vs-CPU parity bar (tests/test_cuda_backend.py:149-161) with the
+sub-voxel tolerance tightened to 1e-3 voxel; interpolation and gradients within float32 round-off;
+integer NCC peaks identical.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import cpu_ref, ncc_ref
+from tests.parity import MASK_NONE, OTF, assert_icgn_parity, bead_case, lattice, truth_at
+
+pytestmark = pytest.mark.gpu
+
+K = pytest.importorskip("pyaldvc_amd.kernels")
+
+
+@pytest.fixture(scope="module")
+def case():
+    return bead_case()
+
+
+def test_gradients_match_oracle(golden):
+    gx, gy, gz = K.compute_gradients(golden["f"])
+    for a, b in ((gx, golden["gx"]), (gy, golden["gy"]), (gz, golden["gz"])):
+        assert np.max(np.abs(a - b)) < 2e-6 * max(1.0, float(np.max(np.abs(b))))
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_sample_points_match_oracle(golden, mode):
+    src = golden["g_bspline"] if mode == 2 else golden["g"]
+    args = (golden["pts_z"], golden["pts_y"], golden["pts_x"], mode)
+    got = K.sample_points(src, *args)
+    ref = cpu_ref.sample_points(src, *args)
+    assert np.array_equal(np.isnan(got), np.isnan(ref))
+    ok = ~np.isnan(ref)
+    assert np.max(np.abs(got[ok] - ref[ok])) < 2e-5 * max(1.0, float(np.max(np.abs(ref[ok]))))
+
+
+PRE = {
+    "plain": dict(half=(6, 6, 6), stride=1, grads="stored", mask=None),
+    "mask": dict(half=(6, 6, 6), stride=1, grads="stored", mask="mask"),
+    "stride2": dict(half=(6, 5, 4), stride=2, grads="stored", mask=None),
+    "otf": dict(half=(6, 6, 6), stride=1, grads="otf", mask=None),
+}
+
+
+def _pre_args(golden, key):
+    c = PRE[key]
+    grads = (golden["gx"], golden["gy"], golden["gz"]) if c["grads"] == "stored" else (OTF, OTF, OTF)
+    mask = golden[c["mask"]] if c["mask"] else MASK_NONE
+    return c["half"], c["stride"], grads, mask
+
+
+@pytest.mark.parametrize("key", list(PRE))
+def test_precompute_matches_oracle(golden, key):
+    half, stride, grads, mask = _pre_args(golden, key)
+    args = (golden["coords"], *half, golden["f"], *grads, mask, 0.5, 1e12, stride)
+    H, L, mf, bf, nv, valid = K.precompute_nodes(*args)
+    Hr, Lr, mfr, bfr, nvr, validr = cpu_ref.precompute_nodes(*args)
+    assert np.array_equal(valid, validr)
+    assert np.array_equal(nv, nvr)
+    assert np.max(np.abs(H - Hr)) <= 2e-6 * np.max(np.abs(Hr))
+    assert np.max(np.abs(mf - mfr)) < 1e-5
+    assert np.max(np.abs(bf - bfr) / bfr) < 1e-5
+    v = validr
+    assert np.max(np.abs(L[v] - Lr[v])) <= 1e-5 * np.sqrt(np.max(np.abs(Hr)))
+
+
+CASES12 = {
+    "cubic": dict(pre="plain", g="g", mode=1),
+    "linear": dict(pre="plain", g="g", mode=0),
+    "bspline": dict(pre="plain", g="g_bspline", mode=2),
+    "mask": dict(pre="mask", g="g", mode=1),
+    "defmask": dict(pre="plain", g="g_nan", mode=1),
+    "stride2": dict(pre="stride2", g="g", mode=1),
+    "otf": dict(pre="otf", g="g", mode=1),
+    "status": dict(pre="plain", g="g", mode=1, far=True),
+    "noise": dict(pre="plain", g="g", mode=1, noise=True),
+    "nopred": dict(pre="plain", g="g", mode=1, predictive=False),
+    "stall": dict(pre="plain", g="g_noise", mode=1),
+}
+
+
+def _start(golden, far=False):
+    P0 = np.zeros((golden["coords"].shape[0], 12))
+    P0[:, 9:] = np.round(golden["U_true"])
+    if far:
+        P0[0, 9] = 60.0
+        P0[1, 10] = np.nan
+    return P0
+
+
+@pytest.mark.parametrize("variant", ["plain", "fused"])
+@pytest.mark.parametrize("key", list(CASES12))
+def test_icgn12_matches_oracle(golden, key, variant):
+    c = CASES12[key]
+    half, stride, grads, mask = _pre_args(golden, c["pre"])
+    H, L, mf, bf, nv, valid = cpu_ref.precompute_nodes(golden["coords"], *half, golden["f"], *grads, mask, 0.5, 1e12, stride)
+    noise = c.get("noise", False)
+    args = (
+        golden["coords"], _start(golden, c.get("far", False)), *half, golden["f"], *grads, mask, golden[c["g"]],
+        c["mode"], L, mf, bf, valid, 1e-2, 1e-3, 100, 5, stride, H, golden["pattern_666_s1"] if noise else None,
+        float(golden["stencil_noise_gain"]) if noise else 0.0, c.get("predictive", True),
+    )  # fmt: skip
+    ref = cpu_ref.icgn_12dof_parallel(*args)
+    got = K.icgn_12dof_parallel(*args, variant=variant)
+    if key == "stall":  # a pure-noise deformed volume: the trajectories are chaotic, compare outcomes
+        assert np.array_equal(ref[2] == 0, got[2] == 0)
+        return
+    assert_icgn_parity(ref, got)
+
+
+@pytest.mark.parametrize("variant", ["plain", "fused"])
+def test_icgn12_split_rows_match_oracle(golden, variant):
+    coords = golden["coords"]
+    si = np.arange(coords.shape[0], dtype=np.int64)
+    rows = golden["split_m0_rows"]
+    f, gx, gy, gz, m = golden["f"], golden["gx"], golden["gy"], golden["gz"], golden["split_mask"]
+    H, L, mf, bf, nv, valid = K.precompute_nodes(coords, 6, 6, 6, f, gx, gy, gz, m, 0.5, 1e12, 1, si, rows)
+    assert np.array_equal(valid, golden["pre_split_valid"])
+    assert np.array_equal(nv, golden["pre_split_nvalid"])
+    args = (coords, _start(golden), 6, 6, 6, f, gx, gy, gz, m, golden["g"], 1, golden["pre_split_L"],
+            golden["pre_split_meanf"], golden["pre_split_bottomf"], golden["pre_split_valid"],
+            1e-2, 1e-3, 100, 5, 1, None, None, 0.0, True, si, rows)  # fmt: skip
+    ref = (golden["i12_split_P"], golden["i12_split_n_iter"], golden["i12_split_status"], golden["i12_split_zncc"])
+    assert_icgn_parity(ref, K.icgn_12dof_parallel(*args, variant=variant))
+
+
+@pytest.mark.parametrize("variant", ["plain", "fused"])
+@pytest.mark.parametrize("mode", [1, 0])
+def test_icgn12_lattice_matches_oracle_and_truth(case, variant, mode):
+    """A full node lattice (subset 16, step 8) on a 64x72x80 bead pair under an affine field."""
+    coords = lattice(case["shape"], 16, 8)
+    half = (8, 8, 8)
+    f, g, gx, gy, gz = case["f"], case["g"], case["gx"], case["gy"], case["gz"]
+    H, L, mf, bf, nv, valid = cpu_ref.precompute_nodes(coords, *half, f, gx, gy, gz, MASK_NONE, 0.5, 1e12)
+    U = truth_at(case, coords)
+    P0 = np.zeros((coords.shape[0], 12))
+    P0[:, 9:] = np.round(U)
+    args = (coords, P0, *half, f, gx, gy, gz, MASK_NONE, g, mode, L, mf, bf, valid, 1e-2, 1e-3, 100, 5)
+    ref = cpu_ref.icgn_12dof_parallel(*args)
+    got = K.icgn_12dof_parallel(*args, variant=variant)
+    assert_icgn_parity(ref, got)
+    conv = got[2] == 0
+    assert conv.sum() > 0.5 * valid.sum()
+    err = np.abs(got[0][conv, 9:] - U[conv])
+    assert np.median(err) < (0.05 if mode == 1 else 0.1)
+
+
+@pytest.mark.parametrize("key,mu,noise", [("i3_plain", 1e-3, False), ("i3_mu", 0.5, False), ("i3_noise", 1e-3, True)])
+def test_icgn3_matches_oracle(golden, key, mu, noise):
+    H, L, mf, bf, nv, valid = cpu_ref.precompute_nodes(
+        golden["coords"], 6, 6, 6, golden["f"], golden["gx"], golden["gy"], golden["gz"], MASK_NONE, 0.5, 1e12
+    )
+    pattern = golden["pattern_666_s1"]
+    args = (
+        golden["coords"], golden["i3_U_old"], golden["i3_F"], golden["i3_vdual"], 6, 6, 6, golden["f"],
+        golden["gx"], golden["gy"], golden["gz"], MASK_NONE, golden["g"], 1, H, mf, bf, valid, mu, 1e-2, 1e-3,
+        100, 5, 1, float(pattern[9, 9]), float(golden["stencil_noise_gain"]) if noise else 0.0, True,
+    )  # fmt: skip
+    assert_icgn_parity(cpu_ref.icgn_3dof_parallel(*args), K.icgn_3dof_parallel(*args))
+
+
+def test_evaluate_zncc_matches_oracle(golden):
+    H, L, mf, bf, nv, valid = cpu_ref.precompute_nodes(
+        golden["coords"], 6, 6, 6, golden["f"], golden["gx"], golden["gy"], golden["gz"], MASK_NONE, 0.5, 1e12
+    )
+    args = (golden["coords"], golden["eval_P"], 6, 6, 6, golden["f"], MASK_NONE, golden["g"], 1, mf, bf, valid)
+    z = K.evaluate_zncc_parallel(*args)
+    zr = cpu_ref.evaluate_zncc_parallel(*args)
+    assert np.array_equal(np.isnan(z), np.isnan(zr))
+    ok = ~np.isnan(zr)
+    assert np.max(np.abs(z[ok] - zr[ok])) < 1e-5
+
+
+def test_zncc_map_matches_oracle(golden):
+    hx, hy, hz, rx, ry, rz = (int(v) for v in golden["zd_params"])
+    args = (golden["zd_f"], golden["g"], golden["zd_coords"], hx, hy, hz, rx, ry, rz,
+            golden["zd_wx0"], golden["zd_wy0"], golden["zd_wz0"], golden["zd_tok"])  # fmt: skip
+    got = K.zncc_direct(*args)
+    ref = np.empty(golden["zd_maps"].shape)
+    cpu_ref.zncc_direct(*args, ref)
+    assert np.max(np.abs(got - ref)) < 2e-5
+
+
+@pytest.mark.parametrize("radius,shift", [((3, 3, 3), False), ((2, 3, 2), True), ((4, 4, 4), False)])
+def test_ncc_search_matches_oracle(golden, radius, shift):
+    from pyaldvc_amd import integer_search as gis
+
+    sc = golden["ncc_coords"]
+    sh = golden["ncc_shift0"] if shift else None
+    subset = (8, 6, 8) if shift else (8, 8, 8)
+    got = gis.ncc_search(golden["f"], golden["g"], sc, subset, radius, sh)
+    ref = ncc_ref.ncc_search(golden["f"], golden["g"], sc, subset, radius, sh)
+    assert np.array_equal(got["ok"], ref["ok"])
+    assert np.array_equal(got["clipped"], ref["clipped"])
+    ok = ref["ok"]
+    assert np.array_equal(np.round(got["disp"][ok]), np.round(ref["disp"][ok]))
+    assert np.max(np.abs(got["disp"][ok] - ref["disp"][ok])) < 1e-3
+    assert np.max(np.abs(got["cc"][ok] - ref["cc"][ok])) < 1e-5
+
+
+def test_ncc_search_lattice_integer_peaks(case):
+    """Integer peaks of a whole lattice equal the float64 oracle's; sub-voxel within 1e-3 voxel."""
+    from pyaldvc_amd import integer_search as gis
+
+    coords = lattice(case["shape"], 16, 8)
+    got = gis.ncc_search(case["f"], case["g"], coords, (16, 16, 16), (3, 3, 3))
+    ref = ncc_ref.ncc_search(case["f"], case["g"], coords, (16, 16, 16), (3, 3, 3))
+    assert np.array_equal(got["ok"], ref["ok"])
+    ok = ref["ok"]
+    assert np.max(np.abs(got["disp"][ok] - ref["disp"][ok])) < 1e-3
