@@ -151,12 +151,14 @@ def cpu_baseline(W, pre, P0, target_s=15.0):
                                           1e-2, 1e-3, 100, 5)  # fmt: skip
         return time.perf_counter() - t0, int(vd.sum()), out
 
-    probe = np.sort(rng.choice(idx, size=min(idx.size, 4 * threads), replace=False))
-    t_probe, n_probe, _ = run(probe)
-    per_node = t_probe / max(n_probe, 1)
-    n = int(min(idx.size, max(n_probe, target_s / max(per_node, 1e-6))))
-    sample = np.sort(rng.choice(idx, size=n, replace=False))
-    t, nv, out = run(sample)
+    # size the sample from a probe so the timed run is about target_s seconds of CPU work
+    n = min(idx.size, 64 * threads)
+    for _ in range(3):
+        sample = np.sort(rng.choice(idx, size=n, replace=False))
+        t, nv, out = run(sample)
+        if t >= 0.5 * target_s or n >= idx.size:
+            break
+        n = int(min(idx.size, n * target_s / max(t, 1e-3)))
     del torch
     return {
         "value": nv / t,

@@ -133,6 +133,13 @@ __device__ inline float sample_rel(const float* __restrict__ base, int rowp, int
             bspline4(fz, wz0, wz1, wz2, wz3);
         }
         const float* p = base + ((iz - 1) * planep + (iy - 1) * rowp + (ix - 1));
+        // all 16 row loads are issued before any of them is consumed: a gather that waits for
+        // each row in turn is bound by the load latency, not by the memory pipeline
+        float4 r[16];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) r[4 * k + j] = load4(p + k * planep + j * rowp);
         const float wz[4] = {wz0, wz1, wz2, wz3};
         const float wy[4] = {wy0, wy1, wy2, wy3};
         float val = 0.0f;
@@ -141,8 +148,8 @@ __device__ inline float sample_rel(const float* __restrict__ base, int rowp, int
             float plane = 0.0f;
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                const float4 r = load4(p + k * planep + j * rowp);
-                plane += wy[j] * (r.x * wx0 + r.y * wx1 + r.z * wx2 + r.w * wx3);
+                const float4 q = r[4 * k + j];
+                plane += wy[j] * (q.x * wx0 + q.y * wx1 + q.z * wx2 + q.w * wx3);
             }
             val += wz[k] * plane;
         }

@@ -467,6 +467,11 @@ __global__ __launch_bounds__(BLOCK) void k_icgn12_fused(IcgnArgs a, const int64_
                 k = !V.mask || V.mask[o] != 0;
             }
             if (!k) continue;
+            // reference-side values first: they do not depend on the warp, so their loads
+            // overlap the gather of the deformed sample
+            const float fraw = V.f[o];
+            float gr[3];
+            grad_at<STORED>(V, o, (int)z0 + dz, (int)y0 + dy, (int)x0 + dx, gr[0], gr[1], gr[2]);
             m[1] += 1.0f;
             const float X = (float)dx, Y = (float)dy, Z = (float)dz;
             const float xr = sw[0] * X + sw[1] * Y + sw[2] * Z + sw[9];
@@ -478,9 +483,7 @@ __global__ __launch_bounds__(BLOCK) void k_icgn12_fused(IcgnArgs a, const int64_
             }
             const float val = sample_rel<MODE>(gbase, rowp, planep, xr, yr, zr, dom.kx, dom.ky, dom.kz);
             if (val != val) continue;
-            const float fv = V.f[o] - pf, gv = val - pg;
-            float gr[3];
-            grad_at<STORED>(V, o, (int)z0 + dz, (int)y0 + dy, (int)x0 + dx, gr[0], gr[1], gr[2]);
+            const float fv = fraw - pf, gv = val - pg;
             m[0] += 1.0f;
             m[3] += fv;
             m[4] += fv * fv;
@@ -655,6 +658,11 @@ __global__ __launch_bounds__(BLOCK) void k_icgn3_fused(Icgn3Args a, const int64_
                 k = !V.mask || V.mask[o] != 0;
             }
             if (!k) continue;
+            // reference-side values first: they do not depend on the warp, so their loads
+            // overlap the gather of the deformed sample
+            const float fraw = V.f[o];
+            float gr[3];
+            grad_at<STORED>(V, o, (int)z0 + dz, (int)y0 + dy, (int)x0 + dx, gr[0], gr[1], gr[2]);
             m[1] += 1.0f;
             const float X = (float)dx, Y = (float)dy, Z = (float)dz;
             const float xr = sw[0] * X + sw[1] * Y + sw[2] * Z + sw[9];
@@ -666,9 +674,7 @@ __global__ __launch_bounds__(BLOCK) void k_icgn3_fused(Icgn3Args a, const int64_
             }
             const float val = sample_rel<MODE>(gbase, rowp, planep, xr, yr, zr, dom.kx, dom.ky, dom.kz);
             if (val != val) continue;
-            const float fv = V.f[o] - pf, gv = val - pg;
-            float gr[3];
-            grad_at<STORED>(V, o, (int)z0 + dz, (int)y0 + dy, (int)x0 + dx, gr[0], gr[1], gr[2]);
+            const float fv = fraw - pf, gv = val - pg;
             m[0] += 1.0f;
             m[3] += fv;
             m[4] += fv * fv;

@@ -153,6 +153,266 @@ __global__ __launch_bounds__(BLOCK) void k_zncc_map_f32(NccArgs a) {
     }
 }
 
+// ---------------------------------------------------------------------------- BF16 MFMA maps
+// The correlation C[oz][oy][ox] = sum_{tz,ty,tx} T~[tz][ty][tx] (W[oz+tz][oy+ty][ox+tx] - c) is
+// a sum over template rows (tz, ty) of 1-D correlations along x, and a 1-D correlation with a
+// fixed row is a product with a banded (Toeplitz) matrix: for one template row,
+//     D[oy][ox] += sum_k W[z][oy + ty][k] * Band[k][ox],   Band[k][ox] = T~[tz][ty][k - ox],
+// a GEMM with M = oy, N = ox, K = window columns. v_mfma_f32_16x16x32_bf16 computes a 16x16
+// output tile over a 32-wide K slice; an output x-tile of 16 columns needs K = 15 + sx window
+// columns (KS slices). Operands are bfloat16 (window centred by the template mean c before the
+// rounding, template centred by its mean), accumulation float32. The window box sums of the
+// normalisation (sum g, sum g^2 over every template-sized box) are exact float32 sliding sums,
+// so only the numerator sees bfloat16 rounding.
+//
+// One workgroup = 4 wave64 = 4 consecutive output planes oz0 .. oz0 + 3 (one per wave) of one
+// node. The window planes z = oz0 .. oz0 + 3 + sz - 2 are staged through LDS once per
+// workgroup; plane z feeds wave g with template plane tz = z - oz0 - g, so every staged plane
+// serves all four waves.
+//
+// MFMA operand layout (16x16x32, wave64): lane l holds A[m = l % 16][k = 8 (l / 16) + i] and
+// B[k = 8 (l / 16) + i][n = l % 16] for i < 8, and D[m = 4 (l / 16) + i][n = l % 16] for i < 4.
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+constexpr int MG = 4;     // output planes per workgroup (one per wave)
+constexpr int TRING = 4;  // template planes held in LDS
+
+struct Bf16Geom {
+    int sx, sy, sz, vx, vy, vz, wx, wy, wz;
+    int nyt, nxt;  // 16-row / 16-column output tiles
+    int AH, AW;    // staged window plane (rows, columns) incl. zero padding
+    int TW;        // staged template row width: x in [-16, 32 KS)
+};
+
+__device__ inline float bf16_round_trip(float v) { return (float)(__bf16)v; }
+
+template <int NT, int KS>
+__global__ __launch_bounds__(256) void k_zncc_map_bf16(NccArgs a, Bf16Geom q) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int n_zgrp = (q.vz + MG - 1) / MG;
+    const int64_t node = blockIdx.x / n_zgrp;
+    const int oz0 = (int)(blockIdx.x % n_zgrp) * MG;
+    const int64_t n_off = (int64_t)q.vx * q.vy * q.vz;
+    float* out = a.out + node * n_off;
+    const int64_t x0 = a.coords[3 * node], y0 = a.coords[3 * node + 1], z0 = a.coords[3 * node + 2];
+    const int64_t xb = a.wx0[node], yb = a.wy0[node], zb = a.wz0[node];
+    const bool tpl_in = x0 - a.hx >= 0 && x0 + a.hx < a.nx && y0 - a.hy >= 0 && y0 + a.hy < a.ny && z0 - a.hz >= 0 &&
+                        z0 + a.hz < a.nz;
+    const bool win_in = xb >= 0 && xb + q.wx <= a.nx && yb >= 0 && yb + q.wy <= a.ny && zb >= 0 && zb + q.wz <= a.nz;
+    const int my_oz = oz0 + wave;
+    const int n_tpl = q.sx * q.sy * q.sz;
+    auto fill_mine = [&](float v) {
+        if (my_oz >= q.vz) return;
+        float* o = out + (int64_t)my_oz * q.vy * q.vx;
+        for (int i = lane; i < q.vy * q.vx; i += 64) o[i] = v;
+    };
+    if (!a.t_ok[node] || !tpl_in || !win_in) {
+        fill_mine(-2.0f);
+        return;
+    }
+    // ---- LDS carve-up
+    __bf16* Ap = (__bf16*)lds;                                    // AH x AW window plane (bf16)
+    __bf16* Tr = Ap + q.AH * q.AW;                                // TRING x sy x TW template planes
+    float* Pf = (float*)(Tr + TRING * q.sy * q.TW);               // wy x wx float32 plane
+    float* R1 = Pf + q.wy * q.wx;                                 // wy x vx row box sums
+    float* R2 = R1 + q.wy * q.vx;
+    float* P1 = R2 + q.wy * q.vx;                                 // vy x vx plane box sums
+    float* P2 = P1 + q.vy * q.vx;
+    __shared__ double red[4];
+    __shared__ double stat[2];
+    // ---- template mean and energy (float64)
+    const int64_t tbase = ((z0 - a.hz) * a.ny + (y0 - a.hy)) * (int64_t)a.nx + (x0 - a.hx);
+    double sacc = 0.0;
+    for (int v = tid; v < n_tpl; v += 256) {
+        const int k = v / (q.sy * q.sx), r = v - k * q.sy * q.sx, j = r / q.sx, i = r - j * q.sx;
+        sacc += a.f[tbase + ((int64_t)k * a.ny + j) * a.nx + i];
+    }
+    sacc = wave_sum(sacc);
+    if (lane == 0) red[wave] = sacc;
+    __syncthreads();
+    if (tid == 0) stat[0] = (red[0] + red[1] + red[2] + red[3]) / n_tpl;
+    __syncthreads();
+    const double mf = stat[0];
+    double ss = 0.0;
+    for (int v = tid; v < n_tpl; v += 256) {
+        const int k = v / (q.sy * q.sx), r = v - k * q.sy * q.sx, j = r / q.sx, i = r - j * q.sx;
+        const double d = (double)a.f[tbase + ((int64_t)k * a.ny + j) * a.nx + i] - mf;
+        ss += d * d;
+    }
+    ss = wave_sum(ss);
+    __syncthreads();
+    if (lane == 0) red[wave] = ss;
+    __syncthreads();
+    if (tid == 0) stat[1] = red[0] + red[1] + red[2] + red[3];
+    __syncthreads();
+    const double ssf = stat[1];
+    if (ssf < 1e-20) {
+        fill_mine(-2.0f);
+        return;
+    }
+    const float c = (float)mf;
+    // zero the padded staging buffers once (data regions are rewritten per plane)
+    for (int i = tid; i < q.AH * q.AW; i += 256) Ap[i] = (__bf16)0.0f;
+    for (int i = tid; i < TRING * q.sy * q.TW; i += 256) Tr[i] = (__bf16)0.0f;
+    // ---- per-wave state: accumulators and box sums in the D-tile layout
+    f32x4_t acc[NT][NT];
+    f32x4_t S1[NT][NT], S2[NT][NT];
+#pragma unroll
+    for (int mt = 0; mt < NT; ++mt)
+#pragma unroll
+        for (int j = 0; j < NT; ++j) {
+            acc[mt][j] = (f32x4_t){0.f, 0.f, 0.f, 0.f};
+            S1[mt][j] = (f32x4_t){0.f, 0.f, 0.f, 0.f};
+            S2[mt][j] = (f32x4_t){0.f, 0.f, 0.f, 0.f};
+        }
+    const int am = lane & 15, akb = 8 * (lane >> 4);  // A row / K offset, also B column / K offset
+    const int z_end = min(oz0 + MG + q.sz - 1, q.wz);
+    for (int z = oz0; z < z_end; ++z) {
+        __syncthreads();  // the previous plane's consumers are done with every buffer
+        // stage window plane z (centred) and template plane tz_new = z - oz0
+        const float* gp = a.g + ((zb + z) * a.ny + yb) * (int64_t)a.nx + xb;
+        for (int i = tid; i < q.wy * q.wx; i += 256) {
+            const int y = i / q.wx, x = i - y * q.wx;
+            const float v = gp[(int64_t)y * a.nx + x] - c;
+            Pf[i] = v;
+            Ap[y * q.AW + x] = (__bf16)v;
+        }
+        const int tz_new = z - oz0;
+        if (tz_new < q.sz) {
+            __bf16* tp = Tr + (tz_new % TRING) * q.sy * q.TW;
+            const float* fp = a.f + tbase + (int64_t)tz_new * a.ny * a.nx;
+            for (int i = tid; i < q.sy * q.sx; i += 256) {
+                const int y = i / q.sx, x = i - y * q.sx;
+                tp[y * q.TW + 16 + x] = (__bf16)(float)((double)fp[(int64_t)y * a.nx + x] - mf);
+            }
+        }
+        __syncthreads();
+        // row box sums over x (sliding), then column box sums over y
+        for (int t = tid; t < 2 * q.wy; t += 256) {
+            const int y = t >> 1, sq = t & 1;
+            const float* row = Pf + y * q.wx;
+            float* R = (sq ? R2 : R1) + y * q.vx;
+            float r = 0.0f;
+            for (int x = 0; x < q.sx; ++x) r += sq ? row[x] * row[x] : row[x];
+            R[0] = r;
+            for (int ox = 1; ox < q.vx; ++ox) {
+                const float vin = row[ox + q.sx - 1], vout = row[ox - 1];
+                r += sq ? (vin * vin - vout * vout) : (vin - vout);
+                R[ox] = r;
+            }
+        }
+        __syncthreads();
+        for (int t = tid; t < 2 * q.vx; t += 256) {
+            const int ox = t >> 1, sq = t & 1;
+            const float* R = sq ? R2 : R1;
+            float* P = sq ? P2 : P1;
+            float r = 0.0f;
+            for (int y = 0; y < q.sy; ++y) r += R[y * q.vx + ox];
+            P[ox] = r;
+            for (int oy = 1; oy < q.vy; ++oy) {
+                r += R[(oy + q.sy - 1) * q.vx + ox] - R[(oy - 1) * q.vx + ox];
+                P[oy * q.vx + ox] = r;
+            }
+        }
+        __syncthreads();
+        const int tz = z - my_oz;
+        if (my_oz < q.vz && tz >= 0 && tz < q.sz) {
+            // box sums of this plane into the wave's tiles
+#pragma unroll
+            for (int mt = 0; mt < NT; ++mt)
+#pragma unroll
+                for (int j = 0; j < NT; ++j) {
+                    const int ox = 16 * j + (lane & 15);
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const int oy = 16 * mt + 4 * (lane >> 4) + i;
+                        if (mt < q.nyt && j < q.nxt && oy < q.vy && ox < q.vx) {
+                            S1[mt][j][i] += P1[oy * q.vx + ox];
+                            S2[mt][j][i] += P2[oy * q.vx + ox];
+                        }
+                    }
+                }
+            const __bf16* tpl = Tr + (tz % TRING) * q.sy * q.TW;
+            for (int ty = 0; ty < q.sy; ++ty) {
+                bf16x8_t bfr[KS];
+#pragma unroll
+                for (int s = 0; s < KS; ++s) {
+                    // B[k][n] = T~[tz][ty][32 s + k - n]; staged row index x + 16
+                    const __bf16* trow = tpl + ty * q.TW + 16 + 32 * s + akb - am;
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) bfr[s][i] = trow[i];
+                }
+#pragma unroll
+                for (int mt = 0; mt < NT; ++mt) {
+                    if (mt >= q.nyt) break;
+                    const __bf16* arow = Ap + (16 * mt + am + ty) * q.AW + akb;
+#pragma unroll
+                    for (int j = 0; j < NT; ++j) {
+                        if (j >= q.nxt) break;
+#pragma unroll
+                        for (int s = 0; s < KS; ++s) {
+                            bf16x8_t afr;
+                            __builtin_memcpy(&afr, arow + 16 * j + 32 * s, sizeof(afr));
+                            acc[mt][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr, bfr[s], acc[mt][j], 0, 0, 0);
+                        }
+                    }
+                }
+            }
+        }
+    }
+    if (my_oz >= q.vz) return;
+    const float inv_n = 1.0f / (float)n_tpl;
+    const float sq_ssf = (float)sqrt(ssf);
+    float* o = out + (int64_t)my_oz * q.vy * q.vx;
+#pragma unroll
+    for (int mt = 0; mt < NT; ++mt)
+#pragma unroll
+        for (int j = 0; j < NT; ++j) {
+            const int ox = 16 * j + (lane & 15);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int oy = 16 * mt + 4 * (lane >> 4) + i;
+                if (mt < q.nyt && j < q.nxt && oy < q.vy && ox < q.vx) {
+                    const float var = S2[mt][j][i] - S1[mt][j][i] * S1[mt][j][i] * inv_n;
+                    float val = 0.0f;
+                    if (var >= 1e-20f) {
+                        val = acc[mt][j][i] / (sqrtf(var) * sq_ssf);
+                        val = val > 1.0f ? 1.0f : (val < -1.0f ? -1.0f : val);
+                    }
+                    o[oy * q.vx + ox] = val;
+                }
+            }
+        }
+}
+
+size_t bf16_lds_bytes(const Bf16Geom& q) {
+    return sizeof(__bf16) * ((size_t)q.AH * q.AW + (size_t)TRING * q.sy * q.TW) +
+           sizeof(float) * ((size_t)q.wy * q.wx + 2 * (size_t)q.wy * q.vx + 2 * (size_t)q.vy * q.vx);
+}
+
+Bf16Geom bf16_geom(int hx, int hy, int hz, int rx, int ry, int rz, int& ks) {
+    Bf16Geom q;
+    q.sx = 2 * hx + 1;
+    q.sy = 2 * hy + 1;
+    q.sz = 2 * hz + 1;
+    q.vx = 2 * rx + 1;
+    q.vy = 2 * ry + 1;
+    q.vz = 2 * rz + 1;
+    q.wx = q.sx + q.vx - 1;
+    q.wy = q.sy + q.vy - 1;
+    q.wz = q.sz + q.vz - 1;
+    q.nyt = (q.vy + 15) / 16;
+    q.nxt = (q.vx + 15) / 16;
+    ks = (q.sx + 15 + 31) / 32;
+    q.AH = 16 * q.nyt + q.sy - 1;
+    q.AW = 16 * (q.nxt - 1) + 32 * ks;
+    q.AW = (q.AW + 7) / 8 * 8;
+    q.TW = 32 * ks + 16;
+    return q;
+}
+
 // ---------------------------------------------------------------------------- peak epilogue
 // Per node (integer_search.py:444-477): the first index of the maximum in C order (a NaN wins,
 // as in np.argmax), the minimum, PCE = (peak - min)^2 / mean((C - min)^2) (0 when the energy is
@@ -303,8 +563,36 @@ int aldvc_zncc_map(const float* f, const float* g, int64_t nz, int64_t ny, int64
     const int64_t n_tpl = sx * sy * sz;
     if (n_tpl > LDS_FLOATS_MAX) return fail(-2, "template does not fit in LDS (at most ~40k voxels)");
     if (n > INT32_MAX) return fail(-2, "too many nodes for one launch");
-    if (precision != 0) return fail(-2, "precision must be 0 (float32)");
+    if (precision != 0 && precision != 1) return fail(-2, "precision must be 0 (float32) or 1 (bfloat16 MFMA)");
     NccArgs a{f, g, (int)nz, (int)ny, (int)nx, coords, n, hx, hy, hz, rx, ry, rz, wx0, wy0, wz0, t_ok, out};
+    if (precision == 1) {
+        int ks = 0;
+        const Bf16Geom q = bf16_geom(hx, hy, hz, rx, ry, rz, ks);
+        const int nt = std::max(q.nyt, q.nxt);
+        if (nt > 3) return fail(-2, "bfloat16 maps support search radii up to 23 voxels in x and y");
+        if (ks > 2) return fail(-2, "bfloat16 maps support templates up to 49 voxels along x");
+        const size_t lds = bf16_lds_bytes(q);
+        if (lds > 160 * 1024 - 256) return fail(-2, "template and window planes do not fit in LDS");
+        const int64_t nblk = n * ((q.vz + MG - 1) / MG);
+        if (nblk > INT32_MAX) return fail(-2, "too many workgroups for one launch");
+        const void* fn = nullptr;
+#define ALDVC_BF16(NTV, KSV)                                                                                \
+    if (nt <= NTV && ks == KSV) {                                                                          \
+        fn = (const void*)k_zncc_map_bf16<NTV, KSV>;                                                       \
+        if (lds > 64 * 1024 && hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess) \
+            return fail(-5, "could not raise the LDS limit");                                              \
+        hipLaunchKernelGGL((k_zncc_map_bf16<NTV, KSV>), dim3((unsigned)nblk), dim3(256), lds, stream, a, q); \
+        return launch_status();                                                                            \
+    }
+        ALDVC_BF16(1, 1)
+        ALDVC_BF16(1, 2)
+        ALDVC_BF16(2, 1)
+        ALDVC_BF16(2, 2)
+        ALDVC_BF16(3, 1)
+        ALDVC_BF16(3, 2)
+#undef ALDVC_BF16
+        return fail(-2, "unsupported bfloat16 geometry");
+    }
     const size_t lds = sizeof(float) * (size_t)n_tpl;
     if (lds > 64 * 1024 &&
         hipFuncSetAttribute((const void*)k_zncc_map_f32, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)

@@ -384,13 +384,16 @@ def zncc_direct(f, g, coords, hx, hy, hz, rx, ry, rz, wx0, wy0, wz0, t_ok, out=N
     prec = {"fp32": 0, "bf16": 1}[precision]
     maps = torch.empty((n, 2 * rz + 1, 2 * ry + 1, 2 * rx + 1), dtype=torch.float32, device=fd.device)
     if n:
+        # every device argument is bound to a name: a temporary's memory would go back to the
+        # caching allocator (and be handed to the next temporary) before the launch
+        x0d, y0d, z0d = A.arr(wx0, np.int64), A.arr(wy0, np.int64), A.arr(wz0, np.int64)
+        okd = A.arr(t_ok, np.uint8)
         ws_bytes = int(lib().aldvc_zncc_map_workspace_bytes(n, hx, hy, hz, rx, ry, rz, prec))
         ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=fd.device)
         check(
             lib().aldvc_zncc_map(
                 _ptr(fd), _ptr(gd), nz, ny, nx, _ptr(c), n, int(hx), int(hy), int(hz), int(rx), int(ry), int(rz),
-                _ptr(A.arr(wx0, np.int64)), _ptr(A.arr(wy0, np.int64)), _ptr(A.arr(wz0, np.int64)),
-                _ptr(A.arr(t_ok, np.uint8)), _ptr(maps), prec, _ptr(ws), ws_bytes, stream_handle(),
+                _ptr(x0d), _ptr(y0d), _ptr(z0d), _ptr(okd), _ptr(maps), prec, _ptr(ws), ws_bytes, stream_handle(),
             ),
             "aldvc_zncc_map",
         )  # fmt: skip

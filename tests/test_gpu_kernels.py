@@ -214,3 +214,63 @@ def test_ncc_search_lattice_integer_peaks(case):
     assert np.array_equal(got["ok"], ref["ok"])
     ok = ref["ok"]
     assert np.max(np.abs(got["disp"][ok] - ref["disp"][ok])) < 1e-3
+
+
+# --------------------------------------------------------------------------- bfloat16 MFMA maps
+
+
+def test_zncc_map_bf16_close_to_oracle(golden):
+    """BF16 operands with float32 accumulation and float32 box sums: maps within bf16 rounding."""
+    hx, hy, hz, rx, ry, rz = (int(v) for v in golden["zd_params"])
+    args = (golden["zd_f"], golden["g"], golden["zd_coords"], hx, hy, hz, rx, ry, rz,
+            golden["zd_wx0"], golden["zd_wy0"], golden["zd_wz0"], golden["zd_tok"])  # fmt: skip
+    got = K.zncc_direct(*args, precision="bf16")
+    ref = np.empty(golden["zd_maps"].shape)
+    cpu_ref.zncc_direct(*args, ref)
+    assert np.array_equal(got == -2.0, ref == -2.0)  # t_ok = 0 and flat templates
+    assert np.max(np.abs(got - ref)) < 1e-2
+    for n in range(ref.shape[0]):
+        if ref[n, 0, 0, 0] > -1.5:
+            assert np.argmax(got[n]) == np.argmax(ref[n])
+
+
+def test_ncc_search_bf16_lattice_peaks(case):
+    """Integer peaks of a whole lattice from the BF16 maps equal the float64 oracle's."""
+    from pyaldvc_amd import integer_search as gis
+
+    coords = lattice(case["shape"], 16, 8)
+    got = gis.ncc_search(case["f"], case["g"], coords, (16, 16, 16), (3, 3, 3), precision="bf16")
+    ref = ncc_ref.ncc_search(case["f"], case["g"], coords, (16, 16, 16), (3, 3, 3))
+    assert np.array_equal(got["ok"], ref["ok"])
+    ok = ref["ok"]
+    d = np.abs(got["disp"][ok] - ref["disp"][ok])
+    assert np.mean(np.all(d < 0.5, axis=1)) >= 0.99  # same integer peak
+    assert np.median(d) < 0.02
+
+
+@pytest.mark.parametrize("radius", [(16, 16, 16), (9, 12, 5)])
+def test_zncc_map_bf16_large_template_matches_fp32(radius):
+    """24^3 subsets (25^3 templates, two K slices) with 16-voxel radii, the shape of the BF16
+    NCC configuration, on a 128^3 bead pair: same integer peaks as the float32 maps."""
+    from pyaldvc_amd import synthetic as syn
+
+    shape = (128, 128, 128)
+    disp = syn.sinusoid_z(6.0, 256.0)
+    f_raw, g_raw = syn.bead_pair_np(shape, int(0.004 * np.prod(shape)), 1.5, disp, seed=21)
+    f, g = syn.normalize_np(f_raw), syn.normalize_np(g_raw)
+    coords = np.array([[64, 64, 64], [40, 50, 60], [80, 70, 48], [56, 88, 72], [72, 40, 80]], dtype=np.int64)
+    rx, ry, rz = radius
+    h = 12
+    wx0 = coords[:, 0] - h - rx
+    wy0 = coords[:, 1] - h - ry
+    wz0 = coords[:, 2] - h - rz
+    ok = np.ones(coords.shape[0], np.uint8)
+    m32 = K.zncc_direct(f, g, coords, h, h, h, rx, ry, rz, wx0, wy0, wz0, ok)
+    m16 = K.zncc_direct(f, g, coords, h, h, h, rx, ry, rz, wx0, wy0, wz0, ok, precision="bf16")
+    assert np.max(np.abs(m16 - m32)) < 1e-2
+    for n in range(coords.shape[0]):
+        assert np.argmax(m16[n]) == np.argmax(m32[n])
+    # the float32 map of one node against the float64 oracle
+    ref = np.empty((1,) + m32.shape[1:])
+    cpu_ref.zncc_direct(f, g, coords[:1], h, h, h, rx, ry, rz, wx0[:1], wy0[:1], wz0[:1], ok[:1], ref)
+    assert np.max(np.abs(m32[0] - ref[0])) < 2e-5
