@@ -191,6 +191,8 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=2, help="timed steps of the plain kernel (0 skips it)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="run one device copy of this many MiB first (the scale of counter-based HBM traffic)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the local step shards by node blocks with no exchange, "
@@ -202,6 +204,14 @@ def main():
     from pyaldvc_amd import kernels as K
 
     dev = K.device()
+    if args.copy_probe_mib > 0:
+        n = args.copy_probe_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy probe: {args.copy_probe_mib} MiB read + {args.copy_probe_mib} MiB written")
+        del src, dst
     W = make_workload(dev)
     cd = torch.from_numpy(W["coords"]).to(dev)
     N = int(cd.shape[0])

@@ -157,6 +157,87 @@ __device__ inline float sample_rel(const float* __restrict__ base, int rowp, int
     }
 }
 
+// Buffer-resource sampler: the volume is addressed through a buffer descriptor, so one 32-bit
+// VGPR byte offset serves all 16 row loads of a sample and the row offsets (k * plane + j * row)
+// are wave-uniform SGPR offsets. Requires the volume to span fewer than 2^32 bytes. `anchor` is
+// the linear index of the anchor voxel; everything else as sample_rel.
+constexpr unsigned BUF_DWORD3 = 0x00020000u;  // gfx9 raw-buffer descriptor word 3 (32-bit data)
+
+struct BufVol {
+    __amdgpu_buffer_rsrc_t rs;
+    int rowb, planeb;  // row / plane strides in bytes
+};
+
+__device__ inline BufVol make_bufvol(const float* vol, int64_t nvox, int rowp, int planep) {
+    BufVol b;
+    b.rs = __builtin_amdgcn_make_buffer_rsrc((void*)vol, (short)0, (int)(uint32_t)(nvox * 4), (int)BUF_DWORD3);
+    b.rowb = 4 * rowp;
+    b.planeb = 4 * planep;
+    return b;
+}
+
+__device__ inline float4 buf_load4(const BufVol& b, uint32_t voff, int soff) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(b.rs, voff, soff, 0);
+    float4 f;
+    __builtin_memcpy(&f, &v, sizeof(f));
+    return f;
+}
+
+template <int MODE>
+__device__ inline float sample_buf(const BufVol& b, int anchor, int rowp, int planep, float rx, float ry, float rz,
+                                   int kx, int ky, int kz) {
+    int ix = (int)floorf(rx), iy = (int)floorf(ry), iz = (int)floorf(rz);
+    ix = ix > kx ? kx : ix;
+    iy = iy > ky ? ky : iy;
+    iz = iz > kz ? kz : iz;
+    const float fx = rx - (float)ix, fy = ry - (float)iy, fz = rz - (float)iz;
+    if constexpr (MODE == INTERP_LINEAR) {
+        const uint32_t voff = 4u * (uint32_t)(anchor + iz * planep + iy * rowp + ix);
+        const float2 r00 = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(b.rs, voff, 0, 0));
+        const float2 r10 = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(b.rs, voff, b.rowb, 0));
+        const float2 r01 = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(b.rs, voff, b.planeb, 0));
+        const float2 r11 = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(b.rs, voff, b.planeb + b.rowb, 0));
+        const float c00 = r00.x * (1.0f - fx) + r00.y * fx;
+        const float c10 = r10.x * (1.0f - fx) + r10.y * fx;
+        const float c01 = r01.x * (1.0f - fx) + r01.y * fx;
+        const float c11 = r11.x * (1.0f - fx) + r11.y * fx;
+        const float c0 = c00 * (1.0f - fy) + c10 * fy;
+        const float c1 = c01 * (1.0f - fy) + c11 * fy;
+        return c0 * (1.0f - fz) + c1 * fz;
+    } else {
+        float wx0, wx1, wx2, wx3, wy0, wy1, wy2, wy3, wz0, wz1, wz2, wz3;
+        if constexpr (MODE == INTERP_CUBIC) {
+            keys4(fx, wx0, wx1, wx2, wx3);
+            keys4(fy, wy0, wy1, wy2, wy3);
+            keys4(fz, wz0, wz1, wz2, wz3);
+        } else {
+            bspline4(fx, wx0, wx1, wx2, wx3);
+            bspline4(fy, wy0, wy1, wy2, wy3);
+            bspline4(fz, wz0, wz1, wz2, wz3);
+        }
+        const uint32_t voff = 4u * (uint32_t)(anchor + (iz - 1) * planep + (iy - 1) * rowp + (ix - 1));
+        float4 r[16];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) r[4 * k + j] = buf_load4(b, voff, k * b.planeb + j * b.rowb);
+        const float wz[4] = {wz0, wz1, wz2, wz3};
+        const float wy[4] = {wy0, wy1, wy2, wy3};
+        float val = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            float plane = 0.0f;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const float4 q = r[4 * k + j];
+                plane += wy[j] * (q.x * wx0 + q.y * wx1 + q.z * wx2 + q.w * wx3);
+            }
+            val += wz[k] * plane;
+        }
+        return val;
+    }
+}
+
 // Admissible sampling domain (interp_kernels.py:127-136) in anchor-relative coordinates:
 // lo = 1 - a, hi = n - 3 - a per axis.
 struct Domain {

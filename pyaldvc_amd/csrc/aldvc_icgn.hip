@@ -384,36 +384,117 @@ __global__ __launch_bounds__(BLOCK) void k_icgn12_plain(IcgnArgs a, const int64_
 }
 
 // ---------------------------------------------------------------------------- fused kernel
-// Moment layout of the one-pass accumulation (per lane float32, reduced in float64):
-//   0 n_valid, 1 n_ref, 2 oob, 3 sum f', 4 sum f'^2, 5 sum g', 6 sum g'^2, 7 sum f'g',
-//   8 + 4c + j   : sum grad_c * q_j          (C)
-//   20 + 4c + j  : sum grad_c * q_j * f'     (A)
-//   32 + 4c + j  : sum grad_c * q_j * g'     (B)
-// with q = (X, Y, Z, 1) and f' = f - pf, g' = g - pg (pilot shifts that keep float32 sums centred).
-constexpr int NM = 44;
+// The ZNSSD gradient of one iteration needs, over the iteration's valid voxel set V,
+//   b_k = sum SD_k (f - m_f) - (b_f / b_g) sum SD_k (g - m_g)            (numba_kernels.py:705-729)
+// with SD_k = grad_c(f) q_j, q = (X, Y, Z, 1). Every term is a raw moment: the reference-side ones
+// (n, sum f', sum f'^2, sum grad_c q_j, sum grad_c q_j f') depend only on V, which is the node's
+// kept set minus the voxels whose deformed sample is NaN (a sample outside the domain ends the
+// node). So they are summed once per call over the kept set (k_ref_moments), and an iteration
+// accumulates only the deformed-side moments; when some samples are NaN, their voxels are
+// flagged in an LDS bitmask and a second pass sums their reference-side moments, which are
+// subtracted. f' = f - pf and g' = g - pg are pilot-shifted (pf: precomputed reference mean,
+// pg: the previous iteration's deformed mean) so that the float32 lane sums stay centred.
+constexpr int NR = 27;  // 0 n, 1 sum f', 2 sum f'^2, 3 + 4c + j: sum grad_c q_j, 15 + 4c + j: sum grad_c q_j f'
+constexpr int NG = 18;  // 0 n_valid, 1 oob, 2 n_nan, 3 sum g', 4 sum g'^2, 5 sum f'g', 6 + 4c + j: sum grad_c q_j g'
+constexpr int MOM_STRIDE = 32;  // doubles per node in the moments workspace
 
-template <int MODE, bool STORED>
+// XCD-aware slot of workgroup b over the M valid nodes: workgroup b runs on XCD b % 8, and each
+// XCD gets a contiguous run of the node list so its concurrent nodes share subset voxels in its
+// L2. Returns -1 for the workgroups beyond M (the grid is sized for every node).
+__device__ inline int64_t xcd_slot(int64_t b, int64_t M) {
+    const int64_t per = (M + 7) / 8;
+    if ((b >> 3) >= per) return -1;
+    const int64_t slot = (b & 7) * per + (b >> 3);
+    return slot < M ? slot : -1;
+}
+
+// Kept-voxel test shared by every pass: the keep row (inside the volume by construction, checked
+// anyway) or the reference mask.
+__device__ inline bool voxel_kept(const Volumes& V, const uint8_t* keep, int v, int64_t o, int64_t xx, int64_t yy,
+                                  int64_t zz) {
+    if (keep) return kept_bit(keep, v) && xx >= 0 && xx < V.nx && yy >= 0 && yy < V.ny && zz >= 0 && zz < V.nz;
+    return !V.mask || V.mask[o] != 0;
+}
+
+// Reference-side moments over the voxels selected by `pick(v)` (kept voxels, or the NaN-flagged
+// ones of an iteration), reduced to `out` (NR doubles in LDS).
+template <bool STORED, typename Pick>
+__device__ inline void ref_moments(const IcgnArgs& a, const uint8_t* keep, int64_t x0, int64_t y0, int64_t z0,
+                                   float pf, Pick pick, double* out, double* red) {
+    const Volumes& V = a.V;
+    const Subset& s = a.sub;
+    const int64_t obase = lin(V, z0, y0, x0);
+    const int rowp = V.nx, planep = V.nx * V.ny;
+    float m[NR];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) m[q] = 0.0f;
+    VoxelWalk w;
+    w.start(s, threadIdx.x);
+    for (; w.v < s.S; w.next(s)) {
+        const int dx = -s.hx + s.stride * w.ix, dy = -s.hy + s.stride * w.iy, dz = -s.hz + s.stride * w.iz;
+        const int64_t o = obase + (int64_t)dz * planep + (int64_t)dy * rowp + dx;
+        if (!pick(w.v, o, x0 + dx, y0 + dy, z0 + dz)) continue;
+        const float fv = V.f[o] - pf;
+        float gr[3];
+        grad_at<STORED>(V, o, (int)z0 + dz, (int)y0 + dy, (int)x0 + dx, gr[0], gr[1], gr[2]);
+        const float q[4] = {(float)dx, (float)dy, (float)dz, 1.0f};
+        m[0] += 1.0f;
+        m[1] += fv;
+        m[2] += fv * fv;
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) {
+            const float tf = gr[cc] * fv;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                m[3 + 4 * cc + j] += gr[cc] * q[j];
+                m[15 + 4 * cc + j] += tf * q[j];
+            }
+        }
+    }
+    block_sum<NR, NWAVES>(m, out, red);
+}
+
+template <bool STORED>
+__global__ __launch_bounds__(BLOCK) void k_ref_moments(IcgnArgs a, const int64_t* __restrict__ list,
+                                                       const int64_t* __restrict__ count, double* __restrict__ mom) {
+    const int64_t slot = xcd_slot(blockIdx.x, *count);
+    if (slot < 0) return;
+    const int64_t n = list[slot];
+    const int64_t x0 = a.coords[3 * n], y0 = a.coords[3 * n + 1], z0 = a.coords[3 * n + 2];
+    const uint8_t* keep = keep_row_of(a, n);
+    __shared__ double red[NWAVES * NR];
+    __shared__ double res[NR];
+    if (!keep && !subset_in_volume(a.V, a.sub, x0, y0, z0)) return;  // refused by the iteration kernel
+    double pfd = a.meanf ? a.meanf[n] : 0.0;
+    const float pf = isfinite(pfd) ? (float)pfd : 0.0f;
+    const Volumes& V = a.V;
+    ref_moments<STORED>(
+        a, keep, x0, y0, z0, pf,
+        [&](int v, int64_t o, int64_t xx, int64_t yy, int64_t zz) { return voxel_kept(V, keep, v, o, xx, yy, zz); },
+        res, red);
+    for (int q = threadIdx.x; q < NR; q += BLOCK) mom[n * MOM_STRIDE + q] = res[q];
+}
+
+template <int MODE, bool STORED, bool BUF>
 __global__ __launch_bounds__(BLOCK) void k_icgn12_fused(IcgnArgs a, const int64_t* __restrict__ list,
-                                                        const int64_t* __restrict__ count) {
+                                                        const int64_t* __restrict__ count,
+                                                        const double* __restrict__ mom) {
     const Volumes& V = a.V;
     const Subset& s = a.sub;
     const int tid = threadIdx.x;
-    // XCD-aware mapping: workgroup b runs on XCD b % 8; give each XCD a contiguous run of the
-    // (spatially ordered) node list so its concurrent nodes share subset voxels in its L2.
-    const int64_t M = *count;
-    const int64_t b = blockIdx.x;
-    const int64_t per = (M + 7) / 8;
-    if ((b >> 3) >= per) return;  // the grid is sized for all N nodes; M <= N are valid
-    const int64_t slot = (b & 7) * per + (b >> 3);
-    if (slot >= M) return;
+    const int64_t slot = xcd_slot(blockIdx.x, *count);
+    if (slot < 0) return;
     const int64_t n = list[slot];
     const int64_t x0 = a.coords[3 * n], y0 = a.coords[3 * n + 1], z0 = a.coords[3 * n + 2];
     const uint8_t* keep = keep_row_of(a, n);
 
+    extern __shared__ unsigned nanbits[];  // one bit per sampled voxel
+    const int nwords = (s.S + 31) >> 5;
     __shared__ float sw[14];  // warp (12), pilots pf, pg
     __shared__ int ctrl;
-    __shared__ double red[NWAVES * NM];
-    __shared__ double res[NM];
+    __shared__ double red[NWAVES * NR];
+    __shared__ double res[NG];
+    __shared__ double exc[NR];
     __shared__ Ctl12 c;
 
     double pg_next = 0.0;
@@ -434,11 +515,13 @@ __global__ __launch_bounds__(BLOCK) void k_icgn12_fused(IcgnArgs a, const int64_
         pg_next = pf;
     }
     __syncthreads();
+    const double* mn = mom + n * MOM_STRIDE;
     const double half_scale = (double)max(s.hx, max(s.hy, s.hz));
     const Domain dom = make_domain(V, (int)x0, (int)y0, (int)z0);
     const float* gbase = V.g + lin(V, z0, y0, x0);
     const int64_t obase = lin(V, z0, y0, x0);
     const int rowp = V.nx, planep = V.nx * V.ny;
+    const BufVol gb = make_bufvol(V.g, (int64_t)V.nz * planep, rowp, planep);
     int it_done = 0;
     int status = ST_MAX_ITER;
     for (int it = 1; it <= a.max_iter; ++it) {
@@ -448,66 +531,66 @@ __global__ __launch_bounds__(BLOCK) void k_icgn12_fused(IcgnArgs a, const int64_
             put_warp(sw, c.P);
             sw[13] = (float)pg_next;
         }
+        for (int i = tid; i < nwords; i += BLOCK) nanbits[i] = 0u;
         __syncthreads();
         const float pf = sw[12], pg = sw[13];
-        float m[NM];
+        float m[NG];
 #pragma unroll
-        for (int q = 0; q < NM; ++q) m[q] = 0.0f;
+        for (int q = 0; q < NG; ++q) m[q] = 0.0f;
         VoxelWalk w;
         w.start(s, tid);
         for (; w.v < s.S; w.next(s)) {
             const int dx = -s.hx + s.stride * w.ix, dy = -s.hy + s.stride * w.iy, dz = -s.hz + s.stride * w.iz;
             const int64_t o = obase + (int64_t)dz * planep + (int64_t)dy * rowp + dx;
-            bool k;
-            if (keep) {
-                k = kept_bit(keep, w.v);
-                const int64_t xx = x0 + dx, yy = y0 + dy, zz = z0 + dz;
-                k = k && xx >= 0 && xx < V.nx && yy >= 0 && yy < V.ny && zz >= 0 && zz < V.nz;
-            } else {
-                k = !V.mask || V.mask[o] != 0;
-            }
-            if (!k) continue;
+            if (!voxel_kept(V, keep, w.v, o, x0 + dx, y0 + dy, z0 + dz)) continue;
             // reference-side values first: they do not depend on the warp, so their loads
             // overlap the gather of the deformed sample
             const float fraw = V.f[o];
             float gr[3];
             grad_at<STORED>(V, o, (int)z0 + dz, (int)y0 + dy, (int)x0 + dx, gr[0], gr[1], gr[2]);
-            m[1] += 1.0f;
             const float X = (float)dx, Y = (float)dy, Z = (float)dz;
             const float xr = sw[0] * X + sw[1] * Y + sw[2] * Z + sw[9];
             const float yr = sw[3] * X + sw[4] * Y + sw[5] * Z + sw[10];
             const float zr = sw[6] * X + sw[7] * Y + sw[8] * Z + sw[11];
             if (!inside(dom, xr, yr, zr)) {
-                m[2] = 1.0f;
+                m[1] = 1.0f;
                 continue;
             }
-            const float val = sample_rel<MODE>(gbase, rowp, planep, xr, yr, zr, dom.kx, dom.ky, dom.kz);
-            if (val != val) continue;
+            float val;
+            if constexpr (BUF)
+                val = sample_buf<MODE>(gb, (int)obase, rowp, planep, xr, yr, zr, dom.kx, dom.ky, dom.kz);
+            else
+                val = sample_rel<MODE>(gbase, rowp, planep, xr, yr, zr, dom.kx, dom.ky, dom.kz);
+            if (val != val) {  // masked in the deformed frame: drop the voxel from this iteration
+                atomicOr(&nanbits[w.v >> 5], 1u << (w.v & 31));
+                m[2] += 1.0f;
+                continue;
+            }
             const float fv = fraw - pf, gv = val - pg;
-            m[0] += 1.0f;
-            m[3] += fv;
-            m[4] += fv * fv;
-            m[5] += gv;
-            m[6] += gv * gv;
-            m[7] += fv * gv;
             const float q[4] = {X, Y, Z, 1.0f};
+            m[0] += 1.0f;
+            m[3] += gv;
+            m[4] += gv * gv;
+            m[5] += fv * gv;
 #pragma unroll
             for (int cc = 0; cc < 3; ++cc) {
-                const float t1 = gr[cc], tf = gr[cc] * fv, tg = gr[cc] * gv;
+                const float tg = gr[cc] * gv;
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    m[8 + 4 * cc + j] += t1 * q[j];
-                    m[20 + 4 * cc + j] += tf * q[j];
-                    m[32 + 4 * cc + j] += tg * q[j];
-                }
+                for (int j = 0; j < 4; ++j) m[6 + 4 * cc + j] += tg * q[j];
             }
         }
-        block_sum<NM, NWAVES>(m, res, red);
-        const double* r = res;
+        block_sum<NG, NWAVES>(m, res, red);
+        const bool correct = res[2] > 0.0 && !(res[1] > 0.0);  // block-uniform
+        if (correct) {
+            ref_moments<STORED>(
+                a, keep, x0, y0, z0, pf,
+                [&](int v, int64_t, int64_t, int64_t, int64_t) { return ((nanbits[v >> 5] >> (v & 31)) & 1u) != 0; },
+                exc, red);
+        }
         if (tid == 0) {
-            const double n_valid = r[0], n_ref = r[1];
+            const double n_valid = res[0], n_ref = mn[0];
             int st = -1;
-            if (r[2] > 0.0) {
+            if (res[1] > 0.0) {
                 st = ST_OUT_OF_BOUNDS;
             } else if (n_valid < MIN_SUBSET_VOXELS || n_valid < MIN_VALID_FRACTION * n_ref) {
                 st = ST_INVALID_SUBSET;
@@ -516,20 +599,20 @@ __global__ __launch_bounds__(BLOCK) void k_icgn12_fused(IcgnArgs a, const int64_
                 c.zncc = NAN;
                 ctrl = st;
             } else {
+                // reference-side moments of this iteration's voxel set
+                auto R = [&](int q) { return correct ? mn[q] - exc[q] : mn[q]; };
                 // centred statistics on the dynamic voxel set (numba_kernels.py:586-598)
-                const double mf = r[3] / n_valid, mg = r[5] / n_valid;
-                double ssf = r[4] - n_valid * mf * mf, ssg = r[6] - n_valid * mg * mg;
+                const double mf = R(1) / n_valid, mg = res[3] / n_valid;
+                double ssf = R(2) - n_valid * mf * mf, ssg = res[4] - n_valid * mg * mg;
                 ssf = ssf < 0.0 ? 0.0 : ssf;
                 ssg = ssg < 0.0 ? 0.0 : ssg;
                 const double bf = sqrt(ssf > 1e-30 ? ssf : 1e-30), bg = sqrt(ssg > 1e-30 ? ssg : 1e-30);
-                const double scc = r[7] - n_valid * mf * mg;
-                const double zncc = scc / (bf * bg);
-                // b_k = bottomf * sum SD_k ((f - mf)/bf - (g - mg)/bg)   (numba_kernels.py:705-729)
+                const double zncc = (res[5] - n_valid * mf * mg) / (bf * bg);
                 const double ratio = bf / bg;
 #pragma unroll 1
                 for (int q = 0; q < 12; ++q) {
                     const int cc = q >> 2, j = q & 3;
-                    const double C = r[8 + q], A = r[20 + q], B = r[32 + q];
+                    const double C = R(3 + q), A = R(15 + q), B = res[6 + q];
                     c.b[j < 3 ? 3 * cc + j : 9 + cc] = (A - mf * C) - ratio * (B - mg * C);
                 }
                 const double* Hn = a.H_all ? a.H_all + 144 * n : nullptr;
@@ -1047,6 +1130,60 @@ __global__ __launch_bounds__(1024) void k_compact(const uint8_t* __restrict__ va
     if (threadIdx.x == 0) *count = base;
 }
 
+// Valid nodes grouped by spatial tile (a counting sort on the tile of each node centre, tiles in
+// (z, y, x) order). Consecutive list entries are then neighbours whose subsets overlap, which is
+// what the XCD mapping of the fused kernels relies on for L2 reuse. The order inside a tile is
+// whatever the atomics give: every node is solved independently, so results do not depend on it.
+constexpr int TILE_HIST_MAX = 12288;
+
+__global__ __launch_bounds__(1024) void k_compact_tiled(const uint8_t* __restrict__ valid, const int64_t* __restrict__ coords,
+                                                        int64_t N, int tx, int ty, int tz, int ntx, int nty, int ntz,
+                                                        int64_t* __restrict__ list, int64_t* __restrict__ count) {
+    __shared__ unsigned hist[TILE_HIST_MAX];
+    __shared__ unsigned wtot[16];
+    const int T = ntx * nty * ntz;
+    for (int t = threadIdx.x; t < T; t += 1024) hist[t] = 0u;
+    __syncthreads();
+    auto tile_of = [&](int64_t n) {
+        int cx = (int)(coords[3 * n] / tx), cy = (int)(coords[3 * n + 1] / ty), cz = (int)(coords[3 * n + 2] / tz);
+        cx = cx < 0 ? 0 : (cx >= ntx ? ntx - 1 : cx);
+        cy = cy < 0 ? 0 : (cy >= nty ? nty - 1 : cy);
+        cz = cz < 0 ? 0 : (cz >= ntz ? ntz - 1 : cz);
+        return (cz * nty + cy) * ntx + cx;
+    };
+    for (int64_t i = threadIdx.x; i < N; i += 1024)
+        if (valid[i]) atomicAdd(&hist[tile_of(i)], 1u);
+    __syncthreads();
+    // exclusive scan of the histogram: each thread scans a contiguous run of tiles
+    const int per = (T + 1023) / 1024;
+    const int t0 = threadIdx.x * per, t1 = min(T, t0 + per);
+    unsigned run = 0;
+    for (int t = t0; t < t1; ++t) run += hist[t];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    unsigned incl = run;
+    for (int off = 1; off < 64; off <<= 1) {
+        const unsigned u = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += u;
+    }
+    if (lane == 63) wtot[wave] = incl;
+    __syncthreads();
+    unsigned before = incl - run;
+    for (int k = 0; k < wave; ++k) before += wtot[k];
+    for (int t = t0; t < t1; ++t) {
+        const unsigned h = hist[t];
+        hist[t] = before;
+        before += h;
+    }
+    __syncthreads();
+    for (int64_t i = threadIdx.x; i < N; i += 1024)
+        if (valid[i]) list[atomicAdd(&hist[tile_of(i)], 1u)] = i;
+    if (threadIdx.x == 0) {
+        unsigned tot = 0;
+        for (int k = 0; k < 16; ++k) tot += wtot[k];
+        *count = tot;
+    }
+}
+
 // Status/iteration/ZNCC defaults of the reference batch wrappers (numba_kernels.py:832-840).
 __global__ void k_init_outputs(const uint8_t* __restrict__ valid, int64_t N, int32_t* n_iter, int8_t* status,
                                double* zncc) {
@@ -1070,7 +1207,16 @@ void launch_icgn12(int variant, const IcgnArgs& a, const int64_t* list, const in
         }
     } else {
         const int64_t nb = ((N + 7) / 8) * 8;
-        hipLaunchKernelGGL((k_icgn12_fused<MODE, STORED>), dim3((unsigned)nb), dim3(BLOCK), 0, st, a, list, count);
+        double* mom = (double*)gbuf;  // variant 1 keeps the reference moments where variant 0 keeps scratch rows
+        hipLaunchKernelGGL((k_ref_moments<STORED>), dim3((unsigned)nb), dim3(BLOCK), 0, st, a, list, count, mom);
+        const size_t lds = sizeof(unsigned) * (size_t)((a.sub.S + 31) / 32);
+        // buffer addressing (32-bit byte offsets) for volumes under 4 GB, pointers beyond
+        if ((int64_t)a.V.nx * a.V.ny * a.V.nz < ((int64_t)1 << 30))
+            hipLaunchKernelGGL((k_icgn12_fused<MODE, STORED, true>), dim3((unsigned)nb), dim3(BLOCK), lds, st, a, list,
+                               count, (const double*)mom);
+        else
+            hipLaunchKernelGGL((k_icgn12_fused<MODE, STORED, false>), dim3((unsigned)nb), dim3(BLOCK), lds, st, a, list,
+                               count, (const double*)mom);
     }
 }
 
@@ -1078,6 +1224,25 @@ template <int MODE, bool STORED>
 void launch_icgn3(const Icgn3Args& a, const int64_t* list, const int64_t* count, int64_t N, hipStream_t st) {
     const int64_t nb = ((N + 7) / 8) * 8;
     hipLaunchKernelGGL((k_icgn3_fused<MODE, STORED>), dim3((unsigned)nb), dim3(BLOCK), 0, st, a, list, count);
+}
+
+// Valid-node list: in index order (the reference's order, used by the plain kernel), or grouped
+// by tiles one subset wide (the fused kernels). Tiles grow until the histogram fits in LDS.
+void launch_compact(bool tiled, const uint8_t* valid, const int64_t* coords, int64_t N, int hx, int hy, int hz,
+                    const Volumes& V, int64_t* list, int64_t* count, hipStream_t stream) {
+    if (!tiled || N > UINT32_MAX) {
+        hipLaunchKernelGGL(k_compact, dim3(1), dim3(1024), 0, stream, valid, N, list, count);
+        return;
+    }
+    int tx = std::max(8, 2 * hx), ty = std::max(8, 2 * hy), tz = std::max(8, 2 * hz);
+    auto tiles = [&](int t, int n) { return (n + t - 1) / t; };
+    while ((int64_t)tiles(tx, V.nx) * tiles(ty, V.ny) * tiles(tz, V.nz) > TILE_HIST_MAX) {
+        tx *= 2;
+        ty *= 2;
+        tz *= 2;
+    }
+    hipLaunchKernelGGL(k_compact_tiled, dim3(1), dim3(1024), 0, stream, valid, coords, N, tx, ty, tz,
+                       tiles(tx, V.nx), tiles(ty, V.ny), tiles(tz, V.nz), list, count);
 }
 
 int check_common(const Volumes& V, const Subset& s, int mode) {
@@ -1157,6 +1322,7 @@ size_t aldvc_icgn12_workspace_bytes(int64_t N, int hx, int hy, int hz, int strid
     size_t bytes = 256 + sizeof(int64_t) * (size_t)(N > 0 ? N : 1);  // count + node list
     bytes = (bytes + 255) / 256 * 256;
     if (variant == 0) bytes += sizeof(float) * (size_t)std::min<int64_t>(N, PLAIN_CHUNK) * (size_t)s.S;
+    else bytes += sizeof(double) * MOM_STRIDE * (size_t)(N > 0 ? N : 1);  // reference moments per node
     return bytes;
 }
 
@@ -1182,6 +1348,8 @@ int aldvc_icgn12(const int64_t* coords, int64_t N, double* P, int hx, int hy, in
         pattern = nullptr;
         noise_gain = 0.0;
     }
+    if (variant == 1 && sizeof(unsigned) * (size_t)((s.S + 31) / 32) > 48 * 1024)
+        return fail(-2, "subset too large for the fused kernel's LDS voxel mask (use variant 0)");
     const size_t need = aldvc_icgn12_workspace_bytes(N, hx, hy, hz, stride, variant);
     if (!workspace || workspace_bytes < need) return fail(-3, "workspace too small (see aldvc_icgn12_workspace_bytes)");
     int64_t* count = (int64_t*)workspace;
@@ -1190,7 +1358,7 @@ int aldvc_icgn12(const int64_t* coords, int64_t N, double* P, int hx, int hy, in
     float* gbuf = (float*)((char*)workspace + list_bytes);
     hipLaunchKernelGGL(k_init_outputs, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, stream, valid, N, n_iter,
                        status, zncc);
-    hipLaunchKernelGGL(k_compact, dim3(1), dim3(1024), 0, stream, valid, N, list, count);
+    launch_compact(variant == 1, valid, coords, N, hx, hy, hz, V, list, count, stream);
     IcgnArgs a{V, s, coords, P, L_all, H_all, pattern, meanf, split_index, split_keep, keep_bytes,
                noise_gain, tol, dp_tol, max_iter, patience, predictive, n_iter, status, zncc};
     const bool stored = gx != nullptr;
@@ -1234,7 +1402,7 @@ int aldvc_icgn3(const int64_t* coords, int64_t N, const double* U_old, double* U
         return fail(-4, "copy of U_old failed");
     hipLaunchKernelGGL(k_init_outputs, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, stream, valid, N, n_iter,
                        status, zncc);
-    hipLaunchKernelGGL(k_compact, dim3(1), dim3(1024), 0, stream, valid, N, list, count);
+    launch_compact(true, valid, coords, N, hx, hy, hz, V, list, count, stream);
     Icgn3Args a{V, s, coords, U_old, U, F_fixed, vdual, H_all, bottomf, meanf, split_index, split_keep, keep_bytes,
                 mu, tol, dp_tol, noise_gain > 0.0 ? noise_gain : 0.0, max_iter, patience, predictive, n_iter, status,
                 zncc};

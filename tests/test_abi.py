@@ -46,12 +46,13 @@ def test_abi_version_and_sizes(lib):
     from pyaldvc_amd import _lib
 
     assert lib.aldvc_abi_version() == _lib.ABI_VERSION
-    # plain variant: node list + 2048 scratch rows of S floats
+    # both variants: count + node list; plain adds 2048 scratch rows of S floats, fused adds
+    # 32 float64 reference moments per node
     S = 33 ** 3
     b0 = lib.aldvc_icgn12_workspace_bytes(10000, 16, 16, 16, 1, 0)
     b1 = lib.aldvc_icgn12_workspace_bytes(10000, 16, 16, 16, 1, 1)
-    assert b0 - b1 == 4 * 2048 * S
-    assert b1 >= 8 * 10000
+    assert b0 - b1 == 4 * 2048 * S - 256 * 10000
+    assert b1 >= 8 * 10000 + 256 * 10000
     assert lib.aldvc_icgn3_workspace_bytes(10) >= 80
 
 
