@@ -128,6 +128,25 @@ def timed_steps(fn, steps, warmup):
     return [a.elapsed_time(b) / 1e3 for a, b in evs], wall, out
 
 
+def kernel_times(fn, steps):
+    """Mean duration (s) of the solver's main iteration kernel, from the HIP events the library
+    records around that launch on its stream."""
+    from pyaldvc_amd import _lib
+
+    L = _lib.lib()
+    _lib.check(L.aldvc_kernel_timing(1), "aldvc_kernel_timing")
+    ms = []
+    try:
+        for _ in range(steps):
+            fn()
+            ms.append(L.aldvc_last_kernel_ms())
+    finally:
+        L.aldvc_kernel_timing(0)
+    if min(ms) <= 0:
+        raise RuntimeError("kernel timing events were not recorded")
+    return float(np.mean(ms)) / 1e3
+
+
 def cpu_baseline(W, pre, P0, target_s=15.0):
     """The float64 CPU restatement (OpenMP, every host core) on a bounded sample of valid nodes."""
     import torch
@@ -232,15 +251,24 @@ def main():
 
     times, wall, out = timed_steps(lambda: step("fused"), args.steps, args.warmup)
     t_step = float(np.mean(times))
+    t_kernel = kernel_times(lambda: step("fused"), args.steps)
     P, n_iter, status, zncc = (x.cpu().numpy() for x in out)
     conv = status == 0
     err = np.linalg.norm(P[conv, 9:] - W["U_true"][conv], axis=1)
     S = (2 * HALF + 1) ** 3
     iters = int(n_iter[valid.cpu().numpy()].sum())
     bytes_alg = 20.0 * S * iters  # f + 3 gradients + 1 deformed sample per sampled voxel-iteration
-    achieved = bytes_alg / t_step / 1e9
+    achieved = bytes_alg / t_kernel / 1e9
     log(f"fused: {t_step * 1e3:.1f} ms/step, {n_valid / t_step:,.0f} nodes/s, {int(conv.sum())} converged, "
         f"mean iterations {iters / max(n_valid, 1):.2f}, median |err| {np.median(err):.4f} vox")  # fmt: skip
+
+    gather = None
+    if args.plain_steps > 0:
+        tg, _, outg = timed_steps(lambda: step("fused_gather"), args.plain_steps, 1)
+        t_gather = float(np.mean(tg))
+        gather = {"ms_per_step": t_gather * 1e3, "nodes_per_s": n_valid / t_gather,
+                  "status_equal_fused": float(np.mean(outg[2].cpu().numpy() == status))}  # fmt: skip
+        log(f"fused, gathers only: {t_gather * 1e3:.1f} ms/step, {n_valid / t_gather:,.0f} nodes/s")
 
     plain = None
     if args.plain_steps > 0:
@@ -290,6 +318,8 @@ def main():
             "mean_iterations": iters / max(n_valid, 1),
             "median_err_vox": float(np.median(err)) if err.size else None,
             "p95_err_vox": float(np.percentile(err, 95)) if err.size else None,
+            "kernel_ms": t_kernel * 1e3,
+            "fused_gather_kernel": gather,
             "plain_kernel": plain,
             "speedup_vs_plain": (plain["ms_per_step"] / (t_step * 1e3)) if plain else None,
             "speedup_vs_cpu": ((n_valid / t_step) / cpu["value"]) if cpu else None,

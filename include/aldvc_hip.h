@@ -37,6 +37,14 @@ int aldvc_abi_version(void);
 /* Message of the last failed call on this thread ("" after a successful one). */
 const char *aldvc_last_error(void);
 
+/* Measurement hook, no counterpart in the reference: with on != 0, the following aldvc_icgn12
+ * (variants 1 and 2) and aldvc_icgn3 calls on this thread record HIP events around their main
+ * iteration kernel on the call's stream. */
+int aldvc_kernel_timing(int on);
+
+/* Duration in ms of the last timed kernel on this thread (waits for it); -1 if there is none. */
+float aldvc_last_kernel_ms(void);
+
 /* 7-point central-difference gradients, zero within `border` voxels of any face.
  * Replaces io/volume_ops.py:124-197 (_gradient_stencil7 / compute_gradients). */
 int aldvc_gradients(const float *f, int64_t nz, int64_t ny, int64_t nx, int border, float *gx, float *gy,
@@ -68,7 +76,9 @@ size_t aldvc_icgn12_workspace_bytes(int64_t N, int hx, int hy, int hz, int strid
  * solver/cuda_kernels.py:1397-1515 (icgn_12dof_cuda). H and pattern (noise-corrected Hessian,
  * used when noise_gain > 0) may be NULL; meanf (N) is the accumulation pilot of variant 1 and
  * may be NULL. variant 0 = plain (one 256-lane workgroup per node, two passes through scratch
- * rows, the reference GPU kernel's design), 1 = fused one-pass kernel. */
+ * rows, the reference GPU kernel's design), 1 = fused one-pass kernel (deformed-volume stencils
+ * staged in LDS bricks where the warped subset fits one; memory gathers elsewhere), 2 = the fused
+ * kernel with memory gathers only. */
 int aldvc_icgn12(const int64_t *coords, int64_t N, double *P, int hx, int hy, int hz, int stride,
                  const float *f, const float *gx, const float *gy, const float *gz, const uint8_t *mask,
                  const float *g, int64_t nz, int64_t ny, int64_t nx, int mode, const double *L,

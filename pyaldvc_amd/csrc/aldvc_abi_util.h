@@ -34,4 +34,26 @@ inline int launch_status() {
     return 0;
 }
 
+// Optional timing of the main kernel of each solver call on this thread: when enabled, events
+// are recorded around that launch on the call's stream, and kernel_ms() waits for the second one.
+struct KernelTimer {
+    bool on = false, have = false;
+    hipEvent_t a = nullptr, b = nullptr;
+};
+
+inline KernelTimer& timer() {
+    static thread_local KernelTimer t;
+    return t;
+}
+
+inline void timer_begin(hipStream_t st) {
+    KernelTimer& t = timer();
+    if (t.on) (void)hipEventRecord(t.a, st);
+}
+
+inline void timer_end(hipStream_t st) {
+    KernelTimer& t = timer();
+    if (t.on) t.have = hipEventRecord(t.b, st) == hipSuccess;
+}
+
 }  // namespace aldvc

@@ -216,26 +216,98 @@ __device__ inline float sample_buf(const BufVol& b, int anchor, int rowp, int pl
             bspline4(fz, wz0, wz1, wz2, wz3);
         }
         const uint32_t voff = 4u * (uint32_t)(anchor + (iz - 1) * planep + (iy - 1) * rowp + (ix - 1));
-        float4 r[16];
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-#pragma unroll
-            for (int j = 0; j < 4; ++j) r[4 * k + j] = buf_load4(b, voff, k * b.planeb + j * b.rowb);
+        // the 16 rows in two batches of 8, each batch issued before any row is consumed: one
+        // latency per batch, and 32 rather than 64 registers held, which leaves room for a third
+        // workgroup per CU
         const float wz[4] = {wz0, wz1, wz2, wz3};
         const float wy[4] = {wy0, wy1, wy2, wy3};
         float val = 0.0f;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            float plane = 0.0f;
+        for (int h = 0; h < 2; ++h) {
+            float4 r[8];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const float4 q = r[4 * k + j];
-                plane += wy[j] * (q.x * wx0 + q.y * wx1 + q.z * wx2 + q.w * wx3);
+            for (int k = 0; k < 2; ++k)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) r[4 * k + j] = buf_load4(b, voff, (2 * h + k) * b.planeb + j * b.rowb);
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                float plane = 0.0f;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const float4 q = r[4 * k + j];
+                    plane += wy[j] * (q.x * wx0 + q.y * wx1 + q.z * wx2 + q.w * wx3);
+                }
+                val += wz[2 * h + k] * plane;
             }
-            val += wz[k] * plane;
         }
         return val;
     }
+}
+
+// LDS brick sampler. A brick is a block of the deformed volume staged in LDS with row stride BRK
+// and plane stride BRK * BRK floats, so every tap offset is a compile-time immediate from one base
+// address. (rx, ry, rz) are anchor-relative positions as in sample_rel and (bx, by, bz) the
+// anchor-relative brick origin. The caller guarantees that the position is inside the sampling
+// domain (so the n - 3 clamp of the stencil base never applies) and that the brick holds its stencil.
+template <int MODE, int BRK>
+__device__ inline float sample_brick(const float* brick, int bx, int by, int bz, float rx, float ry, float rz) {
+    constexpr int ROW = BRK, PLANE = BRK * BRK;
+    const int ix = (int)floorf(rx), iy = (int)floorf(ry), iz = (int)floorf(rz);
+    const float fx = rx - (float)ix, fy = ry - (float)iy, fz = rz - (float)iz;
+    if constexpr (MODE == INTERP_LINEAR) {
+        const float* p = brick + ((iz - bz) * PLANE + (iy - by) * ROW + (ix - bx));
+        const float c00 = p[0] * (1.0f - fx) + p[1] * fx;
+        const float c10 = p[ROW] * (1.0f - fx) + p[ROW + 1] * fx;
+        const float c01 = p[PLANE] * (1.0f - fx) + p[PLANE + 1] * fx;
+        const float c11 = p[PLANE + ROW] * (1.0f - fx) + p[PLANE + ROW + 1] * fx;
+        const float c0 = c00 * (1.0f - fy) + c10 * fy;
+        const float c1 = c01 * (1.0f - fy) + c11 * fy;
+        return c0 * (1.0f - fz) + c1 * fz;
+    } else {
+        float wx0, wx1, wx2, wx3, wy0, wy1, wy2, wy3, wz0, wz1, wz2, wz3;
+        if constexpr (MODE == INTERP_CUBIC) {
+            keys4(fx, wx0, wx1, wx2, wx3);
+            keys4(fy, wy0, wy1, wy2, wy3);
+            keys4(fz, wz0, wz1, wz2, wz3);
+        } else {
+            bspline4(fx, wx0, wx1, wx2, wx3);
+            bspline4(fy, wy0, wy1, wy2, wy3);
+            bspline4(fz, wz0, wz1, wz2, wz3);
+        }
+        const float* p = brick + ((iz - 1 - bz) * PLANE + (iy - 1 - by) * ROW + (ix - 1 - bx));
+        // the 64 taps in two batches of 32, each read before any of it is consumed (as sample_buf)
+        const float wz[4] = {wz0, wz1, wz2, wz3};
+        const float wy[4] = {wy0, wy1, wy2, wy3};
+        float val = 0.0f;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            float t[32];
+#pragma unroll
+            for (int k = 0; k < 2; ++k)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) t[16 * k + 4 * j + i] = p[(2 * h + k) * PLANE + j * ROW + i];
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                float plane = 0.0f;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const float* r = t + 16 * k + 4 * j;
+                    plane += wy[j] * (r[0] * wx0 + r[1] * wx1 + r[2] * wx2 + r[3] * wx3);
+                }
+                val += wz[2 * h + k] * plane;
+            }
+        }
+        return val;
+    }
+}
+
+// Anchor-relative warped position with a pinned evaluation order (explicit fused multiply-adds):
+// evaluated the same way everywhere, it is monotone in each of X, Y, Z, so over a box of subset
+// voxels its extremes are taken at the box corners -- the bound the LDS bricks rely on.
+__device__ inline float warp_coord(float a0, float a1, float a2, float t, float X, float Y, float Z) {
+    return __fmaf_rn(a0, X, __fmaf_rn(a1, Y, __fmaf_rn(a2, Z, t)));
 }
 
 // Admissible sampling domain (interp_kernels.py:127-136) in anchor-relative coordinates:

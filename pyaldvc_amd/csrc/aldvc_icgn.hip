@@ -397,6 +397,8 @@ __global__ __launch_bounds__(BLOCK) void k_icgn12_plain(IcgnArgs a, const int64_
 constexpr int NR = 27;  // 0 n, 1 sum f', 2 sum f'^2, 3 + 4c + j: sum grad_c q_j, 15 + 4c + j: sum grad_c q_j f'
 constexpr int NG = 18;  // 0 n_valid, 1 oob, 2 n_nan, 3 sum g', 4 sum g'^2, 5 sum f'g', 6 + 4c + j: sum grad_c q_j g'
 constexpr int MOM_STRIDE = 32;  // doubles per node in the moments workspace
+constexpr int BRICK_XY = 40;    // LDS brick rows and columns: subsets up to 33 voxels wide
+constexpr int LDS_PER_WG = 54528;  // LDS budget of one fused workgroup: three per CU (160 KB)
 
 // XCD-aware slot of workgroup b over the M valid nodes: workgroup b runs on XCD b % 8, and each
 // XCD gets a contiguous run of the node list so its concurrent nodes share subset voxels in its
@@ -475,10 +477,75 @@ __global__ __launch_bounds__(BLOCK) void k_ref_moments(IcgnArgs a, const int64_t
     for (int q = threadIdx.x; q < NR; q += BLOCK) mom[n * MOM_STRIDE + q] = res[q];
 }
 
-template <int MODE, bool STORED, bool BUF>
-__global__ __launch_bounds__(BLOCK) void k_icgn12_fused(IcgnArgs a, const int64_t* __restrict__ list,
+// Chunk plan of the LDS-brick path. Subset planes [zc, zc + kz) warp into the box spanned by the
+// warped positions of the chunk's 8 corners (warp_coord is monotone in X, Y and Z). When every
+// corner is inside the sampling domain, so is every sample of the chunk, and the brick holds the
+// stencils of all of them. Returns 0 when the chunk fits (origin and extents set), 1 when only its
+// z extent is too large (retry with fewer planes), 2 when it cannot use a brick.
+template <int MODE, int BRK>
+__device__ inline int plan_chunk(const float* sw, const Subset& s, const Domain& dom, int zc, int kz, int planes,
+                                 int& bx, int& by, int& bz, int& ez) {
+    const float Xl = (float)(-s.hx), Xh = (float)(-s.hx + s.stride * (s.sx - 1));
+    const float Yl = (float)(-s.hy), Yh = (float)(-s.hy + s.stride * (s.sy - 1));
+    const float Zl = (float)(-s.hz + s.stride * zc), Zh = (float)(-s.hz + s.stride * (zc + kz - 1));
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const float X = (c & 1) ? Xh : Xl, Y = (c & 2) ? Yh : Yl, Z = (c & 4) ? Zh : Zl;
+        const float r[3] = {warp_coord(sw[0], sw[1], sw[2], sw[9], X, Y, Z),
+                            warp_coord(sw[3], sw[4], sw[5], sw[10], X, Y, Z),
+                            warp_coord(sw[6], sw[7], sw[8], sw[11], X, Y, Z)};
+        if (!inside(dom, r[0], r[1], r[2])) return 2;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = fminf(lo[d], r[d]);
+            hi[d] = fmaxf(hi[d], r[d]);
+        }
+    }
+    constexpr int LO = MODE == INTERP_LINEAR ? 0 : -1, HI = MODE == INTERP_LINEAR ? 1 : 2;
+    bx = (int)floorf(lo[0]) + LO;
+    by = (int)floorf(lo[1]) + LO;
+    bz = (int)floorf(lo[2]) + LO;
+    const int ex = (int)floorf(hi[0]) + HI - bx + 1, ey = (int)floorf(hi[1]) + HI - by + 1;
+    ez = (int)floorf(hi[2]) + HI - bz + 1;
+    if (ex > BRK || ey > BRK) return 2;
+    return ez > planes ? 1 : 0;
+}
+
+// Stage planes [0, ez) of a BRK x BRK brick whose origin is the anchor-relative voxel (bx, by, bz):
+// 16-byte buffer loads of the deformed volume (out-of-range offsets read zero; brick voxels outside
+// the volume are never read by an in-domain sample), 16-byte LDS stores.
+template <int BRK>
+__device__ inline void stage_brick(float* brick, const BufVol& gb, int anchor, int rowp, int planep, int bx, int by,
+                                   int bz, int ez) {
+    static_assert(BRK % 4 == 0, "brick rows are whole 16-byte groups");
+    constexpr int Q = BRK / 4, PLANE_Q = Q * BRK;
+    const int total = PLANE_Q * ez;
+    const int org = anchor + bz * planep + by * rowp + bx;
+    constexpr int U = 8;
+    for (int base = threadIdx.x; base < total; base += U * BLOCK) {
+        float4 r[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int i = base + u * BLOCK;
+            if (i < total) {
+                const int zz = i / PLANE_Q, rem = i - zz * PLANE_Q, yy = rem / Q, qx = rem - yy * Q;
+                r[u] = buf_load4(gb, 4u * (uint32_t)(org + zz * planep + yy * rowp + 4 * qx), 0);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int i = base + u * BLOCK;
+            if (i < total) reinterpret_cast<float4*>(brick)[i] = r[u];
+        }
+    }
+}
+
+template <int MODE, bool STORED, bool BUF, int BRK>
+__global__ __launch_bounds__(BLOCK, 3) void k_icgn12_fused(IcgnArgs a, const int64_t* __restrict__ list,
                                                         const int64_t* __restrict__ count,
-                                                        const double* __restrict__ mom) {
+                                                        const double* __restrict__ mom, int brick_planes) {
+    static_assert(BRK == 0 || BUF, "the brick path stages through the buffer descriptor");
     const Volumes& V = a.V;
     const Subset& s = a.sub;
     const int tid = threadIdx.x;
@@ -488,8 +555,10 @@ __global__ __launch_bounds__(BLOCK) void k_icgn12_fused(IcgnArgs a, const int64_
     const int64_t x0 = a.coords[3 * n], y0 = a.coords[3 * n + 1], z0 = a.coords[3 * n + 2];
     const uint8_t* keep = keep_row_of(a, n);
 
-    extern __shared__ unsigned nanbits[];  // one bit per sampled voxel
+    // dynamic LDS: one NaN bit per sampled voxel, then (brick path) the brick, 16-byte aligned
+    extern __shared__ __align__(16) unsigned nanbits[];
     const int nwords = (s.S + 31) >> 5;
+    float* brick = reinterpret_cast<float*>(nanbits + ((nwords + 3) & ~3));
     __shared__ float sw[14];  // warp (12), pilots pf, pg
     __shared__ int ctrl;
     __shared__ double red[NWAVES * NR];
@@ -537,38 +606,42 @@ __global__ __launch_bounds__(BLOCK) void k_icgn12_fused(IcgnArgs a, const int64_
         float m[NG];
 #pragma unroll
         for (int q = 0; q < NG; ++q) m[q] = 0.0f;
-        VoxelWalk w;
-        w.start(s, tid);
-        for (; w.v < s.S; w.next(s)) {
+        // one subset voxel: reference-side values first (they do not depend on the warp, so their
+        // loads overlap the gather of the deformed sample), then the warp, the sample and the
+        // deformed-side moments. `FROM_BRICK`: every sample of the chunk is known to be inside.
+        auto voxel = [&](const VoxelWalk& w, auto from_brick, int bx, int by, int bz) {
+            constexpr bool FROM_BRICK = decltype(from_brick)::value;
             const int dx = -s.hx + s.stride * w.ix, dy = -s.hy + s.stride * w.iy, dz = -s.hz + s.stride * w.iz;
             const int64_t o = obase + (int64_t)dz * planep + (int64_t)dy * rowp + dx;
-            if (!voxel_kept(V, keep, w.v, o, x0 + dx, y0 + dy, z0 + dz)) continue;
-            // reference-side values first: they do not depend on the warp, so their loads
-            // overlap the gather of the deformed sample
+            if (!voxel_kept(V, keep, w.v, o, x0 + dx, y0 + dy, z0 + dz)) return;
             const float fraw = V.f[o];
             float gr[3];
             grad_at<STORED>(V, o, (int)z0 + dz, (int)y0 + dy, (int)x0 + dx, gr[0], gr[1], gr[2]);
             const float X = (float)dx, Y = (float)dy, Z = (float)dz;
-            const float xr = sw[0] * X + sw[1] * Y + sw[2] * Z + sw[9];
-            const float yr = sw[3] * X + sw[4] * Y + sw[5] * Z + sw[10];
-            const float zr = sw[6] * X + sw[7] * Y + sw[8] * Z + sw[11];
-            if (!inside(dom, xr, yr, zr)) {
-                m[1] = 1.0f;
-                continue;
-            }
+            const float xr = warp_coord(sw[0], sw[1], sw[2], sw[9], X, Y, Z);
+            const float yr = warp_coord(sw[3], sw[4], sw[5], sw[10], X, Y, Z);
+            const float zr = warp_coord(sw[6], sw[7], sw[8], sw[11], X, Y, Z);
             float val;
-            if constexpr (BUF)
-                val = sample_buf<MODE>(gb, (int)obase, rowp, planep, xr, yr, zr, dom.kx, dom.ky, dom.kz);
-            else
-                val = sample_rel<MODE>(gbase, rowp, planep, xr, yr, zr, dom.kx, dom.ky, dom.kz);
-            if (val != val) {  // masked in the deformed frame: drop the voxel from this iteration
-                atomicOr(&nanbits[w.v >> 5], 1u << (w.v & 31));
-                m[2] += 1.0f;
-                continue;
+            if constexpr (FROM_BRICK) {
+                val = sample_brick<MODE, BRK>(brick, bx, by, bz, xr, yr, zr);
+            } else {
+                if (!inside(dom, xr, yr, zr)) {
+                    m[1] = 1.0f;
+                    return;
+                }
+                if constexpr (BUF)
+                    val = sample_buf<MODE>(gb, (int)obase, rowp, planep, xr, yr, zr, dom.kx, dom.ky, dom.kz);
+                else
+                    val = sample_rel<MODE>(gbase, rowp, planep, xr, yr, zr, dom.kx, dom.ky, dom.kz);
             }
-            const float fv = fraw - pf, gv = val - pg;
+            // a NaN sample (masked in the deformed frame) drops the voxel from this iteration; the
+            // accumulation is branch-free (gv = 0 adds nothing), so the reference-side loads above
+            // stay ahead of the sample instead of being sunk into a branch
+            const bool ok = val == val;
+            const float fv = fraw - pf, gv = ok ? val - pg : 0.0f;
             const float q[4] = {X, Y, Z, 1.0f};
-            m[0] += 1.0f;
+            m[0] += ok ? 1.0f : 0.0f;
+            m[2] += ok ? 0.0f : 1.0f;
             m[3] += gv;
             m[4] += gv * gv;
             m[5] += fv * gv;
@@ -578,6 +651,35 @@ __global__ __launch_bounds__(BLOCK) void k_icgn12_fused(IcgnArgs a, const int64_
 #pragma unroll
                 for (int j = 0; j < 4; ++j) m[6 + 4 * cc + j] += tg * q[j];
             }
+            if (!ok) atomicOr(&nanbits[w.v >> 5], 1u << (w.v & 31));
+        };
+        using Gather = std::integral_constant<bool, false>;
+        if constexpr (BRK > 0) {
+            // subset planes in chunks whose warped stencils fit the brick; a chunk that does not
+            // fit (large rotation or shear, or a sample outside the domain) gathers from memory
+            constexpr int OVER = MODE == INTERP_LINEAR ? 1 : 3;
+            const int plane_vox = s.sx * s.sy;
+            for (int zc = 0; zc < s.sz;) {
+                int kz = min(brick_planes - OVER, s.sz - zc), bx = 0, by = 0, bz = 0, ez = 0;
+                int fit = plan_chunk<MODE, BRK>(sw, s, dom, zc, kz, brick_planes, bx, by, bz, ez);
+                while (fit == 1 && kz > 1) fit = plan_chunk<MODE, BRK>(sw, s, dom, zc, --kz, brick_planes, bx, by, bz, ez);
+                const int vend = (zc + kz) * plane_vox;
+                VoxelWalk w;
+                w.start(s, zc * plane_vox + tid);
+                if (fit == 0) {
+                    __syncthreads();  // the previous chunk's samples are done with the brick
+                    stage_brick<BRK>(brick, gb, (int)obase, rowp, planep, bx, by, bz, ez);
+                    __syncthreads();
+                    for (; w.v < vend; w.next(s)) voxel(w, std::integral_constant<bool, true>{}, bx, by, bz);
+                } else {
+                    for (; w.v < vend; w.next(s)) voxel(w, Gather{}, 0, 0, 0);
+                }
+                zc += kz;
+            }
+        } else {
+            VoxelWalk w;
+            w.start(s, tid);
+            for (; w.v < s.S; w.next(s)) voxel(w, Gather{}, 0, 0, 0);
         }
         block_sum<NG, NWAVES>(m, res, red);
         const bool correct = res[2] > 0.0 && !(res[1] > 0.0);  // block-uniform
@@ -1209,21 +1311,44 @@ void launch_icgn12(int variant, const IcgnArgs& a, const int64_t* list, const in
         const int64_t nb = ((N + 7) / 8) * 8;
         double* mom = (double*)gbuf;  // variant 1 keeps the reference moments where variant 0 keeps scratch rows
         hipLaunchKernelGGL((k_ref_moments<STORED>), dim3((unsigned)nb), dim3(BLOCK), 0, st, a, list, count, mom);
-        const size_t lds = sizeof(unsigned) * (size_t)((a.sub.S + 31) / 32);
-        // buffer addressing (32-bit byte offsets) for volumes under 4 GB, pointers beyond
-        if ((int64_t)a.V.nx * a.V.ny * a.V.nz < ((int64_t)1 << 30))
-            hipLaunchKernelGGL((k_icgn12_fused<MODE, STORED, true>), dim3((unsigned)nb), dim3(BLOCK), lds, st, a, list,
-                               count, (const double*)mom);
-        else
-            hipLaunchKernelGGL((k_icgn12_fused<MODE, STORED, false>), dim3((unsigned)nb), dim3(BLOCK), lds, st, a, list,
-                               count, (const double*)mom);
+        const size_t bits = sizeof(unsigned) * (size_t)(((a.sub.S + 31) / 32 + 3) & ~3);
+        const bool buf = (int64_t)a.V.nx * a.V.ny * a.V.nz < ((int64_t)1 << 30);  // 32-bit byte offsets
+        // LDS brick: BRICK_XY^2 floats per plane, as many planes as leave three workgroups per CU
+        int planes = 0;
+        if (variant == 1 && buf && 2 * a.sub.hx + 8 <= BRICK_XY && 2 * a.sub.hy + 8 <= BRICK_XY) {
+            hipFuncAttributes fa{};
+            if (hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&k_icgn12_fused<MODE, STORED, true, BRICK_XY>)) ==
+                hipSuccess) {
+                const int64_t room = (int64_t)LDS_PER_WG - (int64_t)fa.sharedSizeBytes - (int64_t)bits;
+                planes = (int)std::min<int64_t>(a.sub.sz + 3, room / (int64_t)(4 * BRICK_XY * BRICK_XY));
+            }
+            if (planes < 6) planes = 0;  // too few planes per chunk: the staging would dominate
+        }
+        const size_t lds = bits + (size_t)planes * 4 * BRICK_XY * BRICK_XY;
+        auto kern = &k_icgn12_fused<MODE, STORED, true, BRICK_XY>;
+        if (planes > 0 && hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+            planes = 0;
+        timer_begin(st);
+        if (planes > 0) {
+            hipLaunchKernelGGL(kern, dim3((unsigned)nb), dim3(BLOCK), lds, st, a, list, count, (const double*)mom, planes);
+        } else if (buf) {
+            hipLaunchKernelGGL((k_icgn12_fused<MODE, STORED, true, 0>), dim3((unsigned)nb), dim3(BLOCK), bits, st, a,
+                               list, count, (const double*)mom, 0);
+        } else {
+            hipLaunchKernelGGL((k_icgn12_fused<MODE, STORED, false, 0>), dim3((unsigned)nb), dim3(BLOCK), bits, st, a,
+                               list, count, (const double*)mom, 0);
+        }
+        timer_end(st);
     }
 }
 
 template <int MODE, bool STORED>
 void launch_icgn3(const Icgn3Args& a, const int64_t* list, const int64_t* count, int64_t N, hipStream_t st) {
     const int64_t nb = ((N + 7) / 8) * 8;
+    timer_begin(st);
     hipLaunchKernelGGL((k_icgn3_fused<MODE, STORED>), dim3((unsigned)nb), dim3(BLOCK), 0, st, a, list, count);
+    timer_end(st);
 }
 
 // Valid-node list: in index order (the reference's order, used by the plain kernel), or grouped
@@ -1265,6 +1390,26 @@ extern "C" {
 int aldvc_abi_version(void) { return ALDVC_ABI_VERSION; }
 
 const char* aldvc_last_error(void) { return last_error_cstr(); }
+
+int aldvc_kernel_timing(int on) {
+    clear_error();
+    KernelTimer& t = timer();
+    if (on && !t.a) {
+        if (hipEventCreate(&t.a) != hipSuccess || hipEventCreate(&t.b) != hipSuccess)
+            return fail(-5, "hipEventCreate failed");
+    }
+    t.on = on != 0;
+    t.have = false;
+    return 0;
+}
+
+float aldvc_last_kernel_ms(void) {
+    KernelTimer& t = timer();
+    float ms = -1.0f;
+    if (!t.have || hipEventSynchronize(t.b) != hipSuccess || hipEventElapsedTime(&ms, t.a, t.b) != hipSuccess)
+        return -1.0f;
+    return ms;
+}
 
 int aldvc_gradients(const float* f, int64_t nz, int64_t ny, int64_t nx, int border, float* gx, float* gy, float* gz,
                     hipStream_t stream) {
@@ -1337,7 +1482,8 @@ int aldvc_icgn12(const int64_t* coords, int64_t N, double* P, int hx, int hy, in
     Volumes V{f, gx, gy, gz, g, mask, (int)nz, (int)ny, (int)nx};
     const Subset s = make_subset(hx, hy, hz, stride);
     if (int e = check_common(V, s, mode)) return e;
-    if (variant != 0 && variant != 1) return fail(-2, "variant must be 0 (plain) or 1 (fused)");
+    if (variant < 0 || variant > 2)
+        return fail(-2, "variant must be 0 (plain), 1 (fused) or 2 (fused, gathers only)");
     if (N < 0) return fail(-2, "negative node count");
     if (N == 0) return 0;
     if (!coords || !P || !L_all || !valid || !n_iter || !status || !zncc) return fail(-2, "null pointer");
@@ -1348,7 +1494,7 @@ int aldvc_icgn12(const int64_t* coords, int64_t N, double* P, int hx, int hy, in
         pattern = nullptr;
         noise_gain = 0.0;
     }
-    if (variant == 1 && sizeof(unsigned) * (size_t)((s.S + 31) / 32) > 48 * 1024)
+    if (variant != 0 && sizeof(unsigned) * (size_t)((s.S + 31) / 32) > 48 * 1024)
         return fail(-2, "subset too large for the fused kernel's LDS voxel mask (use variant 0)");
     const size_t need = aldvc_icgn12_workspace_bytes(N, hx, hy, hz, stride, variant);
     if (!workspace || workspace_bytes < need) return fail(-3, "workspace too small (see aldvc_icgn12_workspace_bytes)");
@@ -1358,7 +1504,7 @@ int aldvc_icgn12(const int64_t* coords, int64_t N, double* P, int hx, int hy, in
     float* gbuf = (float*)((char*)workspace + list_bytes);
     hipLaunchKernelGGL(k_init_outputs, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, stream, valid, N, n_iter,
                        status, zncc);
-    launch_compact(variant == 1, valid, coords, N, hx, hy, hz, V, list, count, stream);
+    launch_compact(variant != 0, valid, coords, N, hx, hy, hz, V, list, count, stream);
     IcgnArgs a{V, s, coords, P, L_all, H_all, pattern, meanf, split_index, split_keep, keep_bytes,
                noise_gain, tol, dp_tol, max_iter, patience, predictive, n_iter, status, zncc};
     const bool stored = gx != nullptr;

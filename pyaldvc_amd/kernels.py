@@ -42,7 +42,7 @@ STATUS_NAN = 5
 STATUS_SKIPPED = 6
 STATUS_STALLED = 7
 
-VARIANTS = {"plain": 0, "fused": 1}
+VARIANTS = {"plain": 0, "fused": 1, "fused_gather": 2}
 DEVICE_CACHE_SIZE = 8  # uploaded host volumes kept per process (cuda_kernels.py:70)
 
 
@@ -251,7 +251,9 @@ def icgn_12dof_parallel(
     """12-DOF IC-GN over all nodes: ``(P_out (N,12), n_iter (N,) int32, status (N,) int8, zncc (N,))``.
 
     ``variant="plain"`` runs the reference GPU kernel's design (two passes through a scratch row,
-    2048-node launches); ``"fused"`` (default) runs the one-pass kernel. ``meanf_all`` feeds the
+    2048-node launches); ``"fused"`` (default) runs the one-pass kernel, which stages the deformed
+    volume in LDS bricks where the warped subset fits one; ``"fused_gather"`` is the one-pass
+    kernel with memory gathers only. ``meanf_all`` feeds the
     fused kernel's accumulation pilot only; ``bottomf_all`` is accepted for signature parity (the
     12-DOF solver renormalises on the dynamic voxel set, numba_kernels.py:680-708).
     """

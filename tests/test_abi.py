@@ -85,3 +85,4 @@ def test_argument_errors_are_reported_without_a_gpu(lib):
     assert lib.aldvc_icgn3(*args) == -2 and b"half-widths" in lib.aldvc_last_error()
     with pytest.raises(_lib.HipLibraryError):
         _lib.check(-2, "probe")
+    assert lib.aldvc_last_kernel_ms() == -1.0  # nothing timed on this thread

@@ -94,7 +94,7 @@ def _start(golden, far=False):
     return P0
 
 
-@pytest.mark.parametrize("variant", ["plain", "fused"])
+@pytest.mark.parametrize("variant", ["plain", "fused", "fused_gather"])
 @pytest.mark.parametrize("key", list(CASES12))
 def test_icgn12_matches_oracle(golden, key, variant):
     c = CASES12[key]
@@ -114,7 +114,7 @@ def test_icgn12_matches_oracle(golden, key, variant):
     assert_icgn_parity(ref, got)
 
 
-@pytest.mark.parametrize("variant", ["plain", "fused"])
+@pytest.mark.parametrize("variant", ["plain", "fused", "fused_gather"])
 def test_icgn12_split_rows_match_oracle(golden, variant):
     coords = golden["coords"]
     si = np.arange(coords.shape[0], dtype=np.int64)
@@ -130,7 +130,7 @@ def test_icgn12_split_rows_match_oracle(golden, variant):
     assert_icgn_parity(ref, K.icgn_12dof_parallel(*args, variant=variant))
 
 
-@pytest.mark.parametrize("variant", ["plain", "fused"])
+@pytest.mark.parametrize("variant", ["plain", "fused", "fused_gather"])
 @pytest.mark.parametrize("mode", [1, 0])
 def test_icgn12_lattice_matches_oracle_and_truth(case, variant, mode):
     """A full node lattice (subset 16, step 8) on a 64x72x80 bead pair under an affine field."""
@@ -149,6 +149,46 @@ def test_icgn12_lattice_matches_oracle_and_truth(case, variant, mode):
     assert conv.sum() > 0.5 * valid.sum()
     err = np.abs(got[0][conv, 9:] - U[conv])
     assert np.median(err) < (0.05 if mode == 1 else 0.1)
+
+
+def _rot_z(deg, stretch_z=0.0):
+    c, s_ = np.cos(np.radians(deg)), np.sin(np.radians(deg))
+    return np.array([[c - 1.0, -s_, 0.0], [s_, c - 1.0, 0.0], [0.0, 0.0, stretch_z]])
+
+
+# Warps that exercise the fused kernel's LDS bricks: 32^3 subsets that fit a brick, a rotation too
+# large for the brick's rows (every chunk gathers from memory), a z stretch that shrinks the chunks,
+# and nodes whose warped subsets leave the sampling domain (status OUT_OF_BOUNDS).
+BRICK_CASES = {
+    "fits": dict(F=_rot_z(3.0), t=(0.6, -0.4, 0.7)),
+    "rotation": dict(F=_rot_z(12.0), t=(0.3, 0.5, -0.6)),
+    "stretch": dict(F=_rot_z(2.0, 0.3), t=(-0.4, 0.2, 0.5)),
+    "escape": dict(F=_rot_z(1.0), t=(7.4, -1.6, 6.2)),
+}
+
+
+@pytest.mark.parametrize("variant", ["fused", "fused_gather"])
+@pytest.mark.parametrize("key", list(BRICK_CASES))
+def test_icgn12_brick_paths_match_oracle(key, variant):
+    c = BRICK_CASES[key]
+    cs = bead_case(shape=(112, 112, 112), F=c["F"], t=c["t"], seed=9)
+    ax = 24 + 16 * np.arange(5)
+    coords = np.stack(np.meshgrid(ax, ax, ax, indexing="ij"), -1).reshape(-1, 3).astype(np.int64)
+    half = (16, 16, 16)
+    f, g, gx, gy, gz = cs["f"], cs["g"], cs["gx"], cs["gy"], cs["gz"]
+    H, L, mf, bf, nv, valid = cpu_ref.precompute_nodes(coords, *half, f, gx, gy, gz, MASK_NONE, 0.5, 1e12)
+    U = truth_at(cs, coords)
+    P0 = np.zeros((coords.shape[0], 12))
+    P0[:, :9] = c["F"].reshape(-1)
+    P0[:, 9:] = np.round(U)
+    args = (coords, P0, *half, f, gx, gy, gz, MASK_NONE, g, 1, L, mf, bf, valid, 1e-2, 1e-3, 100, 5)
+    ref = cpu_ref.icgn_12dof_parallel(*args)
+    got = K.icgn_12dof_parallel(*args, variant=variant)
+    assert_icgn_parity(ref, got)
+    if key == "escape":
+        assert np.any(ref[2] == 2)
+    else:
+        assert np.sum(ref[2] == 0) > 0.5 * valid.sum()
 
 
 @pytest.mark.parametrize("key,mu,noise", [("i3_plain", 1e-3, False), ("i3_mu", 0.5, False), ("i3_noise", 1e-3, True)])
