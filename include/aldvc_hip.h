@@ -129,6 +129,26 @@ int aldvc_zncc_map(const float *f, const float *g, int64_t nz, int64_t ny, int64
 int aldvc_ncc_peaks(const float *maps, int64_t n, int rx, int ry, int rz, const double *pinv, int64_t *peak_idx,
                     double *peak_val, double *pce, double *sub, hipStream_t stream);
 
+/* Bytes of the summed-volume table aldvc_subset_valid_fraction needs as workspace. */
+size_t aldvc_mask_table_bytes(int64_t nz, int64_t ny, int64_t nx);
+
+/* In-mask fraction of each node's window [c - h, c + h] (clipped to the volume, over the nominal
+ * (2h+1)^3 volume) into frac (N) float64. Replaces mesh/grid_mesh.py:135-188
+ * (subset_valid_fraction); winsize w gives h = w / 2 as there. */
+int aldvc_subset_valid_fraction(const uint8_t *mask, int64_t nz, int64_t ny, int64_t nx,
+                                const int64_t *coords, int64_t N, int hx, int hy, int hz, double *frac,
+                                void *workspace, size_t workspace_bytes, hipStream_t stream);
+
+/* Packed keep rows (M, keep_bytes) of the 6-connected in-mask component around the centre of each
+ * candidate subset cand[m] (bit i of a row = sampled voxel i in the kernels' order), with the kept
+ * and in-mask sampled-voxel counts. Voxels within `margin` of a volume face count as masked.
+ * Replaces solver/numba_kernels.py:223-300 (build_split_rows) and utils/flood_fill.py:12-72.
+ * Windows up to 64 voxels along x and 4096 (y, z) rows. */
+int aldvc_split_rows(const int64_t *coords, const int64_t *cand, int64_t M, int hx, int hy, int hz,
+                     int stride, const uint8_t *mask, int64_t nz, int64_t ny, int64_t nx, int margin,
+                     int64_t keep_bytes, uint8_t *rows, int64_t *n_keep, int64_t *n_inmask,
+                     hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

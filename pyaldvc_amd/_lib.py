@@ -70,6 +70,9 @@ SIGNATURES = {
     ),
     "aldvc_zncc_map_workspace_bytes": (SZ, [I64, I32, I32, I32, I32, I32, I32, I32]),
     "aldvc_ncc_peaks": (I32, [P, I64, I32, I32, I32, P, P, P, P, P, P]),
+    "aldvc_mask_table_bytes": (SZ, [I64, I64, I64]),
+    "aldvc_subset_valid_fraction": (I32, [P, I64, I64, I64, P, I64, I32, I32, I32, P, P, SZ, P]),
+    "aldvc_split_rows": (I32, [P, P, I64, I32, I32, I32, I32, P, I64, I64, I64, I32, I64, P, P, P, P]),
 }
 
 

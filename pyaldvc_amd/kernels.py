@@ -240,6 +240,57 @@ def precompute_nodes(coords, hx, hy, hz, f, gx, gy, gz, mask, min_valid_ratio, c
     return A.out(H), A.out(L), A.out(meanf), A.out(bottomf), A.out(nvalid), A.out(valid.bool())
 
 
+# --------------------------------------------------------------------------- subset splitting
+
+
+def subset_valid_fraction(mask, coordinates, winsize):
+    """In-mask fraction ``(N,)`` of each node's window ``[c - w//2, c + w//2]`` per axis, clipped
+    to the volume, over the nominal window volume (mesh/grid_mesh.py:135-188). ``coordinates``
+    are rounded to voxels; ``winsize`` is ``(wx, wy, wz)``."""
+    A = _Args(mask)
+    md = A.vol(mask, np.uint8)
+    nz, ny, nx = _dims(md)
+    if isinstance(coordinates, torch.Tensor):
+        c = torch.round(coordinates.to(device(), torch.float64)).to(torch.int64).contiguous()
+    else:
+        c = A.arr(np.round(np.asarray(coordinates, dtype=np.float64)).astype(np.int64), np.int64)
+    N = int(c.shape[0])
+    hx, hy, hz = (int(w) // 2 for w in winsize)
+    frac = torch.zeros(N, dtype=torch.float64, device=md.device)
+    ws = torch.empty(int(lib().aldvc_mask_table_bytes(nz, ny, nx)), dtype=torch.uint8, device=md.device)
+    check(
+        lib().aldvc_subset_valid_fraction(_ptr(md), nz, ny, nx, _ptr(c), N, hx, hy, hz, _ptr(frac), _ptr(ws),
+                                          int(ws.numel()), stream_handle()),
+        "aldvc_subset_valid_fraction",
+    )  # fmt: skip
+    return A.out(frac)
+
+
+def build_split_rows(coords, cand, hx, hy, hz, stride, mask, margin=0):
+    """Packed keep rows ``(rows (M, ceil(S/8)) uint8, n_keep (M,), n_inmask (M,))`` of the
+    candidate nodes ``cand`` (numba_kernels.py:271-289): bit i of a row is set when sampled voxel
+    i (kernel order) belongs to the 6-connected in-mask component holding the subset centre."""
+    A = _Args(mask)
+    md = A.vol(mask, np.uint8)
+    nz, ny, nx = _dims(md)
+    c = A.arr(coords, np.int64)
+    cd = A.arr(cand, np.int64)
+    M = int(cd.shape[0])
+    if M and (int(cd.min()) < 0 or int(cd.max()) >= int(c.shape[0])):
+        raise ValueError("cand holds an index outside coords")
+    kb = (subset_count(hx, hy, hz, stride) + 7) // 8
+    rows = torch.zeros((M, kb), dtype=torch.uint8, device=md.device)
+    n_keep = torch.zeros(M, dtype=torch.int64, device=md.device)
+    n_inmask = torch.zeros(M, dtype=torch.int64, device=md.device)
+    if M:
+        check(
+            lib().aldvc_split_rows(_ptr(c), _ptr(cd), M, int(hx), int(hy), int(hz), int(stride), _ptr(md), nz, ny, nx,
+                                   int(margin), kb, _ptr(rows), _ptr(n_keep), _ptr(n_inmask), stream_handle()),
+            "aldvc_split_rows",
+        )  # fmt: skip
+    return A.out(rows), A.out(n_keep), A.out(n_inmask)
+
+
 # --------------------------------------------------------------------------- 12-DOF IC-GN
 
 

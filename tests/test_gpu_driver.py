@@ -1,0 +1,164 @@
+"""Subset splitting on the GPU and the local-step driver, against the reference (needs an MI355X).
+
+Split rows and window fractions: the reference's own vectors (``golden_local_step.npz``,
+``golden_driver.npz``) and the oracle's flood fill on larger windows. The driver:
+``precompute_local_context`` + ``local_icgn`` of al_dvc 1.2.1 run on a 40x44x48 bead pair with a
+damaged block in the deformed frame, without a mask and with a cylinder mask, subset splitting
+and a cut mesh (``golden_driver.npz``); parity bar as the kernels': identical statuses and bad
+flags, >= 95 % identical iteration counts, converged translations within 1e-3 voxel.
+"""
+
+from __future__ import annotations
+
+import os
+from types import SimpleNamespace
+
+import numpy as np
+import pytest
+
+from oracle import cpu_ref
+
+pytestmark = pytest.mark.gpu
+
+K = pytest.importorskip("pyaldvc_amd.kernels")
+from pyaldvc_amd import local as LOC  # noqa: E402
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@pytest.fixture(scope="module")
+def gd():
+    return np.load(os.path.join(HERE, "golden", "golden_driver.npz"))
+
+
+def _cylinder_mask(shape, seed):
+    """A cylinder with ellipsoidal voids and a crack: components of every shape around the centres."""
+    rng = np.random.default_rng(seed)
+    z, y, x = np.meshgrid(*(np.arange(s) for s in shape), indexing="ij")
+    cx, cy = (shape[2] - 1) / 2, (shape[1] - 1) / 2
+    m = np.hypot(x - cx, y - cy) < 0.45 * min(shape[1], shape[2])
+    for _ in range(12):
+        c = rng.uniform(0, 1, 3) * np.array(shape)
+        r = rng.uniform(2, 7, 3)
+        m &= ((z - c[0]) / r[0]) ** 2 + ((y - c[1]) / r[1]) ** 2 + ((x - c[2]) / r[2]) ** 2 > 1
+    m &= ~((np.abs(x - cx - 3) < 1) & (z > shape[0] // 3))  # a crack
+    return m.astype(np.uint8)
+
+
+def test_subset_valid_fraction(gd):
+    got = K.subset_valid_fraction(gd["drvm_mask"], gd["drvm_coords"], (12, 12, 12))
+    assert np.array_equal(got, gd["drvm_svf"])
+    # windows hanging over every face, against a direct count
+    rng = np.random.default_rng(2)
+    mask = (rng.random((23, 29, 31)) < 0.6).astype(np.uint8)
+    coords = np.stack([rng.integers(-5, 36, 200), rng.integers(-5, 34, 200), rng.integers(-5, 28, 200)], 1)
+    h = (6, 4, 5)
+    got = K.subset_valid_fraction(mask, coords, (2 * h[0], 2 * h[1], 2 * h[2]))
+    ref = np.empty(len(coords))
+    for n, (x, y, z) in enumerate(coords):
+        win = mask[max(0, z - h[2]) : max(0, z + h[2] + 1), max(0, y - h[1]) : max(0, y + h[1] + 1), max(0, x - h[0]) : max(0, x + h[0] + 1)]
+        ref[n] = win.sum() / ((2 * h[0] + 1) * (2 * h[1] + 1) * (2 * h[2] + 1))
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("key,half,stride,margin", [("split_m0", (6, 6, 6), 1, 0), ("split_m3", (6, 6, 6), 1, 3), ("split_s2", (6, 5, 4), 2, 0)])
+def test_split_rows_match_reference(golden, key, half, stride, margin):
+    cand = np.arange(golden["coords"].shape[0], dtype=np.int64)
+    rows, nk, nm = K.build_split_rows(golden["coords"], cand, *half, stride, golden["split_mask"], margin)
+    assert np.array_equal(rows, golden[f"{key}_rows"])
+    assert np.array_equal(nk, golden[f"{key}_nkeep"])
+    assert np.array_equal(nm, golden[f"{key}_ninmask"])
+
+
+@pytest.mark.parametrize("half,stride,margin", [((16, 16, 16), 1, 0), ((16, 16, 16), 2, 3), ((24, 24, 12), 1, 0), ((31, 9, 20), 3, 0)])
+def test_split_rows_match_oracle_large_windows(half, stride, margin):
+    shape = (70, 80, 90)
+    mask = _cylinder_mask(shape, seed=sum(half) + stride)
+    rng = np.random.default_rng(7)
+    coords = np.stack([rng.integers(0, 90, 150), rng.integers(0, 80, 150), rng.integers(0, 70, 150)], 1).astype(np.int64)
+    cand = np.flatnonzero(rng.random(150) < 0.8).astype(np.int64)
+    got = K.build_split_rows(coords, cand, *half, stride, mask, margin)
+    ref = cpu_ref.build_split_rows(coords, cand, *half, stride, mask, margin)
+    for a, b in zip(got, ref):
+        assert np.array_equal(a, b)
+    assert (ref[1] > 0).any() and (ref[1] < ref[2]).any()  # some subsets really were cut
+
+
+def _para(**kw):
+    base = dict(winsize=(12, 12, 12), subset_stride=1, min_valid_ratio=0.5, hessian_cond_max=1e12, subset_split=False,
+                interp_method="cubic", icgn_tol=1e-2, icgn_dp_tol=1e-3, icgn_max_iter=100, icgn_patience=5,
+                icgn_predictive_stop=True, icgn_noise_hessian=False, local_outlier_threshold=2.0)  # fmt: skip
+    base.update(kw)
+    return SimpleNamespace(**base)
+
+
+def _check_driver(gd, p, ctx, U, F, info, bad):
+    assert np.array_equal(ctx.valid, gd[f"{p}_valid"])
+    assert np.array_equal(ctx.n_valid, gd[f"{p}_n_valid"])
+    v = gd[f"{p}_valid"]
+    assert np.max(np.abs(ctx.meanf[v] - gd[f"{p}_meanf"][v])) < 1e-5
+    assert np.max(np.abs(ctx.bottomf[v] / gd[f"{p}_bottomf"][v] - 1)) < 1e-5
+    assert np.max(np.abs(ctx.H_all[v] - gd[f"{p}_H"][v])) <= 2e-6 * np.max(np.abs(gd[f"{p}_H"]))
+    st = gd[f"{p}_status"]
+    assert np.array_equal(info.status, st)
+    assert np.mean(info.n_iter == gd[f"{p}_n_iter"]) >= 0.95
+    assert np.array_equal(bad, gd[f"{p}_bad"])
+    assert info.n_bad == int(gd[f"{p}_n_bad"])
+    good = ~bad
+    assert np.max(np.abs(U[good] - gd[f"{p}_U"][good])) < 1e-3
+    assert np.max(np.abs(F[good] - gd[f"{p}_F"][good])) < 1e-4
+    # inpainted nodes are a linear function of the good ones
+    assert np.max(np.abs(U - gd[f"{p}_U"])) < 1e-3
+    fin = np.isfinite(info.zncc) & np.isfinite(gd[f"{p}_zncc"])
+    assert np.max(np.abs(info.zncc[fin] - gd[f"{p}_zncc"][fin])) < 1e-3
+    conv = st == 0  # the error against the imposed field is the reference's, to 1e-3 voxel
+    err, err_ref = (np.median(np.abs(u[conv] - gd[f"{p}_U_true"][conv])) for u in (U, gd[f"{p}_U"]))
+    assert abs(err - err_ref) < 1e-3
+
+
+def test_local_driver_matches_reference(gd):
+    p = "drv"
+    mesh = LOC.NodeMesh(gd[f"{p}_coords"], tuple(gd[f"{p}_grid_shape"]), gd[f"{p}_node_valid_in"].copy())
+    ref = SimpleNamespace(f=gd["drv_f"], mask=None, gradient_mode="stored")
+    para = _para()
+    ctx = LOC.precompute_local_context(mesh, ref, para)
+    mesh.node_valid = ctx.valid.copy()
+    U, F, info, bad = LOC.local_icgn(ctx, ref, gd["drv_g"], gd[f"{p}_U0"], para, mesh)
+    _check_driver(gd, p, ctx, U, F, info, bad)
+
+
+def test_local_driver_with_mask_split_and_cut_mesh_matches_reference(gd):
+    p = "drvm"
+    mesh = LOC.NodeMesh(gd[f"{p}_coords"], tuple(gd[f"{p}_grid_shape"]), gd[f"{p}_node_valid_in"].copy(), gd[f"{p}_edge_ok"])
+    assert mesh.is_cut
+    ref = SimpleNamespace(f=gd["drv_f"], mask=gd[f"{p}_mask"].astype(bool), gradient_mode="stored")
+    para = _para(subset_split=True)
+    ctx = LOC.precompute_local_context(mesh, ref, para)
+    assert np.array_equal(ctx.split_index, gd[f"{p}_split_index"])
+    assert np.array_equal(ctx.split_keep, gd[f"{p}_split_keep"])
+    np.testing.assert_array_equal(ctx.split_fraction, gd[f"{p}_split_fraction"])
+    mesh.node_valid = ctx.valid.copy()
+    U, F, info, bad = LOC.local_icgn(ctx, ref, gd["drv_g"], gd[f"{p}_U0"], para, mesh)
+    _check_driver(gd, p, ctx, U, F, info, bad)
+
+
+def test_local_driver_takes_a_bundle_with_on_the_fly_gradients(gd):
+    """A ReferenceBundle with 1x1x1 gradient placeholders: gradients evaluated inside the kernels."""
+    p = "drv"
+    mesh = LOC.NodeMesh(gd[f"{p}_coords"], tuple(gd[f"{p}_grid_shape"]), gd[f"{p}_node_valid_in"].copy())
+    ph = np.zeros((1, 1, 1), np.float32)
+    ref = SimpleNamespace(f=gd["drv_f"], gx=ph, gy=ph, gz=ph, mask=np.ones((1, 1, 1), np.uint8))
+    para = _para()
+    ctx = LOC.precompute_local_context(mesh, ref, para)
+    H, L, mf, bf, nv, valid = cpu_ref.precompute_nodes(ctx.coords_int, 6, 6, 6, gd["drv_f"], ph, ph, ph,
+                                                       np.ones((1, 1, 1), np.uint8), 0.5, 1e12)  # fmt: skip
+    assert np.array_equal(ctx.valid, valid & gd[f"{p}_node_valid_in"])
+    mesh.node_valid = ctx.valid.copy()
+    U, F, info, bad = LOC.local_icgn(ctx, ref, gd["drv_g"], gd[f"{p}_U0"], para, mesh)
+    P0 = np.zeros((mesh.n_nodes, 12))
+    P0[:, 9:] = gd[f"{p}_U0"]
+    ref12 = cpu_ref.icgn_12dof_parallel(ctx.coords_int, P0, 6, 6, 6, gd["drv_f"], ph, ph, ph, np.ones((1, 1, 1), np.uint8),
+                                        gd["drv_g"], 1, L, mf, bf, ctx.valid, 1e-2, 1e-3, 100, 5)  # fmt: skip
+    assert np.array_equal(info.status, ref12[2])
+    ok = ref12[2] == 0
+    assert np.max(np.abs(U[ok & ~bad] - ref12[0][ok & ~bad, 9:])) < 1e-3
