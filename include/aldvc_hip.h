@@ -149,6 +149,31 @@ int aldvc_split_rows(const int64_t *coords, const int64_t *cand, int64_t M, int 
                      int64_t keep_bytes, uint8_t *rows, int64_t *n_keep, int64_t *n_inmask,
                      hipStream_t stream);
 
+/* Bytes of the workspace aldvc_voi_moments needs for a volume of nz planes of ny rows. */
+size_t aldvc_voi_moments_workspace_bytes(int64_t nz, int64_t ny);
+
+/* sums[0] = sum (v - pilot), sums[1] = sum (v - pilot)^2 over the half-open VOI [z0, z1) x
+ * [y0, y1) x [x0, x1), float64, deterministic. dtype: 0 f32, 1 u16, 2 u8, 3 i16, 4 f64.
+ * Replaces io/volume_ops.py:53-70 (_voi_moments) inside voi_mean_std (:83-102). */
+int aldvc_voi_moments(const void *vol, int dtype, int64_t nz, int64_t ny, int64_t nx, int z0, int z1, int y0,
+                      int y1, int x0, int x1, double pilot, double *sums, void *workspace,
+                      size_t workspace_bytes, hipStream_t stream);
+
+/* out[i] = float32((vol[i] - mean) * inv_std) in float64 arithmetic.
+ * Replaces io/volume_ops.py:73-80 (_normalize_into) inside normalize_volume (:105-121). */
+int aldvc_normalize(const void *vol, int dtype, int64_t n, double mean, double inv_std, float *out,
+                    hipStream_t stream);
+
+/* Cubic B-spline coefficients of vol (whole-sample mirror boundaries) into out; scratch is a third
+ * volume-sized float32 buffer. Replaces io/volume_ops.py:215-222 (prefilter_bspline, i.e.
+ * scipy.ndimage.spline_filter(order=3, mode="mirror") into float32). */
+int aldvc_bspline_prefilter(const float *vol, int64_t nz, int64_t ny, int64_t nx, float *out, float *scratch,
+                            hipStream_t stream);
+
+/* vol[i] = NaN where mask[i] == 0. Replaces the masking step of io/volume_ops.py:225-244
+ * (prepare_deformed). */
+int aldvc_mask_nan(float *vol, const uint8_t *mask, int64_t n, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -73,6 +73,11 @@ SIGNATURES = {
     "aldvc_mask_table_bytes": (SZ, [I64, I64, I64]),
     "aldvc_subset_valid_fraction": (I32, [P, I64, I64, I64, P, I64, I32, I32, I32, P, P, SZ, P]),
     "aldvc_split_rows": (I32, [P, P, I64, I32, I32, I32, I32, P, I64, I64, I64, I32, I64, P, P, P, P]),
+    "aldvc_voi_moments_workspace_bytes": (SZ, [I64, I64]),
+    "aldvc_voi_moments": (I32, [P, I32, I64, I64, I64, I32, I32, I32, I32, I32, I32, F64, P, P, SZ, P]),
+    "aldvc_normalize": (I32, [P, I32, I64, F64, F64, P, P]),
+    "aldvc_bspline_prefilter": (I32, [P, I64, I64, I64, P, P, P]),
+    "aldvc_mask_nan": (I32, [P, P, I64, P]),
 }
 
 
