@@ -174,6 +174,38 @@ int aldvc_bspline_prefilter(const float *vol, int64_t nz, int64_t ny, int64_t nx
  * (prepare_deformed). */
 int aldvc_mask_nan(float *vol, const uint8_t *mask, int64_t n, hipStream_t stream);
 
+/* ---- ADMM global step on the hex8 node grid (float64). A grid of nx*ny*nz nodes, node
+ * n = (iz*ny + iy)*nx + ix; elem_active (nz-1)(ny-1)(nx-1) and node_active (nz)(ny)(nx) uint8;
+ * element matrices 8x8 row-major in the grid's corner order; vectors (N, 3). */
+
+/* y = A x on active nodes, 0 elsewhere, A the operator of the element matrix Ae. Replaces the
+ * assembled products of solver/global_operators.py:58-70 (build_fem_operators). */
+int aldvc_fem_apply(const double *Ae, int nx, int ny, int nz, const uint8_t *elem_active,
+                    const uint8_t *node_active, const double *x, double *y, hipStream_t stream);
+
+/* b = beta sum_j G_j (F - W)[:, :, j] + mu M (U - v) + alpha K U; mats = K, M, G_x, G_y, G_z
+ * (5 x 64), FmW_j (3, N, 3) = (F - W)[:, i, j] per j. Replaces solver/subpb2_solver.py:108-127
+ * (global_rhs). */
+int aldvc_fem_rhs(const double *mats, int nx, int ny, int nz, const uint8_t *elem_active,
+                  const uint8_t *node_active, double beta, double mu, double alpha, const double *FmW_j,
+                  const double *Umv, const double *U, double *b, hipStream_t stream);
+
+/* F (N, 3, 3) = (G_j^T U[:, i]) / lumped mass, NaN at inactive nodes. Replaces
+ * solver/global_operators.py:147-170 (nodal_gradient, FEM). */
+int aldvc_fem_nodal_gradient(const double *mats, int nx, int ny, int nz, const uint8_t *elem_active,
+                             const uint8_t *node_active, const double *U, double *F, hipStream_t stream);
+
+/* Workspace bytes of aldvc_pcg_fem. */
+size_t aldvc_pcg_workspace_bytes(int nx, int ny, int nz);
+
+/* Jacobi-preconditioned CG for A X = B (three columns together) on the active nodes; X holds the
+ * start and receives the solution; B must be 0 at inactive nodes. Stops when every column's
+ * |R| / |B| < tol or after max_iter iterations (count in *iterations, host memory). Replaces
+ * solver/subpb2_solver.py:77-105 (pcg_multi) inside solve_subpb2 (:130-155). */
+int aldvc_pcg_fem(const double *Ae, int nx, int ny, int nz, const uint8_t *elem_active,
+                  const uint8_t *node_active, const double *B, double *X, double tol, int max_iter,
+                  int *iterations, void *workspace, size_t workspace_bytes, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

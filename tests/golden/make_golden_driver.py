@@ -13,6 +13,10 @@ reference's NumPy kernels, ``reference_kernels.py``) and records:
 * ``fill_nan_grid`` / ``fill_nan_nearest`` (``utils/inpaint.py:22-143``), plain and cut;
 * ``noise_hessian_pattern`` (``solver/uncertainty.py:65-80``) at two subset shapes;
 * ``subset_valid_fraction`` (``mesh/grid_mesh.py:135-188``);
+* the global step: ``hex8_box_matrices`` (``mesh/hex8.py:74-105``), ``build_fem_operators``,
+  ``nodal_gradient`` (``solver/global_operators.py:58-170``), ``global_rhs``, ``build_global_system``
+  and ``solve_subpb2`` with the direct and the PCG solver (``solver/subpb2_solver.py:45-155``) on a
+  node grid with dropped elements;
 * the driver: ``precompute_local_context`` + ``local_icgn`` (``solver/local_icgn.py:223-489``)
   on a small bead pair, without a mask and with a cylinder mask, subset splitting and a cut mesh.
 
@@ -41,7 +45,10 @@ from al_dvc.mesh import grid_mesh as rgm  # noqa: E402
 import importlib  # noqa: E402
 
 rli = importlib.import_module("al_dvc.solver.local_icgn")  # the package re-exports a function of that name
+from al_dvc.solver import global_operators as rgo  # noqa: E402
+from al_dvc.solver import subpb2_solver as rsp  # noqa: E402
 from al_dvc.solver import uncertainty as run  # noqa: E402
+from al_dvc.mesh import hex8 as rh8  # noqa: E402
 from al_dvc.solver.tiling import ReferenceSource  # noqa: E402
 from al_dvc.utils import inpaint as rip  # noqa: E402
 from al_dvc.utils import outlier_detection as rod  # noqa: E402
@@ -92,6 +99,48 @@ def grid_cases(out):
 
     out["pattern_886_s1"] = run.noise_hessian_pattern((8, 8, 6), 1)
     out["pattern_12129_s3"] = run.noise_hessian_pattern((12, 12, 9), 3)
+
+
+def global_cases(out):
+    rng = np.random.default_rng(13)
+    for order in (1, 2, 3):
+        box = rh8.hex8_box_matrices((5.0, 6.0, 7.0), order)
+        out[f"hex8_K_o{order}"] = box["K"]
+        out[f"hex8_M_o{order}"] = box["M"]
+        out[f"hex8_G_o{order}"] = box["G"]
+    x0 = 10.0 + 5.0 * np.arange(7)
+    y0 = 12.0 + 6.0 * np.arange(6)
+    z0 = 9.0 + 7.0 * np.arange(5)
+    base = rgm.mesh_setup(x0, y0, z0)
+    mask = np.ones((60, 60, 60), dtype=np.uint8)
+    mask[40:, 40:, :] = 0  # drops a corner column of nodes and their elements
+    mask[:14, :, 30:36] = 0
+    mesh = rgm.apply_mask_to_mesh(base, mask, (10, 10, 10), 0.5)
+    ops = rgo.build_fem_operators(mesh, 2)
+    n = mesh.n_nodes
+    U1 = rng.normal(0, 1.5, (n, 3))
+    F1 = rng.normal(0, 0.02, (n, 3, 3))
+    W = rng.normal(0, 0.01, (n, 3, 3))
+    v = rng.normal(0, 0.1, (n, 3))
+    out["gs_coords"] = mesh.coordinates
+    out["gs_grid_shape"] = np.array(mesh.grid_shape)
+    out["gs_spacing"] = np.array(mesh.spacing)
+    out["gs_elements"] = mesh.elements
+    out["gs_node_valid"] = mesh.node_valid
+    out["gs_active"] = ops.active
+    out["gs_mL"] = ops.mL
+    out.update(gs_U1=U1, gs_F1=F1, gs_W=W, gs_v=v)
+    for beta, mu, alpha, tag in ((1e-3, 1e-3, 0.0, "a"), (0.5, 2e-3, 0.05, "b")):
+        out[f"gs_rhs_{tag}"] = rsp.global_rhs(ops, beta, mu, alpha, U1, F1, W, v)
+        for solver in ("direct", "pcg"):
+            para = DVCPara(global_solver=solver, pcg_tol=1e-8, pcg_max_iter=500)
+            system = rsp.build_global_system(ops, beta, mu, alpha, para)
+            U2, info = rsp.solve_subpb2(system, U1, F1, W, v)
+            out[f"gs_U2_{tag}_{solver}"] = U2
+            out[f"gs_iter_{tag}_{solver}"] = np.array(info["iterations"])
+    out["gs_grad"] = rgo.nodal_gradient(ops, U1)
+    log(f"global step: {n} nodes, {int(ops.active.sum())} active, pcg iterations "
+        f"{int(out['gs_iter_a_pcg'])}/{int(out['gs_iter_b_pcg'])}")  # fmt: skip
 
 
 def volumes(mask_kind):
@@ -164,6 +213,7 @@ def main():
     t0 = time.time()
     out = {}
     grid_cases(out)
+    global_cases(out)
     f, g = driver_case(out, "drv", None, False)
     out["drv_f"] = f
     out["drv_g"] = g

@@ -97,3 +97,29 @@ def test_noise_pattern_is_expected_hessian_of_white_gradient_noise():
             J[9 + c] = 1.0
             E += np.outer(J, J)
     np.testing.assert_allclose(noise.noise_hessian_pattern(half, stride), E)
+
+
+def test_hex8_element_matrices_match_reference(gd):
+    from pyaldvc_amd import fem
+
+    for order in (1, 2, 3):
+        box = fem.hex8_box_matrices((5.0, 6.0, 7.0), order)
+        for key in ("K", "M", "G"):
+            np.testing.assert_allclose(box[key], gd[f"hex8_{key}_o{order}"], rtol=0, atol=1e-14)
+
+
+def test_hex8_element_matrices_closed_forms():
+    """Row sums of K vanish (constants have no energy), M sums to the box volume, and G_j applied
+    to a linear field's nodal values integrates its gradient: sum_a (G_j u)_a = slope_j * volume."""
+    from pyaldvc_amd import fem
+
+    h = np.array([2.0, 3.0, 5.0])
+    box = fem.hex8_box_matrices(h, 2)
+    np.testing.assert_allclose(box["K"].sum(axis=1), 0.0, atol=1e-13)
+    assert abs(box["M"].sum() - h.prod()) < 1e-12
+    corners = (fem.HEX8_SIGNS + 1.0) / 2.0 * h
+    slope = np.array([0.7, -1.3, 0.4])
+    u = corners @ slope
+    for j in range(3):
+        # G_j[a, b] = int dN_a/dx_j N_b: sum over a of dN_a vanishes, so use the transpose
+        assert abs((box["G"][j].T @ u).sum() - slope[j] * h.prod()) < 1e-12
