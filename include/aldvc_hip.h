@@ -206,6 +206,27 @@ int aldvc_pcg_fem(const double *Ae, int nx, int ny, int nz, const uint8_t *elem_
                   const uint8_t *node_active, const double *B, double *X, double tol, int max_iter,
                   int *iterations, void *workspace, size_t workspace_bytes, hipStream_t stream);
 
+/* ---- Differentiable warp residual (no counterpart in the reference's backward; the forward is
+ * solver/numba_kernels.py:1354-1387, evaluate_zncc_parallel, with the subset's affine warp replaced
+ * by the trilinear FE interpolation of mesh/hex8.py:58-61). Node displacements node_u
+ * (gnz, gny, gnx, 3) float32 on nodes at (ox, oy, oz) + (i sx, j sy, k sz); subsets of half-widths
+ * (hx, hy, hz) at centers (S, 3) int64 [x, y, z]. */
+
+/* C[s] = 2 (1 - ZNCC) of subset s (NaN when it leaves the volume or the sampling domain) and
+ * stats (S, 8) = n, m_f, b_f, m_g, b_g, ZNCC, valid, C for the backward pass. */
+int aldvc_warp_znssd(const float *f, const float *g, int64_t nz, int64_t ny, int64_t nx, int mode,
+                     const int64_t *centers, int64_t S, int hx, int hy, int hz, const float *node_u, int gnx,
+                     int gny, int gnz, double ox, double oy, double oz, double sx, double sy, double sz, double *C,
+                     double *stats, hipStream_t stream);
+
+/* Accumulates grad_u += dL/du (float64, node layout of node_u) and, when grad_g is not NULL,
+ * grad_g += dL/dg (float32 volume) for dL/dC = grad_C (S). */
+int aldvc_warp_znssd_backward(const float *f, const float *g, int64_t nz, int64_t ny, int64_t nx, int mode,
+                              const int64_t *centers, int64_t S, int hx, int hy, int hz, const float *node_u,
+                              int gnx, int gny, int gnz, double ox, double oy, double oz, double sx, double sy,
+                              double sz, const double *stats, const double *grad_C, double *grad_u, float *grad_g,
+                              hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

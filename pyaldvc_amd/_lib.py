@@ -83,6 +83,15 @@ SIGNATURES = {
     "aldvc_fem_nodal_gradient": (I32, [P, I32, I32, I32, P, P, P, P, P]),
     "aldvc_pcg_workspace_bytes": (SZ, [I32, I32, I32]),
     "aldvc_pcg_fem": (I32, [P, I32, I32, I32, P, P, P, P, F64, I32, ctypes.POINTER(I32), P, SZ, P]),
+    "aldvc_warp_znssd": (
+        I32,
+        [P, P, I64, I64, I64, I32, P, I64, I32, I32, I32, P, I32, I32, I32, F64, F64, F64, F64, F64, F64, P, P, P],
+    ),
+    "aldvc_warp_znssd_backward": (
+        I32,
+        [P, P, I64, I64, I64, I32, P, I64, I32, I32, I32, P, I32, I32, I32, F64, F64, F64, F64, F64, F64,
+         P, P, P, P, P],
+    ),  # fmt: skip
 }
 
 
