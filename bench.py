@@ -79,6 +79,26 @@ def normalize_torch(v):
     return ((v.double() - m) / s).float().contiguous()
 
 
+def render_pair(dev, shape, n_beads, disp, seed=0):
+    """Bead pair on the GPU: f from the beads, g from the beads moved by ``disp``, noise seeds
+    seed + 1 / seed + 2, both normalised by the HIP normalisation."""
+    import torch
+
+    from pyaldvc_amd import synthetic as syn
+    from pyaldvc_amd import volumes as V
+
+    beads = syn.random_beads(shape, n_beads, SIGMA, seed=seed)
+    moved = syn.moved(beads, disp)
+    gen = torch.Generator(device=dev)
+    f = render_torch(shape, beads.centres, beads.amps, SIGMA, dev)
+    gen.manual_seed(seed + 1)
+    f += NOISE * torch.randn(shape, device=dev, generator=gen)
+    g = render_torch(shape, moved.centres, moved.amps, SIGMA, dev)
+    gen.manual_seed(seed + 2)
+    g += NOISE * torch.randn(shape, device=dev, generator=gen)
+    return V.normalize_volume(f), V.normalize_volume(g)
+
+
 def make_workload(dev):
     import torch
 
@@ -212,11 +232,18 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run one device copy of this many MiB first (the scale of counter-based HBM traffic)")
+    ap.add_argument("--workload", default="icgn512", choices=["icgn512", "ncc-bf16", "warp-residual"],
+                    help="icgn512: the batched 12-DOF IC-GN at 512^3 (the headline line); ncc-bf16: the BF16 MFMA "
+                         "NCC search at 512^3; warp-residual: the differentiable warp residual at 256^3")  # fmt: skip
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the local step shards by node blocks with no exchange, "
               f"and multi-GPU runs are not part of this build (got --gpus {args.gpus})", file=sys.stderr)  # fmt: skip
         sys.exit(2)
+    if args.workload == "ncc-bf16":
+        return main_ncc(args)
+    if args.workload == "warp-residual":
+        return main_warp(args)
 
     import torch
 
@@ -335,6 +362,203 @@ def main():
             "algorithmic_bytes_per_launch": bytes_alg,
         },
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(result), flush=True)
+
+
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def main_ncc(args):
+    """The BF16 MFMA NCC search: 512^3 bead pair under u_z = 6 sin(2 pi x / 256), templates 24^3
+    (25^3 voxels), step 12 (41^3 = 68,921 nodes), search radius 16 (33^3 offsets). Times the
+    full search (maps + peaks + host assembly) in BF16 and in FP32; reports integer-peak agreement
+    between the two, the IC-GN refinement error against the imposed field from either seed, and
+    the map kernel's MFMA throughput."""
+    import torch
+
+    from pyaldvc_amd import integer_search as IS
+    from pyaldvc_amd import kernels as K
+    from pyaldvc_amd import synthetic as syn
+
+    dev = K.device()
+    t0 = time.time()
+    disp = syn.sinusoid_z(6.0, 256.0)
+    f, g = render_pair(dev, SHAPE, N_BEADS, disp)
+    coords = syn.node_lattice(SHAPE, (24, 24, 24), (12, 12, 12))
+    N = coords.shape[0]
+    u, v, w = disp(coords[:, 0].astype(float), coords[:, 1].astype(float), coords[:, 2].astype(float))
+    U_true = np.column_stack([u, v, w])
+    torch.cuda.synchronize()
+    log(f"volumes rendered in {time.time() - t0:.1f}s ({N} nodes)")
+    subset, radius = (24, 24, 24), (16, 16, 16)
+
+    def search(precision):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        r = IS.ncc_search(f, g, coords, subset, radius, precision=precision)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t, r
+
+    search("bf16")  # warm-up (code objects, allocator)
+    t_bf, r_bf = search("bf16")
+    t_fp, r_fp = search("fp32")
+    ok = r_bf["ok"] & r_fp["ok"]
+    n_ok = int(r_bf["ok"].sum())
+    same_peak = float(np.mean(np.all(r_bf["peak"][ok] == r_fp["peak"][ok], axis=1)))
+    # map-kernel time alone: one launch over a chunk of nodes, HIP events on the launch stream
+    hx = hy = hz = 12
+    rx = ry = rz = 16
+    nchunk = min(N, 8192)
+    cd = torch.from_numpy(coords[:nchunk]).to(dev)
+    w0 = [torch.from_numpy(coords[:nchunk, a] - 12 - 16).clamp(0, 512 - 57).to(dev) for a in range(3)]
+    ones = torch.ones(nchunk, dtype=torch.uint8, device=dev)
+    times = {}
+    for prec in ("bf16", "fp32"):
+        K.zncc_direct(f, g, cd, hx, hy, hz, rx, ry, rz, *w0, ones, precision=prec)
+        s = torch.cuda.current_stream()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(s)
+        K.zncc_direct(f, g, cd, hx, hy, hz, rx, ry, rz, *w0, ones, precision=prec)
+        b.record(s)
+        torch.cuda.synchronize()
+        times[prec] = a.elapsed_time(b) / 1e3
+    flops = 2.0 * 15625 * 35937 * nchunk
+    tflops_bf = flops / times["bf16"] / 1e12
+    tflops_fp = flops / times["fp32"] / 1e12
+    # IC-GN refinement of both seeds on the GPU (fused kernel), error against the imposed field
+    gx, gy, gz = K.compute_gradients(f)
+    nomask = np.ones((1, 1, 1), np.uint8)
+    H, L, mf, bf, nv, valid = K.precompute_nodes(torch.from_numpy(coords).to(dev), 12, 12, 12, f, gx, gy, gz, nomask, 0.5, 1e12)
+    errs = {}
+    for name, r in (("bf16", r_bf), ("fp32", r_fp)):
+        P0 = np.zeros((N, 12))
+        seed = np.where(np.isfinite(r["disp"]), np.round(r["disp"]), 0.0)
+        P0[:, 9:] = seed
+        P, it, st, zn = K.icgn_12dof_parallel(torch.from_numpy(coords).to(dev), torch.from_numpy(P0).to(dev), 12, 12, 12,
+                                              f, gx, gy, gz, nomask, g, 1, L, mf, bf, valid, 1e-2, 1e-3, 100, 5)  # fmt: skip
+        P, st = P.cpu().numpy(), st.cpu().numpy()
+        conv = (st == 0) & r["ok"]
+        e = np.linalg.norm(P[conv, 9:] - U_true[conv], axis=1)
+        errs[name] = {"converged": int(conv.sum()), "median_err_vox": float(np.median(e)), "p95_err_vox": float(np.percentile(e, 95)), "P": P, "conv": conv}
+    both = errs["bf16"]["conv"] & errs["fp32"]["conv"]
+    du = float(np.max(np.abs(errs["bf16"]["P"][both, 9:] - errs["fp32"]["P"][both, 9:]))) if both.any() else None
+    for e in errs.values():
+        e.pop("P")
+        e.pop("conv")
+    log(f"ncc bf16 {t_bf:.2f}s ({N / t_bf:,.0f} nodes/s), fp32 {t_fp:.2f}s; map kernel {tflops_bf:.0f} TFLOP/s bf16, "
+        f"{tflops_fp:.0f} TFLOP/s fp32; integer peaks equal {same_peak:.4f}; IC-GN from bf16 vs fp32 seeds max |du| {du}")  # fmt: skip
+    result = {
+        "metric": "NCC search nodes/s at 512³ (BF16 MFMA, 25³ templates, ±16 vox)",
+        "value": N / t_bf,
+        "unit": "nodes/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 1,
+        "ms_per_step": t_bf * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16",
+        "data": "synthetic",
+        "config": {
+            "workload": "512^3 f32 bead pair (250k beads sigma 1.5, noise 0.01), u_z = 6 sin(2 pi x / 256); templates 24^3, "
+                        "step 12, 41^3 nodes, search radius 16 (33^3 offsets), BF16 correlation with FP32 accumulation",
+            "nodes": N,
+            "nodes_searched": n_ok,
+            "fp32_search_s": t_fp,
+            "speedup_vs_fp32": t_fp / t_bf,
+            "integer_peak_agreement_bf16_fp32": same_peak,
+            "icgn_after_bf16_seed": errs["bf16"],
+            "icgn_after_fp32_seed": errs["fp32"],
+            "icgn_max_du_bf16_vs_fp32_seed": du,
+            "map_kernel_ms_8192_nodes": {"bf16": times["bf16"] * 1e3, "fp32": times["fp32"] * 1e3},
+        },
+        "roofline": {"bound": "mfma", "achieved": tflops_bf, "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": tflops_bf / MFMA_BF16_PEAK_TFLOPS, "traffic": None},  # fmt: skip
+        "cpu_baseline": None,
+    }
+    print(json.dumps(result), flush=True)
+
+
+def main_warp(args):
+    """The differentiable warp residual: 256^3 bead pair; node displacements on a 9^3 hex8 grid
+    (spacing 32) drawn from N(0, 1.5^2) voxel and smoothed by a Gaussian of 2 nodes; ZNSSD of
+    32^3 subsets at step 16 (15^3 = 3,375), forward and backward (node displacements and g)."""
+    import torch
+    from scipy.ndimage import gaussian_filter
+
+    from pyaldvc_amd import kernels as K
+    from pyaldvc_amd import warp as W
+
+    dev = K.device()
+    shape = (256, 256, 256)
+    rng = np.random.default_rng(0)
+    node_u = gaussian_filter(rng.normal(0.0, 1.5, (9, 9, 9, 3)), sigma=(2, 2, 2, 0))
+    origin, spacing = (0.0, 0.0, 0.0), (32.0, 32.0, 32.0)
+
+    def disp(x, y, z):
+        q = np.stack([x, y, z], 1) / 32.0
+        c = np.clip(np.floor(q).astype(np.int64), 0, 7)
+        t = q - c
+        u = np.zeros((q.shape[0], 3))
+        for k in (0, 1):
+            for j in (0, 1):
+                for i in (0, 1):
+                    N = (t[:, 0] if i else 1 - t[:, 0]) * (t[:, 1] if j else 1 - t[:, 1]) * (t[:, 2] if k else 1 - t[:, 2])
+                    u += N[:, None] * node_u[c[:, 2] + k, c[:, 1] + j, c[:, 0] + i]
+        return u[:, 0], u[:, 1], u[:, 2]
+
+    f, g = render_pair(dev, shape, N_BEADS // 8, disp)
+    ax = np.arange(16, 256 - 16 + 1, 16)[:15]
+    X, Y, Z = np.meshgrid(ax, ax, ax, indexing="ij")
+    centers = torch.from_numpy(np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1).astype(np.int64)).to(dev)
+    S = int(centers.shape[0])
+    nu = torch.from_numpy(node_u).float().to(dev).requires_grad_(True)
+    gg = g.clone().requires_grad_(True)
+
+    def step():
+        nu.grad = None
+        gg.grad = None
+        C = W.warp_residual(nu, gg, f, centers, (16, 16, 16), origin, spacing, 1)
+        torch.nansum(C).backward()
+        return C
+
+    times, wall, C = timed_steps(step, args.steps, args.warmup)
+    t_step = float(np.mean(times))
+    s = torch.cuda.current_stream()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(s)
+    for _ in range(args.steps):
+        W.warp_znssd_stats(nu.detach(), g, f, centers, (16, 16, 16), origin, spacing, 1)
+    b.record(s)
+    torch.cuda.synchronize()
+    t_fwd = a.elapsed_time(b) / 1e3 / args.steps
+    valid = int(torch.isfinite(C).sum())
+    Sv = 33**3
+    bytes_alg = 8.0 * Sv * valid + 4.0 * Sv * valid  # forward f + g per sample, backward scatter into g
+    achieved = bytes_alg / t_step / 1e9
+    log(f"warp residual: {S} subsets ({valid} valid), forward {t_fwd * 1e3:.2f} ms, forward+backward {t_step * 1e3:.2f} ms")
+    result = {
+        "metric": "warp-residual forward+backward subsets/s at 256³ (32³ subsets, 9³ hex8 node field)",
+        "value": valid / t_step,
+        "unit": "subsets/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_step * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "256^3 f32 bead pair (31,250 beads sigma 1.5, noise 0.01) under a 9^3 hex8 node field "
+                               "N(0, 1.5^2) smoothed by a 2-node Gaussian; 32^3 subsets, step 16; ZNSSD forward and "
+                               "backward to node displacements and g, cubic",
+                   "subsets": S, "subsets_valid": valid, "forward_ms": t_fwd * 1e3, "wall_s_timed": wall},  # fmt: skip
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},  # fmt: skip
+        "cpu_baseline": None,
     }
     print(json.dumps(result), flush=True)
 

@@ -38,6 +38,7 @@ def _empty(N):
         "pce": np.full(N, np.nan),
         "clipped": np.zeros(N, dtype=bool),
         "ok": np.zeros(N, dtype=bool),
+        "peak": np.zeros((N, 3), dtype=np.int64),  # integer displacement of the map peak (extra key)
     }
 
 
@@ -96,6 +97,8 @@ def ncc_search(f, g, coords, subset, radius, shift0=None, chunk: int = 64, backe
             )
         else:
             on_edge = np.zeros(n, dtype=bool)
+        res["peak"][ids] = np.column_stack([wx0[ids] + px + hx - cx[ids], wy0[ids] + py + hy - cy[ids],
+                                            wz0[ids] + pz + hz - cz[ids]])  # fmt: skip
         res["disp"][ids, 0] = wx0[ids] + px + hx - cx[ids] + sub[:, 0]
         res["disp"][ids, 1] = wy0[ids] + py + hy - cy[ids] + sub[:, 1]
         res["disp"][ids, 2] = wz0[ids] + pz + hz - cz[ids] + sub[:, 2]
@@ -126,7 +129,7 @@ def ncc_search_expanding(f, g, coords, subset, radius, shift0=None, auto_expand=
         if not sub["ok"].any():
             break
         idx = np.flatnonzero(sel)[sub["ok"]]
-        for key in ("disp", "cc", "pce", "clipped", "ok"):
+        for key in ("disp", "cc", "pce", "clipped", "ok", "peak"):
             res[key][idx] = sub[key][sub["ok"]]
         res["expansions"] += 1
         res["radius"] = rad

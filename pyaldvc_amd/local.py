@@ -7,6 +7,8 @@ Same entry points, arguments and results as the reference driver:
 * :func:`local_icgn` ``(ctx, ref, g, U0, para, mesh, F0=None) -> (U, F, info, bad)``
   (local_icgn.py:398-489)
 * :func:`fill_bad_nodes` ``(U, F, bad, mesh) -> (U, F)`` (local_icgn.py:492-522)
+* :func:`subpb1_solver` ``(ctx, ref, g, U_hat, F_hat, vdual, mu, para, mesh) -> (U, info, bad)``
+  (subpb1_solver.py:77-164), the ADMM local step
 
 ``mesh``, ``ref`` and ``para`` are duck-typed, so the reference's own ``DVCMesh``,
 ``ReferenceBundle`` / ``ReferenceSource`` and ``DVCPara`` objects work unchanged:
@@ -300,6 +302,50 @@ def local_icgn(ctx: LocalContext, ref, g, U0, para, mesh, F0=None):
     U, F = fill_bad_nodes(U, F, bad, mesh)
     info = LocalSolveInfo(n_iter=n_iter, status=status, zncc=zncc, solve_time=solve_time, n_bad=n_bad)
     return U, F, info, bad
+
+
+def subpb1_solver(ctx: LocalContext, ref, g, U_hat, F_hat, vdual, mu, para, mesh):
+    """ADMM local step: per node, the 3 translations minimising
+    ``ZNSSD(u; F_hat) + mu/2 |u - (U_hat + vdual)|^2`` from ``U_hat``; median test on the result,
+    and the nodes that fail keep ``U_hat`` (subpb1_solver.py:77-164). ``(U, info, bad)``."""
+    N = ctx.n_nodes
+    U_hat = np.ascontiguousarray(U_hat, dtype=np.float64).reshape(N, 3)
+    F_hat = np.ascontiguousarray(F_hat, dtype=np.float64).reshape(N, 3, 3)
+    vdual = np.ascontiguousarray(vdual, dtype=np.float64).reshape(N, 3)
+    mode = INTERP_MODE_BY_NAME[para.interp_method]
+    hx, hy, hz = ctx.half
+    pattern, gain = ctx.noise_args(para)
+    vols = ctx._volumes
+    d = ctx.device
+    t0 = time.perf_counter()
+    gd = K._Args.vol(g)
+    dev = gd.device
+    gx, gy, gz = (vols[k] if vols["stored"] else _otf() for k in ("gx", "gy", "gz"))
+    mask = vols["mask"] if vols["mask"] is not None else _nomask()
+    U, n_iter, status, zncc = K.icgn_3dof_parallel(
+        d["coords"], torch.from_numpy(U_hat).to(dev), torch.from_numpy(F_hat).to(dev), torch.from_numpy(vdual).to(dev),
+        hx, hy, hz, vols["f"], gx, gy, gz, mask, gd, mode, d["H"], d["meanf"], d["bottomf"], d["valid"], float(mu),
+        float(para.icgn_tol), float(para.icgn_dp_tol), int(para.icgn_max_iter), int(para.icgn_patience), ctx.stride,
+        float(pattern[9, 9]), gain, bool(para.icgn_predictive_stop), **ctx.split_args(),
+    )  # fmt: skip
+    U = U.cpu().numpy()
+    n_iter = n_iter.cpu().numpy().astype(np.int32)
+    status = status.cpu().numpy().astype(np.int8)
+    zncc = zncc.cpu().numpy().astype(np.float64)
+    solve_time = time.perf_counter() - t0
+    bad = status != STATUS_CONVERGED
+    if para.local_outlier_threshold > 0:
+        good = ~bad
+        if good.sum() > 27:
+            grid = tuple(mesh.grid_shape)
+            flag = universal_median_test(U.reshape(grid + (3,)), good.reshape(grid), para.local_outlier_threshold,
+                                         edge_ok=mesh.edges_ok_grid())  # fmt: skip
+            bad |= flag.ravel()
+    n_bad = int(np.sum(bad & (status != STATUS_INVALID_SUBSET) & (status != STATUS_SKIPPED)))
+    logger.info("Subpb1 (3-DOF): %d/%d converged, %d bad, %.2fs", int(np.sum(status == 0)), N, int(bad.sum()), solve_time)
+    U_filled = U.copy()
+    U_filled[bad] = U_hat[bad]  # nodes the local solver could not handle keep the global estimate
+    return U_filled, LocalSolveInfo(n_iter=n_iter, status=status, zncc=zncc, solve_time=solve_time, n_bad=n_bad), bad
 
 
 def fill_bad_nodes(U, F, bad, mesh):

@@ -18,6 +18,7 @@ reference's NumPy kernels, ``reference_kernels.py``) and records:
   and ``solve_subpb2`` with the direct and the PCG solver (``solver/subpb2_solver.py:45-155``) on a
   node grid with dropped elements;
 * the driver: ``precompute_local_context`` + ``local_icgn`` (``solver/local_icgn.py:223-489``)
+  and the ADMM local step ``subpb1_solver`` (``solver/subpb1_solver.py:77-164``)
   on a small bead pair, without a mask and with a cylinder mask, subset splitting and a cut mesh.
 
 Inputs are this repository's seeded generator output; they are stored next to the reference
@@ -47,6 +48,7 @@ import importlib  # noqa: E402
 rli = importlib.import_module("al_dvc.solver.local_icgn")  # the package re-exports a function of that name
 from al_dvc.solver import global_operators as rgo  # noqa: E402
 from al_dvc.solver import subpb2_solver as rsp  # noqa: E402
+from al_dvc.solver.subpb1_solver import subpb1_solver  # noqa: E402
 from al_dvc.solver import uncertainty as run  # noqa: E402
 from al_dvc.mesh import hex8 as rh8  # noqa: E402
 from al_dvc.solver.tiling import ReferenceSource  # noqa: E402
@@ -203,6 +205,16 @@ def driver_case(out, prefix, mask_kind, split):
     out[f"{prefix}_zncc"] = info.zncc
     out[f"{prefix}_bad"] = bad
     out[f"{prefix}_n_bad"] = np.array(info.n_bad)
+    rng = np.random.default_rng(17)
+    vdual = rng.normal(0.0, 0.05, U.shape)
+    U3, info3, bad3 = subpb1_solver(ctx, source, g_prep, U, F, vdual, 1e-3, para, mesh)
+    out[f"{prefix}_sp1_vdual"] = vdual
+    out[f"{prefix}_sp1_U"] = U3
+    out[f"{prefix}_sp1_status"] = info3.status
+    out[f"{prefix}_sp1_n_iter"] = info3.n_iter
+    out[f"{prefix}_sp1_zncc"] = info3.zncc
+    out[f"{prefix}_sp1_bad"] = bad3
+    out[f"{prefix}_sp1_n_bad"] = np.array(info3.n_bad)
     if mask is not None:
         out[f"{prefix}_mask"] = mask.astype(np.uint8)
         out[f"{prefix}_svf"] = rgm.subset_valid_fraction(mask.astype(np.uint8), c, para.winsize)

@@ -162,3 +162,21 @@ def test_local_driver_takes_a_bundle_with_on_the_fly_gradients(gd):
     assert np.array_equal(info.status, ref12[2])
     ok = ref12[2] == 0
     assert np.max(np.abs(U[ok & ~bad] - ref12[0][ok & ~bad, 9:])) < 1e-3
+
+
+@pytest.mark.parametrize("p", ["drv", "drvm"])
+def test_subpb1_solver_matches_reference(gd, p):
+    """The ADMM local step after the local solve: the reference's own subpb1_solver outputs."""
+    mask = gd[f"{p}_mask"].astype(bool) if p == "drvm" else None
+    edge = gd[f"{p}_edge_ok"] if p == "drvm" else None
+    mesh = LOC.NodeMesh(gd[f"{p}_coords"], tuple(gd[f"{p}_grid_shape"]), gd[f"{p}_node_valid_in"].copy(), edge)
+    ref = SimpleNamespace(f=gd["drv_f"], mask=mask, gradient_mode="stored")
+    para = _para(subset_split=p == "drvm")
+    ctx = LOC.precompute_local_context(mesh, ref, para)
+    mesh.node_valid = ctx.valid.copy()
+    U3, info, bad = LOC.subpb1_solver(ctx, ref, gd["drv_g"], gd[f"{p}_U"], gd[f"{p}_F"], gd[f"{p}_sp1_vdual"], 1e-3, para, mesh)
+    assert np.array_equal(info.status, gd[f"{p}_sp1_status"])
+    assert np.mean(info.n_iter == gd[f"{p}_sp1_n_iter"]) >= 0.95
+    assert np.array_equal(bad, gd[f"{p}_sp1_bad"])
+    assert info.n_bad == int(gd[f"{p}_sp1_n_bad"])
+    assert np.max(np.abs(U3 - gd[f"{p}_sp1_U"])) < 1e-3
