@@ -245,15 +245,19 @@ __device__ inline float sample_buf(const BufVol& b, int anchor, int rowp, int pl
 }
 
 // LDS brick sampler. A brick is a block of the deformed volume staged in LDS with row stride BRK
-// and plane stride BRK * BRK floats, so every tap offset is a compile-time immediate from one base
-// address. (rx, ry, rz) are anchor-relative positions as in sample_rel and (bx, by, bz) the
+// floats and plane stride `plane` floats (a run-time value: the compiler then forms one base per
+// stencil plane, and every tap of a plane is a small immediate offset from it, which lets it pair
+// the taps into 2-dword LDS reads). (rx, ry, rz) are anchor-relative positions as in sample_rel and (bx, by, bz) the
 // anchor-relative brick origin. The caller guarantees that the position is inside the sampling
 // domain (so the n - 3 clamp of the stencil base never applies) and that the brick holds its stencil.
 template <int MODE, int BRK>
-__device__ inline float sample_brick(const float* brick, int bx, int by, int bz, float rx, float ry, float rz) {
-    constexpr int ROW = BRK, PLANE = BRK * BRK;
-    const int ix = (int)floorf(rx), iy = (int)floorf(ry), iz = (int)floorf(rz);
-    const float fx = rx - (float)ix, fy = ry - (float)iy, fz = rz - (float)iz;
+__device__ inline float sample_brick(const float* brick, int plane, int bx, int by, int bz, float rx, float ry,
+                                     float rz) {
+    constexpr int ROW = BRK;
+    const int PLANE = plane;
+    const float flx = floorf(rx), fly = floorf(ry), flz = floorf(rz);
+    const int ix = (int)flx, iy = (int)fly, iz = (int)flz;
+    const float fx = rx - flx, fy = ry - fly, fz = rz - flz;
     if constexpr (MODE == INTERP_LINEAR) {
         const float* p = brick + ((iz - bz) * PLANE + (iy - by) * ROW + (ix - bx));
         const float c00 = p[0] * (1.0f - fx) + p[1] * fx;
@@ -274,7 +278,9 @@ __device__ inline float sample_brick(const float* brick, int bx, int by, int bz,
             bspline4(fy, wy0, wy1, wy2, wy3);
             bspline4(fz, wz0, wz1, wz2, wz3);
         }
-        const float* p = brick + ((iz - 1 - bz) * PLANE + (iy - 1 - by) * ROW + (ix - 1 - bx));
+        // small factors: 24-bit multiplies run at full rate, 32-bit ones at a quarter
+        const float* p = brick + (__mul24(iz - 1 - bz, PLANE) + __mul24(iy - 1 - by, ROW) + (ix - 1 - bx));
+        const float* pk[4] = {p, p + PLANE, p + 2 * PLANE, p + 3 * PLANE};
         // the 64 taps in two batches of 32, each read before any of it is consumed (as sample_buf)
         const float wz[4] = {wz0, wz1, wz2, wz3};
         const float wy[4] = {wy0, wy1, wy2, wy3};
@@ -287,7 +293,7 @@ __device__ inline float sample_brick(const float* brick, int bx, int by, int bz,
 #pragma unroll
                 for (int j = 0; j < 4; ++j)
 #pragma unroll
-                    for (int i = 0; i < 4; ++i) t[16 * k + 4 * j + i] = p[(2 * h + k) * PLANE + j * ROW + i];
+                    for (int i = 0; i < 4; ++i) t[16 * k + 4 * j + i] = pk[2 * h + k][j * ROW + i];
 #pragma unroll
             for (int k = 0; k < 2; ++k) {
                 float plane = 0.0f;

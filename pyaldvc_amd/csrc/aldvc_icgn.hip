@@ -206,6 +206,63 @@ struct VoxelWalk {
     }
 };
 
+// VoxelWalk that also carries the anchor-relative offsets (dx, dy, dz) and the linear voxel
+// offset o = dz * planep + dy * rowp + dx, updated by additions only (no per-voxel multiplies).
+struct OffsetWalk {
+    int v, ix, iy, dx, dy, dz, o;
+    int stx, sty, sdx, sdy, sdz, odx, ody, odz;  // per-step increments (uniform)
+    int wx, wy, ow_x, ow_y, oc_y, oc_z;          // wraps and carries (uniform)
+    __device__ inline void start(const Subset& s, int v0, int rowp, int planep) {
+        v = v0;
+        const int iz = v0 / (s.sy * s.sx), rem = v0 - iz * s.sy * s.sx;
+        iy = rem / s.sx;
+        ix = rem - iy * s.sx;
+        dx = -s.hx + s.stride * ix;
+        dy = -s.hy + s.stride * iy;
+        dz = -s.hz + s.stride * iz;
+        o = dz * planep + dy * rowp + dx;
+        const int stz = BLOCK / (s.sy * s.sx), r2 = BLOCK - stz * s.sy * s.sx;
+        sty = r2 / s.sx;
+        stx = r2 - sty * s.sx;
+        sdx = s.stride * stx;
+        sdy = s.stride * sty;
+        sdz = s.stride * stz;
+        odx = sdx;
+        ody = sdy * rowp;
+        odz = sdz * planep;
+        wx = s.stride * s.sx;
+        wy = s.stride * s.sy;
+        ow_x = wx - s.stride * rowp;   // wrap in x carries one row
+        ow_y = wy * rowp - s.stride * planep;  // wrap in y carries one plane
+        oc_y = s.stride * rowp;
+        oc_z = s.stride * planep;
+    }
+    __device__ inline void next(const Subset& s) {
+        v += BLOCK;
+        ix += stx;
+        dx += sdx;
+        o += odx;
+        if (ix >= s.sx) {
+            ix -= s.sx;
+            dx -= wx;
+            o -= ow_x;
+            ++iy;
+            dy += s.stride;
+        }
+        iy += sty;
+        dy += sdy;
+        o += ody;
+        if (iy >= s.sy) {
+            iy -= s.sy;
+            dy -= wy;
+            o -= ow_y;
+            dz += s.stride;
+        }
+        dz += sdz;
+        o += odz;
+    }
+};
+
 __device__ inline bool subset_in_volume(const Volumes& V, const Subset& s, int64_t x0, int64_t y0, int64_t z0) {
     return x0 - s.hx >= 0 && x0 + s.hx < V.nx && y0 - s.hy >= 0 && y0 + s.hy < V.ny && z0 - s.hz >= 0 &&
            z0 + s.hz < V.nz;
@@ -516,10 +573,11 @@ __device__ inline int plan_chunk(const float* sw, const Subset& s, const Domain&
 // 16-byte buffer loads of the deformed volume (out-of-range offsets read zero; brick voxels outside
 // the volume are never read by an in-domain sample), 16-byte LDS stores.
 template <int BRK>
-__device__ inline void stage_brick(float* brick, const BufVol& gb, int anchor, int rowp, int planep, int bx, int by,
-                                   int bz, int ez) {
+__device__ inline void stage_brick(float* brick, int bplane, const BufVol& gb, int anchor, int rowp, int planep, int bx,
+                                   int by, int bz, int ez) {
     static_assert(BRK % 4 == 0, "brick rows are whole 16-byte groups");
     constexpr int Q = BRK / 4, PLANE_Q = Q * BRK;
+    const int bplane_q = bplane / 4;  // plane stride in 16-byte groups (bplane is a multiple of 4)
     const int total = PLANE_Q * ez;
     const int org = anchor + bz * planep + by * rowp + bx;
     constexpr int U = 8;
@@ -536,15 +594,20 @@ __device__ inline void stage_brick(float* brick, const BufVol& gb, int anchor, i
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int i = base + u * BLOCK;
-            if (i < total) reinterpret_cast<float4*>(brick)[i] = r[u];
+            if (i < total) {
+                const int zz = i / PLANE_Q;
+                reinterpret_cast<float4*>(brick)[i + zz * (bplane_q - PLANE_Q)] = r[u];
+            }
         }
     }
 }
 
-template <int MODE, bool STORED, bool BUF, int BRK>
+// SEL: a reference mask or split keep rows select the subset voxels (false: every voxel counts)
+template <int MODE, bool STORED, bool BUF, int BRK, bool SEL>
 __global__ __launch_bounds__(BLOCK, 3) void k_icgn12_fused(IcgnArgs a, const int64_t* __restrict__ list,
                                                         const int64_t* __restrict__ count,
-                                                        const double* __restrict__ mom, int brick_planes) {
+                                                        const double* __restrict__ mom, int brick_planes,
+                                                        int brick_plane) {
     static_assert(BRK == 0 || BUF, "the brick path stages through the buffer descriptor");
     const Volumes& V = a.V;
     const Subset& s = a.sub;
@@ -609,21 +672,32 @@ __global__ __launch_bounds__(BLOCK, 3) void k_icgn12_fused(IcgnArgs a, const int
         // one subset voxel: reference-side values first (they do not depend on the warp, so their
         // loads overlap the gather of the deformed sample), then the warp, the sample and the
         // deformed-side moments. `FROM_BRICK`: every sample of the chunk is known to be inside.
-        auto voxel = [&](const VoxelWalk& w, auto from_brick, int bx, int by, int bz) {
+        auto voxel = [&](const OffsetWalk& w, auto from_brick, int bx, int by, int bz) {
             constexpr bool FROM_BRICK = decltype(from_brick)::value;
-            const int dx = -s.hx + s.stride * w.ix, dy = -s.hy + s.stride * w.iy, dz = -s.hz + s.stride * w.iz;
-            const int64_t o = obase + (int64_t)dz * planep + (int64_t)dy * rowp + dx;
-            if (!voxel_kept(V, keep, w.v, o, x0 + dx, y0 + dy, z0 + dz)) return;
-            const float fraw = V.f[o];
-            float gr[3];
-            grad_at<STORED>(V, o, (int)z0 + dz, (int)y0 + dy, (int)x0 + dx, gr[0], gr[1], gr[2]);
+            const int dx = w.dx, dy = w.dy, dz = w.dz;
+            const int64_t o = obase + w.o;
+            if constexpr (SEL)
+                if (!voxel_kept(V, keep, w.v, o, x0 + dx, y0 + dy, z0 + dz)) return;
+            float fraw, gr[3];
+            if constexpr (BUF && STORED) {
+                // a volume under 4 GB: one 32-bit byte offset serves all four reference arrays
+                const uint32_t ob = 4u * (uint32_t)((int)obase + w.o);
+                auto at = [ob](const float* vol) { return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(vol) + ob); };
+                fraw = at(V.f);
+                gr[0] = at(V.gx);
+                gr[1] = at(V.gy);
+                gr[2] = at(V.gz);
+            } else {
+                fraw = V.f[o];
+                grad_at<STORED>(V, o, (int)z0 + dz, (int)y0 + dy, (int)x0 + dx, gr[0], gr[1], gr[2]);
+            }
             const float X = (float)dx, Y = (float)dy, Z = (float)dz;
             const float xr = warp_coord(sw[0], sw[1], sw[2], sw[9], X, Y, Z);
             const float yr = warp_coord(sw[3], sw[4], sw[5], sw[10], X, Y, Z);
             const float zr = warp_coord(sw[6], sw[7], sw[8], sw[11], X, Y, Z);
             float val;
             if constexpr (FROM_BRICK) {
-                val = sample_brick<MODE, BRK>(brick, bx, by, bz, xr, yr, zr);
+                val = sample_brick<MODE, BRK>(brick, brick_plane, bx, by, bz, xr, yr, zr);
             } else {
                 if (!inside(dom, xr, yr, zr)) {
                     m[1] = 1.0f;
@@ -664,11 +738,11 @@ __global__ __launch_bounds__(BLOCK, 3) void k_icgn12_fused(IcgnArgs a, const int
                 int fit = plan_chunk<MODE, BRK>(sw, s, dom, zc, kz, brick_planes, bx, by, bz, ez);
                 while (fit == 1 && kz > 1) fit = plan_chunk<MODE, BRK>(sw, s, dom, zc, --kz, brick_planes, bx, by, bz, ez);
                 const int vend = (zc + kz) * plane_vox;
-                VoxelWalk w;
-                w.start(s, zc * plane_vox + tid);
+                OffsetWalk w;
+                w.start(s, zc * plane_vox + tid, rowp, planep);
                 if (fit == 0) {
                     __syncthreads();  // the previous chunk's samples are done with the brick
-                    stage_brick<BRK>(brick, gb, (int)obase, rowp, planep, bx, by, bz, ez);
+                    stage_brick<BRK>(brick, brick_plane, gb, (int)obase, rowp, planep, bx, by, bz, ez);
                     __syncthreads();
                     for (; w.v < vend; w.next(s)) voxel(w, std::integral_constant<bool, true>{}, bx, by, bz);
                 } else {
@@ -677,8 +751,8 @@ __global__ __launch_bounds__(BLOCK, 3) void k_icgn12_fused(IcgnArgs a, const int
                 zc += kz;
             }
         } else {
-            VoxelWalk w;
-            w.start(s, tid);
+            OffsetWalk w;
+            w.start(s, tid, rowp, planep);
             for (; w.v < s.S; w.next(s)) voxel(w, Gather{}, 0, 0, 0);
         }
         block_sum<NG, NWAVES>(m, res, red);
@@ -1296,6 +1370,40 @@ __global__ void k_init_outputs(const uint8_t* __restrict__ valid, int64_t N, int
     zncc[i] = NAN;
 }
 
+// The fused iteration kernel: LDS bricks (variant 1, when the subset's warped rows can fit one and
+// the volume is addressable with 32-bit byte offsets), else memory gathers.
+template <int MODE, bool STORED, bool SEL>
+void launch_fused(int variant, const IcgnArgs& a, const int64_t* list, const int64_t* count, int64_t nb, size_t bits,
+                  bool buf, double* mom, hipStream_t st) {
+    // LDS brick: BRICK_XY^2 floats per plane, as many planes as leave three workgroups per CU
+    int planes = 0;
+    if (variant == 1 && buf && 2 * a.sub.hx + 8 <= BRICK_XY && 2 * a.sub.hy + 8 <= BRICK_XY) {
+        hipFuncAttributes fa{};
+        if (hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&k_icgn12_fused<MODE, STORED, true, BRICK_XY, SEL>)) ==
+            hipSuccess) {
+            const int64_t room = (int64_t)LDS_PER_WG - (int64_t)fa.sharedSizeBytes - (int64_t)bits;
+            planes = (int)std::min<int64_t>(a.sub.sz + 3, room / (int64_t)(4 * BRICK_XY * BRICK_XY));
+        }
+        if (planes < 6) planes = 0;  // too few planes per chunk: the staging would dominate
+    }
+    const size_t lds = bits + (size_t)planes * 4 * BRICK_XY * BRICK_XY;
+    auto kern = &k_icgn12_fused<MODE, STORED, true, BRICK_XY, SEL>;
+    if (planes > 0 && hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+        planes = 0;
+    timer_begin(st);
+    if (planes > 0) {
+        hipLaunchKernelGGL(kern, dim3((unsigned)nb), dim3(BLOCK), lds, st, a, list, count, (const double*)mom, planes,
+                           BRICK_XY * BRICK_XY);
+    } else if (buf) {
+        hipLaunchKernelGGL((k_icgn12_fused<MODE, STORED, true, 0, SEL>), dim3((unsigned)nb), dim3(BLOCK), bits, st, a,
+                           list, count, (const double*)mom, 0, 0);
+    } else {
+        hipLaunchKernelGGL((k_icgn12_fused<MODE, STORED, false, 0, SEL>), dim3((unsigned)nb), dim3(BLOCK), bits, st, a,
+                           list, count, (const double*)mom, 0, 0);
+    }
+}
+
 // The valid-node count lives on the device (written by k_compact); grids are sized for all N
 // nodes and the workgroups past the count exit at once, so no host round trip is needed.
 template <int MODE, bool STORED>
@@ -1313,32 +1421,9 @@ void launch_icgn12(int variant, const IcgnArgs& a, const int64_t* list, const in
         hipLaunchKernelGGL((k_ref_moments<STORED>), dim3((unsigned)nb), dim3(BLOCK), 0, st, a, list, count, mom);
         const size_t bits = sizeof(unsigned) * (size_t)(((a.sub.S + 31) / 32 + 3) & ~3);
         const bool buf = (int64_t)a.V.nx * a.V.ny * a.V.nz < ((int64_t)1 << 30);  // 32-bit byte offsets
-        // LDS brick: BRICK_XY^2 floats per plane, as many planes as leave three workgroups per CU
-        int planes = 0;
-        if (variant == 1 && buf && 2 * a.sub.hx + 8 <= BRICK_XY && 2 * a.sub.hy + 8 <= BRICK_XY) {
-            hipFuncAttributes fa{};
-            if (hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&k_icgn12_fused<MODE, STORED, true, BRICK_XY>)) ==
-                hipSuccess) {
-                const int64_t room = (int64_t)LDS_PER_WG - (int64_t)fa.sharedSizeBytes - (int64_t)bits;
-                planes = (int)std::min<int64_t>(a.sub.sz + 3, room / (int64_t)(4 * BRICK_XY * BRICK_XY));
-            }
-            if (planes < 6) planes = 0;  // too few planes per chunk: the staging would dominate
-        }
-        const size_t lds = bits + (size_t)planes * 4 * BRICK_XY * BRICK_XY;
-        auto kern = &k_icgn12_fused<MODE, STORED, true, BRICK_XY>;
-        if (planes > 0 && hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
-            planes = 0;
-        timer_begin(st);
-        if (planes > 0) {
-            hipLaunchKernelGGL(kern, dim3((unsigned)nb), dim3(BLOCK), lds, st, a, list, count, (const double*)mom, planes);
-        } else if (buf) {
-            hipLaunchKernelGGL((k_icgn12_fused<MODE, STORED, true, 0>), dim3((unsigned)nb), dim3(BLOCK), bits, st, a,
-                               list, count, (const double*)mom, 0);
-        } else {
-            hipLaunchKernelGGL((k_icgn12_fused<MODE, STORED, false, 0>), dim3((unsigned)nb), dim3(BLOCK), bits, st, a,
-                               list, count, (const double*)mom, 0);
-        }
+        const bool sel = a.V.mask != nullptr || (a.split_index != nullptr && a.split_keep != nullptr);
+        if (sel) launch_fused<MODE, STORED, true>(variant, a, list, count, nb, bits, buf, mom, st);
+        else launch_fused<MODE, STORED, false>(variant, a, list, count, nb, bits, buf, mom, st);
         timer_end(st);
     }
 }
