@@ -206,6 +206,11 @@ struct VoxelWalk {
     }
 };
 
+// One subset voxel: sampling-order index v, anchor-relative offsets and linear voxel offset.
+struct Vox {
+    int v, dx, dy, dz, o;
+};
+
 // VoxelWalk that also carries the anchor-relative offsets (dx, dy, dz) and the linear voxel
 // offset o = dz * planep + dy * rowp + dx, updated by additions only (no per-voxel multiplies).
 struct OffsetWalk {
@@ -541,7 +546,7 @@ __global__ __launch_bounds__(BLOCK) void k_ref_moments(IcgnArgs a, const int64_t
 // z extent is too large (retry with fewer planes), 2 when it cannot use a brick.
 template <int MODE, int BRK>
 __device__ inline int plan_chunk(const float* sw, const Subset& s, const Domain& dom, int zc, int kz, int planes,
-                                 int& bx, int& by, int& bz, int& ez) {
+                                 int rows, int& bx, int& by, int& bz, int& ez) {
     const float Xl = (float)(-s.hx), Xh = (float)(-s.hx + s.stride * (s.sx - 1));
     const float Yl = (float)(-s.hy), Yh = (float)(-s.hy + s.stride * (s.sy - 1));
     const float Zl = (float)(-s.hz + s.stride * zc), Zh = (float)(-s.hz + s.stride * (zc + kz - 1));
@@ -565,38 +570,47 @@ __device__ inline int plan_chunk(const float* sw, const Subset& s, const Domain&
     bz = (int)floorf(lo[2]) + LO;
     const int ex = (int)floorf(hi[0]) + HI - bx + 1, ey = (int)floorf(hi[1]) + HI - by + 1;
     ez = (int)floorf(hi[2]) + HI - bz + 1;
-    if (ex > BRK || ey > BRK) return 2;
+    if (ex > BRK || ey > rows) return 2;
     return ez > planes ? 1 : 0;
 }
 
-// Stage planes [0, ez) of a BRK x BRK brick whose origin is the anchor-relative voxel (bx, by, bz):
-// 16-byte buffer loads of the deformed volume (out-of-range offsets read zero; brick voxels outside
-// the volume are never read by an in-domain sample), 16-byte LDS stores.
+// Stage planes [0, ez) of a brick of `rows` rows of BRK floats (row stride BRK, plane stride
+// `bplane` floats) whose origin is the anchor-relative voxel (bx, by, bz): 16-byte buffer loads of
+// the deformed volume (out-of-range offsets read zero; brick voxels outside the volume are never
+// read by an in-domain sample), eight loads in flight per lane, 4-byte LDS stores (the plane
+// stride need not keep rows 16-byte aligned).
 template <int BRK>
-__device__ inline void stage_brick(float* brick, int bplane, const BufVol& gb, int anchor, int rowp, int planep, int bx,
-                                   int by, int bz, int ez) {
+__device__ inline void stage_brick(float* brick, int bplane, int rows, const BufVol& gb, int anchor, int rowp,
+                                   int planep, int bx, int by, int bz, int ez) {
     static_assert(BRK % 4 == 0, "brick rows are whole 16-byte groups");
-    constexpr int Q = BRK / 4, PLANE_Q = Q * BRK;
-    const int bplane_q = bplane / 4;  // plane stride in 16-byte groups (bplane is a multiple of 4)
-    const int total = PLANE_Q * ez;
+    constexpr int Q = BRK / 4;
+    const int RQ = rows * Q;  // 16-byte groups per plane
+    const int total = RQ * ez;
     const int org = anchor + bz * planep + by * rowp + bx;
     constexpr int U = 8;
     for (int base = threadIdx.x; base < total; base += U * BLOCK) {
         float4 r[U];
+        int dst[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int i = base + u * BLOCK;
             if (i < total) {
-                const int zz = i / PLANE_Q, rem = i - zz * PLANE_Q, yy = rem / Q, qx = rem - yy * Q;
+                int zz = 0;
+                for (int t = RQ; t <= i; t += RQ) ++zz;  // ez is a handful of planes
+                const int rem = i - zz * RQ, yy = rem / Q, qx = rem - yy * Q;
                 r[u] = buf_load4(gb, 4u * (uint32_t)(org + zz * planep + yy * rowp + 4 * qx), 0);
+                dst[u] = zz * bplane + yy * BRK + 4 * qx;
             }
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int i = base + u * BLOCK;
             if (i < total) {
-                const int zz = i / PLANE_Q;
-                reinterpret_cast<float4*>(brick)[i + zz * (bplane_q - PLANE_Q)] = r[u];
+                float* d = brick + dst[u];
+                d[0] = r[u].x;
+                d[1] = r[u].y;
+                d[2] = r[u].z;
+                d[3] = r[u].w;
             }
         }
     }
@@ -607,7 +621,7 @@ template <int MODE, bool STORED, bool BUF, int BRK, bool SEL>
 __global__ __launch_bounds__(BLOCK, 3) void k_icgn12_fused(IcgnArgs a, const int64_t* __restrict__ list,
                                                         const int64_t* __restrict__ count,
                                                         const double* __restrict__ mom, int brick_planes,
-                                                        int brick_plane) {
+                                                        int brick_plane, int brick_rows, int pair_map) {
     static_assert(BRK == 0 || BUF, "the brick path stages through the buffer descriptor");
     const Volumes& V = a.V;
     const Subset& s = a.sub;
@@ -672,7 +686,7 @@ __global__ __launch_bounds__(BLOCK, 3) void k_icgn12_fused(IcgnArgs a, const int
         // one subset voxel: reference-side values first (they do not depend on the warp, so their
         // loads overlap the gather of the deformed sample), then the warp, the sample and the
         // deformed-side moments. `FROM_BRICK`: every sample of the chunk is known to be inside.
-        auto voxel = [&](const OffsetWalk& w, auto from_brick, int bx, int by, int bz) {
+        auto voxel = [&](const auto& w, auto from_brick, int bx, int by, int bz) {
             constexpr bool FROM_BRICK = decltype(from_brick)::value;
             const int dx = w.dx, dy = w.dy, dz = w.dz;
             const int64_t o = obase + w.o;
@@ -735,16 +749,81 @@ __global__ __launch_bounds__(BLOCK, 3) void k_icgn12_fused(IcgnArgs a, const int
             const int plane_vox = s.sx * s.sy;
             for (int zc = 0; zc < s.sz;) {
                 int kz = min(brick_planes - OVER, s.sz - zc), bx = 0, by = 0, bz = 0, ez = 0;
-                int fit = plan_chunk<MODE, BRK>(sw, s, dom, zc, kz, brick_planes, bx, by, bz, ez);
-                while (fit == 1 && kz > 1) fit = plan_chunk<MODE, BRK>(sw, s, dom, zc, --kz, brick_planes, bx, by, bz, ez);
+                int fit = plan_chunk<MODE, BRK>(sw, s, dom, zc, kz, brick_planes, brick_rows, bx, by, bz, ez);
+                while (fit == 1 && kz > 1)
+                    fit = plan_chunk<MODE, BRK>(sw, s, dom, zc, --kz, brick_planes, brick_rows, bx, by, bz, ez);
                 const int vend = (zc + kz) * plane_vox;
                 OffsetWalk w;
                 w.start(s, zc * plane_vox + tid, rowp, planep);
                 if (fit == 0) {
                     __syncthreads();  // the previous chunk's samples are done with the brick
-                    stage_brick<BRK>(brick, brick_plane, gb, (int)obase, rowp, planep, bx, by, bz, ez);
+                    stage_brick<BRK>(brick, brick_plane, brick_rows, gb, (int)obase, rowp, planep, bx, by, bz, ez);
                     __syncthreads();
-                    for (; w.v < vend; w.next(s)) voxel(w, std::integral_constant<bool, true>{}, bx, by, bz);
+                    using Brick = std::integral_constant<bool, true>;
+                    if (pair_map) {
+                        // Planes in pairs: lane l < sx takes voxel x = l of row y in the first plane,
+                        // lane l >= sx voxel x = l - sx of row y in the second. The brick's plane
+                        // stride is sx modulo 64 banks, so every lane of a wave reads its own LDS bank.
+                        // The second plane's last 2 sx - 64 voxels of each row, and a last unpaired
+                        // plane, follow.
+                        const int lane = tid & 63, sx = s.sx, sy = s.sy;
+                        const int inB = lane >= sx ? 1 : 0, xl = lane - inB * sx;
+                        const int npair = kz >> 1;
+                        int p = 0, y = tid >> 6;
+                        while (y >= sy) {
+                            y -= sy;
+                            ++p;
+                        }
+                        Vox q;
+                        q.dx = -s.hx + xl;
+                        q.dy = -s.hy + y;
+                        q.dz = -s.hz + zc + 2 * p + inB;
+                        q.v = ((zc + 2 * p + inB) * sy + y) * sx + xl;
+                        q.o = q.dz * planep + q.dy * rowp + q.dx;
+                        const int wrap_v = sy * sx, wrap_o = 2 * planep - sy * rowp;
+                        while (p < npair) {
+                            voxel(q, Brick{}, bx, by, bz);
+                            y += NWAVES;
+                            q.dy += NWAVES;
+                            q.v += NWAVES * sx;
+                            q.o += NWAVES * rowp;
+                            if (y >= sy) {
+                                y -= sy;
+                                ++p;
+                                q.dy -= sy;
+                                q.dz += 2;
+                                q.v += wrap_v;
+                                q.o += wrap_o;
+                            }
+                        }
+                        const int nt = 2 * sx - 64, ntail = npair * sy * nt;
+                        for (int t = tid; t < ntail; t += BLOCK) {
+                            const int r = t / nt, k = t - r * nt, pp = r / sy, yy = r - pp * sy;
+                            const int iz = zc + 2 * pp + 1, xx = 64 - sx + k;
+                            Vox e;
+                            e.dx = -s.hx + xx;
+                            e.dy = -s.hy + yy;
+                            e.dz = -s.hz + iz;
+                            e.v = (iz * sy + yy) * sx + xx;
+                            e.o = e.dz * planep + e.dy * rowp + e.dx;
+                            voxel(e, Brick{}, bx, by, bz);
+                        }
+                        if (kz & 1) {
+                            const int iz = zc + kz - 1;
+                            for (int u = tid; u < sy * sx; u += BLOCK) {
+                                const int yy = u / sx, xx = u - yy * sx;
+                                Vox e;
+                                e.dx = -s.hx + xx;
+                                e.dy = -s.hy + yy;
+                                e.dz = -s.hz + iz;
+                                e.v = (iz * sy + yy) * sx + xx;
+                                e.o = e.dz * planep + e.dy * rowp + e.dx;
+                                voxel(e, Brick{}, bx, by, bz);
+                            }
+                        }
+                    } else {
+                        for (; w.v < vend; w.next(s)) voxel(w, Brick{}, bx, by, bz);
+                    }
                 } else {
                     for (; w.v < vend; w.next(s)) voxel(w, Gather{}, 0, 0, 0);
                 }
@@ -1376,17 +1455,24 @@ template <int MODE, bool STORED, bool SEL>
 void launch_fused(int variant, const IcgnArgs& a, const int64_t* list, const int64_t* count, int64_t nb, size_t bits,
                   bool buf, double* mom, hipStream_t st) {
     // LDS brick: BRICK_XY^2 floats per plane, as many planes as leave three workgroups per CU
+    // Pair mapping (stride 1, rows of 32..64 voxels): 39 rows per plane and a plane stride of
+    // sx modulo 64 floats; otherwise 40 rows and a plane of 40 x 40.
+    const bool pair = a.sub.stride == 1 && a.sub.sx >= 32 && a.sub.sx <= 64 && a.sub.sy >= NWAVES;
+    const int rows = pair ? BRICK_XY - 1 : BRICK_XY;
+    const int plane = pair ? BRICK_XY * rows + ((a.sub.sx - BRICK_XY * rows) % 64 + 64) % 64 : BRICK_XY * rows;
     int planes = 0;
-    if (variant == 1 && buf && 2 * a.sub.hx + 8 <= BRICK_XY && 2 * a.sub.hy + 8 <= BRICK_XY) {
+    // the subset span plus the interpolation stencil and a voxel of slack must fit; whether the
+    // warped stencils of a chunk do is decided per chunk on the device
+    if (variant == 1 && buf && 2 * a.sub.hx + 5 <= BRICK_XY && 2 * a.sub.hy + 5 <= rows) {
         hipFuncAttributes fa{};
         if (hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&k_icgn12_fused<MODE, STORED, true, BRICK_XY, SEL>)) ==
             hipSuccess) {
             const int64_t room = (int64_t)LDS_PER_WG - (int64_t)fa.sharedSizeBytes - (int64_t)bits;
-            planes = (int)std::min<int64_t>(a.sub.sz + 3, room / (int64_t)(4 * BRICK_XY * BRICK_XY));
+            planes = (int)std::min<int64_t>(a.sub.sz + 3, room / (int64_t)(4 * plane));
         }
         if (planes < 6) planes = 0;  // too few planes per chunk: the staging would dominate
     }
-    const size_t lds = bits + (size_t)planes * 4 * BRICK_XY * BRICK_XY;
+    const size_t lds = bits + (size_t)planes * 4 * plane;
     auto kern = &k_icgn12_fused<MODE, STORED, true, BRICK_XY, SEL>;
     if (planes > 0 && hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
@@ -1394,13 +1480,13 @@ void launch_fused(int variant, const IcgnArgs& a, const int64_t* list, const int
     timer_begin(st);
     if (planes > 0) {
         hipLaunchKernelGGL(kern, dim3((unsigned)nb), dim3(BLOCK), lds, st, a, list, count, (const double*)mom, planes,
-                           BRICK_XY * BRICK_XY);
+                           plane, rows, pair ? 1 : 0);
     } else if (buf) {
         hipLaunchKernelGGL((k_icgn12_fused<MODE, STORED, true, 0, SEL>), dim3((unsigned)nb), dim3(BLOCK), bits, st, a,
-                           list, count, (const double*)mom, 0, 0);
+                           list, count, (const double*)mom, 0, 0, 0, 0);
     } else {
         hipLaunchKernelGGL((k_icgn12_fused<MODE, STORED, false, 0, SEL>), dim3((unsigned)nb), dim3(BLOCK), bits, st, a,
-                           list, count, (const double*)mom, 0, 0);
+                           list, count, (const double*)mom, 0, 0, 0, 0);
     }
 }
 

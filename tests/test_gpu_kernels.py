@@ -9,6 +9,7 @@ from __future__ import annotations
 
 import numpy as np
 import pytest
+from scipy import ndimage
 
 from oracle import cpu_ref, ncc_ref
 from tests.parity import MASK_NONE, OTF, assert_icgn_parity, bead_case, lattice, truth_at
@@ -158,12 +159,18 @@ def _rot_z(deg, stretch_z=0.0):
 
 # Warps that exercise the fused kernel's LDS bricks: 32^3 subsets that fit a brick, a rotation too
 # large for the brick's rows (every chunk gathers from memory), a z stretch that shrinks the chunks,
-# and nodes whose warped subsets leave the sampling domain (status OUT_OF_BOUNDS).
+# nodes whose warped subsets leave the sampling domain (status OUT_OF_BOUNDS), 35 x 33 x 27 subsets
+# (rows of 35 voxels in plane pairs, an odd plane count per chunk) in the three interpolation modes,
+# and a 15^3 subset (rows too short for plane pairs).
 BRICK_CASES = {
     "fits": dict(F=_rot_z(3.0), t=(0.6, -0.4, 0.7)),
     "rotation": dict(F=_rot_z(12.0), t=(0.3, 0.5, -0.6)),
     "stretch": dict(F=_rot_z(2.0, 0.3), t=(-0.4, 0.2, 0.5)),
     "escape": dict(F=_rot_z(1.0), t=(7.4, -1.6, 6.2)),
+    "rows35_linear": dict(F=_rot_z(2.0), t=(0.4, 0.3, -0.5), half=(17, 16, 13), mode=0),
+    "rows35_cubic": dict(F=_rot_z(2.0), t=(0.4, 0.3, -0.5), half=(17, 16, 13), mode=1),
+    "rows35_bspline": dict(F=_rot_z(2.0), t=(0.4, 0.3, -0.5), half=(17, 16, 13), mode=2),
+    "rows15": dict(F=_rot_z(3.0), t=(0.6, -0.4, 0.7), half=(7, 7, 7)),
 }
 
 
@@ -174,14 +181,16 @@ def test_icgn12_brick_paths_match_oracle(key, variant):
     cs = bead_case(shape=(112, 112, 112), F=c["F"], t=c["t"], seed=9)
     ax = 24 + 16 * np.arange(5)
     coords = np.stack(np.meshgrid(ax, ax, ax, indexing="ij"), -1).reshape(-1, 3).astype(np.int64)
-    half = (16, 16, 16)
+    half, mode = c.get("half", (16, 16, 16)), c.get("mode", 1)
     f, g, gx, gy, gz = cs["f"], cs["g"], cs["gx"], cs["gy"], cs["gz"]
+    if mode == 2:  # B-spline coefficients of g (both sides sample the same array)
+        g = ndimage.spline_filter(g, order=3, output=np.float32, mode="mirror")
     H, L, mf, bf, nv, valid = cpu_ref.precompute_nodes(coords, *half, f, gx, gy, gz, MASK_NONE, 0.5, 1e12)
     U = truth_at(cs, coords)
     P0 = np.zeros((coords.shape[0], 12))
     P0[:, :9] = c["F"].reshape(-1)
     P0[:, 9:] = np.round(U)
-    args = (coords, P0, *half, f, gx, gy, gz, MASK_NONE, g, 1, L, mf, bf, valid, 1e-2, 1e-3, 100, 5)
+    args = (coords, P0, *half, f, gx, gy, gz, MASK_NONE, g, mode, L, mf, bf, valid, 1e-2, 1e-3, 100, 5)
     ref = cpu_ref.icgn_12dof_parallel(*args)
     got = K.icgn_12dof_parallel(*args, variant=variant)
     assert_icgn_parity(ref, got)
