@@ -104,6 +104,19 @@ int aldvc_icgn3(const int64_t *coords, int64_t N, const double *U_old, double *U
                 int64_t keep_bytes, int32_t *n_iter, int8_t *status, double *zncc, void *workspace,
                 size_t workspace_bytes, hipStream_t stream);
 
+/* aldvc_icgn3 with a device flag `gate` (int): when *gate is nonzero at execution time the call
+ * writes nothing (U, n_iter, status, zncc keep their contents). For ADMM loops recorded into a
+ * graph, whose iterations after the stopping rule fired must leave the state alone (the loop of
+ * core/pipeline.py:320-358 breaks there). */
+int aldvc_icgn3_gated(const int64_t *coords, int64_t N, const double *U_old, double *U, const double *F_fixed,
+                      const double *vdual, int hx, int hy, int hz, int stride, const float *f, const float *gx,
+                      const float *gy, const float *gz, const uint8_t *mask, const float *g, int64_t nz,
+                      int64_t ny, int64_t nx, int mode, const double *H_all, const double *meanf,
+                      const double *bottomf, const uint8_t *valid, double mu, double tol, double dp_tol, int max_iter,
+                      int patience, double noise_gain, int predictive, const int64_t *split_index,
+                      const uint8_t *split_keep, int64_t keep_bytes, int32_t *n_iter, int8_t *status, double *zncc,
+                      const int *gate, void *workspace, size_t workspace_bytes, hipStream_t stream);
+
 /* ZNCC of each valid node's subset under the warp P_all (NaN where not sampled).
  * Replaces solver/numba_kernels.py:1354-1387 (evaluate_zncc_parallel). */
 int aldvc_evaluate_zncc(const int64_t *coords, int64_t N, const double *P_all, int hx, int hy, int hz,
@@ -205,6 +218,72 @@ size_t aldvc_pcg_workspace_bytes(int nx, int ny, int nz);
 int aldvc_pcg_fem(const double *Ae, int nx, int ny, int nz, const uint8_t *elem_active,
                   const uint8_t *node_active, const double *B, double *X, double tol, int max_iter,
                   int *iterations, void *workspace, size_t workspace_bytes, hipStream_t stream);
+
+/* aldvc_pcg_fem with the stopping rule left on the device: every one of the max_iter iterations
+ * is enqueued (those after convergence return at once) and nothing is read back, so the call can
+ * be recorded into a graph. *iterations_dev (device, nullable) receives the count; a nonzero
+ * *gate (device, nullable) switches the solve off. Same workspace as aldvc_pcg_fem. */
+int aldvc_pcg_fem_async(const double *Ae, int nx, int ny, int nz, const uint8_t *elem_active,
+                        const uint8_t *node_active, const double *B, double *X, double tol, int max_iter,
+                        int *iterations_dev, const int *gate, void *workspace, size_t workspace_bytes,
+                        hipStream_t stream);
+
+/* ---- One ADMM iteration on the device (core/pipeline.py:286-366). Every call takes `stop`, a
+ * device int (nullable): when nonzero at execution time the call leaves its outputs alone. */
+
+/* Workspace bytes of aldvc_admm_global_step. */
+size_t aldvc_admm_global_workspace_bytes(int nx, int ny, int nz);
+
+/* The global step from the per-node arrays: b = beta sum_j G_j (F - W)[:, :, j] + mu M (U - v) +
+ * alpha K U (zero rows at inactive nodes), U2 = solution of A U2 = b from U2 = U by the
+ * device-controlled PCG (inactive nodes keep U), F2 = nodal gradient of U2 (NaN at inactive
+ * nodes). U, v, U2 (N, 3); F, W, F2 (N, 3, 3); Ae = (beta + alpha) K + mu M (8 x 8).
+ * Replaces solver/subpb2_solver.py:108-155 (global_rhs, solve_subpb2) followed by
+ * solver/global_operators.py:147-170 (nodal_gradient). */
+int aldvc_admm_global_step(const double *mats, const double *Ae, int nx, int ny, int nz, const uint8_t *elem_active,
+                           const uint8_t *node_active, double beta, double mu, double alpha, const double *U,
+                           const double *F, const double *W, const double *v, double tol, int max_iter, double *U2,
+                           double *F2, int *iterations_dev, const int *gate, void *workspace, size_t workspace_bytes,
+                           hipStream_t stream);
+
+/* Workspace bytes of aldvc_subpb1_cleanup. */
+size_t aldvc_subpb1_cleanup_workspace_bytes(int nx, int ny, int nz);
+
+/* The local step's cleanup on the node grid (nx, ny, nz): bad = status != CONVERGED, or, when
+ * threshold > 0 and more than 27 nodes converged, flagged by the universal median test of U
+ * (eps: the test's noise level, 0.1 in the reference); U_out = bad ? U_hat : U; bad (N) 0/1.
+ * edge_ok (N, 3) uint8, the +x, +y, +z edge of every node of a cut mesh, or NULL: without it the
+ * unknown nodes are spring-filled first (tolerance fill_tol relative, at most fill_max_iter CG
+ * iterations, all enqueued). Replaces solver/subpb1_solver.py:140-164 with
+ * utils/outlier_detection.py:98-160 and utils/inpaint.py:35-121. */
+int aldvc_subpb1_cleanup(int nx, int ny, int nz, const uint8_t *edge_ok, const int8_t *status, const double *U,
+                         const double *U_hat, double threshold, double eps, double fill_tol, int fill_max_iter,
+                         double *U_out, uint8_t *bad, const int *stop, void *workspace, size_t workspace_bytes,
+                         hipStream_t stream);
+
+/* Workspace bytes of aldvc_spring_fill. */
+size_t aldvc_spring_fill_workspace_bytes(int nx, int ny, int nz);
+
+/* Spring-model fill of the NaN entries of vals (N, 3) on the node grid, column by column, in place:
+ * the discrete Laplace equation over 6-neighbour springs (none across cut edges when edge_ok is
+ * given), known nodes as data, a 1e-10 Tikhonov term. Jacobi-CG to tol (relative), at most
+ * max_iter iterations; *iterations (host) receives the count. Replaces utils/inpaint.py:35-121
+ * (fill_nan_grid, "spring"). */
+int aldvc_spring_fill(int nx, int ny, int nz, const uint8_t *edge_ok, double *vals, double tol, int max_iter,
+                      int *iterations, void *workspace, size_t workspace_bytes, hipStream_t stream);
+
+/* Workspace bytes of aldvc_dual_update. */
+size_t aldvc_dual_update_workspace_bytes(int64_t N);
+
+/* The end of an ADMM iteration: history[4 step + k] = RMS of U2n - U2, U1n - U1, U2n - U1n and
+ * F2n' - F2 where F2n' = finite(F2n) ? F2n : F2; then W = W + (F2n' - F2) and v = v + (U2n - U1n)
+ * (accumulate; else the differences alone), U1 = U1n, U2 = U2n, F2 = F2n'; *steps = step. With
+ * check_stop, *stop = 1 when either of the first two RMS values is below admm_tol.
+ * Replaces core/pipeline.py:300-358 (the dual update and the stopping rule). */
+int aldvc_dual_update(int64_t N, const double *U1n, const double *U2n, const double *F2n, double *U1, double *U2,
+                      double *F2, double *W, double *v, int accumulate, int check_stop, double admm_tol, int step,
+                      double *history, int *stop, int *steps, void *workspace, size_t workspace_bytes,
+                      hipStream_t stream);
 
 /* ---- Differentiable warp residual (no counterpart in the reference's backward; the forward is
  * solver/numba_kernels.py:1354-1387, evaluate_zncc_parallel, with the subset's affine warp replaced

@@ -60,6 +60,17 @@ SIGNATURES = {
             P, P, P, P, SZ, P,  # n_iter, status, zncc, workspace, bytes, stream
         ],
     ),  # fmt: skip
+    "aldvc_icgn3_gated": (
+        I32,
+        [
+            P, I64, P, P, P, P, I32, I32, I32, I32,  # coords, N, U_old, U, F, vdual, hx, hy, hz, stride
+            P, P, P, P, P, P, I64, I64, I64, I32,  # f, gx, gy, gz, mask, g, nz, ny, nx, mode
+            P, P, P, P,  # H, meanf, bottomf, valid
+            F64, F64, F64, I32, I32, F64, I32,  # mu, tol, dp_tol, max_iter, patience, noise_gain, predictive
+            P, P, I64,  # split
+            P, P, P, P, P, SZ, P,  # n_iter, status, zncc, gate, workspace, bytes, stream
+        ],
+    ),  # fmt: skip
     "aldvc_evaluate_zncc": (
         I32,
         [P, I64, P, I32, I32, I32, I32, P, P, P, I64, I64, I64, I32, P, P, P],
@@ -83,6 +94,18 @@ SIGNATURES = {
     "aldvc_fem_nodal_gradient": (I32, [P, I32, I32, I32, P, P, P, P, P]),
     "aldvc_pcg_workspace_bytes": (SZ, [I32, I32, I32]),
     "aldvc_pcg_fem": (I32, [P, I32, I32, I32, P, P, P, P, F64, I32, ctypes.POINTER(I32), P, SZ, P]),
+    "aldvc_pcg_fem_async": (I32, [P, I32, I32, I32, P, P, P, P, F64, I32, P, P, P, SZ, P]),
+    "aldvc_admm_global_workspace_bytes": (SZ, [I32, I32, I32]),
+    "aldvc_admm_global_step": (
+        I32,
+        [P, P, I32, I32, I32, P, P, F64, F64, F64, P, P, P, P, F64, I32, P, P, P, P, P, SZ, P],
+    ),
+    "aldvc_subpb1_cleanup_workspace_bytes": (SZ, [I32, I32, I32]),
+    "aldvc_subpb1_cleanup": (I32, [I32, I32, I32, P, P, P, P, F64, F64, F64, I32, P, P, P, P, SZ, P]),
+    "aldvc_spring_fill_workspace_bytes": (SZ, [I32, I32, I32]),
+    "aldvc_spring_fill": (I32, [I32, I32, I32, P, P, F64, I32, ctypes.POINTER(I32), P, SZ, P]),
+    "aldvc_dual_update_workspace_bytes": (SZ, [I64]),
+    "aldvc_dual_update": (I32, [I64, P, P, P, P, P, P, P, P, I32, I32, F64, I32, P, P, P, P, SZ, P]),
     "aldvc_warp_znssd": (
         I32,
         [P, P, I64, I64, I64, I32, P, I64, I32, I32, I32, P, I32, I32, I32, F64, F64, F64, F64, F64, F64, P, P, P],

@@ -1,0 +1,319 @@
+"""The ADMM iterations of a frame on the GPU, with no host round trip inside an iteration, so the
+whole loop can be recorded into one HIP graph and replayed (core/pipeline.py:286-366).
+
+One iteration (steps 2..``admm_max_iter`` of the reference loop) is:
+
+1. subproblem 1: translation IC-GN of every node with ``F`` held at ``F2`` and the proximal pull
+   towards ``U2 + v`` (``aldvc_icgn3_gated``);
+2. its cleanup: bad = not converged, or flagged by the universal median test (unknown nodes
+   spring-filled first); bad nodes keep ``U2`` (``aldvc_subpb1_cleanup``,
+   solver/subpb1_solver.py:140-164);
+3. subproblem 2: the FEM right-hand side, the float64 PCG solve and the nodal gradient
+   (``aldvc_admm_global_step``, solver/subpb2_solver.py:108-155, global_operators.py:147-170);
+4. the dual update, the RMS updates and the stopping rule (``aldvc_dual_update``).
+
+The first pass after the local 12-DOF solve (the global step from ``W = 0``, ``v = 0``) is the same
+sequence without 1-2. Every launch reads a device flag ``stop`` that the dual update sets when the
+reference loop would break (RMS update of ``U`` global or local below ``admm_tol``), so the
+iterations recorded after that point change nothing and the results equal the reference loop's.
+
+The spring fill and the median test are also exposed on their own (``spring_fill``,
+``subpb1_cleanup``) with the reference's host signatures.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import kernels as K
+from ._lib import check, lib
+from .local import INTERP_MODE_BY_NAME, LocalSolveInfo
+
+FILL_TOL = 1e-12  # relative residual of the spring fill's CG (the reference solves it directly up to 60k unknowns)
+FILL_MAX_ITER = 2000  # the reference's CG limit for the fill (inpaint.py:118)
+MEDIAN_EPS = 0.1  # noise level of the universal median test (outlier_detection.py:138)
+DIRECT_GRAPH_ITER = 2000  # PCG iterations recorded for the "direct" global solver (driven to round-off)
+STATUS_CONVERGED, STATUS_INVALID_SUBSET, STATUS_SKIPPED = 0, 3, 6
+
+
+def _dev(a, dtype=torch.float64, shape=None):
+    """A device tensor of ``a`` (host array or tensor), optionally reshaped."""
+    if isinstance(a, torch.Tensor):
+        t = a.to(device=K.device(), dtype=dtype)
+    else:
+        t = torch.from_numpy(np.ascontiguousarray(a)).to(device=K.device(), dtype=dtype)
+    return (t if shape is None else t.reshape(shape)).contiguous()
+
+
+def _ws(nbytes: int) -> torch.Tensor:
+    return torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=K.device())
+
+
+def _edges(edge_ok, grid_shape):
+    if edge_ok is None:
+        return None
+    e = np.asarray(edge_ok, dtype=np.uint8).reshape(int(np.prod(grid_shape)), 3)
+    return _dev(e, torch.uint8)
+
+
+# ---------------------------------------------------------------------------- stand-alone pieces
+def spring_fill(values, grid_shape, edge_ok=None, tol: float = FILL_TOL, max_iter: int = FILL_MAX_ITER) -> np.ndarray:
+    """``values`` (N, 3) with NaN holes on ``grid_shape`` (nz, ny, nx), each column filled by the
+    spring model (utils/inpaint.py:35-121 with method "spring"); host arrays in and out."""
+    nz, ny, nx = (int(v) for v in grid_shape)
+    vals = _dev(values, shape=(nz * ny * nx, 3))
+    ws = _ws(lib().aldvc_spring_fill_workspace_bytes(nx, ny, nz))
+    edges = _edges(edge_ok, grid_shape)
+    it = ctypes.c_int(0)
+    check(
+        lib().aldvc_spring_fill(nx, ny, nz, K._ptr(edges), K._ptr(vals), float(tol), int(max_iter),
+                                ctypes.byref(it), K._ptr(ws), int(ws.numel()), K.stream_handle()),
+        "aldvc_spring_fill",
+    )  # fmt: skip
+    return vals.cpu().numpy()
+
+
+def subpb1_cleanup(U, U_hat, status, grid_shape, threshold: float, edge_ok=None, eps: float = MEDIAN_EPS):
+    """``(U_out, bad)``: bad = not converged, or (threshold > 0 and more than 27 converged) flagged
+    by the universal median test; bad nodes take ``U_hat`` (solver/subpb1_solver.py:140-164)."""
+    nz, ny, nx = (int(v) for v in grid_shape)
+    N = nz * ny * nx
+    Ud, Uh = _dev(U, shape=(N, 3)), _dev(U_hat, shape=(N, 3))
+    st = _dev(status, torch.int8, (N,))
+    out = torch.empty_like(Ud)
+    bad = torch.empty(N, dtype=torch.uint8, device=Ud.device)
+    ws = _ws(lib().aldvc_subpb1_cleanup_workspace_bytes(nx, ny, nz))
+    edges = _edges(edge_ok, grid_shape)
+    check(
+        lib().aldvc_subpb1_cleanup(nx, ny, nz, K._ptr(edges), K._ptr(st), K._ptr(Ud), K._ptr(Uh),
+                                   float(threshold), float(eps), FILL_TOL, FILL_MAX_ITER, K._ptr(out), K._ptr(bad),
+                                   None, K._ptr(ws), int(ws.numel()), K.stream_handle()),
+        "aldvc_subpb1_cleanup",
+    )  # fmt: skip
+    return out.cpu().numpy(), bad.cpu().numpy().astype(bool)
+
+
+# ---------------------------------------------------------------------------- the loop
+@dataclass
+class ADMMInfo:
+    """The reference's ``ADMMInfo`` (core/pipeline.py:359-370)."""
+
+    beta: float
+    mu: float
+    n_steps: int
+    update_global: tuple
+    update_local: tuple
+    primal_residual_u: tuple
+    primal_residual_f: tuple
+    local_info: tuple
+    subpb2_time: float
+    pcg_iterations: tuple = field(default=())
+    beta_sweep: object = None
+
+
+class AdmmLoop:
+    """Device state of the ADMM iterations of one frame.
+
+    ``ctx`` (LocalContext), ``g`` (deformed volume), ``mesh`` (NodeMesh), ``system``
+    (fem.GlobalSystem) and ``para`` as for the reference loop; ``U_local`` (N, 3) and ``F_local``
+    (N, 3, 3) the output of the local 12-DOF step. ``run(graph=...)`` executes the first pass and
+    steps 2..``para.admm_max_iter``; ``graph=True`` records them once into a HIP graph (a
+    ``torch.cuda.CUDAGraph``) and replays it."""
+
+    def __init__(self, ctx, g, mesh, system, para):
+        self.ctx, self.mesh, self.system, self.para = ctx, mesh, system, para
+        ops = system.ops
+        self.grid = tuple(int(v) for v in ops.grid_shape)
+        nz, ny, nx = self.grid
+        N = nz * ny * nx
+        if N != ctx.n_nodes:
+            raise ValueError("the operators' node grid and the local context disagree")
+        self.N = N
+        self.max_steps = int(para.admm_max_iter)
+        dev = K.device()
+        f64 = dict(dtype=torch.float64, device=dev)
+        z3 = lambda: torch.zeros((N, 3), **f64)  # noqa: E731
+        z9 = lambda: torch.zeros((N, 9), **f64)  # noqa: E731
+        self.U1, self.U2, self.v = z3(), z3(), z3()
+        self.F2, self.W = z9(), z9()
+        self.U1raw, self.U1n, self.U2n = z3(), z3(), z3()
+        self.F2n = z9()
+        S = self.max_steps + 1
+        self.status = torch.zeros((S, N), dtype=torch.int8, device=dev)
+        self.zncc = torch.zeros((S, N), **f64)
+        self.n_iter = torch.zeros((S, N), dtype=torch.int32, device=dev)
+        self.bad = torch.zeros((S, N), dtype=torch.uint8, device=dev)
+        self.stop = torch.zeros(1, dtype=torch.int32, device=dev)
+        self.steps = torch.zeros(1, dtype=torch.int32, device=dev)
+        self.hist = torch.full((S, 4), float("nan"), **f64)
+        self.pcg_iters = torch.zeros(S, dtype=torch.int32, device=dev)
+        L = lib()
+        self.ws_global = _ws(L.aldvc_admm_global_workspace_bytes(nx, ny, nz))
+        self.ws_clean = _ws(L.aldvc_subpb1_cleanup_workspace_bytes(nx, ny, nz))
+        self.ws_dual = _ws(L.aldvc_dual_update_workspace_bytes(N))
+        self.ws_icgn3 = _ws(L.aldvc_icgn3_workspace_bytes(N))
+        self.edge_ok = _edges(mesh.edges_ok_grid(), self.grid)
+        # 3-DOF inputs (solver/subpb1_solver.py:77-133)
+        vols = ctx._volumes
+        self.vols = vols
+        self.gd = K._Args.vol(g)
+        self.mode = INTERP_MODE_BY_NAME[para.interp_method]
+        self.pattern, self.gain = ctx.noise_args(para)
+        sp = ctx.split_args()
+        self.split = K._split(K._Args(self.gd), N, sp.get("split_index"), sp.get("split_keep"))
+        if system.solver == "direct":
+            self.pcg_tol, self.pcg_max = 1e-13, int(min(max(system.pcg_max_iter, 10 * ops.n_active), DIRECT_GRAPH_ITER))
+        else:
+            self.pcg_tol, self.pcg_max = float(system.pcg_tol), int(system.pcg_max_iter)
+        self.graph = None
+        self.time_s = 0.0
+
+    # -- launches (all on the current stream, no host synchronisation) -------------------------
+    def _global(self, step):
+        s, ops = self.system, self.system.ops
+        nz, ny, nx = self.grid
+        check(
+            lib().aldvc_admm_global_step(
+                K._ptr(ops.mats), K._ptr(s.Ae), nx, ny, nz, K._ptr(ops.elem_active), K._ptr(ops.node_active),
+                s.beta, s.mu, s.alpha, K._ptr(self.U1n), K._ptr(self.F2), K._ptr(self.W), K._ptr(self.v),
+                self.pcg_tol, self.pcg_max, K._ptr(self.U2n), K._ptr(self.F2n), K._ptr(self.pcg_iters[step:]),
+                K._ptr(self.stop), K._ptr(self.ws_global), int(self.ws_global.numel()), K.stream_handle(),
+            ),
+            "aldvc_admm_global_step",
+        )  # fmt: skip
+
+    def _dual(self, step, check_stop):
+        check(
+            lib().aldvc_dual_update(
+                self.N, K._ptr(self.U1n), K._ptr(self.U2n), K._ptr(self.F2n), K._ptr(self.U1), K._ptr(self.U2),
+                K._ptr(self.F2), K._ptr(self.W), K._ptr(self.v), int(self.para.dual_update == "accumulate"),
+                int(check_stop), float(self.para.admm_tol), int(step), K._ptr(self.hist), K._ptr(self.stop),
+                K._ptr(self.steps), K._ptr(self.ws_dual), int(self.ws_dual.numel()), K.stream_handle(),
+            ),
+            "aldvc_dual_update",
+        )  # fmt: skip
+
+    def _subpb1(self, step):
+        ctx, p, d, vols = self.ctx, self.para, self.ctx.device, self.vols
+        hx, hy, hz = ctx.half
+        nz, ny, nx = (int(v) for v in vols["f"].shape)
+        stored = vols["stored"]
+        gx, gy, gz = (vols[k] if stored else None for k in ("gx", "gy", "gz"))
+        si, sk, kb = self.split
+        check(
+            lib().aldvc_icgn3_gated(
+                K._ptr(d["coords"]), self.N, K._ptr(self.U2), K._ptr(self.U1raw), K._ptr(self.F2), K._ptr(self.v),
+                hx, hy, hz, int(ctx.stride), K._ptr(vols["f"]), K._ptr(gx), K._ptr(gy), K._ptr(gz),
+                K._ptr(vols["mask"]), K._ptr(self.gd), nz, ny, nx, self.mode, K._ptr(d["H"]), K._ptr(d["meanf"]),
+                K._ptr(d["bottomf"]), K._ptr(d["valid"]), float(self.system.mu), float(p.icgn_tol),
+                float(p.icgn_dp_tol), int(p.icgn_max_iter), int(p.icgn_patience),
+                float(self.gain) if self.gain > 0 else 0.0, int(bool(p.icgn_predictive_stop)), K._ptr(si), K._ptr(sk),
+                kb, K._ptr(self.n_iter[step]), K._ptr(self.status[step]), K._ptr(self.zncc[step]), K._ptr(self.stop),
+                K._ptr(self.ws_icgn3), int(self.ws_icgn3.numel()), K.stream_handle(),
+            ),
+            "aldvc_icgn3_gated",
+        )  # fmt: skip
+        nz_, ny_, nx_ = self.grid
+        check(
+            lib().aldvc_subpb1_cleanup(
+                nx_, ny_, nz_, K._ptr(self.edge_ok), K._ptr(self.status[step]), K._ptr(self.U1raw), K._ptr(self.U2),
+                float(p.local_outlier_threshold), MEDIAN_EPS, FILL_TOL, FILL_MAX_ITER, K._ptr(self.U1n),
+                K._ptr(self.bad[step]), K._ptr(self.stop), K._ptr(self.ws_clean), int(self.ws_clean.numel()),
+                K.stream_handle(),
+            ),
+            "aldvc_subpb1_cleanup",
+        )  # fmt: skip
+
+    def _enqueue(self):
+        """The first pass and every later step, in stream order."""
+        self.U1n.copy_(self.U1)
+        self._global(1)
+        self._dual(1, False)
+        for step in range(2, self.max_steps + 1):
+            self._subpb1(step)
+            self._global(step)
+            self._dual(step, True)
+
+    # -- public ----------------------------------------------------------------------------------
+    def load(self, U_local, F_local, status_local=None, zncc_local=None):
+        """The local 12-DOF result as the loop's start: U1 = U_local, F2 = F_local (the F1 of the
+        first pass), W = v = 0."""
+        N = self.N
+        self.U1.copy_(_dev(U_local, shape=(N, 3)))
+        self.U2.copy_(self.U1)
+        self.F2.copy_(_dev(F_local, shape=(N, 9)))
+        self.W.zero_()
+        self.v.zero_()
+        self.stop.zero_()
+        self.steps.zero_()
+        self.hist.fill_(float("nan"))
+        self.pcg_iters.zero_()
+        self.status.zero_()
+        self.zncc.zero_()
+        self.n_iter.zero_()
+        self.bad.zero_()
+
+    def run(self, graph: bool = False):
+        """Executes the loop on the current state (call ``load`` first). With ``graph`` the
+        sequence is recorded once (the first call) and replayed."""
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        if not graph:
+            self._enqueue()
+        else:
+            if self.graph is None:
+                self.capture()
+            self.graph.replay()
+        torch.cuda.synchronize()
+        self.time_s = time.perf_counter() - t0
+
+    def capture(self):
+        """Records the loop into a HIP graph (the state buffers are fixed, so a replay runs the
+        loop on whatever ``load`` put there)."""
+        g = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            with torch.cuda.graph(g, stream=s):
+                self._enqueue()
+        torch.cuda.current_stream().wait_stream(s)
+        self.graph = g
+
+    def results(self, info_local: LocalSolveInfo | None = None):
+        """``(U_final, F_final, ADMMInfo, status, zncc, bad_last)`` on the host, as the reference
+        loop leaves them (core/pipeline.py:359-380)."""
+        n = int(self.steps.item())
+        hist = self.hist.cpu().numpy()
+        iters = self.pcg_iters.cpu().numpy()
+        infos = [info_local] if info_local is not None else []
+        st = self.status.cpu().numpy()
+        zn = self.zncc.cpu().numpy()
+        ni = self.n_iter.cpu().numpy()
+        bad = self.bad.cpu().numpy().astype(bool)
+        per_step = self.time_s / max(1, n)
+        for k in range(2, n + 1):
+            s = st[k]
+            n_bad = int(np.sum(bad[k] & (s != STATUS_INVALID_SUBSET) & (s != STATUS_SKIPPED)))
+            infos.append(LocalSolveInfo(n_iter=ni[k], status=s, zncc=zn[k], solve_time=per_step, n_bad=n_bad))
+        info = ADMMInfo(
+            beta=self.system.beta, mu=self.system.mu, n_steps=n,
+            update_global=tuple(float(hist[k, 0]) for k in range(2, n + 1)),
+            update_local=tuple(float(hist[k, 1]) for k in range(2, n + 1)),
+            primal_residual_u=tuple(float(hist[k, 2]) for k in range(1, n + 1)),
+            primal_residual_f=tuple(float(hist[k, 3]) for k in range(1, n + 1)),
+            local_info=tuple(infos), subpb2_time=float("nan"), pcg_iterations=tuple(int(i) for i in iters[1 : n + 1]),
+        )  # fmt: skip
+        U = self.U2.cpu().numpy()
+        F = self.F2.cpu().numpy().reshape(self.N, 3, 3)
+        if n >= 2:
+            status, zncc, bad_last = st[n], zn[n], bad[n]
+        elif info_local is not None:
+            status, zncc, bad_last = info_local.status, info_local.zncc, None
+        else:
+            status = zncc = bad_last = None
+        return U, F, info, status, zncc, bad_last

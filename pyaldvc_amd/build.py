@@ -15,7 +15,15 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libaldvc_hip.so")
-SOURCES = ["aldvc_icgn.hip", "aldvc_ncc.hip", "aldvc_split.hip", "aldvc_volume.hip", "aldvc_global.hip", "aldvc_warp.hip"]
+SOURCES = [
+    "aldvc_icgn.hip",
+    "aldvc_ncc.hip",
+    "aldvc_split.hip",
+    "aldvc_volume.hip",
+    "aldvc_global.hip",
+    "aldvc_warp.hip",
+    "aldvc_admm.hip",
+]
 ARCH = os.environ.get("ALDVC_OFFLOAD_ARCH", "gfx950")
 
 

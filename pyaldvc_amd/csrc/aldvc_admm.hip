@@ -1,0 +1,553 @@
+// The rest of an ADMM iteration on the GPU: the local step's outlier cleanup and the dual update,
+// so that a whole iteration (3-DOF IC-GN, cleanup, global step, dual update) runs without host
+// round trips and can be recorded into a HIP graph.
+//
+// Reference:
+//   solver/subpb1_solver.py:135-164   bad = status != CONVERGED, plus the universal median test on
+//                                     the good nodes when more than 27 are good; bad nodes keep U_hat
+//   utils/outlier_detection.py:98-160 normalized fluctuation |u - med| / (med |r| + eps) over the
+//                                     26 neighbours (median_filter, mode "nearest"); unknown nodes
+//                                     spring-filled first; cut meshes: reachable neighbours only,
+//                                     nanmedian, fewer than 6 neighbours not judged (:24-90)
+//   utils/inpaint.py:35-121           spring fill: the discrete Laplace equation on the unknown
+//                                     nodes, known neighbours as data, 1e-10 Tikhonov term
+//   core/pipeline.py:286-366          F2 = nodal gradient where finite else F1; W += F2 - F1,
+//                                     v += U2 - U1 (or reset); RMS updates; stop when the RMS
+//                                     update of U_global or U_local drops below admm_tol
+//
+// Every launch takes a device flag `stop` (nullable): when set, the launch does nothing, which is
+// how the iterations after the stopping rule fired are skipped inside a replayed graph.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "aldvc_abi_util.h"
+#include "aldvc_pcg.h"
+
+using namespace aldvc;
+
+namespace {
+
+constexpr int ABLOCK = 256;
+constexpr double SPRING_TIKHONOV = 1e-10;  // inpaint.py:103-104
+constexpr int MIN_NEIGHBOURS = 6;          // outlier_detection.py:21
+constexpr int ST_CONVERGED = 0;
+
+struct NodeGrid {
+    int nx, ny, nz;
+    __device__ int64_t n() const { return (int64_t)nx * ny * nz; }
+    __device__ int dim(int ax) const { return ax == 0 ? nx : (ax == 1 ? ny : nz); }
+    __device__ int64_t lin(const int p[3]) const { return ((int64_t)p[2] * ny + p[1]) * nx + p[0]; }
+    __device__ void pos(int64_t a, int p[3]) const {
+        p[0] = (int)(a % nx);
+        p[1] = (int)((a / nx) % ny);
+        p[2] = (int)(a / ((int64_t)nx * ny));
+    }
+};
+
+// Control of one cleanup: whether the median test runs, and per column whether any node is known.
+struct MedCtl {
+    int skip;       // 1: no median test (stopped, threshold <= 0, or at most 27 good nodes)
+    int col_on[3];  // column c has at least one known node
+    unsigned ticket;
+    int pad[3];
+};
+
+__device__ inline bool gated(const int* stop) { return stop && *stop; }
+
+// ---------------------------------------------------------------------------- spring fill
+// Unknown nodes of column c: bit c of unknown[a]. Springs join 6-neighbours inside the grid, except
+// across cut edges (edge_ok (N, 3): the +x, +y, +z edge of every node; null for an uncut mesh).
+struct SpringOp {
+    NodeGrid g;
+    const uint8_t* unknown;
+    const uint8_t* edge_ok;
+
+    template <class F>
+    __device__ void for_springs(int64_t a, F f) const {
+        int p[3];
+        g.pos(a, p);
+        const int64_t stride[3] = {1, (int64_t)g.nx, (int64_t)g.nx * g.ny};
+        for (int ax = 0; ax < 3; ++ax) {
+            if (p[ax] > 0 && (!edge_ok || edge_ok[3 * (a - stride[ax]) + ax])) f(a - stride[ax]);
+            if (p[ax] < g.dim(ax) - 1 && (!edge_ok || edge_ok[3 * a + ax])) f(a + stride[ax]);
+        }
+    }
+    __device__ double diag(int64_t a) const {
+        int deg = 0;
+        for_springs(a, [&](int64_t) { ++deg; });
+        return (double)deg + SPRING_TIKHONOV;
+    }
+    __device__ void inv_diag(int64_t a, double d[3]) const {
+        const double dg = diag(a);
+        for (int c = 0; c < 3; ++c) d[c] = (unknown[a] >> c) & 1 ? 1.0 / dg : 0.0;
+    }
+    template <class X>
+    __device__ void apply(int64_t a, const X& x, double y[3]) const {
+        const unsigned u = unknown[a];
+        if (!u) {
+            y[0] = y[1] = y[2] = 0.0;
+            return;
+        }
+        const double dg = diag(a);
+        for (int c = 0; c < 3; ++c) y[c] = (u >> c) & 1 ? dg * x(a, c) : 0.0;
+        for_springs(a, [&](int64_t q) {
+            const unsigned uq = unknown[q] & u;
+            for (int c = 0; c < 3; ++c)
+                if ((uq >> c) & 1) y[c] -= x(q, c);
+        });
+    }
+};
+
+// bad = status != CONVERGED; unknown bits (not good, or NaN); the median test's switches
+__global__ __launch_bounds__(ABLOCK) void k_cleanup_setup(NodeGrid g, const int8_t* __restrict__ status,
+                                                          const double* __restrict__ U, double threshold,
+                                                          uint8_t* __restrict__ unknown, double* __restrict__ part,
+                                                          MedCtl* ctl, const int* __restrict__ stop) {
+    if (gated(stop)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) ctl->skip = 1;
+        return;
+    }
+    __shared__ double red[4][pcg::BLOCK];
+    double s[4] = {0.0, 0.0, 0.0, 0.0};  // good nodes, known nodes per column
+    const int64_t N = g.n();
+    for (int64_t a = (int64_t)blockIdx.x * ABLOCK + threadIdx.x; a < N; a += (int64_t)gridDim.x * ABLOCK) {
+        const bool good = status[a] == ST_CONVERGED;
+        unsigned u = 0;
+        for (int c = 0; c < 3; ++c) {
+            const bool unk = !good || isnan(U[3 * a + c]);
+            u |= (unk ? 1u : 0u) << c;
+            s[1 + c] += unk ? 0.0 : 1.0;
+        }
+        unknown[a] = (uint8_t)u;
+        s[0] += good ? 1.0 : 0.0;
+    }
+    pcg::write_partials<4>(s, part, red);
+    if (pcg::last_block(&ctl->ticket)) {
+        double t[4];
+        pcg::total_partials<4>(part, t, red);
+        if (threadIdx.x == 0) {
+            ctl->skip = (threshold > 0.0 && t[0] > 27.0) ? 0 : 1;
+            for (int c = 0; c < 3; ++c) ctl->col_on[c] = t[1 + c] > 0.0 ? 1 : 0;
+            ctl->ticket = 0u;
+        }
+    }
+}
+
+// right-hand side of the fill: the sum of the known spring neighbours; X = 0
+__global__ void k_spring_rhs(SpringOp op, const double* __restrict__ vals, double* __restrict__ b,
+                             double* __restrict__ X, const int* __restrict__ skip) {
+    if (skip && *skip) return;
+    const int64_t N = op.g.n();
+    for (int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; a < N; a += (int64_t)gridDim.x * blockDim.x) {
+        const unsigned u = op.unknown[a];
+        double r[3] = {0.0, 0.0, 0.0};
+        if (u) {
+            op.for_springs(a, [&](int64_t q) {
+                const unsigned kq = ~op.unknown[q] & u;  // known at q, unknown here
+                for (int c = 0; c < 3; ++c)
+                    if ((kq >> c) & 1) r[c] += vals[3 * q + c];
+            });
+        }
+        for (int c = 0; c < 3; ++c) {
+            b[3 * a + c] = r[c];
+            X[3 * a + c] = 0.0;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------- median test
+// Values of ranks r0 and r1 (0-based) among the entries of v with ok set; registers only.
+template <int M>
+__device__ inline void select2(const double (&v)[M], const bool (&ok)[M], int r0, int r1, double& o0, double& o1) {
+    o0 = o1 = __builtin_nan("");
+#pragma unroll
+    for (int i = 0; i < M; ++i) {
+        int rank = 0;
+#pragma unroll
+        for (int j = 0; j < M; ++j) rank += (ok[j] && (v[j] < v[i] || (v[j] == v[i] && j < i))) ? 1 : 0;
+        if (ok[i] && rank == r0) o0 = v[i];
+        if (ok[i] && rank == r1) o1 = v[i];
+    }
+}
+
+// Move by d (+1 or -1) along axis ax from p through an ok edge, inside the grid.
+__device__ inline bool step_ok(const NodeGrid& g, const uint8_t* edge_ok, const int p[3], int ax, int d) {
+    if (d > 0) return p[ax] < g.dim(ax) - 1 && edge_ok[3 * g.lin(p) + ax];
+    if (p[ax] <= 0) return false;
+    int q[3] = {p[0], p[1], p[2]};
+    q[ax] -= 1;
+    return edge_ok[3 * g.lin(q) + ax] != 0;
+}
+
+// Neighbour at offset (dx, dy, dz) reached through ok edges, the axis steps taken in some order
+// (outlier_detection.py:41-70).
+__device__ inline bool reachable(const NodeGrid& g, const uint8_t* edge_ok, const int p[3], const int off[3]) {
+    constexpr int perms[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+    bool any = false;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        int q[3] = {p[0], p[1], p[2]};
+        bool ok = true;
+#pragma unroll
+        for (int t = 0; t < 3; ++t) {
+            const int ax = perms[k][t];
+            if (off[ax] != 0) {
+                ok = ok && step_ok(g, edge_ok, q, ax, off[ax]);
+                q[ax] += off[ax];
+            }
+        }
+        any = any || ok;
+    }
+    return any;
+}
+
+// The 26 neighbours of node p: uncut, indices clamped to the grid (mode "nearest"); cut, the
+// reachable ones (ok false for the others).
+template <bool CUT, class V>
+__device__ inline void gather26(const NodeGrid& g, const uint8_t* edge_ok, const int p[3], int c, const V& val,
+                                double (&v)[26], bool (&ok)[26]) {
+    int k = 0;
+#pragma unroll
+    for (int dz = -1; dz <= 1; ++dz)
+#pragma unroll
+        for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+            for (int dx = -1; dx <= 1; ++dx) {
+                if (dx == 0 && dy == 0 && dz == 0) continue;
+                const int off[3] = {dx, dy, dz};
+                int q[3];
+                bool in = true;
+                for (int ax = 0; ax < 3; ++ax) {
+                    q[ax] = p[ax] + off[ax];
+                    if (!CUT) q[ax] = q[ax] < 0 ? 0 : (q[ax] >= g.dim(ax) ? g.dim(ax) - 1 : q[ax]);
+                }
+                if (CUT) in = reachable(g, edge_ok, p, off);
+                double x = in ? val(g.lin(q), c) : __builtin_nan("");
+                v[k] = x;
+                ok[k] = in && !isnan(x);
+                ++k;
+            }
+}
+
+// median of the ok entries: uncut, rank 13 of all 26 (median_filter's rank for 26 values); cut,
+// numpy's nanmedian (mean of the two middle values for an even count; NaN when there is none)
+template <bool CUT>
+__device__ inline double median26(const double (&v)[26], const bool (&ok)[26], int& n) {
+    n = 0;
+#pragma unroll
+    for (int k = 0; k < 26; ++k) n += ok[k] ? 1 : 0;
+    double lo, hi;
+    if (!CUT) {
+        select2<26>(v, ok, 13, 13, lo, hi);
+        return lo;
+    }
+    if (n == 0) return __builtin_nan("");
+    select2<26>(v, ok, (n - 1) / 2, n / 2, lo, hi);
+    return (n & 1) ? lo : (lo + hi) / 2.0;
+}
+
+// pass 1: absf = |x - med(neighbours)| per column (cut: also the neighbour count)
+template <bool CUT>
+__global__ __launch_bounds__(ABLOCK) void k_median_pass1(NodeGrid g, const uint8_t* __restrict__ edge_ok,
+                                                         const uint8_t* __restrict__ unknown,
+                                                         const double* __restrict__ U, const double* __restrict__ X,
+                                                         double* __restrict__ absf, uint8_t* __restrict__ cnt,
+                                                         const MedCtl* ctl) {
+    if (ctl->skip) return;
+    // uncut: unknown nodes carry their spring-filled value X; cut: they are NaN
+    auto val = [&](int64_t n, int c) {
+        const bool unk = (unknown[n] >> c) & 1;
+        return unk ? (CUT ? __builtin_nan("") : X[3 * n + c]) : U[3 * n + c];
+    };
+    const int64_t N = g.n();
+    for (int64_t a = (int64_t)blockIdx.x * ABLOCK + threadIdx.x; a < N; a += (int64_t)gridDim.x * ABLOCK) {
+        int p[3];
+        g.pos(a, p);
+        for (int c = 0; c < 3; ++c) {
+            double v[26];
+            bool ok[26];
+            gather26<CUT>(g, edge_ok, p, c, val, v, ok);
+            int n;
+            const double med = median26<CUT>(v, ok, n);
+            absf[3 * a + c] = fabs(val(a, c) - med);
+            cnt[3 * a + c] = (uint8_t)n;
+        }
+    }
+}
+
+// pass 2: nf = absf / (med(neighbours' absf) + eps), the flag, and the cleaned field:
+// bad = !good | flag; U_out = bad ? U_hat : U
+template <bool CUT>
+__global__ __launch_bounds__(ABLOCK) void k_median_pass2(NodeGrid g, const uint8_t* __restrict__ edge_ok,
+                                                         const int8_t* __restrict__ status,
+                                                         const uint8_t* __restrict__ unknown,
+                                                         const double* __restrict__ U,
+                                                         const double* __restrict__ U_hat,
+                                                         const double* __restrict__ absf,
+                                                         const uint8_t* __restrict__ cnt, double threshold, double eps,
+                                                         double* __restrict__ U_out, uint8_t* __restrict__ bad_out,
+                                                         const MedCtl* ctl, const int* __restrict__ stop) {
+    if (gated(stop)) return;
+    const bool test = ctl->skip == 0;
+    auto af = [absf](int64_t n, int c) { return absf[3 * n + c]; };
+    const int64_t N = g.n();
+    for (int64_t a = (int64_t)blockIdx.x * ABLOCK + threadIdx.x; a < N; a += (int64_t)gridDim.x * ABLOCK) {
+        const bool good = status[a] == ST_CONVERGED;
+        bool flag = false;
+        if (test && good) {
+            int p[3];
+            g.pos(a, p);
+            double mag2 = 0.0;
+            for (int c = 0; c < 3; ++c) {
+                double nf = 0.0;
+                if (ctl->col_on[c] && !((unknown[a] >> c) & 1)) {
+                    double v[26];
+                    bool ok[26];
+                    gather26<CUT>(g, edge_ok, p, c, af, v, ok);
+                    int n;
+                    const double med_res = median26<CUT>(v, ok, n);
+                    nf = absf[3 * a + c] / (med_res + eps);
+                    if (CUT && (!isfinite(nf) || cnt[3 * a + c] < MIN_NEIGHBOURS)) nf = 0.0;
+                }
+                mag2 += nf * nf;
+            }
+            flag = sqrt(mag2) > threshold;
+        }
+        const bool bad = !good || flag;
+        for (int c = 0; c < 3; ++c) U_out[3 * a + c] = bad ? U_hat[3 * a + c] : U[3 * a + c];
+        bad_out[a] = bad ? 1 : 0;
+    }
+}
+
+// ---------------------------------------------------------------------------- dual update
+// hist[4 * step + k]: RMS of U2n - U2 (global update), U1n - U1 (local update), U2n - U1n and
+// F2n - F2 (primal residuals), then the state moves on: F2 = finite(F2n) ? F2n : F2, W, v, U2 = U2n,
+// U1 = U1n. With check_stop the stopping rule sets *stop and records the step in *steps.
+__global__ __launch_bounds__(ABLOCK) void k_dual_update(int64_t N, const double* U1n,
+                                                        const double* __restrict__ U2n,
+                                                        const double* __restrict__ F2n, double* U1,
+                                                        double* __restrict__ U2, double* __restrict__ F2,
+                                                        double* __restrict__ W, double* __restrict__ v,
+                                                        int accumulate, int check_stop, double admm_tol, int step,
+                                                        double* __restrict__ hist, double* __restrict__ part,
+                                                        unsigned* ticket, int* __restrict__ stop,
+                                                        int* __restrict__ steps) {
+    if (gated(stop)) return;
+    __shared__ double red[4][pcg::BLOCK];
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int64_t a = (int64_t)blockIdx.x * ABLOCK + threadIdx.x; a < N; a += (int64_t)gridDim.x * ABLOCK) {
+        for (int c = 0; c < 3; ++c) {
+            const int64_t i = 3 * a + c;
+            const double u1n = U1n[i], u2n = U2n[i];
+            const double dg = u2n - U2[i], dl = u1n - U1[i], ru = u2n - u1n;
+            s[0] += dg * dg;
+            s[1] += dl * dl;
+            s[2] += ru * ru;
+            v[i] = accumulate ? v[i] + ru : ru;
+            U1[i] = u1n;
+            U2[i] = u2n;
+        }
+        for (int k = 0; k < 9; ++k) {
+            const int64_t i = 9 * a + k;
+            const double f1 = F2[i], fr = F2n[i];
+            const double f2 = isfinite(fr) ? fr : f1;
+            const double df = f2 - f1;
+            s[3] += df * df;
+            W[i] = accumulate ? W[i] + df : df;
+            F2[i] = f2;
+        }
+    }
+    pcg::write_partials<4>(s, part, red);
+    if (pcg::last_block(ticket)) {
+        double t[4];
+        pcg::total_partials<4>(part, t, red);
+        if (threadIdx.x == 0) {
+            const double n3 = 3.0 * (double)N, n9 = 9.0 * (double)N;
+            const double r[4] = {sqrt(t[0] / n3), sqrt(t[1] / n3), sqrt(t[2] / n3), sqrt(t[3] / n9)};
+            for (int k = 0; k < 4; ++k) hist[4 * step + k] = r[k];
+            *steps = step;
+            if (check_stop && (r[0] < admm_tol || r[1] < admm_tol)) *stop = 1;
+            *ticket = 0u;
+        }
+    }
+}
+
+__global__ void k_zero_u32(unsigned* p) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0u;
+}
+
+__global__ void k_unknown_from_nan(int64_t N, const double* __restrict__ vals, uint8_t* __restrict__ unknown) {
+    for (int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; a < N; a += (int64_t)gridDim.x * blockDim.x) {
+        unsigned u = 0;
+        for (int c = 0; c < 3; ++c) u |= (isnan(vals[3 * a + c]) ? 1u : 0u) << c;
+        unknown[a] = (uint8_t)u;
+    }
+}
+
+__global__ void k_fill_merge(int64_t N, const uint8_t* __restrict__ unknown, const double* __restrict__ X,
+                             double* __restrict__ vals) {
+    for (int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; a < N; a += (int64_t)gridDim.x * blockDim.x)
+        for (int c = 0; c < 3; ++c)
+            if ((unknown[a] >> c) & 1) vals[3 * a + c] = X[3 * a + c];
+}
+
+// Carves consecutive 256-byte aligned regions out of a workspace; with a null base it only
+// measures, so the size query and the carving follow the same sequence.
+struct Carver {
+    char* base;
+    size_t used = 0;
+    template <class T>
+    T* take(size_t count) {
+        T* p = base ? reinterpret_cast<T*>(base + used) : nullptr;
+        used += (sizeof(T) * count + 255) / 256 * 256;
+        return p;
+    }
+};
+
+struct CleanupWs {
+    uint8_t *unknown, *cnt;
+    double *absf, *b, *X, *part;
+    MedCtl* ctl;
+    void* pcg_ws;
+    size_t bytes;
+    CleanupWs(void* ws, int64_t N) {
+        Carver c{static_cast<char*>(ws)};
+        unknown = c.take<uint8_t>(N);
+        cnt = c.take<uint8_t>(3 * N);
+        absf = c.take<double>(3 * N);
+        b = c.take<double>(3 * N);
+        X = c.take<double>(3 * N);
+        part = c.take<double>(4 * pcg::MAX_BLOCKS);
+        ctl = c.take<MedCtl>(1);
+        pcg_ws = c.take<char>(pcg::workspace_bytes(N));
+        bytes = c.used;
+    }
+};
+
+struct FillWs {
+    uint8_t* unknown;
+    double *b, *X;
+    void* pcg_ws;
+    size_t bytes;
+    FillWs(void* ws, int64_t N) {
+        Carver c{static_cast<char*>(ws)};
+        unknown = c.take<uint8_t>(N);
+        b = c.take<double>(3 * N);
+        X = c.take<double>(3 * N);
+        pcg_ws = c.take<char>(pcg::workspace_bytes(N));
+        bytes = c.used;
+    }
+};
+
+int check_node_grid(int nx, int ny, int nz) {
+    if (nx < 1 || ny < 1 || nz < 1) return fail(-2, "empty node grid");
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t aldvc_subpb1_cleanup_workspace_bytes(int nx, int ny, int nz) {
+    return CleanupWs(nullptr, (int64_t)nx * ny * nz).bytes;
+}
+
+int aldvc_subpb1_cleanup(int nx, int ny, int nz, const uint8_t* edge_ok, const int8_t* status, const double* U,
+                         const double* U_hat, double threshold, double eps, double fill_tol, int fill_max_iter,
+                         double* U_out, uint8_t* bad, const int* stop, void* workspace, size_t workspace_bytes,
+                         hipStream_t stream) {
+    clear_error();
+    if (int rc = check_node_grid(nx, ny, nz)) return rc;
+    if (!status || !U || !U_hat || !U_out || !bad || !workspace) return fail(-2, "null pointer");
+    if (fill_max_iter < 1) return fail(-2, "fill_max_iter must be at least 1");
+    if (workspace_bytes < aldvc_subpb1_cleanup_workspace_bytes(nx, ny, nz)) return fail(-3, "workspace too small");
+    const NodeGrid g{nx, ny, nz};
+    const int64_t N = (int64_t)nx * ny * nz;
+    const CleanupWs w(workspace, N);
+    uint8_t* unknown = w.unknown;
+    uint8_t* cnt = w.cnt;
+    double *absf = w.absf, *b = w.b, *X = w.X, *part = w.part;
+    MedCtl* ctl = w.ctl;
+    void* pcg_ws = w.pcg_ws;
+    const unsigned nb = pcg::blocks_for(N);
+    const SpringOp op{g, unknown, nullptr};  // the median test fills without cut edges (outlier_detection.py:127)
+
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, &ctl->ticket);
+    hipLaunchKernelGGL(k_cleanup_setup, dim3(nb), dim3(ABLOCK), 0, stream, g, status, U, threshold, unknown, part, ctl,
+                       stop);
+    if (!edge_ok) {
+        hipLaunchKernelGGL(k_spring_rhs, dim3(nb), dim3(ABLOCK), 0, stream, op, U, b, X, (const int*)&ctl->skip);
+        (void)pcg::solve(op, N, (const double*)b, X, fill_tol, fill_max_iter, pcg_ws, &ctl->skip, 0, nullptr, nullptr,
+                   stream);
+        hipLaunchKernelGGL(k_median_pass1<false>, dim3(nb), dim3(ABLOCK), 0, stream, g, edge_ok,
+                           (const uint8_t*)unknown, U, (const double*)X, absf, cnt, (const MedCtl*)ctl);
+        hipLaunchKernelGGL(k_median_pass2<false>, dim3(nb), dim3(ABLOCK), 0, stream, g, edge_ok, status,
+                           (const uint8_t*)unknown, U, U_hat, (const double*)absf, (const uint8_t*)cnt, threshold,
+                           eps, U_out, bad, (const MedCtl*)ctl, stop);
+    } else {
+        hipLaunchKernelGGL(k_median_pass1<true>, dim3(nb), dim3(ABLOCK), 0, stream, g, edge_ok,
+                           (const uint8_t*)unknown, U, (const double*)X, absf, cnt, (const MedCtl*)ctl);
+        hipLaunchKernelGGL(k_median_pass2<true>, dim3(nb), dim3(ABLOCK), 0, stream, g, edge_ok, status,
+                           (const uint8_t*)unknown, U, U_hat, (const double*)absf, (const uint8_t*)cnt, threshold,
+                           eps, U_out, bad, (const MedCtl*)ctl, stop);
+    }
+    return launch_status();
+}
+
+size_t aldvc_spring_fill_workspace_bytes(int nx, int ny, int nz) {
+    return FillWs(nullptr, (int64_t)nx * ny * nz).bytes;
+}
+
+int aldvc_spring_fill(int nx, int ny, int nz, const uint8_t* edge_ok, double* vals, double tol, int max_iter,
+                      int* iterations, void* workspace, size_t workspace_bytes, hipStream_t stream) {
+    clear_error();
+    if (int rc = check_node_grid(nx, ny, nz)) return rc;
+    if (!vals || !iterations || !workspace) return fail(-2, "null pointer");
+    if (max_iter < 1) return fail(-2, "max_iter must be at least 1");
+    if (workspace_bytes < aldvc_spring_fill_workspace_bytes(nx, ny, nz)) return fail(-3, "workspace too small");
+    const NodeGrid g{nx, ny, nz};
+    const int64_t N = (int64_t)nx * ny * nz;
+    const FillWs w(workspace, N);
+    uint8_t* unknown = w.unknown;
+    double *b = w.b, *X = w.X;
+    void* pcg_ws = w.pcg_ws;
+    const unsigned nb = pcg::blocks_for(N);
+    const SpringOp op{g, unknown, edge_ok};
+    hipLaunchKernelGGL(k_unknown_from_nan, dim3(nb), dim3(ABLOCK), 0, stream, N, (const double*)vals, unknown);
+    hipLaunchKernelGGL(k_spring_rhs, dim3(nb), dim3(ABLOCK), 0, stream, op, (const double*)vals, b, X,
+                       (const int*)nullptr);
+    *iterations = 0;
+    if (pcg::solve(op, N, (const double*)b, X, tol, max_iter, pcg_ws, nullptr, 8, iterations, nullptr, stream) !=
+        hipSuccess)
+        return fail(-5, "the spring-fill solve failed");
+    hipLaunchKernelGGL(k_fill_merge, dim3(nb), dim3(ABLOCK), 0, stream, N, (const uint8_t*)unknown, (const double*)X,
+                       vals);
+    return launch_status();
+}
+
+size_t aldvc_dual_update_workspace_bytes(int64_t N) {
+    (void)N;
+    return 256 + sizeof(double) * 4 * pcg::MAX_BLOCKS;
+}
+
+int aldvc_dual_update(int64_t N, const double* U1n, const double* U2n, const double* F2n, double* U1, double* U2,
+                      double* F2, double* W, double* v, int accumulate, int check_stop, double admm_tol, int step,
+                      double* history, int* stop, int* steps, void* workspace, size_t workspace_bytes,
+                      hipStream_t stream) {
+    clear_error();
+    if (N < 1) return fail(-2, "no nodes");
+    if (!U1n || !U2n || !F2n || !U1 || !U2 || !F2 || !W || !v || !history || !stop || !steps || !workspace)
+        return fail(-2, "null pointer");
+    if (step < 0) return fail(-2, "negative step");
+    if (workspace_bytes < aldvc_dual_update_workspace_bytes(N)) return fail(-3, "workspace too small");
+    unsigned* ticket = static_cast<unsigned*>(workspace);
+    double* part = reinterpret_cast<double*>(static_cast<char*>(workspace) + 256);
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, ticket);
+    hipLaunchKernelGGL(k_dual_update, dim3(pcg::blocks_for(N)), dim3(ABLOCK), 0, stream, N, U1n, U2n, F2n, U1, U2, F2,
+                       W, v, accumulate, check_stop, admm_tol, step, history, part, ticket, stop, steps);
+    return launch_status();
+}
+
+}  // extern "C"

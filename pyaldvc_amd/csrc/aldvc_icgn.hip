@@ -1439,11 +1439,23 @@ __global__ __launch_bounds__(1024) void k_compact_tiled(const uint8_t* __restric
     }
 }
 
+// A launch switched off by a device flag (a stopped ADMM loop under graph replay): no node runs.
+__global__ void k_gate_count(int64_t* __restrict__ count, const int* __restrict__ gate) {
+    if (threadIdx.x == 0 && blockIdx.x == 0 && gate && *gate) *count = 0;
+}
+
+__global__ void k_copy_gated(const double* __restrict__ src, double* __restrict__ dst, int64_t n,
+                             const int* __restrict__ gate) {
+    if (gate && *gate) return;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 // Status/iteration/ZNCC defaults of the reference batch wrappers (numba_kernels.py:832-840).
 __global__ void k_init_outputs(const uint8_t* __restrict__ valid, int64_t N, int32_t* n_iter, int8_t* status,
-                               double* zncc) {
+                               double* zncc, const int* __restrict__ gate) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
+    if (i >= N || (gate && *gate)) return;
     n_iter[i] = 0;
     status[i] = valid[i] ? ST_SKIPPED : ST_INVALID_SUBSET;
     zncc[i] = NAN;
@@ -1674,7 +1686,7 @@ int aldvc_icgn12(const int64_t* coords, int64_t N, double* P, int hx, int hy, in
     size_t list_bytes = (256 + sizeof(int64_t) * (size_t)N + 255) / 256 * 256;
     float* gbuf = (float*)((char*)workspace + list_bytes);
     hipLaunchKernelGGL(k_init_outputs, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, stream, valid, N, n_iter,
-                       status, zncc);
+                       status, zncc, (const int*)nullptr);
     launch_compact(variant != 0, valid, coords, N, hx, hy, hz, V, list, count, stream);
     IcgnArgs a{V, s, coords, P, L_all, H_all, pattern, meanf, split_index, split_keep, keep_bytes,
                noise_gain, tol, dp_tol, max_iter, patience, predictive, n_iter, status, zncc};
@@ -1693,14 +1705,14 @@ int aldvc_icgn12(const int64_t* coords, int64_t N, double* P, int hx, int hy, in
 
 size_t aldvc_icgn3_workspace_bytes(int64_t N) { return aldvc_icgn12_workspace_bytes(N, 0, 0, 0, 1, 1); }
 
-int aldvc_icgn3(const int64_t* coords, int64_t N, const double* U_old, double* U, const double* F_fixed,
-                const double* vdual, int hx, int hy, int hz, int stride, const float* f, const float* gx,
-                const float* gy, const float* gz, const uint8_t* mask, const float* g, int64_t nz, int64_t ny,
-                int64_t nx, int mode, const double* H_all, const double* meanf, const double* bottomf,
-                const uint8_t* valid, double mu, double tol, double dp_tol, int max_iter, int patience,
-                double noise_gain, int predictive, const int64_t* split_index, const uint8_t* split_keep,
-                int64_t keep_bytes, int32_t* n_iter, int8_t* status, double* zncc, void* workspace,
-                size_t workspace_bytes, hipStream_t stream) {
+int aldvc_icgn3_gated(const int64_t* coords, int64_t N, const double* U_old, double* U, const double* F_fixed,
+                      const double* vdual, int hx, int hy, int hz, int stride, const float* f, const float* gx,
+                      const float* gy, const float* gz, const uint8_t* mask, const float* g, int64_t nz, int64_t ny,
+                      int64_t nx, int mode, const double* H_all, const double* meanf, const double* bottomf,
+                      const uint8_t* valid, double mu, double tol, double dp_tol, int max_iter, int patience,
+                      double noise_gain, int predictive, const int64_t* split_index, const uint8_t* split_keep,
+                      int64_t keep_bytes, int32_t* n_iter, int8_t* status, double* zncc, const int* gate,
+                      void* workspace, size_t workspace_bytes, hipStream_t stream) {
     clear_error();
     Volumes V{f, gx, gy, gz, g, mask, (int)nz, (int)ny, (int)nx};
     const Subset s = make_subset(hx, hy, hz, stride);
@@ -1715,11 +1727,16 @@ int aldvc_icgn3(const int64_t* coords, int64_t N, const double* U_old, double* U
     if (!workspace || workspace_bytes < need) return fail(-3, "workspace too small (see aldvc_icgn3_workspace_bytes)");
     int64_t* count = (int64_t*)workspace;
     int64_t* list = (int64_t*)((char*)workspace + 256);
-    if (hipMemcpyAsync(U, U_old, sizeof(double) * 3 * (size_t)N, hipMemcpyDeviceToDevice, stream) != hipSuccess)
+    if (gate) {
+        hipLaunchKernelGGL(k_copy_gated, dim3((unsigned)std::min<int64_t>((3 * N + 255) / 256, 4096)), dim3(256), 0,
+                           stream, U_old, U, 3 * N, gate);
+    } else if (hipMemcpyAsync(U, U_old, sizeof(double) * 3 * (size_t)N, hipMemcpyDeviceToDevice, stream) != hipSuccess) {
         return fail(-4, "copy of U_old failed");
+    }
     hipLaunchKernelGGL(k_init_outputs, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, stream, valid, N, n_iter,
-                       status, zncc);
+                       status, zncc, gate);
     launch_compact(true, valid, coords, N, hx, hy, hz, V, list, count, stream);
+    if (gate) hipLaunchKernelGGL(k_gate_count, dim3(1), dim3(64), 0, stream, count, gate);
     Icgn3Args a{V, s, coords, U_old, U, F_fixed, vdual, H_all, bottomf, meanf, split_index, split_keep, keep_bytes,
                 mu, tol, dp_tol, noise_gain > 0.0 ? noise_gain : 0.0, max_iter, patience, predictive, n_iter, status,
                 zncc};
@@ -1734,6 +1751,20 @@ int aldvc_icgn3(const int64_t* coords, int64_t N, const double* U_old, double* U
     }
 #undef ALDVC_DISPATCH3
     return launch_status();
+}
+
+int aldvc_icgn3(const int64_t* coords, int64_t N, const double* U_old, double* U, const double* F_fixed,
+                const double* vdual, int hx, int hy, int hz, int stride, const float* f, const float* gx,
+                const float* gy, const float* gz, const uint8_t* mask, const float* g, int64_t nz, int64_t ny,
+                int64_t nx, int mode, const double* H_all, const double* meanf, const double* bottomf,
+                const uint8_t* valid, double mu, double tol, double dp_tol, int max_iter, int patience,
+                double noise_gain, int predictive, const int64_t* split_index, const uint8_t* split_keep,
+                int64_t keep_bytes, int32_t* n_iter, int8_t* status, double* zncc, void* workspace,
+                size_t workspace_bytes, hipStream_t stream) {
+    return aldvc_icgn3_gated(coords, N, U_old, U, F_fixed, vdual, hx, hy, hz, stride, f, gx, gy, gz, mask, g, nz, ny,
+                             nx, mode, H_all, meanf, bottomf, valid, mu, tol, dp_tol, max_iter, patience, noise_gain,
+                             predictive, split_index, split_keep, keep_bytes, n_iter, status, zncc, nullptr, workspace,
+                             workspace_bytes, stream);
 }
 
 int aldvc_evaluate_zncc(const int64_t* coords, int64_t N, const double* P_all, int hx, int hy, int hz, int stride,

@@ -1,0 +1,291 @@
+// Jacobi-preconditioned conjugate gradient for three right-hand sides, controlled on the device.
+//
+// The iteration of solver/subpb2_solver.py:77-105 (pcg_multi) and of the spring-model fill
+// (utils/inpaint.py:100-126): every scalar of the iteration (<r, z>, alpha, beta, |r|^2, |b|)
+// lives in a control block in device memory, and the stopping rule is evaluated by the device, so a
+// solve is a fixed sequence of kernel launches with no host read-back. A solve can therefore be
+// recorded into a HIP graph once and replayed; launches after convergence return at once.
+//
+// One iteration is two kernels:
+//   iter1: P = Z + beta P_old (formed on the fly for every stencil neighbour, written for the own
+//          node into the other P buffer), AP = A P, <P, AP>; alpha = <r, z> / <P, AP>
+//   iter2: X += alpha P, R -= alpha AP, Z = D^-1 R, <R, Z>, <R, R>; converged when
+//          |R_c| / |B_c| < tol for all three columns, else beta = <R, Z> / <r, z>_old
+// Grid-wide sums are per-block partials that the block finishing last adds up in block order (a
+// ticket counter in the control block), so the reduction order, and with it the iteration, does
+// not depend on scheduling.
+//
+// An operator supplies, per node, its three diagonal inverses and the stencil product:
+//   __device__ void inv_diag(int64_t a, double d[3]) const;
+//   template <class X> __device__ void apply(int64_t a, const X& x, double y[3]) const;
+// where x(b, c) is the value of column c at node b.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+namespace aldvc {
+namespace pcg {
+
+constexpr int BLOCK = 256;
+constexpr int MAX_BLOCKS = 2048;
+
+struct Ctl {
+    double rz[3], alpha[3], beta[3], rr[3], bnorm[3];
+    int done;        // 1: converged, or gated off; the remaining launches of the solve return at once
+    int iterations;  // iterations executed
+    unsigned ticket; // blocks of the running kernel that have finished (the last one resets it)
+    int pad;
+};
+
+inline unsigned blocks_for(int64_t n) {
+    const int64_t b = (n + BLOCK - 1) / BLOCK;
+    return (unsigned)(b < MAX_BLOCKS ? (b > 0 ? b : 1) : MAX_BLOCKS);
+}
+
+// Sum of K doubles per thread over the block, in a fixed tree order; the total is in red[k][0].
+template <int K>
+__device__ inline void block_sum(const double v[K], double (*red)[BLOCK]) {
+    for (int k = 0; k < K; ++k) red[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int off = BLOCK / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off)
+            for (int k = 0; k < K; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + off];
+        __syncthreads();
+    }
+}
+
+// True in the block that finishes last: this block's partials are released before its ticket is
+// taken, and the last block acquires everyone's before it reads them.
+__device__ inline bool last_block(unsigned* ticket) {
+    __shared__ int is_last;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) is_last = atomicAdd(ticket, 1u) == gridDim.x - 1 ? 1 : 0;
+    __syncthreads();
+    if (is_last) __threadfence();
+    return is_last != 0;
+}
+
+// Per-block partials of K sums: block `blk` writes part[K * blk + k].
+template <int K>
+__device__ inline void write_partials(const double v[K], double* part, double (*red)[BLOCK]) {
+    block_sum<K>(v, red);
+    if ((int)threadIdx.x < K) part[K * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// Totals of the partials of all blocks (called by the last block only), in block order.
+template <int K>
+__device__ inline void total_partials(const double* part, double out[K], double (*red)[BLOCK]) {
+    const volatile double* p = part;  // written by other workgroups during this launch
+    double s[K];
+    for (int k = 0; k < K; ++k) s[k] = 0.0;
+    for (int i = threadIdx.x; i < (int)gridDim.x; i += BLOCK)
+        for (int k = 0; k < K; ++k) s[k] += p[K * i + k];
+    block_sum<K>(s, red);
+    for (int k = 0; k < K; ++k) out[k] = red[k][0];
+}
+
+// (kernels that are not templates have internal linkage: every source including this has its own)
+static __global__ void k_reset(Ctl* ctl, const int* gate) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        ctl->ticket = 0u;
+        ctl->iterations = 0;
+        ctl->done = (gate && *gate) ? 1 : 0;
+        for (int c = 0; c < 3; ++c) {
+            ctl->beta[c] = 0.0;
+            ctl->alpha[c] = 0.0;
+        }
+    }
+}
+
+// R = B - A X, Z = D^-1 R, P_old = 0; <R, Z> and |B|
+template <class Op>
+__global__ __launch_bounds__(BLOCK) void k_setup(Op op, int64_t N, const double* __restrict__ B,
+                                                 const double* __restrict__ X, double* __restrict__ R,
+                                                 double* __restrict__ Z, double* __restrict__ Pold,
+                                                 double* __restrict__ inv_diag, double* __restrict__ part, Ctl* ctl) {
+    if (ctl->done) return;
+    __shared__ double red[6][BLOCK];
+    double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    auto xv = [X](int64_t b, int c) { return X[3 * b + c]; };
+    for (int64_t a = (int64_t)blockIdx.x * BLOCK + threadIdx.x; a < N; a += (int64_t)gridDim.x * BLOCK) {
+        double ax[3], d[3];
+        op.apply(a, xv, ax);
+        op.inv_diag(a, d);
+        for (int c = 0; c < 3; ++c) {
+            const double b = B[3 * a + c];
+            const double r = b - ax[c];
+            const double z = d[c] * r;
+            R[3 * a + c] = r;
+            Z[3 * a + c] = z;
+            Pold[3 * a + c] = 0.0;
+            inv_diag[3 * a + c] = d[c];
+            s[c] += r * z;
+            s[3 + c] += b * b;
+        }
+    }
+    write_partials<6>(s, part, red);
+    if (last_block(&ctl->ticket)) {
+        double t[6];
+        total_partials<6>(part, t, red);
+        if (threadIdx.x == 0) {
+            for (int c = 0; c < 3; ++c) {
+                ctl->rz[c] = t[c];
+                ctl->bnorm[c] = t[3 + c] > 0.0 ? sqrt(t[3 + c]) : 1.0;  // a zero column counts as 1
+            }
+            ctl->ticket = 0u;
+        }
+    }
+}
+
+// P = Z + beta P_old, AP = A P, alpha = <r, z> / <P, AP>
+template <class Op>
+__global__ __launch_bounds__(BLOCK) void k_iter1(Op op, int64_t N, const double* __restrict__ Z,
+                                                 const double* __restrict__ Pold, double* __restrict__ Pnew,
+                                                 double* __restrict__ AP, double* __restrict__ part, Ctl* ctl) {
+    if (ctl->done) return;
+    __shared__ double red[3][BLOCK];
+    const double be[3] = {ctl->beta[0], ctl->beta[1], ctl->beta[2]};
+    auto pv = [Z, Pold, &be](int64_t b, int c) { return Z[3 * b + c] + Pold[3 * b + c] * be[c]; };
+    double s[3] = {0.0, 0.0, 0.0};
+    for (int64_t a = (int64_t)blockIdx.x * BLOCK + threadIdx.x; a < N; a += (int64_t)gridDim.x * BLOCK) {
+        double ap[3];
+        op.apply(a, pv, ap);
+        for (int c = 0; c < 3; ++c) {
+            const double p = pv(a, c);
+            Pnew[3 * a + c] = p;
+            AP[3 * a + c] = ap[c];
+            s[c] += p * ap[c];
+        }
+    }
+    write_partials<3>(s, part, red);
+    if (last_block(&ctl->ticket)) {
+        double t[3];
+        total_partials<3>(part, t, red);
+        if (threadIdx.x == 0) {
+            for (int c = 0; c < 3; ++c) ctl->alpha[c] = t[c] == 0.0 ? 0.0 : ctl->rz[c] / t[c];
+            ctl->ticket = 0u;
+        }
+    }
+}
+
+// X += alpha P, R -= alpha AP, Z = D^-1 R; the stopping rule; beta and <r, z> of the next step
+static __global__ __launch_bounds__(BLOCK) void k_iter2(int64_t N, const double* __restrict__ P, const double* __restrict__ AP,
+                                                 const double* __restrict__ inv_diag, double* __restrict__ X,
+                                                 double* __restrict__ R, double* __restrict__ Z,
+                                                 double* __restrict__ part, Ctl* ctl, double tol) {
+    if (ctl->done) return;
+    __shared__ double red[6][BLOCK];
+    const double al[3] = {ctl->alpha[0], ctl->alpha[1], ctl->alpha[2]};
+    double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int64_t a = (int64_t)blockIdx.x * BLOCK + threadIdx.x; a < N; a += (int64_t)gridDim.x * BLOCK) {
+        for (int c = 0; c < 3; ++c) {
+            const int64_t i = 3 * a + c;
+            X[i] += P[i] * al[c];
+            const double r = R[i] - AP[i] * al[c];
+            R[i] = r;
+            const double z = inv_diag[i] * r;
+            Z[i] = z;
+            s[c] += r * z;
+            s[3 + c] += r * r;
+        }
+    }
+    write_partials<6>(s, part, red);
+    if (last_block(&ctl->ticket)) {
+        double t[6];
+        total_partials<6>(part, t, red);
+        if (threadIdx.x == 0) {
+            ctl->iterations += 1;
+            bool done = true;
+            for (int c = 0; c < 3; ++c) {
+                ctl->rr[c] = t[3 + c];
+                done = done && (sqrt(t[3 + c]) / ctl->bnorm[c] < tol);
+            }
+            if (done) {
+                ctl->done = 1;
+            } else {
+                for (int c = 0; c < 3; ++c) {
+                    const double rz = ctl->rz[c];
+                    ctl->beta[c] = rz == 0.0 ? 0.0 : t[c] / rz;
+                    ctl->rz[c] = t[c];
+                }
+            }
+            ctl->ticket = 0u;
+        }
+    }
+}
+
+static __global__ void k_report(const Ctl* ctl, int* iterations) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *iterations = ctl->iterations;
+}
+
+// Workspace of a solve over N nodes: R, Z, two P buffers, AP and D^-1 (3N doubles each), the
+// per-block partials and the control block.
+inline size_t workspace_bytes(int64_t N) {
+    return sizeof(double) * (6 * 3 * (size_t)N + 6 * (size_t)MAX_BLOCKS) + sizeof(Ctl) + 256;
+}
+
+struct Workspace {
+    double *R, *Z, *P[2], *AP, *inv_diag, *part;
+    Ctl* ctl;
+};
+
+inline Workspace carve(void* ws, int64_t N) {
+    Workspace w;
+    double* p = static_cast<double*>(ws);
+    const size_t n3 = 3 * (size_t)N;
+    w.R = p;
+    w.Z = p + n3;
+    w.P[0] = p + 2 * n3;
+    w.P[1] = p + 3 * n3;
+    w.AP = p + 4 * n3;
+    w.inv_diag = p + 5 * n3;
+    w.part = p + 6 * n3;
+    w.ctl = reinterpret_cast<Ctl*>(w.part + 6 * MAX_BLOCKS);
+    return w;
+}
+
+// Enqueue a whole solve of A X = B from the X given. With `sync_every` > 0 the host reads the
+// control block after every `sync_every` iterations and stops enqueueing once converged (the
+// iterations count is returned in *iterations_host); with 0 all `max_iter` iterations are
+// enqueued (those after convergence return at once) and nothing is read back, so the sequence can
+// be captured into a graph; *iterations_dev, when given, receives the count on the device.
+// `gate` (nullable, device) switches the whole solve off when nonzero.
+template <class Op>
+inline hipError_t solve(const Op& op, int64_t N, const double* B, double* X, double tol, int max_iter, void* ws,
+                        const int* gate, int sync_every, int* iterations_host, int* iterations_dev,
+                        hipStream_t st) {
+    Workspace w = carve(ws, N);
+    const unsigned nb = blocks_for(N);
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, w.ctl, gate);
+    hipLaunchKernelGGL(k_setup<Op>, dim3(nb), dim3(BLOCK), 0, st, op, N, B, (const double*)X, w.R, w.Z, w.P[1],
+                       w.inv_diag, w.part, w.ctl);
+    int it = 0;
+    while (it < max_iter) {
+        const int batch = sync_every > 0 ? (max_iter - it < sync_every ? max_iter - it : sync_every) : max_iter;
+        for (int k = 0; k < batch; ++k, ++it) {
+            double* Pold = w.P[(it + 1) & 1];
+            double* Pnew = w.P[it & 1];
+            hipLaunchKernelGGL(k_iter1<Op>, dim3(nb), dim3(BLOCK), 0, st, op, N, (const double*)w.Z,
+                               (const double*)Pold, Pnew, w.AP, w.part, w.ctl);
+            hipLaunchKernelGGL(k_iter2, dim3(nb), dim3(BLOCK), 0, st, N, (const double*)Pnew, (const double*)w.AP,
+                               (const double*)w.inv_diag, X, w.R, w.Z, w.part, w.ctl, tol);
+        }
+        if (sync_every > 0) {
+            Ctl c;
+            hipError_t e = hipMemcpyAsync(&c, w.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st);
+            if (e == hipSuccess) e = hipStreamSynchronize(st);
+            if (e != hipSuccess) return e;
+            if (iterations_host) *iterations_host = c.iterations;
+            if (c.done) break;
+        }
+    }
+    if (iterations_dev) hipLaunchKernelGGL(k_report, dim3(1), dim3(64), 0, st, (const Ctl*)w.ctl, iterations_dev);
+    return hipGetLastError();
+}
+
+}  // namespace pcg
+}  // namespace aldvc

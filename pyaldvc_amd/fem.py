@@ -71,6 +71,20 @@ def hex8_box_matrices(spacing, order: int = 2) -> dict:
     return {"K": Km, "M": Mm, "G": Gm, "volume": float(h.prod())}
 
 
+def hex8_elements(grid_shape, node_valid=None) -> np.ndarray:
+    """``(E, 8)`` corner nodes of every grid cell, cells in (z, y, x) order, corners in the hex8
+    order of the grid (mesh/grid_mesh.py:79-101); rows of -1 for cells with an invalid corner
+    (grid_mesh.py:217-219)."""
+    nz, ny, nx = (int(v) for v in grid_shape)
+    iz, iy, ix = (a.ravel() for a in np.meshgrid(np.arange(nz - 1), np.arange(ny - 1), np.arange(nx - 1), indexing="ij"))
+    corners = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)]
+    el = np.column_stack([(iz + dz) * ny * nx + (iy + dy) * nx + (ix + dx) for dx, dy, dz in corners]).astype(np.int64)
+    if node_valid is not None:
+        bad = ~np.asarray(node_valid, dtype=bool)[el].all(axis=1)
+        el[bad, :] = -1
+    return el
+
+
 @dataclass
 class GlobalOperators:
     """The FEM operators of a node grid: element matrices and activity on the GPU."""

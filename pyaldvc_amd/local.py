@@ -38,7 +38,7 @@ import numpy as np
 import torch
 
 from . import kernels as K
-from .cleanup import fill_nan_grid, universal_median_test
+from .cleanup import SPRING_MAX_UNKNOWN, fill_nan_grid, universal_median_test
 from .noise import STENCIL_NOISE_GAIN, noise_hessian_pattern
 
 logger = logging.getLogger(__name__)
@@ -328,23 +328,31 @@ def subpb1_solver(ctx: LocalContext, ref, g, U_hat, F_hat, vdual, mu, para, mesh
         float(para.icgn_tol), float(para.icgn_dp_tol), int(para.icgn_max_iter), int(para.icgn_patience), ctx.stride,
         float(pattern[9, 9]), gain, bool(para.icgn_predictive_stop), **ctx.split_args(),
     )  # fmt: skip
-    U = U.cpu().numpy()
+    grid = tuple(mesh.grid_shape)
+    if N <= SPRING_MAX_UNKNOWN:
+        # the cleanup on the device: median test (unknown nodes spring-filled, as the reference does
+        # up to this many nodes) and U_filled = bad ? U_hat : U
+        from .admm import subpb1_cleanup
+
+        U_filled, bad = subpb1_cleanup(U, torch.from_numpy(U_hat).to(dev), status, grid,
+                                       float(para.local_outlier_threshold), edge_ok=mesh.edges_ok_grid())  # fmt: skip
+        U_filled = U_filled.reshape(N, 3)
+    else:
+        Uh = U.cpu().numpy()
+        good = status.cpu().numpy() == STATUS_CONVERGED
+        bad = ~good
+        if para.local_outlier_threshold > 0 and good.sum() > 27:
+            flag = universal_median_test(Uh.reshape(grid + (3,)), good.reshape(grid), para.local_outlier_threshold,
+                                         edge_ok=mesh.edges_ok_grid())  # fmt: skip
+            bad |= flag.ravel()
+        U_filled = Uh.copy()
+        U_filled[bad] = U_hat[bad]  # nodes the local solver could not handle keep the global estimate
     n_iter = n_iter.cpu().numpy().astype(np.int32)
     status = status.cpu().numpy().astype(np.int8)
     zncc = zncc.cpu().numpy().astype(np.float64)
     solve_time = time.perf_counter() - t0
-    bad = status != STATUS_CONVERGED
-    if para.local_outlier_threshold > 0:
-        good = ~bad
-        if good.sum() > 27:
-            grid = tuple(mesh.grid_shape)
-            flag = universal_median_test(U.reshape(grid + (3,)), good.reshape(grid), para.local_outlier_threshold,
-                                         edge_ok=mesh.edges_ok_grid())  # fmt: skip
-            bad |= flag.ravel()
     n_bad = int(np.sum(bad & (status != STATUS_INVALID_SUBSET) & (status != STATUS_SKIPPED)))
     logger.info("Subpb1 (3-DOF): %d/%d converged, %d bad, %.2fs", int(np.sum(status == 0)), N, int(bad.sum()), solve_time)
-    U_filled = U.copy()
-    U_filled[bad] = U_hat[bad]  # nodes the local solver could not handle keep the global estimate
     return U_filled, LocalSolveInfo(n_iter=n_iter, status=status, zncc=zncc, solve_time=solve_time, n_bad=n_bad), bad
 
 
