@@ -79,7 +79,7 @@ def normalize_torch(v):
     return ((v.double() - m) / s).float().contiguous()
 
 
-def render_pair(dev, shape, n_beads, disp, seed=0):
+def render_pair(dev, shape, n_beads, disp, seed=0, sigma=SIGMA):
     """Bead pair on the GPU: f from the beads, g from the beads moved by ``disp``, noise seeds
     seed + 1 / seed + 2, both normalised by the HIP normalisation."""
     import torch
@@ -87,13 +87,13 @@ def render_pair(dev, shape, n_beads, disp, seed=0):
     from pyaldvc_amd import synthetic as syn
     from pyaldvc_amd import volumes as V
 
-    beads = syn.random_beads(shape, n_beads, SIGMA, seed=seed)
+    beads = syn.random_beads(shape, n_beads, sigma, seed=seed)
     moved = syn.moved(beads, disp)
     gen = torch.Generator(device=dev)
-    f = render_torch(shape, beads.centres, beads.amps, SIGMA, dev)
+    f = render_torch(shape, beads.centres, beads.amps, sigma, dev)
     gen.manual_seed(seed + 1)
     f += NOISE * torch.randn(shape, device=dev, generator=gen)
-    g = render_torch(shape, moved.centres, moved.amps, SIGMA, dev)
+    g = render_torch(shape, moved.centres, moved.amps, sigma, dev)
     gen.manual_seed(seed + 2)
     g += NOISE * torch.randn(shape, device=dev, generator=gen)
     return V.normalize_volume(f), V.normalize_volume(g)
@@ -232,9 +232,11 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run one device copy of this many MiB first (the scale of counter-based HBM traffic)")
-    ap.add_argument("--workload", default="icgn512", choices=["icgn512", "ncc-bf16", "warp-residual"],
+    ap.add_argument("--workload", default="icgn512", choices=["icgn512", "ncc-bf16", "warp-residual", "masked-admm"],
                     help="icgn512: the batched 12-DOF IC-GN at 512^3 (the headline line); ncc-bf16: the BF16 MFMA "
-                         "NCC search at 512^3; warp-residual: the differentiable warp residual at 256^3")  # fmt: skip
+                         "NCC search at 512^3; warp-residual: the differentiable warp residual at 256^3; "
+                         "masked-admm: masked anisotropic subsets at 1024x1024x306 with the ADMM iterations "
+                         "recorded into a HIP graph")  # fmt: skip
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the local step shards by node blocks with no exchange, "
@@ -244,6 +246,8 @@ def main():
         return main_ncc(args)
     if args.workload == "warp-residual":
         return main_warp(args)
+    if args.workload == "masked-admm":
+        return main_admm(args)
 
     import torch
 
@@ -558,6 +562,166 @@ def main_warp(args):
                    "subsets": S, "subsets_valid": valid, "forward_ms": t_fwd * 1e3, "wall_s_timed": wall},  # fmt: skip
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None},  # fmt: skip
+        "cpu_baseline": None,
+    }
+    print(json.dumps(result), flush=True)
+
+
+def cylinder_with_voids(shape, dev, n_voids=40, seed=0, radius_frac=0.47, axes=(2.5, 30.0)):
+    """uint8 mask: a cylinder along z (radius radius_frac * min(ny, nx)) minus n_voids axis-aligned
+    ellipsoids with semi-axes uniform in ``axes`` voxels, centres uniform in the volume."""
+    import torch
+
+    nz, ny, nx = shape
+    rng = np.random.default_rng(seed)
+    y = torch.arange(ny, device=dev, dtype=torch.float32)[:, None]
+    x = torch.arange(nx, device=dev, dtype=torch.float32)[None, :]
+    r = radius_frac * min(ny, nx)
+    disk = ((x - (nx - 1) / 2) ** 2 + (y - (ny - 1) / 2) ** 2) < r * r
+    m = disk[None, :, :].expand(nz, ny, nx).contiguous().to(torch.uint8)
+    for _ in range(n_voids):
+        c = rng.uniform(0, 1, 3) * np.array([nz, ny, nx])
+        a = rng.uniform(axes[0], axes[1], 3)
+        lo = [max(0, int(np.floor(c[k] - a[k]))) for k in range(3)]
+        hi = [min(shape[k], int(np.ceil(c[k] + a[k])) + 1) for k in range(3)]
+        if any(h <= l for l, h in zip(lo, hi)):
+            continue
+        zz = torch.arange(lo[0], hi[0], device=dev, dtype=torch.float32)[:, None, None]
+        yy = torch.arange(lo[1], hi[1], device=dev, dtype=torch.float32)[None, :, None]
+        xx = torch.arange(lo[2], hi[2], device=dev, dtype=torch.float32)[None, None, :]
+        inside = ((zz - c[0]) / a[0]) ** 2 + ((yy - c[1]) / a[1]) ** 2 + ((xx - c[2]) / a[2]) ** 2 <= 1.0
+        m[lo[0] : hi[0], lo[1] : hi[1], lo[2] : hi[2]][inside] = 0
+    return m
+
+
+def main_admm(args):
+    """Masked, anisotropic subsets with the ADMM iterations on the GPU: a 1024 x 1024 x 306 pair
+    (0.02 beads per voxel, sigma 1.8) under a Hertzian dimple u_z = -10.5 max(0, 1 - r^2/200^2)^1.5
+    (uniform in depth); mask = a cylinder minus 40 ellipsoidal voids (semi-axes 2.5-30 voxels);
+    subsets 48 x 48 x 24, step 12 (82 x 82 x 24 nodes). One step = the local 12-DOF solve of every
+    node from round(u_imposed) (split subsets, masked samples) plus the first global pass and 4
+    ADMM iterations (3-DOF solve, median-test cleanup, FP64 PCG global step, dual update), the
+    ADMM part recorded once into a HIP graph and replayed."""
+    import torch
+
+    from pyaldvc_amd import admm as AD
+    from pyaldvc_amd import fem as FEM
+    from pyaldvc_amd import kernels as K
+    from pyaldvc_amd import local as LOC
+    from pyaldvc_amd import synthetic as syn
+    from types import SimpleNamespace
+
+    dev = K.device()
+    shape = (306, 1024, 1024)
+    nz, ny, nx = shape
+    n_beads = int(round(0.02 * nz * ny * nx))
+    t0 = time.time()
+    disp = syn.hertz_dimple(10.5, 200.0, ((nx - 1) / 2, (ny - 1) / 2), nz - 1)
+    f, g = render_pair(dev, shape, n_beads, disp, seed=0, sigma=1.8)
+    mask = cylinder_with_voids(shape, dev)
+    torch.cuda.synchronize()
+    log(f"volumes rendered in {time.time() - t0:.1f}s ({n_beads:,} beads), mask {float(mask.float().mean()):.3f} inside")
+    win, step_ = (48, 48, 24), (12, 12, 12)
+    coords = syn.node_lattice(shape, win, step_)
+    gs = (len(np.unique(coords[:, 2])), len(np.unique(coords[:, 1])), len(np.unique(coords[:, 0])))
+    N = coords.shape[0]
+    cd = torch.from_numpy(coords).to(dev)
+    # apply_mask_to_mesh: centre inside and at least min_valid_ratio of the subset inside
+    frac = K.subset_valid_fraction(mask, cd, win)
+    frac = frac if isinstance(frac, torch.Tensor) else torch.from_numpy(np.asarray(frac)).to(dev)
+    centre_ok = mask[cd[:, 2], cd[:, 1], cd[:, 0]] > 0
+    node_valid = (centre_ok & (frac.to(dev) >= 0.5)).cpu().numpy()
+    u, v, w = disp(coords[:, 0].astype(float), coords[:, 1].astype(float), coords[:, 2].astype(float))
+    U_true = np.column_stack([u, v, w])
+    mesh = LOC.NodeMesh(coords.astype(np.float64), gs, node_valid.copy())
+    para = SimpleNamespace(winsize=win, subset_stride=1, min_valid_ratio=0.5, hessian_cond_max=1e12, subset_split=True,
+                           interp_method="cubic", icgn_tol=1e-2, icgn_dp_tol=1e-3, icgn_max_iter=100, icgn_patience=5,
+                           icgn_predictive_stop=True, icgn_noise_hessian=False, local_outlier_threshold=2.0,
+                           mu=1e-3, admm_max_iter=5, admm_tol=1e-12, dual_update="accumulate", global_solver="pcg",
+                           pcg_tol=1e-8, pcg_max_iter=500, alpha=0.0)  # fmt: skip
+    ref = SimpleNamespace(f=f, mask=mask, gradient_mode="stored")
+    t0 = time.time()
+    ctx = LOC.precompute_local_context(mesh, ref, para)
+    torch.cuda.synchronize()
+    mesh.node_valid = ctx.valid.copy()
+    n_valid = int(ctx.valid.sum())
+    log(f"precompute {time.time() - t0:.2f}s: {n_valid}/{N} valid nodes, split fraction {ctx.split_fraction}")
+    gm = torch.where(mask > 0, g, torch.full_like(g, float("nan")))  # the deformed frame's masked voxels
+    U0 = np.round(U_true)
+
+    def local():
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        out = LOC.local_icgn(ctx, ref, gm, U0, para, mesh)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t, out
+
+    local()  # warm-up
+    t_local, (U1, F1, info, bad) = local()
+    t_kernel = kernel_times(lambda: LOC.local_icgn(ctx, ref, gm, U0, para, mesh), 1)
+    n_iter = info.n_iter.astype(np.int64)
+    kept = ctx.n_valid.astype(np.float64)
+    bytes_alg = 20.0 * float(np.sum(n_iter * kept))
+    conv = info.status == 0
+    err_local = np.linalg.norm(U1[conv] - U_true[conv], axis=1)
+    log(f"local 12-DOF: {t_local:.2f}s ({n_valid / t_local:,.0f} nodes/s), kernel {t_kernel * 1e3:.1f} ms, "
+        f"{int(conv.sum())} converged, median |err| {np.median(err_local):.4f} vox")  # fmt: skip
+
+    spacing = tuple(float(s) for s in step_)
+    el = FEM.hex8_elements(gs, mesh.node_valid)
+    ops = FEM.build_global_operators(SimpleNamespace(grid_shape=gs, elements=el, spacing=spacing), "fem", 2)
+    beta = 1e-2 * float(np.mean(spacing)) ** 2 * para.mu  # one of the reference's beta sweep values
+    system = FEM.build_global_system(ops, beta, para.mu, para.alpha, para)
+    loop = AD.AdmmLoop(ctx, gm, mesh, system, para)
+    loop.load(U1, F1)
+    loop.run(graph=False)  # eager, also the warm-up
+    t_eager = loop.time_s
+    loop.load(U1, F1)
+    t0 = time.perf_counter()
+    loop.capture()
+    t_capture = time.perf_counter() - t0
+    times = []
+    for _ in range(max(1, args.steps)):
+        loop.load(U1, F1)
+        loop.run(graph=True)
+        times.append(loop.time_s)
+    t_graph = float(np.mean(times))
+    U, F, ainfo, status, zncc, bad_last = loop.results(info)
+    ok = (status == 0) & ~bad_last if bad_last is not None else status == 0
+    err = np.linalg.norm(U[ok] - U_true[ok], axis=1)
+    t_step = t_local + t_graph
+    log(f"ADMM (first pass + {ainfo.n_steps - 1} iterations): graph {t_graph * 1e3:.1f} ms, eager {t_eager * 1e3:.1f} ms, "
+        f"capture {t_capture:.2f}s; PCG iterations {ainfo.pcg_iterations}; median |err| {np.median(err):.4f} vox")  # fmt: skip
+    achieved = bytes_alg / t_kernel / 1e9
+    result = {
+        "metric": "masked ADMM frame nodes/s (1024x1024x306, 48x48x24 subsets: local 12-DOF + 5 ADMM passes)",
+        "value": n_valid / t_step,
+        "unit": "nodes/s",
+        "n_gpus": 1,
+        "steps": max(1, args.steps),
+        "warmup": 1,
+        "ms_per_step": t_step * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "1024x1024x306 f32 bead pair (0.02 beads/voxel sigma 1.8, noise 0.01), Hertzian dimple depth "
+                        "10.5 radius 200; cylinder mask minus 40 ellipsoidal voids; subsets 48x48x24 step 12; local "
+                        "12-DOF (split subsets) + first global pass + 4 ADMM iterations in one HIP graph; cubic",
+            "nodes": N, "nodes_valid": n_valid, "mask_fraction": float(mask.float().mean()),
+            "local_s": t_local, "local_kernel_ms": t_kernel * 1e3, "local_converged": int(conv.sum()),
+            "local_mean_iterations": float(n_iter[ctx.valid].mean()),
+            "local_median_err_vox": float(np.median(err_local)),
+            "admm_graph_ms": t_graph * 1e3, "admm_eager_ms": t_eager * 1e3, "admm_capture_s": t_capture,
+            "admm_steps": ainfo.n_steps, "pcg_iterations": list(ainfo.pcg_iterations),
+            "admm_update_global": list(ainfo.update_global), "admm_update_local": list(ainfo.update_local),
+            "final_median_err_vox": float(np.median(err)), "final_p95_err_vox": float(np.percentile(err, 95)),
+            "beta": beta,
+        },  # fmt: skip
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes_per_launch": bytes_alg},  # fmt: skip
         "cpu_baseline": None,
     }
     print(json.dumps(result), flush=True)

@@ -50,8 +50,6 @@ struct NodeGrid {
 struct MedCtl {
     int skip;       // 1: no median test (stopped, threshold <= 0, or at most 27 good nodes)
     int col_on[3];  // column c has at least one known node
-    unsigned ticket;
-    int pad[3];
 };
 
 __device__ inline bool gated(const int* stop) { return stop && *stop; }
@@ -100,17 +98,14 @@ struct SpringOp {
     }
 };
 
-// bad = status != CONVERGED; unknown bits (not good, or NaN); the median test's switches
+// bad = status != CONVERGED; unknown bits (not good, or NaN); per-block counts of good nodes and
+// of known nodes per column
 __global__ __launch_bounds__(ABLOCK) void k_cleanup_setup(NodeGrid g, const int8_t* __restrict__ status,
-                                                          const double* __restrict__ U, double threshold,
-                                                          uint8_t* __restrict__ unknown, double* __restrict__ part,
-                                                          MedCtl* ctl, const int* __restrict__ stop) {
-    if (gated(stop)) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) ctl->skip = 1;
-        return;
-    }
+                                                          const double* __restrict__ U, uint8_t* __restrict__ unknown,
+                                                          double* __restrict__ part, const int* __restrict__ stop) {
+    if (gated(stop)) return;
     __shared__ double red[4][pcg::BLOCK];
-    double s[4] = {0.0, 0.0, 0.0, 0.0};  // good nodes, known nodes per column
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
     const int64_t N = g.n();
     for (int64_t a = (int64_t)blockIdx.x * ABLOCK + threadIdx.x; a < N; a += (int64_t)gridDim.x * ABLOCK) {
         const bool good = status[a] == ST_CONVERGED;
@@ -124,14 +119,22 @@ __global__ __launch_bounds__(ABLOCK) void k_cleanup_setup(NodeGrid g, const int8
         s[0] += good ? 1.0 : 0.0;
     }
     pcg::write_partials<4>(s, part, red);
-    if (pcg::last_block(&ctl->ticket)) {
-        double t[4];
-        pcg::total_partials<4>(part, t, red);
-        if (threadIdx.x == 0) {
-            ctl->skip = (threshold > 0.0 && t[0] > 27.0) ? 0 : 1;
-            for (int c = 0; c < 3; ++c) ctl->col_on[c] = t[1 + c] > 0.0 ? 1 : 0;
-            ctl->ticket = 0u;
-        }
+}
+
+// the median test's switches from the counts
+__global__ __launch_bounds__(ABLOCK) void k_cleanup_switches(const double* __restrict__ part, int nparts,
+                                                             double threshold, MedCtl* ctl,
+                                                             const int* __restrict__ stop) {
+    __shared__ double red[4][pcg::BLOCK];
+    if (gated(stop)) {
+        if (threadIdx.x == 0) ctl->skip = 1;
+        return;
+    }
+    double t[4];
+    pcg::total_partials<4>(part, nparts, t, red);
+    if (threadIdx.x == 0) {
+        ctl->skip = (threshold > 0.0 && t[0] > 27.0) ? 0 : 1;
+        for (int c = 0; c < 3; ++c) ctl->col_on[c] = t[1 + c] > 0.0 ? 1 : 0;
     }
 }
 
@@ -330,10 +333,8 @@ __global__ __launch_bounds__(ABLOCK) void k_dual_update(int64_t N, const double*
                                                         const double* __restrict__ F2n, double* U1,
                                                         double* __restrict__ U2, double* __restrict__ F2,
                                                         double* __restrict__ W, double* __restrict__ v,
-                                                        int accumulate, int check_stop, double admm_tol, int step,
-                                                        double* __restrict__ hist, double* __restrict__ part,
-                                                        unsigned* ticket, int* __restrict__ stop,
-                                                        int* __restrict__ steps) {
+                                                        int accumulate, double* __restrict__ part,
+                                                        const int* __restrict__ stop) {
     if (gated(stop)) return;
     __shared__ double red[4][pcg::BLOCK];
     double s[4] = {0.0, 0.0, 0.0, 0.0};
@@ -360,22 +361,24 @@ __global__ __launch_bounds__(ABLOCK) void k_dual_update(int64_t N, const double*
         }
     }
     pcg::write_partials<4>(s, part, red);
-    if (pcg::last_block(ticket)) {
-        double t[4];
-        pcg::total_partials<4>(part, t, red);
-        if (threadIdx.x == 0) {
-            const double n3 = 3.0 * (double)N, n9 = 9.0 * (double)N;
-            const double r[4] = {sqrt(t[0] / n3), sqrt(t[1] / n3), sqrt(t[2] / n3), sqrt(t[3] / n9)};
-            for (int k = 0; k < 4; ++k) hist[4 * step + k] = r[k];
-            *steps = step;
-            if (check_stop && (r[0] < admm_tol || r[1] < admm_tol)) *stop = 1;
-            *ticket = 0u;
-        }
-    }
 }
 
-__global__ void k_zero_u32(unsigned* p) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0u;
+// the RMS values of the step from the partials, and the stopping rule
+__global__ __launch_bounds__(ABLOCK) void k_dual_final(const double* __restrict__ part, int nparts, int64_t N,
+                                                       int check_stop, double admm_tol, int step,
+                                                       double* __restrict__ hist, int* __restrict__ stop,
+                                                       int* __restrict__ steps) {
+    if (gated(stop)) return;
+    __shared__ double red[4][pcg::BLOCK];
+    double t[4];
+    pcg::total_partials<4>(part, nparts, t, red);
+    if (threadIdx.x == 0) {
+        const double n3 = 3.0 * (double)N, n9 = 9.0 * (double)N;
+        const double r[4] = {sqrt(t[0] / n3), sqrt(t[1] / n3), sqrt(t[2] / n3), sqrt(t[3] / n9)};
+        for (int k = 0; k < 4; ++k) hist[4 * step + k] = r[k];
+        *steps = step;
+        if (check_stop && (r[0] < admm_tol || r[1] < admm_tol)) *stop = 1;
+    }
 }
 
 __global__ void k_unknown_from_nan(int64_t N, const double* __restrict__ vals, uint8_t* __restrict__ unknown) {
@@ -474,9 +477,9 @@ int aldvc_subpb1_cleanup(int nx, int ny, int nz, const uint8_t* edge_ok, const i
     const unsigned nb = pcg::blocks_for(N);
     const SpringOp op{g, unknown, nullptr};  // the median test fills without cut edges (outlier_detection.py:127)
 
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, &ctl->ticket);
-    hipLaunchKernelGGL(k_cleanup_setup, dim3(nb), dim3(ABLOCK), 0, stream, g, status, U, threshold, unknown, part, ctl,
-                       stop);
+    hipLaunchKernelGGL(k_cleanup_setup, dim3(nb), dim3(ABLOCK), 0, stream, g, status, U, unknown, part, stop);
+    hipLaunchKernelGGL(k_cleanup_switches, dim3(1), dim3(ABLOCK), 0, stream, (const double*)part, (int)nb, threshold,
+                       ctl, stop);
     if (!edge_ok) {
         hipLaunchKernelGGL(k_spring_rhs, dim3(nb), dim3(ABLOCK), 0, stream, op, U, b, X, (const int*)&ctl->skip);
         (void)pcg::solve(op, N, (const double*)b, X, fill_tol, fill_max_iter, pcg_ws, &ctl->skip, 0, nullptr, nullptr,
@@ -529,7 +532,7 @@ int aldvc_spring_fill(int nx, int ny, int nz, const uint8_t* edge_ok, double* va
 
 size_t aldvc_dual_update_workspace_bytes(int64_t N) {
     (void)N;
-    return 256 + sizeof(double) * 4 * pcg::MAX_BLOCKS;
+    return sizeof(double) * 4 * pcg::MAX_BLOCKS;
 }
 
 int aldvc_dual_update(int64_t N, const double* U1n, const double* U2n, const double* F2n, double* U1, double* U2,
@@ -542,11 +545,12 @@ int aldvc_dual_update(int64_t N, const double* U1n, const double* U2n, const dou
         return fail(-2, "null pointer");
     if (step < 0) return fail(-2, "negative step");
     if (workspace_bytes < aldvc_dual_update_workspace_bytes(N)) return fail(-3, "workspace too small");
-    unsigned* ticket = static_cast<unsigned*>(workspace);
-    double* part = reinterpret_cast<double*>(static_cast<char*>(workspace) + 256);
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, ticket);
-    hipLaunchKernelGGL(k_dual_update, dim3(pcg::blocks_for(N)), dim3(ABLOCK), 0, stream, N, U1n, U2n, F2n, U1, U2, F2,
-                       W, v, accumulate, check_stop, admm_tol, step, history, part, ticket, stop, steps);
+    double* part = static_cast<double*>(workspace);
+    const unsigned nb = pcg::blocks_for(N);
+    hipLaunchKernelGGL(k_dual_update, dim3(nb), dim3(ABLOCK), 0, stream, N, U1n, U2n, F2n, U1, U2, F2, W, v, accumulate,
+                       part, (const int*)stop);
+    hipLaunchKernelGGL(k_dual_final, dim3(1), dim3(ABLOCK), 0, stream, (const double*)part, (int)nb, N, check_stop,
+                       admm_tol, step, history, stop, steps);
     return launch_status();
 }
 

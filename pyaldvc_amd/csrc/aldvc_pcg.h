@@ -6,14 +6,18 @@
 // solve is a fixed sequence of kernel launches with no host read-back. A solve can therefore be
 // recorded into a HIP graph once and replayed; launches after convergence return at once.
 //
-// One iteration is two kernels:
-//   iter1: P = Z + beta P_old (formed on the fly for every stencil neighbour, written for the own
-//          node into the other P buffer), AP = A P, <P, AP>; alpha = <r, z> / <P, AP>
-//   iter2: X += alpha P, R -= alpha AP, Z = D^-1 R, <R, Z>, <R, R>; converged when
-//          |R_c| / |B_c| < tol for all three columns, else beta = <R, Z> / <r, z>_old
-// Grid-wide sums are per-block partials that the block finishing last adds up in block order (a
-// ticket counter in the control block), so the reduction order, and with it the iteration, does
-// not depend on scheduling.
+// One iteration is two kernels, and no kernel waits on another workgroup: each writes per-block
+// partial sums, and every block of the NEXT kernel adds all of them up in block order (a few KB
+// read from L2), so every block derives the same scalars with no fence, atomic or extra launch:
+//   iter1(k): from iter2(k-1)'s partials: |R|^2 -> converged? (then stop), <R, Z> -> beta;
+//             P = Z + beta P_old (formed on the fly for every stencil neighbour, written for the own
+//             node into the other P buffer), AP = A P, partials of <P, AP>
+//   iter2(k): alpha = <r, z> / <P, AP> from iter1(k)'s partials; X += alpha P, R -= alpha AP,
+//             Z = D^-1 R, partials of <R, Z> and <R, R>
+// Block 0 keeps the control block (<r, z> by iteration parity, |B|, the stop flag and the
+// iteration count) for the later kernels and the host. The convergence test of an iteration is
+// made by the following iter1 (or by the closing kernel after the last slot), so the iteration
+// count and the iterate equal those of the textbook loop that tests after every update.
 //
 // An operator supplies, per node, its three diagonal inverses and the stencil product:
 //   __device__ void inv_diag(int64_t a, double d[3]) const;
@@ -30,14 +34,13 @@ namespace aldvc {
 namespace pcg {
 
 constexpr int BLOCK = 256;
-constexpr int MAX_BLOCKS = 2048;
+constexpr int MAX_BLOCKS = 512;  // partials a block of the next kernel adds up
 
 struct Ctl {
-    double rz[3], alpha[3], beta[3], rr[3], bnorm[3];
-    int done;        // 1: converged, or gated off; the remaining launches of the solve return at once
-    int iterations;  // iterations executed
-    unsigned ticket; // blocks of the running kernel that have finished (the last one resets it)
-    int pad;
+    double rz[2][3];  // <r, z> of the last two iterations, by parity
+    double bnorm[3];  // |B| per column (1 for a zero column)
+    int done;         // 1: converged, max_iter reached, or gated off; later launches return at once
+    int iterations;   // iterations executed (valid once done)
 };
 
 inline unsigned blocks_for(int64_t n) {
@@ -57,18 +60,6 @@ __device__ inline void block_sum(const double v[K], double (*red)[BLOCK]) {
     }
 }
 
-// True in the block that finishes last: this block's partials are released before its ticket is
-// taken, and the last block acquires everyone's before it reads them.
-__device__ inline bool last_block(unsigned* ticket) {
-    __shared__ int is_last;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) is_last = atomicAdd(ticket, 1u) == gridDim.x - 1 ? 1 : 0;
-    __syncthreads();
-    if (is_last) __threadfence();
-    return is_last != 0;
-}
-
 // Per-block partials of K sums: block `blk` writes part[K * blk + k].
 template <int K>
 __device__ inline void write_partials(const double v[K], double* part, double (*red)[BLOCK]) {
@@ -76,37 +67,34 @@ __device__ inline void write_partials(const double v[K], double* part, double (*
     if ((int)threadIdx.x < K) part[K * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
 }
 
-// Totals of the partials of all blocks (called by the last block only), in block order.
+// Totals of nparts partials written by an earlier kernel, in block order (every caller gets the
+// same bits).
 template <int K>
-__device__ inline void total_partials(const double* part, double out[K], double (*red)[BLOCK]) {
-    const volatile double* p = part;  // written by other workgroups during this launch
+__device__ inline void total_partials(const double* part, int nparts, double out[K], double (*red)[BLOCK]) {
     double s[K];
     for (int k = 0; k < K; ++k) s[k] = 0.0;
-    for (int i = threadIdx.x; i < (int)gridDim.x; i += BLOCK)
-        for (int k = 0; k < K; ++k) s[k] += p[K * i + k];
+    for (int i = threadIdx.x; i < nparts; i += BLOCK)
+        for (int k = 0; k < K; ++k) s[k] += part[K * i + k];
     block_sum<K>(s, red);
     for (int k = 0; k < K; ++k) out[k] = red[k][0];
+    __syncthreads();  // red is reused by the caller
 }
 
 // (kernels that are not templates have internal linkage: every source including this has its own)
 static __global__ void k_reset(Ctl* ctl, const int* gate) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
-        ctl->ticket = 0u;
         ctl->iterations = 0;
         ctl->done = (gate && *gate) ? 1 : 0;
-        for (int c = 0; c < 3; ++c) {
-            ctl->beta[c] = 0.0;
-            ctl->alpha[c] = 0.0;
-        }
     }
 }
 
-// R = B - A X, Z = D^-1 R, P_old = 0; <R, Z> and |B|
+// R = B - A X, Z = D^-1 R, P_old = 0; partials of <R, Z> and <B, B>
 template <class Op>
 __global__ __launch_bounds__(BLOCK) void k_setup(Op op, int64_t N, const double* __restrict__ B,
                                                  const double* __restrict__ X, double* __restrict__ R,
                                                  double* __restrict__ Z, double* __restrict__ Pold,
-                                                 double* __restrict__ inv_diag, double* __restrict__ part, Ctl* ctl) {
+                                                 double* __restrict__ inv_diag, double* __restrict__ part2,
+                                                 const Ctl* ctl) {
     if (ctl->done) return;
     __shared__ double red[6][BLOCK];
     double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
@@ -127,28 +115,48 @@ __global__ __launch_bounds__(BLOCK) void k_setup(Op op, int64_t N, const double*
             s[3 + c] += b * b;
         }
     }
-    write_partials<6>(s, part, red);
-    if (last_block(&ctl->ticket)) {
-        double t[6];
-        total_partials<6>(part, t, red);
-        if (threadIdx.x == 0) {
-            for (int c = 0; c < 3; ++c) {
-                ctl->rz[c] = t[c];
-                ctl->bnorm[c] = t[3 + c] > 0.0 ? sqrt(t[3 + c]) : 1.0;  // a zero column counts as 1
-            }
-            ctl->ticket = 0u;
-        }
-    }
+    write_partials<6>(s, part2, red);
 }
 
-// P = Z + beta P_old, AP = A P, alpha = <r, z> / <P, AP>
+// Convergence of iteration k (k >= 1) from iter2(k-1)'s partials of <R, R>.
+__device__ inline bool converged(const double t[6], const Ctl* ctl, double tol) {
+    bool done = true;
+    for (int c = 0; c < 3; ++c) done = done && (sqrt(t[3 + c]) / ctl->bnorm[c] < tol);
+    return done;
+}
+
+// iteration slot k: the stopping test of iteration k, then P = Z + beta P_old, AP = A P
 template <class Op>
-__global__ __launch_bounds__(BLOCK) void k_iter1(Op op, int64_t N, const double* __restrict__ Z,
+__global__ __launch_bounds__(BLOCK) void k_iter1(Op op, int64_t N, int k, const double* __restrict__ Z,
                                                  const double* __restrict__ Pold, double* __restrict__ Pnew,
-                                                 double* __restrict__ AP, double* __restrict__ part, Ctl* ctl) {
+                                                 double* __restrict__ AP, const double* __restrict__ part2,
+                                                 double* __restrict__ part1, Ctl* ctl, double tol) {
     if (ctl->done) return;
-    __shared__ double red[3][BLOCK];
-    const double be[3] = {ctl->beta[0], ctl->beta[1], ctl->beta[2]};
+    __shared__ double red[6][BLOCK];
+    double t[6];
+    total_partials<6>(part2, gridDim.x, t, red);
+    double be[3] = {0.0, 0.0, 0.0};
+    if (k == 0) {  // t = <R, Z>, <B, B> of the setup
+        if (blockIdx.x == 0 && threadIdx.x == 0)
+            for (int c = 0; c < 3; ++c) {
+                ctl->rz[0][c] = t[c];
+                ctl->bnorm[c] = t[3 + c] > 0.0 ? sqrt(t[3 + c]) : 1.0;  // a zero column counts as 1
+            }
+    } else {  // t = <R, Z>, <R, R> after iteration k
+        if (converged(t, ctl, tol)) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) {
+                ctl->iterations = k;
+                ctl->done = 1;
+            }
+            return;
+        }
+        for (int c = 0; c < 3; ++c) {
+            const double rz = ctl->rz[(k - 1) & 1][c];
+            be[c] = rz == 0.0 ? 0.0 : t[c] / rz;
+        }
+        if (blockIdx.x == 0 && threadIdx.x == 0)
+            for (int c = 0; c < 3; ++c) ctl->rz[k & 1][c] = t[c];
+    }
     auto pv = [Z, Pold, &be](int64_t b, int c) { return Z[3 * b + c] + Pold[3 * b + c] * be[c]; };
     double s[3] = {0.0, 0.0, 0.0};
     for (int64_t a = (int64_t)blockIdx.x * BLOCK + threadIdx.x; a < N; a += (int64_t)gridDim.x * BLOCK) {
@@ -161,25 +169,22 @@ __global__ __launch_bounds__(BLOCK) void k_iter1(Op op, int64_t N, const double*
             s[c] += p * ap[c];
         }
     }
-    write_partials<3>(s, part, red);
-    if (last_block(&ctl->ticket)) {
-        double t[3];
-        total_partials<3>(part, t, red);
-        if (threadIdx.x == 0) {
-            for (int c = 0; c < 3; ++c) ctl->alpha[c] = t[c] == 0.0 ? 0.0 : ctl->rz[c] / t[c];
-            ctl->ticket = 0u;
-        }
-    }
+    write_partials<3>(s, part1, red);
 }
 
-// X += alpha P, R -= alpha AP, Z = D^-1 R; the stopping rule; beta and <r, z> of the next step
-static __global__ __launch_bounds__(BLOCK) void k_iter2(int64_t N, const double* __restrict__ P, const double* __restrict__ AP,
-                                                 const double* __restrict__ inv_diag, double* __restrict__ X,
-                                                 double* __restrict__ R, double* __restrict__ Z,
-                                                 double* __restrict__ part, Ctl* ctl, double tol) {
+// iteration slot k: alpha = <r, z> / <P, AP>; X += alpha P, R -= alpha AP, Z = D^-1 R
+static __global__ __launch_bounds__(BLOCK) void k_iter2(int64_t N, int k, const double* __restrict__ P,
+                                                        const double* __restrict__ AP,
+                                                        const double* __restrict__ inv_diag, double* __restrict__ X,
+                                                        double* __restrict__ R, double* __restrict__ Z,
+                                                        const double* __restrict__ part1, double* __restrict__ part2,
+                                                        const Ctl* ctl) {
     if (ctl->done) return;
     __shared__ double red[6][BLOCK];
-    const double al[3] = {ctl->alpha[0], ctl->alpha[1], ctl->alpha[2]};
+    double t[3];
+    total_partials<3>(part1, gridDim.x, t, red);
+    double al[3];
+    for (int c = 0; c < 3; ++c) al[c] = t[c] == 0.0 ? 0.0 : ctl->rz[k & 1][c] / t[c];
     double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     for (int64_t a = (int64_t)blockIdx.x * BLOCK + threadIdx.x; a < N; a += (int64_t)gridDim.x * BLOCK) {
         for (int c = 0; c < 3; ++c) {
@@ -193,28 +198,14 @@ static __global__ __launch_bounds__(BLOCK) void k_iter2(int64_t N, const double*
             s[3 + c] += r * r;
         }
     }
-    write_partials<6>(s, part, red);
-    if (last_block(&ctl->ticket)) {
-        double t[6];
-        total_partials<6>(part, t, red);
-        if (threadIdx.x == 0) {
-            ctl->iterations += 1;
-            bool done = true;
-            for (int c = 0; c < 3; ++c) {
-                ctl->rr[c] = t[3 + c];
-                done = done && (sqrt(t[3 + c]) / ctl->bnorm[c] < tol);
-            }
-            if (done) {
-                ctl->done = 1;
-            } else {
-                for (int c = 0; c < 3; ++c) {
-                    const double rz = ctl->rz[c];
-                    ctl->beta[c] = rz == 0.0 ? 0.0 : t[c] / rz;
-                    ctl->rz[c] = t[c];
-                }
-            }
-            ctl->ticket = 0u;
-        }
+    write_partials<6>(s, part2, red);
+}
+
+// after the last slot: the iteration limit was reached (converged or not)
+static __global__ void k_close(int max_iter, Ctl* ctl) {
+    if (threadIdx.x == 0 && blockIdx.x == 0 && !ctl->done) {
+        ctl->iterations = max_iter;
+        ctl->done = 1;
     }
 }
 
@@ -225,11 +216,11 @@ static __global__ void k_report(const Ctl* ctl, int* iterations) {
 // Workspace of a solve over N nodes: R, Z, two P buffers, AP and D^-1 (3N doubles each), the
 // per-block partials and the control block.
 inline size_t workspace_bytes(int64_t N) {
-    return sizeof(double) * (6 * 3 * (size_t)N + 6 * (size_t)MAX_BLOCKS) + sizeof(Ctl) + 256;
+    return sizeof(double) * (6 * 3 * (size_t)N + 9 * (size_t)MAX_BLOCKS) + sizeof(Ctl) + 256;
 }
 
 struct Workspace {
-    double *R, *Z, *P[2], *AP, *inv_diag, *part;
+    double *R, *Z, *P[2], *AP, *inv_diag, *part1, *part2;
     Ctl* ctl;
 };
 
@@ -243,14 +234,15 @@ inline Workspace carve(void* ws, int64_t N) {
     w.P[1] = p + 3 * n3;
     w.AP = p + 4 * n3;
     w.inv_diag = p + 5 * n3;
-    w.part = p + 6 * n3;
-    w.ctl = reinterpret_cast<Ctl*>(w.part + 6 * MAX_BLOCKS);
+    w.part1 = p + 6 * n3;
+    w.part2 = w.part1 + 3 * MAX_BLOCKS;
+    w.ctl = reinterpret_cast<Ctl*>(w.part2 + 6 * MAX_BLOCKS);
     return w;
 }
 
 // Enqueue a whole solve of A X = B from the X given. With `sync_every` > 0 the host reads the
-// control block after every `sync_every` iterations and stops enqueueing once converged (the
-// iterations count is returned in *iterations_host); with 0 all `max_iter` iterations are
+// control block after every `sync_every` iterations and stops enqueueing once it is done (the
+// iteration count is returned in *iterations_host); with 0 all `max_iter` iteration slots are
 // enqueued (those after convergence return at once) and nothing is read back, so the sequence can
 // be captured into a graph; *iterations_dev, when given, receives the count on the device.
 // `gate` (nullable, device) switches the whole solve off when nonzero.
@@ -262,28 +254,31 @@ inline hipError_t solve(const Op& op, int64_t N, const double* B, double* X, dou
     const unsigned nb = blocks_for(N);
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, w.ctl, gate);
     hipLaunchKernelGGL(k_setup<Op>, dim3(nb), dim3(BLOCK), 0, st, op, N, B, (const double*)X, w.R, w.Z, w.P[1],
-                       w.inv_diag, w.part, w.ctl);
-    int it = 0;
-    while (it < max_iter) {
-        const int batch = sync_every > 0 ? (max_iter - it < sync_every ? max_iter - it : sync_every) : max_iter;
-        for (int k = 0; k < batch; ++k, ++it) {
-            double* Pold = w.P[(it + 1) & 1];
-            double* Pnew = w.P[it & 1];
-            hipLaunchKernelGGL(k_iter1<Op>, dim3(nb), dim3(BLOCK), 0, st, op, N, (const double*)w.Z,
-                               (const double*)Pold, Pnew, w.AP, w.part, w.ctl);
-            hipLaunchKernelGGL(k_iter2, dim3(nb), dim3(BLOCK), 0, st, N, (const double*)Pnew, (const double*)w.AP,
-                               (const double*)w.inv_diag, X, w.R, w.Z, w.part, w.ctl, tol);
-        }
-        if (sync_every > 0) {
-            Ctl c;
-            hipError_t e = hipMemcpyAsync(&c, w.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st);
+                       w.inv_diag, w.part2, (const Ctl*)w.ctl);
+    for (int k = 0; k < max_iter; ++k) {
+        double* Pold = w.P[(k + 1) & 1];
+        double* Pnew = w.P[k & 1];
+        hipLaunchKernelGGL(k_iter1<Op>, dim3(nb), dim3(BLOCK), 0, st, op, N, k, (const double*)w.Z,
+                           (const double*)Pold, Pnew, w.AP, (const double*)w.part2, w.part1, w.ctl, tol);
+        hipLaunchKernelGGL(k_iter2, dim3(nb), dim3(BLOCK), 0, st, N, k, (const double*)Pnew, (const double*)w.AP,
+                           (const double*)w.inv_diag, X, w.R, w.Z, (const double*)w.part1, w.part2,
+                           (const Ctl*)w.ctl);
+        if (sync_every > 0 && (k + 1) % sync_every == 0 && k + 1 < max_iter) {
+            int done = 0;
+            hipError_t e = hipMemcpyAsync(&done, &w.ctl->done, sizeof(int), hipMemcpyDeviceToHost, st);
             if (e == hipSuccess) e = hipStreamSynchronize(st);
             if (e != hipSuccess) return e;
-            if (iterations_host) *iterations_host = c.iterations;
-            if (c.done) break;
+            if (done) break;
         }
     }
+    // not stopped by then: max_iter iterations (whether or not the last one converged)
+    hipLaunchKernelGGL(k_close, dim3(1), dim3(64), 0, st, max_iter, w.ctl);
     if (iterations_dev) hipLaunchKernelGGL(k_report, dim3(1), dim3(64), 0, st, (const Ctl*)w.ctl, iterations_dev);
+    if (sync_every > 0) {
+        hipError_t e = hipMemcpyAsync(iterations_host, &w.ctl->iterations, sizeof(int), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) return e;
+    }
     return hipGetLastError();
 }
 

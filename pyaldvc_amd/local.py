@@ -258,6 +258,26 @@ def _P_from_UF(U, F):
     return P
 
 
+def _outliers(U, status, mesh, para) -> np.ndarray:
+    """bad = not converged, or flagged by the universal median test of U when the test is on and
+    more than 27 nodes converged (local_icgn.py:470-480, subpb1_solver.py:140-150). On the device
+    up to SPRING_MAX_UNKNOWN nodes (the unknown nodes are spring-filled there, as the reference
+    fills them up to that many), on the host beyond."""
+    N = int(np.shape(status)[0])
+    grid = tuple(mesh.grid_shape)
+    if N <= SPRING_MAX_UNKNOWN:
+        from .admm import subpb1_cleanup
+
+        _, bad = subpb1_cleanup(U, U, status, grid, float(para.local_outlier_threshold), edge_ok=mesh.edges_ok_grid())
+        return bad
+    bad = np.asarray(status) != STATUS_CONVERGED
+    if para.local_outlier_threshold > 0 and (~bad).sum() > 27:
+        flag = universal_median_test(np.asarray(U).reshape(grid + (3,)), (~bad).reshape(grid),
+                                     para.local_outlier_threshold, edge_ok=mesh.edges_ok_grid())  # fmt: skip
+        bad |= flag.ravel()
+    return bad
+
+
 def local_icgn(ctx: LocalContext, ref, g, U0, para, mesh, F0=None):
     """12-DOF IC-GN over all nodes from ``U0`` (and ``F0``), then the median test and inpainting
     of the failed / outlying nodes: ``(U (N,3), F (N,3,3), info, bad (N,))`` (local_icgn.py:398-489).
@@ -289,14 +309,7 @@ def local_icgn(ctx: LocalContext, ref, g, U0, para, mesh, F0=None):
 
     U = P[:, 9:12].copy()
     F = P[:, :9].reshape(N, 3, 3).copy()
-    bad = status != STATUS_CONVERGED
-    if para.local_outlier_threshold > 0:  # universal median test on the converged nodes
-        good = ~bad
-        if good.sum() > 27:
-            grid = tuple(mesh.grid_shape)
-            flag = universal_median_test(U.reshape(grid + (3,)), good.reshape(grid), para.local_outlier_threshold,
-                                         edge_ok=mesh.edges_ok_grid())  # fmt: skip
-            bad |= flag.ravel()
+    bad = _outliers(U, status, mesh, para)
     n_bad = int(np.sum(bad & (status != STATUS_INVALID_SUBSET) & (status != STATUS_SKIPPED)))
     logger.info("Local IC-GN: %d/%d converged, %d bad, %.2fs", int(np.sum(status == 0)), N, int(bad.sum()), solve_time)
     U, F = fill_bad_nodes(U, F, bad, mesh)
@@ -328,23 +341,17 @@ def subpb1_solver(ctx: LocalContext, ref, g, U_hat, F_hat, vdual, mu, para, mesh
         float(para.icgn_tol), float(para.icgn_dp_tol), int(para.icgn_max_iter), int(para.icgn_patience), ctx.stride,
         float(pattern[9, 9]), gain, bool(para.icgn_predictive_stop), **ctx.split_args(),
     )  # fmt: skip
-    grid = tuple(mesh.grid_shape)
     if N <= SPRING_MAX_UNKNOWN:
         # the cleanup on the device: median test (unknown nodes spring-filled, as the reference does
         # up to this many nodes) and U_filled = bad ? U_hat : U
         from .admm import subpb1_cleanup
 
-        U_filled, bad = subpb1_cleanup(U, torch.from_numpy(U_hat).to(dev), status, grid,
+        U_filled, bad = subpb1_cleanup(U, torch.from_numpy(U_hat).to(dev), status, tuple(mesh.grid_shape),
                                        float(para.local_outlier_threshold), edge_ok=mesh.edges_ok_grid())  # fmt: skip
         U_filled = U_filled.reshape(N, 3)
     else:
         Uh = U.cpu().numpy()
-        good = status.cpu().numpy() == STATUS_CONVERGED
-        bad = ~good
-        if para.local_outlier_threshold > 0 and good.sum() > 27:
-            flag = universal_median_test(Uh.reshape(grid + (3,)), good.reshape(grid), para.local_outlier_threshold,
-                                         edge_ok=mesh.edges_ok_grid())  # fmt: skip
-            bad |= flag.ravel()
+        bad = _outliers(Uh, status.cpu().numpy(), mesh, para)
         U_filled = Uh.copy()
         U_filled[bad] = U_hat[bad]  # nodes the local solver could not handle keep the global estimate
     n_iter = n_iter.cpu().numpy().astype(np.int32)
@@ -366,18 +373,23 @@ def fill_bad_nodes(U, F, bad, mesh):
         return np.zeros_like(U), (None if F is None else np.zeros_like(F))
     shape = tuple(mesh.grid_shape)
     edge_ok = mesh.edges_ok_grid()
-    holes = bad.reshape(shape)
-    U_out = U.copy()
-    for c in range(3):
-        comp = U[:, c].reshape(shape).copy()
-        comp[holes] = np.nan
-        U_out[:, c] = fill_nan_grid(comp, edge_ok=edge_ok).ravel()
+    holes = bad.reshape(-1)
+
+    def fill3(cols):
+        """Three columns (N, 3) with the bad nodes as holes, filled like fill_nan_grid: by springs
+        on the GPU up to SPRING_MAX_UNKNOWN unknowns, nearest-neighbour on the host beyond."""
+        v = np.array(cols, dtype=np.float64, copy=True)
+        v[holes] = np.nan
+        if int(np.isnan(v).sum(axis=0).max()) <= SPRING_MAX_UNKNOWN:
+            from .admm import spring_fill
+
+            return spring_fill(v, shape, edge_ok=edge_ok)
+        return np.stack([fill_nan_grid(v[:, c].reshape(shape), edge_ok=edge_ok).ravel() for c in range(3)], 1)
+
+    U_out = fill3(U)
     F_out = None
     if F is not None:
-        F_out = F.copy()
-        for i in range(3):
-            for j in range(3):
-                comp = F[:, i, j].reshape(shape).copy()
-                comp[holes] = np.nan
-                F_out[:, i, j] = fill_nan_grid(comp, edge_ok=edge_ok).ravel()
+        F_out = fill3(F.reshape(-1, 9)[:, 0:3])
+        F_out = np.concatenate([F_out, fill3(F.reshape(-1, 9)[:, 3:6]), fill3(F.reshape(-1, 9)[:, 6:9])], 1)
+        F_out = F_out.reshape(F.shape)
     return U_out, F_out
