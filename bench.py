@@ -230,6 +230,8 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=2, help="timed steps of the plain kernel (0 skips it)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="time the fused kernel's default launch shape without running the autotuner first")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run one device copy of this many MiB first (the scale of counter-based HBM traffic)")
     ap.add_argument("--workload", default="icgn512", choices=["icgn512", "ncc-bf16", "warp-residual", "masked-admm"],
@@ -276,13 +278,29 @@ def main():
     P0[:, 9:] = np.round(W["U_true"])
     P0d = torch.from_numpy(P0).to(dev)
 
-    def step(variant):
+    def step(variant, tuning=None):
         return K.icgn_12dof_parallel(cd, P0d, HALF, HALF, HALF, W["f"], W["gx"], W["gy"], W["gz"], nomask, W["g"], 1,
-                                     L, mf, bf, valid, 1e-2, 1e-3, 100, 5, variant=variant)  # fmt: skip
+                                     L, mf, bf, valid, 1e-2, 1e-3, 100, 5, variant=variant, tuning=tuning)  # fmt: skip
 
-    times, wall, out = timed_steps(lambda: step("fused"), args.steps, args.warmup)
+    # the autotuner picks the fused launch shape on a node sample (untimed, as a first call would)
+    from pyaldvc_amd import autotune
+
+    autotune.clear(memory_only=True)
+    t0 = time.time()
+    if args.no_autotune:
+        tuning, tune_report = autotune.DEFAULT, {"choice": autotune.DEFAULT.label() + " (autotuner not run)"}
+    else:
+        tuning = autotune.choose_icgn12(
+            (cd, P0d, HALF, HALF, HALF, W["f"], W["gx"], W["gy"], W["gz"], nomask, W["g"], 1, L, mf, bf, valid, 1e-2, 1e-3,
+             100, 5),
+            dict(stride=1, H_all=None, pattern=None, noise_gain=0.0, predictive=True, split_index=None, split_keep=None),
+        )  # fmt: skip
+        tune_report = dict(autotune.last_report)
+        log(f"autotune {time.time() - t0:.1f}s on {tune_report.get('nodes')} nodes: {tune_report.get('choice')} "
+            f"({', '.join(f'{k} {v * 1e3:.1f} ms' for k, v in tune_report.get('seconds', {}).items())})")  # fmt: skip
+    times, wall, out = timed_steps(lambda: step("fused", tuning), args.steps, args.warmup)
     t_step = float(np.mean(times))
-    t_kernel = kernel_times(lambda: step("fused"), args.steps)
+    t_kernel = kernel_times(lambda: step("fused", tuning), args.steps)
     P, n_iter, status, zncc = (x.cpu().numpy() for x in out)
     conv = status == 0
     err = np.linalg.norm(P[conv, 9:] - W["U_true"][conv], axis=1)
@@ -292,6 +310,12 @@ def main():
     achieved = bytes_alg / t_kernel / 1e9
     log(f"fused: {t_step * 1e3:.1f} ms/step, {n_valid / t_step:,.0f} nodes/s, {int(conv.sum())} converged, "
         f"mean iterations {iters / max(n_valid, 1):.2f}, median |err| {np.median(err):.4f} vox")  # fmt: skip
+
+    default = None
+    if args.plain_steps > 0 and tuning != autotune.DEFAULT:
+        td, _, _ = timed_steps(lambda: step("fused"), args.plain_steps, 1)
+        default = {"ms_per_step": float(np.mean(td)) * 1e3, "nodes_per_s": n_valid / float(np.mean(td))}
+        log(f"fused, default launch shape: {default['ms_per_step']:.1f} ms/step")
 
     gather = None
     if args.plain_steps > 0:
@@ -342,7 +366,7 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": "512^3 f32 bead pair (250k beads sigma 1.5, noise 0.01), 2% z strain + 5 deg z rotation; "
-            "subset 32^3, step 8, 61^3 nodes; fused 12-DOF IC-GN, cubic",
+            "subset 32^3, step 8, 61^3 nodes; fused 12-DOF IC-GN (autotuned launch shape), cubic",
             "nodes": N,
             "nodes_valid": n_valid,
             "nodes_converged": int(conv.sum()),
@@ -350,6 +374,9 @@ def main():
             "median_err_vox": float(np.median(err)) if err.size else None,
             "p95_err_vox": float(np.percentile(err, 95)) if err.size else None,
             "kernel_ms": t_kernel * 1e3,
+            "autotune": {"choice": tune_report.get("choice"), "sample_nodes": tune_report.get("nodes"),
+                         "candidate_ms": {k: v * 1e3 for k, v in tune_report.get("seconds", {}).items()}},
+            "fused_default_shape": default,
             "fused_gather_kernel": gather,
             "plain_kernel": plain,
             "speedup_vs_plain": (plain["ms_per_step"] / (t_step * 1e3)) if plain else None,

@@ -88,6 +88,20 @@ int aldvc_icgn12(const int64_t *coords, int64_t N, double *P, int hx, int hy, in
                  int8_t *status, double *zncc, int variant, void *workspace, size_t workspace_bytes,
                  hipStream_t stream);
 
+/* aldvc_icgn12 with the launch choices of the fused variants made explicit (the autotuner's search
+ * space; results keep their meaning, float32 sums may round differently): lds_budget = LDS bytes
+ * per workgroup for the brick (0: three workgroups per CU; at most 160 KiB), flags = 1: no
+ * plane-pair lane mapping, 2: node list in index order instead of spatial tiles, 4: workgroup b
+ * takes list slot b instead of the XCD-aware mapping. Same reference function as aldvc_icgn12. */
+int aldvc_icgn12_tuned(const int64_t *coords, int64_t N, double *P, int hx, int hy, int hz, int stride,
+                       const float *f, const float *gx, const float *gy, const float *gz, const uint8_t *mask,
+                       const float *g, int64_t nz, int64_t ny, int64_t nx, int mode, const double *L_all,
+                       const double *H_all, const double *pattern, double noise_gain, const double *meanf,
+                       const uint8_t *valid, double tol, double dp_tol, int max_iter, int patience, int predictive,
+                       const int64_t *split_index, const uint8_t *split_keep, int64_t keep_bytes, int32_t *n_iter,
+                       int8_t *status, double *zncc, int variant, int lds_budget, int flags, void *workspace,
+                       size_t workspace_bytes, hipStream_t stream);
+
 /* Workspace bytes aldvc_icgn3 needs for N nodes. */
 size_t aldvc_icgn3_workspace_bytes(int64_t N);
 

@@ -45,7 +45,13 @@ struct IcgnArgs {
     int32_t* n_iter;
     int8_t* status;
     double* zncc;
+    int xcd;  // 1: XCD-aware workgroup -> node mapping; 0: workgroup b takes list slot b
 };
+
+// Launch choices of the fused 12-DOF kernels that leave the results' meaning unchanged (the
+// autotuner's search space): LDS bytes per workgroup for the brick (0: three workgroups per CU),
+// and flag bits.
+enum : int { TUNE_NO_PAIRS = 1, TUNE_INDEX_ORDER = 2, TUNE_NO_XCD = 4 };
 
 // Per-node iteration state of the control lane (numba_kernels.py:663-677). It lives in LDS and
 // only lane 0 touches it: held in registers, the float64 control would set the register budget
@@ -465,7 +471,8 @@ constexpr int LDS_PER_WG = 54528;  // LDS budget of one fused workgroup: three p
 // XCD-aware slot of workgroup b over the M valid nodes: workgroup b runs on XCD b % 8, and each
 // XCD gets a contiguous run of the node list so its concurrent nodes share subset voxels in its
 // L2. Returns -1 for the workgroups beyond M (the grid is sized for every node).
-__device__ inline int64_t xcd_slot(int64_t b, int64_t M) {
+__device__ inline int64_t xcd_slot(int64_t b, int64_t M, int xcd = 1) {
+    if (!xcd) return b < M ? b : -1;
     const int64_t per = (M + 7) / 8;
     if ((b >> 3) >= per) return -1;
     const int64_t slot = (b & 7) * per + (b >> 3);
@@ -521,7 +528,7 @@ __device__ inline void ref_moments(const IcgnArgs& a, const uint8_t* keep, int64
 template <bool STORED>
 __global__ __launch_bounds__(BLOCK) void k_ref_moments(IcgnArgs a, const int64_t* __restrict__ list,
                                                        const int64_t* __restrict__ count, double* __restrict__ mom) {
-    const int64_t slot = xcd_slot(blockIdx.x, *count);
+    const int64_t slot = xcd_slot(blockIdx.x, *count, a.xcd);
     if (slot < 0) return;
     const int64_t n = list[slot];
     const int64_t x0 = a.coords[3 * n], y0 = a.coords[3 * n + 1], z0 = a.coords[3 * n + 2];
@@ -626,7 +633,7 @@ __global__ __launch_bounds__(BLOCK, 3) void k_icgn12_fused(IcgnArgs a, const int
     const Volumes& V = a.V;
     const Subset& s = a.sub;
     const int tid = threadIdx.x;
-    const int64_t slot = xcd_slot(blockIdx.x, *count);
+    const int64_t slot = xcd_slot(blockIdx.x, *count, a.xcd);
     if (slot < 0) return;
     const int64_t n = list[slot];
     const int64_t x0 = a.coords[3 * n], y0 = a.coords[3 * n + 1], z0 = a.coords[3 * n + 2];
@@ -1465,11 +1472,12 @@ __global__ void k_init_outputs(const uint8_t* __restrict__ valid, int64_t N, int
 // the volume is addressable with 32-bit byte offsets), else memory gathers.
 template <int MODE, bool STORED, bool SEL>
 void launch_fused(int variant, const IcgnArgs& a, const int64_t* list, const int64_t* count, int64_t nb, size_t bits,
-                  bool buf, double* mom, hipStream_t st) {
-    // LDS brick: BRICK_XY^2 floats per plane, as many planes as leave three workgroups per CU
-    // Pair mapping (stride 1, rows of 32..64 voxels): 39 rows per plane and a plane stride of
-    // sx modulo 64 floats; otherwise 40 rows and a plane of 40 x 40.
-    const bool pair = a.sub.stride == 1 && a.sub.sx >= 32 && a.sub.sx <= 64 && a.sub.sy >= NWAVES;
+                  bool buf, double* mom, int lds_budget, int flags, hipStream_t st) {
+    // LDS brick: BRICK_XY^2 floats per plane, as many planes as the LDS budget holds (by default
+    // three workgroups per CU). Pair mapping (stride 1, rows of 32..64 voxels): 39 rows per plane
+    // and a plane stride of sx modulo 64 floats; otherwise 40 rows and a plane of 40 x 40.
+    const bool pair = !(flags & TUNE_NO_PAIRS) && a.sub.stride == 1 && a.sub.sx >= 32 && a.sub.sx <= 64 &&
+                      a.sub.sy >= NWAVES;
     const int rows = pair ? BRICK_XY - 1 : BRICK_XY;
     const int plane = pair ? BRICK_XY * rows + ((a.sub.sx - BRICK_XY * rows) % 64 + 64) % 64 : BRICK_XY * rows;
     int planes = 0;
@@ -1479,7 +1487,8 @@ void launch_fused(int variant, const IcgnArgs& a, const int64_t* list, const int
         hipFuncAttributes fa{};
         if (hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&k_icgn12_fused<MODE, STORED, true, BRICK_XY, SEL>)) ==
             hipSuccess) {
-            const int64_t room = (int64_t)LDS_PER_WG - (int64_t)fa.sharedSizeBytes - (int64_t)bits;
+            const int64_t budget = lds_budget > 0 ? lds_budget : LDS_PER_WG;
+            const int64_t room = budget - (int64_t)fa.sharedSizeBytes - (int64_t)bits;
             planes = (int)std::min<int64_t>(a.sub.sz + 3, room / (int64_t)(4 * plane));
         }
         if (planes < 6) planes = 0;  // too few planes per chunk: the staging would dominate
@@ -1506,7 +1515,7 @@ void launch_fused(int variant, const IcgnArgs& a, const int64_t* list, const int
 // nodes and the workgroups past the count exit at once, so no host round trip is needed.
 template <int MODE, bool STORED>
 void launch_icgn12(int variant, const IcgnArgs& a, const int64_t* list, const int64_t* count, int64_t N, float* gbuf,
-                   hipStream_t st) {
+                   int lds_budget, int flags, hipStream_t st) {
     if (variant == 0) {
         for (int64_t off = 0; off < N; off += PLAIN_CHUNK) {
             const int64_t cnt = std::min<int64_t>(PLAIN_CHUNK, N - off);
@@ -1520,8 +1529,8 @@ void launch_icgn12(int variant, const IcgnArgs& a, const int64_t* list, const in
         const size_t bits = sizeof(unsigned) * (size_t)(((a.sub.S + 31) / 32 + 3) & ~3);
         const bool buf = (int64_t)a.V.nx * a.V.ny * a.V.nz < ((int64_t)1 << 30);  // 32-bit byte offsets
         const bool sel = a.V.mask != nullptr || (a.split_index != nullptr && a.split_keep != nullptr);
-        if (sel) launch_fused<MODE, STORED, true>(variant, a, list, count, nb, bits, buf, mom, st);
-        else launch_fused<MODE, STORED, false>(variant, a, list, count, nb, bits, buf, mom, st);
+        if (sel) launch_fused<MODE, STORED, true>(variant, a, list, count, nb, bits, buf, mom, lds_budget, flags, st);
+        else launch_fused<MODE, STORED, false>(variant, a, list, count, nb, bits, buf, mom, lds_budget, flags, st);
         timer_end(st);
     }
 }
@@ -1654,14 +1663,17 @@ size_t aldvc_icgn12_workspace_bytes(int64_t N, int hx, int hy, int hz, int strid
     return bytes;
 }
 
-int aldvc_icgn12(const int64_t* coords, int64_t N, double* P, int hx, int hy, int hz, int stride, const float* f,
-                 const float* gx, const float* gy, const float* gz, const uint8_t* mask, const float* g, int64_t nz,
-                 int64_t ny, int64_t nx, int mode, const double* L_all, const double* H_all, const double* pattern,
-                 double noise_gain, const double* meanf, const uint8_t* valid, double tol, double dp_tol,
-                 int max_iter, int patience, int predictive, const int64_t* split_index, const uint8_t* split_keep,
-                 int64_t keep_bytes, int32_t* n_iter, int8_t* status, double* zncc, int variant, void* workspace,
-                 size_t workspace_bytes, hipStream_t stream) {
+int aldvc_icgn12_tuned(const int64_t* coords, int64_t N, double* P, int hx, int hy, int hz, int stride,
+                       const float* f, const float* gx, const float* gy, const float* gz, const uint8_t* mask,
+                       const float* g, int64_t nz, int64_t ny, int64_t nx, int mode, const double* L_all,
+                       const double* H_all, const double* pattern, double noise_gain, const double* meanf,
+                       const uint8_t* valid, double tol, double dp_tol, int max_iter, int patience, int predictive,
+                       const int64_t* split_index, const uint8_t* split_keep, int64_t keep_bytes, int32_t* n_iter,
+                       int8_t* status, double* zncc, int variant, int lds_budget, int flags, void* workspace,
+                       size_t workspace_bytes, hipStream_t stream) {
     clear_error();
+    if (lds_budget < 0 || lds_budget > 160 * 1024) return fail(-2, "lds_budget must be 0 or at most 160 KiB");
+    if (flags & ~(TUNE_NO_PAIRS | TUNE_INDEX_ORDER | TUNE_NO_XCD)) return fail(-2, "unknown tuning flags");
     Volumes V{f, gx, gy, gz, g, mask, (int)nz, (int)ny, (int)nx};
     const Subset s = make_subset(hx, hy, hz, stride);
     if (int e = check_common(V, s, mode)) return e;
@@ -1687,13 +1699,14 @@ int aldvc_icgn12(const int64_t* coords, int64_t N, double* P, int hx, int hy, in
     float* gbuf = (float*)((char*)workspace + list_bytes);
     hipLaunchKernelGGL(k_init_outputs, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, stream, valid, N, n_iter,
                        status, zncc, (const int*)nullptr);
-    launch_compact(variant != 0, valid, coords, N, hx, hy, hz, V, list, count, stream);
+    launch_compact(variant != 0 && !(flags & TUNE_INDEX_ORDER), valid, coords, N, hx, hy, hz, V, list, count, stream);
     IcgnArgs a{V, s, coords, P, L_all, H_all, pattern, meanf, split_index, split_keep, keep_bytes,
-               noise_gain, tol, dp_tol, max_iter, patience, predictive, n_iter, status, zncc};
+               noise_gain, tol, dp_tol, max_iter, patience, predictive, n_iter, status, zncc,
+               (flags & TUNE_NO_XCD) ? 0 : 1};
     const bool stored = gx != nullptr;
-#define ALDVC_DISPATCH(MD)                                                            \
-    if (stored) launch_icgn12<MD, true>(variant, a, list, count, N, gbuf, stream);    \
-    else launch_icgn12<MD, false>(variant, a, list, count, N, gbuf, stream);
+#define ALDVC_DISPATCH(MD)                                                                                \
+    if (stored) launch_icgn12<MD, true>(variant, a, list, count, N, gbuf, lds_budget, flags, stream);    \
+    else launch_icgn12<MD, false>(variant, a, list, count, N, gbuf, lds_budget, flags, stream);
     switch (mode) {
         case 0: ALDVC_DISPATCH(0) break;
         case 1: ALDVC_DISPATCH(1) break;
@@ -1701,6 +1714,19 @@ int aldvc_icgn12(const int64_t* coords, int64_t N, double* P, int hx, int hy, in
     }
 #undef ALDVC_DISPATCH
     return launch_status();
+}
+
+int aldvc_icgn12(const int64_t* coords, int64_t N, double* P, int hx, int hy, int hz, int stride, const float* f,
+                 const float* gx, const float* gy, const float* gz, const uint8_t* mask, const float* g, int64_t nz,
+                 int64_t ny, int64_t nx, int mode, const double* L_all, const double* H_all, const double* pattern,
+                 double noise_gain, const double* meanf, const uint8_t* valid, double tol, double dp_tol,
+                 int max_iter, int patience, int predictive, const int64_t* split_index, const uint8_t* split_keep,
+                 int64_t keep_bytes, int32_t* n_iter, int8_t* status, double* zncc, int variant, void* workspace,
+                 size_t workspace_bytes, hipStream_t stream) {
+    return aldvc_icgn12_tuned(coords, N, P, hx, hy, hz, stride, f, gx, gy, gz, mask, g, nz, ny, nx, mode, L_all, H_all,
+                              pattern, noise_gain, meanf, valid, tol, dp_tol, max_iter, patience, predictive,
+                              split_index, split_keep, keep_bytes, n_iter, status, zncc, variant, 0, 0, workspace,
+                              workspace_bytes, stream);
 }
 
 size_t aldvc_icgn3_workspace_bytes(int64_t N) { return aldvc_icgn12_workspace_bytes(N, 0, 0, 0, 1, 1); }

@@ -297,19 +297,32 @@ def build_split_rows(coords, cand, hx, hy, hz, stride, mask, margin=0):
 def icgn_12dof_parallel(
     coords, P0, hx, hy, hz, f, gx, gy, gz, mask, g, mode, L_all, meanf_all, bottomf_all, valid,
     tol, dp_tol, max_iter, patience, stride=1, H_all=None, pattern=None, noise_gain=0.0,
-    predictive=True, split_index=None, split_keep=None, variant: str = "fused", stream=None,
+    predictive=True, split_index=None, split_keep=None, variant: str = "fused", stream=None, tuning=None,
 ):  # fmt: skip
     """12-DOF IC-GN over all nodes: ``(P_out (N,12), n_iter (N,) int32, status (N,) int8, zncc (N,))``.
 
     ``variant="plain"`` runs the reference GPU kernel's design (two passes through a scratch row,
     2048-node launches); ``"fused"`` (default) runs the one-pass kernel, which stages the deformed
     volume in LDS bricks where the warped subset fits one; ``"fused_gather"`` is the one-pass
-    kernel with memory gathers only. ``meanf_all`` feeds the
-    fused kernel's accumulation pilot only; ``bottomf_all`` is accepted for signature parity (the
-    12-DOF solver renormalises on the dynamic voxel set, numba_kernels.py:680-708).
+    kernel with memory gathers only; ``"auto"`` lets the autotuner pick the fused launch shape for
+    this subset shape on this device (timed once on a node sample, then cached). ``tuning`` (an
+    ``autotune.Tuning``) sets the launch shape explicitly. ``meanf_all`` feeds the fused kernel's
+    accumulation pilot only; ``bottomf_all`` is accepted for signature parity (the 12-DOF solver
+    renormalises on the dynamic voxel set, numba_kernels.py:680-708).
     """
+    if variant == "auto" and tuning is None:
+        from . import autotune
+
+        args = (coords, P0, hx, hy, hz, f, gx, gy, gz, mask, g, mode, L_all, meanf_all, bottomf_all, valid, tol, dp_tol,
+                max_iter, patience)  # fmt: skip
+        kw = dict(stride=stride, H_all=H_all, pattern=pattern, noise_gain=noise_gain, predictive=predictive,
+                  split_index=split_index, split_keep=split_keep, stream=stream)  # fmt: skip
+        tuning = autotune.choose_icgn12(args, kw)
+    if tuning is not None:
+        variant = {v: k for k, v in VARIANTS.items()}[tuning.variant]
     if variant not in VARIANTS:
-        raise ValueError(f"variant must be one of {sorted(VARIANTS)}")
+        raise ValueError(f"variant must be one of {sorted(VARIANTS) + ['auto']}")
+    lds_budget, flags = (int(tuning.lds_budget), int(tuning.flags)) if tuning is not None else (0, 0)
     A = _Args(f)
     c = A.arr(coords, np.int64)
     N = int(c.shape[0])
@@ -336,12 +349,12 @@ def icgn_12dof_parallel(
     ws_bytes = int(lib().aldvc_icgn12_workspace_bytes(N, int(hx), int(hy), int(hz), int(stride), var))
     ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
     check(
-        lib().aldvc_icgn12(
+        lib().aldvc_icgn12_tuned(
             _ptr(c), N, _ptr(P), int(hx), int(hy), int(hz), int(stride),
             _ptr(fd), _ptr(gxd), _ptr(gyd), _ptr(gzd), _ptr(md), _ptr(gd), nz, ny, nx, int(mode),
             _ptr(Ld), _ptr(Hd), _ptr(pat), float(noise_gain) if use_noise else 0.0, _ptr(mf), _ptr(vd),
             float(tol), float(dp_tol), int(max_iter), int(patience), int(bool(predictive)),
-            _ptr(si), _ptr(sk), kb, _ptr(n_iter), _ptr(status), _ptr(zncc), var, _ptr(ws), ws_bytes,
+            _ptr(si), _ptr(sk), kb, _ptr(n_iter), _ptr(status), _ptr(zncc), var, lds_budget, flags, _ptr(ws), ws_bytes,
             stream_handle(stream),
         ),
         "aldvc_icgn12",

@@ -48,6 +48,7 @@ STATUS_OUT_OF_BOUNDS = 2
 STATUS_INVALID_SUBSET = 3
 STATUS_SKIPPED = 6
 INTERP_MODE_BY_NAME = {"linear": 0, "cubic": 1, "bspline": 2}  # interp_kernels.py:27
+ICGN12_VARIANT = "auto"  # the fused kernel with the autotuned launch shape
 MAX_SPLIT_BYTES = 512 * 1024 * 1024  # packed keep rows beyond this: no splitting (local_icgn.py:37)
 GRADIENT_AUTO_BYTES = 8 * 1024**3  # "auto" gradients are stored below this (volume_ops.py:247)
 
@@ -299,7 +300,7 @@ def local_icgn(ctx: LocalContext, ref, g, U0, para, mesh, F0=None):
     P, n_iter, status, zncc = K.icgn_12dof_parallel(
         d["coords"], P0, hx, hy, hz, vols["f"], gx, gy, gz, mask, gd, mode, d["L"], d["meanf"], d["bottomf"],
         d["valid"], float(para.icgn_tol), float(para.icgn_dp_tol), int(para.icgn_max_iter), int(para.icgn_patience),
-        ctx.stride, d["H"], pattern, gain, bool(para.icgn_predictive_stop), **ctx.split_args(),
+        ctx.stride, d["H"], pattern, gain, bool(para.icgn_predictive_stop), **ctx.split_args(), variant=ICGN12_VARIANT,
     )  # fmt: skip
     P = P.cpu().numpy()
     n_iter = n_iter.cpu().numpy().astype(np.int32)

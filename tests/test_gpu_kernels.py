@@ -323,3 +323,39 @@ def test_zncc_map_bf16_large_template_matches_fp32(radius):
     ref = np.empty((1,) + m32.shape[1:])
     cpu_ref.zncc_direct(f, g, coords[:1], h, h, h, rx, ry, rz, wx0[:1], wy0[:1], wz0[:1], ok[:1], ref)
     assert np.max(np.abs(m32[0] - ref[0])) < 2e-5
+
+
+def test_autotuned_launch_shapes_agree_and_cache(tmp_path, monkeypatch):
+    """Every candidate launch shape of the fused kernel gives the oracle's statuses and the same
+    translations to 1e-4 voxel; the tuner picks one of them, caches it, and does not time again."""
+    from pyaldvc_amd import autotune as AT
+
+    monkeypatch.setenv("PYALDVC_AMD_TUNE_CACHE", str(tmp_path / "tune.json"))
+    AT.clear(memory_only=True)
+    c = BRICK_CASES["fits"]
+    cs = bead_case(shape=(112, 112, 112), F=c["F"], t=c["t"], seed=9)
+    ax = 24 + 16 * np.arange(5)
+    coords = np.stack(np.meshgrid(ax, ax, ax, indexing="ij"), -1).reshape(-1, 3).astype(np.int64)
+    half = (16, 16, 16)
+    f, g, gx, gy, gz = cs["f"], cs["g"], cs["gx"], cs["gy"], cs["gz"]
+    H, L, mf, bf, nv, valid = cpu_ref.precompute_nodes(coords, *half, f, gx, gy, gz, MASK_NONE, 0.5, 1e12)
+    P0 = np.zeros((coords.shape[0], 12))
+    P0[:, :9] = c["F"].reshape(-1)
+    P0[:, 9:] = np.round(truth_at(cs, coords))
+    args = (coords, P0, *half, f, gx, gy, gz, MASK_NONE, g, 1, L, mf, bf, valid, 1e-2, 1e-3, 100, 5)
+    ref = cpu_ref.icgn_12dof_parallel(*args)
+    base = K.icgn_12dof_parallel(*args, tuning=AT.DEFAULT)
+    for cand in AT.CANDIDATES:
+        got = K.icgn_12dof_parallel(*args, tuning=cand)
+        assert_icgn_parity(ref, got)
+        ok = (got[2] == 0) & (base[2] == 0)
+        assert np.array_equal(got[2], base[2]), cand
+        assert np.max(np.abs(got[0][ok] - base[0][ok])) < 1e-4, cand
+    auto = K.icgn_12dof_parallel(*args, variant="auto")
+    assert_icgn_parity(ref, auto)
+    rep = dict(AT.last_report)
+    assert rep["choice"] in {cand.label() for cand in AT.CANDIDATES} and len(rep["seconds"]) == len(AT.CANDIDATES)
+    assert (tmp_path / "tune.json").exists()
+    AT.last_report.clear()
+    K.icgn_12dof_parallel(*args, variant="auto")
+    assert AT.last_report == {}  # served from the cache
