@@ -161,22 +161,28 @@ __global__ __launch_bounds__(BLOCK) void k_zncc_map_f32(NccArgs a) {
 // a GEMM with M = oy, N = ox, K = window columns. v_mfma_f32_16x16x32_bf16 computes a 16x16
 // output tile over a 32-wide K slice; an output x-tile of 16 columns needs K = 15 + sx window
 // columns (KS slices). Operands are bfloat16 (window centred by the template mean c before the
-// rounding, template centred by its mean), accumulation float32. The window box sums of the
-// normalisation (sum g, sum g^2 over every template-sized box) are exact float32 sliding sums,
-// so only the numerator sees bfloat16 rounding.
+// rounding, template centred by its mean), accumulation float32.
 //
-// One workgroup = 4 wave64 = 4 consecutive output planes oz0 .. oz0 + 3 (one per wave) of one
-// node. The window planes z = oz0 .. oz0 + 3 + sz - 2 are staged through LDS once per
-// workgroup; plane z feeds wave g with template plane tz = z - oz0 - g, so every staged plane
-// serves all four waves.
+// Two kernels per call. k_box_var computes the window variance of every offset,
+// var = sum (W - c)^2 - (sum (W - c))^2 / n over the template-sized box, with exact float32
+// sliding sums (only the numerator sees bfloat16 rounding) and writes it into the map buffer.
+// k_zncc_map_bf16 then computes the numerator on MFMA and replaces var by the ZNCC in place. The
+// MFMA kernel thus only stages bfloat16 planes: one workgroup = 4 wave64 = 4 consecutive output
+// planes oz0 .. oz0 + 3 (one per wave) of one node; window plane z feeds wave g with template
+// plane tz = z - oz0 - g, so every staged plane serves all four waves, and the next plane's
+// global loads are issued before the current plane's MFMAs (two window buffers, five template
+// slots) so that they overlap. (Two output planes per wave, sharing each A fragment between two
+// MFMAs, needs ~310 registers: one wave per SIMD, and measured 12 % slower.)
 //
 // MFMA operand layout (16x16x32, wave64): lane l holds A[m = l % 16][k = 8 (l / 16) + i] and
 // B[k = 8 (l / 16) + i][n = l % 16] for i < 8, and D[m = 4 (l / 16) + i][n = l % 16] for i < 4.
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
 
-constexpr int MG = 4;     // output planes per workgroup (one per wave)
-constexpr int TRING = 4;  // template planes held in LDS
+constexpr int MG = 4;     // output planes per workgroup of the MFMA kernel (one per wave)
+constexpr int TRING = 5;  // template plane slots: four in use and the one being loaded
+constexpr int VG = 4;     // output planes per workgroup of the box-sum kernel
+constexpr int PRE = 16;   // window values per lane prefetched for the next plane (57^2 / 256 <= 13)
 
 struct Bf16Geom {
     int sx, sy, sz, vx, vy, vz, wx, wy, wz;
@@ -187,45 +193,22 @@ struct Bf16Geom {
 
 __device__ inline float bf16_round_trip(float v) { return (float)(__bf16)v; }
 
-template <int NT, int KS>
-__global__ __launch_bounds__(256) void k_zncc_map_bf16(NccArgs a, Bf16Geom q) {
-    extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
+// Shared prologue of both kernels: whether the node has a map, and the template mean and energy
+// (float64). Returns false when the map is the flat-template value -2.
+__device__ inline bool ncc_node_stats(const NccArgs& a, const Bf16Geom& q, int64_t node, double& mf, double& ssf) {
+    __shared__ double red[NWAVES];
+    __shared__ double stat[2];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int n_zgrp = (q.vz + MG - 1) / MG;
-    const int64_t node = blockIdx.x / n_zgrp;
-    const int oz0 = (int)(blockIdx.x % n_zgrp) * MG;
-    const int64_t n_off = (int64_t)q.vx * q.vy * q.vz;
-    float* out = a.out + node * n_off;
     const int64_t x0 = a.coords[3 * node], y0 = a.coords[3 * node + 1], z0 = a.coords[3 * node + 2];
     const int64_t xb = a.wx0[node], yb = a.wy0[node], zb = a.wz0[node];
     const bool tpl_in = x0 - a.hx >= 0 && x0 + a.hx < a.nx && y0 - a.hy >= 0 && y0 + a.hy < a.ny && z0 - a.hz >= 0 &&
                         z0 + a.hz < a.nz;
     const bool win_in = xb >= 0 && xb + q.wx <= a.nx && yb >= 0 && yb + q.wy <= a.ny && zb >= 0 && zb + q.wz <= a.nz;
-    const int my_oz = oz0 + wave;
+    if (!a.t_ok[node] || !tpl_in || !win_in) return false;  // block-uniform
     const int n_tpl = q.sx * q.sy * q.sz;
-    auto fill_mine = [&](float v) {
-        if (my_oz >= q.vz) return;
-        float* o = out + (int64_t)my_oz * q.vy * q.vx;
-        for (int i = lane; i < q.vy * q.vx; i += 64) o[i] = v;
-    };
-    if (!a.t_ok[node] || !tpl_in || !win_in) {
-        fill_mine(-2.0f);
-        return;
-    }
-    // ---- LDS carve-up
-    __bf16* Ap = (__bf16*)lds;                                    // AH x AW window plane (bf16)
-    __bf16* Tr = Ap + q.AH * q.AW;                                // TRING x sy x TW template planes
-    float* Pf = (float*)(Tr + TRING * q.sy * q.TW);               // wy x wx float32 plane
-    float* R1 = Pf + q.wy * q.wx;                                 // wy x vx row box sums
-    float* R2 = R1 + q.wy * q.vx;
-    float* P1 = R2 + q.wy * q.vx;                                 // vy x vx plane box sums
-    float* P2 = P1 + q.vy * q.vx;
-    __shared__ double red[4];
-    __shared__ double stat[2];
-    // ---- template mean and energy (float64)
     const int64_t tbase = ((z0 - a.hz) * a.ny + (y0 - a.hy)) * (int64_t)a.nx + (x0 - a.hx);
     double sacc = 0.0;
-    for (int v = tid; v < n_tpl; v += 256) {
+    for (int v = tid; v < n_tpl; v += BLOCK) {
         const int k = v / (q.sy * q.sx), r = v - k * q.sy * q.sx, j = r / q.sx, i = r - j * q.sx;
         sacc += a.f[tbase + ((int64_t)k * a.ny + j) * a.nx + i];
     }
@@ -234,9 +217,9 @@ __global__ __launch_bounds__(256) void k_zncc_map_bf16(NccArgs a, Bf16Geom q) {
     __syncthreads();
     if (tid == 0) stat[0] = (red[0] + red[1] + red[2] + red[3]) / n_tpl;
     __syncthreads();
-    const double mf = stat[0];
+    mf = stat[0];
     double ss = 0.0;
-    for (int v = tid; v < n_tpl; v += 256) {
+    for (int v = tid; v < n_tpl; v += BLOCK) {
         const int k = v / (q.sy * q.sx), r = v - k * q.sy * q.sx, j = r / q.sx, i = r - j * q.sx;
         const double d = (double)a.f[tbase + ((int64_t)k * a.ny + j) * a.nx + i] - mf;
         ss += d * d;
@@ -247,93 +230,208 @@ __global__ __launch_bounds__(256) void k_zncc_map_bf16(NccArgs a, Bf16Geom q) {
     __syncthreads();
     if (tid == 0) stat[1] = red[0] + red[1] + red[2] + red[3];
     __syncthreads();
-    const double ssf = stat[1];
-    if (ssf < 1e-20) {
+    ssf = stat[1];
+    return ssf >= 1e-20;
+}
+
+// Window variance of every offset of VG consecutive output planes: per window plane, 2-D box
+// sums (x then y, sliding in runs of 8 outputs), added to the planes whose template range holds
+// the window plane. A thread owns the offsets tid + 256 k of each of the VG planes.
+constexpr int VPT = 9;  // offsets per thread and plane: vy * vx <= 256 * VPT (radii up to 23)
+
+__global__ __launch_bounds__(BLOCK) void k_box_var(NccArgs a, Bf16Geom q) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
+    const int tid = threadIdx.x;
+    const int n_zgrp = (q.vz + VG - 1) / VG;
+    const int64_t node = blockIdx.x / n_zgrp;
+    const int oz0 = (int)(blockIdx.x % n_zgrp) * VG;
+    double mf, ssf;
+    if (!ncc_node_stats(a, q, node, mf, ssf)) return;  // the MFMA kernel writes the -2 map
+    const float c = (float)mf;
+    float* Pf = reinterpret_cast<float*>(lds);  // wy x wx centred plane
+    float* R1 = Pf + q.wy * q.wx;                // wy x vx row sums
+    float* R2 = R1 + q.wy * q.vx;
+    float* P1 = R2 + q.wy * q.vx;  // vy x vx box sums
+    float* P2 = P1 + q.vy * q.vx;
+    const int64_t xb = a.wx0[node], yb = a.wy0[node], zb = a.wz0[node];
+    const int nplane = q.vy * q.vx;
+    float S1[VG][VPT], S2[VG][VPT];
+#pragma unroll
+    for (int gq = 0; gq < VG; ++gq)
+#pragma unroll
+        for (int k = 0; k < VPT; ++k) S1[gq][k] = S2[gq][k] = 0.0f;
+    const int nseg_x = (q.vx + 7) / 8, nseg_y = (q.vy + 7) / 8;
+    const int z_end = min(oz0 + VG + q.sz - 1, q.wz);
+    for (int z = oz0; z < z_end; ++z) {
+        __syncthreads();
+        const float* gp = a.g + ((zb + z) * a.ny + yb) * (int64_t)a.nx + xb;
+        for (int i = tid; i < q.wy * q.wx; i += BLOCK) {
+            const int y = i / q.wx, x = i - y * q.wx;
+            Pf[i] = gp[(int64_t)y * a.nx + x] - c;
+        }
+        __syncthreads();
+        for (int t = tid; t < q.wy * nseg_x; t += BLOCK) {  // row sums, 8 outputs per item
+            const int y = t / nseg_x, ox0 = 8 * (t - y * nseg_x);
+            const float* row = Pf + y * q.wx;
+            float r1 = 0.0f, r2 = 0.0f;
+            for (int x = 0; x < q.sx; ++x) {
+                const float v = row[ox0 + x];
+                r1 += v;
+                r2 += v * v;
+            }
+            R1[y * q.vx + ox0] = r1;
+            R2[y * q.vx + ox0] = r2;
+            for (int ox = ox0 + 1; ox < min(ox0 + 8, q.vx); ++ox) {
+                const float vin = row[ox + q.sx - 1], vout = row[ox - 1];
+                r1 += vin - vout;
+                r2 += vin * vin - vout * vout;
+                R1[y * q.vx + ox] = r1;
+                R2[y * q.vx + ox] = r2;
+            }
+        }
+        __syncthreads();
+        for (int t = tid; t < q.vx * nseg_y; t += BLOCK) {  // column sums, 8 outputs per item
+            const int ox = t / nseg_y, oy0 = 8 * (t - ox * nseg_y);
+            float r1 = 0.0f, r2 = 0.0f;
+            for (int y = 0; y < q.sy; ++y) {
+                r1 += R1[(oy0 + y) * q.vx + ox];
+                r2 += R2[(oy0 + y) * q.vx + ox];
+            }
+            P1[oy0 * q.vx + ox] = r1;
+            P2[oy0 * q.vx + ox] = r2;
+            for (int oy = oy0 + 1; oy < min(oy0 + 8, q.vy); ++oy) {
+                r1 += R1[(oy + q.sy - 1) * q.vx + ox] - R1[(oy - 1) * q.vx + ox];
+                r2 += R2[(oy + q.sy - 1) * q.vx + ox] - R2[(oy - 1) * q.vx + ox];
+                P1[oy * q.vx + ox] = r1;
+                P2[oy * q.vx + ox] = r2;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < VPT; ++k) {
+            const int i = tid + BLOCK * k;
+            const float p1 = i < nplane ? P1[i] : 0.0f, p2 = i < nplane ? P2[i] : 0.0f;
+#pragma unroll
+            for (int gq = 0; gq < VG; ++gq) {
+                const int tz = z - (oz0 + gq);
+                if (tz >= 0 && tz < q.sz) {
+                    S1[gq][k] += p1;
+                    S2[gq][k] += p2;
+                }
+            }
+        }
+    }
+    const float inv_n = 1.0f / (float)(q.sx * q.sy * q.sz);
+    const int64_t n_off = (int64_t)q.vx * q.vy * q.vz;
+#pragma unroll
+    for (int gq = 0; gq < VG; ++gq) {
+        const int oz = oz0 + gq;
+        if (oz >= q.vz) break;
+        float* o = a.out + node * n_off + (int64_t)oz * nplane;
+#pragma unroll
+        for (int k = 0; k < VPT; ++k) {
+            const int i = tid + BLOCK * k;
+            if (i < nplane) o[i] = S2[gq][k] - S1[gq][k] * S1[gq][k] * inv_n;
+        }
+    }
+}
+
+template <int NT, int KS>
+__global__ __launch_bounds__(256) void k_zncc_map_bf16(NccArgs a, Bf16Geom q) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int n_zgrp = (q.vz + MG - 1) / MG;
+    const int64_t node = blockIdx.x / n_zgrp;
+    const int oz0 = (int)(blockIdx.x % n_zgrp) * MG;
+    const int64_t n_off = (int64_t)q.vx * q.vy * q.vz;
+    float* out = a.out + node * n_off;
+    const int my_oz = oz0 + wave;
+    auto fill_mine = [&](float v) {
+        if (my_oz >= q.vz) return;
+        float* o = out + (int64_t)my_oz * q.vy * q.vx;
+        for (int i = lane; i < q.vy * q.vx; i += 64) o[i] = v;
+    };
+    double mf, ssf;
+    if (!ncc_node_stats(a, q, node, mf, ssf)) {
         fill_mine(-2.0f);
         return;
     }
+    const int64_t x0 = a.coords[3 * node], y0 = a.coords[3 * node + 1], z0 = a.coords[3 * node + 2];
+    const int64_t xb = a.wx0[node], yb = a.wy0[node], zb = a.wz0[node];
+    const int64_t tbase = ((z0 - a.hz) * a.ny + (y0 - a.hy)) * (int64_t)a.nx + (x0 - a.hx);
     const float c = (float)mf;
-    // zero the padded staging buffers once (data regions are rewritten per plane)
-    for (int i = tid; i < q.AH * q.AW; i += 256) Ap[i] = (__bf16)0.0f;
-    for (int i = tid; i < TRING * q.sy * q.TW; i += 256) Tr[i] = (__bf16)0.0f;
-    // ---- per-wave state: accumulators and box sums in the D-tile layout
+    // ---- LDS: two window planes (bf16, zero-padded) and TRING template planes
+    const int plane_el = q.AH * q.AW;
+    __bf16* Ab = (__bf16*)lds;
+    __bf16* Tr = Ab + 2 * plane_el;
+    for (int i = tid; i < 2 * plane_el; i += BLOCK) Ab[i] = (__bf16)0.0f;
+    for (int i = tid; i < TRING * q.sy * q.TW; i += BLOCK) Tr[i] = (__bf16)0.0f;
+    __syncthreads();
+    // per-lane prefetch of one window plane (and one template plane)
+    const int wpl = q.wy * q.wx, tpl_n = q.sy * q.sx;
+    float pre[PRE], tpre[4];
+    auto load_plane = [&](int z) {
+        const float* gp = a.g + ((zb + z) * a.ny + yb) * (int64_t)a.nx + xb;
+#pragma unroll
+        for (int u = 0; u < PRE; ++u) {
+            const int i = tid + BLOCK * u;
+            if (i < wpl) {
+                const int y = i / q.wx, x = i - y * q.wx;
+                pre[u] = gp[(int64_t)y * a.nx + x];
+            }
+        }
+        const int tz = z - oz0;
+        if (tz < q.sz) {
+            const float* fp = a.f + tbase + (int64_t)tz * a.ny * a.nx;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int i = tid + BLOCK * u;
+                if (i < tpl_n) {
+                    const int y = i / q.sx, x = i - y * q.sx;
+                    tpre[u] = fp[(int64_t)y * a.nx + x];
+                }
+            }
+        }
+    };
+    auto store_plane = [&](int z) {
+        __bf16* Ap = Ab + (z & 1) * plane_el;
+#pragma unroll
+        for (int u = 0; u < PRE; ++u) {
+            const int i = tid + BLOCK * u;
+            if (i < wpl) {
+                const int y = i / q.wx, x = i - y * q.wx;
+                Ap[y * q.AW + x] = (__bf16)(pre[u] - c);
+            }
+        }
+        const int tz = z - oz0;
+        if (tz < q.sz) {
+            __bf16* tp = Tr + (tz % TRING) * q.sy * q.TW;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int i = tid + BLOCK * u;
+                if (i < tpl_n) {
+                    const int y = i / q.sx, x = i - y * q.sx;
+                    tp[y * q.TW + 16 + x] = (__bf16)(float)((double)tpre[u] - mf);
+                }
+            }
+        }
+    };
     f32x4_t acc[NT][NT];
-    f32x4_t S1[NT][NT], S2[NT][NT];
 #pragma unroll
     for (int mt = 0; mt < NT; ++mt)
 #pragma unroll
-        for (int j = 0; j < NT; ++j) {
-            acc[mt][j] = (f32x4_t){0.f, 0.f, 0.f, 0.f};
-            S1[mt][j] = (f32x4_t){0.f, 0.f, 0.f, 0.f};
-            S2[mt][j] = (f32x4_t){0.f, 0.f, 0.f, 0.f};
-        }
+        for (int j = 0; j < NT; ++j) acc[mt][j] = (f32x4_t){0.f, 0.f, 0.f, 0.f};
     const int am = lane & 15, akb = 8 * (lane >> 4);  // A row / K offset, also B column / K offset
     const int z_end = min(oz0 + MG + q.sz - 1, q.wz);
+    load_plane(oz0);
+    store_plane(oz0);
+    __syncthreads();
     for (int z = oz0; z < z_end; ++z) {
-        __syncthreads();  // the previous plane's consumers are done with every buffer
-        // stage window plane z (centred) and template plane tz_new = z - oz0
-        const float* gp = a.g + ((zb + z) * a.ny + yb) * (int64_t)a.nx + xb;
-        for (int i = tid; i < q.wy * q.wx; i += 256) {
-            const int y = i / q.wx, x = i - y * q.wx;
-            const float v = gp[(int64_t)y * a.nx + x] - c;
-            Pf[i] = v;
-            Ap[y * q.AW + x] = (__bf16)v;
-        }
-        const int tz_new = z - oz0;
-        if (tz_new < q.sz) {
-            __bf16* tp = Tr + (tz_new % TRING) * q.sy * q.TW;
-            const float* fp = a.f + tbase + (int64_t)tz_new * a.ny * a.nx;
-            for (int i = tid; i < q.sy * q.sx; i += 256) {
-                const int y = i / q.sx, x = i - y * q.sx;
-                tp[y * q.TW + 16 + x] = (__bf16)(float)((double)fp[(int64_t)y * a.nx + x] - mf);
-            }
-        }
-        __syncthreads();
-        // row box sums over x (sliding), then column box sums over y
-        for (int t = tid; t < 2 * q.wy; t += 256) {
-            const int y = t >> 1, sq = t & 1;
-            const float* row = Pf + y * q.wx;
-            float* R = (sq ? R2 : R1) + y * q.vx;
-            float r = 0.0f;
-            for (int x = 0; x < q.sx; ++x) r += sq ? row[x] * row[x] : row[x];
-            R[0] = r;
-            for (int ox = 1; ox < q.vx; ++ox) {
-                const float vin = row[ox + q.sx - 1], vout = row[ox - 1];
-                r += sq ? (vin * vin - vout * vout) : (vin - vout);
-                R[ox] = r;
-            }
-        }
-        __syncthreads();
-        for (int t = tid; t < 2 * q.vx; t += 256) {
-            const int ox = t >> 1, sq = t & 1;
-            const float* R = sq ? R2 : R1;
-            float* P = sq ? P2 : P1;
-            float r = 0.0f;
-            for (int y = 0; y < q.sy; ++y) r += R[y * q.vx + ox];
-            P[ox] = r;
-            for (int oy = 1; oy < q.vy; ++oy) {
-                r += R[(oy + q.sy - 1) * q.vx + ox] - R[(oy - 1) * q.vx + ox];
-                P[oy * q.vx + ox] = r;
-            }
-        }
-        __syncthreads();
+        const bool more = z + 1 < z_end;
+        if (more) load_plane(z + 1);  // in flight during this plane's MFMAs
         const int tz = z - my_oz;
         if (my_oz < q.vz && tz >= 0 && tz < q.sz) {
-            // box sums of this plane into the wave's tiles
-#pragma unroll
-            for (int mt = 0; mt < NT; ++mt)
-#pragma unroll
-                for (int j = 0; j < NT; ++j) {
-                    const int ox = 16 * j + (lane & 15);
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) {
-                        const int oy = 16 * mt + 4 * (lane >> 4) + i;
-                        if (mt < q.nyt && j < q.nxt && oy < q.vy && ox < q.vx) {
-                            S1[mt][j][i] += P1[oy * q.vx + ox];
-                            S2[mt][j][i] += P2[oy * q.vx + ox];
-                        }
-                    }
-                }
+            const __bf16* Ap = Ab + (z & 1) * plane_el;
             const __bf16* tpl = Tr + (tz % TRING) * q.sy * q.TW;
             for (int ty = 0; ty < q.sy; ++ty) {
                 bf16x8_t bfr[KS];
@@ -361,9 +459,10 @@ __global__ __launch_bounds__(256) void k_zncc_map_bf16(NccArgs a, Bf16Geom q) {
                 }
             }
         }
+        if (more) store_plane(z + 1);  // the other window buffer and a free template slot
+        __syncthreads();
     }
     if (my_oz >= q.vz) return;
-    const float inv_n = 1.0f / (float)n_tpl;
     const float sq_ssf = (float)sqrt(ssf);
     float* o = out + (int64_t)my_oz * q.vy * q.vx;
 #pragma unroll
@@ -375,7 +474,7 @@ __global__ __launch_bounds__(256) void k_zncc_map_bf16(NccArgs a, Bf16Geom q) {
             for (int i = 0; i < 4; ++i) {
                 const int oy = 16 * mt + 4 * (lane >> 4) + i;
                 if (mt < q.nyt && j < q.nxt && oy < q.vy && ox < q.vx) {
-                    const float var = S2[mt][j][i] - S1[mt][j][i] * S1[mt][j][i] * inv_n;
+                    const float var = o[oy * q.vx + ox];  // written by k_box_var
                     float val = 0.0f;
                     if (var >= 1e-20f) {
                         val = acc[mt][j][i] / (sqrtf(var) * sq_ssf);
@@ -388,8 +487,11 @@ __global__ __launch_bounds__(256) void k_zncc_map_bf16(NccArgs a, Bf16Geom q) {
 }
 
 size_t bf16_lds_bytes(const Bf16Geom& q) {
-    return sizeof(__bf16) * ((size_t)q.AH * q.AW + (size_t)TRING * q.sy * q.TW) +
-           sizeof(float) * ((size_t)q.wy * q.wx + 2 * (size_t)q.wy * q.vx + 2 * (size_t)q.vy * q.vx);
+    return sizeof(__bf16) * (2 * (size_t)q.AH * q.AW + (size_t)TRING * q.sy * q.TW);
+}
+
+size_t box_lds_bytes(const Bf16Geom& q) {
+    return sizeof(float) * ((size_t)q.wy * q.wx + 2 * (size_t)q.wy * q.vx + 2 * (size_t)q.vy * q.vx);
 }
 
 Bf16Geom bf16_geom(int hx, int hy, int hz, int rx, int ry, int rz, int& ks) {
@@ -408,7 +510,10 @@ Bf16Geom bf16_geom(int hx, int hy, int hz, int rx, int ry, int rz, int& ks) {
     ks = (q.sx + 15 + 31) / 32;
     q.AH = 16 * q.nyt + q.sy - 1;
     q.AW = 16 * (q.nxt - 1) + 32 * ks;
+    // row stride = 8 mod 16 elements: the 16 rows an A fragment reads start on 16 different
+    // 4-bank groups
     q.AW = (q.AW + 7) / 8 * 8;
+    if (q.AW % 16 == 0) q.AW += 8;
     q.TW = 32 * ks + 16;
     return q;
 }
@@ -573,8 +678,17 @@ int aldvc_zncc_map(const float* f, const float* g, int64_t nz, int64_t ny, int64
         if (ks > 2) return fail(-2, "bfloat16 maps support templates up to 49 voxels along x");
         const size_t lds = bf16_lds_bytes(q);
         if (lds > 160 * 1024 - 256) return fail(-2, "template and window planes do not fit in LDS");
+        if ((int64_t)q.wy * q.wx > (int64_t)BLOCK * PRE) return fail(-2, "window plane too large for the prefetch");
+        if ((int64_t)q.sy * q.sx > (int64_t)BLOCK * 4) return fail(-2, "template plane too large for the prefetch");
+        if ((int64_t)q.vy * q.vx > (int64_t)BLOCK * VPT) return fail(-2, "search plane too large for the box sums");
         const int64_t nblk = n * ((q.vz + MG - 1) / MG);
+        const int64_t nbox = n * ((q.vz + VG - 1) / VG);
         if (nblk > INT32_MAX) return fail(-2, "too many workgroups for one launch");
+        const size_t blds = box_lds_bytes(q);
+        if (blds > 64 * 1024 &&
+            hipFuncSetAttribute((const void*)k_box_var, hipFuncAttributeMaxDynamicSharedMemorySize, (int)blds) != hipSuccess)
+            return fail(-5, "could not raise the LDS limit");
+        hipLaunchKernelGGL(k_box_var, dim3((unsigned)nbox), dim3(BLOCK), blds, stream, a, q);
         const void* fn = nullptr;
 #define ALDVC_BF16(NTV, KSV)                                                                                \
     if (nt <= NTV && ks == KSV) {                                                                          \
