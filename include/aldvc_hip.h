@@ -92,7 +92,8 @@ int aldvc_icgn12(const int64_t *coords, int64_t N, double *P, int hx, int hy, in
  * space; results keep their meaning, float32 sums may round differently): lds_budget = LDS bytes
  * per workgroup for the brick (0: three workgroups per CU; at most 160 KiB), flags = 1: no
  * plane-pair lane mapping, 2: node list in index order instead of spatial tiles, 4: workgroup b
- * takes list slot b instead of the XCD-aware mapping. Same reference function as aldvc_icgn12. */
+ * takes list slot b instead of the XCD-aware mapping, 8: no 8 x 8 lane patches (the default
+ * mapping of the brick path). Same reference function as aldvc_icgn12. */
 int aldvc_icgn12_tuned(const int64_t *coords, int64_t N, double *P, int hx, int hy, int hz, int stride,
                        const float *f, const float *gx, const float *gy, const float *gz, const uint8_t *mask,
                        const float *g, int64_t nz, int64_t ny, int64_t nx, int mode, const double *L_all,

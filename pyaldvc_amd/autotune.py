@@ -2,8 +2,9 @@
 
 The fused kernel has launch choices that do not change what it computes, only how fast: the LDS
 budget of a workgroup's brick (three workgroups per CU with 7 planes, or two with 11), the
-plane-pair lane mapping of 32-64-voxel rows, the node order (spatial tiles or index order), the
-XCD-aware workgroup -> node mapping, and bricks versus memory gathers. Which is fastest depends
+voxel-to-lane mapping of the brick path (8 x 8 patches per wave, plane pairs of 32-64-voxel rows,
+or sampling order), the node order (spatial tiles or index order), the XCD-aware workgroup ->
+node mapping, and bricks versus memory gathers. Which is fastest depends
 on the subset shape, the interpolation, the mask/split selection and the deformation (how many
 chunks fit a brick). ``choose_icgn12`` times every candidate on a contiguous run of the call's
 valid nodes (neighbouring nodes, so that subset overlap and L2 reuse are as in the full call;
@@ -29,7 +30,7 @@ from dataclasses import asdict, dataclass
 import numpy as np
 import torch
 
-TUNE_NO_PAIRS, TUNE_INDEX_ORDER, TUNE_NO_XCD = 1, 2, 4
+TUNE_NO_PAIRS, TUNE_INDEX_ORDER, TUNE_NO_XCD, TUNE_NO_PATCHES = 1, 2, 4, 8
 SAMPLE_NODES = 16384  # nodes timed per candidate: 256 CUs x 3 workgroups, 20 times over
 MARGIN = 0.03  # a candidate replaces the default only when it is this much faster
 REPEATS = 3
@@ -45,7 +46,8 @@ class Tuning:
         parts = ["bricks" if self.variant == 1 else "gathers"]
         if self.lds_budget:
             parts.append(f"lds{self.lds_budget // 1024}k")
-        for bit, name in ((TUNE_NO_PAIRS, "no-pairs"), (TUNE_INDEX_ORDER, "index-order"), (TUNE_NO_XCD, "no-xcd")):
+        for bit, name in ((TUNE_NO_PATCHES, "no-patches"), (TUNE_NO_PAIRS, "no-pairs"), (TUNE_INDEX_ORDER, "index-order"),
+                          (TUNE_NO_XCD, "no-xcd")):  # fmt: skip
             if self.flags & bit:
                 parts.append(name)
         return "+".join(parts)
@@ -53,10 +55,10 @@ class Tuning:
 
 DEFAULT = Tuning(1)
 CANDIDATES = (
-    DEFAULT,
-    Tuning(1, 0, TUNE_NO_PAIRS),
+    DEFAULT,  # bricks, 8 x 8 lane patches
+    Tuning(1, 0, TUNE_NO_PATCHES),  # plane pairs for 32-64-voxel rows
+    Tuning(1, 0, TUNE_NO_PATCHES | TUNE_NO_PAIRS),  # voxels in sampling order
     Tuning(1, 80 * 1024),  # two workgroups per CU, 11 planes per brick
-    Tuning(1, 80 * 1024, TUNE_NO_PAIRS),
     Tuning(1, 0, TUNE_NO_XCD),
     Tuning(1, 0, TUNE_INDEX_ORDER),
     Tuning(2),

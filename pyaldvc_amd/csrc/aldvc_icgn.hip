@@ -51,7 +51,8 @@ struct IcgnArgs {
 // Launch choices of the fused 12-DOF kernels that leave the results' meaning unchanged (the
 // autotuner's search space): LDS bytes per workgroup for the brick (0: three workgroups per CU),
 // and flag bits.
-enum : int { TUNE_NO_PAIRS = 1, TUNE_INDEX_ORDER = 2, TUNE_NO_XCD = 4 };
+enum : int { TUNE_NO_PAIRS = 1, TUNE_INDEX_ORDER = 2, TUNE_NO_XCD = 4, TUNE_NO_PATCHES = 8 };
+enum : int { LANES_WALK = 0, LANES_PAIRS = 1, LANES_PATCHES = 2 };  // voxel -> lane mappings of the brick path
 
 // Per-node iteration state of the control lane (numba_kernels.py:663-677). It lives in LDS and
 // only lane 0 touches it: held in registers, the float64 control would set the register budget
@@ -628,7 +629,7 @@ template <int MODE, bool STORED, bool BUF, int BRK, bool SEL>
 __global__ __launch_bounds__(BLOCK, 3) void k_icgn12_fused(IcgnArgs a, const int64_t* __restrict__ list,
                                                         const int64_t* __restrict__ count,
                                                         const double* __restrict__ mom, int brick_planes,
-                                                        int brick_plane, int brick_rows, int pair_map) {
+                                                        int brick_plane, int brick_rows, int lane_map) {
     static_assert(BRK == 0 || BUF, "the brick path stages through the buffer descriptor");
     const Volumes& V = a.V;
     const Subset& s = a.sub;
@@ -767,7 +768,47 @@ __global__ __launch_bounds__(BLOCK, 3) void k_icgn12_fused(IcgnArgs a, const int
                     stage_brick<BRK>(brick, brick_plane, brick_rows, gb, (int)obase, rowp, planep, bx, by, bz, ez);
                     __syncthreads();
                     using Brick = std::integral_constant<bool, true>;
-                    if (pair_map) {
+                    if (lane_map == LANES_PATCHES) {
+                        // 8 x 8 patches of one plane per wave (lane l: x + (l & 7), y + (l >> 3)). With a
+                        // brick row stride of 40 floats the 8 rows of a patch start 40 y mod 64 = 8
+                        // different bank groups apart, so the 64 lanes read 64 different banks, and a
+                        // small rotation or shear only shifts whole patch rows by a bank or two. The
+                        // columns and rows beyond the last whole patch follow in a short tail.
+                        const int lane = tid & 63, sx = s.sx, sy = s.sy;
+                        const int pxn = sx >> 3, pyn = sy >> 3, pp = pxn * pyn, npatch = kz * pp;
+                        for (int pi = tid >> 6; pi < npatch; pi += NWAVES) {
+                            const int pz = pi / pp, r = pi - pz * pp, pyi = r / pxn, pxi = r - pyi * pxn;
+                            const int iz = zc + pz, x = 8 * pxi + (lane & 7), y = 8 * pyi + (lane >> 3);
+                            Vox q;
+                            q.dx = -s.hx + x;
+                            q.dy = -s.hy + y;
+                            q.dz = -s.hz + iz;
+                            q.v = (iz * sy + y) * sx + x;
+                            q.o = q.dz * planep + q.dy * rowp + q.dx;
+                            voxel(q, Brick{}, bx, by, bz);
+                        }
+                        const int tx = sx - 8 * pxn, wxp = 8 * pxn, per_plane = tx * sy + (sy - 8 * pyn) * wxp;
+                        for (int t = tid; t < kz * per_plane; t += BLOCK) {
+                            const int pz = t / per_plane, r = t - pz * per_plane;
+                            int x, y;
+                            if (r < tx * sy) {
+                                y = r / tx;
+                                x = wxp + (r - y * tx);
+                            } else {
+                                const int r2 = r - tx * sy, yy = r2 / wxp;
+                                y = 8 * pyn + yy;
+                                x = r2 - yy * wxp;
+                            }
+                            const int iz = zc + pz;
+                            Vox q;
+                            q.dx = -s.hx + x;
+                            q.dy = -s.hy + y;
+                            q.dz = -s.hz + iz;
+                            q.v = (iz * sy + y) * sx + x;
+                            q.o = q.dz * planep + q.dy * rowp + q.dx;
+                            voxel(q, Brick{}, bx, by, bz);
+                        }
+                    } else if (lane_map == LANES_PAIRS) {
                         // Planes in pairs: lane l < sx takes voxel x = l of row y in the first plane,
                         // lane l >= sx voxel x = l - sx of row y in the second. The brick's plane
                         // stride is sx modulo 64 banks, so every lane of a wave reads its own LDS bank.
@@ -1476,8 +1517,9 @@ void launch_fused(int variant, const IcgnArgs& a, const int64_t* list, const int
     // LDS brick: BRICK_XY^2 floats per plane, as many planes as the LDS budget holds (by default
     // three workgroups per CU). Pair mapping (stride 1, rows of 32..64 voxels): 39 rows per plane
     // and a plane stride of sx modulo 64 floats; otherwise 40 rows and a plane of 40 x 40.
-    const bool pair = !(flags & TUNE_NO_PAIRS) && a.sub.stride == 1 && a.sub.sx >= 32 && a.sub.sx <= 64 &&
-                      a.sub.sy >= NWAVES;
+    const bool patches = !(flags & TUNE_NO_PATCHES) && a.sub.stride == 1 && a.sub.sx >= 8 && a.sub.sy >= 8;
+    const bool pair = !patches && !(flags & TUNE_NO_PAIRS) && a.sub.stride == 1 && a.sub.sx >= 32 &&
+                      a.sub.sx <= 64 && a.sub.sy >= NWAVES;
     const int rows = pair ? BRICK_XY - 1 : BRICK_XY;
     const int plane = pair ? BRICK_XY * rows + ((a.sub.sx - BRICK_XY * rows) % 64 + 64) % 64 : BRICK_XY * rows;
     int planes = 0;
@@ -1501,7 +1543,7 @@ void launch_fused(int variant, const IcgnArgs& a, const int64_t* list, const int
     timer_begin(st);
     if (planes > 0) {
         hipLaunchKernelGGL(kern, dim3((unsigned)nb), dim3(BLOCK), lds, st, a, list, count, (const double*)mom, planes,
-                           plane, rows, pair ? 1 : 0);
+                           plane, rows, patches ? LANES_PATCHES : (pair ? LANES_PAIRS : LANES_WALK));
     } else if (buf) {
         hipLaunchKernelGGL((k_icgn12_fused<MODE, STORED, true, 0, SEL>), dim3((unsigned)nb), dim3(BLOCK), bits, st, a,
                            list, count, (const double*)mom, 0, 0, 0, 0);
@@ -1673,7 +1715,8 @@ int aldvc_icgn12_tuned(const int64_t* coords, int64_t N, double* P, int hx, int 
                        size_t workspace_bytes, hipStream_t stream) {
     clear_error();
     if (lds_budget < 0 || lds_budget > 160 * 1024) return fail(-2, "lds_budget must be 0 or at most 160 KiB");
-    if (flags & ~(TUNE_NO_PAIRS | TUNE_INDEX_ORDER | TUNE_NO_XCD)) return fail(-2, "unknown tuning flags");
+    if (flags & ~(TUNE_NO_PAIRS | TUNE_INDEX_ORDER | TUNE_NO_XCD | TUNE_NO_PATCHES))
+        return fail(-2, "unknown tuning flags");
     Volumes V{f, gx, gy, gz, g, mask, (int)nz, (int)ny, (int)nx};
     const Subset s = make_subset(hx, hy, hz, stride);
     if (int e = check_common(V, s, mode)) return e;
