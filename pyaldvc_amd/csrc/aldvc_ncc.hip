@@ -164,8 +164,8 @@ __global__ __launch_bounds__(BLOCK) void k_zncc_map_f32(NccArgs a) {
 // rounding, template centred by its mean), accumulation float32.
 //
 // Two kernels per call. k_box_var computes the window variance of every offset,
-// var = sum (W - c)^2 - (sum (W - c))^2 / n over the template-sized box, with exact float32
-// sliding sums (only the numerator sees bfloat16 rounding) and writes it into the map buffer.
+// var = sum (W - c)^2 - (sum (W - c))^2 / n over the template-sized box, with float32 sliding
+// sums (only the numerator sees bfloat16 rounding) and writes it into the map buffer.
 // k_zncc_map_bf16 then computes the numerator on MFMA and replaces var by the ZNCC in place. The
 // MFMA kernel thus only stages bfloat16 planes: one workgroup = 4 wave64 = 4 consecutive output
 // planes oz0 .. oz0 + 3 (one per wave) of one node; window plane z feeds wave g with template
@@ -181,7 +181,6 @@ typedef float f32x4_t __attribute__((ext_vector_type(4)));
 
 constexpr int MG = 4;     // output planes per workgroup of the MFMA kernel (one per wave)
 constexpr int TRING = 5;  // template plane slots: four in use and the one being loaded
-constexpr int VG = 4;     // output planes per workgroup of the box-sum kernel
 constexpr int PRE = 16;   // window values per lane prefetched for the next plane (57^2 / 256 <= 13)
 
 struct Bf16Geom {
@@ -234,17 +233,68 @@ __device__ inline bool ncc_node_stats(const NccArgs& a, const Bf16Geom& q, int64
     return ssf >= 1e-20;
 }
 
-// Window variance of every offset of VG consecutive output planes: per window plane, 2-D box
-// sums (x then y, sliding in runs of 8 outputs), added to the planes whose template range holds
-// the window plane. A thread owns the offsets tid + 256 k of each of the VG planes.
+// Window variance of every offset: one workgroup per node walks the window planes once. Per plane,
+// 2-D box sums (x then y, sliding in runs of 8 outputs) are added to running sums over the last sz
+// planes; the plane leaving the box (sz planes back) has its 2-D sums recomputed and subtracted,
+// and every plane from sz - 1 on completes one output plane. A thread owns the offsets
+// tid + 256 k of the plane.
 constexpr int VPT = 9;  // offsets per thread and plane: vy * vx <= 256 * VPT (radii up to 23)
+
+__device__ inline void box2d(const NccArgs& a, const Bf16Geom& q, int64_t node, int z, float c, float* Pf, float* R1,
+                             float* R2, float* P1, float* P2) {
+    const int tid = threadIdx.x;
+    const int64_t xb = a.wx0[node], yb = a.wy0[node], zb = a.wz0[node];
+    const int nseg_x = (q.vx + 7) / 8, nseg_y = (q.vy + 7) / 8;
+    __syncthreads();  // the previous plane's readers are done
+    const float* gp = a.g + ((zb + z) * a.ny + yb) * (int64_t)a.nx + xb;
+    for (int i = tid; i < q.wy * q.wx; i += BLOCK) {
+        const int y = i / q.wx, x = i - y * q.wx;
+        Pf[i] = gp[(int64_t)y * a.nx + x] - c;
+    }
+    __syncthreads();
+    for (int t = tid; t < q.wy * nseg_x; t += BLOCK) {  // row sums, 8 outputs per item
+        const int y = t / nseg_x, ox0 = 8 * (t - y * nseg_x);
+        const float* row = Pf + y * q.wx;
+        float r1 = 0.0f, r2 = 0.0f;
+        for (int x = 0; x < q.sx; ++x) {
+            const float v = row[ox0 + x];
+            r1 += v;
+            r2 += v * v;
+        }
+        R1[y * q.vx + ox0] = r1;
+        R2[y * q.vx + ox0] = r2;
+        for (int ox = ox0 + 1; ox < min(ox0 + 8, q.vx); ++ox) {
+            const float vin = row[ox + q.sx - 1], vout = row[ox - 1];
+            r1 += vin - vout;
+            r2 += vin * vin - vout * vout;
+            R1[y * q.vx + ox] = r1;
+            R2[y * q.vx + ox] = r2;
+        }
+    }
+    __syncthreads();
+    for (int t = tid; t < q.vx * nseg_y; t += BLOCK) {  // column sums, 8 outputs per item
+        const int ox = t / nseg_y, oy0 = 8 * (t - ox * nseg_y);
+        float r1 = 0.0f, r2 = 0.0f;
+        for (int y = 0; y < q.sy; ++y) {
+            r1 += R1[(oy0 + y) * q.vx + ox];
+            r2 += R2[(oy0 + y) * q.vx + ox];
+        }
+        P1[oy0 * q.vx + ox] = r1;
+        P2[oy0 * q.vx + ox] = r2;
+        for (int oy = oy0 + 1; oy < min(oy0 + 8, q.vy); ++oy) {
+            r1 += R1[(oy + q.sy - 1) * q.vx + ox] - R1[(oy - 1) * q.vx + ox];
+            r2 += R2[(oy + q.sy - 1) * q.vx + ox] - R2[(oy - 1) * q.vx + ox];
+            P1[oy * q.vx + ox] = r1;
+            P2[oy * q.vx + ox] = r2;
+        }
+    }
+    __syncthreads();
+}
 
 __global__ __launch_bounds__(BLOCK) void k_box_var(NccArgs a, Bf16Geom q) {
     extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
     const int tid = threadIdx.x;
-    const int n_zgrp = (q.vz + VG - 1) / VG;
-    const int64_t node = blockIdx.x / n_zgrp;
-    const int oz0 = (int)(blockIdx.x % n_zgrp) * VG;
+    const int64_t node = blockIdx.x;
     double mf, ssf;
     if (!ncc_node_stats(a, q, node, mf, ssf)) return;  // the MFMA kernel writes the -2 map
     const float c = (float)mf;
@@ -253,85 +303,40 @@ __global__ __launch_bounds__(BLOCK) void k_box_var(NccArgs a, Bf16Geom q) {
     float* R2 = R1 + q.wy * q.vx;
     float* P1 = R2 + q.wy * q.vx;  // vy x vx box sums
     float* P2 = P1 + q.vy * q.vx;
-    const int64_t xb = a.wx0[node], yb = a.wy0[node], zb = a.wz0[node];
     const int nplane = q.vy * q.vx;
-    float S1[VG][VPT], S2[VG][VPT];
+    const float inv_n = 1.0f / (float)(q.sx * q.sy * q.sz);
+    float* o = a.out + node * (int64_t)nplane * q.vz;
+    float S1[VPT], S2[VPT];
 #pragma unroll
-    for (int gq = 0; gq < VG; ++gq)
-#pragma unroll
-        for (int k = 0; k < VPT; ++k) S1[gq][k] = S2[gq][k] = 0.0f;
-    const int nseg_x = (q.vx + 7) / 8, nseg_y = (q.vy + 7) / 8;
-    const int z_end = min(oz0 + VG + q.sz - 1, q.wz);
-    for (int z = oz0; z < z_end; ++z) {
-        __syncthreads();
-        const float* gp = a.g + ((zb + z) * a.ny + yb) * (int64_t)a.nx + xb;
-        for (int i = tid; i < q.wy * q.wx; i += BLOCK) {
-            const int y = i / q.wx, x = i - y * q.wx;
-            Pf[i] = gp[(int64_t)y * a.nx + x] - c;
-        }
-        __syncthreads();
-        for (int t = tid; t < q.wy * nseg_x; t += BLOCK) {  // row sums, 8 outputs per item
-            const int y = t / nseg_x, ox0 = 8 * (t - y * nseg_x);
-            const float* row = Pf + y * q.wx;
-            float r1 = 0.0f, r2 = 0.0f;
-            for (int x = 0; x < q.sx; ++x) {
-                const float v = row[ox0 + x];
-                r1 += v;
-                r2 += v * v;
-            }
-            R1[y * q.vx + ox0] = r1;
-            R2[y * q.vx + ox0] = r2;
-            for (int ox = ox0 + 1; ox < min(ox0 + 8, q.vx); ++ox) {
-                const float vin = row[ox + q.sx - 1], vout = row[ox - 1];
-                r1 += vin - vout;
-                r2 += vin * vin - vout * vout;
-                R1[y * q.vx + ox] = r1;
-                R2[y * q.vx + ox] = r2;
-            }
-        }
-        __syncthreads();
-        for (int t = tid; t < q.vx * nseg_y; t += BLOCK) {  // column sums, 8 outputs per item
-            const int ox = t / nseg_y, oy0 = 8 * (t - ox * nseg_y);
-            float r1 = 0.0f, r2 = 0.0f;
-            for (int y = 0; y < q.sy; ++y) {
-                r1 += R1[(oy0 + y) * q.vx + ox];
-                r2 += R2[(oy0 + y) * q.vx + ox];
-            }
-            P1[oy0 * q.vx + ox] = r1;
-            P2[oy0 * q.vx + ox] = r2;
-            for (int oy = oy0 + 1; oy < min(oy0 + 8, q.vy); ++oy) {
-                r1 += R1[(oy + q.sy - 1) * q.vx + ox] - R1[(oy - 1) * q.vx + ox];
-                r2 += R2[(oy + q.sy - 1) * q.vx + ox] - R2[(oy - 1) * q.vx + ox];
-                P1[oy * q.vx + ox] = r1;
-                P2[oy * q.vx + ox] = r2;
-            }
-        }
-        __syncthreads();
+    for (int k = 0; k < VPT; ++k) S1[k] = S2[k] = 0.0f;
+    for (int z = 0; z < q.wz; ++z) {
+        box2d(a, q, node, z, c, Pf, R1, R2, P1, P2);
 #pragma unroll
         for (int k = 0; k < VPT; ++k) {
             const int i = tid + BLOCK * k;
-            const float p1 = i < nplane ? P1[i] : 0.0f, p2 = i < nplane ? P2[i] : 0.0f;
+            if (i < nplane) {
+                S1[k] += P1[i];
+                S2[k] += P2[i];
+            }
+        }
+        if (z >= q.sz) {  // the plane that left the box
+            box2d(a, q, node, z - q.sz, c, Pf, R1, R2, P1, P2);
 #pragma unroll
-            for (int gq = 0; gq < VG; ++gq) {
-                const int tz = z - (oz0 + gq);
-                if (tz >= 0 && tz < q.sz) {
-                    S1[gq][k] += p1;
-                    S2[gq][k] += p2;
+            for (int k = 0; k < VPT; ++k) {
+                const int i = tid + BLOCK * k;
+                if (i < nplane) {
+                    S1[k] -= P1[i];
+                    S2[k] -= P2[i];
                 }
             }
         }
-    }
-    const float inv_n = 1.0f / (float)(q.sx * q.sy * q.sz);
-    const int64_t n_off = (int64_t)q.vx * q.vy * q.vz;
+        if (z >= q.sz - 1) {
+            float* op = o + (int64_t)(z - q.sz + 1) * nplane;
 #pragma unroll
-    for (int gq = 0; gq < VG; ++gq) {
-        const int oz = oz0 + gq;
-        if (oz >= q.vz) break;
-        float* o = a.out + node * n_off + (int64_t)oz * nplane;
-#pragma unroll
-        for (int k = 0; k < VPT; ++k) {
-            const int i = tid + BLOCK * k;
-            if (i < nplane) o[i] = S2[gq][k] - S1[gq][k] * S1[gq][k] * inv_n;
+            for (int k = 0; k < VPT; ++k) {
+                const int i = tid + BLOCK * k;
+                if (i < nplane) op[i] = S2[k] - S1[k] * S1[k] * inv_n;
+            }
         }
     }
 }
@@ -682,7 +687,7 @@ int aldvc_zncc_map(const float* f, const float* g, int64_t nz, int64_t ny, int64
         if ((int64_t)q.sy * q.sx > (int64_t)BLOCK * 4) return fail(-2, "template plane too large for the prefetch");
         if ((int64_t)q.vy * q.vx > (int64_t)BLOCK * VPT) return fail(-2, "search plane too large for the box sums");
         const int64_t nblk = n * ((q.vz + MG - 1) / MG);
-        const int64_t nbox = n * ((q.vz + VG - 1) / VG);
+        const int64_t nbox = n;
         if (nblk > INT32_MAX) return fail(-2, "too many workgroups for one launch");
         const size_t blds = box_lds_bytes(q);
         if (blds > 64 * 1024 &&
