@@ -632,6 +632,7 @@ def main_admm(args):
     import torch
 
     from pyaldvc_amd import admm as AD
+    from pyaldvc_amd import autotune as AT
     from pyaldvc_amd import fem as FEM
     from pyaldvc_amd import kernels as K
     from pyaldvc_amd import local as LOC
@@ -700,7 +701,10 @@ def main_admm(args):
     beta = 1e-2 * float(np.mean(spacing)) ** 2 * para.mu  # one of the reference's beta sweep values
     system = FEM.build_global_system(ops, beta, para.mu, para.alpha, para)
     loop = AD.AdmmLoop(ctx, gm, mesh, system, para)
-    loop.load(U1, F1)
+    loop.load(U1, F1)  # also picks the 3-DOF launch shape (autotuner)
+    rep = dict(AT.last_report) if AT.last_report.get("key", "").startswith("icgn3|") else {}
+    timed = ", ".join(f"{k} {v * 1e3:.1f} ms" for k, v in rep.get("seconds", {}).items())
+    log(f"3-DOF launch shape: {loop.tuning3.label()}" + (f" ({timed})" if timed else " (cached)"))
     loop.run(graph=False)  # eager, also the warm-up
     t_eager = loop.time_s
     loop.load(U1, F1)
@@ -738,6 +742,7 @@ def main_admm(args):
                         "10.5 radius 200; cylinder mask minus 40 ellipsoidal voids; subsets 48x48x24 step 12; local "
                         "12-DOF (split subsets) + first global pass + 4 ADMM iterations in one HIP graph; cubic",
             "nodes": N, "nodes_valid": n_valid, "mask_fraction": float(mask.float().mean()),
+            "icgn3_launch_shape": loop.tuning3.label(),
             "local_s": t_local, "local_kernel_ms": t_kernel * 1e3, "local_converged": int(conv.sum()),
             "local_mean_iterations": float(n_iter[ctx.valid].mean()),
             "local_median_err_vox": float(np.median(err_local)),

@@ -132,6 +132,20 @@ int aldvc_icgn3_gated(const int64_t *coords, int64_t N, const double *U_old, dou
                       const uint8_t *split_keep, int64_t keep_bytes, int32_t *n_iter, int8_t *status, double *zncc,
                       const int *gate, void *workspace, size_t workspace_bytes, hipStream_t stream);
 
+/* aldvc_icgn3_gated (gate may be NULL) with the launch choices made explicit, as aldvc_icgn12_tuned:
+ * variant = 1: LDS bricks where a chunk's warped stencils fit one (subsets wider than a brick in
+ * x/y boxes), 2: memory gathers only; lds_budget and flags as aldvc_icgn12_tuned. The other two
+ * 3-DOF entries run variant 1 with lds_budget 0 and flags 0. Same reference function as aldvc_icgn3. */
+int aldvc_icgn3_tuned(const int64_t *coords, int64_t N, const double *U_old, double *U, const double *F_fixed,
+                      const double *vdual, int hx, int hy, int hz, int stride, const float *f, const float *gx,
+                      const float *gy, const float *gz, const uint8_t *mask, const float *g, int64_t nz,
+                      int64_t ny, int64_t nx, int mode, const double *H_all, const double *meanf,
+                      const double *bottomf, const uint8_t *valid, double mu, double tol, double dp_tol, int max_iter,
+                      int patience, double noise_gain, int predictive, const int64_t *split_index,
+                      const uint8_t *split_keep, int64_t keep_bytes, int32_t *n_iter, int8_t *status, double *zncc,
+                      const int *gate, int variant, int lds_budget, int flags, void *workspace,
+                      size_t workspace_bytes, hipStream_t stream);
+
 /* ZNCC of each valid node's subset under the warp P_all (NaN where not sampled).
  * Replaces solver/numba_kernels.py:1354-1387 (evaluate_zncc_parallel). */
 int aldvc_evaluate_zncc(const int64_t *coords, int64_t N, const double *P_all, int hx, int hy, int hz,

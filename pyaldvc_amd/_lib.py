@@ -82,6 +82,18 @@ SIGNATURES = {
             P, P, P, P, P, SZ, P,  # n_iter, status, zncc, gate, workspace, bytes, stream
         ],
     ),  # fmt: skip
+    "aldvc_icgn3_tuned": (
+        I32,
+        [
+            P, I64, P, P, P, P, I32, I32, I32, I32,  # coords, N, U_old, U, F, vdual, hx, hy, hz, stride
+            P, P, P, P, P, P, I64, I64, I64, I32,  # f, gx, gy, gz, mask, g, nz, ny, nx, mode
+            P, P, P, P,  # H, meanf, bottomf, valid
+            F64, F64, F64, I32, I32, F64, I32,  # mu, tol, dp_tol, max_iter, patience, noise_gain, predictive
+            P, P, I64,  # split
+            P, P, P, P,  # n_iter, status, zncc, gate
+            I32, I32, I32, P, SZ, P,  # variant, lds_budget, flags, workspace, bytes, stream
+        ],
+    ),  # fmt: skip
     "aldvc_evaluate_zncc": (
         I32,
         [P, I64, P, I32, I32, I32, I32, P, P, P, I64, I64, I64, I32, P, P, P],

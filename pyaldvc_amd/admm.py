@@ -30,6 +30,7 @@ from dataclasses import dataclass, field
 import numpy as np
 import torch
 
+from . import autotune as AT
 from . import kernels as K
 from ._lib import check, lib
 from .local import INTERP_MODE_BY_NAME, LocalSolveInfo
@@ -165,7 +166,9 @@ class AdmmLoop:
         self.mode = INTERP_MODE_BY_NAME[para.interp_method]
         self.pattern, self.gain = ctx.noise_args(para)
         sp = ctx.split_args()
+        self.split_args = sp
         self.split = K._split(K._Args(self.gd), N, sp.get("split_index"), sp.get("split_keep"))
+        self.tuning3 = None  # launch shape of the 3-DOF solves, chosen by the autotuner in load()
         if system.solver == "direct":
             self.pcg_tol, self.pcg_max = 1e-13, int(min(max(system.pcg_max_iter, 10 * ops.n_active), DIRECT_GRAPH_ITER))
         else:
@@ -205,8 +208,9 @@ class AdmmLoop:
         stored = vols["stored"]
         gx, gy, gz = (vols[k] if stored else None for k in ("gx", "gy", "gz"))
         si, sk, kb = self.split
+        t = self.tuning3 if self.tuning3 is not None else AT.DEFAULT
         check(
-            lib().aldvc_icgn3_gated(
+            lib().aldvc_icgn3_tuned(
                 K._ptr(d["coords"]), self.N, K._ptr(self.U2), K._ptr(self.U1raw), K._ptr(self.F2), K._ptr(self.v),
                 hx, hy, hz, int(ctx.stride), K._ptr(vols["f"]), K._ptr(gx), K._ptr(gy), K._ptr(gz),
                 K._ptr(vols["mask"]), K._ptr(self.gd), nz, ny, nx, self.mode, K._ptr(d["H"]), K._ptr(d["meanf"]),
@@ -214,9 +218,10 @@ class AdmmLoop:
                 float(p.icgn_dp_tol), int(p.icgn_max_iter), int(p.icgn_patience),
                 float(self.gain) if self.gain > 0 else 0.0, int(bool(p.icgn_predictive_stop)), K._ptr(si), K._ptr(sk),
                 kb, K._ptr(self.n_iter[step]), K._ptr(self.status[step]), K._ptr(self.zncc[step]), K._ptr(self.stop),
-                K._ptr(self.ws_icgn3), int(self.ws_icgn3.numel()), K.stream_handle(),
+                int(t.variant), int(t.lds_budget), int(t.flags), K._ptr(self.ws_icgn3), int(self.ws_icgn3.numel()),
+                K.stream_handle(),
             ),
-            "aldvc_icgn3_gated",
+            "aldvc_icgn3_tuned",
         )  # fmt: skip
         nz_, ny_, nx_ = self.grid
         check(
@@ -257,6 +262,22 @@ class AdmmLoop:
         self.zncc.zero_()
         self.n_iter.zero_()
         self.bad.zero_()
+        if self.tuning3 is None:
+            self.tuning3 = self._choose_tuning3()
+
+    def _choose_tuning3(self):
+        """The autotuner's launch shape for this loop's 3-DOF solves, timed (eagerly, before any
+        graph capture) on a node sample of the loaded state."""
+        ctx, p, d, vols = self.ctx, self.para, self.ctx.device, self.vols
+        stored = vols["stored"]
+        gx, gy, gz = (vols[k] if stored else None for k in ("gx", "gy", "gz"))
+        args = (d["coords"], self.U2, self.F2, self.v, *ctx.half, vols["f"], gx, gy, gz, vols["mask"], self.gd,
+                self.mode, d["H"], d["meanf"], d["bottomf"], d["valid"], float(self.system.mu), float(p.icgn_tol),
+                float(p.icgn_dp_tol), int(p.icgn_max_iter), int(p.icgn_patience))  # fmt: skip
+        kw = dict(stride=int(ctx.stride), noise_gain=float(self.gain) if self.gain > 0 else 0.0,
+                  predictive=bool(p.icgn_predictive_stop), split_index=self.split_args.get("split_index"),
+                  split_keep=self.split_args.get("split_keep"))  # fmt: skip
+        return AT.choose_icgn3(args, kw)
 
     def run(self, graph: bool = False):
         """Executes the loop on the current state (call ``load`` first). With ``graph`` the

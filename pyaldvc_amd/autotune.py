@@ -1,4 +1,4 @@
-"""Launch-shape autotuner of the fused 12-DOF IC-GN kernel.
+"""Launch-shape autotuner of the fused IC-GN kernel (12-DOF solves and translation-only solves).
 
 The fused kernel has launch choices that do not change what it computes, only how fast: the LDS
 budget of a workgroup's brick (three workgroups per CU with 7 planes, or two with 11), the
@@ -6,10 +6,10 @@ voxel-to-lane mapping of the brick path (8 x 8 patches per wave, plane pairs of 
 or sampling order), the node order (spatial tiles or index order), the XCD-aware workgroup ->
 node mapping, and bricks versus memory gathers. Which is fastest depends
 on the subset shape, the interpolation, the mask/split selection and the deformation (how many
-chunks fit a brick). ``choose_icgn12`` times every candidate on a contiguous run of the call's
+chunks fit a brick). ``choose_icgn12`` / ``choose_icgn3`` time every candidate on a contiguous run of the call's
 valid nodes (neighbouring nodes, so that subset overlap and L2 reuse are as in the full call;
 enough of them to fill the chip several times), best of three, and keeps the fastest unless the
-default is within 3 % of it; the choice is cached per (device, subset shape, stride,
+default is within 3 % of it; the choice is cached per (solve kind, device, subset shape, stride,
 interpolation, gradient storage, selection).
 The cache lives in memory and in a JSON file (``$PYALDVC_AMD_TUNE_CACHE``, default
 ``~/.cache/pyaldvc_amd/tune.json``), so a later process starts from the tuned shape.
@@ -112,9 +112,9 @@ def clear(memory_only: bool = False) -> None:
                 pass
 
 
-def _key(hx, hy, hz, stride, mode, stored, selected) -> str:
+def _key(kind, hx, hy, hz, stride, mode, stored, selected) -> str:
     name = torch.cuda.get_device_name(torch.cuda.current_device()).replace(" ", "_")
-    return f"icgn12|{name}|{hx}x{hy}x{hz}|s{stride}|m{mode}|{'stored' if stored else 'otf'}|{'sel' if selected else 'all'}"
+    return f"{kind}|{name}|{hx}x{hy}x{hz}|s{stride}|m{mode}|{'stored' if stored else 'otf'}|{'sel' if selected else 'all'}"
 
 
 def _sample(valid, n: int) -> torch.Tensor:
@@ -127,6 +127,35 @@ def _sample(valid, n: int) -> torch.Tensor:
     return idx[start : start + n]
 
 
+def _tune(key: str, nodes: int, run) -> Tuning:
+    """Times ``run(candidate)`` for every candidate (best of REPEATS) and caches the choice."""
+    times = {}
+    for cand in CANDIDATES:
+        best = float("inf")
+        for _ in range(REPEATS):  # the first run also loads the code object
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            run(cand)
+            torch.cuda.synchronize()
+            best = min(best, time.perf_counter() - t0)
+        times[cand.label()] = best
+    choice = min(CANDIDATES, key=lambda c: times[c.label()])
+    if times[choice.label()] > (1.0 - MARGIN) * times[DEFAULT.label()]:
+        choice = DEFAULT
+    with _lock:
+        _memo[key] = choice
+        _save()
+    last_report.clear()
+    last_report.update(key=key, nodes=nodes, seconds=times, choice=choice.label())
+    return choice
+
+
+def _cached(key: str):
+    with _lock:
+        _load()
+        return _memo.get(key)
+
+
 def choose_icgn12(args, kw) -> Tuning:
     """The tuned launch shape for ``kernels.icgn_12dof_parallel(*args, **kw)`` (see the module
     docstring); times the candidates on first use of a key."""
@@ -137,10 +166,8 @@ def choose_icgn12(args, kw) -> Tuning:
     fd = K._Args.vol(f)
     stored = gx is not None and tuple(gx.shape) == tuple(fd.shape)
     selected = (mask is not None and tuple(mask.shape) == tuple(fd.shape)) or kw.get("split_index") is not None
-    key = _key(int(hx), int(hy), int(hz), int(kw.get("stride", 1)), int(mode), stored, selected)
-    with _lock:
-        _load()
-        hit = _memo.get(key)
+    key = _key("icgn12", int(hx), int(hy), int(hz), int(kw.get("stride", 1)), int(mode), stored, selected)
+    hit = _cached(key)
     if hit is not None:
         return hit
 
@@ -157,23 +184,36 @@ def choose_icgn12(args, kw) -> Tuning:
         sub_kw["H_all"] = take(kw["H_all"], np.float64)
     if kw.get("split_index") is not None:
         sub_kw["split_index"] = take(kw["split_index"], np.int64)
+    return _tune(key, int(sel.numel()), lambda cand: K.icgn_12dof_parallel(*sub_args, **sub_kw, tuning=cand))
 
-    times = {}
-    for cand in CANDIDATES:
-        best = float("inf")
-        for _ in range(REPEATS):  # the first run also loads the code object
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            K.icgn_12dof_parallel(*sub_args, **sub_kw, tuning=cand)
-            torch.cuda.synchronize()
-            best = min(best, time.perf_counter() - t0)
-        times[cand.label()] = best
-    choice = min(CANDIDATES, key=lambda c: times[c.label()])
-    if times[choice.label()] > (1.0 - MARGIN) * times[DEFAULT.label()]:
-        choice = DEFAULT
-    with _lock:
-        _memo[key] = choice
-        _save()
-    last_report.clear()
-    last_report.update(key=key, nodes=int(sel.numel()), seconds=times, choice=choice.label())
-    return choice
+
+def choose_icgn3(args, kw) -> Tuning:
+    """The tuned launch shape for ``kernels.icgn_3dof_parallel(*args, **kw)``: the same candidates
+    as the 12-DOF solve, timed on the translation-only kernel (its few iterations per node weigh
+    brick staging against gathers differently), cached under its own key."""
+    from . import kernels as K
+
+    (coords, U_old, F_fixed, vdual, hx, hy, hz, f, gx, gy, gz, mask, g, mode, H_all, meanf, bottomf,
+     valid) = args[:18]  # fmt: skip
+    rest = args[18:]
+    fd = K._Args.vol(f)
+    stored = gx is not None and tuple(gx.shape) == tuple(fd.shape)
+    selected = (mask is not None and tuple(mask.shape) == tuple(fd.shape)) or kw.get("split_index") is not None
+    key = _key("icgn3", int(hx), int(hy), int(hz), int(kw.get("stride", 1)), int(mode), stored, selected)
+    hit = _cached(key)
+    if hit is not None:
+        return hit
+
+    A = K._Args(fd)
+    vd = A.arr(valid, np.uint8)
+    sel = _sample(vd, SAMPLE_NODES)
+    if sel.numel() == 0:
+        return DEFAULT
+    take = lambda x, dt: None if x is None else A.arr(x, dt)[sel].contiguous()  # noqa: E731
+    sub_args = (take(coords, np.int64), take(U_old, np.float64), take(F_fixed, np.float64), take(vdual, np.float64),
+                hx, hy, hz, fd, gx, gy, gz, mask, g, mode, take(H_all, np.float64), take(meanf, np.float64),
+                take(bottomf, np.float64), vd[sel].contiguous(), *rest)  # fmt: skip
+    sub_kw = dict(kw)
+    if kw.get("split_index") is not None:
+        sub_kw["split_index"] = take(kw["split_index"], np.int64)
+    return _tune(key, int(sel.numel()), lambda cand: K.icgn_3dof_parallel(*sub_args, **sub_kw, tuning=cand))

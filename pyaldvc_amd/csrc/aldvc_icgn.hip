@@ -46,6 +46,14 @@ struct IcgnArgs {
     int8_t* status;
     double* zncc;
     int xcd;  // 1: XCD-aware workgroup -> node mapping; 0: workgroup b takes list slot b
+    // translation-only solves (ADMM subproblem 1): fixed displacement gradient, previous and
+    // output displacements, dual variable, proximal weight
+    const double* F;      // (N, 9)
+    const double* U_old;  // (N, 3)
+    const double* vdual;  // (N, 3)
+    const double* bottomf;  // (N) reference subset norms of the precompute
+    double* U;            // (N, 3) out
+    double mu;
 };
 
 // Launch choices of the fused 12-DOF kernels that leave the results' meaning unchanged (the
@@ -161,6 +169,112 @@ __device__ int ctl12_step(Ctl12& c, double zncc, double bottomf_d, double n_vali
         }
     }
     return compose_warp(c.P, c.dP, c.work) ? -1 : ST_SINGULAR;
+}
+
+// Per-node state of a translation-only solve (ADMM subproblem 1, numba_kernels.py:931-1082):
+// P[0..8] the fixed displacement gradient, P[9..11] the translation; lane 0 only, in LDS.
+struct Ctl3 {
+    double P[12], Hinv[9];
+    double bf2, best_dn, last_dn, dn_prev, norm_init, zncc;
+    int stall;
+};
+
+// Loads the node's state; returns -1, or the status that ends the node before any iteration
+// (non-finite input, subset outside the volume, singular proximal Hessian: numba_kernels.py:962-982).
+__device__ inline int ctl3_init(Ctl3& c, const IcgnArgs& a, int64_t n, bool in_volume) {
+    bool finite = true;
+#pragma unroll 1
+    for (int k = 0; k < 9; ++k) {
+        c.P[k] = a.F[9 * n + k];
+        if (!isfinite(c.P[k])) finite = false;
+    }
+#pragma unroll 1
+    for (int k = 0; k < 3; ++k) {
+        c.P[9 + k] = a.U_old[3 * n + k];
+        if (!isfinite(c.P[9 + k])) finite = false;
+    }
+    c.bf2 = a.bottomf[n] * a.bottomf[n];
+    if (c.bf2 < 1e-30) c.bf2 = 1e-30;
+    c.best_dn = c.last_dn = 1e300;
+    c.dn_prev = c.norm_init = -1.0;
+    c.zncc = NAN;
+    c.stall = 0;
+    if (!finite) return ST_NAN;
+    if (!in_volume) return ST_INVALID_SUBSET;
+    if (!build_h3(a.H_all + 144 * n, c.bf2, a.mu, 0.0, c.Hinv)) return ST_SINGULAR;
+    return -1;
+}
+
+// One Gauss-Newton step of the translation with the proximal term mu (u - u_old - v)
+// (numba_kernels.py:1000-1075): gradient from the voxel-set moments (C = sum grad, A = sum grad f',
+// B = sum grad g' per component; centred f', g' statistics), convergence on the relative or absolute
+// gradient norm, noise-corrected Hessian once the steps are small, step-norm and stall rules, and
+// the inverse-compositional update u -= (I + F) dt. Returns -1 to continue or the final status.
+__device__ inline int ctl3_step(Ctl3& c, const IcgnArgs& a, int64_t n, double n_valid, double mf, double mg,
+                                double bf, double bg, double sfg, const double* C, const double* A, const double* B) {
+    const double ratio = bf / bg;
+    double tb[3], norm_abs = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double bk = (A[k] - mf * C[k]) - ratio * (B[k] - mg * C[k]);
+        tb[k] = bk * 2.0 / c.bf2 + a.mu * (c.P[9 + k] - a.U_old[3 * n + k] - a.vdual[3 * n + k]);
+        norm_abs += tb[k] * tb[k];
+    }
+    norm_abs = sqrt(norm_abs);
+    if (!isfinite(norm_abs)) {
+        c.zncc = NAN;
+        return ST_NAN;
+    }
+    if (c.norm_init < 0.0) c.norm_init = norm_abs;
+    const double norm_rel = c.norm_init > 1e-300 ? norm_abs / c.norm_init : 0.0;
+    c.zncc = (sfg - n_valid * mf * mg) / (bf * bg);
+    if (norm_rel < a.tol || norm_abs < a.mu * 1e-4) return ST_CONVERGED;
+    if (a.noise_gain > 0.0 && c.last_dn < NOISE_CORR_STEP) {
+        const double om = 1.0 - c.zncc;
+        const double s2 = (om > 0.0 ? om : 0.0) * bf * bf / n_valid;
+        const double* H = a.H_all + 144 * n;
+        if (!build_h3(H, c.bf2, a.mu, a.noise_gain * s2 * n_valid, c.Hinv)) build_h3(H, c.bf2, a.mu, 0.0, c.Hinv);
+    }
+    double dt[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) dt[i] = -(c.Hinv[3 * i] * tb[0] + c.Hinv[3 * i + 1] * tb[1] + c.Hinv[3 * i + 2] * tb[2]);
+    const double dn = sqrt(dt[0] * dt[0] + dt[1] * dt[1] + dt[2] * dt[2]);
+    if (!isfinite(dn)) {
+        c.zncc = NAN;
+        return ST_NAN;
+    }
+    c.last_dn = dn;
+    int ctrl = -1;
+    bool apply = true;
+    if (dn < a.dp_tol) {
+        ctrl = ST_CONVERGED;
+        apply = false;
+    } else if (a.predictive && c.dn_prev > 0.0 && dn < PREDICT_CONTRACTION * c.dn_prev && dn * dn < a.dp_tol * c.dn_prev) {
+        ctrl = ST_CONVERGED;  // the predicted remaining steps are below dp_tol: take this one and stop
+    } else {
+        c.dn_prev = dn;
+        if (dn < c.best_dn * STALL_STEP_DECAY) {
+            c.best_dn = dn;
+            c.stall = 0;
+        } else {
+            c.stall += 1;
+            if (a.patience > 0 && c.stall >= a.patience) {
+                ctrl = ST_STALLED;
+                apply = false;
+            }
+        }
+    }
+    if (apply) {
+        const double* P = c.P;
+        const double d0 = dt[0], d1 = dt[1], d2 = dt[2];
+        const double u0 = (1.0 + P[0]) * d0 + P[1] * d1 + P[2] * d2;
+        const double u1 = P[3] * d0 + (1.0 + P[4]) * d1 + P[5] * d2;
+        const double u2 = P[6] * d0 + P[7] * d1 + (1.0 + P[8]) * d2;
+        c.P[9] -= u0;
+        c.P[10] -= u1;
+        c.P[11] -= u2;
+    }
+    return ctrl;
 }
 
 // Warp coefficients broadcast to the sampling lanes: A = I + F (row-major) and the translation,
@@ -465,8 +579,9 @@ __global__ __launch_bounds__(BLOCK) void k_icgn12_plain(IcgnArgs a, const int64_
 // pg: the previous iteration's deformed mean) so that the float32 lane sums stay centred.
 constexpr int NR = 27;  // 0 n, 1 sum f', 2 sum f'^2, 3 + 4c + j: sum grad_c q_j, 15 + 4c + j: sum grad_c q_j f'
 constexpr int NG = 18;  // 0 n_valid, 1 oob, 2 n_nan, 3 sum g', 4 sum g'^2, 5 sum f'g', 6 + 4c + j: sum grad_c q_j g'
+constexpr int NG3 = 9;  // translation-only: 0..5 as NG, 6 + c: sum grad_c g'
 constexpr int MOM_STRIDE = 32;  // doubles per node in the moments workspace
-constexpr int BRICK_XY = 40;    // LDS brick rows and columns: subsets up to 33 voxels wide
+constexpr int BRICK_XY = 40;    // LDS brick rows and columns: one box for subsets up to 35 voxels wide
 constexpr int LDS_PER_WG = 54528;  // LDS budget of one fused workgroup: three per CU (160 KB)
 
 // XCD-aware slot of workgroup b over the M valid nodes: workgroup b runs on XCD b % 8, and each
@@ -547,16 +662,17 @@ __global__ __launch_bounds__(BLOCK) void k_ref_moments(IcgnArgs a, const int64_t
     for (int q = threadIdx.x; q < NR; q += BLOCK) mom[n * MOM_STRIDE + q] = res[q];
 }
 
-// Chunk plan of the LDS-brick path. Subset planes [zc, zc + kz) warp into the box spanned by the
-// warped positions of the chunk's 8 corners (warp_coord is monotone in X, Y and Z). When every
-// corner is inside the sampling domain, so is every sample of the chunk, and the brick holds the
-// stencils of all of them. Returns 0 when the chunk fits (origin and extents set), 1 when only its
-// z extent is too large (retry with fewer planes), 2 when it cannot use a brick.
+// Chunk plan of the LDS-brick path. The box of subset voxels [xc, xc + kx) x [yc, yc + ky) x
+// [zc, zc + kz) (sampling indices) warps into the box spanned by the warped positions of its 8
+// corners (warp_coord is monotone in X, Y and Z). When every corner is inside the sampling domain,
+// so is every sample of the chunk, and the brick holds the stencils of all of them. Returns 0 when
+// the chunk fits (origin and y, z extents set), 1 when only its z extent is too large (retry with
+// fewer planes), 2 when it cannot use a brick.
 template <int MODE, int BRK>
-__device__ inline int plan_chunk(const float* sw, const Subset& s, const Domain& dom, int zc, int kz, int planes,
-                                 int rows, int& bx, int& by, int& bz, int& ez) {
-    const float Xl = (float)(-s.hx), Xh = (float)(-s.hx + s.stride * (s.sx - 1));
-    const float Yl = (float)(-s.hy), Yh = (float)(-s.hy + s.stride * (s.sy - 1));
+__device__ inline int plan_chunk(const float* sw, const Subset& s, const Domain& dom, int xc, int kx, int yc, int ky,
+                                 int zc, int kz, int planes, int rows, int& bx, int& by, int& bz, int& ey, int& ez) {
+    const float Xl = (float)(-s.hx + s.stride * xc), Xh = (float)(-s.hx + s.stride * (xc + kx - 1));
+    const float Yl = (float)(-s.hy + s.stride * yc), Yh = (float)(-s.hy + s.stride * (yc + ky - 1));
     const float Zl = (float)(-s.hz + s.stride * zc), Zh = (float)(-s.hz + s.stride * (zc + kz - 1));
     float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
 #pragma unroll
@@ -576,7 +692,8 @@ __device__ inline int plan_chunk(const float* sw, const Subset& s, const Domain&
     bx = (int)floorf(lo[0]) + LO;
     by = (int)floorf(lo[1]) + LO;
     bz = (int)floorf(lo[2]) + LO;
-    const int ex = (int)floorf(hi[0]) + HI - bx + 1, ey = (int)floorf(hi[1]) + HI - by + 1;
+    const int ex = (int)floorf(hi[0]) + HI - bx + 1;
+    ey = (int)floorf(hi[1]) + HI - by + 1;
     ez = (int)floorf(hi[2]) + HI - bz + 1;
     if (ex > BRK || ey > rows) return 2;
     return ez > planes ? 1 : 0;
@@ -625,11 +742,14 @@ __device__ inline void stage_brick(float* brick, int bplane, int rows, const Buf
 }
 
 // SEL: a reference mask or split keep rows select the subset voxels (false: every voxel counts)
-template <int MODE, bool STORED, bool BUF, int BRK, bool SEL>
+// T3: translation-only solve of ADMM subproblem 1 (fixed F, proximal term; numba_kernels.py:931-1082)
+// on the same sampling paths, with the deformed-side moments restricted to q = 1.
+template <int MODE, bool STORED, bool BUF, int BRK, bool SEL, bool T3 = false>
 __global__ __launch_bounds__(BLOCK, 3) void k_icgn12_fused(IcgnArgs a, const int64_t* __restrict__ list,
                                                         const int64_t* __restrict__ count,
                                                         const double* __restrict__ mom, int brick_planes,
-                                                        int brick_plane, int brick_rows, int lane_map) {
+                                                        int brick_plane, int brick_rows, int lane_map, int tile_x,
+                                                        int tile_y) {
     static_assert(BRK == 0 || BUF, "the brick path stages through the buffer descriptor");
     const Volumes& V = a.V;
     const Subset& s = a.sub;
@@ -649,20 +769,26 @@ __global__ __launch_bounds__(BLOCK, 3) void k_icgn12_fused(IcgnArgs a, const int
     __shared__ double red[NWAVES * NR];
     __shared__ double res[NG];
     __shared__ double exc[NR];
-    __shared__ Ctl12 c;
+    __shared__ std::conditional_t<T3, Ctl3, Ctl12> c;
+    constexpr int NM = T3 ? NG3 : NG;  // deformed-side moments accumulated per lane
 
     double pg_next = 0.0;
     if (tid == 0) {
-        bool finite = true;
-        ctl12_init(c, a.P + 12 * n);
+        const bool in_volume = keep || subset_in_volume(V, s, x0, y0, z0);
+        if constexpr (T3) {
+            ctrl = ctl3_init(c, a, n, in_volume);
+        } else {
+            bool finite = true;
+            ctl12_init(c, a.P + 12 * n);
 #pragma unroll 1
-        for (int k = 0; k < 12; ++k)
-            if (!isfinite(c.P[k])) finite = false;
-        ctrl = -1;
-        if (!finite)
-            ctrl = ST_NAN;
-        else if (!keep && !subset_in_volume(V, s, x0, y0, z0))
-            ctrl = ST_INVALID_SUBSET;
+            for (int k = 0; k < 12; ++k)
+                if (!isfinite(c.P[k])) finite = false;
+            ctrl = -1;
+            if (!finite)
+                ctrl = ST_NAN;
+            else if (!in_volume)
+                ctrl = ST_INVALID_SUBSET;
+        }
         double pf = a.meanf ? a.meanf[n] : 0.0;
         if (!isfinite(pf)) pf = 0.0;
         sw[12] = (float)pf;
@@ -688,9 +814,9 @@ __global__ __launch_bounds__(BLOCK, 3) void k_icgn12_fused(IcgnArgs a, const int
         for (int i = tid; i < nwords; i += BLOCK) nanbits[i] = 0u;
         __syncthreads();
         const float pf = sw[12], pg = sw[13];
-        float m[NG];
+        float m[NM];
 #pragma unroll
-        for (int q = 0; q < NG; ++q) m[q] = 0.0f;
+        for (int q = 0; q < NM; ++q) m[q] = 0.0f;
         // one subset voxel: reference-side values first (they do not depend on the warp, so their
         // loads overlap the gather of the deformed sample), then the warp, the sample and the
         // deformed-side moments. `FROM_BRICK`: every sample of the chunk is known to be inside.
@@ -741,148 +867,210 @@ __global__ __launch_bounds__(BLOCK, 3) void k_icgn12_fused(IcgnArgs a, const int
             m[3] += gv;
             m[4] += gv * gv;
             m[5] += fv * gv;
+            if constexpr (T3) {
 #pragma unroll
-            for (int cc = 0; cc < 3; ++cc) {
-                const float tg = gr[cc] * gv;
+                for (int cc = 0; cc < 3; ++cc) m[6 + cc] += gr[cc] * gv;
+            } else {
 #pragma unroll
-                for (int j = 0; j < 4; ++j) m[6 + 4 * cc + j] += tg * q[j];
+                for (int cc = 0; cc < 3; ++cc) {
+                    const float tg = gr[cc] * gv;
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) m[6 + 4 * cc + j] += tg * q[j];
+                }
             }
             if (!ok) atomicOr(&nanbits[w.v >> 5], 1u << (w.v & 31));
         };
         using Gather = std::integral_constant<bool, false>;
         if constexpr (BRK > 0) {
-            // subset planes in chunks whose warped stencils fit the brick; a chunk that does not
-            // fit (large rotation or shear, or a sample outside the domain) gathers from memory
+            // The subset in boxes of tile_x x tile_y voxel columns (one box when its rows fit the
+            // brick), each box in chunks of planes whose warped stencils fit the brick; a chunk
+            // that does not fit (large rotation or shear, or a sample outside the domain) gathers
+            // from memory.
             constexpr int OVER = MODE == INTERP_LINEAR ? 1 : 3;
-            const int plane_vox = s.sx * s.sy;
-            for (int zc = 0; zc < s.sz;) {
-                int kz = min(brick_planes - OVER, s.sz - zc), bx = 0, by = 0, bz = 0, ez = 0;
-                int fit = plan_chunk<MODE, BRK>(sw, s, dom, zc, kz, brick_planes, brick_rows, bx, by, bz, ez);
-                while (fit == 1 && kz > 1)
-                    fit = plan_chunk<MODE, BRK>(sw, s, dom, zc, --kz, brick_planes, brick_rows, bx, by, bz, ez);
-                const int vend = (zc + kz) * plane_vox;
-                OffsetWalk w;
-                w.start(s, zc * plane_vox + tid, rowp, planep);
-                if (fit == 0) {
-                    __syncthreads();  // the previous chunk's samples are done with the brick
-                    stage_brick<BRK>(brick, brick_plane, brick_rows, gb, (int)obase, rowp, planep, bx, by, bz, ez);
-                    __syncthreads();
-                    using Brick = std::integral_constant<bool, true>;
-                    if (lane_map == LANES_PATCHES) {
-                        // 8 x 8 patches of one plane per wave (lane l: x + (l & 7), y + (l >> 3)). With a
-                        // brick row stride of 40 floats the 8 rows of a patch start 40 y mod 64 = 8
-                        // different bank groups apart, so the 64 lanes read 64 different banks, and a
-                        // small rotation or shear only shifts whole patch rows by a bank or two. The
-                        // columns and rows beyond the last whole patch follow in a short tail.
-                        const int lane = tid & 63, sx = s.sx, sy = s.sy;
-                        const int pxn = sx >> 3, pyn = sy >> 3, pp = pxn * pyn, npatch = kz * pp;
-                        for (int pi = tid >> 6; pi < npatch; pi += NWAVES) {
-                            const int pz = pi / pp, r = pi - pz * pp, pyi = r / pxn, pxi = r - pyi * pxn;
-                            const int iz = zc + pz, x = 8 * pxi + (lane & 7), y = 8 * pyi + (lane >> 3);
-                            Vox q;
-                            q.dx = -s.hx + x;
-                            q.dy = -s.hy + y;
-                            q.dz = -s.hz + iz;
-                            q.v = (iz * sy + y) * sx + x;
-                            q.o = q.dz * planep + q.dy * rowp + q.dx;
-                            voxel(q, Brick{}, bx, by, bz);
-                        }
-                        const int tx = sx - 8 * pxn, wxp = 8 * pxn, per_plane = tx * sy + (sy - 8 * pyn) * wxp;
-                        for (int t = tid; t < kz * per_plane; t += BLOCK) {
-                            const int pz = t / per_plane, r = t - pz * per_plane;
-                            int x, y;
-                            if (r < tx * sy) {
-                                y = r / tx;
-                                x = wxp + (r - y * tx);
-                            } else {
-                                const int r2 = r - tx * sy, yy = r2 / wxp;
-                                y = 8 * pyn + yy;
-                                x = r2 - yy * wxp;
-                            }
-                            const int iz = zc + pz;
-                            Vox q;
-                            q.dx = -s.hx + x;
-                            q.dy = -s.hy + y;
-                            q.dz = -s.hz + iz;
-                            q.v = (iz * sy + y) * sx + x;
-                            q.o = q.dz * planep + q.dy * rowp + q.dx;
-                            voxel(q, Brick{}, bx, by, bz);
-                        }
-                    } else if (lane_map == LANES_PAIRS) {
-                        // Planes in pairs: lane l < sx takes voxel x = l of row y in the first plane,
-                        // lane l >= sx voxel x = l - sx of row y in the second. The brick's plane
-                        // stride is sx modulo 64 banks, so every lane of a wave reads its own LDS bank.
-                        // The second plane's last 2 sx - 64 voxels of each row, and a last unpaired
-                        // plane, follow.
-                        const int lane = tid & 63, sx = s.sx, sy = s.sy;
-                        const int inB = lane >= sx ? 1 : 0, xl = lane - inB * sx;
-                        const int npair = kz >> 1;
-                        int p = 0, y = tid >> 6;
-                        while (y >= sy) {
-                            y -= sy;
-                            ++p;
-                        }
-                        Vox q;
-                        q.dx = -s.hx + xl;
-                        q.dy = -s.hy + y;
-                        q.dz = -s.hz + zc + 2 * p + inB;
-                        q.v = ((zc + 2 * p + inB) * sy + y) * sx + xl;
-                        q.o = q.dz * planep + q.dy * rowp + q.dx;
-                        const int wrap_v = sy * sx, wrap_o = 2 * planep - sy * rowp;
-                        while (p < npair) {
-                            voxel(q, Brick{}, bx, by, bz);
-                            y += NWAVES;
-                            q.dy += NWAVES;
-                            q.v += NWAVES * sx;
-                            q.o += NWAVES * rowp;
-                            if (y >= sy) {
-                                y -= sy;
-                                ++p;
-                                q.dy -= sy;
-                                q.dz += 2;
-                                q.v += wrap_v;
-                                q.o += wrap_o;
-                            }
-                        }
-                        const int nt = 2 * sx - 64, ntail = npair * sy * nt;
-                        for (int t = tid; t < ntail; t += BLOCK) {
-                            const int r = t / nt, k = t - r * nt, pp = r / sy, yy = r - pp * sy;
-                            const int iz = zc + 2 * pp + 1, xx = 64 - sx + k;
-                            Vox e;
-                            e.dx = -s.hx + xx;
-                            e.dy = -s.hy + yy;
-                            e.dz = -s.hz + iz;
-                            e.v = (iz * sy + yy) * sx + xx;
-                            e.o = e.dz * planep + e.dy * rowp + e.dx;
-                            voxel(e, Brick{}, bx, by, bz);
-                        }
-                        if (kz & 1) {
-                            const int iz = zc + kz - 1;
-                            for (int u = tid; u < sy * sx; u += BLOCK) {
-                                const int yy = u / sx, xx = u - yy * sx;
-                                Vox e;
-                                e.dx = -s.hx + xx;
-                                e.dy = -s.hy + yy;
-                                e.dz = -s.hz + iz;
-                                e.v = (iz * sy + yy) * sx + xx;
-                                e.o = e.dz * planep + e.dy * rowp + e.dx;
-                                voxel(e, Brick{}, bx, by, bz);
-                            }
-                        }
-                    } else {
-                        for (; w.v < vend; w.next(s)) voxel(w, Brick{}, bx, by, bz);
-                    }
-                } else {
-                    for (; w.v < vend; w.next(s)) voxel(w, Gather{}, 0, 0, 0);
+            const int sx = s.sx, sy = s.sy, plane_vox = sx * sy, lane = tid & 63;
+            // the voxels of box [xc, xc + kx) x [yc, yc + ky) x [zc, zc + kz), t = tid, tid + BLOCK,
+            // ... with x fastest, decoded per voxel (box chunks that gather, boxes of other lane maps)
+            auto box_voxels = [&](auto from_brick, int xc, int kx, int yc, int ky, int zc, int kz, int bx, int by,
+                                  int bz) {
+                const int per_plane = kx * ky;
+                for (int t = tid; t < kz * per_plane; t += BLOCK) {
+                    const int pz = t / per_plane, r = t - pz * per_plane, py = r / kx;
+                    const int ix = xc + r - py * kx, iy = yc + py, iz = zc + pz;
+                    Vox q;
+                    q.dx = -s.hx + s.stride * ix;
+                    q.dy = -s.hy + s.stride * iy;
+                    q.dz = -s.hz + s.stride * iz;
+                    q.v = (iz * sy + iy) * sx + ix;
+                    q.o = q.dz * planep + q.dy * rowp + q.dx;
+                    voxel(q, from_brick, bx, by, bz);
                 }
-                zc += kz;
+            };
+            for (int yc = 0; yc < sy; yc += tile_y) {
+                for (int xc = 0; xc < sx; xc += tile_x) {
+                    const int kx = min(tile_x, sx - xc), ky = min(tile_y, sy - yc);
+                    const bool whole = kx == sx && ky == sy;
+                    for (int zc = 0; zc < s.sz;) {
+                        int kz = min(brick_planes - OVER, s.sz - zc), bx = 0, by = 0, bz = 0, ey = 0, ez = 0;
+                        int fit = plan_chunk<MODE, BRK>(sw, s, dom, xc, kx, yc, ky, zc, kz, brick_planes, brick_rows,
+                                                        bx, by, bz, ey, ez);
+                        while (fit == 1 && kz > 1)
+                            fit = plan_chunk<MODE, BRK>(sw, s, dom, xc, kx, yc, ky, zc, --kz, brick_planes,
+                                                        brick_rows, bx, by, bz, ey, ez);
+                        using Brick = std::integral_constant<bool, true>;
+                        if (fit == 0) {
+                            __syncthreads();  // the previous chunk's samples are done with the brick
+                            stage_brick<BRK>(brick, brick_plane, ey, gb, (int)obase, rowp, planep, bx, by, bz, ez);
+                            __syncthreads();
+                            if (lane_map == LANES_PATCHES) {
+                                // 8 x 8 patches of one plane per wave (lane l: x + (l & 7), y + (l >> 3)).
+                                // With a brick row stride of 40 floats the 8 rows of a patch start
+                                // 40 y mod 64 = 8 different bank groups apart, so the 64 lanes read 64
+                                // different banks, and a small rotation or shear only shifts whole patch
+                                // rows by a bank or two. The patch counters are wave-uniform and advance
+                                // by NWAVES patches with carries (scalar work, no division); a lane adds
+                                // its fixed offsets within the patch. The box's columns and rows beyond
+                                // its last whole patch follow in a short tail.
+                                const int pxn = kx >> 3, pyn = ky >> 3;
+                                if (pxn > 0 && pyn > 0) {
+                                    const int lx = lane & 7, ly = lane >> 3;
+                                    const int lane_v = ly * sx + lx, lane_o = ly * rowp + lx;
+                                    int pxi = __builtin_amdgcn_readfirstlane(tid >> 6), pyi = 0, pz = 0;
+                                    while (pxi >= pxn) {
+                                        pxi -= pxn;
+                                        ++pyi;
+                                    }
+                                    while (pyi >= pyn) {
+                                        pyi -= pyn;
+                                        ++pz;
+                                    }
+                                    while (pz < kz) {
+                                        const int iz = zc + pz, x = xc + 8 * pxi, y = yc + 8 * pyi;
+                                        const int ux = -s.hx + x, uy = -s.hy + y, uz = -s.hz + iz;
+                                        Vox q;
+                                        q.dx = ux + lx;
+                                        q.dy = uy + ly;
+                                        q.dz = uz;
+                                        q.v = (iz * sy + y) * sx + x + lane_v;
+                                        q.o = uz * planep + uy * rowp + ux + lane_o;
+                                        voxel(q, Brick{}, bx, by, bz);
+                                        pxi += NWAVES;
+                                        while (pxi >= pxn) {
+                                            pxi -= pxn;
+                                            ++pyi;
+                                        }
+                                        while (pyi >= pyn) {
+                                            pyi -= pyn;
+                                            ++pz;
+                                        }
+                                    }
+                                }
+                                const int tx = kx - 8 * pxn, wxp = 8 * pxn, per_plane = tx * ky + (ky - 8 * pyn) * wxp;
+                                for (int t = tid; t < kz * per_plane; t += BLOCK) {
+                                    const int pz = t / per_plane, r = t - pz * per_plane;
+                                    int x, y;
+                                    if (r < tx * ky) {
+                                        y = r / tx;
+                                        x = wxp + (r - y * tx);
+                                    } else {
+                                        const int r2 = r - tx * ky, yy = r2 / wxp;
+                                        y = 8 * pyn + yy;
+                                        x = r2 - yy * wxp;
+                                    }
+                                    const int ix = xc + x, iy = yc + y, iz = zc + pz;
+                                    Vox q;
+                                    q.dx = -s.hx + ix;
+                                    q.dy = -s.hy + iy;
+                                    q.dz = -s.hz + iz;
+                                    q.v = (iz * sy + iy) * sx + ix;
+                                    q.o = q.dz * planep + q.dy * rowp + q.dx;
+                                    voxel(q, Brick{}, bx, by, bz);
+                                }
+                            } else if (whole && lane_map == LANES_PAIRS) {
+                                // Planes in pairs: lane l < sx takes voxel x = l of row y in the first plane,
+                                // lane l >= sx voxel x = l - sx of row y in the second. The brick's plane
+                                // stride is sx modulo 64 banks, so every lane of a wave reads its own LDS bank.
+                                // The second plane's last 2 sx - 64 voxels of each row, and a last unpaired
+                                // plane, follow.
+                                                        const int inB = lane >= sx ? 1 : 0, xl = lane - inB * sx;
+                                const int npair = kz >> 1;
+                                int p = 0, y = tid >> 6;
+                                while (y >= sy) {
+                                    y -= sy;
+                                    ++p;
+                                }
+                                Vox q;
+                                q.dx = -s.hx + xl;
+                                q.dy = -s.hy + y;
+                                q.dz = -s.hz + zc + 2 * p + inB;
+                                q.v = ((zc + 2 * p + inB) * sy + y) * sx + xl;
+                                q.o = q.dz * planep + q.dy * rowp + q.dx;
+                                const int wrap_v = sy * sx, wrap_o = 2 * planep - sy * rowp;
+                                while (p < npair) {
+                                    voxel(q, Brick{}, bx, by, bz);
+                                    y += NWAVES;
+                                    q.dy += NWAVES;
+                                    q.v += NWAVES * sx;
+                                    q.o += NWAVES * rowp;
+                                    if (y >= sy) {
+                                        y -= sy;
+                                        ++p;
+                                        q.dy -= sy;
+                                        q.dz += 2;
+                                        q.v += wrap_v;
+                                        q.o += wrap_o;
+                                    }
+                                }
+                                const int nt = 2 * sx - 64, ntail = npair * sy * nt;
+                                for (int t = tid; t < ntail; t += BLOCK) {
+                                    const int r = t / nt, k = t - r * nt, pp = r / sy, yy = r - pp * sy;
+                                    const int iz = zc + 2 * pp + 1, xx = 64 - sx + k;
+                                    Vox e;
+                                    e.dx = -s.hx + xx;
+                                    e.dy = -s.hy + yy;
+                                    e.dz = -s.hz + iz;
+                                    e.v = (iz * sy + yy) * sx + xx;
+                                    e.o = e.dz * planep + e.dy * rowp + e.dx;
+                                    voxel(e, Brick{}, bx, by, bz);
+                                }
+                                if (kz & 1) {
+                                    const int iz = zc + kz - 1;
+                                    for (int u = tid; u < sy * sx; u += BLOCK) {
+                                        const int yy = u / sx, xx = u - yy * sx;
+                                        Vox e;
+                                        e.dx = -s.hx + xx;
+                                        e.dy = -s.hy + yy;
+                                        e.dz = -s.hz + iz;
+                                        e.v = (iz * sy + yy) * sx + xx;
+                                        e.o = e.dz * planep + e.dy * rowp + e.dx;
+                                        voxel(e, Brick{}, bx, by, bz);
+                                    }
+                                }
+                            } else if (whole) {
+                                OffsetWalk w;
+                                w.start(s, zc * plane_vox + tid, rowp, planep);
+                                const int vend = (zc + kz) * plane_vox;
+                                for (; w.v < vend; w.next(s)) voxel(w, Brick{}, bx, by, bz);
+                            } else {
+                                box_voxels(Brick{}, xc, kx, yc, ky, zc, kz, bx, by, bz);
+                            }
+                        } else if (whole) {
+                            OffsetWalk w;
+                            w.start(s, zc * plane_vox + tid, rowp, planep);
+                            const int vend = (zc + kz) * plane_vox;
+                            for (; w.v < vend; w.next(s)) voxel(w, Gather{}, 0, 0, 0);
+                        } else {
+                            box_voxels(Gather{}, xc, kx, yc, ky, zc, kz, 0, 0, 0);
+                        }
+                        zc += kz;
+                    }
+                }
             }
         } else {
             OffsetWalk w;
             w.start(s, tid, rowp, planep);
             for (; w.v < s.S; w.next(s)) voxel(w, Gather{}, 0, 0, 0);
         }
-        block_sum<NG, NWAVES>(m, res, red);
+        block_sum<NM, NWAVES>(m, res, red);
         const bool correct = res[2] > 0.0 && !(res[1] > 0.0);  // block-uniform
         if (correct) {
             ref_moments<STORED>(
@@ -910,17 +1098,23 @@ __global__ __launch_bounds__(BLOCK, 3) void k_icgn12_fused(IcgnArgs a, const int
                 ssf = ssf < 0.0 ? 0.0 : ssf;
                 ssg = ssg < 0.0 ? 0.0 : ssg;
                 const double bf = sqrt(ssf > 1e-30 ? ssf : 1e-30), bg = sqrt(ssg > 1e-30 ? ssg : 1e-30);
-                const double zncc = (res[5] - n_valid * mf * mg) / (bf * bg);
-                const double ratio = bf / bg;
+                if constexpr (T3) {
+                    // q = 1 columns of the reference moments: C_c = R(6 + 4c), A_c = R(18 + 4c)
+                    const double C[3] = {R(6), R(10), R(14)}, A[3] = {R(18), R(22), R(26)};
+                    ctrl = ctl3_step(c, a, n, n_valid, mf, mg, bf, bg, res[5], C, A, res + 6);
+                } else {
+                    const double zncc = (res[5] - n_valid * mf * mg) / (bf * bg);
+                    const double ratio = bf / bg;
 #pragma unroll 1
-                for (int q = 0; q < 12; ++q) {
-                    const int cc = q >> 2, j = q & 3;
-                    const double C = R(3 + q), A = R(15 + q), B = res[6 + q];
-                    c.b[j < 3 ? 3 * cc + j : 9 + cc] = (A - mf * C) - ratio * (B - mg * C);
+                    for (int q = 0; q < 12; ++q) {
+                        const int cc = q >> 2, j = q & 3;
+                        const double C = R(3 + q), A = R(15 + q), B = res[6 + q];
+                        c.b[j < 3 ? 3 * cc + j : 9 + cc] = (A - mf * C) - ratio * (B - mg * C);
+                    }
+                    const double* Hn = a.H_all ? a.H_all + 144 * n : nullptr;
+                    ctrl = ctl12_step(c, zncc, bf, n_valid, a.L_all + 144 * n, Hn, a.pattern, a.noise_gain, a.tol,
+                                      a.dp_tol, a.patience, a.predictive, half_scale);
                 }
-                const double* Hn = a.H_all ? a.H_all + 144 * n : nullptr;
-                ctrl = ctl12_step(c, zncc, bf, n_valid, a.L_all + 144 * n, Hn, a.pattern, a.noise_gain, a.tol,
-                                  a.dp_tol, a.patience, a.predictive, half_scale);
                 pg_next = (double)pg + mg;
             }
         }
@@ -928,248 +1122,14 @@ __global__ __launch_bounds__(BLOCK, 3) void k_icgn12_fused(IcgnArgs a, const int
     }
     if (tid == 0) {
         if (ctrl >= 0) status = ctrl;
-        for (int k = 0; k < 12; ++k) a.P[12 * n + k] = c.P[k];
+        if constexpr (T3) {
+            for (int k = 0; k < 3; ++k) a.U[3 * n + k] = c.P[9 + k];
+        } else {
+            for (int k = 0; k < 12; ++k) a.P[12 * n + k] = c.P[k];
+        }
         a.n_iter[n] = it_done;
         a.status[n] = (int8_t)status;
         a.zncc[n] = (status == ST_CONVERGED || status == ST_STALLED || status == ST_MAX_ITER) ? c.zncc : NAN;
-    }
-}
-
-// ---------------------------------------------------------------------------- 3-DOF kernel
-struct Icgn3Args {
-    Volumes V;
-    Subset sub;
-    const int64_t* coords;
-    const double* U_old;  // (N, 3)
-    double* U;            // (N, 3) out
-    const double* F;      // (N, 9)
-    const double* vdual;  // (N, 3)
-    const double* H_all;
-    const double* bottomf;
-    const double* meanf;
-    const int64_t* split_index;
-    const uint8_t* split_keep;
-    int64_t keep_bytes;
-    double mu, tol, dp_tol, noise_gain;
-    int max_iter, patience, predictive;
-    int32_t* n_iter;
-    int8_t* status;
-    double* zncc;
-};
-
-// ADMM subproblem 1: translation-only IC-GN with a proximal term (numba_kernels.py:931-1082),
-// one pass per iteration with the moments of the fused 12-DOF kernel restricted to q = 1.
-template <int MODE, bool STORED>
-__global__ __launch_bounds__(BLOCK) void k_icgn3_fused(Icgn3Args a, const int64_t* __restrict__ list,
-                                                       const int64_t* __restrict__ count) {
-    const Volumes& V = a.V;
-    const Subset& s = a.sub;
-    const int tid = threadIdx.x;
-    const int64_t M = *count;
-    const int64_t b = blockIdx.x, per = (M + 7) / 8;
-    if ((b >> 3) >= per) return;
-    const int64_t slot = (b & 7) * per + (b >> 3);
-    if (slot >= M) return;
-    const int64_t n = list[slot];
-    const int64_t x0 = a.coords[3 * n], y0 = a.coords[3 * n + 1], z0 = a.coords[3 * n + 2];
-    const uint8_t* keep = (a.split_index && a.split_keep && a.split_index[n] >= 0)
-                              ? a.split_keep + a.split_index[n] * a.keep_bytes
-                              : nullptr;
-    constexpr int NQ = 17;  // n, n_ref, oob, f', f'^2, g', g'^2, f'g', C(3), A(3), B(3)
-    __shared__ float sw[14];
-    __shared__ int ctrl;
-    __shared__ double red[NWAVES * NQ];
-    __shared__ double res[NQ];
-    __shared__ double P[12], Hinv[9];  // control-lane state kept out of the sampling lanes' registers
-    double bf2 = 1.0, best_dn = 1e300, last_dn = 1e300, dn_prev = -1.0, norm_init = -1.0, zncc = NAN,
-           pg_next = 0.0;
-    int stall = 0;
-    const double* H = a.H_all + 144 * n;
-    if (tid == 0) {
-        bool finite = true;
-        for (int k = 0; k < 9; ++k) {
-            P[k] = a.F[9 * n + k];
-            if (!isfinite(P[k])) finite = false;
-        }
-        for (int k = 0; k < 3; ++k) {
-            P[9 + k] = a.U_old[3 * n + k];
-            if (!isfinite(P[9 + k])) finite = false;
-        }
-        ctrl = -1;
-        bf2 = a.bottomf[n] * a.bottomf[n];
-        if (bf2 < 1e-30) bf2 = 1e-30;
-        if (!finite)
-            ctrl = ST_NAN;
-        else if (!keep && !subset_in_volume(V, s, x0, y0, z0))
-            ctrl = ST_INVALID_SUBSET;
-        else if (!build_h3(H, bf2, a.mu, 0.0, Hinv))
-            ctrl = ST_SINGULAR;  // before any iteration: n_iter stays 0 (numba_kernels.py:980-982)
-        double pf = a.meanf ? a.meanf[n] : 0.0;
-        if (!isfinite(pf)) pf = 0.0;
-        sw[12] = (float)pf;
-        pg_next = pf;
-        put_warp(sw, P);
-    }
-    __syncthreads();
-    const Domain dom = make_domain(V, (int)x0, (int)y0, (int)z0);
-    const float* gbase = V.g + lin(V, z0, y0, x0);
-    const int64_t obase = lin(V, z0, y0, x0);
-    const int rowp = V.nx, planep = V.nx * V.ny;
-    int it_done = 0, status = ST_MAX_ITER;
-    for (int it = 1; it <= a.max_iter; ++it) {
-        if (ctrl >= 0) break;
-        it_done = it;
-        if (tid == 0) {
-            sw[9] = (float)P[9];
-            sw[10] = (float)P[10];
-            sw[11] = (float)P[11];
-            sw[13] = (float)pg_next;
-        }
-        __syncthreads();
-        const float pf = sw[12], pg = sw[13];
-        float m[NQ];
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) m[q] = 0.0f;
-        VoxelWalk w;
-        w.start(s, tid);
-        for (; w.v < s.S; w.next(s)) {
-            const int dx = -s.hx + s.stride * w.ix, dy = -s.hy + s.stride * w.iy, dz = -s.hz + s.stride * w.iz;
-            const int64_t o = obase + (int64_t)dz * planep + (int64_t)dy * rowp + dx;
-            bool k;
-            if (keep) {
-                k = kept_bit(keep, w.v);
-                const int64_t xx = x0 + dx, yy = y0 + dy, zz = z0 + dz;
-                k = k && xx >= 0 && xx < V.nx && yy >= 0 && yy < V.ny && zz >= 0 && zz < V.nz;
-            } else {
-                k = !V.mask || V.mask[o] != 0;
-            }
-            if (!k) continue;
-            // reference-side values first: they do not depend on the warp, so their loads
-            // overlap the gather of the deformed sample
-            const float fraw = V.f[o];
-            float gr[3];
-            grad_at<STORED>(V, o, (int)z0 + dz, (int)y0 + dy, (int)x0 + dx, gr[0], gr[1], gr[2]);
-            m[1] += 1.0f;
-            const float X = (float)dx, Y = (float)dy, Z = (float)dz;
-            const float xr = sw[0] * X + sw[1] * Y + sw[2] * Z + sw[9];
-            const float yr = sw[3] * X + sw[4] * Y + sw[5] * Z + sw[10];
-            const float zr = sw[6] * X + sw[7] * Y + sw[8] * Z + sw[11];
-            if (!inside(dom, xr, yr, zr)) {
-                m[2] = 1.0f;
-                continue;
-            }
-            const float val = sample_rel<MODE>(gbase, rowp, planep, xr, yr, zr, dom.kx, dom.ky, dom.kz);
-            if (val != val) continue;
-            const float fv = fraw - pf, gv = val - pg;
-            m[0] += 1.0f;
-            m[3] += fv;
-            m[4] += fv * fv;
-            m[5] += gv;
-            m[6] += gv * gv;
-            m[7] += fv * gv;
-#pragma unroll
-            for (int cc = 0; cc < 3; ++cc) {
-                m[8 + cc] += gr[cc];
-                m[11 + cc] += gr[cc] * fv;
-                m[14 + cc] += gr[cc] * gv;
-            }
-        }
-        block_sum<NQ, NWAVES>(m, res, red);
-        const double* r = res;
-        if (tid == 0) {
-            const double n_valid = r[0], n_ref = r[1];
-            int st = -1;
-            if (r[2] > 0.0)
-                st = ST_OUT_OF_BOUNDS;
-            else if (n_valid < MIN_SUBSET_VOXELS || n_valid < MIN_VALID_FRACTION * n_ref)
-                st = ST_INVALID_SUBSET;
-            if (st >= 0) {
-                zncc = NAN;
-                ctrl = st;
-            } else {
-                const double mf = r[3] / n_valid, mg = r[5] / n_valid;
-                double ssf = r[4] - n_valid * mf * mf, ssg = r[6] - n_valid * mg * mg;
-                ssf = ssf < 0.0 ? 0.0 : ssf;
-                ssg = ssg < 0.0 ? 0.0 : ssg;
-                const double bf = sqrt(ssf > 1e-30 ? ssf : 1e-30), bg = sqrt(ssg > 1e-30 ? ssg : 1e-30);
-                const double ratio = bf / bg;
-                double tb[3], norm_abs = 0.0;
-                for (int k = 0; k < 3; ++k) {
-                    const double C = r[8 + k], A = r[11 + k], B = r[14 + k];
-                    const double bk = (A - mf * C) - ratio * (B - mg * C);
-                    tb[k] = bk * 2.0 / bf2 + a.mu * (P[9 + k] - a.U_old[3 * n + k] - a.vdual[3 * n + k]);
-                    norm_abs += tb[k] * tb[k];
-                }
-                norm_abs = sqrt(norm_abs);
-                pg_next = (double)pg + mg;
-                if (!isfinite(norm_abs)) {
-                    zncc = NAN;
-                    ctrl = ST_NAN;
-                } else {
-                    if (norm_init < 0.0) norm_init = norm_abs;
-                    const double norm_rel = norm_init > 1e-300 ? norm_abs / norm_init : 0.0;
-                    zncc = (r[7] - n_valid * mf * mg) / (bf * bg);
-                    if (norm_rel < a.tol || norm_abs < a.mu * 1e-4) {
-                        ctrl = ST_CONVERGED;
-                    } else {
-                        if (a.noise_gain > 0.0 && last_dn < NOISE_CORR_STEP) {
-                            const double om = 1.0 - zncc;
-                            const double s2 = (om > 0.0 ? om : 0.0) * bf * bf / n_valid;
-                            if (!build_h3(H, bf2, a.mu, a.noise_gain * s2 * n_valid, Hinv))
-                                build_h3(H, bf2, a.mu, 0.0, Hinv);
-                        }
-                        double dt[3];
-                        for (int i = 0; i < 3; ++i) {
-                            double sacc = 0.0;
-                            for (int j = 0; j < 3; ++j) sacc += Hinv[3 * i + j] * tb[j];
-                            dt[i] = -sacc;
-                        }
-                        const double dn = sqrt(dt[0] * dt[0] + dt[1] * dt[1] + dt[2] * dt[2]);
-                        if (!isfinite(dn)) {
-                            zncc = NAN;
-                            ctrl = ST_NAN;
-                        } else {
-                            last_dn = dn;
-                            const double a00 = 1.0 + P[0], a01 = P[1], a02 = P[2], a10 = P[3], a11 = 1.0 + P[4],
-                                         a12 = P[5], a20 = P[6], a21 = P[7], a22 = 1.0 + P[8];
-                            bool apply = true;
-                            if (dn < a.dp_tol) {
-                                ctrl = ST_CONVERGED;
-                                apply = false;
-                            } else if (a.predictive && dn_prev > 0.0 && dn < PREDICT_CONTRACTION * dn_prev &&
-                                       dn * dn < a.dp_tol * dn_prev) {
-                                ctrl = ST_CONVERGED;
-                            } else {
-                                dn_prev = dn;
-                                if (dn < best_dn * STALL_STEP_DECAY) {
-                                    best_dn = dn;
-                                    stall = 0;
-                                } else {
-                                    stall += 1;
-                                    if (a.patience > 0 && stall >= a.patience) {
-                                        ctrl = ST_STALLED;
-                                        apply = false;
-                                    }
-                                }
-                            }
-                            if (apply) {
-                                P[9] -= a00 * dt[0] + a01 * dt[1] + a02 * dt[2];
-                                P[10] -= a10 * dt[0] + a11 * dt[1] + a12 * dt[2];
-                                P[11] -= a20 * dt[0] + a21 * dt[1] + a22 * dt[2];
-                            }
-                        }
-                    }
-                }
-            }
-        }
-        __syncthreads();
-    }
-    if (tid == 0) {
-        if (ctrl >= 0) status = ctrl;
-        for (int k = 0; k < 3; ++k) a.U[3 * n + k] = P[9 + k];
-        a.n_iter[n] = it_done;
-        a.status[n] = (int8_t)status;
-        a.zncc[n] = (status == ST_CONVERGED || status == ST_STALLED || status == ST_MAX_ITER) ? zncc : NAN;
     }
 }
 
@@ -1509,47 +1469,61 @@ __global__ void k_init_outputs(const uint8_t* __restrict__ valid, int64_t N, int
     zncc[i] = NAN;
 }
 
-// The fused iteration kernel: LDS bricks (variant 1, when the subset's warped rows can fit one and
-// the volume is addressable with 32-bit byte offsets), else memory gathers.
-template <int MODE, bool STORED, bool SEL>
+// Brick tiles along one axis: a subset whose span plus the interpolation stencil and a voxel of
+// slack fits `room` brick floats is one tile; a wider one is cut into equal tiles of at most
+// (room - 5) / stride + 1 voxels, rounded up to whole 8-voxel patches where that still fits.
+int brick_tile(int n, int stride, int room) {
+    const int kmax = (room - 5) / stride + 1;
+    if (n <= kmax) return n;
+    const int nt = (n + kmax - 1) / kmax;
+    const int t = (n + nt - 1) / nt, t8 = (t + 7) & ~7;
+    return t8 <= kmax ? t8 : t;
+}
+
+// The fused iteration kernel: LDS bricks (variant 1, when the volume is addressable with 32-bit
+// byte offsets), else memory gathers.
+template <int MODE, bool STORED, bool SEL, bool T3 = false>
 void launch_fused(int variant, const IcgnArgs& a, const int64_t* list, const int64_t* count, int64_t nb, size_t bits,
                   bool buf, double* mom, int lds_budget, int flags, hipStream_t st) {
     // LDS brick: BRICK_XY^2 floats per plane, as many planes as the LDS budget holds (by default
-    // three workgroups per CU). Pair mapping (stride 1, rows of 32..64 voxels): 39 rows per plane
-    // and a plane stride of sx modulo 64 floats; otherwise 40 rows and a plane of 40 x 40.
-    const bool patches = !(flags & TUNE_NO_PATCHES) && a.sub.stride == 1 && a.sub.sx >= 8 && a.sub.sy >= 8;
-    const bool pair = !patches && !(flags & TUNE_NO_PAIRS) && a.sub.stride == 1 && a.sub.sx >= 32 &&
-                      a.sub.sx <= 64 && a.sub.sy >= NWAVES;
+    // three workgroups per CU); subsets wider than the brick in x/y boxes of tile_x x tile_y voxel
+    // columns. Pair mapping (stride 1, one box, rows of 32..64 voxels): 39 rows per plane and a
+    // plane stride of sx modulo 64 floats; otherwise 40 rows and a plane of 40 x 40.
+    const Subset& sb = a.sub;
+    const int tile_x = brick_tile(sb.sx, sb.stride, BRICK_XY), tile_y = brick_tile(sb.sy, sb.stride, BRICK_XY);
+    const bool whole = tile_x == sb.sx && tile_y == sb.sy;
+    const bool patches = !(flags & TUNE_NO_PATCHES) && sb.stride == 1 && sb.sx >= 8 && sb.sy >= 8;
+    const bool pair = !patches && whole && !(flags & TUNE_NO_PAIRS) && sb.stride == 1 && sb.sx >= 32 &&
+                      sb.sx <= 64 && sb.sy >= NWAVES && 2 * sb.hy + 5 <= BRICK_XY - 1;
     const int rows = pair ? BRICK_XY - 1 : BRICK_XY;
-    const int plane = pair ? BRICK_XY * rows + ((a.sub.sx - BRICK_XY * rows) % 64 + 64) % 64 : BRICK_XY * rows;
+    const int plane = pair ? BRICK_XY * rows + ((sb.sx - BRICK_XY * rows) % 64 + 64) % 64 : BRICK_XY * rows;
     int planes = 0;
-    // the subset span plus the interpolation stencil and a voxel of slack must fit; whether the
-    // warped stencils of a chunk do is decided per chunk on the device
-    if (variant == 1 && buf && 2 * a.sub.hx + 5 <= BRICK_XY && 2 * a.sub.hy + 5 <= rows) {
+    // whether the warped stencils of a chunk fit is decided per chunk on the device
+    if (variant == 1 && buf) {
         hipFuncAttributes fa{};
-        if (hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&k_icgn12_fused<MODE, STORED, true, BRICK_XY, SEL>)) ==
+        if (hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&k_icgn12_fused<MODE, STORED, true, BRICK_XY, SEL, T3>)) ==
             hipSuccess) {
             const int64_t budget = lds_budget > 0 ? lds_budget : LDS_PER_WG;
             const int64_t room = budget - (int64_t)fa.sharedSizeBytes - (int64_t)bits;
-            planes = (int)std::min<int64_t>(a.sub.sz + 3, room / (int64_t)(4 * plane));
+            planes = (int)std::min<int64_t>(sb.sz + 3, room / (int64_t)(4 * plane));
         }
         if (planes < 6) planes = 0;  // too few planes per chunk: the staging would dominate
     }
     const size_t lds = bits + (size_t)planes * 4 * plane;
-    auto kern = &k_icgn12_fused<MODE, STORED, true, BRICK_XY, SEL>;
+    auto kern = &k_icgn12_fused<MODE, STORED, true, BRICK_XY, SEL, T3>;
     if (planes > 0 && hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
         planes = 0;
     timer_begin(st);
     if (planes > 0) {
         hipLaunchKernelGGL(kern, dim3((unsigned)nb), dim3(BLOCK), lds, st, a, list, count, (const double*)mom, planes,
-                           plane, rows, patches ? LANES_PATCHES : (pair ? LANES_PAIRS : LANES_WALK));
+                           plane, rows, patches ? LANES_PATCHES : (pair ? LANES_PAIRS : LANES_WALK), tile_x, tile_y);
     } else if (buf) {
-        hipLaunchKernelGGL((k_icgn12_fused<MODE, STORED, true, 0, SEL>), dim3((unsigned)nb), dim3(BLOCK), bits, st, a,
-                           list, count, (const double*)mom, 0, 0, 0, 0);
+        hipLaunchKernelGGL((k_icgn12_fused<MODE, STORED, true, 0, SEL, T3>), dim3((unsigned)nb), dim3(BLOCK), bits, st,
+                           a, list, count, (const double*)mom, 0, 0, 0, 0, 0, 0);
     } else {
-        hipLaunchKernelGGL((k_icgn12_fused<MODE, STORED, false, 0, SEL>), dim3((unsigned)nb), dim3(BLOCK), bits, st, a,
-                           list, count, (const double*)mom, 0, 0, 0, 0);
+        hipLaunchKernelGGL((k_icgn12_fused<MODE, STORED, false, 0, SEL, T3>), dim3((unsigned)nb), dim3(BLOCK), bits, st,
+                           a, list, count, (const double*)mom, 0, 0, 0, 0, 0, 0);
     }
 }
 
@@ -1577,11 +1551,18 @@ void launch_icgn12(int variant, const IcgnArgs& a, const int64_t* list, const in
     }
 }
 
+// Translation-only solves: reference moments, then the fused kernel in its T3 form (LDS bricks when
+// the subset's warped rows fit one, else buffer or pointer gathers).
 template <int MODE, bool STORED>
-void launch_icgn3(const Icgn3Args& a, const int64_t* list, const int64_t* count, int64_t N, hipStream_t st) {
+void launch_icgn3(int variant, const IcgnArgs& a, const int64_t* list, const int64_t* count, int64_t N, double* mom,
+                  int lds_budget, int flags, hipStream_t st) {
     const int64_t nb = ((N + 7) / 8) * 8;
-    timer_begin(st);
-    hipLaunchKernelGGL((k_icgn3_fused<MODE, STORED>), dim3((unsigned)nb), dim3(BLOCK), 0, st, a, list, count);
+    hipLaunchKernelGGL((k_ref_moments<STORED>), dim3((unsigned)nb), dim3(BLOCK), 0, st, a, list, count, mom);
+    const size_t bits = sizeof(unsigned) * (size_t)(((a.sub.S + 31) / 32 + 3) & ~3);
+    const bool buf = (int64_t)a.V.nx * a.V.ny * a.V.nz < ((int64_t)1 << 30);
+    const bool sel = a.V.mask != nullptr || (a.split_index != nullptr && a.split_keep != nullptr);
+    if (sel) launch_fused<MODE, STORED, true, true>(variant, a, list, count, nb, bits, buf, mom, lds_budget, flags, st);
+    else launch_fused<MODE, STORED, false, true>(variant, a, list, count, nb, bits, buf, mom, lds_budget, flags, st);
     timer_end(st);
 }
 
@@ -1774,15 +1755,20 @@ int aldvc_icgn12(const int64_t* coords, int64_t N, double* P, int hx, int hy, in
 
 size_t aldvc_icgn3_workspace_bytes(int64_t N) { return aldvc_icgn12_workspace_bytes(N, 0, 0, 0, 1, 1); }
 
-int aldvc_icgn3_gated(const int64_t* coords, int64_t N, const double* U_old, double* U, const double* F_fixed,
+int aldvc_icgn3_tuned(const int64_t* coords, int64_t N, const double* U_old, double* U, const double* F_fixed,
                       const double* vdual, int hx, int hy, int hz, int stride, const float* f, const float* gx,
                       const float* gy, const float* gz, const uint8_t* mask, const float* g, int64_t nz, int64_t ny,
                       int64_t nx, int mode, const double* H_all, const double* meanf, const double* bottomf,
                       const uint8_t* valid, double mu, double tol, double dp_tol, int max_iter, int patience,
                       double noise_gain, int predictive, const int64_t* split_index, const uint8_t* split_keep,
                       int64_t keep_bytes, int32_t* n_iter, int8_t* status, double* zncc, const int* gate,
-                      void* workspace, size_t workspace_bytes, hipStream_t stream) {
+                      int variant, int lds_budget, int flags, void* workspace, size_t workspace_bytes,
+                      hipStream_t stream) {
     clear_error();
+    if (variant != 1 && variant != 2) return fail(-2, "variant must be 1 (LDS bricks where they fit) or 2 (gathers only)");
+    if (lds_budget < 0 || lds_budget > 160 * 1024) return fail(-2, "lds_budget must be 0 or at most 160 KiB");
+    if (flags & ~(TUNE_NO_PAIRS | TUNE_INDEX_ORDER | TUNE_NO_XCD | TUNE_NO_PATCHES))
+        return fail(-2, "unknown tuning flags");
     Volumes V{f, gx, gy, gz, g, mask, (int)nz, (int)ny, (int)nx};
     const Subset s = make_subset(hx, hy, hz, stride);
     if (int e = check_common(V, s, mode)) return e;
@@ -1792,6 +1778,8 @@ int aldvc_icgn3_gated(const int64_t* coords, int64_t N, const double* U_old, dou
         return fail(-2, "null pointer");
     if (N > INT32_MAX) return fail(-2, "too many nodes for one launch");
     if (split_keep && keep_bytes < (s.S + 7) / 8) return fail(-2, "keep rows shorter than ceil(S/8) bytes");
+    if (sizeof(unsigned) * (size_t)((s.S + 31) / 32) > 48 * 1024)
+        return fail(-2, "subset too large for the fused kernel's LDS voxel mask");
     const size_t need = aldvc_icgn3_workspace_bytes(N);
     if (!workspace || workspace_bytes < need) return fail(-3, "workspace too small (see aldvc_icgn3_workspace_bytes)");
     int64_t* count = (int64_t*)workspace;
@@ -1804,15 +1792,39 @@ int aldvc_icgn3_gated(const int64_t* coords, int64_t N, const double* U_old, dou
     }
     hipLaunchKernelGGL(k_init_outputs, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, stream, valid, N, n_iter,
                        status, zncc, gate);
-    launch_compact(true, valid, coords, N, hx, hy, hz, V, list, count, stream);
+    launch_compact(!(flags & TUNE_INDEX_ORDER), valid, coords, N, hx, hy, hz, V, list, count, stream);
     if (gate) hipLaunchKernelGGL(k_gate_count, dim3(1), dim3(64), 0, stream, count, gate);
-    Icgn3Args a{V, s, coords, U_old, U, F_fixed, vdual, H_all, bottomf, meanf, split_index, split_keep, keep_bytes,
-                mu, tol, dp_tol, noise_gain > 0.0 ? noise_gain : 0.0, max_iter, patience, predictive, n_iter, status,
-                zncc};
+    size_t list_bytes = (256 + sizeof(int64_t) * (size_t)N + 255) / 256 * 256;
+    double* mom = (double*)((char*)workspace + list_bytes);
+    IcgnArgs a{};
+    a.V = V;
+    a.sub = s;
+    a.coords = coords;
+    a.H_all = H_all;
+    a.meanf = meanf;
+    a.split_index = split_index;
+    a.split_keep = split_keep;
+    a.keep_bytes = keep_bytes;
+    a.noise_gain = noise_gain > 0.0 ? noise_gain : 0.0;
+    a.tol = tol;
+    a.dp_tol = dp_tol;
+    a.max_iter = max_iter;
+    a.patience = patience;
+    a.predictive = predictive;
+    a.n_iter = n_iter;
+    a.status = status;
+    a.zncc = zncc;
+    a.xcd = (flags & TUNE_NO_XCD) ? 0 : 1;
+    a.F = F_fixed;
+    a.U_old = U_old;
+    a.vdual = vdual;
+    a.bottomf = bottomf;
+    a.U = U;
+    a.mu = mu;
     const bool stored = gx != nullptr;
-#define ALDVC_DISPATCH3(MD)                                   \
-    if (stored) launch_icgn3<MD, true>(a, list, count, N, stream);  \
-    else launch_icgn3<MD, false>(a, list, count, N, stream);
+#define ALDVC_DISPATCH3(MD)                                                                           \
+    if (stored) launch_icgn3<MD, true>(variant, a, list, count, N, mom, lds_budget, flags, stream);    \
+    else launch_icgn3<MD, false>(variant, a, list, count, N, mom, lds_budget, flags, stream);
     switch (mode) {
         case 0: ALDVC_DISPATCH3(0) break;
         case 1: ALDVC_DISPATCH3(1) break;
@@ -1820,6 +1832,20 @@ int aldvc_icgn3_gated(const int64_t* coords, int64_t N, const double* U_old, dou
     }
 #undef ALDVC_DISPATCH3
     return launch_status();
+}
+
+int aldvc_icgn3_gated(const int64_t* coords, int64_t N, const double* U_old, double* U, const double* F_fixed,
+                      const double* vdual, int hx, int hy, int hz, int stride, const float* f, const float* gx,
+                      const float* gy, const float* gz, const uint8_t* mask, const float* g, int64_t nz, int64_t ny,
+                      int64_t nx, int mode, const double* H_all, const double* meanf, const double* bottomf,
+                      const uint8_t* valid, double mu, double tol, double dp_tol, int max_iter, int patience,
+                      double noise_gain, int predictive, const int64_t* split_index, const uint8_t* split_keep,
+                      int64_t keep_bytes, int32_t* n_iter, int8_t* status, double* zncc, const int* gate,
+                      void* workspace, size_t workspace_bytes, hipStream_t stream) {
+    return aldvc_icgn3_tuned(coords, N, U_old, U, F_fixed, vdual, hx, hy, hz, stride, f, gx, gy, gz, mask, g, nz, ny,
+                             nx, mode, H_all, meanf, bottomf, valid, mu, tol, dp_tol, max_iter, patience, noise_gain,
+                             predictive, split_index, split_keep, keep_bytes, n_iter, status, zncc, gate, 1, 0, 0,
+                             workspace, workspace_bytes, stream);
 }
 
 int aldvc_icgn3(const int64_t* coords, int64_t N, const double* U_old, double* U, const double* F_fixed,

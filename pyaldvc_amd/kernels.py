@@ -368,9 +368,29 @@ def icgn_12dof_parallel(
 def icgn_3dof_parallel(
     coords, U_old, F_fixed, vdual, hx, hy, hz, f, gx, gy, gz, mask, g, mode, H_all, meanf_all,
     bottomf_all, valid, mu, tol, dp_tol, max_iter, patience, stride=1, n_full=0.0, noise_gain=0.0,
-    predictive=True, split_index=None, split_keep=None, stream=None,
+    predictive=True, split_index=None, split_keep=None, stream=None, variant: str = "auto", tuning=None,
 ):  # fmt: skip
-    """ADMM subproblem-1 translation IC-GN: ``(U_out (N,3), n_iter, status, zncc)``."""
+    """ADMM subproblem-1 translation IC-GN: ``(U_out (N,3), n_iter, status, zncc)``.
+
+    The fused kernel's translation-only form. ``variant="auto"`` (default) takes the autotuner's
+    launch shape for this subset shape on this device (timed once on a node sample, then cached);
+    ``"fused"``: LDS bricks where a chunk fits one; ``"fused_gather"``: memory gathers only.
+    ``tuning`` (an ``autotune.Tuning``) sets the launch shape explicitly."""
+    if tuning is None:
+        if variant == "auto":
+            from . import autotune
+
+            args = (coords, U_old, F_fixed, vdual, hx, hy, hz, f, gx, gy, gz, mask, g, mode, H_all, meanf_all,
+                    bottomf_all, valid, mu, tol, dp_tol, max_iter, patience)  # fmt: skip
+            kw = dict(stride=stride, n_full=n_full, noise_gain=noise_gain, predictive=predictive,
+                      split_index=split_index, split_keep=split_keep, stream=stream)  # fmt: skip
+            tuning = autotune.choose_icgn3(args, kw)
+        elif variant in ("fused", "fused_gather"):
+            from .autotune import Tuning
+
+            tuning = Tuning(VARIANTS[variant])
+        else:
+            raise ValueError("variant must be 'auto', 'fused' or 'fused_gather'")
     A = _Args(f)
     c = A.arr(coords, np.int64)
     N = int(c.shape[0])
@@ -397,15 +417,15 @@ def icgn_3dof_parallel(
     ws_bytes = int(lib().aldvc_icgn3_workspace_bytes(N))
     ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
     check(
-        lib().aldvc_icgn3(
+        lib().aldvc_icgn3_tuned(
             _ptr(c), N, _ptr(Uo), _ptr(U), _ptr(Fd), _ptr(vdd), int(hx), int(hy), int(hz), int(stride),
             _ptr(fd), _ptr(gxd), _ptr(gyd), _ptr(gzd), _ptr(md), _ptr(gd), nz, ny, nx, int(mode),
             _ptr(Hd), _ptr(mf), _ptr(bf), _ptr(vd), float(mu), float(tol), float(dp_tol), int(max_iter),
             int(patience), float(noise_gain) if noise_gain > 0.0 else 0.0, int(bool(predictive)),
-            _ptr(si), _ptr(sk), kb, _ptr(n_iter), _ptr(status), _ptr(zncc), _ptr(ws), ws_bytes,
-            stream_handle(stream),
+            _ptr(si), _ptr(sk), kb, _ptr(n_iter), _ptr(status), _ptr(zncc), None, int(tuning.variant),
+            int(tuning.lds_budget), int(tuning.flags), _ptr(ws), ws_bytes, stream_handle(stream),
         ),
-        "aldvc_icgn3",
+        "aldvc_icgn3_tuned",
     )  # fmt: skip
     return A.out(U), A.out(n_iter), A.out(status), A.out(zncc)
 

@@ -161,7 +161,7 @@ def _rot_z(deg, stretch_z=0.0):
 # large for the brick's rows (every chunk gathers from memory), a z stretch that shrinks the chunks,
 # nodes whose warped subsets leave the sampling domain (status OUT_OF_BOUNDS), 35 x 33 x 27 subsets
 # (rows of 35 voxels in plane pairs, an odd plane count per chunk) in the three interpolation modes,
-# and a 15^3 subset (rows too short for plane pairs).
+# a 15^3 subset (rows too short for plane pairs), and 49-voxel rows cut into x/y boxes.
 BRICK_CASES = {
     "fits": dict(F=_rot_z(3.0), t=(0.6, -0.4, 0.7)),
     "rotation": dict(F=_rot_z(12.0), t=(0.3, 0.5, -0.6)),
@@ -171,6 +171,11 @@ BRICK_CASES = {
     "rows35_cubic": dict(F=_rot_z(2.0), t=(0.4, 0.3, -0.5), half=(17, 16, 13), mode=1),
     "rows35_bspline": dict(F=_rot_z(2.0), t=(0.4, 0.3, -0.5), half=(17, 16, 13), mode=2),
     "rows15": dict(F=_rot_z(3.0), t=(0.6, -0.4, 0.7), half=(7, 7, 7)),
+    # subsets wider than a brick row: x/y boxes of 32 x 24 and 17 x 17 voxel columns
+    "tiled_cubic": dict(F=_rot_z(3.0), t=(0.6, -0.4, 0.7), half=(24, 20, 10), ax=28 + 14 * np.arange(5)),
+    "tiled_linear": dict(F=_rot_z(2.0), t=(0.4, 0.3, -0.5), half=(24, 20, 10), mode=0, ax=28 + 14 * np.arange(5)),
+    # corner boxes rotate out of the brick's rows and gather, the others stay in bricks
+    "tiled_rotation": dict(F=_rot_z(9.0), t=(0.3, 0.5, -0.6), half=(24, 24, 8), ax=28 + 14 * np.arange(5)),
 }
 
 
@@ -179,7 +184,7 @@ BRICK_CASES = {
 def test_icgn12_brick_paths_match_oracle(key, variant):
     c = BRICK_CASES[key]
     cs = bead_case(shape=(112, 112, 112), F=c["F"], t=c["t"], seed=9)
-    ax = 24 + 16 * np.arange(5)
+    ax = c.get("ax", 24 + 16 * np.arange(5))
     coords = np.stack(np.meshgrid(ax, ax, ax, indexing="ij"), -1).reshape(-1, 3).astype(np.int64)
     half, mode = c.get("half", (16, 16, 16)), c.get("mode", 1)
     f, g, gx, gy, gz = cs["f"], cs["g"], cs["gx"], cs["gy"], cs["gz"]
@@ -212,6 +217,35 @@ def test_icgn3_matches_oracle(golden, key, mu, noise):
         100, 5, 1, float(pattern[9, 9]), float(golden["stencil_noise_gain"]) if noise else 0.0, True,
     )  # fmt: skip
     assert_icgn_parity(cpu_ref.icgn_3dof_parallel(*args), K.icgn_3dof_parallel(*args))
+
+
+@pytest.mark.parametrize("variant", ["fused", "fused_gather", "auto"])
+@pytest.mark.parametrize("half", [(16, 16, 16), (24, 20, 10)], ids=["whole", "tiled"])
+def test_icgn3_brick_paths_match_oracle(half, variant, tmp_path, monkeypatch):
+    """Translation-only solves on the fused kernel's brick path: 33^3 subsets in one box, 49 x 41 x
+    21 subsets in x/y boxes; fixed F from the imposed field, U_old one voxel off, a small dual."""
+    F = _rot_z(3.0)
+    cs = bead_case(shape=(112, 112, 112), F=F, t=(0.6, -0.4, 0.7), seed=9)
+    ax = 28 + 14 * np.arange(5)
+    coords = np.stack(np.meshgrid(ax, ax, ax, indexing="ij"), -1).reshape(-1, 3).astype(np.int64)
+    f, g, gx, gy, gz = cs["f"], cs["g"], cs["gx"], cs["gy"], cs["gz"]
+    H, L, mf, bf, nv, valid = cpu_ref.precompute_nodes(coords, *half, f, gx, gy, gz, MASK_NONE, 0.5, 1e12)
+    N = coords.shape[0]
+    U_old = np.round(truth_at(cs, coords)) + np.array([0.4, -0.3, 0.2])
+    Ff = np.tile(F.reshape(1, 9), (N, 1))
+    vdual = np.random.default_rng(3).normal(0.0, 0.05, (N, 3))
+    args = (coords, U_old, Ff, vdual, *half, f, gx, gy, gz, MASK_NONE, g, 1, H, mf, bf, valid, 1e-3, 1e-2, 1e-3, 100, 5)
+    from pyaldvc_amd import autotune as AT
+
+    monkeypatch.setenv("PYALDVC_AMD_TUNE_CACHE", str(tmp_path / "tune.json"))
+    AT.clear(memory_only=True)
+    ref = cpu_ref.icgn_3dof_parallel(*args)
+    got = K.icgn_3dof_parallel(*args, variant=variant)
+    assert_icgn_parity(ref, got)
+    assert np.sum(ref[2] == 0) > 0.5 * valid.sum()
+    if variant == "auto":
+        assert AT.last_report["key"].startswith("icgn3|")
+        assert len(AT.last_report["seconds"]) == len(AT.CANDIDATES)
 
 
 def test_evaluate_zncc_matches_oracle(golden):
