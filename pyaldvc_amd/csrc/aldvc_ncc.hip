@@ -183,6 +183,18 @@ constexpr int MG = 4;     // output planes per workgroup of the MFMA kernel (one
 constexpr int TRING = 5;  // template plane slots: four in use and the one being loaded
 constexpr int PRE = 16;   // window values per lane prefetched for the next plane (57^2 / 256 <= 13)
 
+// A fragment rows moved down by one within each 16-lane DPP row: lane m takes lane m + 1's value
+// (row_shl:1); lane 15 of each row, which has no source, reads zero (bound_ctrl).
+__device__ inline bf16x8_t shift_rows(bf16x8_t frag) {
+    int v[4];
+    __builtin_memcpy(v, &frag, sizeof(v));
+#pragma unroll
+    for (int i = 0; i < 4; ++i) v[i] = __builtin_amdgcn_mov_dpp(v[i], 0x101, 0xF, 0xF, true);
+    bf16x8_t r;
+    __builtin_memcpy(&r, v, sizeof(r));
+    return r;
+}
+
 struct Bf16Geom {
     int sx, sy, sz, vx, vy, vz, wx, wy, wz;
     int nyt, nxt;  // 16-row / 16-column output tiles
@@ -368,9 +380,12 @@ __global__ __launch_bounds__(256) void k_zncc_map_bf16(NccArgs a, Bf16Geom q) {
     // ---- LDS: two window planes (bf16, zero-padded) and TRING template planes
     const int plane_el = q.AH * q.AW;
     __bf16* Ab = (__bf16*)lds;
+    // template slots twice: as staged, and shifted by one element (Tr1[e] = Tr[e - 1]) so that every
+    // lane's B fragment starts on a 4-byte boundary in one of them
     __bf16* Tr = Ab + 2 * plane_el;
+    __bf16* Tr1 = Tr + TRING * q.sy * q.TW;
     for (int i = tid; i < 2 * plane_el; i += BLOCK) Ab[i] = (__bf16)0.0f;
-    for (int i = tid; i < TRING * q.sy * q.TW; i += BLOCK) Tr[i] = (__bf16)0.0f;
+    for (int i = tid; i < 2 * TRING * q.sy * q.TW; i += BLOCK) Tr[i] = (__bf16)0.0f;
     __syncthreads();
     // per-lane prefetch of one window plane (and one template plane)
     const int wpl = q.wy * q.wx, tpl_n = q.sy * q.sx;
@@ -411,12 +426,15 @@ __global__ __launch_bounds__(256) void k_zncc_map_bf16(NccArgs a, Bf16Geom q) {
         const int tz = z - oz0;
         if (tz < q.sz) {
             __bf16* tp = Tr + (tz % TRING) * q.sy * q.TW;
+            __bf16* tp1 = Tr1 + (tz % TRING) * q.sy * q.TW;
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const int i = tid + BLOCK * u;
                 if (i < tpl_n) {
                     const int y = i / q.sx, x = i - y * q.sx;
-                    tp[y * q.TW + 16 + x] = (__bf16)(float)((double)tpre[u] - mf);
+                    const __bf16 v = (__bf16)(float)((double)tpre[u] - mf);
+                    tp[y * q.TW + 16 + x] = v;
+                    tp1[y * q.TW + 17 + x] = v;
                 }
             }
         }
@@ -437,28 +455,57 @@ __global__ __launch_bounds__(256) void k_zncc_map_bf16(NccArgs a, Bf16Geom q) {
         const int tz = z - my_oz;
         if (my_oz < q.vz && tz >= 0 && tz < q.sz) {
             const __bf16* Ap = Ab + (z & 1) * plane_el;
-            const __bf16* tpl = Tr + (tz % TRING) * q.sy * q.TW;
+            // B fragment of lane (n, g): elements 16 + 32 s + 8 g - n + i of the staged row; odd n read
+            // the shifted copy one element further, so the start is even (4-byte aligned) for all lanes
+            const int odd = am & 1;
+            const __bf16* tpl = (odd ? Tr1 : Tr) + (tz % TRING) * q.sy * q.TW + 16 + akb - am + odd;
+            // A fragments of the window rows 16 mt + m + ty at the NC distinct column offsets
+            // 16 (j + 2 s), held in registers across the template rows: from row ty to ty + 1 every
+            // lane takes its lower neighbour's row (a DPP shift inside the 16 lanes of a K group)
+            // and only lane 15 of each group reads a new row from LDS.
+            constexpr int NC = NT + 2 * (KS - 1);
+            bf16x8_t afr[NT][NC];
+#pragma unroll
+            for (int mt = 0; mt < NT; ++mt) {
+                if (mt >= q.nyt) break;
+#pragma unroll
+                for (int c = 0; c < NC; ++c)
+                    __builtin_memcpy(&afr[mt][c], Ap + (16 * mt + am) * q.AW + 16 * c + akb, sizeof(bf16x8_t));
+            }
             for (int ty = 0; ty < q.sy; ++ty) {
                 bf16x8_t bfr[KS];
 #pragma unroll
                 for (int s = 0; s < KS; ++s) {
-                    // B[k][n] = T~[tz][ty][32 s + k - n]; staged row index x + 16
-                    const __bf16* trow = tpl + ty * q.TW + 16 + 32 * s + akb - am;
-#pragma unroll
-                    for (int i = 0; i < 8; ++i) bfr[s][i] = trow[i];
+                    // B[k][n] = T~[tz][ty][32 s + k - n]: four aligned dwords
+                    const uint32_t* w = reinterpret_cast<const uint32_t*>(tpl + ty * q.TW + 32 * s);
+                    uint32_t v[4] = {w[0], w[1], w[2], w[3]};
+                    __builtin_memcpy(&bfr[s], v, sizeof(v));
                 }
 #pragma unroll
                 for (int mt = 0; mt < NT; ++mt) {
                     if (mt >= q.nyt) break;
-                    const __bf16* arow = Ap + (16 * mt + am + ty) * q.AW + akb;
 #pragma unroll
                     for (int j = 0; j < NT; ++j) {
                         if (j >= q.nxt) break;
 #pragma unroll
-                        for (int s = 0; s < KS; ++s) {
-                            bf16x8_t afr;
-                            __builtin_memcpy(&afr, arow + 16 * j + 32 * s, sizeof(afr));
-                            acc[mt][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr, bfr[s], acc[mt][j], 0, 0, 0);
+                        for (int s = 0; s < KS; ++s)
+                            acc[mt][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr[mt][j + 2 * s], bfr[s], acc[mt][j], 0, 0, 0);
+                    }
+                }
+                if (ty + 1 < q.sy) {
+#pragma unroll
+                    for (int mt = 0; mt < NT; ++mt) {
+                        if (mt >= q.nyt) break;
+#pragma unroll
+                        for (int c = 0; c < NC; ++c) afr[mt][c] = shift_rows(afr[mt][c]);
+                    }
+                    if (am == 15) {  // lane 15 of each K group: window row 16 mt + 15 + (ty + 1)
+#pragma unroll
+                        for (int mt = 0; mt < NT; ++mt) {
+                            if (mt >= q.nyt) break;
+#pragma unroll
+                            for (int c = 0; c < NC; ++c)
+                                __builtin_memcpy(&afr[mt][c], Ap + (16 * mt + 16 + ty) * q.AW + 16 * c + akb, sizeof(bf16x8_t));
                         }
                     }
                 }
@@ -492,7 +539,7 @@ __global__ __launch_bounds__(256) void k_zncc_map_bf16(NccArgs a, Bf16Geom q) {
 }
 
 size_t bf16_lds_bytes(const Bf16Geom& q) {
-    return sizeof(__bf16) * (2 * (size_t)q.AH * q.AW + (size_t)TRING * q.sy * q.TW);
+    return sizeof(__bf16) * (2 * (size_t)q.AH * q.AW + 2 * (size_t)TRING * q.sy * q.TW);
 }
 
 size_t box_lds_bytes(const Bf16Geom& q) {
