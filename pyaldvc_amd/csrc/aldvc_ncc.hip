@@ -472,15 +472,18 @@ __global__ __launch_bounds__(256) void k_zncc_map_bf16(NccArgs a, Bf16Geom q) {
                 for (int c = 0; c < NC; ++c)
                     __builtin_memcpy(&afr[mt][c], Ap + (16 * mt + am) * q.AW + 16 * c + akb, sizeof(bf16x8_t));
             }
-            for (int ty = 0; ty < q.sy; ++ty) {
-                bf16x8_t bfr[KS];
+            // B[k][n] = T~[tz][ty][32 s + k - n]: four aligned dwords per slice
+            auto load_b = [&](int ty, bf16x8_t* dst) {
 #pragma unroll
                 for (int s = 0; s < KS; ++s) {
-                    // B[k][n] = T~[tz][ty][32 s + k - n]: four aligned dwords
                     const uint32_t* w = reinterpret_cast<const uint32_t*>(tpl + ty * q.TW + 32 * s);
                     uint32_t v[4] = {w[0], w[1], w[2], w[3]};
-                    __builtin_memcpy(&bfr[s], v, sizeof(v));
+                    __builtin_memcpy(&dst[s], v, sizeof(v));
                 }
+            };
+            bf16x8_t bfr[KS];
+            load_b(0, bfr);
+            for (int ty = 0; ty < q.sy; ++ty) {
 #pragma unroll
                 for (int mt = 0; mt < NT; ++mt) {
                     if (mt >= q.nyt) break;
@@ -493,16 +496,15 @@ __global__ __launch_bounds__(256) void k_zncc_map_bf16(NccArgs a, Bf16Geom q) {
                     }
                 }
                 if (ty + 1 < q.sy) {
+                    // the next row's B fragments and, tile by tile, the A rows one further: their LDS
+                    // latency overlaps the row shifts of the later tiles
+                    load_b(ty + 1, bfr);
 #pragma unroll
                     for (int mt = 0; mt < NT; ++mt) {
                         if (mt >= q.nyt) break;
 #pragma unroll
                         for (int c = 0; c < NC; ++c) afr[mt][c] = shift_rows(afr[mt][c]);
-                    }
-                    if (am == 15) {  // lane 15 of each K group: window row 16 mt + 15 + (ty + 1)
-#pragma unroll
-                        for (int mt = 0; mt < NT; ++mt) {
-                            if (mt >= q.nyt) break;
+                        if (am == 15) {  // lane 15 of each K group: window row 16 mt + 15 + (ty + 1)
 #pragma unroll
                             for (int c = 0; c < NC; ++c)
                                 __builtin_memcpy(&afr[mt][c], Ap + (16 * mt + 16 + ty) * q.AW + 16 * c + akb, sizeof(bf16x8_t));
