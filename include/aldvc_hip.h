@@ -134,8 +134,11 @@ int aldvc_icgn3_gated(const int64_t *coords, int64_t N, const double *U_old, dou
 
 /* aldvc_icgn3_gated (gate may be NULL) with the launch choices made explicit, as aldvc_icgn12_tuned:
  * variant = 1: LDS bricks where a chunk's warped stencils fit one (subsets wider than a brick in
- * x/y boxes), 2: memory gathers only; lds_budget and flags as aldvc_icgn12_tuned. The other two
- * 3-DOF entries run variant 1 with lds_budget 0 and flags 0. Same reference function as aldvc_icgn3. */
+ * x/y boxes), 2: memory gathers only; lds_budget and flags as aldvc_icgn12_tuned, plus flag 16:
+ * the workspace already holds these nodes' reference moments from a previous call with the same
+ * nodes, reference volume, gradients, mask / keep rows and meanf (they are then not recomputed;
+ * an ADMM loop's later 3-DOF passes). The other two 3-DOF entries run variant 1 with lds_budget 0
+ * and flags 0. Same reference function as aldvc_icgn3. */
 int aldvc_icgn3_tuned(const int64_t *coords, int64_t N, const double *U_old, double *U, const double *F_fixed,
                       const double *vdual, int hx, int hy, int hz, int stride, const float *f, const float *gx,
                       const float *gy, const float *gz, const uint8_t *mask, const float *g, int64_t nz,

@@ -39,6 +39,7 @@ FILL_TOL = 1e-12  # relative residual of the spring fill's CG (the reference sol
 FILL_MAX_ITER = 2000  # the reference's CG limit for the fill (inpaint.py:118)
 MEDIAN_EPS = 0.1  # noise level of the universal median test (outlier_detection.py:138)
 DIRECT_GRAPH_ITER = 2000  # PCG iterations recorded for the "direct" global solver (driven to round-off)
+REUSE_MOMENTS = 16  # aldvc_icgn3_tuned flag: the workspace holds the nodes' reference moments
 STATUS_CONVERGED, STATUS_INVALID_SUBSET, STATUS_SKIPPED = 0, 3, 6
 
 
@@ -209,6 +210,9 @@ class AdmmLoop:
         gx, gy, gz = (vols[k] if stored else None for k in ("gx", "gy", "gz"))
         si, sk, kb = self.split
         t = self.tuning3 if self.tuning3 is not None else AT.DEFAULT
+        # the reference moments depend only on the frame's fixed inputs: the first 3-DOF pass
+        # (step 2) computes them into the workspace, later passes reuse them
+        reuse = REUSE_MOMENTS if step > 2 else 0
         check(
             lib().aldvc_icgn3_tuned(
                 K._ptr(d["coords"]), self.N, K._ptr(self.U2), K._ptr(self.U1raw), K._ptr(self.F2), K._ptr(self.v),
@@ -218,7 +222,7 @@ class AdmmLoop:
                 float(p.icgn_dp_tol), int(p.icgn_max_iter), int(p.icgn_patience),
                 float(self.gain) if self.gain > 0 else 0.0, int(bool(p.icgn_predictive_stop)), K._ptr(si), K._ptr(sk),
                 kb, K._ptr(self.n_iter[step]), K._ptr(self.status[step]), K._ptr(self.zncc[step]), K._ptr(self.stop),
-                int(t.variant), int(t.lds_budget), int(t.flags), K._ptr(self.ws_icgn3), int(self.ws_icgn3.numel()),
+                int(t.variant), int(t.lds_budget), int(t.flags) | reuse, K._ptr(self.ws_icgn3), int(self.ws_icgn3.numel()),
                 K.stream_handle(),
             ),
             "aldvc_icgn3_tuned",

@@ -60,6 +60,9 @@ struct IcgnArgs {
 // autotuner's search space): LDS bytes per workgroup for the brick (0: three workgroups per CU),
 // and flag bits.
 enum : int { TUNE_NO_PAIRS = 1, TUNE_INDEX_ORDER = 2, TUNE_NO_XCD = 4, TUNE_NO_PATCHES = 8 };
+// translation-only solves: the workspace already holds the reference moments of these nodes
+// (a previous call with the same nodes, reference volume, gradients, selection and pilots)
+enum : int { ICGN3_REUSE_MOMENTS = 16 };
 enum : int { LANES_WALK = 0, LANES_PAIRS = 1, LANES_PATCHES = 2 };  // voxel -> lane mappings of the brick path
 
 // Per-node iteration state of the control lane (numba_kernels.py:663-677). It lives in LDS and
@@ -1557,7 +1560,8 @@ template <int MODE, bool STORED>
 void launch_icgn3(int variant, const IcgnArgs& a, const int64_t* list, const int64_t* count, int64_t N, double* mom,
                   int lds_budget, int flags, hipStream_t st) {
     const int64_t nb = ((N + 7) / 8) * 8;
-    hipLaunchKernelGGL((k_ref_moments<STORED>), dim3((unsigned)nb), dim3(BLOCK), 0, st, a, list, count, mom);
+    if (!(flags & ICGN3_REUSE_MOMENTS))
+        hipLaunchKernelGGL((k_ref_moments<STORED>), dim3((unsigned)nb), dim3(BLOCK), 0, st, a, list, count, mom);
     const size_t bits = sizeof(unsigned) * (size_t)(((a.sub.S + 31) / 32 + 3) & ~3);
     const bool buf = (int64_t)a.V.nx * a.V.ny * a.V.nz < ((int64_t)1 << 30);
     const bool sel = a.V.mask != nullptr || (a.split_index != nullptr && a.split_keep != nullptr);
@@ -1767,7 +1771,7 @@ int aldvc_icgn3_tuned(const int64_t* coords, int64_t N, const double* U_old, dou
     clear_error();
     if (variant != 1 && variant != 2) return fail(-2, "variant must be 1 (LDS bricks where they fit) or 2 (gathers only)");
     if (lds_budget < 0 || lds_budget > 160 * 1024) return fail(-2, "lds_budget must be 0 or at most 160 KiB");
-    if (flags & ~(TUNE_NO_PAIRS | TUNE_INDEX_ORDER | TUNE_NO_XCD | TUNE_NO_PATCHES))
+    if (flags & ~(TUNE_NO_PAIRS | TUNE_INDEX_ORDER | TUNE_NO_XCD | TUNE_NO_PATCHES | ICGN3_REUSE_MOMENTS))
         return fail(-2, "unknown tuning flags");
     Volumes V{f, gx, gy, gz, g, mask, (int)nz, (int)ny, (int)nx};
     const Subset s = make_subset(hx, hy, hz, stride);
