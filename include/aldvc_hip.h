@@ -270,13 +270,17 @@ size_t aldvc_admm_global_workspace_bytes(int nx, int ny, int nz);
  * alpha K U (zero rows at inactive nodes), U2 = solution of A U2 = b from U2 = U by the
  * device-controlled PCG (inactive nodes keep U), F2 = nodal gradient of U2 (NaN at inactive
  * nodes). U, v, U2 (N, 3); F, W, F2 (N, 3, 3); Ae = (beta + alpha) K + mu M (8 x 8).
+ * sync_every = 0: all max_iter CG slots are enqueued (graph capture); > 0: the host reads the stop
+ * flag every sync_every iterations and enqueues no more after convergence (eager callers).
+ * capped (device int, nullable) is set to 1 when the solve used all max_iter iterations, so a
+ * graph recorded with a smaller iteration budget can be checked after replay. */
  * Replaces solver/subpb2_solver.py:108-155 (global_rhs, solve_subpb2) followed by
  * solver/global_operators.py:147-170 (nodal_gradient). */
 int aldvc_admm_global_step(const double *mats, const double *Ae, int nx, int ny, int nz, const uint8_t *elem_active,
                            const uint8_t *node_active, double beta, double mu, double alpha, const double *U,
                            const double *F, const double *W, const double *v, double tol, int max_iter, double *U2,
-                           double *F2, int *iterations_dev, const int *gate, void *workspace, size_t workspace_bytes,
-                           hipStream_t stream);
+                           double *F2, int *iterations_dev, const int *gate, int sync_every, int *capped,
+                           void *workspace, size_t workspace_bytes, hipStream_t stream);
 
 /* Workspace bytes of aldvc_subpb1_cleanup. */
 size_t aldvc_subpb1_cleanup_workspace_bytes(int nx, int ny, int nz);
@@ -286,12 +290,13 @@ size_t aldvc_subpb1_cleanup_workspace_bytes(int nx, int ny, int nz);
  * (eps: the test's noise level, 0.1 in the reference); U_out = bad ? U_hat : U; bad (N) 0/1.
  * edge_ok (N, 3) uint8, the +x, +y, +z edge of every node of a cut mesh, or NULL: without it the
  * unknown nodes are spring-filled first (tolerance fill_tol relative, at most fill_max_iter CG
- * iterations, all enqueued). Replaces solver/subpb1_solver.py:140-164 with
+ * iterations; sync_every and capped as aldvc_admm_global_step; fill_iterations (device int,
+ * nullable) receives the fill's CG iteration count). Replaces solver/subpb1_solver.py:140-164 with
  * utils/outlier_detection.py:98-160 and utils/inpaint.py:35-121. */
 int aldvc_subpb1_cleanup(int nx, int ny, int nz, const uint8_t *edge_ok, const int8_t *status, const double *U,
                          const double *U_hat, double threshold, double eps, double fill_tol, int fill_max_iter,
-                         double *U_out, uint8_t *bad, const int *stop, void *workspace, size_t workspace_bytes,
-                         hipStream_t stream);
+                         double *U_out, uint8_t *bad, const int *stop, int sync_every, int *fill_iterations,
+                         int *capped, void *workspace, size_t workspace_bytes, hipStream_t stream);
 
 /* Workspace bytes of aldvc_spring_fill. */
 size_t aldvc_spring_fill_workspace_bytes(int nx, int ny, int nz);

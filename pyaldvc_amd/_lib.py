@@ -121,10 +121,10 @@ SIGNATURES = {
     "aldvc_admm_global_workspace_bytes": (SZ, [I32, I32, I32]),
     "aldvc_admm_global_step": (
         I32,
-        [P, P, I32, I32, I32, P, P, F64, F64, F64, P, P, P, P, F64, I32, P, P, P, P, P, SZ, P],
+        [P, P, I32, I32, I32, P, P, F64, F64, F64, P, P, P, P, F64, I32, P, P, P, P, I32, P, P, SZ, P],
     ),
     "aldvc_subpb1_cleanup_workspace_bytes": (SZ, [I32, I32, I32]),
-    "aldvc_subpb1_cleanup": (I32, [I32, I32, I32, P, P, P, P, F64, F64, F64, I32, P, P, P, P, SZ, P]),
+    "aldvc_subpb1_cleanup": (I32, [I32, I32, I32, P, P, P, P, F64, F64, F64, I32, P, P, P, I32, P, P, P, SZ, P]),
     "aldvc_spring_fill_workspace_bytes": (SZ, [I32, I32, I32]),
     "aldvc_spring_fill": (I32, [I32, I32, I32, P, P, F64, I32, ctypes.POINTER(I32), P, SZ, P]),
     "aldvc_dual_update_workspace_bytes": (SZ, [I64]),

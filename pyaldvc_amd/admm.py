@@ -40,6 +40,12 @@ FILL_MAX_ITER = 2000  # the reference's CG limit for the fill (inpaint.py:118)
 MEDIAN_EPS = 0.1  # noise level of the universal median test (outlier_detection.py:138)
 DIRECT_GRAPH_ITER = 2000  # PCG iterations recorded for the "direct" global solver (driven to round-off)
 REUSE_MOMENTS = 16  # aldvc_icgn3_tuned flag: the workspace holds the nodes' reference moments
+SYNC_EVERY = 8  # eager runs: the host reads a CG solve's stop flag every 8 iterations
+# A recorded graph enqueues a fixed number of CG slots per solve; idle slots after convergence still
+# cost a dispatch each, so a graph is recorded with a budget learned from the eager run (twice the
+# largest iteration count seen, plus a margin, at most the solver's own limit). A solve that uses
+# its whole budget raises a device flag, and the loop then reruns eagerly with the full limits.
+BUDGET_FACTOR, BUDGET_MARGIN = 2, 16
 STATUS_CONVERGED, STATUS_INVALID_SUBSET, STATUS_SKIPPED = 0, 3, 6
 
 
@@ -94,7 +100,7 @@ def subpb1_cleanup(U, U_hat, status, grid_shape, threshold: float, edge_ok=None,
     check(
         lib().aldvc_subpb1_cleanup(nx, ny, nz, K._ptr(edges), K._ptr(st), K._ptr(Ud), K._ptr(Uh),
                                    float(threshold), float(eps), FILL_TOL, FILL_MAX_ITER, K._ptr(out), K._ptr(bad),
-                                   None, K._ptr(ws), int(ws.numel()), K.stream_handle()),
+                                   None, SYNC_EVERY, None, None, K._ptr(ws), int(ws.numel()), K.stream_handle()),
         "aldvc_subpb1_cleanup",
     )  # fmt: skip
     return out.cpu().numpy(), bad.cpu().numpy().astype(bool)
@@ -154,6 +160,8 @@ class AdmmLoop:
         self.steps = torch.zeros(1, dtype=torch.int32, device=dev)
         self.hist = torch.full((S, 4), float("nan"), **f64)
         self.pcg_iters = torch.zeros(S, dtype=torch.int32, device=dev)
+        self.fill_iters = torch.zeros(S, dtype=torch.int32, device=dev)  # CG iterations of each cleanup's fill
+        self.capped = torch.zeros(1, dtype=torch.int32, device=dev)  # a budgeted solve used its whole budget
         L = lib()
         self.ws_global = _ws(L.aldvc_admm_global_workspace_bytes(nx, ny, nz))
         self.ws_clean = _ws(L.aldvc_subpb1_cleanup_workspace_bytes(nx, ny, nz))
@@ -175,6 +183,12 @@ class AdmmLoop:
         else:
             self.pcg_tol, self.pcg_max = float(system.pcg_tol), int(system.pcg_max_iter)
         self.graph = None
+        self.graph_budgets = None  # (PCG, fill) iteration budgets the recorded graph was built with
+        self.budgets = None  # learned from the last eager run
+        self.fell_back = False  # the last graph run was redone eagerly (a budget was too small)
+        self._loaded = None
+        self._sync = 0  # sync_every of the launches being enqueued
+        self._limits = None  # (PCG, fill) iteration limits of the launches being enqueued
         self.time_s = 0.0
 
     # -- launches (all on the current stream, no host synchronisation) -------------------------
@@ -185,8 +199,9 @@ class AdmmLoop:
             lib().aldvc_admm_global_step(
                 K._ptr(ops.mats), K._ptr(s.Ae), nx, ny, nz, K._ptr(ops.elem_active), K._ptr(ops.node_active),
                 s.beta, s.mu, s.alpha, K._ptr(self.U1n), K._ptr(self.F2), K._ptr(self.W), K._ptr(self.v),
-                self.pcg_tol, self.pcg_max, K._ptr(self.U2n), K._ptr(self.F2n), K._ptr(self.pcg_iters[step:]),
-                K._ptr(self.stop), K._ptr(self.ws_global), int(self.ws_global.numel()), K.stream_handle(),
+                self.pcg_tol, int(self._limits[0]), K._ptr(self.U2n), K._ptr(self.F2n), K._ptr(self.pcg_iters[step:]),
+                K._ptr(self.stop), self._sync, K._ptr(self.capped) if self._limits[0] < self.pcg_max else None,
+                K._ptr(self.ws_global), int(self.ws_global.numel()), K.stream_handle(),
             ),
             "aldvc_admm_global_step",
         )  # fmt: skip
@@ -231,9 +246,10 @@ class AdmmLoop:
         check(
             lib().aldvc_subpb1_cleanup(
                 nx_, ny_, nz_, K._ptr(self.edge_ok), K._ptr(self.status[step]), K._ptr(self.U1raw), K._ptr(self.U2),
-                float(p.local_outlier_threshold), MEDIAN_EPS, FILL_TOL, FILL_MAX_ITER, K._ptr(self.U1n),
-                K._ptr(self.bad[step]), K._ptr(self.stop), K._ptr(self.ws_clean), int(self.ws_clean.numel()),
-                K.stream_handle(),
+                float(p.local_outlier_threshold), MEDIAN_EPS, FILL_TOL, int(self._limits[1]), K._ptr(self.U1n),
+                K._ptr(self.bad[step]), K._ptr(self.stop), self._sync, K._ptr(self.fill_iters[step:]),
+                K._ptr(self.capped) if self._limits[1] < FILL_MAX_ITER else None, K._ptr(self.ws_clean),
+                int(self.ws_clean.numel()), K.stream_handle(),
             ),
             "aldvc_subpb1_cleanup",
         )  # fmt: skip
@@ -262,10 +278,13 @@ class AdmmLoop:
         self.steps.zero_()
         self.hist.fill_(float("nan"))
         self.pcg_iters.zero_()
+        self.fill_iters.zero_()
+        self.capped.zero_()
         self.status.zero_()
         self.zncc.zero_()
         self.n_iter.zero_()
         self.bad.zero_()
+        self._loaded = (U_local, F_local)
         if self.tuning3 is None:
             self.tuning3 = self._choose_tuning3()
 
@@ -284,22 +303,47 @@ class AdmmLoop:
         return AT.choose_icgn3(args, kw)
 
     def run(self, graph: bool = False):
-        """Executes the loop on the current state (call ``load`` first). With ``graph`` the
-        sequence is recorded once (the first call) and replayed."""
+        """Executes the loop on the current state (call ``load`` first). Eagerly (the default) the
+        CG solves are polled by the host and enqueue nothing after convergence; the run also sets
+        the iteration budgets of later graph recordings. With ``graph`` the sequence is recorded
+        once (the first call, or again after the budgets changed) and replayed; if a budgeted solve
+        used its whole budget, the loop reloads and reruns eagerly with the full limits, so the
+        result is always that of the unbudgeted loop."""
         torch.cuda.synchronize()
         t0 = time.perf_counter()
+        self.fell_back = False
         if not graph:
-            self._enqueue()
+            self._run_eager()
         else:
-            if self.graph is None:
+            if self.graph is None or self.graph_budgets != self._graph_budgets():
                 self.capture()
             self.graph.replay()
+            if int(self.capped.item()):
+                self.fell_back = True
+                self.load(*self._loaded)
+                self._run_eager()
+                self.graph = None  # recorded again with the larger budgets on the next call
         torch.cuda.synchronize()
         self.time_s = time.perf_counter() - t0
 
+    def _run_eager(self):
+        self._sync, self._limits = SYNC_EVERY, (self.pcg_max, FILL_MAX_ITER)
+        self._enqueue()
+        torch.cuda.synchronize()
+        n = int(self.steps.item())
+        pcg = int(self.pcg_iters[1 : n + 1].max().item()) if n >= 1 else 0
+        fill = int(self.fill_iters[2 : n + 1].max().item()) if n >= 2 else 0
+        budget = lambda used, limit: int(min(limit, BUDGET_FACTOR * used + BUDGET_MARGIN))  # noqa: E731
+        self.budgets = (budget(pcg, self.pcg_max), budget(fill, FILL_MAX_ITER))
+
+    def _graph_budgets(self):
+        return self.budgets if self.budgets is not None else (self.pcg_max, FILL_MAX_ITER)
+
     def capture(self):
         """Records the loop into a HIP graph (the state buffers are fixed, so a replay runs the
-        loop on whatever ``load`` put there)."""
+        loop on whatever ``load`` put there), with the iteration budgets of the last eager run (the
+        full limits before any)."""
+        self._sync, self._limits = 0, self._graph_budgets()
         g = torch.cuda.CUDAGraph()
         s = torch.cuda.Stream()
         s.wait_stream(torch.cuda.current_stream())
@@ -307,7 +351,7 @@ class AdmmLoop:
             with torch.cuda.graph(g, stream=s):
                 self._enqueue()
         torch.cuda.current_stream().wait_stream(s)
-        self.graph = g
+        self.graph, self.graph_budgets = g, self._limits
 
     def results(self, info_local: LocalSolveInfo | None = None):
         """``(U_final, F_final, ADMMInfo, status, zncc, bad_last)`` on the host, as the reference

@@ -459,12 +459,13 @@ size_t aldvc_subpb1_cleanup_workspace_bytes(int nx, int ny, int nz) {
 
 int aldvc_subpb1_cleanup(int nx, int ny, int nz, const uint8_t* edge_ok, const int8_t* status, const double* U,
                          const double* U_hat, double threshold, double eps, double fill_tol, int fill_max_iter,
-                         double* U_out, uint8_t* bad, const int* stop, void* workspace, size_t workspace_bytes,
-                         hipStream_t stream) {
+                         double* U_out, uint8_t* bad, const int* stop, int sync_every, int* fill_iterations,
+                         int* capped, void* workspace, size_t workspace_bytes, hipStream_t stream) {
     clear_error();
     if (int rc = check_node_grid(nx, ny, nz)) return rc;
     if (!status || !U || !U_hat || !U_out || !bad || !workspace) return fail(-2, "null pointer");
     if (fill_max_iter < 1) return fail(-2, "fill_max_iter must be at least 1");
+    if (sync_every < 0) return fail(-2, "sync_every must be 0 (all iterations enqueued) or positive");
     if (workspace_bytes < aldvc_subpb1_cleanup_workspace_bytes(nx, ny, nz)) return fail(-3, "workspace too small");
     const NodeGrid g{nx, ny, nz};
     const int64_t N = (int64_t)nx * ny * nz;
@@ -482,8 +483,9 @@ int aldvc_subpb1_cleanup(int nx, int ny, int nz, const uint8_t* edge_ok, const i
                        ctl, stop);
     if (!edge_ok) {
         hipLaunchKernelGGL(k_spring_rhs, dim3(nb), dim3(ABLOCK), 0, stream, op, U, b, X, (const int*)&ctl->skip);
-        (void)pcg::solve(op, N, (const double*)b, X, fill_tol, fill_max_iter, pcg_ws, &ctl->skip, 0, nullptr, nullptr,
-                   stream);
+        if (pcg::solve(op, N, (const double*)b, X, fill_tol, fill_max_iter, pcg_ws, &ctl->skip, sync_every, nullptr,
+                       fill_iterations, stream, capped) != hipSuccess)
+            return fail(-5, "the cleanup's spring-fill solve failed");
         hipLaunchKernelGGL(k_median_pass1<false>, dim3(nb), dim3(ABLOCK), 0, stream, g, edge_ok,
                            (const uint8_t*)unknown, U, (const double*)X, absf, cnt, (const MedCtl*)ctl);
         hipLaunchKernelGGL(k_median_pass2<false>, dim3(nb), dim3(ABLOCK), 0, stream, g, edge_ok, status,

@@ -321,12 +321,13 @@ size_t aldvc_admm_global_workspace_bytes(int nx, int ny, int nz) {
 int aldvc_admm_global_step(const double* mats, const double* Ae, int nx, int ny, int nz, const uint8_t* elem_active,
                            const uint8_t* node_active, double beta, double mu, double alpha, const double* U,
                            const double* F, const double* W, const double* v, double tol, int max_iter, double* U2,
-                           double* F2, int* iterations_dev, const int* gate, void* workspace, size_t workspace_bytes,
-                           hipStream_t stream) {
+                           double* F2, int* iterations_dev, const int* gate, int sync_every, int* capped,
+                           void* workspace, size_t workspace_bytes, hipStream_t stream) {
     clear_error();
     if (int rc = check_grid(nx, ny, nz, elem_active, node_active)) return rc;
     if (!mats || !Ae || !U || !F || !W || !v || !U2 || !F2 || !workspace) return fail(-2, "null pointer");
     if (max_iter < 1) return fail(-2, "max_iter must be at least 1");
+    if (sync_every < 0) return fail(-2, "sync_every must be 0 (all iterations enqueued) or positive");
     if (workspace_bytes < aldvc_admm_global_workspace_bytes(nx, ny, nz)) return fail(-3, "workspace too small");
     const Grid g{nx, ny, nz, elem_active, node_active};
     const int64_t N = (int64_t)nx * ny * nz;
@@ -334,7 +335,9 @@ int aldvc_admm_global_step(const double* mats, const double* Ae, int nx, int ny,
     void* pcg_ws = reinterpret_cast<char*>(workspace) + 256 + sizeof(double) * 3 * (size_t)N;
     hipLaunchKernelGGL(k_admm_rhs, dim3(blocks_for(N)), dim3(GBLOCK), 0, stream, g, mats, beta, mu, alpha, F, W, U, v,
                        b, U2, gate);
-    (void)pcg::solve(FemOp{g, Ae}, N, (const double*)b, U2, tol, max_iter, pcg_ws, gate, 0, nullptr, iterations_dev, stream);
+    if (pcg::solve(FemOp{g, Ae}, N, (const double*)b, U2, tol, max_iter, pcg_ws, gate, sync_every, nullptr,
+                   iterations_dev, stream, capped) != hipSuccess)
+        return fail(-5, "the global-step solve failed");
     hipLaunchKernelGGL(k_nodal_gradient, dim3(blocks_for(N)), dim3(GBLOCK), 0, stream, g, mats, (const double*)U2, F2,
                        gate);
     return launch_status();

@@ -202,10 +202,13 @@ static __global__ __launch_bounds__(BLOCK) void k_iter2(int64_t N, int k, const 
 }
 
 // after the last slot: the iteration limit was reached (converged or not)
-static __global__ void k_close(int max_iter, Ctl* ctl) {
+// After the last slot: a solve still running took all max_iter iterations; `capped` (nullable)
+// records that it did (whether or not the last update met the tolerance).
+static __global__ void k_close(int max_iter, Ctl* ctl, int* capped) {
     if (threadIdx.x == 0 && blockIdx.x == 0 && !ctl->done) {
         ctl->iterations = max_iter;
         ctl->done = 1;
+        if (capped) *capped = 1;
     }
 }
 
@@ -249,7 +252,7 @@ inline Workspace carve(void* ws, int64_t N) {
 template <class Op>
 inline hipError_t solve(const Op& op, int64_t N, const double* B, double* X, double tol, int max_iter, void* ws,
                         const int* gate, int sync_every, int* iterations_host, int* iterations_dev,
-                        hipStream_t st) {
+                        hipStream_t st, int* capped = nullptr) {
     Workspace w = carve(ws, N);
     const unsigned nb = blocks_for(N);
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, w.ctl, gate);
@@ -272,9 +275,9 @@ inline hipError_t solve(const Op& op, int64_t N, const double* B, double* X, dou
         }
     }
     // not stopped by then: max_iter iterations (whether or not the last one converged)
-    hipLaunchKernelGGL(k_close, dim3(1), dim3(64), 0, st, max_iter, w.ctl);
+    hipLaunchKernelGGL(k_close, dim3(1), dim3(64), 0, st, max_iter, w.ctl, capped);
     if (iterations_dev) hipLaunchKernelGGL(k_report, dim3(1), dim3(64), 0, st, (const Ctl*)w.ctl, iterations_dev);
-    if (sync_every > 0) {
+    if (sync_every > 0 && iterations_host) {
         hipError_t e = hipMemcpyAsync(iterations_host, &w.ctl->iterations, sizeof(int), hipMemcpyDeviceToHost, st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);
         if (e != hipSuccess) return e;

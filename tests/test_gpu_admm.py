@@ -136,7 +136,7 @@ def test_global_step_matches_host_solve(gd, ops):
     ins = [t(U1, (N, 3)), t(F1, (N, 9)), t(W, (N, 9)), t(v, (N, 3))]  # held until the calls are done
     args = lambda st: (K._ptr(ops.mats), K._ptr(system.Ae), nx, ny, nz, K._ptr(ops.elem_active), K._ptr(ops.node_active),  # noqa: E731
                        0.5, 2e-3, 0.05, *(K._ptr(x) for x in ins), 1e-8, 500, K._ptr(U2), K._ptr(F2), K._ptr(it),
-                       K._ptr(st), K._ptr(ws), int(ws.numel()), K.stream_handle())  # fmt: skip
+                       K._ptr(st), 0, None, K._ptr(ws), int(ws.numel()), K.stream_handle())  # fmt: skip
     K.check(L.aldvc_admm_global_step(*args(stop)), "global step")
     assert int(it.item()) == info["iterations"]
     assert np.max(np.abs(U2.cpu().numpy() - U2_ref)) < 1e-9
@@ -297,4 +297,31 @@ def test_admm_loop_graph_replays_equal_eager(frame, stops):
         assert np.array_equal(got[0], eager[0]) and np.array_equal(got[1], eager[1]), rep
         assert np.array_equal(got[3], eager[3]) and np.array_equal(got[5], eager[5]), rep
         assert got[2].pcg_iterations == eager[2].pcg_iterations, rep
+        assert not loop.fell_back and loop.graph_budgets == loop.budgets, rep
         _ = torch.randn(1000, device=K.device()).sum().item()  # other work between replays
+    assert loop.budgets[0] < loop.pcg_max  # the graph was recorded with a learned, smaller budget
+
+
+def test_admm_loop_graph_budget_too_small_falls_back(frame):
+    """A graph recorded with CG budgets below what the solves need raises the device flag; the run
+    is redone eagerly with the full limits and equals the eager result, and the next call records
+    the graph again with the budgets of that run."""
+    fr = frame
+    para = _para(admm_tol=1e-12)
+    loop = A.AdmmLoop(fr.ctx, fr.g, fr.mesh, fr.system, para)
+    loop.load(fr.U1, fr.F1)
+    loop.run(graph=False)
+    eager = loop.results(fr.info)
+    learned = loop.budgets
+    loop.budgets = (2, 2)  # far too few iterations for the global and fill solves
+    loop.load(fr.U1, fr.F1)
+    loop.run(graph=True)
+    assert loop.fell_back and loop.graph is None and loop.budgets == learned
+    got = loop.results(fr.info)
+    assert np.array_equal(got[0], eager[0]) and np.array_equal(got[1], eager[1])
+    assert got[2].pcg_iterations == eager[2].pcg_iterations
+    loop.load(fr.U1, fr.F1)
+    loop.run(graph=True)
+    assert not loop.fell_back and loop.graph_budgets == learned
+    again = loop.results(fr.info)
+    assert np.array_equal(again[0], eager[0]) and np.array_equal(again[1], eager[1])
