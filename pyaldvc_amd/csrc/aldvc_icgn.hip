@@ -669,11 +669,12 @@ __global__ __launch_bounds__(BLOCK) void k_ref_moments(IcgnArgs a, const int64_t
 // [zc, zc + kz) (sampling indices) warps into the box spanned by the warped positions of its 8
 // corners (warp_coord is monotone in X, Y and Z). When every corner is inside the sampling domain,
 // so is every sample of the chunk, and the brick holds the stencils of all of them. Returns 0 when
-// the chunk fits (origin and y, z extents set), 1 when only its z extent is too large (retry with
+// the chunk fits (origin and extents set), 1 when only its z extent is too large (retry with
 // fewer planes), 2 when it cannot use a brick.
 template <int MODE, int BRK>
 __device__ inline int plan_chunk(const float* sw, const Subset& s, const Domain& dom, int xc, int kx, int yc, int ky,
-                                 int zc, int kz, int planes, int rows, int& bx, int& by, int& bz, int& ey, int& ez) {
+                                 int zc, int kz, int planes, int rows, int& bx, int& by, int& bz, int& ex, int& ey,
+                                 int& ez) {
     const float Xl = (float)(-s.hx + s.stride * xc), Xh = (float)(-s.hx + s.stride * (xc + kx - 1));
     const float Yl = (float)(-s.hy + s.stride * yc), Yh = (float)(-s.hy + s.stride * (yc + ky - 1));
     const float Zl = (float)(-s.hz + s.stride * zc), Zh = (float)(-s.hz + s.stride * (zc + kz - 1));
@@ -695,7 +696,7 @@ __device__ inline int plan_chunk(const float* sw, const Subset& s, const Domain&
     bx = (int)floorf(lo[0]) + LO;
     by = (int)floorf(lo[1]) + LO;
     bz = (int)floorf(lo[2]) + LO;
-    const int ex = (int)floorf(hi[0]) + HI - bx + 1;
+    ex = (int)floorf(hi[0]) + HI - bx + 1;
     ey = (int)floorf(hi[1]) + HI - by + 1;
     ez = (int)floorf(hi[2]) + HI - bz + 1;
     if (ex > BRK || ey > rows) return 2;
@@ -708,13 +709,16 @@ __device__ inline int plan_chunk(const float* sw, const Subset& s, const Domain&
 // read by an in-domain sample), eight loads in flight per lane, 4-byte LDS stores (the plane
 // stride need not keep rows 16-byte aligned).
 template <int BRK>
-__device__ inline void stage_brick(float* brick, int bplane, int rows, const BufVol& gb, int anchor, int rowp,
-                                   int planep, int bx, int by, int bz, int ez) {
+__device__ inline void stage_brick(float* brick, int bplane, int cols, int rows, const BufVol& gb, int anchor,
+                                   int rowp, int planep, int bx, int by, int bz, int ez) {
     static_assert(BRK % 4 == 0, "brick rows are whole 16-byte groups");
-    constexpr int Q = BRK / 4;
-    const int RQ = rows * Q;  // 16-byte groups per plane
+    const int Q = (cols + 3) >> 2;  // 16-byte groups per row: the columns the chunk's stencils reach
+    const int RQ = rows * Q;        // 16-byte groups per plane
     const int total = RQ * ez;
     const int org = anchor + bz * planep + by * rowp + bx;
+    // rem / Q as a float product: rem < 40 rows x 10 groups, so (rem + 0.5) / Q is at least 0.05
+    // from an integer and the float rounding (< 1e-4) cannot change the floor
+    const float inv_q = 1.0f / (float)Q;
     constexpr int U = 8;
     for (int base = threadIdx.x; base < total; base += U * BLOCK) {
         float4 r[U];
@@ -725,7 +729,7 @@ __device__ inline void stage_brick(float* brick, int bplane, int rows, const Buf
             if (i < total) {
                 int zz = 0;
                 for (int t = RQ; t <= i; t += RQ) ++zz;  // ez is a handful of planes
-                const int rem = i - zz * RQ, yy = rem / Q, qx = rem - yy * Q;
+                const int rem = i - zz * RQ, yy = (int)(((float)rem + 0.5f) * inv_q), qx = rem - yy * Q;
                 r[u] = buf_load4(gb, 4u * (uint32_t)(org + zz * planep + yy * rowp + 4 * qx), 0);
                 dst[u] = zz * bplane + yy * BRK + 4 * qx;
             }
@@ -913,16 +917,16 @@ __global__ __launch_bounds__(BLOCK, 3) void k_icgn12_fused(IcgnArgs a, const int
                     const int kx = min(tile_x, sx - xc), ky = min(tile_y, sy - yc);
                     const bool whole = kx == sx && ky == sy;
                     for (int zc = 0; zc < s.sz;) {
-                        int kz = min(brick_planes - OVER, s.sz - zc), bx = 0, by = 0, bz = 0, ey = 0, ez = 0;
+                        int kz = min(brick_planes - OVER, s.sz - zc), bx = 0, by = 0, bz = 0, ex = 0, ey = 0, ez = 0;
                         int fit = plan_chunk<MODE, BRK>(sw, s, dom, xc, kx, yc, ky, zc, kz, brick_planes, brick_rows,
-                                                        bx, by, bz, ey, ez);
+                                                        bx, by, bz, ex, ey, ez);
                         while (fit == 1 && kz > 1)
                             fit = plan_chunk<MODE, BRK>(sw, s, dom, xc, kx, yc, ky, zc, --kz, brick_planes,
-                                                        brick_rows, bx, by, bz, ey, ez);
+                                                        brick_rows, bx, by, bz, ex, ey, ez);
                         using Brick = std::integral_constant<bool, true>;
                         if (fit == 0) {
                             __syncthreads();  // the previous chunk's samples are done with the brick
-                            stage_brick<BRK>(brick, brick_plane, ey, gb, (int)obase, rowp, planep, bx, by, bz, ez);
+                            stage_brick<BRK>(brick, brick_plane, ex, ey, gb, (int)obase, rowp, planep, bx, by, bz, ez);
                             __syncthreads();
                             if (lane_map == LANES_PATCHES) {
                                 // 8 x 8 patches of one plane per wave (lane l: x + (l & 7), y + (l >> 3)).
