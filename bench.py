@@ -99,6 +99,51 @@ def render_pair(dev, shape, n_beads, disp, seed=0, sigma=SIGMA):
     return V.normalize_volume(f), V.normalize_volume(g)
 
 
+def plain_single_node(cd, P0d, W, L, mf, valid, status_fused, n_sample):
+    """The plain kernel (the reference GPU design) run one node per launch, as a per-node driver
+    would call it: a random sample of the valid nodes (seed 0), launched back to back through the
+    C ABI with preallocated per-node buffers; wall time between two device synchronisations."""
+    import torch
+
+    from pyaldvc_amd import kernels as K
+    from pyaldvc_amd._lib import check, lib
+
+    vidx = np.flatnonzero(valid.cpu().numpy())
+    pick = np.sort(np.random.default_rng(0).choice(vidx, size=min(n_sample, vidx.size), replace=False))
+    dev = cd.device
+    f, g, gx, gy, gz = (W[k] for k in ("f", "g", "gx", "gy", "gz"))
+    nz, ny, nx = (int(v) for v in f.shape)
+    n = len(pick)
+    P = P0d[torch.from_numpy(pick).to(dev)].clone()
+    n_iter = torch.zeros(n, dtype=torch.int32, device=dev)
+    st = torch.zeros(n, dtype=torch.int8, device=dev)
+    zn = torch.zeros(n, dtype=torch.float64, device=dev)
+    one = torch.ones(1, dtype=torch.uint8, device=dev)
+    ws_bytes = int(lib().aldvc_icgn12_workspace_bytes(1, HALF, HALF, HALF, 1, 0))
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+    stream = K.stream_handle()
+    P_ = K._ptr
+    args = [(P_(cd[i : i + 1]), P_(P[k : k + 1]), P_(L[i : i + 1]), P_(mf[i : i + 1]), P_(n_iter[k : k + 1]),
+             P_(st[k : k + 1]), P_(zn[k : k + 1])) for k, i in enumerate(pick)]  # fmt: skip
+    fp, gp, gxp, gyp, gzp, onep, wsp = P_(f), P_(g), P_(gx), P_(gy), P_(gz), P_(one), P_(ws)
+
+    def run():
+        for c, p, l, m, it, s_, z in args:
+            check(lib().aldvc_icgn12(c, 1, p, HALF, HALF, HALF, 1, fp, gxp, gyp, gzp, None, gp, nz, ny, nx, 1, l, None,
+                                     None, 0.0, m, onep, 1e-2, 1e-3, 100, 5, 1, None, None, 0, it, s_, z, 0, wsp,
+                                     ws_bytes, stream), "aldvc_icgn12")  # fmt: skip
+
+    run()  # warm-up (the outputs are recomputed from the same P0 below)
+    P.copy_(P0d[torch.from_numpy(pick).to(dev)])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run()
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    return {"nodes_per_s": n / t, "sample_nodes": n, "seconds": t,
+            "status_equal_fused": float(np.mean(st.cpu().numpy() == status_fused[pick]))}  # fmt: skip
+
+
 def make_workload(dev):
     import torch
 
@@ -228,6 +273,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--plain-steps", type=int, default=2, help="timed steps of the plain kernel (0 skips it)")
+    ap.add_argument("--single-node-sample", type=int, default=1024,
+                    help="valid nodes timed with the plain kernel one node per launch (0 skips it)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-autotune", action="store_true",
@@ -325,6 +372,12 @@ def main():
                   "status_equal_fused": float(np.mean(outg[2].cpu().numpy() == status))}  # fmt: skip
         log(f"fused, gathers only: {t_gather * 1e3:.1f} ms/step, {n_valid / t_gather:,.0f} nodes/s")
 
+    single = None
+    if args.plain_steps > 0 and args.single_node_sample > 0:
+        single = plain_single_node(cd, P0d, W, L, mf, valid, status, args.single_node_sample)
+        log(f"plain, one node per launch: {single['nodes_per_s']:,.0f} nodes/s over {single['sample_nodes']} nodes "
+            f"(statuses equal to the fused run's: {single['status_equal_fused']:.4f})")  # fmt: skip
+
     plain = None
     if args.plain_steps > 0:
         tp, _, outp = timed_steps(lambda: step("plain"), args.plain_steps, 1)
@@ -380,6 +433,8 @@ def main():
             "fused_gather_kernel": gather,
             "plain_kernel": plain,
             "speedup_vs_plain": (plain["ms_per_step"] / (t_step * 1e3)) if plain else None,
+            "plain_single_node": single,
+            "speedup_vs_plain_single_node": (n_valid / t_step / single["nodes_per_s"]) if single else None,
             "speedup_vs_cpu": ((n_valid / t_step) / cpu["value"]) if cpu else None,
             "wall_s_timed": wall,
         },
