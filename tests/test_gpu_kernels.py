@@ -176,7 +176,17 @@ BRICK_CASES = {
     "tiled_linear": dict(F=_rot_z(2.0), t=(0.4, 0.3, -0.5), half=(24, 20, 10), mode=0, ax=28 + 14 * np.arange(5)),
     # corner boxes rotate out of the brick's rows and gather, the others stay in bricks
     "tiled_rotation": dict(F=_rot_z(9.0), t=(0.3, 0.5, -0.6), half=(24, 24, 8), ax=28 + 14 * np.arange(5)),
+    # a cylinder mask on the reference and NaN outside it in the deformed volume: selection and the
+    # NaN correction on x/y boxes
+    "tiled_masked": dict(F=_rot_z(3.0), t=(0.6, -0.4, 0.7), half=(24, 20, 10), ax=28 + 14 * np.arange(5), mask=0.42),
 }
+
+
+def _cylinder(shape, radius_frac):
+    nz, ny, nx = shape
+    y, x = np.mgrid[0:ny, 0:nx]
+    disk = (x - (nx - 1) / 2) ** 2 + (y - (ny - 1) / 2) ** 2 < (radius_frac * min(ny, nx)) ** 2
+    return np.broadcast_to(disk, shape).astype(np.uint8).copy()
 
 
 @pytest.mark.parametrize("variant", ["fused", "fused_gather"])
@@ -190,19 +200,23 @@ def test_icgn12_brick_paths_match_oracle(key, variant):
     f, g, gx, gy, gz = cs["f"], cs["g"], cs["gx"], cs["gy"], cs["gz"]
     if mode == 2:  # B-spline coefficients of g (both sides sample the same array)
         g = ndimage.spline_filter(g, order=3, output=np.float32, mode="mirror")
-    H, L, mf, bf, nv, valid = cpu_ref.precompute_nodes(coords, *half, f, gx, gy, gz, MASK_NONE, 0.5, 1e12)
+    mask = MASK_NONE
+    if "mask" in c:
+        mask = _cylinder(f.shape, c["mask"])
+        g = np.where(mask > 0, g, np.float32(np.nan)).astype(np.float32)
+    H, L, mf, bf, nv, valid = cpu_ref.precompute_nodes(coords, *half, f, gx, gy, gz, mask, 0.5, 1e12)
     U = truth_at(cs, coords)
     P0 = np.zeros((coords.shape[0], 12))
     P0[:, :9] = c["F"].reshape(-1)
     P0[:, 9:] = np.round(U)
-    args = (coords, P0, *half, f, gx, gy, gz, MASK_NONE, g, mode, L, mf, bf, valid, 1e-2, 1e-3, 100, 5)
+    args = (coords, P0, *half, f, gx, gy, gz, mask, g, mode, L, mf, bf, valid, 1e-2, 1e-3, 100, 5)
     ref = cpu_ref.icgn_12dof_parallel(*args)
     got = K.icgn_12dof_parallel(*args, variant=variant)
     assert_icgn_parity(ref, got)
     if key == "escape":
         assert np.any(ref[2] == 2)
     else:
-        assert np.sum(ref[2] == 0) > 0.5 * valid.sum()
+        assert np.sum(ref[2] == 0) > (0.25 if "mask" in c else 0.5) * valid.sum()
 
 
 @pytest.mark.parametrize("key,mu,noise", [("i3_plain", 1e-3, False), ("i3_mu", 0.5, False), ("i3_noise", 1e-3, True)])
@@ -246,6 +260,23 @@ def test_icgn3_brick_paths_match_oracle(half, variant, tmp_path, monkeypatch):
     if variant == "auto":
         assert AT.last_report["key"].startswith("icgn3|")
         assert len(AT.last_report["seconds"]) == len(AT.CANDIDATES)
+
+
+@pytest.mark.parametrize("variant", ["fused", "fused_gather"])
+@pytest.mark.parametrize("split", [False, True], ids=["mask", "mask+split"])
+def test_icgn3_masked_and_split_match_oracle(golden, variant, split):
+    """Translation-only solves with the reference mask (and split keep rows) on both sampling
+    paths of the fused kernel, against the oracle on the same inputs."""
+    coords = golden["coords"]
+    f, gx, gy, gz, m = golden["f"], golden["gx"], golden["gy"], golden["gz"], golden["split_mask"]
+    kw = dict(split_index=np.arange(coords.shape[0], dtype=np.int64), split_keep=golden["split_m0_rows"]) if split else {}
+    extra = (kw["split_index"], kw["split_keep"]) if split else ()
+    H, L, mf, bf, nv, valid = cpu_ref.precompute_nodes(coords, 6, 6, 6, f, gx, gy, gz, m, 0.5, 1e12, 1, *extra)
+    args = (coords, golden["i3_U_old"], golden["i3_F"], golden["i3_vdual"], 6, 6, 6, f, gx, gy, gz, m, golden["g"], 1,
+            H, mf, bf, valid, 1e-3, 1e-2, 1e-3, 100, 5)  # fmt: skip
+    ref = cpu_ref.icgn_3dof_parallel(*args, **kw)
+    assert_icgn_parity(ref, K.icgn_3dof_parallel(*args, **kw, variant=variant))
+    assert np.sum(ref[2] == 0) > 0
 
 
 def test_evaluate_zncc_matches_oracle(golden):
