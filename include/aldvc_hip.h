@@ -219,6 +219,12 @@ int aldvc_bspline_prefilter(const float *vol, int64_t nz, int64_t ny, int64_t nx
  * (prepare_deformed). */
 int aldvc_mask_nan(float *vol, const uint8_t *mask, int64_t n, hipStream_t stream);
 
+/* Block mean of vol (nz, ny, nx) by an integer factor into out (nz/f, ny/f, nx/f), trailing voxels
+ * dropped; each mean summed in float64 and rounded once to float32. Replaces
+ * solver/integer_search.py:235-243 (block_downsample). */
+int aldvc_block_downsample(const float *vol, int64_t nz, int64_t ny, int64_t nx, int factor, float *out,
+                           hipStream_t stream);
+
 /* ---- ADMM global step on the hex8 node grid (float64). A grid of nx*ny*nz nodes, node
  * n = (iz*ny + iy)*nx + ix; elem_active (nz-1)(ny-1)(nx-1) and node_active (nz)(ny)(nx) uint8;
  * element matrices 8x8 row-major in the grid's corner order; vectors (N, 3). */

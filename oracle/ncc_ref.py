@@ -11,6 +11,16 @@ the float64 direct ZNCC map of ``cpu_ref.zncc_direct``:
 * ``ncc_search``          integer_search.py:329-483 with the direct engine (the path the
                           reference takes whenever offsets x template <= 3e7, :395)
 * ``ncc_search_expanding`` integer_search.py:486-524
+* ``block_downsample``     integer_search.py:235-243
+* ``phase_correlation_shift`` integer_search.py:246-294 (numpy FFT)
+* ``coarse_radius`` / ``auto_pyramid_levels`` integer_search.py:566-614
+* ``pyramid_search``       integer_search.py:617-694; the between-level cleaning (median test +
+                          spring inpainting, integer_search.py:545-556) is passed in as ``clean``
+                          so that a test can supply a separately pinned implementation
+
+Pinned by closed forms in tests/test_oracle_init_guess.py: block means of known arrays, an exact
+cyclic shift recovered by the phase correlation, the coarse-radius and level rules on hand-worked
+shapes, and a rigid integer translation recovered exactly by the pyramid.
 """
 
 from __future__ import annotations
@@ -181,3 +191,115 @@ def ncc_search_expanding(f, g, coords, subset, radius, shift0=None, auto_expand=
         res["expansions"] += 1
         res["radius"] = rad
     return res
+
+
+# ---------------------------------------------------------------------------- coarse-to-fine search
+PYRAMID_FINE_RADIUS = 2  # integer_search.py:33-36
+COARSE_RADIUS_MARGIN = 1
+MIN_COARSE_RADIUS = 2
+MIN_PYRAMID_SUBSET = 8
+MIN_COARSE_STD_RATIO = 0.35  # integer_search.py:563
+
+
+def block_downsample(vol, factor):
+    """integer_search.py:235-243 (NumPy's float32 mean over the blocks)."""
+    v = np.asarray(vol, dtype=np.float32)
+    if factor <= 1:
+        return np.ascontiguousarray(v)
+    nz, ny, nx = (s // factor * factor for s in v.shape)
+    v = v[:nz, :ny, :nx]
+    out = v.reshape(nz // factor, factor, ny // factor, factor, nx // factor, factor).mean(axis=(1, 3, 5), dtype=np.float32)
+    return np.ascontiguousarray(out, dtype=np.float32)
+
+
+def phase_correlation_shift(f, g, max_dim=128):
+    """integer_search.py:246-294 (whole volume): ``(dx, dy, dz)``."""
+    f = np.asarray(f, dtype=np.float32)
+    g = np.asarray(g, dtype=np.float32)
+    factor = max(1, int(np.ceil(max(f.shape) / max_dim)))
+    fs = block_downsample(f, factor).astype(np.float64)
+    gs = block_downsample(g, factor).astype(np.float64)
+    fs -= fs.mean()
+    gs -= gs.mean()
+    win = np.ones(fs.shape)
+    for ax, n in enumerate(fs.shape):
+        shape = [1, 1, 1]
+        shape[ax] = n
+        win = win * np.hanning(n).reshape(shape)
+    R = np.fft.fftn(gs * win) * np.conj(np.fft.fftn(fs * win))
+    mag = np.abs(R)
+    R = R / np.where(mag > 1e-12, mag, 1.0)
+    r = np.real(np.fft.ifftn(R))
+    shift = np.array(np.unravel_index(int(np.argmax(r)), r.shape), dtype=np.float64)
+    for ax, n in enumerate(r.shape):
+        if shift[ax] > n // 2:
+            shift[ax] -= n
+    dz, dy, dx = shift * factor
+    return np.array([dx, dy, dz])
+
+
+def coarse_radius(radius, fac):
+    """integer_search.py:566-572."""
+    return tuple(max(MIN_COARSE_RADIUS, -(-int(r) // fac) + COARSE_RADIUS_MARGIN) for r in radius)
+
+
+def auto_pyramid_levels(shape, subset, radius, max_levels=3, f=None, min_std_ratio=MIN_COARSE_STD_RATIO):
+    """integer_search.py:575-614 (standard deviations in float64)."""
+    levels = 0
+    std0 = float(np.std(np.asarray(f, dtype=np.float64))) if f is not None else None
+    for lv in range(1, max_levels + 1):
+        fac = 2**lv
+        fits = True
+        for n, w, r in zip(tuple(shape)[::-1], subset, coarse_radius(radius, fac)):
+            w_l = max(MIN_PYRAMID_SUBSET, (w // fac) // 2 * 2)
+            if n // fac < w_l + 1 + 2 * max(r, 1) + 4:
+                fits = False
+        if fits and std0 is not None and std0 > 0:
+            if float(np.std(block_downsample(f, fac).astype(np.float64))) < min_std_ratio * std0:
+                fits = False
+        if fits:
+            levels = lv
+        else:
+            break
+    return levels
+
+
+def pyramid_search(f, g, coords, grid_shape, subset, radius, levels, clean, global_shift=None, outlier_threshold=2.0,
+                   auto_expand=True, max_expand=3, fine_radius=PYRAMID_FINE_RADIUS, valid=None):
+    """integer_search.py:617-694; ``levels <= 0`` selects the level count automatically
+    (integer_search.py:648-650); ``clean(disp (N, 3), grid_shape, threshold)`` is the between-level
+    median test + spring inpainting."""
+    if levels <= 0:
+        levels = auto_pyramid_levels(np.shape(f), subset, radius, f=f)
+    coords = np.asarray(coords, dtype=np.int64)
+    N = coords.shape[0]
+    active = np.arange(N) if valid is None else np.flatnonzero(np.asarray(valid, dtype=bool))
+    if active.size == 0:
+        active = np.arange(N)
+    disp_full = np.zeros((N, 3))
+    if global_shift is not None:
+        disp_full += np.asarray(global_shift, dtype=np.float64)[None, :]
+    out = {"per_level": []}
+    for lv in range(levels, -1, -1):
+        fac = 2**lv
+        fl = block_downsample(f, fac)
+        gl = block_downsample(g, fac)
+        subset_l = tuple(max(MIN_PYRAMID_SUBSET, (w // fac) // 2 * 2) for w in subset)
+        rad_l = coarse_radius(radius, fac) if lv == levels else (int(fine_radius),) * 3
+        coords_l = np.round(coords[active] / fac).astype(np.int64)
+        shift_l = np.round(disp_full[active] / fac).astype(np.int64)
+        sub = ncc_search_expanding(fl, gl, coords_l, subset_l, rad_l, shift_l, auto_expand, max_expand)
+        res = {k: (np.full((N, 3), np.nan) if k == "disp" else (np.zeros(N, bool) if k in ("ok", "clipped") else np.full(N, np.nan)))
+               for k in ("disp", "cc", "pce", "clipped", "ok")}  # fmt: skip
+        for k in res:
+            res[k][active] = sub[k]
+        d = res["disp"] * fac
+        d[~res["ok"]] = np.nan
+        n_ok = int(res["ok"].sum())
+        out["per_level"].append({"level": lv, "n_ok": n_ok, "radius": sub["radius"], "subset": subset_l})
+        if n_ok == 0:
+            continue
+        disp_full = clean(d, grid_shape, outlier_threshold)
+        out.update(ok=res["ok"], pce=res["pce"], cc=res["cc"], clipped=res["clipped"])
+    out["disp"] = disp_full
+    return out

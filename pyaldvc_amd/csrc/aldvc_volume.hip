@@ -158,6 +158,26 @@ __global__ void k_mask_nan(float* __restrict__ vol, const uint8_t* __restrict__ 
         if (mask[i] == 0) vol[i] = __builtin_nanf("");
 }
 
+// Block mean by an integer factor (trailing voxels dropped): one output voxel per thread, its f^3
+// inputs summed in float64 (z, y, x order) and the mean rounded once to float32
+// (integer_search.py:235-243, block_downsample).
+__global__ void k_block_downsample(const float* __restrict__ vol, int64_t ny, int64_t nx, int fac, int64_t oz,
+                                   int64_t oy, int64_t ox, float* __restrict__ out) {
+    const int64_t n = oz * oy * ox;
+    const double inv = 1.0 / ((double)fac * fac * fac);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t z = i / (oy * ox), r = i - z * oy * ox, y = r / ox, x = r - y * ox;
+        const float* b = vol + ((z * fac) * ny + y * fac) * nx + x * fac;
+        double acc = 0.0;
+        for (int dz = 0; dz < fac; ++dz)
+            for (int dy = 0; dy < fac; ++dy) {
+                const float* row = b + ((int64_t)dz * ny + dy) * nx;
+                for (int dx = 0; dx < fac; ++dx) acc += (double)row[dx];
+            }
+        out[i] = (float)(acc * inv);
+    }
+}
+
 unsigned grid_for(int64_t n) {
     const int64_t b = (n + 255) / 256;
     return (unsigned)(b < 65536 ? (b > 0 ? b : 1) : 65536);
@@ -238,6 +258,19 @@ int aldvc_bspline_prefilter(const float* vol, int64_t nz, int64_t ny, int64_t nx
     hipLaunchKernelGGL(k_bspline_axis<0>, g, dim3(256), 0, stream, vol, out, nz, ny, nx);
     hipLaunchKernelGGL(k_bspline_axis<1>, g, dim3(256), 0, stream, (const float*)out, scratch, nz, ny, nx);
     hipLaunchKernelGGL(k_bspline_axis<2>, g, dim3(256), 0, stream, (const float*)scratch, out, nz, ny, nx);
+    return launch_status();
+}
+
+int aldvc_block_downsample(const float* vol, int64_t nz, int64_t ny, int64_t nx, int factor, float* out,
+                           hipStream_t stream) {
+    clear_error();
+    if (!vol || !out) return fail(-2, "null pointer");
+    if (factor < 1) return fail(-2, "factor must be at least 1");
+    if (nz < 0 || ny < 0 || nx < 0) return fail(-2, "negative volume shape");
+    const int64_t oz = nz / factor, oy = ny / factor, ox = nx / factor;
+    if (oz * oy * ox == 0) return 0;
+    hipLaunchKernelGGL(k_block_downsample, dim3(grid_for(oz * oy * ox)), dim3(256), 0, stream, vol, ny, nx, factor, oz,
+                       oy, ox, out);
     return launch_status();
 }
 

@@ -85,6 +85,13 @@ class NodeMesh:
         e = self.edge_ok
         return e is not None and e.shape == (self.n_nodes, 3) and not bool(e.all())
 
+    def edges_ok(self):
+        """``(N, 3)`` edge flags, all True for a mesh that was not cut (data_structures.py:276-280)."""
+        e = self.edge_ok
+        if e is not None and np.shape(e) == (self.n_nodes, 3):
+            return np.asarray(e, dtype=bool)
+        return np.ones((self.n_nodes, 3), dtype=bool)
+
     def edges_ok_grid(self):
         """``(nz, ny, nx, 3)`` edge flags, or ``None`` when nothing was cut."""
         return np.asarray(self.edge_ok).reshape(tuple(self.grid_shape) + (3,)) if self.is_cut else None
