@@ -281,11 +281,13 @@ def main():
                     help="time the fused kernel's default launch shape without running the autotuner first")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run one device copy of this many MiB first (the scale of counter-based HBM traffic)")
-    ap.add_argument("--workload", default="icgn512", choices=["icgn512", "ncc-bf16", "warp-residual", "masked-admm"],
+    ap.add_argument("--workload", default="icgn512", choices=["icgn512", "ncc-bf16", "warp-residual", "masked-admm", "init-guess"],
                     help="icgn512: the batched 12-DOF IC-GN at 512^3 (the headline line); ncc-bf16: the BF16 MFMA "
                          "NCC search at 512^3; warp-residual: the differentiable warp residual at 256^3; "
                          "masked-admm: masked anisotropic subsets at 1024x1024x306 with the ADMM iterations "
-                         "recorded into a HIP graph")  # fmt: skip
+                         "recorded into a HIP graph; init-guess: the initial guess end to end at 512^3")  # fmt: skip
+    ap.add_argument("--cpu-stride", type=int, default=3,
+                    help="init-guess CPU leg: every n-th node per axis of the lattice")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the local step shards by node blocks with no exchange, "
@@ -297,6 +299,8 @@ def main():
         return main_warp(args)
     if args.workload == "masked-admm":
         return main_admm(args)
+    if args.workload == "init-guess":
+        return main_init(args)
 
     import torch
 
@@ -565,6 +569,149 @@ def main_ncc(args):
         "cpu_baseline": None,
     }
     print(json.dumps(result), flush=True)
+
+
+def _init_para(**kw):
+    """The reference's initial-guess defaults (config.py:61-71) with a 32^3 window."""
+    from types import SimpleNamespace
+
+    para = dict(init_guess_method="pyramid", winsize=(WINSIZE,) * 3, init_subset=None, search_radius=(8, 8, 8),
+                global_shift=True, voi=None, pyramid_levels=0, pyramid_fine_radius=2, ncc_auto_expand=True,
+                ncc_max_expand=3, init_outlier_threshold=2.0, init_min_pce=0.0)  # fmt: skip
+    para.update(kw)
+    return SimpleNamespace(**para)
+
+
+def cpu_init_baseline(W, stride, para):
+    """The oracle's coarse-to-fine search (oracle/ncc_ref.py, float64 direct ZNCC maps on every host
+    core) on the node sub-lattice of every ``stride``-th node per axis, cleaned between levels by the
+    host median test and spring fill."""
+    from oracle import cpu_ref, ncc_ref
+    from pyaldvc_amd import cleanup
+
+    f = W["f"].cpu().numpy()
+    g = W["g"].cpu().numpy()
+    n_ax = (SHAPE[0] - WINSIZE) // STEP + 1
+    sel = np.arange(0, n_ax, stride)
+    iz, iy, ix = np.meshgrid(sel, sel, sel, indexing="ij")
+    pick = ((iz * n_ax + iy) * n_ax + ix).ravel()
+    grid = (sel.size,) * 3
+
+    def clean(d, grid_shape, threshold):
+        d = np.array(d, dtype=np.float64).reshape(tuple(grid_shape) + (3,))
+        ok = np.all(np.isfinite(d), axis=-1)
+        if threshold > 0 and ok.sum() > 27:
+            d[cleanup.universal_median_test(d, ok, threshold)] = np.nan
+        return np.stack([cleanup.fill_nan_grid(d[..., c]) for c in range(3)], -1).reshape(-1, 3)
+
+    subset = tuple(min(int(w), 16) for w in para.winsize)
+    t0 = time.perf_counter()
+    shift = ncc_ref.phase_correlation_shift(f, g)
+    info = ncc_ref.pyramid_search(f, g, W["coords"][pick], grid, subset, para.search_radius, 0, clean,
+                                  global_shift=shift, outlier_threshold=para.init_outlier_threshold)  # fmt: skip
+    t = time.perf_counter() - t0
+    err = np.linalg.norm(info["disp"] - W["U_true"][pick], axis=1)
+    return {
+        "value": pick.size / t,
+        "unit": "nodes/s",
+        "cores": cpu_ref.num_threads(),
+        "kind": "port",
+        "sample": f"{pick.size} nodes: one node in {stride} per axis of the same 61^3 lattice ({grid[0]}^3 grid), "
+                  f"phase correlation + pyramid, {t:.1f}s",
+        "median_err_vox": float(np.median(err)),
+        "cpu": _cpu_model(),
+    }
+
+
+def main_init(args):
+    """The initial guess end to end on the 512^3 headline workload (2 % z strain + 5 deg rotation,
+    |u| up to 31.6 voxels, 61^3 nodes): the reference's default method -- global phase-correlation
+    shift, automatic block-mean pyramid, 16^3 NCC templates, radius 8 with expansion, median test
+    and spring fill between levels and at the end -- then the 12-DOF IC-GN seeded by round(U0)."""
+    import torch
+
+    from pyaldvc_amd import init_guess as IG
+    from pyaldvc_amd import kernels as K
+    from pyaldvc_amd.local import NodeMesh
+
+    dev = K.device()
+    W = make_workload(dev)
+    coords = W["coords"]
+    N = coords.shape[0]
+    n_ax = (SHAPE[0] - WINSIZE) // STEP + 1
+    mesh = NodeMesh(coords.astype(np.float64), (n_ax,) * 3, np.ones(N, dtype=bool))
+    para = _init_para()
+
+    def run():
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        U0, info = IG.compute_initial_guess(W["f"], W["g"], mesh, para)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t, U0, info
+
+    run()  # warm-up: FFT plans, allocator
+    times = []
+    for _ in range(max(1, min(args.steps, 3))):
+        t, U0, info = run()
+        times.append(t)
+    t_init = float(np.median(times))
+    err0 = np.linalg.norm(U0 - W["U_true"], axis=1)
+    levels = [{k: (list(v) if isinstance(v, tuple) else v) for k, v in p.items()} for p in info.get("per_level", [])]
+    log(f"initial guess {t_init * 1e3:.0f} ms ({N / t_init:,.0f} nodes/s), {info.get('levels')} coarse levels, "
+        f"{info.get('expansions')} expansions, {info['n_bad']} nodes inpainted; |U0 - u| median {np.median(err0):.3f}, "
+        f"max {err0.max():.2f} voxel")  # fmt: skip
+    # the local step from that seed, as the pipeline runs it
+    cd = torch.from_numpy(coords).to(dev)
+    nomask = np.ones((1, 1, 1), np.uint8)
+    H, L, mf, bf, nv, valid = K.precompute_nodes(cd, HALF, HALF, HALF, W["f"], W["gx"], W["gy"], W["gz"], nomask, 0.5, 1e12)
+    P0 = np.zeros((N, 12))
+    P0[:, 9:] = np.round(U0)
+    P, it, st, zn = K.icgn_12dof_parallel(cd, torch.from_numpy(P0).to(dev), HALF, HALF, HALF, W["f"], W["gx"], W["gy"],
+                                          W["gz"], nomask, W["g"], 1, L, mf, bf, valid, 1e-2, 1e-3, 100, 5)  # fmt: skip
+    P, st = P.cpu().numpy(), st.cpu().numpy()
+    conv = st == 0
+    e = np.linalg.norm(P[conv, 9:] - W["U_true"][conv], axis=1)
+    seeded_true = np.round(W["U_true"])
+    log(f"IC-GN from the pyramid seed: {int(conv.sum())}/{N} converged, median error {np.median(e):.4f} voxel; "
+        f"seed equals round(u) at {np.mean(np.all(np.round(U0) == seeded_true, axis=1)):.4f} of the nodes")  # fmt: skip
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_init_baseline(W, args.cpu_stride, para)
+        log(f"cpu baseline {cpu['value']:,.0f} nodes/s on {cpu['cores']} threads ({cpu['sample']})")
+    result = {
+        "metric": "initial-guess nodes/s at 512^3 (phase correlation + block-mean pyramid NCC + cleaning)",
+        "value": N / t_init,
+        "unit": "nodes/s",
+        "n_gpus": 1,
+        "steps": len(times),
+        "warmup": 1,
+        "ms_per_step": t_init * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "512^3 f32 bead pair (250k beads sigma 1.5, noise 0.01), 2% z strain + 5 deg z rotation; "
+                        "61^3 nodes (step 8), reference initial-guess defaults: global phase-correlation shift, "
+                        "automatic pyramid, 16^3 templates, radius 8 with up to 3 doublings, median test 2.0",
+            "nodes": N,
+            "levels": info.get("levels"),
+            "expansions": info.get("expansions"),
+            "per_level": levels,
+            "global_shift": None if info.get("global_shift") is None else [float(v) for v in info["global_shift"]],
+            "nodes_inpainted": info["n_bad"],
+            "guess_err_median_vox": float(np.median(err0)),
+            "guess_err_max_vox": float(err0.max()),
+            "seed_equals_round_u": float(np.mean(np.all(np.round(U0) == seeded_true, axis=1))),
+            "icgn_from_guess": {"converged": int(conv.sum()), "median_err_vox": float(np.median(e)),
+                                "p95_err_vox": float(np.percentile(e, 95))},  # fmt: skip
+            "step_times_s": times,
+        },
+        "roofline": None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(result, default=float), flush=True)
 
 
 def main_warp(args):
