@@ -349,6 +349,21 @@ int aldvc_warp_znssd_backward(const float *f, const float *g, int64_t nz, int64_
                               double sz, const double *stats, const double *grad_C, double *grad_u, float *grad_g,
                               hipStream_t stream);
 
+/* ---- Post-step: per-node displacement uncertainty (float64). */
+
+/* Standard deviation of u, v, w (out: n x 3, voxels) at every node from its IC-GN Hessian H
+ * (n x 12 x 12), final zncc, status (null: all converged), valid, n_valid and bottomf; NaN where
+ * no estimate exists (not valid, zncc not finite, n_valid <= 14, or status not converged).
+ * moment_diag is a HOST array of 4 doubles: the diagonal of the sampled subset's moment matrix
+ * (sums of X^2, Y^2, Z^2 and the voxel count); gain is the gradient stencil's noise gain and
+ * noise_correction selects the two-term model (0: 2 s^2 H^-1 with the stored Hessian).
+ * *singular (device int, cleared by the caller) is set to 1 when a node that gets a value has an
+ * exactly singular corrected Hessian; the caller then recomputes the batch with the
+ * pseudo-inverse. Replaces solver/uncertainty.py:94-133 (displacement_uncertainty). */
+int aldvc_displacement_uncertainty(const double *H, const double *zncc, const int8_t *status, const uint8_t *valid,
+                                   const int64_t *n_valid, const double *bottomf, int64_t n, const double *moment_diag,
+                                   double gain, int noise_correction, double *out, int *singular, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

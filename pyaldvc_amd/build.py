@@ -23,6 +23,7 @@ SOURCES = [
     "aldvc_global.hip",
     "aldvc_warp.hip",
     "aldvc_admm.hip",
+    "aldvc_uncertainty.hip",
 ]
 ARCH = os.environ.get("ALDVC_OFFLOAD_ARCH", "gfx950")
 
