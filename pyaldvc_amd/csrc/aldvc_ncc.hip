@@ -20,6 +20,7 @@ constexpr int BLOCK = 256;
 constexpr int NWAVES = BLOCK / 64;
 constexpr int RX = 8;  // consecutive x offsets per lane (register blocking of the window row)
 constexpr int LDS_FLOATS_MAX = 160 * 1024 / 4 - 1024;
+constexpr size_t SMALL_LDS_MAX = 80 * 1024;  // two small-window workgroups per CU at the most
 
 struct NccArgs {
     const float* f;
@@ -39,32 +40,13 @@ __device__ inline void fill_map(float* o, int64_t n_off, float v) {
     for (int64_t i = threadIdx.x; i < n_off; i += BLOCK) o[i] = v;
 }
 
-// Direct float32 correlation: one workgroup per node, the centred template in LDS, every lane
-// owns RX consecutive x offsets of one (dz, dy) output row and slides along the window rows.
-__global__ __launch_bounds__(BLOCK) void k_zncc_map_f32(NccArgs a) {
-    extern __shared__ float tpl[];
+// The node's template into LDS, centred by its mean: float64 mean and energy (sum T~^2). Returns
+// false for a flat template (energy below 1e-20). Block-uniform; every thread must call it.
+__device__ inline bool centred_template(const NccArgs& a, float* tpl, int64_t x0, int64_t y0, int64_t z0, int sx,
+                                        int sy, int n_tpl, float& mf_out, double& ssf_out) {
     __shared__ double red[NWAVES];
     __shared__ double bc[2];
-    const int64_t node = blockIdx.x;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int sx = 2 * a.hx + 1, sy = 2 * a.hy + 1, sz = 2 * a.hz + 1;
-    const int vx = 2 * a.rx + 1, vy = 2 * a.ry + 1, vz = 2 * a.rz + 1;
-    const int wx = sx + vx - 1;
-    const int n_tpl = sx * sy * sz;
-    const int64_t n_off = (int64_t)vx * vy * vz;
-    float* o = a.out + node * n_off;
-    const int64_t x0 = a.coords[3 * node], y0 = a.coords[3 * node + 1], z0 = a.coords[3 * node + 2];
-    const int64_t xb = a.wx0[node], yb = a.wy0[node], zb = a.wz0[node];
-    // a template or a search window that leaves the volume is treated like t_ok == 0
-    const bool tpl_in = x0 - a.hx >= 0 && x0 + a.hx < a.nx && y0 - a.hy >= 0 && y0 + a.hy < a.ny && z0 - a.hz >= 0 &&
-                        z0 + a.hz < a.nz;
-    const bool win_in = xb >= 0 && xb + wx <= a.nx && yb >= 0 && yb + sy + vy - 1 <= a.ny && zb >= 0 &&
-                        zb + sz + vz - 1 <= a.nz;
-    if (!a.t_ok[node] || !tpl_in || !win_in) {
-        fill_map(o, n_off, -2.0f);
-        return;
-    }
-    // template and its mean (float64 sums)
     double s = 0.0;
     for (int v = tid; v < n_tpl; v += BLOCK) {
         const int k = v / (sy * sx), r = v - k * sy * sx, j = r / sx, i = r - j * sx;
@@ -98,8 +80,37 @@ __global__ __launch_bounds__(BLOCK) void k_zncc_map_f32(NccArgs a) {
         bc[1] = t;
     }
     __syncthreads();
-    const double ssf = bc[1];
-    if (ssf < 1e-20) {
+    mf_out = mf;
+    ssf_out = bc[1];
+    return !(ssf_out < 1e-20);
+}
+
+// Direct float32 correlation: one workgroup per node, the centred template in LDS, every lane
+// owns RX consecutive x offsets of one (dz, dy) output row and slides along the window rows.
+__global__ __launch_bounds__(BLOCK) void k_zncc_map_f32(NccArgs a) {
+    extern __shared__ float tpl[];
+    const int64_t node = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int sx = 2 * a.hx + 1, sy = 2 * a.hy + 1, sz = 2 * a.hz + 1;
+    const int vx = 2 * a.rx + 1, vy = 2 * a.ry + 1, vz = 2 * a.rz + 1;
+    const int wx = sx + vx - 1;
+    const int n_tpl = sx * sy * sz;
+    const int64_t n_off = (int64_t)vx * vy * vz;
+    float* o = a.out + node * n_off;
+    const int64_t x0 = a.coords[3 * node], y0 = a.coords[3 * node + 1], z0 = a.coords[3 * node + 2];
+    const int64_t xb = a.wx0[node], yb = a.wy0[node], zb = a.wz0[node];
+    // a template or a search window that leaves the volume is treated like t_ok == 0
+    const bool tpl_in = x0 - a.hx >= 0 && x0 + a.hx < a.nx && y0 - a.hy >= 0 && y0 + a.hy < a.ny && z0 - a.hz >= 0 &&
+                        z0 + a.hz < a.nz;
+    const bool win_in = xb >= 0 && xb + wx <= a.nx && yb >= 0 && yb + sy + vy - 1 <= a.ny && zb >= 0 &&
+                        zb + sz + vz - 1 <= a.nz;
+    if (!a.t_ok[node] || !tpl_in || !win_in) {
+        fill_map(o, n_off, -2.0f);
+        return;
+    }
+    double ssf;
+    float mf;
+    if (!centred_template(a, tpl, x0, y0, z0, sx, sy, n_tpl, mf, ssf)) {
         fill_map(o, n_off, -2.0f);
         return;
     }
@@ -150,6 +161,139 @@ __global__ __launch_bounds__(BLOCK) void k_zncc_map_f32(NccArgs a) {
             }
             o[((int64_t)oz * vy + oy) * vx + ox0 + r] = val;
         }
+    }
+}
+
+// ---------------------------------------------------------------------------- small search windows
+// Search radii up to 4 along x (the pyramid's refinement and coarse levels, 5^3 .. 9^3 offsets):
+// k_zncc_map_f32 would leave most of a workgroup idle (one lane per output row). Here the whole
+// centred window and template sit in LDS and the work is split over template planes as well:
+// item (tz, oz, oy) correlates template plane tz with the window rows of output row (oz, oy),
+// RX = 2 rx + 1 outputs per lane with a sliding register window, and writes a partial numerator;
+// the partials are then summed over tz in a fixed order (deterministic). The window variance comes
+// from separable float32 box sums (x, then y, then z) of the centred window instead of being
+// re-accumulated per output. LDS (floats): template sx sy sz + window wx wy wz + max(x-pass sums
+// 2 wz wy vx, partials sz vz vy vx) + y-pass sums 2 wz vy vx.
+__host__ __device__ inline int64_t small_lds_floats(int sx, int sy, int sz, int vx, int vy, int vz) {
+    const int64_t wx = sx + vx - 1, wy = sy + vy - 1, wz = sz + vz - 1;
+    const int64_t xs = 2 * wz * wy * vx, part = (int64_t)sz * vz * vy * vx;
+    return (int64_t)sx * sy * sz + wx * wy * wz + (xs > part ? xs : part) + 2 * wz * vy * vx;
+}
+
+template <int RX>
+__global__ __launch_bounds__(BLOCK) void k_zncc_map_small(NccArgs a) {
+    extern __shared__ float sm[];
+    const int64_t node = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int sx = 2 * a.hx + 1, sy = 2 * a.hy + 1, sz = 2 * a.hz + 1;
+    const int vy = 2 * a.ry + 1, vz = 2 * a.rz + 1;  // vx == RX
+    const int wx = sx + RX - 1, wy = sy + vy - 1, wz = sz + vz - 1;
+    const int n_tpl = sx * sy * sz;
+    const int n_off = RX * vy * vz;
+    float* o = a.out + node * (int64_t)n_off;
+    const int64_t x0 = a.coords[3 * node], y0 = a.coords[3 * node + 1], z0 = a.coords[3 * node + 2];
+    const int64_t xb = a.wx0[node], yb = a.wy0[node], zb = a.wz0[node];
+    const bool tpl_in = x0 - a.hx >= 0 && x0 + a.hx < a.nx && y0 - a.hy >= 0 && y0 + a.hy < a.ny && z0 - a.hz >= 0 &&
+                        z0 + a.hz < a.nz;
+    const bool win_in = xb >= 0 && xb + wx <= a.nx && yb >= 0 && yb + wy <= a.ny && zb >= 0 && zb + wz <= a.nz;
+    if (!a.t_ok[node] || !tpl_in || !win_in) {
+        fill_map(o, n_off, -2.0f);
+        return;
+    }
+    float* tpl = sm;
+    float* win = tpl + n_tpl;                  // wz x wy x wx, centred by the template mean
+    float* xs1 = win + wz * wy * wx;           // wz x wy x RX row sums (then the partials)
+    float* xs2 = xs1 + wz * wy * RX;
+    float* part = xs1;                         // sz x n_off partial numerators, after the y pass
+    const int n_x = wz * wy * RX, n_part = sz * n_off;
+    float* ys1 = xs1 + (2 * n_x > n_part ? 2 * n_x : n_part);  // wz x vy x RX box sums over (x, y)
+    float* ys2 = ys1 + wz * vy * RX;
+    double ssf;
+    float mf;
+    if (!centred_template(a, tpl, x0, y0, z0, sx, sy, n_tpl, mf, ssf)) {
+        fill_map(o, n_off, -2.0f);
+        return;
+    }
+    const float c = mf;  // window pilot: sum T~ (g - c) = sum T~ g because sum T~ = 0
+    const int n_win = wz * wy * wx;
+    for (int v = tid; v < n_win; v += BLOCK) {
+        const int k = v / (wy * wx), r = v - k * wy * wx, j = r / wx, i = r - j * wx;
+        win[v] = a.g[((zb + k) * a.ny + (yb + j)) * (int64_t)a.nx + (xb + i)] - c;
+    }
+    __syncthreads();
+    // x pass: sums over sx consecutive columns of every window row
+    for (int v = tid; v < n_x; v += BLOCK) {
+        const int row = v / RX, ox = v - row * RX;
+        const float* w = win + row * wx + ox;
+        float p1 = 0.0f, p2 = 0.0f;
+        for (int tx = 0; tx < sx; ++tx) {
+            p1 += w[tx];
+            p2 += w[tx] * w[tx];
+        }
+        xs1[v] = p1;
+        xs2[v] = p2;
+    }
+    __syncthreads();
+    // y pass: sums over sy consecutive rows
+    const int n_y = wz * vy * RX;
+    for (int v = tid; v < n_y; v += BLOCK) {
+        const int z = v / (vy * RX), r = v - z * vy * RX, oy = r / RX, ox = r - oy * RX;
+        const int b = (z * wy + oy) * RX + ox;
+        float p1 = 0.0f, p2 = 0.0f;
+        for (int ty = 0; ty < sy; ++ty) {
+            p1 += xs1[b + ty * RX];
+            p2 += xs2[b + ty * RX];
+        }
+        ys1[v] = p1;
+        ys2[v] = p2;
+    }
+    __syncthreads();
+    // numerators: item (tz, oz, oy), RX outputs along x
+    const int items = sz * vz * vy;
+    for (int it = tid; it < items; it += BLOCK) {
+        const int tz = it / (vz * vy), r = it - tz * vz * vy, oz = r / vy, oy = r - oz * vy;
+        float acc[RX];
+#pragma unroll
+        for (int q = 0; q < RX; ++q) acc[q] = 0.0f;
+        for (int ty = 0; ty < sy; ++ty) {
+            const float* wrow = win + ((oz + tz) * wy + oy + ty) * wx;
+            const float* trow = tpl + (tz * sy + ty) * sx;
+            float w[RX];
+#pragma unroll
+            for (int q = 0; q < RX - 1; ++q) w[q] = wrow[q];
+            for (int tx = 0; tx < sx; ++tx) {
+                w[RX - 1] = wrow[tx + RX - 1];
+                const float t = trow[tx];
+#pragma unroll
+                for (int q = 0; q < RX; ++q) acc[q] += t * w[q];
+#pragma unroll
+                for (int q = 0; q < RX - 1; ++q) w[q] = w[q + 1];
+            }
+        }
+        float* dst = part + tz * n_off + (oz * vy + oy) * RX;
+#pragma unroll
+        for (int q = 0; q < RX; ++q) dst[q] = acc[q];
+    }
+    __syncthreads();
+    const float inv_n = 1.0f / (float)n_tpl;
+    const float sqrt_ssf = (float)sqrt(ssf);
+    for (int off = tid; off < n_off; off += BLOCK) {
+        const int oz = off / (vy * RX), r = off - oz * vy * RX;  // r = oy * RX + ox
+        float num = 0.0f, s1 = 0.0f, s2 = 0.0f;
+        for (int tz = 0; tz < sz; ++tz) {
+            num += part[tz * n_off + off];
+            s1 += ys1[(oz + tz) * vy * RX + r];
+            s2 += ys2[(oz + tz) * vy * RX + r];
+        }
+        const float var = s2 - s1 * s1 * inv_n;
+        float val;
+        if (var < 1e-20f) {
+            val = 0.0f;
+        } else {
+            val = num / (sqrtf(var) * sqrt_ssf);
+            val = val > 1.0f ? 1.0f : (val < -1.0f ? -1.0f : val);
+        }
+        o[off] = val;
     }
 }
 
@@ -760,6 +904,23 @@ int aldvc_zncc_map(const float* f, const float* g, int64_t nz, int64_t ny, int64
         ALDVC_BF16(3, 2)
 #undef ALDVC_BF16
         return fail(-2, "unsupported bfloat16 geometry");
+    }
+    {
+        const int vx = 2 * rx + 1;
+        const size_t slds = sizeof(float) * (size_t)small_lds_floats((int)sx, (int)sy, (int)sz, vx, 2 * ry + 1, 2 * rz + 1);
+#define ALDVC_SMALL(RXV)                                                                                  \
+        if (vx == RXV && slds <= SMALL_LDS_MAX) {                                                             \
+            const void* fn = (const void*)k_zncc_map_small<RXV>;                                              \
+            if (slds > 64 * 1024 && hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)slds) != hipSuccess) \
+                return fail(-5, "could not raise the LDS limit for the window");                              \
+            hipLaunchKernelGGL((k_zncc_map_small<RXV>), dim3((unsigned)n), dim3(BLOCK), slds, stream, a);     \
+            return launch_status();                                                                           \
+        }
+        ALDVC_SMALL(3)
+        ALDVC_SMALL(5)
+        ALDVC_SMALL(7)
+        ALDVC_SMALL(9)
+#undef ALDVC_SMALL
     }
     const size_t lds = sizeof(float) * (size_t)n_tpl;
     if (lds > 64 * 1024 &&

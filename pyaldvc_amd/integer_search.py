@@ -75,37 +75,41 @@ def ncc_search(f, g, coords, subset, radius, shift0=None, chunk: int = 64, backe
     step = max(1, min(idx_all.size, DEFAULT_CHUNK_BYTES // (4 * n_off)))
     fd = K._Args.vol(f)
     gd = K._Args.vol(g)
+    dev = fd.device
+    half = torch.tensor([hx, hy, hz], dtype=torch.int64, device=dev)
+    top = torch.tensor([2 * rx, 2 * ry, 2 * rz], dtype=torch.int64, device=dev)
+    win0 = np.column_stack([wx0, wy0, wz0])
+    free = np.concatenate([free_lo, free_hi], axis=1)
     for start in range(0, idx_all.size, step):
         ids = idx_all[start : start + step]
         n = ids.size
-        maps = K.zncc_direct(fd, gd, torch.from_numpy(coords[ids]), hx, hy, hz, rx, ry, rz,
-                             torch.from_numpy(wx0[ids]), torch.from_numpy(wy0[ids]), torch.from_numpy(wz0[ids]),
-                             torch.ones(n, dtype=torch.uint8), precision=precision)  # fmt: skip
-        flat_tpl = (maps[:, 0, 0, 0] > -1.5).cpu().numpy()  # a flat template's map is all -2
-        pk, pv, pce, sub = (t.cpu().numpy() for t in K.ncc_peaks(maps, rx, ry, rz, PINV))
-        pz, rem = np.divmod(pk, vy * vx)
-        py, px = np.divmod(rem, vx)
-        lo, hi = free_lo[ids], free_hi[ids]
+        # contiguous node ranges (every template inside the volume) are written through slices
+        sel = slice(int(ids[0]), int(ids[0]) + n) if int(ids[-1]) - int(ids[0]) + 1 == n else ids
+        cd = torch.from_numpy(coords[sel]).to(dev)
+        w0 = torch.from_numpy(win0[sel]).to(dev)
+        fr = torch.from_numpy(free[sel]).to(dev)
+        maps = K.zncc_direct(fd, gd, cd, hx, hy, hz, rx, ry, rz, w0[:, 0].contiguous(), w0[:, 1].contiguous(),
+                             w0[:, 2].contiguous(), torch.ones(n, dtype=torch.uint8, device=dev), precision=precision)  # fmt: skip
+        flat_ok = maps[:, 0, 0, 0] > -1.5  # a flat template's map is all -2
+        pk, pv, pce, sub = K.ncc_peaks(maps, rx, ry, rz, PINV)
+        # the result assembly on the device: map peak (x, y, z) -> integer and sub-voxel displacement,
+        # clipped flag (peak on a face of the window that the volume did not clamp), ok flag
+        pz, rem = pk // (vy * vx), pk % (vy * vx)
+        pidx = torch.stack([rem % vx, rem // vx, pz], 1)
+        peak = w0 + pidx + half - cd
         if rx > 0 or ry > 0 or rz > 0:
-            on_edge = (
-                ((px == 0) & lo[:, 0])
-                | ((px == 2 * rx) & hi[:, 0])
-                | ((py == 0) & lo[:, 1])
-                | ((py == 2 * ry) & hi[:, 1])
-                | ((pz == 0) & lo[:, 2])
-                | ((pz == 2 * rz) & hi[:, 2])
-            )
+            on_edge = ((pidx == 0) & fr[:, :3]).any(1) | ((pidx == top) & fr[:, 3:]).any(1)
         else:
-            on_edge = np.zeros(n, dtype=bool)
-        res["peak"][ids] = np.column_stack([wx0[ids] + px + hx - cx[ids], wy0[ids] + py + hy - cy[ids],
-                                            wz0[ids] + pz + hz - cz[ids]])  # fmt: skip
-        res["disp"][ids, 0] = wx0[ids] + px + hx - cx[ids] + sub[:, 0]
-        res["disp"][ids, 1] = wy0[ids] + py + hy - cy[ids] + sub[:, 1]
-        res["disp"][ids, 2] = wz0[ids] + pz + hz - cz[ids] + sub[:, 2]
-        res["cc"][ids] = pv
-        res["pce"][ids] = pce
-        res["clipped"][ids] = on_edge
-        res["ok"][ids] = np.isfinite(pv) & flat_tpl
+            on_edge = torch.zeros(n, dtype=torch.bool, device=dev)
+        ok = torch.isfinite(pv) & flat_ok
+        packed = torch.cat([peak.double() + sub, peak.double(), pv[:, None], pce[:, None], on_edge.double()[:, None],
+                            ok.double()[:, None]], 1).cpu().numpy()  # fmt: skip
+        res["disp"][sel] = packed[:, 0:3]
+        res["peak"][sel] = packed[:, 3:6].astype(np.int64)
+        res["cc"][sel] = packed[:, 6]
+        res["pce"][sel] = packed[:, 7]
+        res["clipped"][sel] = packed[:, 8] > 0
+        res["ok"][sel] = packed[:, 9] > 0
     res["disp"][~res["ok"]] = np.nan
     return res
 

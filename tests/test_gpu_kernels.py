@@ -301,6 +301,33 @@ def test_zncc_map_matches_oracle(golden):
     assert np.max(np.abs(got - ref)) < 2e-5
 
 
+@pytest.mark.parametrize("h,r", [((8, 8, 8), (2, 2, 2)), ((4, 4, 4), (4, 4, 4)), ((3, 5, 2), (1, 3, 0)),
+                                 ((5, 2, 4), (3, 1, 2)), ((8, 8, 8), (5, 5, 5)), ((2, 2, 2), (4, 0, 1))])
+def test_zncc_map_small_and_large_windows_match_oracle(h, r):
+    """Float32 maps against the float64 oracle: x radii up to 4 take the small-window kernel
+    (window in LDS, work split over template planes, separable box sums), radius 5 the row
+    kernel. A flat template, t_ok = 0 and a window at the volume's faces are included."""
+    from pyaldvc_amd import synthetic as syn
+
+    shape = (48, 52, 56)
+    f_raw, g_raw = syn.bead_pair_np(shape, 600, 1.5, syn.translation((0.4, -1.3, 0.7)), seed=4)
+    f, g = syn.normalize_np(f_raw), syn.normalize_np(g_raw)
+    hx, hy, hz = h
+    rx, ry, rz = r
+    f[0:12, 0:12, 0:12] = 1.0  # flat template around the first node
+    coords = np.array([[6, 6, 6], [28, 26, 24], [hx + rx, hy + ry, hz + rz],
+                       [55 - hx - rx, 51 - hy - ry, 47 - hz - rz], [20, 30, 22]], dtype=np.int64)  # fmt: skip
+    wx0, wy0, wz0 = coords[:, 0] - hx - rx, coords[:, 1] - hy - ry, coords[:, 2] - hz - rz
+    ok = np.array([1, 1, 1, 1, 0], np.uint8)
+    keep = (wx0 >= 0) & (wy0 >= 0) & (wz0 >= 0)
+    coords, wx0, wy0, wz0, ok = coords[keep], wx0[keep], wy0[keep], wz0[keep], ok[keep]
+    got = K.zncc_direct(f, g, coords, hx, hy, hz, rx, ry, rz, wx0, wy0, wz0, ok)
+    ref = np.empty(got.shape)
+    cpu_ref.zncc_direct(f, g, coords, hx, hy, hz, rx, ry, rz, wx0, wy0, wz0, ok, ref)
+    assert np.max(np.abs(got - ref)) < 2e-5
+    assert np.all(got[ok == 0] == -2.0)
+
+
 @pytest.mark.parametrize("radius,shift", [((3, 3, 3), False), ((2, 3, 2), True), ((4, 4, 4), False)])
 def test_ncc_search_matches_oracle(golden, radius, shift):
     from pyaldvc_amd import integer_search as gis
