@@ -141,6 +141,11 @@ class AdmmLoop:
         N = nz * ny * nx
         if N != ctx.n_nodes:
             raise ValueError("the operators' node grid and the local context disagree")
+        from .local import _tiled
+
+        if _tiled(ctx, para):
+            raise ValueError("the ADMM loop on the device runs on one volume box; a tiled reference (tile_local, or a "
+                             "volume beyond 2^31 voxels) takes local.subpb1_solver and fem.solve_subpb2 per step")  # fmt: skip
         self.N = N
         self.max_steps = int(para.admm_max_iter)
         dev = K.device()

@@ -40,6 +40,7 @@ import torch
 from . import kernels as K
 from .cleanup import SPRING_MAX_UNKNOWN, fill_nan_grid, universal_median_test
 from .noise import STENCIL_NOISE_GAIN, noise_hessian_pattern
+from .tiling import KERNEL_MAX_VOXELS, local_split, plan_tiles, whole_box_tile
 
 logger = logging.getLogger(__name__)
 
@@ -158,9 +159,10 @@ def _is_volume(x, shape) -> bool:
     return x is not None and tuple(x.shape) == tuple(shape)
 
 
-def _reference_volumes(ref) -> dict:
+def _reference_volumes(ref, whole_gradients: bool = True) -> dict:
     """Device copies of the reference volumes: f, stored gradients (or None: on the fly), mask
-    (or None), and whether gradients are stored."""
+    (or None), and whether gradients are stored. With ``whole_gradients`` False, stored gradients
+    that the reference does not carry are left to be computed per tile box."""
     fd = K._Args.vol(ref.f)
     shape = tuple(fd.shape)
     mask = getattr(ref, "mask", None)
@@ -174,8 +176,27 @@ def _reference_volumes(ref) -> dict:
         if mode == "auto":
             mode = "on_the_fly" if 12 * int(np.prod(shape)) > GRADIENT_AUTO_BYTES else "stored"
         stored = mode == "stored"
-        grads = K.compute_gradients(fd) if stored else (None, None, None)
+        grads = K.compute_gradients(fd) if stored and whole_gradients else (None, None, None)
     return {"f": fd, "gx": grads[0], "gy": grads[1], "gz": grads[2], "mask": md, "stored": stored}
+
+
+def _tile_volumes(vols, box):
+    """``(f, (gx, gy, gz), mask)`` of one box: crops of the device volumes; stored gradients the
+    reference does not carry are computed from the crop (the box holds every window plus the
+    stencil's 3 voxels, so the values the kernels read equal the whole-volume ones)."""
+    f = box.crop(vols["f"])
+    mask = box.crop(vols["mask"]) if vols["mask"] is not None else _nomask()
+    if not vols["stored"]:
+        grads = (_otf(), _otf(), _otf())
+    elif vols["gx"] is not None:
+        grads = tuple(box.crop(vols[k]) for k in ("gx", "gy", "gz"))
+    else:
+        grads = K.compute_gradients(f)
+    return f, grads, mask
+
+
+def _volume_shape(vols) -> tuple:
+    return tuple(int(v) for v in vols["f"].shape)
 
 
 def _otf():
@@ -224,11 +245,17 @@ def precompute_local_context(mesh, ref, para) -> LocalContext:
     hx, hy, hz = (int(w) // 2 for w in para.winsize)
     half = (hx, hy, hz)
     stride = int(getattr(para, "subset_stride", 1))
-    vols = _reference_volumes(ref)
+    shape = tuple(int(v) for v in np.shape(ref.f))
+    plan = plan_tiles(mesh.grid_shape, coords_int, shape, para, half)
+    if plan.fallback:
+        logger.warning("%s", plan.fallback)
+    vols = _reference_volumes(ref, whole_gradients=plan.is_whole)
     coords_d = torch.from_numpy(coords_int).to(dev)
     node_valid = np.asarray(mesh.node_valid, dtype=bool)
     node_valid_d = torch.from_numpy(node_valid).to(dev)
     N = coords_int.shape[0]
+    if not plan.is_whole:
+        return _precompute_tiled(plan, vols, para, coords_int, coords_d, node_valid_d, half, stride, t0)
 
     split = _split_rows(vols, para, coords_d, node_valid_d, half, stride)
     split_kw = {} if split is None else {"split_index": split[0], "split_keep": split[1]}
@@ -257,7 +284,133 @@ def precompute_local_context(mesh, ref, para) -> LocalContext:
     return LocalContext(coords_int, half, device, dt, stride, noise_hessian_pattern(half, stride), split_fraction, vols)
 
 
+def _precompute_tiled(plan, vols, para, coords_int, coords_d, node_valid_d, half, stride, t0) -> LocalContext:
+    """precompute_local_context over the boxes of ``plan`` (local_icgn.py:250-300): per box the
+    crops of f, mask and gradients, the split rows of its nodes, the precompute kernel on the
+    shifted coordinates; the per-node results and split rows scattered back into whole arrays."""
+    dev = coords_d.device
+    N = coords_int.shape[0]
+    hx, hy, hz = half
+    H = torch.zeros((N, 12, 12), dtype=torch.float64, device=dev)
+    L = torch.zeros_like(H)
+    meanf = torch.zeros(N, dtype=torch.float64, device=dev)
+    bottomf = torch.ones(N, dtype=torch.float64, device=dev)
+    n_valid = torch.zeros(N, dtype=torch.int64, device=dev)
+    valid = torch.zeros(N, dtype=torch.bool, device=dev)
+    winsize = tuple(int(w) for w in para.winsize)
+    split_on = bool(getattr(para, "subset_split", False)) and vols["mask"] is not None
+    tiles = []
+    for nodes, box in plan:
+        nodes_d = torch.from_numpy(nodes).to(dev)
+        loc = torch.from_numpy(box.shift(coords_int[nodes])).to(dev)
+        cand = None
+        if split_on:  # one decision for the whole reference: count the candidates of every box first
+            frac = K.subset_valid_fraction(box.crop(vols["mask"]), loc, winsize)
+            cand = torch.nonzero(node_valid_d[nodes_d] & (frac < 1.0)).flatten()
+        tiles.append((nodes_d, loc, box, cand))
+    if split_on:
+        n_cand = sum(int(t[3].numel()) for t in tiles)
+        need = split_keep_bytes(n_cand, half, stride)
+        if need > MAX_SPLIT_BYTES:
+            logger.warning("Subset splitting off for this reference: %d subsets touch a boundary and their keep rows "
+                           "would need %.1f GB (limit %.1f GB).", n_cand, need / 1e9, MAX_SPLIT_BYTES / 1e9)  # fmt: skip
+            split_on = False
+    margin = 0 if vols["stored"] else 3
+    parts = []
+    for nodes_d, loc, box, cand in tiles:
+        f_t, (gx, gy, gz), m_t = _tile_volumes(vols, box)
+        kw = {}
+        if split_on:
+            si = torch.full((loc.shape[0],), -1, dtype=torch.int64, device=dev)
+            rows = torch.zeros((1, 1), dtype=torch.uint8, device=dev)
+            n_keep = n_inmask = None
+            if cand.numel():
+                rows, n_keep, n_inmask = K.build_split_rows(loc, cand, hx, hy, hz, stride, m_t, margin)
+                si[cand] = torch.arange(cand.numel(), dtype=torch.int64, device=dev)
+            kw = {"split_index": si, "split_keep": rows}
+            parts.append((nodes_d, cand, rows, n_keep, n_inmask))
+        out = K.precompute_nodes(loc, hx, hy, hz, f_t, gx, gy, gz, m_t, float(para.min_valid_ratio),
+                                 float(para.hessian_cond_max), stride, **kw)  # fmt: skip
+        for dst, src in zip((H, L, meanf, bottomf, n_valid), out[:5]):
+            dst[nodes_d] = src
+        valid[nodes_d] = out[5].bool()
+        del f_t, gx, gy, gz, m_t
+    valid &= node_valid_d
+    device = {"coords": coords_d, "H": H, "L": L, "meanf": meanf, "bottomf": bottomf, "n_valid": n_valid,
+              "valid": valid.to(torch.uint8)}  # fmt: skip
+    split_fraction = None
+    if split_on:
+        split_index = torch.full((N,), -1, dtype=torch.int64, device=dev)
+        sf = torch.ones(N, dtype=torch.float32, device=dev)
+        rows_all, offset = [], 0
+        for nodes_d, cand, rows, n_keep, n_inmask in parts:
+            if cand.numel() == 0:
+                continue
+            split_index[nodes_d[cand]] = offset + torch.arange(cand.numel(), dtype=torch.int64, device=dev)
+            rows_all.append(rows)
+            offset += int(cand.numel())
+            sf[nodes_d[cand]] = (n_keep.double() / torch.clamp(n_inmask, min=1).double()).float()
+        width = (K.subset_count(hx, hy, hz, stride) + 7) // 8
+        device["split_index"] = split_index
+        device["split_keep"] = torch.cat(rows_all) if rows_all else torch.zeros((1, width), dtype=torch.uint8, device=dev)
+        sf[~valid] = float("nan")
+        split_fraction = sf.cpu().numpy()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    logger.info("Local precompute: %d nodes, %d valid, %d tiles, %.2fs", N, int(valid.sum()), len(plan), dt)
+    ctx = LocalContext(coords_int, half, device, dt, stride, noise_hessian_pattern(half, stride), split_fraction, vols)
+    ctx.n_tiles = len(plan)
+    return ctx
+
+
+def _solve_tiled(ctx, mesh, para, disp, run, n_out):
+    """Run ``run(nodes_d, box) -> n_out per-node tensors`` over the boxes planned around ``disp``
+    and scatter the results; nodes that left their box through a face the tile cut (status out of
+    bounds) are solved again on the whole volume, or, beyond the kernels' size limit, in boxes with
+    twice the deformed halo (local_icgn.py:427-454, subpb1_solver.py:110-135)."""
+    dev = ctx.device["coords"].device
+    N = ctx.n_nodes
+    shape = _volume_shape(ctx._volumes)
+    plan = plan_tiles(mesh.grid_shape, ctx.coords_int, shape, para, ctx.half, disp=disp)
+    outs = None
+    open_face = np.zeros(N, dtype=bool)
+
+    def solve(plan_):
+        nonlocal outs
+        for nodes, box in plan_:
+            nodes_d = torch.from_numpy(np.asarray(nodes, dtype=np.int64)).to(dev)
+            res = run(nodes_d, box)
+            if outs is None:
+                outs = [torch.zeros((N,) + tuple(r.shape[1:]), dtype=r.dtype, device=dev) for r in res]
+            for dst, src in zip(outs, res):
+                dst[nodes_d] = src
+            open_face[nodes] = box.has_open_face
+
+    solve(plan)
+    status = outs[n_out - 2].cpu().numpy()
+    escaped = np.flatnonzero((status == STATUS_OUT_OF_BOUNDS) & open_face)
+    if escaped.size:
+        logger.warning("%d node(s) left their tile instead of the volume (the halo was short); solving them again",
+                       int(escaped.size))  # fmt: skip
+        open_face[:] = False
+        if int(np.prod(shape)) <= KERNEL_MAX_VOXELS:
+            solve([(escaped, whole_box_tile(shape))])
+        else:
+            d = np.abs(np.asarray(disp, dtype=np.float64)).reshape(N, 3)
+            wider = np.where(np.isfinite(d), 2.0 * d, 0.0) + np.asarray(ctx.half, dtype=np.float64)[None, :]
+            solve(plan_tiles(mesh.grid_shape, ctx.coords_int, shape, para, ctx.half, disp=wider, nodes=escaped))
+    return outs
+
+
 # ---------------------------------------------------------------------------- solve + cleanup
+def _tiled(ctx, para) -> bool:
+    """Whether the solves run in boxes: the context was tiled, ``para.tile_local`` asks for boxes,
+    or the volume exceeds the kernels' 32-bit offsets."""
+    if getattr(ctx, "n_tiles", 1) > 1 or int(getattr(para, "tile_local", 0) or 0) > 0:
+        return True
+    return int(np.prod(_volume_shape(ctx._volumes))) > KERNEL_MAX_VOXELS
+
+
 def _P_from_UF(U, F):
     n = U.shape[0]
     P = np.empty((n, 12))
@@ -301,14 +454,29 @@ def local_icgn(ctx: LocalContext, ref, g, U0, para, mesh, F0=None):
     d = ctx.device
     t0 = time.perf_counter()
     gd = K._Args.vol(g)
-    gx, gy, gz = (vols[k] if vols["stored"] else _otf() for k in ("gx", "gy", "gz"))
+    gx, gy, gz = (vols[k] if vols["stored"] and vols[k] is not None else _otf() for k in ("gx", "gy", "gz"))
     mask = vols["mask"] if vols["mask"] is not None else _nomask()
     P0 = torch.from_numpy(_P_from_UF(U0, F0)).to(gd.device)
-    P, n_iter, status, zncc = K.icgn_12dof_parallel(
-        d["coords"], P0, hx, hy, hz, vols["f"], gx, gy, gz, mask, gd, mode, d["L"], d["meanf"], d["bottomf"],
-        d["valid"], float(para.icgn_tol), float(para.icgn_dp_tol), int(para.icgn_max_iter), int(para.icgn_patience),
-        ctx.stride, d["H"], pattern, gain, bool(para.icgn_predictive_stop), **ctx.split_args(), variant=ICGN12_VARIANT,
-    )  # fmt: skip
+    if _tiled(ctx, para):
+
+        def run(nodes_d, box):
+            f_t, (gx_t, gy_t, gz_t), m_t = _tile_volumes(vols, box)
+            loc = torch.from_numpy(box.shift(ctx.coords_int[nodes_d.cpu().numpy()])).to(gd.device)
+            return K.icgn_12dof_parallel(
+                loc, P0[nodes_d], hx, hy, hz, f_t, gx_t, gy_t, gz_t, m_t, box.crop(gd), mode, d["L"][nodes_d],
+                d["meanf"][nodes_d], d["bottomf"][nodes_d], d["valid"][nodes_d], float(para.icgn_tol),
+                float(para.icgn_dp_tol), int(para.icgn_max_iter), int(para.icgn_patience), ctx.stride, d["H"][nodes_d],
+                pattern, gain, bool(para.icgn_predictive_stop),
+                **local_split(d.get("split_index"), d.get("split_keep"), nodes_d), variant=ICGN12_VARIANT,
+            )  # fmt: skip
+
+        P, n_iter, status, zncc = _solve_tiled(ctx, mesh, para, U0, run, 4)
+    else:
+        P, n_iter, status, zncc = K.icgn_12dof_parallel(
+            d["coords"], P0, hx, hy, hz, vols["f"], gx, gy, gz, mask, gd, mode, d["L"], d["meanf"], d["bottomf"],
+            d["valid"], float(para.icgn_tol), float(para.icgn_dp_tol), int(para.icgn_max_iter), int(para.icgn_patience),
+            ctx.stride, d["H"], pattern, gain, bool(para.icgn_predictive_stop), **ctx.split_args(), variant=ICGN12_VARIANT,
+        )  # fmt: skip
     P = P.cpu().numpy()
     n_iter = n_iter.cpu().numpy().astype(np.int32)
     status = status.cpu().numpy().astype(np.int8)
@@ -341,20 +509,36 @@ def subpb1_solver(ctx: LocalContext, ref, g, U_hat, F_hat, vdual, mu, para, mesh
     t0 = time.perf_counter()
     gd = K._Args.vol(g)
     dev = gd.device
-    gx, gy, gz = (vols[k] if vols["stored"] else _otf() for k in ("gx", "gy", "gz"))
+    gx, gy, gz = (vols[k] if vols["stored"] and vols[k] is not None else _otf() for k in ("gx", "gy", "gz"))
     mask = vols["mask"] if vols["mask"] is not None else _nomask()
-    U, n_iter, status, zncc = K.icgn_3dof_parallel(
-        d["coords"], torch.from_numpy(U_hat).to(dev), torch.from_numpy(F_hat).to(dev), torch.from_numpy(vdual).to(dev),
-        hx, hy, hz, vols["f"], gx, gy, gz, mask, gd, mode, d["H"], d["meanf"], d["bottomf"], d["valid"], float(mu),
-        float(para.icgn_tol), float(para.icgn_dp_tol), int(para.icgn_max_iter), int(para.icgn_patience), ctx.stride,
-        float(pattern[9, 9]), gain, bool(para.icgn_predictive_stop), **ctx.split_args(),
-    )  # fmt: skip
+    Uh_d, Fh_d, vd_d = (torch.from_numpy(a).to(dev) for a in (U_hat, F_hat, vdual))
+    if _tiled(ctx, para):
+
+        def run(nodes_d, box):
+            f_t, (gx_t, gy_t, gz_t), m_t = _tile_volumes(vols, box)
+            loc = torch.from_numpy(box.shift(ctx.coords_int[nodes_d.cpu().numpy()])).to(dev)
+            return K.icgn_3dof_parallel(
+                loc, Uh_d[nodes_d], Fh_d[nodes_d], vd_d[nodes_d], hx, hy, hz, f_t, gx_t, gy_t, gz_t, m_t, box.crop(gd),
+                mode, d["H"][nodes_d], d["meanf"][nodes_d], d["bottomf"][nodes_d], d["valid"][nodes_d], float(mu),
+                float(para.icgn_tol), float(para.icgn_dp_tol), int(para.icgn_max_iter), int(para.icgn_patience),
+                ctx.stride, float(pattern[9, 9]), gain, bool(para.icgn_predictive_stop),
+                **local_split(d.get("split_index"), d.get("split_keep"), nodes_d),
+            )  # fmt: skip
+
+        U, n_iter, status, zncc = _solve_tiled(ctx, mesh, para, U_hat + vdual, run, 4)
+    else:
+        U, n_iter, status, zncc = K.icgn_3dof_parallel(
+            d["coords"], Uh_d, Fh_d, vd_d, hx, hy, hz, vols["f"], gx, gy, gz, mask, gd, mode, d["H"], d["meanf"],
+            d["bottomf"], d["valid"], float(mu), float(para.icgn_tol), float(para.icgn_dp_tol), int(para.icgn_max_iter),
+            int(para.icgn_patience), ctx.stride, float(pattern[9, 9]), gain, bool(para.icgn_predictive_stop),
+            **ctx.split_args(),
+        )  # fmt: skip
     if N <= SPRING_MAX_UNKNOWN:
         # the cleanup on the device: median test (unknown nodes spring-filled, as the reference does
         # up to this many nodes) and U_filled = bad ? U_hat : U
         from .admm import subpb1_cleanup
 
-        U_filled, bad = subpb1_cleanup(U, torch.from_numpy(U_hat).to(dev), status, tuple(mesh.grid_shape),
+        U_filled, bad = subpb1_cleanup(U, Uh_d, status, tuple(mesh.grid_shape),
                                        float(para.local_outlier_threshold), edge_ok=mesh.edges_ok_grid())  # fmt: skip
         U_filled = U_filled.reshape(N, 3)
     else:
