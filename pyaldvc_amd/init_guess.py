@@ -14,6 +14,7 @@ from __future__ import annotations
 
 import logging
 import math
+import time
 
 import numpy as np
 import torch
@@ -34,6 +35,7 @@ DEFAULT_INIT_SUBSET = 16
 MIN_COARSE_STD_RATIO = 0.35
 SPLIT_GUESS_MAX_KEEP = 0.9
 SPLIT_GUESS_MIN_NEIGHBOURS = 2
+MIN_COARSE_NODES_PER_AXIS = 2  # coarse_init.py:30
 
 
 def _device_f32(vol) -> torch.Tensor:
@@ -315,3 +317,104 @@ def compute_initial_guess(f, g, mesh, para, previous=None, split_fraction=None):
     info["n_bad"] = int(bad.sum())
     info["global_shift"] = shift
     return U0, info
+
+
+# ---------------------------------------------------------------------------- coarse-lattice start
+def _axes(mesh):
+    """The lattice axes ``(z0, y0, x0)`` of a full node grid."""
+    nz, ny, nx = (int(v) for v in mesh.grid_shape)
+    c = np.asarray(mesh.coordinates, dtype=np.float64).reshape(nz, ny, nx, 3)
+    return c[:, 0, 0, 2].copy(), c[0, :, 0, 1].copy(), c[0, 0, :, 0].copy()
+
+
+def coarse_lattice(mesh, factor: int):
+    """Every ``factor``-th node per axis as a mesh of its own, and the fine index of each of its
+    nodes (coarse_init.py:33-48); ``None`` when fewer than two nodes per axis would remain."""
+    from .local import NodeMesh
+
+    k = int(factor)
+    if k <= 1:
+        return None
+    sel = [np.arange(0, int(n), k) for n in mesh.grid_shape]
+    if min(a.size for a in sel) < MIN_COARSE_NODES_PER_AXIS:
+        return None
+    nz, ny, nx = (int(v) for v in mesh.grid_shape)
+    iz, iy, ix = np.meshgrid(*sel, indexing="ij")
+    fine_index = (iz * ny * nx + iy * nx + ix).ravel()
+    coords = np.asarray(mesh.coordinates, dtype=np.float64)[fine_index]
+    valid = np.asarray(mesh.node_valid, dtype=bool)
+    node_valid = valid[fine_index].copy() if valid.size == mesh.n_nodes else np.ones(fine_index.size, dtype=bool)
+    return NodeMesh(coords, (sel[0].size, sel[1].size, sel[2].size), node_valid), fine_index
+
+
+def _nearest_fill(flat: np.ndarray) -> np.ndarray:
+    """Non-finite coarse nodes take the values of the nearest finite one (coarse_init.py:70-78)."""
+    from scipy.ndimage import distance_transform_edt
+
+    bad = ~np.all(np.isfinite(flat), axis=-1)
+    if bad.all():
+        return np.nan_to_num(flat)
+    _, idx = distance_transform_edt(bad, return_distances=True, return_indices=True)
+    return flat[tuple(idx)]
+
+
+def interpolate_to_mesh(coarse, values, mesh) -> np.ndarray:
+    """Trilinear interpolation of per-node values ``(Nc, ...)`` from the coarse lattice to every node
+    of ``mesh``, linear extrapolation beyond the lattice, non-finite coarse values filled from the
+    nearest finite node first (coarse_init.py:51-67). On the device in float64."""
+    v = np.asarray(values, dtype=np.float64)
+    gz, gy, gx = (int(n) for n in coarse.grid_shape)
+    flat = v.reshape(gz, gy, gx, -1)
+    if not np.all(np.isfinite(flat)):
+        flat = _nearest_fill(flat)
+    dev = K.device()
+    table = torch.from_numpy(np.ascontiguousarray(flat)).to(dev)
+    pts = torch.from_numpy(np.asarray(mesh.coordinates, dtype=np.float64)).to(dev)
+    idx, frac = [], []
+    for axis, col in zip(_axes(coarse), (2, 1, 0)):
+        a = torch.from_numpy(axis).to(dev)
+        x = pts[:, col].contiguous()
+        i = torch.clamp(torch.searchsorted(a, x, right=True) - 1, 0, a.numel() - 2)
+        idx.append(i)
+        frac.append((x - a[i]) / (a[i + 1] - a[i]))  # outside [0, 1] beyond the lattice: extrapolation
+    (iz, iy, ix), (tz, ty, tx) = idx, frac
+    out = torch.zeros((pts.shape[0], flat.shape[-1]), dtype=torch.float64, device=dev)
+    for dz in (0, 1):
+        wz = tz if dz else 1.0 - tz
+        for dy in (0, 1):
+            wy = ty if dy else 1.0 - ty
+            for dx in (0, 1):
+                wx = tx if dx else 1.0 - tx
+                out += (wz * wy * wx)[:, None] * table[iz + dz, iy + dy, ix + dx]
+    return out.cpu().numpy().reshape((mesh.n_nodes,) + v.shape[1:])
+
+
+def coarse_initial_guess(ref, g_norm, g_prep, mesh, para):
+    """``(U0, F0, info)`` for every node from the solve on every ``para.init_coarse_factor``-th node:
+    the initial guess and the 12-DOF local step on the coarse lattice, U and F interpolated to all
+    nodes (coarse_init.py:81-128). Falls back to :func:`compute_initial_guess` (``F0 = None``)
+    when the coarse lattice would have fewer than two nodes per axis."""
+    from .local import local_icgn, precompute_local_context
+
+    t0 = time.perf_counter()
+    lattice = coarse_lattice(mesh, int(para.init_coarse_factor))
+    if lattice is None:
+        U0, info = compute_initial_guess(ref.f, g_norm, mesh, para)
+        info["coarse_factor"] = 1
+        return U0, None, info
+    coarse, _fine_index = lattice
+    U0c, info = compute_initial_guess(ref.f, g_norm, coarse, para)
+    t_ncc = time.perf_counter() - t0
+    ctx_c = precompute_local_context(coarse, ref, para)
+    Uc, Fc, info_c, bad_c = local_icgn(ctx_c, ref, g_prep, U0c, para, coarse)
+    U0 = interpolate_to_mesh(coarse, Uc, mesh)
+    F0 = interpolate_to_mesh(coarse, np.asarray(Fc).reshape(coarse.n_nodes, 9), mesh).reshape(mesh.n_nodes, 3, 3)
+    dt = time.perf_counter() - t0
+    info.update({"method": f"coarse x{int(para.init_coarse_factor)} + {info.get('method', 'ncc')}",
+                 "coarse_factor": int(para.init_coarse_factor), "coarse_nodes": int(coarse.n_nodes),
+                 "coarse_converged": float(np.mean(info_c.status == 0)), "coarse_bad": int(bad_c.sum()),
+                 "coarse_ncc_time": t_ncc, "coarse_icgn_time": float(info_c.solve_time), "coarse_total_time": dt})  # fmt: skip
+    logger.info("Coarse initial guess: %d of %d nodes solved (%.1f%% converged, %d filled), %.2fs NCC + %.2fs IC-GN",
+                coarse.n_nodes, mesh.n_nodes, 100.0 * info["coarse_converged"], int(bad_c.sum()), t_ncc,
+                info_c.solve_time)  # fmt: skip
+    return U0, F0, info

@@ -129,3 +129,77 @@ def test_clean_initial_guess_median_test_matches_host():
     ref = np.stack([cleanup.fill_nan_grid(d[:, c].reshape(grid)).ravel() for c in range(3)], 1)
     assert np.array_equal(bad, hb)
     assert np.max(np.abs(out - ref)) < 1e-8
+
+
+def _fine_mesh(grid=(5, 7, 8), step=(6.0, 5.0, 4.0), origin=(10.0, 12.0, 14.0)):
+    nz, ny, nx = grid
+    z0 = origin[2] + step[2] * np.arange(nz)
+    y0 = origin[1] + step[1] * np.arange(ny)
+    x0 = origin[0] + step[0] * np.arange(nx)
+    Z, Y, X = np.meshgrid(z0, y0, x0, indexing="ij")
+    coords = np.column_stack([X.ravel(), Y.ravel(), Z.ravel()])
+    return NodeMesh(coords, grid, np.ones(coords.shape[0], dtype=bool)), (z0, y0, x0)
+
+
+def test_coarse_lattice_and_interpolation_match_scipy():
+    """coarse_init.py:33-67: every k-th node per axis; trilinear interpolation with linear
+    extrapolation past the lattice (the fine grid's last x node lies beyond it) and nearest-node
+    filling of non-finite coarse values, against scipy's RegularGridInterpolator."""
+    from scipy.interpolate import RegularGridInterpolator
+    from scipy.ndimage import distance_transform_edt
+
+    mesh, (z0, y0, x0) = _fine_mesh()
+    coarse, fine_index = IG.coarse_lattice(mesh, 3)
+    assert tuple(coarse.grid_shape) == (2, 3, 3)
+    assert np.array_equal(coarse.coordinates, mesh.coordinates[fine_index])
+    assert IG.coarse_lattice(mesh, 1) is None and IG.coarse_lattice(mesh, 5) is None  # 5 leaves 1 z node
+    rng = np.random.default_rng(4)
+    vals = rng.normal(size=(coarse.n_nodes, 4))
+    vals[3, 1] = np.nan
+    got = IG.interpolate_to_mesh(coarse, vals, mesh)
+    grid = vals.reshape(2, 3, 3, 4).copy()
+    bad = ~np.all(np.isfinite(grid), axis=-1)
+    _, idx = distance_transform_edt(bad, return_distances=True, return_indices=True)
+    grid = grid[tuple(idx)]
+    interp = RegularGridInterpolator((z0[::3], y0[::3], x0[::3]), grid, method="linear", bounds_error=False, fill_value=None)
+    ref = interp(np.column_stack([mesh.coordinates[:, 2], mesh.coordinates[:, 1], mesh.coordinates[:, 0]]))
+    assert np.max(np.abs(got - ref)) < 1e-12
+    # an affine field is reproduced exactly, extrapolated nodes included
+    A = np.array([[0.01, -0.02, 0.03], [0.0, 0.02, -0.01], [0.015, 0.0, -0.02]])
+    aff = coarse.coordinates @ A.T + np.array([0.3, -0.4, 0.5])
+    fine = IG.interpolate_to_mesh(coarse, aff, mesh)
+    assert np.max(np.abs(fine - (mesh.coordinates @ A.T + np.array([0.3, -0.4, 0.5])))) < 1e-12
+
+
+def test_coarse_initial_guess_starts_near_the_affine_field():
+    """coarse_init.py:81-128 on a bead pair under an affine field: the guess from every second
+    node's solve, interpolated, is within 0.05 voxel of u and 5e-3 of F at every node."""
+    from pyaldvc_amd import local as LOC
+
+    shape = (80, 88, 96)
+    F = syn.strain_rotation_F(0.01, 2.0)
+    centre = tuple((s - 1) / 2 for s in shape[::-1])
+    disp = syn.affine(F, centre, t=(1.3, -0.8, 0.6))
+    f_raw, g_raw = syn.bead_pair_np(shape, 3000, 1.5, disp, seed=8, noise=0.005)
+    f, g = syn.normalize_np(f_raw), syn.normalize_np(g_raw)
+    # nodes 24 voxels clear of every face, so that every subset (and every coarse one) is solvable
+    axes = [np.arange(24, n - 23, 8) for n in shape[::-1]]  # x, y, z
+    Z, Y, X = np.meshgrid(axes[2], axes[1], axes[0], indexing="ij")
+    coords = np.column_stack([X.ravel(), Y.ravel(), Z.ravel()]).astype(np.int64)
+    grid = (axes[2].size, axes[1].size, axes[0].size)
+    mesh = NodeMesh(coords.astype(np.float64), grid, np.ones(coords.shape[0], dtype=bool))
+    para = SimpleNamespace(init_guess_method="pyramid", winsize=(16, 16, 16), init_subset=None, search_radius=(4, 4, 4),
+                           global_shift=True, voi=None, pyramid_levels=0, pyramid_fine_radius=2, ncc_auto_expand=True,
+                           ncc_max_expand=3, init_outlier_threshold=2.0, init_min_pce=0.0, init_coarse_factor=2,
+                           subset_stride=1, min_valid_ratio=0.5, hessian_cond_max=1e12, subset_split=False,
+                           interp_method="cubic", icgn_tol=1e-2, icgn_dp_tol=1e-3, icgn_max_iter=100, icgn_patience=5,
+                           icgn_predictive_stop=True, icgn_noise_hessian=False, local_outlier_threshold=2.0,
+                           tile_local=0)  # fmt: skip
+    ref = SimpleNamespace(f=f, mask=None, gradient_mode="stored")
+    U0, F0, info = IG.coarse_initial_guess(ref, g, g, mesh, para)
+    assert info["coarse_factor"] == 2 and info["coarse_nodes"] == int(np.prod([(n + 1) // 2 for n in grid]))
+    u, v, w = disp(*(coords[:, a].astype(float) for a in range(3)))
+    inner = np.ones(coords.shape[0], dtype=bool)
+    err = np.abs(U0 - np.column_stack([u, v, w]))
+    assert np.max(err[inner]) < 0.05, (info, np.median(err[inner], axis=0), coords[inner][np.argmax(err[inner].max(1))])
+    assert np.max(np.abs(F0[inner] - np.asarray(F))) < 5e-3  # F is the displacement gradient, as P stores it
