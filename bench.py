@@ -281,12 +281,13 @@ def main():
                     help="time the fused kernel's default launch shape without running the autotuner first")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run one device copy of this many MiB first (the scale of counter-based HBM traffic)")
-    ap.add_argument("--workload", default="icgn512", choices=["icgn512", "ncc-bf16", "warp-residual", "masked-admm", "init-guess", "published-shape"],
+    ap.add_argument("--workload", default="icgn512", choices=["icgn512", "ncc-bf16", "warp-residual", "masked-admm", "init-guess", "published-shape", "post-step"],
                     help="icgn512: the batched 12-DOF IC-GN at 512^3 (the headline line); ncc-bf16: the BF16 MFMA "
                          "NCC search at 512^3; warp-residual: the differentiable warp residual at 256^3; "
                          "masked-admm: masked anisotropic subsets at 1024x1024x306 with the ADMM iterations "
                          "recorded into a HIP graph; init-guess: the initial guess end to end at 512^3; published-shape: "
-                         "the shape of the reference's published 1024x1024x306 timing run")  # fmt: skip
+                         "the shape of the reference's published 1024x1024x306 timing run; post-step: the displacement "
+                         "uncertainty and the tiled local step at 512^3")  # fmt: skip
     ap.add_argument("--cpu-stride", type=int, default=3,
                     help="init-guess CPU leg: every n-th node per axis of the lattice")
     args = ap.parse_args()
@@ -304,6 +305,8 @@ def main():
         return main_init(args)
     if args.workload == "published-shape":
         return main_published(args)
+    if args.workload == "post-step":
+        return main_post(args)
 
     import torch
 
@@ -815,6 +818,109 @@ def main_published(args):
         },
         "roofline": None,
         "cpu_baseline": None,
+    }
+    print(json.dumps(result), flush=True)
+
+
+def main_post(args):
+    """The post-step and the tiled local step on the 512^3 headline workload: the displacement
+    uncertainty of all 226,981 nodes from their Hessians (float64, device) with the oracle timed on a
+    sample of the host cores, and the 12-DOF local step driven through ``local.local_icgn`` whole and
+    in boxes of at most 288 voxels per edge (tile_local), whose results must be bit-identical."""
+    from types import SimpleNamespace
+
+    import torch
+
+    from oracle import uncertainty_ref as UR
+    from pyaldvc_amd import kernels as K
+    from pyaldvc_amd import local as LOC
+    from pyaldvc_amd.uncertainty import displacement_uncertainty
+
+    dev = K.device()
+    W = make_workload(dev)
+    coords = W["coords"]
+    N = coords.shape[0]
+    n_ax = (SHAPE[0] - WINSIZE) // STEP + 1
+    mesh = LOC.NodeMesh(coords.astype(np.float64), (n_ax,) * 3, np.ones(N, dtype=bool))
+    para = SimpleNamespace(winsize=(WINSIZE,) * 3, winstepsize=(STEP,) * 3, subset_stride=1, min_valid_ratio=0.5,
+                           hessian_cond_max=1e12, subset_split=False, interp_method="cubic", icgn_tol=1e-2,
+                           icgn_dp_tol=1e-3, icgn_max_iter=100, icgn_patience=5, icgn_predictive_stop=True,
+                           icgn_noise_hessian=False, local_outlier_threshold=2.0, tile_local=0, search_radius=(0, 0, 0))  # fmt: skip
+    ref = SimpleNamespace(f=W["f"], gx=W["gx"], gy=W["gy"], gz=W["gz"], mask=None)
+    U0 = np.round(W["U_true"])
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        out = fn()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t, out
+
+    ctx = LOC.precompute_local_context(mesh, ref, para)
+    timed(lambda: LOC.local_icgn(ctx, ref, W["g"], U0, para, mesh))  # warm-up
+    t_whole, (U, F, info, bad) = timed(lambda: LOC.local_icgn(ctx, ref, W["g"], U0, para, mesh))
+    tpara = SimpleNamespace(**{**vars(para), "tile_local": 288})
+    t_pre_tiled, ctx_t = timed(lambda: LOC.precompute_local_context(mesh, ref, tpara))
+    timed(lambda: LOC.local_icgn(ctx_t, ref, W["g"], U0, tpara, mesh))
+    t_tiled, (Ut, Ft, info_t, bad_t) = timed(lambda: LOC.local_icgn(ctx_t, ref, W["g"], U0, tpara, mesh))
+    same = bool(np.array_equal(U, Ut) and np.array_equal(F, Ft) and np.array_equal(info.status, info_t.status))
+    log(f"local step whole {t_whole * 1e3:.0f} ms, in {ctx_t.n_tiles} boxes {t_tiled * 1e3:.0f} ms (bit-identical: {same})")
+    # the displacement uncertainty from the context's Hessians (device) and its oracle on a sample
+    zn = torch.from_numpy(info.zncc).to(dev)
+    st = torch.from_numpy(info.status).to(dev)
+    displacement_uncertainty(ctx, zn, st)
+    d = ctx.device
+    s_ev = torch.cuda.current_stream()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(5)]
+    for a, b in evs:
+        a.record(s_ev)
+        std = displacement_uncertainty(ctx, zn, st)
+        b.record(s_ev)
+    torch.cuda.synchronize()
+    t_unc = float(np.median([a.elapsed_time(b) for a, b in evs])) / 1e3
+    std = std.cpu().numpy()
+    nbytes = N * (144 * 8 + 8 + 1 + 1 + 8 + 8 + 3 * 8)  # H, zncc, status, valid, n_valid, bottomf, out
+    pick = np.sort(np.random.default_rng(0).choice(N, size=min(N, 40000), replace=False))
+    H = d["H"].cpu().numpy()
+    args_ = (H[pick], info.zncc[pick], ctx.valid[pick], ctx.n_valid[pick], ctx.bottomf[pick], ctx.half)
+    t0 = time.perf_counter()
+    ref_std = UR.displacement_uncertainty(*args_, status=info.status[pick])
+    t_cpu = time.perf_counter() - t0
+    fin = np.isfinite(ref_std)
+    rel = float(np.max(np.abs(std[pick][fin] - ref_std[fin]) / ref_std[fin])) if fin.any() else None
+    conv = info.status == 0
+    log(f"uncertainty {t_unc * 1e3:.3f} ms ({N / t_unc:,.0f} nodes/s, {nbytes / t_unc / 1e9:.0f} GB/s); oracle "
+        f"{pick.size / t_cpu:,.0f} nodes/s on one host process; max relative difference {rel}; median std "
+        f"{np.nanmedian(std[conv], axis=0)} voxel")  # fmt: skip
+    result = {
+        "metric": "displacement uncertainty nodes/s at 512^3 (226,981 nodes, float64)",
+        "value": N / t_unc,
+        "unit": "nodes/s",
+        "n_gpus": 1,
+        "steps": len(evs),
+        "warmup": 1,
+        "ms_per_step": t_unc * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "512^3 headline bead pair, 61^3 nodes: per-node standard deviation of u, v, w from the 12x12 "
+                        "IC-GN Hessians; and the 12-DOF local step through local_icgn whole and in boxes",
+            "nodes": N,
+            "uncertainty_max_rel_diff_vs_oracle": rel,
+            "median_std_vox": [float(v) for v in np.nanmedian(std[conv], axis=0)],
+            "local_step_whole_s": t_whole,
+            "local_step_tiled_s": t_tiled,
+            "tiled_boxes": int(ctx_t.n_tiles),
+            "tiled_precompute_s": t_pre_tiled,
+            "tiled_bit_identical": same,
+        },
+        "roofline": {"bound": "hbm", "achieved": nbytes / t_unc / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": nbytes / t_unc / 1e9 / HBM_PEAK_GBS, "traffic": None},  # fmt: skip
+        "cpu_baseline": {"value": pick.size / t_cpu, "unit": "nodes/s", "cores": 1, "kind": "port",
+                         "sample": f"{pick.size} nodes drawn at random (seed 0), vectorised NumPy, {t_cpu:.2f}s"},  # fmt: skip
     }
     print(json.dumps(result), flush=True)
 
