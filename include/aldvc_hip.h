@@ -349,6 +349,14 @@ int aldvc_warp_znssd_backward(const float *f, const float *g, int64_t nz, int64_
                               double sz, const double *stats, const double *grad_C, double *grad_u, float *grad_g,
                               hipStream_t stream);
 
+/* Bridging elements of a cut mesh: out[e] = 1 when the eight corners (voxels [x, y, z], int64,
+ * corners[e * 24 + 3 * k + d]) of element e do not all lie in the 6-connected in-mask component
+ * of its first corner inside the corners' bounding box (clamped to the volume), or when that
+ * corner is masked; 0 otherwise; 2 when the box exceeds max_extent (1..64) voxels along an axis.
+ * Replaces mesh/mesh_cut.py:44-106 (_bridging_jit, bridging_elements). */
+int aldvc_bridging_elements(const uint8_t *mask, int64_t nz, int64_t ny, int64_t nx, const int64_t *corners, int64_t E,
+                            int max_extent, uint8_t *out, hipStream_t stream);
+
 /* ---- Post-step: per-node displacement uncertainty (float64). */
 
 /* Standard deviation of u, v, w (out: n x 3, voxels) at every node from its IC-GN Hessian H

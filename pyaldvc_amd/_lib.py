@@ -113,6 +113,7 @@ SIGNATURES = {
     "aldvc_bspline_prefilter": (I32, [P, I64, I64, I64, P, P, P]),
     "aldvc_mask_nan": (I32, [P, P, I64, P]),
     "aldvc_block_downsample": (I32, [P, I64, I64, I64, I32, P, P]),
+    "aldvc_bridging_elements": (I32, [P, I64, I64, I64, P, I64, I32, P, P]),
     "aldvc_displacement_uncertainty": (I32, [P, P, P, P, P, P, I64, P, F64, I32, P, P, P]),
     "aldvc_fem_apply": (I32, [P, I32, I32, I32, P, P, P, P, P]),
     "aldvc_fem_rhs": (I32, [P, I32, I32, I32, P, P, F64, F64, F64, P, P, P, P, P]),

@@ -201,9 +201,126 @@ __global__ __launch_bounds__(SPLIT_BLOCK) void k_split_rows(const int64_t* __res
     }
 }
 
+// ---------------------------------------------------------------------------- bridging elements
+// An element of the node grid spans a masked boundary when its eight corners (voxels, rounded) do
+// not all lie in the 6-connected in-mask component of its first corner within the corners'
+// bounding box (mesh/mesh_cut.py:44-95); a first corner on masked material makes it bridging at
+// once. One wave per element: the box in LDS as one 64-bit word per (z, y) row, the component
+// grown from the first corner by the dilation sweeps of the split rows. out[e] = 1 bridging, 0
+// not, 2 when the box exceeds `max_extent` voxels along an axis (the caller raises).
+constexpr int BRIDGE_BLOCK = 64;
+
+__global__ __launch_bounds__(BRIDGE_BLOCK) void k_bridging(const int64_t* __restrict__ corners, int64_t E,
+                                                           const uint8_t* __restrict__ mask, int64_t nz, int64_t ny,
+                                                           int64_t nx, int max_extent, uint8_t* __restrict__ out) {
+    extern __shared__ __align__(16) uint64_t words[];
+    __shared__ int changed;
+    const int tid = threadIdx.x;
+    const int64_t e = blockIdx.x;
+    const int64_t* c = corners + e * 24;
+    int64_t lo[3] = {c[0], c[1], c[2]}, hi[3] = {c[0], c[1], c[2]};  // x, y, z
+    for (int k = 1; k < 8; ++k)
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = min(lo[d], c[3 * k + d]);
+            hi[d] = max(hi[d], c[3 * k + d]);
+        }
+    const int64_t dims[3] = {nx, ny, nz};
+    for (int d = 0; d < 3; ++d) {
+        lo[d] = max(lo[d], (int64_t)0);
+        hi[d] = min(hi[d], dims[d] - 1);
+    }
+    const int64_t Sx = hi[0] - lo[0] + 1, Sy = hi[1] - lo[1] + 1, Sz = hi[2] - lo[2] + 1;
+    if (Sx < 1 || Sy < 1 || Sz < 1 || Sx > max_extent || Sy > max_extent || Sz > max_extent) {
+        if (tid == 0) out[e] = Sx < 1 || Sy < 1 || Sz < 1 ? 1 : 2;  // an empty box: no in-mask corner
+        return;
+    }
+    const int R = (int)(Sz * Sy), sx = (int)Sx, sy = (int)Sy;
+    uint64_t* sub = words;
+    uint64_t* comp = words + (int64_t)max_extent * max_extent;
+    for (int r = tid; r < R; r += BRIDGE_BLOCK) {
+        const int iz = r / sy, iy = r - iz * sy;
+        const uint8_t* row = mask + ((lo[2] + iz) * ny + (lo[1] + iy)) * nx + lo[0];
+        uint64_t w = 0;
+        for (int ix = 0; ix < sx; ++ix)
+            if (row[ix] != 0) w |= (uint64_t)1 << ix;
+        sub[r] = w;
+        comp[r] = 0;
+    }
+    auto local = [&](int k, int& rz, int& ry, int& rx) {
+        rx = (int)min(max(c[3 * k] - lo[0], (int64_t)0), Sx - 1);
+        ry = (int)min(max(c[3 * k + 1] - lo[1], (int64_t)0), Sy - 1);
+        rz = (int)min(max(c[3 * k + 2] - lo[2], (int64_t)0), Sz - 1);
+    };
+    int z0, y0, x0;
+    local(0, z0, y0, x0);
+    if (tid == 0) changed = 0;
+    __syncthreads();
+    const int r0 = z0 * sy + y0;
+    const bool seeded = (sub[r0] >> x0) & 1;
+    if (!seeded) {
+        if (tid == 0) out[e] = 1;
+        return;
+    }
+    if (tid == 0) {
+        comp[r0] = fill_runs((uint64_t)1 << x0, sub[r0]);
+        changed = 1;
+    }
+    __syncthreads();
+    while (changed) {
+        __syncthreads();
+        if (tid == 0) changed = 0;
+        __syncthreads();
+        for (int r = tid; r < R; r += BRIDGE_BLOCK) {
+            const uint64_t sm = sub[r];
+            if (!sm) continue;
+            const int iz = r / sy, iy = r - iz * sy;
+            const uint64_t o = comp[r];
+            uint64_t nb = o;
+            if (iy > 0) nb |= comp[r - 1];
+            if (iy < sy - 1) nb |= comp[r + 1];
+            if (iz > 0) nb |= comp[r - sy];
+            if (iz < Sz - 1) nb |= comp[r + sy];
+            nb &= sm;
+            if (nb & ~o) {
+                comp[r] = fill_runs(nb, sm);
+                changed = 1;
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        bool bridging = false;
+        for (int k = 1; k < 8; ++k) {
+            int rz, ry, rx;
+            local(k, rz, ry, rx);
+            if (!((comp[rz * sy + ry] >> rx) & 1)) bridging = true;
+        }
+        out[e] = bridging ? 1 : 0;
+    }
+}
+
 }  // namespace
 
 extern "C" {
+
+int aldvc_bridging_elements(const uint8_t* mask, int64_t nz, int64_t ny, int64_t nx, const int64_t* corners, int64_t E,
+                            int max_extent, uint8_t* out, hipStream_t stream) {
+    clear_error();
+    if (E < 0) return fail(-2, "negative element count");
+    if (E == 0) return 0;
+    if (!mask || !corners || !out) return fail(-2, "null pointer");
+    if (nz < 1 || ny < 1 || nx < 1) return fail(-2, "bad volume shape");
+    if (max_extent < 1 || max_extent > 64) return fail(-2, "element boxes must be 1 to 64 voxels along every axis");
+    if (E > INT32_MAX) return fail(-2, "too many elements for one launch");
+    const size_t lds = 2 * sizeof(uint64_t) * (size_t)max_extent * max_extent;
+    if (lds > 64 * 1024 &&
+        hipFuncSetAttribute((const void*)k_bridging, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+        return fail(-5, "could not raise the LDS limit");
+    hipLaunchKernelGGL(k_bridging, dim3((unsigned)E), dim3(BRIDGE_BLOCK), lds, stream, corners, E, mask, nz, ny, nx,
+                       max_extent, out);
+    return launch_status();
+}
+
 
 size_t aldvc_mask_table_bytes(int64_t nz, int64_t ny, int64_t nx) {
     return sizeof(int32_t) * (size_t)(nz + 1) * (size_t)(ny + 1) * (size_t)(nx + 1);

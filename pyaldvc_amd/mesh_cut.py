@@ -1,0 +1,179 @@
+"""Masking and cutting of the node grid on the GPU (mesh/grid_mesh.py:191-255, mesh/mesh_cut.py).
+
+``apply_mask_to_mesh`` marks a node valid when its centre voxel is in the mask and at least
+``min_valid_ratio`` of its subset is (the fraction from the summed-volume table kernel,
+``aldvc_subset_valid_fraction``), drops the elements with an invalid corner and, with
+``cut_bridging``, the elements that span a masked boundary: ``aldvc_bridging_elements`` grows,
+per element, the 6-connected in-mask component of its first corner inside the corners' bounding
+box (one wave per element, the box as 64-bit rows in LDS) and flags the element when a corner lies
+outside it. An edge of the grid is cut when every element containing it is bridging
+(``edge_ok``). Only node- and element-sized arrays touch the host.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import kernels as K
+from ._lib import check, lib
+
+# mesh_cut.py:37-41: axis (x, y, z) -> (lower, upper) corner pairs of the four element edges along
+# it, for the hex8 corner order of hex8_elements
+EDGES = {
+    0: ((0, 1), (3, 2), (4, 5), (7, 6)),
+    1: ((0, 3), (1, 2), (4, 7), (5, 6)),
+    2: ((0, 4), (1, 5), (2, 6), (3, 7)),
+}
+MAX_ELEMENT_EXTENT = 64  # voxels per axis of an element box (one 64-bit LDS word per row)
+
+
+def hex8_elements(grid_shape) -> np.ndarray:
+    """``(E, 8)`` node indices of the grid's elements, corners ordered (x, y, z) = 000, 100, 110,
+    010, 001, 101, 111, 011 with node ``iz*ny*nx + iy*nx + ix`` (grid_mesh.py:76-100)."""
+    nz, ny, nx = (int(v) for v in grid_shape)
+    iz, iy, ix = (a.ravel() for a in np.meshgrid(np.arange(nz - 1), np.arange(ny - 1), np.arange(nx - 1), indexing="ij"))
+
+    def nid(a, b, c):
+        return c * ny * nx + b * nx + a
+
+    return np.column_stack([nid(ix, iy, iz), nid(ix + 1, iy, iz), nid(ix + 1, iy + 1, iz), nid(ix, iy + 1, iz),
+                            nid(ix, iy, iz + 1), nid(ix + 1, iy, iz + 1), nid(ix + 1, iy + 1, iz + 1),
+                            nid(ix, iy + 1, iz + 1)]).astype(np.int64)  # fmt: skip
+
+
+def bridging_elements(mask, coordinates, elements) -> np.ndarray:
+    """``(E,)`` True for every element (rows of 8 node indices) that spans a masked boundary
+    (mesh_cut.py:98-106)."""
+    elements = np.asarray(elements, dtype=np.int64)
+    E = elements.shape[0]
+    if E == 0:
+        return np.zeros(0, dtype=bool)
+    corners = np.round(np.asarray(coordinates, dtype=np.float64)[elements]).astype(np.int64)  # (E, 8, 3)
+    extent = int((corners.max(axis=1) - corners.min(axis=1)).max()) + 1
+    if extent > MAX_ELEMENT_EXTENT:
+        raise ValueError(f"element boxes up to {MAX_ELEMENT_EXTENT} voxels per axis are supported (got {extent})")
+    dev = K.device()
+    m = K._Args.vol(mask, np.uint8)
+    nz, ny, nx = (int(v) for v in m.shape)
+    cd = torch.from_numpy(np.ascontiguousarray(corners)).to(dev)
+    out = torch.zeros(E, dtype=torch.uint8, device=dev)
+    check(lib().aldvc_bridging_elements(K._ptr(m), nz, ny, nx, K._ptr(cd), E, extent, K._ptr(out), K.stream_handle()),
+          "aldvc_bridging_elements")  # fmt: skip
+    res = out.cpu().numpy()
+    if (res == 2).any():
+        raise RuntimeError("an element box exceeded the declared extent")
+    return res == 1
+
+
+def edge_ok_from_elements(elements, bridging, n_nodes: int) -> np.ndarray:
+    """``(N, 3)`` edge flags: the +x, +y, +z edge of a node is cut when every element containing it
+    is bridging (mesh_cut.py:109-119)."""
+    elements = np.asarray(elements, dtype=np.int64)
+    b = np.asarray(bridging, dtype=np.int64)
+    total = np.zeros((n_nodes, 3), dtype=np.int64)
+    bridged = np.zeros((n_nodes, 3), dtype=np.int64)
+    for axis, pairs in EDGES.items():
+        for lo, _hi in pairs:
+            np.add.at(total[:, axis], elements[:, lo], 1)
+            np.add.at(bridged[:, axis], elements[:, lo], b)
+    return ~((total > 0) & (bridged == total))
+
+
+def cut_mesh(elements, coordinates, mask, n_nodes: int):
+    """Drop the bridging elements among the live rows (``elements[:, 0] >= 0``): ``(elements,
+    edge_ok, n_dropped)`` (mesh_cut.py:122-138)."""
+    elements = np.array(elements, dtype=np.int64, copy=True)
+    rows = np.flatnonzero(elements[:, 0] >= 0) if elements.size else np.zeros(0, dtype=np.int64)
+    if rows.size == 0:
+        return elements, np.ones((n_nodes, 3), dtype=bool), 0
+    live = elements[rows]
+    bridging = bridging_elements(mask, coordinates, live)
+    edge_ok = edge_ok_from_elements(live, bridging, n_nodes)
+    elements[rows[bridging]] = -1
+    return elements, edge_ok, int(bridging.sum())
+
+
+def grid_surface_nodes(grid_shape) -> np.ndarray:
+    """Nodes on the outer surface of the node grid (grid_mesh.py:125-132)."""
+    on = np.zeros(tuple(int(v) for v in grid_shape), dtype=bool)
+    on[0], on[-1] = True, True
+    on[:, 0], on[:, -1] = True, True
+    on[:, :, 0], on[:, :, -1] = True, True
+    return np.flatnonzero(on.ravel()).astype(np.int64)
+
+
+def cut_edge_nodes(edge_ok, grid_shape) -> np.ndarray:
+    """The nodes at either end of a cut edge (mesh_cut.py:154-165)."""
+    nz, ny, nx = (int(v) for v in grid_shape)
+    strides = (1, nx, nx * ny)
+    e = np.asarray(edge_ok, dtype=bool).reshape(-1, 3)
+    nodes = []
+    for axis in range(3):
+        lo = np.flatnonzero(~e[:, axis])
+        nodes += [lo, lo + strides[axis]]
+    out = np.unique(np.concatenate(nodes)) if nodes else np.zeros(0, dtype=np.int64)
+    return out[out < nz * ny * nx]
+
+
+@dataclass
+class MaskedMesh:
+    """What ``apply_mask_to_mesh`` returns for the local step: the reference's ``DVCMesh`` fields
+    this path uses (coordinates, grid_shape, node_valid, edge_ok) plus its elements and boundary
+    nodes."""
+
+    coordinates: np.ndarray
+    grid_shape: tuple
+    node_valid: np.ndarray
+    edge_ok: np.ndarray
+    elements: np.ndarray
+    boundary_nodes: np.ndarray
+
+    @property
+    def n_nodes(self) -> int:
+        return int(self.coordinates.shape[0])
+
+    def node_mesh(self):
+        from .local import NodeMesh
+
+        return NodeMesh(self.coordinates, tuple(self.grid_shape), self.node_valid, self.edge_ok)
+
+
+def apply_mask_to_mesh(coordinates, grid_shape, mask, winsize, min_valid_ratio: float, cut_bridging: bool = False,
+                       elements=None) -> MaskedMesh:  # fmt: skip
+    """Mark nodes invalid (centre outside the mask, or less than ``min_valid_ratio`` of the subset
+    inside) and drop elements with an invalid corner; with ``cut_bridging`` drop the elements that
+    span a masked boundary and record the cut edges (grid_mesh.py:191-255)."""
+    coordinates = np.asarray(coordinates, dtype=np.float64)
+    grid_shape = tuple(int(v) for v in grid_shape)
+    n = coordinates.shape[0]
+    elements = hex8_elements(grid_shape) if elements is None else np.array(elements, dtype=np.int64, copy=True)
+    if mask is None:
+        node_valid = np.ones(n, dtype=bool)
+    else:
+        m = K._Args.vol(mask, np.uint8)
+        nz, ny, nx = (int(v) for v in m.shape)
+        c = np.round(coordinates).astype(np.int64)
+        cx, cy, cz = (np.clip(c[:, a], 0, lim - 1) for a, lim in ((0, nx), (1, ny), (2, nz)))
+        idx = torch.from_numpy((cz * ny + cy) * nx + cx).to(m.device)
+        centre_ok = (m.reshape(-1)[idx] > 0).cpu().numpy()
+        frac = np.asarray(K.subset_valid_fraction(m, torch.from_numpy(c).to(m.device), tuple(int(w) for w in winsize)).cpu())
+        node_valid = centre_ok & (frac >= min_valid_ratio)
+        if elements.size:
+            elements[~node_valid[elements].all(axis=1)] = -1
+    edge_ok = np.zeros((0, 3), dtype=bool)
+    if mask is not None and cut_bridging and elements.size:
+        elements, edge_ok, _n = cut_mesh(elements, coordinates, mask, n)
+    boundary = set(grid_surface_nodes(grid_shape).tolist())
+    if not node_valid.all():
+        v = node_valid.reshape(grid_shape)
+        pad = np.pad(v, 1, mode="constant", constant_values=True)
+        near = np.zeros_like(v)
+        for sh in ((1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)):
+            near |= ~pad[tuple(slice(1 + s, 1 + s + d) for s, d in zip(sh, grid_shape))]
+        boundary |= set(np.flatnonzero((near & v).ravel()).tolist())
+    if edge_ok.size:
+        boundary |= set(cut_edge_nodes(edge_ok, grid_shape).tolist())
+    return MaskedMesh(coordinates, grid_shape, node_valid, edge_ok, elements, np.array(sorted(boundary), dtype=np.int64))
