@@ -36,19 +36,21 @@ struct NccArgs {
     float* out;
 };
 
+template <int NT = BLOCK>
 __device__ inline void fill_map(float* o, int64_t n_off, float v) {
-    for (int64_t i = threadIdx.x; i < n_off; i += BLOCK) o[i] = v;
+    for (int64_t i = threadIdx.x; i < n_off; i += NT) o[i] = v;
 }
 
 // The node's template into LDS, centred by its mean: float64 mean and energy (sum T~^2). Returns
 // false for a flat template (energy below 1e-20). Block-uniform; every thread must call it.
+template <int NT = BLOCK>
 __device__ inline bool centred_template(const NccArgs& a, float* tpl, int64_t x0, int64_t y0, int64_t z0, int sx,
                                         int sy, int n_tpl, float& mf_out, double& ssf_out) {
-    __shared__ double red[NWAVES];
+    __shared__ double red[NT / 64];
     __shared__ double bc[2];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     double s = 0.0;
-    for (int v = tid; v < n_tpl; v += BLOCK) {
+    for (int v = tid; v < n_tpl; v += NT) {
         const int k = v / (sy * sx), r = v - k * sy * sx, j = r / sx, i = r - j * sx;
         const float val = a.f[((z0 - a.hz + k) * a.ny + (y0 - a.hy + j)) * (int64_t)a.nx + (x0 - a.hx + i)];
         tpl[v] = val;
@@ -59,13 +61,13 @@ __device__ inline bool centred_template(const NccArgs& a, float* tpl, int64_t x0
     __syncthreads();
     if (tid == 0) {
         double t = 0.0;
-        for (int w = 0; w < NWAVES; ++w) t += red[w];
+        for (int w = 0; w < NT / 64; ++w) t += red[w];
         bc[0] = t / n_tpl;
     }
     __syncthreads();
     const float mf = (float)bc[0];
     double ss = 0.0;
-    for (int v = tid; v < n_tpl; v += BLOCK) {
+    for (int v = tid; v < n_tpl; v += NT) {
         const float val = tpl[v] - mf;
         tpl[v] = val;
         ss += (double)val * val;
@@ -76,7 +78,7 @@ __device__ inline bool centred_template(const NccArgs& a, float* tpl, int64_t x0
     __syncthreads();
     if (tid == 0) {
         double t = 0.0;
-        for (int w = 0; w < NWAVES; ++w) t += red[w];
+        for (int w = 0; w < NT / 64; ++w) t += red[w];
         bc[1] = t;
     }
     __syncthreads();
@@ -180,8 +182,8 @@ __host__ __device__ inline int64_t small_lds_floats(int sx, int sy, int sz, int 
     return (int64_t)sx * sy * sz + wx * wy * wz + (xs > part ? xs : part) + 2 * wz * vy * vx;
 }
 
-template <int RX>
-__global__ __launch_bounds__(BLOCK) void k_zncc_map_small(NccArgs a) {
+template <int RX, int NT>
+__global__ __launch_bounds__(NT) void k_zncc_map_small(NccArgs a) {
     extern __shared__ float sm[];
     const int64_t node = blockIdx.x;
     const int tid = threadIdx.x;
@@ -197,7 +199,7 @@ __global__ __launch_bounds__(BLOCK) void k_zncc_map_small(NccArgs a) {
                         z0 + a.hz < a.nz;
     const bool win_in = xb >= 0 && xb + wx <= a.nx && yb >= 0 && yb + wy <= a.ny && zb >= 0 && zb + wz <= a.nz;
     if (!a.t_ok[node] || !tpl_in || !win_in) {
-        fill_map(o, n_off, -2.0f);
+        fill_map<NT>(o, n_off, -2.0f);
         return;
     }
     float* tpl = sm;
@@ -210,19 +212,19 @@ __global__ __launch_bounds__(BLOCK) void k_zncc_map_small(NccArgs a) {
     float* ys2 = ys1 + wz * vy * RX;
     double ssf;
     float mf;
-    if (!centred_template(a, tpl, x0, y0, z0, sx, sy, n_tpl, mf, ssf)) {
-        fill_map(o, n_off, -2.0f);
+    if (!centred_template<NT>(a, tpl, x0, y0, z0, sx, sy, n_tpl, mf, ssf)) {
+        fill_map<NT>(o, n_off, -2.0f);
         return;
     }
     const float c = mf;  // window pilot: sum T~ (g - c) = sum T~ g because sum T~ = 0
     const int n_win = wz * wy * wx;
-    for (int v = tid; v < n_win; v += BLOCK) {
+    for (int v = tid; v < n_win; v += NT) {
         const int k = v / (wy * wx), r = v - k * wy * wx, j = r / wx, i = r - j * wx;
         win[v] = a.g[((zb + k) * a.ny + (yb + j)) * (int64_t)a.nx + (xb + i)] - c;
     }
     __syncthreads();
     // x pass: sums over sx consecutive columns of every window row
-    for (int v = tid; v < n_x; v += BLOCK) {
+    for (int v = tid; v < n_x; v += NT) {
         const int row = v / RX, ox = v - row * RX;
         const float* w = win + row * wx + ox;
         float p1 = 0.0f, p2 = 0.0f;
@@ -236,7 +238,7 @@ __global__ __launch_bounds__(BLOCK) void k_zncc_map_small(NccArgs a) {
     __syncthreads();
     // y pass: sums over sy consecutive rows
     const int n_y = wz * vy * RX;
-    for (int v = tid; v < n_y; v += BLOCK) {
+    for (int v = tid; v < n_y; v += NT) {
         const int z = v / (vy * RX), r = v - z * vy * RX, oy = r / RX, ox = r - oy * RX;
         const int b = (z * wy + oy) * RX + ox;
         float p1 = 0.0f, p2 = 0.0f;
@@ -250,7 +252,7 @@ __global__ __launch_bounds__(BLOCK) void k_zncc_map_small(NccArgs a) {
     __syncthreads();
     // numerators: item (tz, oz, oy), RX outputs along x
     const int items = sz * vz * vy;
-    for (int it = tid; it < items; it += BLOCK) {
+    for (int it = tid; it < items; it += NT) {
         const int tz = it / (vz * vy), r = it - tz * vz * vy, oz = r / vy, oy = r - oz * vy;
         float acc[RX];
 #pragma unroll
@@ -277,7 +279,7 @@ __global__ __launch_bounds__(BLOCK) void k_zncc_map_small(NccArgs a) {
     __syncthreads();
     const float inv_n = 1.0f / (float)n_tpl;
     const float sqrt_ssf = (float)sqrt(ssf);
-    for (int off = tid; off < n_off; off += BLOCK) {
+    for (int off = tid; off < n_off; off += NT) {
         const int oz = off / (vy * RX), r = off - oz * vy * RX;  // r = oy * RX + ox
         float num = 0.0f, s1 = 0.0f, s2 = 0.0f;
         for (int tz = 0; tz < sz; ++tz) {
@@ -908,19 +910,28 @@ int aldvc_zncc_map(const float* f, const float* g, int64_t nz, int64_t ny, int64
     {
         const int vx = 2 * rx + 1;
         const size_t slds = sizeof(float) * (size_t)small_lds_floats((int)sx, (int)sy, (int)sz, vx, 2 * ry + 1, 2 * rz + 1);
+        // two workgroups per CU at the most (LDS): 512 lanes each when the (template plane, output
+        // row) items fit one round of them, 256 otherwise
+        const bool wide = sz * (2 * rz + 1) * (2 * ry + 1) <= 512 && sz * (2 * rz + 1) * (2 * ry + 1) > 256;
+#define ALDVC_SMALL_NT(RXV, NTV)                                                                          \
+        {                                                                                                    \
+            const void* fn = (const void*)k_zncc_map_small<RXV, NTV>;                                        \
+            if (slds > 64 * 1024 && hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)slds) != hipSuccess) \
+                return fail(-5, "could not raise the LDS limit for the window");                             \
+            hipLaunchKernelGGL((k_zncc_map_small<RXV, NTV>), dim3((unsigned)n), dim3(NTV), slds, stream, a); \
+            return launch_status();                                                                          \
+        }
 #define ALDVC_SMALL(RXV)                                                                                  \
         if (vx == RXV && slds <= SMALL_LDS_MAX) {                                                             \
-            const void* fn = (const void*)k_zncc_map_small<RXV>;                                              \
-            if (slds > 64 * 1024 && hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)slds) != hipSuccess) \
-                return fail(-5, "could not raise the LDS limit for the window");                              \
-            hipLaunchKernelGGL((k_zncc_map_small<RXV>), dim3((unsigned)n), dim3(BLOCK), slds, stream, a);     \
-            return launch_status();                                                                           \
+            if (wide) ALDVC_SMALL_NT(RXV, 512)                                                                \
+            ALDVC_SMALL_NT(RXV, 256)                                                                          \
         }
         ALDVC_SMALL(3)
         ALDVC_SMALL(5)
         ALDVC_SMALL(7)
         ALDVC_SMALL(9)
 #undef ALDVC_SMALL
+#undef ALDVC_SMALL_NT
     }
     const size_t lds = sizeof(float) * (size_t)n_tpl;
     if (lds > 64 * 1024 &&
