@@ -34,6 +34,10 @@ extern "C" {
 /* ABI version of this header (1). */
 int aldvc_abi_version(void);
 
+/* Hash of the sources the library was compiled from (16 hex digits); no counterpart in the
+ * reference: the Python loader refuses a library whose hash differs from the sources on disk. */
+const char *aldvc_source_hash(void);
+
 /* Message of the last failed call on this thread ("" after a successful one). */
 const char *aldvc_last_error(void);
 

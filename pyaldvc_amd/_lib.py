@@ -3,7 +3,8 @@
 torch is imported first on purpose: it loads its HIP runtime (``libamdhip64.so.7``), and the
 library then resolves to that same runtime, so device pointers and streams are shared with
 torch tensors. The library is required: there is no CPU fallback, and a missing or stale
-build raises :class:`HipLibraryError`.
+build raises :class:`HipLibraryError` (stale: the source hash compiled into the library differs
+from the hash of ``csrc/`` on disk).
 """
 
 from __future__ import annotations
@@ -13,6 +14,8 @@ import os
 import threading
 
 import torch  # noqa: F401  (must precede the CDLL load, see the module docstring)
+
+from . import build
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libaldvc_hip.so")
@@ -27,6 +30,7 @@ SZ = ctypes.c_size_t
 # name -> (restype, argtypes); mirrors include/aldvc_hip.h
 SIGNATURES = {
     "aldvc_abi_version": (I32, []),
+    "aldvc_source_hash": (ctypes.c_char_p, []),
     "aldvc_last_error": (ctypes.c_char_p, []),
     "aldvc_kernel_timing": (I32, [I32]),
     "aldvc_last_kernel_ms": (ctypes.c_float, []),
@@ -174,6 +178,14 @@ def lib():
         v = L.aldvc_abi_version()
         if v != ABI_VERSION:
             raise HipLibraryError(f"ABI version {v} of {LIB_PATH} != {ABI_VERSION} expected; rebuild it")
+        built = (L.aldvc_source_hash() or b"").decode()
+        if os.path.isdir(build.CSRC):
+            disk = build.source_hash()
+            if built != disk:
+                raise HipLibraryError(
+                    f"{LIB_PATH} was built from other sources (hash {built}, sources on disk {disk}); "
+                    "rebuild it with `python -m pyaldvc_amd.build`"
+                )
         _lib = L
     return _lib
 

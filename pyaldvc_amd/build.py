@@ -2,18 +2,25 @@
 
     python -m pyaldvc_amd.build
 
-All device code is compiled with packed-FP32 instruction selection disabled
-(``-target-feature -packed-fp32-ops``): ordinary float code is left to the scalar FP32 path.
+Every source is compiled to its own object in parallel (``_build/``), then linked. All device
+code is compiled with packed-FP32 instruction selection disabled (``-target-feature
+-packed-fp32-ops``): ordinary float code is left to the scalar FP32 path.
+
+The library carries a hash of the sources it was built from (``aldvc_source_hash()``); the loader
+compares it with the sources on disk, so a library built from other sources is refused.
 """
 
 from __future__ import annotations
 
+import hashlib
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
+OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libaldvc_hip.so")
 SOURCES = [
     "aldvc_icgn.hip",
@@ -25,7 +32,20 @@ SOURCES = [
     "aldvc_admm.hip",
     "aldvc_uncertainty.hip",
 ]
+# compiled with the source hash defined; holds nothing else
+INFO_SOURCE = "aldvc_build_info.hip"
 ARCH = os.environ.get("ALDVC_OFFLOAD_ARCH", "gfx950")
+FLAGS = [
+    "-O3",
+    f"--offload-arch={ARCH}",
+    "-std=c++17",
+    "-fPIC",
+    "-Xclang",
+    "-target-feature",
+    "-Xclang",
+    "-packed-fp32-ops",
+    "-Wno-unused-command-line-argument",
+]
 
 
 def hipcc() -> str:
@@ -34,36 +54,53 @@ def hipcc() -> str:
     return cand if os.path.exists(cand) else "hipcc"
 
 
-def command(out: str = LIB) -> list[str]:
-    srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
-    return [
-        hipcc(),
-        "-O3",
-        f"--offload-arch={ARCH}",
-        "-std=c++17",
-        "-fPIC",
-        "-shared",
-        "-Xclang",
-        "-target-feature",
-        "-Xclang",
-        "-packed-fp32-ops",
-        "-Wno-unused-command-line-argument",
-        "-o",
-        out,
-        *srcs,
-    ]
+def source_hash(csrc: str | None = None) -> str:
+    """SHA-256 (first 16 hex digits) over the names and bytes of the sources under ``csrc``."""
+    csrc = csrc or CSRC
+    h = hashlib.sha256()
+    for name in sorted(os.listdir(csrc)):
+        path = os.path.join(csrc, name)
+        if not os.path.isfile(path) or not name.endswith((".hip", ".h")):
+            continue
+        h.update(name.encode())
+        h.update(b"\0")
+        with open(path, "rb") as fh:
+            h.update(fh.read())
+        h.update(b"\0")
+    return h.hexdigest()[:16]
 
 
-def build(verbose: bool = False) -> str:
-    """Compile the library (always) and return its path."""
+def _compile(src: str, obj: str, defines: list[str]) -> tuple[str, subprocess.CompletedProcess]:
+    cmd = [hipcc(), *FLAGS, *defines, "-c", "-o", obj, os.path.join(CSRC, src)]
+    return " ".join(cmd), subprocess.run(cmd, capture_output=True, text=True)
+
+
+def build(verbose: bool = False, jobs: int | None = None) -> str:
+    """Compile every source (always) and link the library; return its path."""
+    os.makedirs(OBJ, exist_ok=True)
+    digest = source_hash()
+    units = [(s, []) for s in SOURCES] + [(INFO_SOURCE, [f'-DALDVC_SOURCE_HASH="{digest}"'])]
+    objs = [os.path.join(OBJ, os.path.splitext(s)[0] + ".o") for s, _ in units]
+    jobs = jobs or min(len(units), max(1, min(16, os.cpu_count() or 1)))
+    with ThreadPoolExecutor(max_workers=jobs) as pool:
+        results = list(pool.map(lambda u: _compile(u[0], u[1], u[2]), [(s, o, d) for (s, d), o in zip(units, objs)]))
+    failed = False
+    for cmd, res in results:
+        if verbose:
+            print(cmd, flush=True)
+        if res.returncode != 0:
+            sys.stderr.write(res.stdout + res.stderr)
+            failed = True
+    if failed:
+        raise RuntimeError("hipcc failed")
     tmp = LIB + ".tmp"
-    cmd = command(tmp)
+    link = [hipcc(), f"--offload-arch={ARCH}", "-fPIC", "-shared", "-o", tmp, *objs]
     if verbose:
-        print(" ".join(cmd), flush=True)
-    res = subprocess.run(cmd, capture_output=True, text=True)
+        print(" ".join(link), flush=True)
+    res = subprocess.run(link, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError(f"hipcc failed with exit code {res.returncode}")
+        raise RuntimeError(f"link failed with exit code {res.returncode}")
     os.replace(tmp, LIB)
     return LIB
 

@@ -86,3 +86,26 @@ def test_argument_errors_are_reported_without_a_gpu(lib):
     with pytest.raises(_lib.HipLibraryError):
         _lib.check(-2, "probe")
     assert lib.aldvc_last_kernel_ms() == -1.0  # nothing timed on this thread
+
+
+def test_library_carries_the_hash_of_the_sources_on_disk(lib):
+    from pyaldvc_amd import build
+
+    assert lib.aldvc_source_hash().decode() == build.source_hash()
+
+
+def test_a_library_built_from_other_sources_is_refused(tmp_path, monkeypatch):
+    import shutil
+
+    from pyaldvc_amd import _lib, build
+
+    # a copy of the sources with one byte changed: the loaded library no longer matches it
+    csrc = tmp_path / "csrc"
+    shutil.copytree(build.CSRC, csrc)
+    info = csrc / "aldvc_build_info.hip"
+    info.write_text(info.read_text() + "\n")
+    assert build.source_hash(str(csrc)) != build.source_hash()
+    monkeypatch.setattr(build, "CSRC", str(csrc))
+    monkeypatch.setattr(_lib, "_lib", None)
+    with pytest.raises(_lib.HipLibraryError, match="built from other sources"):
+        _lib.lib()
