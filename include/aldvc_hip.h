@@ -107,6 +107,22 @@ int aldvc_icgn12_tuned(const int64_t *coords, int64_t N, double *P, int hx, int 
                        int8_t *status, double *zncc, int variant, int lds_budget, int flags, void *workspace,
                        size_t workspace_bytes, hipStream_t stream);
 
+/* aldvc_icgn12 that allocates nothing. No counterpart in the reference: for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). The workspace is a fixed 64 MiB device array inside the library, one per GPU, held
+ * until the process ends; its address is looked up on the first call on each device (the stream's
+ * device must be current). Calls whose workspace (aldvc_icgn12_workspace_bytes) exceeds it return
+ * -2: at most 254,199 nodes for the fused variants (1, 2); the plain variant (0) needs 2048 rows of
+ * S floats, so it fits only small subsets. Nothing in the scratch must start at zero (every call
+ * writes what it reads). Calls that overlap on one GPU share the array and race: serialise them. */
+int aldvc_icgn12_noalloc(const int64_t *coords, int64_t N, double *P, int hx, int hy, int hz, int stride,
+                         const float *f, const float *gx, const float *gy, const float *gz, const uint8_t *mask,
+                         const float *g, int64_t nz, int64_t ny, int64_t nx, int mode, const double *L,
+                         const double *H, const double *pattern, double noise_gain, const double *meanf,
+                         const uint8_t *valid, double tol, double dp_tol, int max_iter, int patience, int predictive,
+                         const int64_t *split_index, const uint8_t *split_keep, int64_t keep_bytes,
+                         int32_t *n_iter, int8_t *status, double *zncc, int variant, hipStream_t stream);
+
 /* Workspace bytes aldvc_icgn3 needs for N nodes. */
 size_t aldvc_icgn3_workspace_bytes(int64_t N);
 

@@ -63,6 +63,17 @@ SIGNATURES = {
             P, P, P, I32, I32, I32, P, SZ, P,  # n_iter, status, zncc, variant, lds_budget, flags, workspace, bytes, stream
         ],
     ),  # fmt: skip
+    "aldvc_icgn12_noalloc": (
+        I32,
+        [
+            P, I64, P, I32, I32, I32, I32,  # coords, N, P, hx, hy, hz, stride
+            P, P, P, P, P, P, I64, I64, I64, I32,  # f, gx, gy, gz, mask, g, nz, ny, nx, mode
+            P, P, P, F64, P, P,  # L, H, pattern, noise_gain, meanf, valid
+            F64, F64, I32, I32, I32,  # tol, dp_tol, max_iter, patience, predictive
+            P, P, I64,  # split_index, split_keep, keep_bytes
+            P, P, P, I32, P,  # n_iter, status, zncc, variant, stream
+        ],
+    ),  # fmt: skip
     "aldvc_icgn3_workspace_bytes": (SZ, [I64]),
     "aldvc_icgn3": (
         I32,

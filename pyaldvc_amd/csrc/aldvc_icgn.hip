@@ -1605,6 +1605,35 @@ int check_common(const Volumes& V, const Subset& s, int mode) {
     return 0;
 }
 
+// ---------------------------------------------------------------------------- no-alloc scratch
+// Workspace of aldvc_icgn12_noalloc: a fixed device array, one per GPU the library runs on, held
+// until the process ends. In host code `scratch` names a host-side stand-in, never device memory:
+// its device address is looked up once per device (with that device current) and only that
+// address is handed to kernels.
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex scratch_mu;
+void* scratch_dev[NOALLOC_MAX_DEVICES] = {};
+
+// Device address of the current device's scratch array (looked up on the first call per device).
+int scratch_address(void** out) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess) return fail(-5, "hipGetDevice failed");
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return fail(-2, "device index beyond the no-alloc scratch table");
+    std::lock_guard<std::mutex> guard(scratch_mu);
+    if (!scratch_dev[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(-5, (std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e)).c_str());
+        scratch_dev[dev] = p;
+    }
+    *out = scratch_dev[dev];
+    return 0;
+}
+
 }  // namespace
 
 // ============================================================================ C ABI
@@ -1759,6 +1788,31 @@ int aldvc_icgn12(const int64_t* coords, int64_t N, double* P, int hx, int hy, in
                               pattern, noise_gain, meanf, valid, tol, dp_tol, max_iter, patience, predictive,
                               split_index, split_keep, keep_bytes, n_iter, status, zncc, variant, 0, 0, workspace,
                               workspace_bytes, stream);
+}
+
+int aldvc_icgn12_noalloc(const int64_t* coords, int64_t N, double* P, int hx, int hy, int hz, int stride,
+                         const float* f, const float* gx, const float* gy, const float* gz, const uint8_t* mask,
+                         const float* g, int64_t nz, int64_t ny, int64_t nx, int mode, const double* L_all,
+                         const double* H_all, const double* pattern, double noise_gain, const double* meanf,
+                         const uint8_t* valid, double tol, double dp_tol, int max_iter, int patience, int predictive,
+                         const int64_t* split_index, const uint8_t* split_keep, int64_t keep_bytes,
+                         int32_t* n_iter, int8_t* status, double* zncc, int variant, hipStream_t stream) {
+    clear_error();
+    const size_t need = aldvc_icgn12_workspace_bytes(N, hx, hy, hz, stride, variant);
+    if (need > NOALLOC_SCRATCH_BYTES) {
+        const std::string msg = "aldvc_icgn12_noalloc: " + std::to_string(N) + " nodes need " + std::to_string(need) +
+                                " bytes of workspace, beyond its fixed " + std::to_string(NOALLOC_SCRATCH_BYTES) +
+                                "-byte scratch (at most 254,199 nodes for the fused variants); "
+                                "call aldvc_icgn12 with a workspace";
+        return fail(-2, msg.c_str());
+    }
+    if (N <= 0) return N < 0 ? fail(-2, "negative node count") : 0;
+    void* ws = nullptr;
+    if (int e = scratch_address(&ws)) return e;
+    return aldvc_icgn12_tuned(coords, N, P, hx, hy, hz, stride, f, gx, gy, gz, mask, g, nz, ny, nx, mode, L_all, H_all,
+                              pattern, noise_gain, meanf, valid, tol, dp_tol, max_iter, patience, predictive,
+                              split_index, split_keep, keep_bytes, n_iter, status, zncc, variant, 0, 0, ws,
+                              NOALLOC_SCRATCH_BYTES, stream);
 }
 
 size_t aldvc_icgn3_workspace_bytes(int64_t N) { return aldvc_icgn12_workspace_bytes(N, 0, 0, 0, 1, 1); }

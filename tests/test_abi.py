@@ -109,3 +109,17 @@ def test_a_library_built_from_other_sources_is_refused(tmp_path, monkeypatch):
     monkeypatch.setattr(_lib, "_lib", None)
     with pytest.raises(_lib.HipLibraryError, match="built from other sources"):
         _lib.lib()
+
+
+def test_noalloc_entry_rejects_calls_beyond_its_scratch(lib):
+    # 254,199 nodes fit the fixed 64 MiB scratch of the fused variants; one more does not. The size
+    # check comes before any HIP call, so this runs without a GPU.
+    args = _default_args("aldvc_icgn12_noalloc")
+    args[1] = 254_200
+    args[3] = args[4] = args[5] = 8
+    args[6] = 1
+    args[-2] = 1  # fused variant
+    assert lib.aldvc_icgn12_noalloc(*args) == -2
+    assert b"254,199 nodes" in lib.aldvc_last_error()
+    assert lib.aldvc_icgn12_workspace_bytes(254_199, 8, 8, 8, 1, 1) <= 64 << 20
+    assert lib.aldvc_icgn12_workspace_bytes(254_200, 8, 8, 8, 1, 1) > 64 << 20
