@@ -826,7 +826,7 @@ def cpu_init_baseline(W, stride, para):
     core) on the node sub-lattice of every ``stride``-th node per axis, cleaned between levels by the
     host median test and spring fill."""
     from oracle import cpu_ref, ncc_ref
-    from pyaldvc_amd import cleanup
+    from oracle import cleanup_ref as cleanup
 
     f = W["f"].cpu().numpy()
     g = W["g"].cpu().numpy()

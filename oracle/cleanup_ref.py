@@ -1,11 +1,19 @@
-"""Node-grid cleanup after the local solve: universal median test and spring-model inpainting.
+"""ORACLE -- test infrastructure only, never product code.
 
-Host-side NumPy/SciPy, as in the reference (the fields are per node, ~1 % of the local step's
-work): ``universal_median_test`` (utils/outlier_detection.py:141-170, Westerweel & Scarano 2005)
-and ``fill_nan_grid`` (utils/inpaint.py:35-126, the "spring" model of D'Errico's
-``inpaint_nans``: the discrete Laplace equation on the unknown nodes with the known nodes as
-Dirichlet data). Grids are ``(nz, ny, nx)``; ``edge_ok`` is ``(nz, ny, nx, 3)`` with the +x, +y,
-+z edge of every node (False where a masked boundary cut the mesh).
+Host NumPy/SciPy restatement of the reference's node-grid cleanup, the checker of the device median
+test and spring fill (``aldvc_subpb1_cleanup``, ``aldvc_spring_fill``):
+
+* ``universal_median_test`` / ``normalized_fluctuation``  utils/outlier_detection.py:24-170
+  (Westerweel & Scarano 2005; the cut-mesh neighbourhoods of :60-132)
+* ``fill_nan_grid`` / ``fill_nan_nodes``  utils/inpaint.py:35-126, the "spring" model of D'Errico's
+  ``inpaint_nans`` (the discrete Laplace equation on the unknown nodes, known nodes as Dirichlet
+  data; sparse direct solve up to 60,000 unknowns, CG beyond, nearest-neighbour past 300,000)
+* ``fill_nan_nearest``  utils/inpaint.py:22-33
+
+Pinned by the reference's own outputs (``tests/golden/golden_driver.npz``, written by
+``tests/golden/make_golden_driver.py``) and by closed forms (``tests/test_oracle_cleanup.py``).
+Grids are ``(nz, ny, nx)``; ``edge_ok`` is ``(nz, ny, nx, 3)`` with the +x, +y, +z edge of every
+node (False where a masked boundary cut the mesh).
 """
 
 from __future__ import annotations

@@ -72,7 +72,8 @@ def _edges(edge_ok, grid_shape):
 # ---------------------------------------------------------------------------- stand-alone pieces
 def spring_fill(values, grid_shape, edge_ok=None, tol: float = FILL_TOL, max_iter: int = FILL_MAX_ITER) -> np.ndarray:
     """``values`` (N, 3) with NaN holes on ``grid_shape`` (nz, ny, nx), each column filled by the
-    spring model (utils/inpaint.py:35-121 with method "spring"); host arrays in and out."""
+    spring model (utils/inpaint.py:35-121 with method "spring"); host arrays in and out, or device
+    tensors in and out."""
     nz, ny, nx = (int(v) for v in grid_shape)
     vals = _dev(values, shape=(nz * ny * nx, 3))
     ws = _ws(lib().aldvc_spring_fill_workspace_bytes(nx, ny, nz))
@@ -83,12 +84,13 @@ def spring_fill(values, grid_shape, edge_ok=None, tol: float = FILL_TOL, max_ite
                                 ctypes.byref(it), K._ptr(ws), int(ws.numel()), K.stream_handle()),
         "aldvc_spring_fill",
     )  # fmt: skip
-    return vals.cpu().numpy()
+    return vals if isinstance(values, torch.Tensor) else vals.cpu().numpy()
 
 
 def subpb1_cleanup(U, U_hat, status, grid_shape, threshold: float, edge_ok=None, eps: float = MEDIAN_EPS):
     """``(U_out, bad)``: bad = not converged, or (threshold > 0 and more than 27 converged) flagged
-    by the universal median test; bad nodes take ``U_hat`` (solver/subpb1_solver.py:140-164)."""
+    by the universal median test; bad nodes take ``U_hat`` (solver/subpb1_solver.py:140-164). Host
+    arrays out, or device tensors (``bad`` as bool) when ``U`` is a device tensor."""
     nz, ny, nx = (int(v) for v in grid_shape)
     N = nz * ny * nx
     Ud, Uh = _dev(U, shape=(N, 3)), _dev(U_hat, shape=(N, 3))
@@ -103,6 +105,8 @@ def subpb1_cleanup(U, U_hat, status, grid_shape, threshold: float, edge_ok=None,
                                    None, SYNC_EVERY, None, None, K._ptr(ws), int(ws.numel()), K.stream_handle()),
         "aldvc_subpb1_cleanup",
     )  # fmt: skip
+    if isinstance(U, torch.Tensor):
+        return out, bad.bool()
     return out.cpu().numpy(), bad.cpu().numpy().astype(bool)
 
 

@@ -1,13 +1,26 @@
-"""Initial-guess orchestration on the GPU: block-mean pyramid, phase-correlation shift, coarse-to-fine
-NCC search, and the cleaning of the guess (median test + spring inpainting).
+"""Initial guess of the local step on the GPU: block-mean pyramid, phase-correlation shift,
+coarse-to-fine NCC search, cleaning (median test + spring inpainting), and the coarse-lattice start.
 
-The reference runs these on the host around its GPU NCC search (solver/integer_search.py:235-694,
-solver/init_disp.py:53-189). Here the volumes stay on the device: the pyramid levels come from
-``aldvc_block_downsample``, the search is ``integer_search.ncc_search_expanding`` (HIP maps and the
-fused peak kernel), the global shift is a phase correlation with the FFT on the device, and the
-cleaning uses the device median test and spring fill of the local step (``admm.subpb1_cleanup``,
-``admm.spring_fill``) up to ``SPRING_MAX_UNKNOWN`` nodes, the host restatement beyond (as the
-local step does). Only node-sized arrays move to the host.
+The reference runs this stage on the host around its NCC search (solver/integer_search.py:235-694,
+solver/init_disp.py:29-189, solver/coarse_init.py:33-128). Here the node fields never leave the
+device between the stages: a :class:`Pyramid` keeps its running estimate ``(N, 3)`` in HBM, each
+level downsamples the volumes with ``aldvc_block_downsample``, searches with
+``integer_search.search_expanding`` (device maps, peaks and result assembly), and cleans the level's
+field with the device median test and spring fill (``aldvc_subpb1_cleanup``, ``aldvc_spring_fill``).
+Host reads inside a run are a node count per level, the expansion decisions and the conjugate-
+gradient stop flags; the fields come back once, at the end.
+
+Contract kept from the reference (its rules, not its code):
+
+* level factor ``2**l``; template ``max(8, even(w // 2**l))``; coarsest radius ``ceil(r / 2**l) + 1``
+  (at least 2), finer levels ``fine_radius``; the estimate is scaled into each level with
+  round-half-to-even, and a level whose search solves nothing keeps the previous estimate;
+* the automatic level count: consecutive levels from 1 while the level's template + window + a
+  4-voxel margin fits every axis and the level keeps 35 % of the full-resolution standard deviation;
+* cleaning between levels: median test (threshold, eps 0.1) then spring inpainting, uncut grid;
+* the final cleaning: not solved, non-finite, invalid node, PCE below ``init_min_pce``, median-test
+  outlier (cut mesh: reachable neighbours), and split nodes below 90 % of their subset with at least
+  two usable same-side neighbours are all replaced by spring inpainting (no spring across a cut).
 """
 
 from __future__ import annotations
@@ -19,23 +32,22 @@ import time
 import numpy as np
 import torch
 
+from . import integer_search as IS
 from . import kernels as K
 from ._lib import check, lib
-from .cleanup import SPRING_MAX_UNKNOWN, fill_nan_grid, universal_median_test
-from .integer_search import ncc_search_expanding
 
 logger = logging.getLogger(__name__)
 
-# integer_search.py:33-38, init_disp.py:25-26
 PYRAMID_FINE_RADIUS = 2
 COARSE_RADIUS_MARGIN = 1
 MIN_COARSE_RADIUS = 2
 MIN_PYRAMID_SUBSET = 8
 DEFAULT_INIT_SUBSET = 16
 MIN_COARSE_STD_RATIO = 0.35
+LEVEL_FIT_MARGIN = 4
 SPLIT_GUESS_MAX_KEEP = 0.9
 SPLIT_GUESS_MIN_NEIGHBOURS = 2
-MIN_COARSE_NODES_PER_AXIS = 2  # coarse_init.py:30
+MIN_COARSE_NODES_PER_AXIS = 2
 
 
 def _device_f32(vol) -> torch.Tensor:
@@ -44,6 +56,13 @@ def _device_f32(vol) -> torch.Tensor:
     return torch.from_numpy(np.ascontiguousarray(vol, dtype=np.float32)).to(K.device())
 
 
+def _device(a, dtype) -> torch.Tensor:
+    if isinstance(a, torch.Tensor):
+        return a.to(device=K.device(), dtype=dtype)
+    return torch.from_numpy(np.ascontiguousarray(a)).to(device=K.device(), dtype=dtype)
+
+
+# ---------------------------------------------------------------------------- volumes
 def block_downsample(vol, factor: int) -> torch.Tensor:
     """Block-mean downsampling by an integer ``factor``, trailing voxels dropped
     (integer_search.py:235-243); a device float32 volume. Each mean is summed in float64 and
@@ -69,109 +88,168 @@ def phase_correlation_shift(f, g, voi=None, max_dim: int = 128) -> np.ndarray:
     factor = max(1, int(math.ceil(max(fd.shape) / max_dim)))
     fs = block_downsample(fd, factor).double()
     gs = block_downsample(gd, factor).double()
-    fs = fs - fs.mean()
-    gs = gs - gs.mean()
     win = torch.ones_like(fs)
     for ax, n in enumerate(fs.shape):
-        w1 = torch.from_numpy(np.hanning(n)).to(fs)  # the reference's window, float64 here
         shape = [1, 1, 1]
         shape[ax] = n
-        win = win * w1.reshape(shape)
-    Ff = torch.fft.fftn(fs * win)
-    Fg = torch.fft.fftn(gs * win)
-    R = Fg * torch.conj(Ff)
-    mag = torch.abs(R)
-    R = R / torch.where(mag > 1e-12, mag, torch.ones_like(mag))
-    r = torch.real(torch.fft.ifftn(R))
-    peak = np.unravel_index(int(torch.argmax(r).item()), tuple(r.shape))
-    shift = np.array(peak, dtype=np.float64)
-    for ax, n in enumerate(r.shape):
-        if shift[ax] > n // 2:
-            shift[ax] -= n
-    dz, dy, dx = shift * factor  # r peaks at the shift that maps f onto g: g(x) = f(x - shift)
-    return np.array([dx, dy, dz], dtype=np.float64)
+        win = win * torch.from_numpy(np.hanning(n)).to(fs).reshape(shape)
+    cross = torch.fft.fftn((gs - gs.mean()) * win) * torch.conj(torch.fft.fftn((fs - fs.mean()) * win))
+    mag = torch.abs(cross)
+    corr = torch.real(torch.fft.ifftn(cross / torch.where(mag > 1e-12, mag, torch.ones_like(mag))))
+    at = np.array(np.unravel_index(int(torch.argmax(corr).item()), tuple(corr.shape)), dtype=np.float64)
+    n = np.array(corr.shape, dtype=np.float64)
+    at = np.where(at > n // 2, at - n, at) * factor  # wrap to signed: g(x) = f(x - shift)
+    return at[::-1].copy()  # (z, y, x) -> (x, y, z)
 
 
-def coarse_radius(radius, fac: int) -> tuple[int, int, int]:
-    """Search radius at a coarse level (integer_search.py:566-572)."""
-    return tuple(max(MIN_COARSE_RADIUS, -(-int(r) // fac) + COARSE_RADIUS_MARGIN) for r in radius)
-
-
-def _std(vol: torch.Tensor) -> float:
+def _voi_std(vol: torch.Tensor) -> float:
     from .volumes import voi_mean_std
 
     return float(voi_mean_std(vol)[1])
 
 
+# ---------------------------------------------------------------------------- level rules
+def coarse_radius(radius, fac: int) -> tuple[int, int, int]:
+    """Search radius at the coarsest level: the radius in that level's voxels, rounded up, plus one
+    (integer_search.py:566-572)."""
+    return tuple(max(MIN_COARSE_RADIUS, math.ceil(int(r) / fac) + COARSE_RADIUS_MARGIN) for r in radius)
+
+
+def level_template(subset, fac: int) -> tuple[int, int, int]:
+    """Template size at a level: the size in that level's voxels, made even, at least 8."""
+    return tuple(max(MIN_PYRAMID_SUBSET, 2 * ((int(w) // fac) // 2)) for w in subset)
+
+
+def _level_fits(shape, subset, radius, fac: int) -> bool:
+    """Whether the level's template, search window and margin fit every axis of the level's volume."""
+    tpl = level_template(subset, fac)
+    rad = coarse_radius(radius, fac)
+    return all(int(n) // fac >= t + 1 + 2 * max(r, 1) + LEVEL_FIT_MARGIN for n, t, r in zip(tuple(shape)[::-1], tpl, rad))
+
+
 def auto_pyramid_levels(shape, subset, radius, max_levels: int = 3, f=None, min_std_ratio: float = MIN_COARSE_STD_RATIO) -> int:
-    """Number of coarse levels that fit the search window and keep texture (integer_search.py:575-614):
-    a level needs the coarse template + search window + margin inside the volume and, when ``f`` is
-    given, at least ``min_std_ratio`` of the full-resolution standard deviation."""
-    levels = 0
-    std0 = _std(_device_f32(f)) if f is not None else None
-    for lv in range(1, max_levels + 1):
-        fac = 2**lv
-        fits = True
-        for n, w, r in zip(tuple(shape)[::-1], subset, coarse_radius(radius, fac)):
-            w_l = max(MIN_PYRAMID_SUBSET, (w // fac) // 2 * 2)
-            need = w_l + 1 + 2 * max(r, 1) + 4
-            if n // fac < need:
-                fits = False
-        if fits and std0 is not None and std0 > 0:
-            if _std(block_downsample(f, fac)) < min_std_ratio * std0:
-                fits = False
-        if fits:
-            levels = lv
-        else:
+    """Number of coarse levels (integer_search.py:575-614): consecutive levels from 1 while a level
+    fits (:func:`_level_fits`) and, when ``f`` is given, its block means keep ``min_std_ratio`` of the
+    full-resolution standard deviation (a finer speckle than the block averages away)."""
+    fd = _device_f32(f) if f is not None else None
+    std_full = _voi_std(fd) if fd is not None else 0.0
+    count = 0
+    while count < max_levels:
+        fac = 2 ** (count + 1)
+        if not _level_fits(shape, subset, radius, fac):
             break
-    return levels
+        if std_full > 0 and _voi_std(block_downsample(fd, fac)) < min_std_ratio * std_full:
+            break
+        count += 1
+    return count
 
 
-def _scatter_search(sub: dict, active: np.ndarray, n: int) -> dict:
-    """Search results of the ``active`` nodes as full-size arrays (integer_search.py:527-542)."""
-    if active.size == n:
-        return sub
-    res = {"disp": np.full((n, 3), np.nan), "cc": np.full(n, np.nan), "pce": np.full(n, np.nan),
-           "clipped": np.zeros(n, dtype=bool), "ok": np.zeros(n, dtype=bool),
-           "expansions": sub.get("expansions", 0), "radius": sub.get("radius")}  # fmt: skip
-    for key in ("disp", "cc", "pce", "clipped", "ok"):
-        res[key][active] = sub[key]
-    return res
+# ---------------------------------------------------------------------------- cleaning on the device
+def _median_outliers(d: torch.Tensor, usable: torch.Tensor, grid, threshold: float, edge_ok=None) -> torch.Tensor:
+    """Nodes the universal median test flags among the ``usable`` ones of the field ``d`` (N, 3), the
+    others spring-filled first; nothing when the test is off or 27 or fewer nodes are usable."""
+    from .admm import subpb1_cleanup
+
+    if threshold <= 0:
+        return torch.zeros_like(usable)
+    status = torch.where(usable, 0, 1).to(torch.int8)
+    vals = torch.nan_to_num(d)
+    _, flagged = subpb1_cleanup(vals, vals, status, grid, float(threshold), edge_ok=edge_ok)
+    return flagged & usable
 
 
-def _median_flags(d: np.ndarray, valid: np.ndarray, grid_shape, threshold: float, edge_ok=None) -> np.ndarray:
-    """Universal median test flags of the ``valid`` nodes of ``d`` (N, 3) on the node grid, the
-    invalid nodes spring-filled first (outlier_detection.py:135-160); on the device up to
-    SPRING_MAX_UNKNOWN nodes."""
-    N = d.shape[0]
-    if N <= SPRING_MAX_UNKNOWN:
-        from .admm import subpb1_cleanup
+def _inpaint(d: torch.Tensor, grid, edge_ok=None) -> torch.Tensor:
+    from .admm import spring_fill
 
-        status = np.where(valid, 0, 1).astype(np.int8)
-        _, bad = subpb1_cleanup(np.nan_to_num(d), np.nan_to_num(d), status, grid_shape, float(threshold), edge_ok=edge_ok)
-        return bad & valid
-    flag = universal_median_test(d.reshape(tuple(grid_shape) + (3,)), valid.reshape(grid_shape), threshold, edge_ok=edge_ok)
-    return flag.ravel() & valid
+    return spring_fill(d.contiguous(), grid, edge_ok=edge_ok)
 
 
-def _fill(d: np.ndarray, grid_shape, edge_ok=None) -> np.ndarray:
-    """Spring inpainting of the NaN entries of ``d`` (N, 3), column by column (inpaint.py:35-121):
-    on the device up to SPRING_MAX_UNKNOWN unknowns, the host restatement beyond."""
-    if int(np.isnan(d).sum(axis=0).max()) <= SPRING_MAX_UNKNOWN:
-        from .admm import spring_fill
+def _clean_level(d: torch.Tensor, grid, threshold: float) -> torch.Tensor:
+    """A pyramid level's field with its median-test outliers removed and every hole inpainted
+    (integer_search.py:545-556: uncut grid)."""
+    usable = torch.isfinite(d).all(1)
+    out = d.clone()
+    out[_median_outliers(d, usable, grid, threshold)] = float("nan")
+    return _inpaint(out, grid)
 
-        return spring_fill(d, grid_shape, edge_ok=edge_ok)
-    return np.stack([fill_nan_grid(d[:, c].reshape(grid_shape), edge_ok=edge_ok).ravel() for c in range(3)], 1)
+
+def _same_side_count(good: torch.Tensor, edge_ok: torch.Tensor, grid) -> torch.Tensor:
+    """Per node, how many of its six face neighbours are ``good`` and joined to it by an intact edge
+    (init_disp.py:29-41). ``edge_ok`` (N, 3): the +x, +y, +z edge of every node."""
+    nz, ny, nx = grid
+    gd = good.reshape(nz, ny, nx).to(torch.int32)
+    eo = edge_ok.reshape(nz, ny, nx, 3).to(torch.int32)
+    count = torch.zeros((nz, ny, nx), dtype=torch.int32, device=good.device)
+    for axis, comp in ((2, 0), (1, 1), (0, 2)):  # array axis of x, y, z and its edge component
+        n = gd.shape[axis]
+        if n < 2:
+            continue
+        lo, hi = gd.narrow(axis, 0, n - 1), gd.narrow(axis, 1, n - 1)
+        link = eo[..., comp].narrow(axis, 0, n - 1)  # the edge between node i and i + 1
+        count.narrow(axis, 0, n - 1).add_(hi * link)  # neighbour ahead
+        count.narrow(axis, 1, n - 1).add_(lo * link)  # neighbour behind
+    return count.reshape(-1)
 
 
-def _clean_field(disp: np.ndarray, grid_shape, threshold: float) -> np.ndarray:
-    """Median-test outlier removal + spring inpainting on the node grid (integer_search.py:545-556)."""
-    d = np.array(disp, dtype=np.float64).reshape(-1, 3)
-    valid = np.all(np.isfinite(d), axis=1)
-    if threshold > 0 and valid.sum() > 27:
-        flag = _median_flags(d, valid, grid_shape, threshold)
-        d[flag] = np.nan
-    return _fill(d, grid_shape)
+# ---------------------------------------------------------------------------- the pyramid
+class Pyramid:
+    """Coarse-to-fine NCC search over ``levels + 1`` levels, state on the device.
+
+    ``coords`` (N, 3) node centres, ``grid_shape`` (nz, ny, nx) of the node lattice; ``active`` the
+    nodes correlated (their reference subset is usable), the rest are inpainted at every level."""
+
+    def __init__(self, f, g, coords, grid_shape, subset, radius, levels: int, global_shift=None, active=None,
+                 fine_radius: int = PYRAMID_FINE_RADIUS, precision: str = "fp32"):  # fmt: skip
+        self.f, self.g = _device_f32(f), _device_f32(g)
+        dev = self.f.device
+        self.coords = _device(np.asarray(coords, dtype=np.int64), torch.int64).reshape(-1, 3)
+        self.N = int(self.coords.shape[0])
+        self.grid = tuple(int(v) for v in grid_shape)
+        self.subset, self.radius = tuple(int(w) for w in subset), tuple(int(r) for r in radius)
+        self.levels, self.fine_radius, self.precision = int(levels), int(fine_radius), precision
+        rows = None if active is None else torch.nonzero(_device(active, torch.bool)).squeeze(1)
+        self.rows = torch.arange(self.N, device=dev) if rows is None or rows.numel() == 0 else rows
+        self.estimate = torch.zeros((self.N, 3), dtype=torch.float64, device=dev)
+        if global_shift is not None:
+            self.estimate += torch.as_tensor(np.asarray(global_shift, dtype=np.float64), device=dev)
+        self.found = IS.Found.unsolved(self.N, dev)  # the finest level that solved anything
+        self.report: list = []
+
+    def _level(self, lv: int, threshold: float, auto_expand: bool, max_expand: int) -> int:
+        fac = 2**lv
+        fl = block_downsample(self.f, fac)
+        gl = block_downsample(self.g, fac)
+        tpl = level_template(self.subset, fac)
+        rad = coarse_radius(self.radius, fac) if lv == self.levels else (self.fine_radius,) * 3
+        c = torch.round(self.coords[self.rows].double() / fac).long()
+        s = torch.round(self.estimate[self.rows] / fac).long()
+        res = IS.search_expanding(fl, gl, c, tuple(t // 2 for t in tpl), rad, s, auto_expand, max_expand, self.precision)
+        n_ok = int(res.ok.sum())
+        entry = {"level": lv, "n_ok": n_ok, "radius": res.radius, "subset": tpl, "expansions": res.expansions}
+        self.report.append(entry)
+        if n_ok == 0:
+            logger.warning("Pyramid level %d: no NCC peak found; the previous estimate stands.", lv)
+            return res.expansions
+        field = torch.full((self.N, 3), float("nan"), dtype=torch.float64, device=self.f.device)
+        field[self.rows] = res.disp * fac  # NaN where the search did not solve
+        self.estimate = _clean_level(field, self.grid, threshold)
+        self.found = IS.Found.unsolved(self.N, self.f.device)
+        res.scatter_into(self.found, self.rows)
+        entry["_stats"] = (res.cc, res.clipped, res.ok)
+        return res.expansions
+
+    def run(self, threshold: float = 2.0, auto_expand: bool = True, max_expand: int = 3) -> dict:
+        expansions = sum(self._level(lv, threshold, auto_expand, max_expand) for lv in range(self.levels, -1, -1))
+        for entry in self.report:
+            stats = entry.pop("_stats", None)
+            if stats is not None:
+                cc, clipped, ok = stats
+                entry["median_cc"] = float(torch.nanmedian(cc)) if cc.numel() else float("nan")
+                entry["clipped_frac"] = float(clipped[ok].double().mean()) if bool(ok.any()) else 0.0
+        host = self.found.host()
+        return {"levels": self.levels, "per_level": self.report, "expansions": expansions, "n_searched": int(self.rows.numel()),
+                "disp": self.estimate.cpu().numpy(), "cc": host["cc"], "pce": host["pce"], "ok": host["ok"],
+                "clipped": host["clipped"], "estimate_device": self.estimate}  # fmt: skip
 
 
 def pyramid_search(f, g, coords, grid_shape, subset, radius, levels: int = 0, global_shift=None,
@@ -179,140 +257,110 @@ def pyramid_search(f, g, coords, grid_shape, subset, radius, levels: int = 0, gl
                    fine_radius: int = PYRAMID_FINE_RADIUS, backend: str = "hip", valid=None,
                    precision: str = "fp32") -> dict:  # fmt: skip
     """Coarse-to-fine NCC search (integer_search.py:617-694): ``disp`` (N, 3) in full-resolution
-    voxels + info. The coarsest level searches ``radius`` scaled to its voxels, the finer levels
-    refine with ``fine_radius`` around the up-scaled estimate, and the estimate is median-cleaned
-    and inpainted between levels. ``valid`` restricts the correlation to those nodes."""
+    voxels plus the last solving level's ``cc``, ``pce``, ``ok``, ``clipped`` and a per-level report.
+    ``valid`` restricts the correlation to those nodes; ``levels`` 0 picks the count automatically."""
     if backend != "hip":
         raise ValueError("pyramid_search here runs on the HIP backend")
-    fd, gd = _device_f32(f), _device_f32(g)
-    coords = np.asarray(coords, dtype=np.int64)
-    grid_shape = tuple(int(v) for v in grid_shape)
+    fd = _device_f32(f)
     if levels <= 0:
         levels = auto_pyramid_levels(tuple(fd.shape), subset, radius, f=fd)
-        logger.info("Pyramid: %d coarse level(s) selected automatically", levels)
-    N = coords.shape[0]
-    active = np.arange(N) if valid is None else np.flatnonzero(np.asarray(valid, dtype=bool))
-    if active.size == 0:
-        active = np.arange(N)
-    disp_full = np.zeros((N, 3), dtype=np.float64)
-    if global_shift is not None:
-        disp_full += np.asarray(global_shift, dtype=np.float64)[None, :]
-    info: dict = {"levels": levels, "per_level": [], "expansions": 0, "n_searched": int(active.size)}
-    for lv in range(levels, -1, -1):
-        fac = 2**lv
-        fl = block_downsample(fd, fac) if fac > 1 else fd
-        gl = block_downsample(gd, fac) if fac > 1 else gd
-        subset_l = tuple(max(MIN_PYRAMID_SUBSET, (w // fac) // 2 * 2) for w in subset)
-        rad_l = coarse_radius(radius, fac) if lv == levels else (int(fine_radius),) * 3
-        coords_l = np.round(coords[active] / fac).astype(np.int64)
-        shift_l = np.round(disp_full[active] / fac).astype(np.int64)
-        sub = ncc_search_expanding(fl, gl, coords_l, subset_l, rad_l, shift_l, auto_expand, max_expand, precision=precision)
-        res = _scatter_search(sub, active, N)
-        rad_l = res["radius"]
-        info["expansions"] += res["expansions"]
-        d = res["disp"] * fac
-        d[~res["ok"]] = np.nan
-        n_ok = int(res["ok"].sum())
-        if n_ok == 0:
-            logger.warning("Pyramid level %d produced no valid NCC peaks; keeping previous estimate.", lv)
-            info["per_level"].append({"level": lv, "n_ok": 0, "radius": rad_l, "subset": subset_l})
-            continue
-        disp_full = _clean_field(d, grid_shape, outlier_threshold)
-        info["per_level"].append({"level": lv, "n_ok": n_ok, "radius": rad_l, "subset": subset_l,
-                                  "median_cc": float(np.nanmedian(res["cc"])),
-                                  "clipped_frac": float(np.mean(res["clipped"][res["ok"]]))})  # fmt: skip
-        info["cc"], info["pce"], info["ok"], info["clipped"] = res["cc"], res["pce"], res["ok"], res["clipped"]
-    info["disp"] = disp_full
-    return info
+        logger.info("Pyramid: %d coarse level(s) chosen", levels)
+    pyr = Pyramid(fd, g, coords, grid_shape, subset, radius, levels, global_shift, valid, fine_radius, precision)
+    return pyr.run(outlier_threshold, auto_expand, max_expand)
 
 
-def same_side_neighbours(good, edge_ok, grid_shape) -> np.ndarray:
-    """``(N,)`` count of the six direct neighbours that are ``good`` and not across a cut edge
-    (init_disp.py:29-41)."""
-    g = np.asarray(good, dtype=bool).reshape(grid_shape)
-    eo = np.asarray(edge_ok, dtype=bool).reshape(tuple(grid_shape) + (3,))
-    cnt = np.zeros(grid_shape, dtype=np.int64)
-    cnt[:, :, :-1] += g[:, :, 1:] & eo[:, :, :-1, 0]
-    cnt[:, :, 1:] += g[:, :, :-1] & eo[:, :, :-1, 0]
-    cnt[:, :-1, :] += g[:, 1:, :] & eo[:, :-1, :, 1]
-    cnt[:, 1:, :] += g[:, :-1, :] & eo[:, :-1, :, 1]
-    cnt[:-1, :, :] += g[1:, :, :] & eo[:-1, :, :, 2]
-    cnt[1:, :, :] += g[:-1, :, :] & eo[:-1, :, :, 2]
-    return cnt.ravel()
-
-
+# ---------------------------------------------------------------------------- the guess
 def clean_initial_guess(disp, ok, pce, mesh, para, split_fraction=None):
-    """Reject failed / low-quality / outlying nodes and inpaint them (init_disp.py:144-189):
-    ``(U0 (N, 3), bad (N,))``. Split nodes that lost more than ``1 - SPLIT_GUESS_MAX_KEEP`` of their
-    subset take their guess from their own side when enough same-side neighbours are usable."""
+    """``(U0 (N, 3), bad (N,))``: the nodes that did not solve, are not finite, are not valid, fall
+    below ``para.init_min_pce`` or fail the median test are inpainted; split nodes below 90 % of their
+    subset with enough usable same-side neighbours are inpainted from their own side
+    (init_disp.py:144-189). On the device; host arrays out."""
     N = mesh.n_nodes
-    d = np.array(disp, dtype=np.float64).reshape(N, 3)
-    bad = ~np.asarray(ok, dtype=bool) | ~np.all(np.isfinite(d), axis=1)
-    if mesh.node_valid.size == N:
-        bad |= ~mesh.node_valid
+    grid = tuple(int(v) for v in mesh.grid_shape)
+    d = _device(disp, torch.float64).reshape(N, 3).clone()
+    bad = ~_device(ok, torch.bool).reshape(N) | ~torch.isfinite(d).all(1)
+    if np.size(mesh.node_valid) == N:
+        bad |= ~_device(mesh.node_valid, torch.bool)
     if para.init_min_pce > 0 and pce is not None:
-        p = np.asarray(pce, dtype=np.float64)
-        bad |= np.isfinite(p) & (p < para.init_min_pce)
-    d[bad] = np.nan
-    grid = tuple(mesh.grid_shape)
-    edge_ok = mesh.edges_ok_grid()
-    if para.init_outlier_threshold > 0 and (~bad).sum() > 27:
-        bad |= _median_flags(d, ~bad, grid, para.init_outlier_threshold, edge_ok=edge_ok)
-        d[bad] = np.nan
-    if split_fraction is not None and edge_ok is not None:
-        sf = np.asarray(split_fraction, dtype=np.float64)
-        cut = np.isfinite(sf) & (sf < SPLIT_GUESS_MAX_KEEP) & ~bad
-        if cut.any():
-            n_side = same_side_neighbours(~bad, mesh.edges_ok(), grid)
-            take = cut & (n_side >= SPLIT_GUESS_MIN_NEIGHBOURS)
-            bad |= take
-            d[take] = np.nan
-            logger.info("Initial guess: %d split nodes take their guess from their own side", int(take.sum()))
-    if bad.all():
-        logger.warning("Initial guess failed at every node; starting from zero displacement.")
-        return np.zeros((N, 3)), bad
-    out = _fill(d, grid, edge_ok=edge_ok)
-    logger.info("Initial guess: %d/%d nodes replaced by inpainting (%.1f%%)", int(bad.sum()), N, 100 * bad.mean())
-    return out, bad
+        p = _device(pce, torch.float64)
+        bad |= torch.isfinite(p) & (p < para.init_min_pce)
+    d[bad] = float("nan")
+    edge_grid = mesh.edges_ok_grid()
+    bad |= _median_outliers(d, ~bad, grid, float(para.init_outlier_threshold), edge_ok=edge_grid)
+    d[bad] = float("nan")
+    if split_fraction is not None and edge_grid is not None:
+        sf = _device(split_fraction, torch.float64)
+        cut = torch.isfinite(sf) & (sf < SPLIT_GUESS_MAX_KEEP) & ~bad
+        if bool(cut.any()):
+            sides = _same_side_count(~bad, _device(mesh.edges_ok(), torch.bool), grid)
+            own = cut & (sides >= SPLIT_GUESS_MIN_NEIGHBOURS)
+            bad |= own
+            d[own] = float("nan")
+            logger.info("Initial guess: %d split nodes filled from their own side", int(own.sum()))
+    n_bad = int(bad.sum())
+    if n_bad == N:
+        logger.warning("Initial guess: no node solved; U0 = 0.")
+        return np.zeros((N, 3)), bad.cpu().numpy()
+    out = _inpaint(d, grid, edge_ok=edge_grid)
+    logger.info("Initial guess: %d of %d nodes inpainted", n_bad, N)
+    return out.cpu().numpy(), bad.cpu().numpy()
+
+
+def _template_size(para) -> tuple[int, int, int]:
+    """``para.init_subset``, or the subset capped at 16 voxels (init_disp.py:76-81)."""
+    if para.init_subset is not None:
+        return tuple(int(w) for w in para.init_subset)
+    return tuple(min(int(w), DEFAULT_INIT_SUBSET) for w in para.winsize)
+
+
+def _search_direct(f, g, coords, para, shift, active):
+    """The single-level search with radius doubling (init_guess_method "ncc")."""
+    N = coords.shape[0]
+    fd, gd = _device_f32(f), _device_f32(g)
+    rows = torch.arange(N, device=fd.device) if active is None else torch.nonzero(_device(active, torch.bool)).squeeze(1)
+    if rows.numel() == 0:
+        rows = torch.arange(N, device=fd.device)
+    c = _device(coords, torch.int64)[rows]
+    s = None if shift is None else torch.round(torch.as_tensor(shift, device=fd.device)).long().expand(rows.numel(), 3)
+    tpl = _template_size(para)
+    res = IS.search_expanding(fd, gd, c, tuple(w // 2 for w in tpl), tuple(int(r) for r in para.search_radius),
+                              s, bool(para.ncc_auto_expand), int(para.ncc_max_expand))  # fmt: skip
+    full = IS.Found.unsolved(N, fd.device)
+    res.scatter_into(full, rows)
+    host = full.host()
+    return host["disp"], host["ok"], host["pce"], {"method": "ncc", "radius": host["radius"], "expansions": host["expansions"],
+                                                   "cc": host["cc"], "pce": host["pce"], "ok": host["ok"],
+                                                   "clipped": host["clipped"]}  # fmt: skip
 
 
 def compute_initial_guess(f, g, mesh, para, previous=None, split_fraction=None):
-    """``U0`` (N, 3) and an info dict according to ``para.init_guess_method`` (init_disp.py:53-141):
-    "zero", "previous" (falls back to "pyramid"), "pyramid", or the direct expanding NCC search."""
+    """``U0`` (N, 3) and an info dict by ``para.init_guess_method`` (init_disp.py:53-141): "zero";
+    "previous" (the previous frame's field when it is whole, else "pyramid"); "pyramid"; "ncc" (the
+    single-level search with radius doubling). The searched guess is then cleaned
+    (:func:`clean_initial_guess`)."""
     N = mesh.n_nodes
     method = para.init_guess_method
     if method == "zero":
         return np.zeros((N, 3)), {"method": "zero"}
     if method == "previous":
-        if previous is not None and previous.shape == (N, 3) and np.all(np.isfinite(previous)):
-            return previous.copy(), {"method": "previous"}
+        if previous is not None and np.shape(previous) == (N, 3) and np.isfinite(previous).all():
+            return np.array(previous, dtype=np.float64), {"method": "previous"}
         method = "pyramid"
-    coords = np.round(mesh.coordinates).astype(np.int64)
-    subset = para.init_subset if para.init_subset is not None else tuple(min(int(w), DEFAULT_INIT_SUBSET) for w in para.winsize)
-    valid = mesh.node_valid if mesh.node_valid.size == N else None
-    shift = None
-    if para.global_shift:
-        shift = phase_correlation_shift(f, g, para.voi)
-        logger.info("Global phase-correlation shift (dx, dy, dz) = %s voxels", np.array2string(shift, precision=1))
+    coords = np.round(np.asarray(mesh.coordinates)).astype(np.int64)
+    active = mesh.node_valid if np.size(mesh.node_valid) == N else None
+    shift = phase_correlation_shift(f, g, para.voi) if para.global_shift else None
+    if shift is not None:
+        logger.info("Global shift (dx, dy, dz) = %s voxels", np.array2string(shift, precision=1))
     if method == "pyramid":
-        info = pyramid_search(f, g, coords, mesh.grid_shape, subset, para.search_radius, para.pyramid_levels,
+        info = pyramid_search(f, g, coords, mesh.grid_shape, _template_size(para), para.search_radius, para.pyramid_levels,
                               global_shift=shift, outlier_threshold=para.init_outlier_threshold,
                               auto_expand=para.ncc_auto_expand, max_expand=para.ncc_max_expand,
-                              fine_radius=int(getattr(para, "pyramid_fine_radius", PYRAMID_FINE_RADIUS)), valid=valid)  # fmt: skip
-        disp = info["disp"]
-        ok = info.get("ok", np.ones(N, dtype=bool))
-        pce = info.get("pce", np.full(N, np.nan))
+                              fine_radius=int(getattr(para, "pyramid_fine_radius", PYRAMID_FINE_RADIUS)), valid=active)  # fmt: skip
+        info.pop("estimate_device", None)
+        disp, ok, pce = info["disp"], info["ok"], info["pce"]
         info["method"] = "pyramid"
     else:
-        shift0 = None if shift is None else np.tile(np.round(shift).astype(np.int64), (N, 1))
-        active = np.arange(N) if valid is None or not valid.any() else np.flatnonzero(valid)
-        res = ncc_search_expanding(f, g, coords[active], subset, tuple(int(r) for r in para.search_radius),
-                                   None if shift0 is None else shift0[active], para.ncc_auto_expand,
-                                   para.ncc_max_expand)  # fmt: skip
-        res = _scatter_search(res, active, N)
-        disp, ok, pce = res["disp"], res["ok"], res["pce"]
-        info = {"method": "ncc", "radius": res["radius"], "expansions": res["expansions"], "cc": res["cc"],
-                "pce": pce, "ok": ok, "clipped": res["clipped"]}  # fmt: skip
+        disp, ok, pce, info = _search_direct(f, g, coords, para, shift, active)
     U0, bad = clean_initial_guess(disp, ok, pce, mesh, para, split_fraction)
     info["n_bad"] = int(bad.sum())
     info["global_shift"] = shift
@@ -320,65 +368,75 @@ def compute_initial_guess(f, g, mesh, para, previous=None, split_fraction=None):
 
 
 # ---------------------------------------------------------------------------- coarse-lattice start
-def _axes(mesh):
-    """The lattice axes ``(z0, y0, x0)`` of a full node grid."""
+def _lattice_axes(mesh):
+    """Node positions along z, y and x of a full node grid."""
     nz, ny, nx = (int(v) for v in mesh.grid_shape)
     c = np.asarray(mesh.coordinates, dtype=np.float64).reshape(nz, ny, nx, 3)
     return c[:, 0, 0, 2].copy(), c[0, :, 0, 1].copy(), c[0, 0, :, 0].copy()
 
 
 def coarse_lattice(mesh, factor: int):
-    """Every ``factor``-th node per axis as a mesh of its own, and the fine index of each of its
-    nodes (coarse_init.py:33-48); ``None`` when fewer than two nodes per axis would remain."""
+    """``(coarse mesh, fine index)``: the nodes whose grid index is a multiple of ``factor`` along
+    every axis, as a mesh of their own (coarse_init.py:33-48); ``None`` for ``factor`` 1 or fewer than
+    two such nodes along an axis."""
     from .local import NodeMesh
 
     k = int(factor)
-    if k <= 1:
+    grid = tuple(int(v) for v in mesh.grid_shape)
+    picked = np.arange(mesh.n_nodes).reshape(grid)[::k, ::k, ::k] if k > 1 else None
+    if picked is None or min(picked.shape) < MIN_COARSE_NODES_PER_AXIS:
         return None
-    sel = [np.arange(0, int(n), k) for n in mesh.grid_shape]
-    if min(a.size for a in sel) < MIN_COARSE_NODES_PER_AXIS:
-        return None
-    nz, ny, nx = (int(v) for v in mesh.grid_shape)
-    iz, iy, ix = np.meshgrid(*sel, indexing="ij")
-    fine_index = (iz * ny * nx + iy * nx + ix).ravel()
-    coords = np.asarray(mesh.coordinates, dtype=np.float64)[fine_index]
-    valid = np.asarray(mesh.node_valid, dtype=bool)
-    node_valid = valid[fine_index].copy() if valid.size == mesh.n_nodes else np.ones(fine_index.size, dtype=bool)
-    return NodeMesh(coords, (sel[0].size, sel[1].size, sel[2].size), node_valid), fine_index
+    fine_index = picked.ravel()
+    node_valid = np.asarray(mesh.node_valid, dtype=bool)
+    sub_valid = node_valid[fine_index].copy() if node_valid.size == mesh.n_nodes else np.ones(fine_index.size, dtype=bool)
+    return NodeMesh(np.asarray(mesh.coordinates, dtype=np.float64)[fine_index], picked.shape, sub_valid), fine_index
 
 
-def _nearest_fill(flat: np.ndarray) -> np.ndarray:
-    """Non-finite coarse nodes take the values of the nearest finite one (coarse_init.py:70-78)."""
-    from scipy.ndimage import distance_transform_edt
+def _nearest_finite(table: torch.Tensor) -> torch.Tensor:
+    """Rows of ``table`` (nz, ny, nx, C) with a non-finite entry take the row of the nearest grid node
+    whose row is finite (Euclidean distance in grid steps; among equally near nodes the first in C
+    order). By brute force on the device, in blocks of holes."""
+    nz, ny, nx, C = table.shape
+    flat = table.reshape(-1, C)
+    finite = torch.isfinite(flat).all(1)
+    holes = torch.nonzero(~finite).squeeze(1)
+    if holes.numel() == 0:
+        return table
+    donors = torch.nonzero(finite).squeeze(1)
+    if donors.numel() == 0:
+        return torch.nan_to_num(table)
 
-    bad = ~np.all(np.isfinite(flat), axis=-1)
-    if bad.all():
-        return np.nan_to_num(flat)
-    _, idx = distance_transform_edt(bad, return_distances=True, return_indices=True)
-    return flat[tuple(idx)]
+    def ijk(idx):
+        return torch.stack([idx // (ny * nx), (idx // nx) % ny, idx % nx], 1).double()
+
+    pd = ijk(donors)
+    out = flat.clone()
+    step = max(1, (1 << 26) // max(int(donors.numel()), 1))
+    for a in range(0, int(holes.numel()), step):
+        h = holes[a : a + step]
+        dist = torch.cdist(ijk(h), pd)  # argmin returns the first of equal minima
+        out[h] = flat[donors[torch.argmin(dist, 1)]]
+    return out.reshape(table.shape)
 
 
 def interpolate_to_mesh(coarse, values, mesh) -> np.ndarray:
     """Trilinear interpolation of per-node values ``(Nc, ...)`` from the coarse lattice to every node
-    of ``mesh``, linear extrapolation beyond the lattice, non-finite coarse values filled from the
-    nearest finite node first (coarse_init.py:51-67). On the device in float64."""
+    of ``mesh``, linear extrapolation beyond the lattice, non-finite coarse values replaced by the
+    nearest finite node's first (coarse_init.py:51-67). On the device in float64."""
     v = np.asarray(values, dtype=np.float64)
     gz, gy, gx = (int(n) for n in coarse.grid_shape)
-    flat = v.reshape(gz, gy, gx, -1)
-    if not np.all(np.isfinite(flat)):
-        flat = _nearest_fill(flat)
     dev = K.device()
-    table = torch.from_numpy(np.ascontiguousarray(flat)).to(dev)
+    table = _nearest_finite(torch.from_numpy(np.ascontiguousarray(v.reshape(gz, gy, gx, -1))).to(dev))
     pts = torch.from_numpy(np.asarray(mesh.coordinates, dtype=np.float64)).to(dev)
     idx, frac = [], []
-    for axis, col in zip(_axes(coarse), (2, 1, 0)):
+    for axis, col in zip(_lattice_axes(coarse), (2, 1, 0)):
         a = torch.from_numpy(axis).to(dev)
         x = pts[:, col].contiguous()
         i = torch.clamp(torch.searchsorted(a, x, right=True) - 1, 0, a.numel() - 2)
         idx.append(i)
         frac.append((x - a[i]) / (a[i + 1] - a[i]))  # outside [0, 1] beyond the lattice: extrapolation
     (iz, iy, ix), (tz, ty, tx) = idx, frac
-    out = torch.zeros((pts.shape[0], flat.shape[-1]), dtype=torch.float64, device=dev)
+    out = torch.zeros((pts.shape[0], table.shape[-1]), dtype=torch.float64, device=dev)
     for dz in (0, 1):
         wz = tz if dz else 1.0 - tz
         for dy in (0, 1):
@@ -390,31 +448,30 @@ def interpolate_to_mesh(coarse, values, mesh) -> np.ndarray:
 
 
 def coarse_initial_guess(ref, g_norm, g_prep, mesh, para):
-    """``(U0, F0, info)`` for every node from the solve on every ``para.init_coarse_factor``-th node:
-    the initial guess and the 12-DOF local step on the coarse lattice, U and F interpolated to all
-    nodes (coarse_init.py:81-128). Falls back to :func:`compute_initial_guess` (``F0 = None``)
-    when the coarse lattice would have fewer than two nodes per axis."""
+    """``(U0, F0, info)``: the initial guess and the 12-DOF local step on the coarse lattice of every
+    ``para.init_coarse_factor``-th node, U and F interpolated to all nodes (coarse_init.py:81-128);
+    with too small a lattice, the plain initial guess and ``F0 = None``."""
     from .local import local_icgn, precompute_local_context
 
-    t0 = time.perf_counter()
-    lattice = coarse_lattice(mesh, int(para.init_coarse_factor))
+    t_start = time.perf_counter()
+    k = int(para.init_coarse_factor)
+    lattice = coarse_lattice(mesh, k)
     if lattice is None:
         U0, info = compute_initial_guess(ref.f, g_norm, mesh, para)
         info["coarse_factor"] = 1
         return U0, None, info
-    coarse, _fine_index = lattice
+    coarse, _ = lattice
     U0c, info = compute_initial_guess(ref.f, g_norm, coarse, para)
-    t_ncc = time.perf_counter() - t0
-    ctx_c = precompute_local_context(coarse, ref, para)
-    Uc, Fc, info_c, bad_c = local_icgn(ctx_c, ref, g_prep, U0c, para, coarse)
+    t_search = time.perf_counter() - t_start
+    ctx = precompute_local_context(coarse, ref, para)
+    Uc, Fc, solve, bad = local_icgn(ctx, ref, g_prep, U0c, para, coarse)
     U0 = interpolate_to_mesh(coarse, Uc, mesh)
     F0 = interpolate_to_mesh(coarse, np.asarray(Fc).reshape(coarse.n_nodes, 9), mesh).reshape(mesh.n_nodes, 3, 3)
-    dt = time.perf_counter() - t0
-    info.update({"method": f"coarse x{int(para.init_coarse_factor)} + {info.get('method', 'ncc')}",
-                 "coarse_factor": int(para.init_coarse_factor), "coarse_nodes": int(coarse.n_nodes),
-                 "coarse_converged": float(np.mean(info_c.status == 0)), "coarse_bad": int(bad_c.sum()),
-                 "coarse_ncc_time": t_ncc, "coarse_icgn_time": float(info_c.solve_time), "coarse_total_time": dt})  # fmt: skip
-    logger.info("Coarse initial guess: %d of %d nodes solved (%.1f%% converged, %d filled), %.2fs NCC + %.2fs IC-GN",
-                coarse.n_nodes, mesh.n_nodes, 100.0 * info["coarse_converged"], int(bad_c.sum()), t_ncc,
-                info_c.solve_time)  # fmt: skip
+    converged = float(np.mean(solve.status == 0))
+    info.update({"method": f"coarse x{k} + {info.get('method', 'ncc')}", "coarse_factor": k,
+                 "coarse_nodes": int(coarse.n_nodes), "coarse_converged": converged, "coarse_bad": int(bad.sum()),
+                 "coarse_ncc_time": t_search, "coarse_icgn_time": float(solve.solve_time),
+                 "coarse_total_time": time.perf_counter() - t_start})  # fmt: skip
+    logger.info("Coarse start: %d nodes of %d solved on the x%d lattice, %.1f %% converged, %d inpainted",
+                coarse.n_nodes, mesh.n_nodes, k, 100.0 * converged, int(bad.sum()))  # fmt: skip
     return U0, F0, info

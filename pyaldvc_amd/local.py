@@ -38,7 +38,6 @@ import numpy as np
 import torch
 
 from . import kernels as K
-from .cleanup import SPRING_MAX_UNKNOWN, fill_nan_grid, universal_median_test
 from .noise import STENCIL_NOISE_GAIN, noise_hessian_pattern
 from .tiling import KERNEL_MAX_VOXELS, local_split, plan_tiles, whole_box_tile
 
@@ -421,21 +420,12 @@ def _P_from_UF(U, F):
 
 def _outliers(U, status, mesh, para) -> np.ndarray:
     """bad = not converged, or flagged by the universal median test of U when the test is on and
-    more than 27 nodes converged (local_icgn.py:470-480, subpb1_solver.py:140-150). On the device
-    up to SPRING_MAX_UNKNOWN nodes (the unknown nodes are spring-filled there, as the reference
-    fills them up to that many), on the host beyond."""
-    N = int(np.shape(status)[0])
-    grid = tuple(mesh.grid_shape)
-    if N <= SPRING_MAX_UNKNOWN:
-        from .admm import subpb1_cleanup
+    more than 27 nodes converged (local_icgn.py:470-480, subpb1_solver.py:140-150); the test runs
+    on the device, the unknown nodes spring-filled there first."""
+    from .admm import subpb1_cleanup
 
-        _, bad = subpb1_cleanup(U, U, status, grid, float(para.local_outlier_threshold), edge_ok=mesh.edges_ok_grid())
-        return bad
-    bad = np.asarray(status) != STATUS_CONVERGED
-    if para.local_outlier_threshold > 0 and (~bad).sum() > 27:
-        flag = universal_median_test(np.asarray(U).reshape(grid + (3,)), (~bad).reshape(grid),
-                                     para.local_outlier_threshold, edge_ok=mesh.edges_ok_grid())  # fmt: skip
-        bad |= flag.ravel()
+    _, bad = subpb1_cleanup(U, U, status, tuple(mesh.grid_shape), float(para.local_outlier_threshold),
+                            edge_ok=mesh.edges_ok_grid())  # fmt: skip
     return bad
 
 
@@ -533,19 +523,14 @@ def subpb1_solver(ctx: LocalContext, ref, g, U_hat, F_hat, vdual, mu, para, mesh
             int(para.icgn_patience), ctx.stride, float(pattern[9, 9]), gain, bool(para.icgn_predictive_stop),
             **ctx.split_args(),
         )  # fmt: skip
-    if N <= SPRING_MAX_UNKNOWN:
-        # the cleanup on the device: median test (unknown nodes spring-filled, as the reference does
-        # up to this many nodes) and U_filled = bad ? U_hat : U
-        from .admm import subpb1_cleanup
+    # the cleanup on the device: median test (unknown nodes spring-filled first) and
+    # U_filled = bad ? U_hat : U (nodes the local solver could not handle keep the global estimate)
+    from .admm import subpb1_cleanup
 
-        U_filled, bad = subpb1_cleanup(U, Uh_d, status, tuple(mesh.grid_shape),
-                                       float(para.local_outlier_threshold), edge_ok=mesh.edges_ok_grid())  # fmt: skip
-        U_filled = U_filled.reshape(N, 3)
-    else:
-        Uh = U.cpu().numpy()
-        bad = _outliers(Uh, status.cpu().numpy(), mesh, para)
-        U_filled = Uh.copy()
-        U_filled[bad] = U_hat[bad]  # nodes the local solver could not handle keep the global estimate
+    U_filled, bad = subpb1_cleanup(U, Uh_d, status, tuple(mesh.grid_shape), float(para.local_outlier_threshold),
+                                   edge_ok=mesh.edges_ok_grid())  # fmt: skip
+    U_filled = U_filled.cpu().numpy().reshape(N, 3)
+    bad = bad.cpu().numpy()
     n_iter = n_iter.cpu().numpy().astype(np.int32)
     status = status.cpu().numpy().astype(np.int8)
     zncc = zncc.cpu().numpy().astype(np.float64)
@@ -556,32 +541,25 @@ def subpb1_solver(ctx: LocalContext, ref, g, U_hat, F_hat, vdual, mu, para, mesh
 
 
 def fill_bad_nodes(U, F, bad, mesh):
-    """Replace the ``bad`` nodes of U (and F) by spring-model inpainting on the node grid, each
-    side of a cut mesh on its own (local_icgn.py:492-522)."""
+    """The ``bad`` nodes of U (and F) replaced by the spring model on the node grid, no spring across
+    a cut edge (local_icgn.py:492-522). U and the nine F columns are filled on the device, three
+    columns per solve."""
+    from .admm import spring_fill
+
     if not bad.any():
         return U, F
     if bad.all():
-        logger.warning("All nodes are bad; filling with zeros.")
+        logger.warning("Every node is bad: U and F set to zero.")
         return np.zeros_like(U), (None if F is None else np.zeros_like(F))
     shape = tuple(mesh.grid_shape)
     edge_ok = mesh.edges_ok_grid()
-    holes = bad.reshape(-1)
-
-    def fill3(cols):
-        """Three columns (N, 3) with the bad nodes as holes, filled like fill_nan_grid: by springs
-        on the GPU up to SPRING_MAX_UNKNOWN unknowns, nearest-neighbour on the host beyond."""
-        v = np.array(cols, dtype=np.float64, copy=True)
-        v[holes] = np.nan
-        if int(np.isnan(v).sum(axis=0).max()) <= SPRING_MAX_UNKNOWN:
-            from .admm import spring_fill
-
-            return spring_fill(v, shape, edge_ok=edge_ok)
-        return np.stack([fill_nan_grid(v[:, c].reshape(shape), edge_ok=edge_ok).ravel() for c in range(3)], 1)
-
-    U_out = fill3(U)
-    F_out = None
+    holes = np.asarray(bad, dtype=bool).reshape(-1)
+    cols = np.asarray(U, dtype=np.float64).reshape(-1, 3)
     if F is not None:
-        F_out = fill3(F.reshape(-1, 9)[:, 0:3])
-        F_out = np.concatenate([F_out, fill3(F.reshape(-1, 9)[:, 3:6]), fill3(F.reshape(-1, 9)[:, 6:9])], 1)
-        F_out = F_out.reshape(F.shape)
+        cols = np.concatenate([cols, np.asarray(F, dtype=np.float64).reshape(-1, 9)], 1)
+    cols = np.array(cols, copy=True)
+    cols[holes] = np.nan
+    filled = np.concatenate([spring_fill(cols[:, k : k + 3], shape, edge_ok=edge_ok) for k in range(0, cols.shape[1], 3)], 1)
+    U_out = filled[:, :3].reshape(np.shape(U))
+    F_out = None if F is None else filled[:, 3:].reshape(np.shape(F))
     return U_out, F_out

@@ -1,8 +1,8 @@
 """ADMM iterations on the GPU (needs an MI355X).
 
 Each device piece against its host restatement, which the CPU tests pin to the reference and
-SciPy: the spring fill against ``cleanup.fill_nan_grid`` (uncut and cut meshes), the local-step
-cleanup against ``cleanup.universal_median_test`` (identical flags: index work), the global step
+SciPy: the spring fill against the oracle's ``fill_nan_grid`` (uncut and cut meshes), the local-step
+cleanup against the oracle's ``universal_median_test`` (identical flags: index work), the global step
 against the host-controlled PCG and nodal gradient, the dual update against NumPy. The reference's
 own ``subpb1_solver`` vectors cover the cleanup inside ``local.subpb1_solver`` (test_gpu_driver).
 The whole loop (``AdmmLoop``, eager and replayed from a HIP graph, several replays with other work
@@ -21,7 +21,7 @@ pytestmark = pytest.mark.gpu
 
 K = pytest.importorskip("pyaldvc_amd.kernels")
 from pyaldvc_amd import admm as A  # noqa: E402
-from pyaldvc_amd import cleanup as C  # noqa: E402
+from oracle import cleanup_ref as C  # noqa: E402
 from pyaldvc_amd import fem as FEM  # noqa: E402
 from pyaldvc_amd import local as LOC  # noqa: E402
 

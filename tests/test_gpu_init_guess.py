@@ -9,7 +9,7 @@ from scipy import ndimage
 from types import SimpleNamespace
 
 from oracle import ncc_ref
-from pyaldvc_amd import cleanup
+from oracle import cleanup_ref as cleanup
 from pyaldvc_amd import init_guess as IG
 from pyaldvc_amd import synthetic as syn
 from pyaldvc_amd.local import NodeMesh

@@ -1,4 +1,4 @@
-"""Node-grid cleanup and the noise pattern against the reference's own outputs (host code, CPU).
+"""The oracle's node-grid cleanup and the noise pattern against the reference's own outputs (CPU).
 
 Golden vectors: ``tests/golden/make_golden_driver.py`` (``universal_median_test``,
 ``fill_nan_grid``, ``fill_nan_nearest``, ``noise_hessian_pattern`` of al_dvc 1.2.1).
@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 from scipy import ndimage
 
-from pyaldvc_amd import cleanup as C
+from oracle import cleanup_ref as C
 from pyaldvc_amd import noise
 
 

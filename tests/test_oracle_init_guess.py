@@ -8,7 +8,7 @@ import pytest
 from scipy import ndimage
 
 from oracle import ncc_ref
-from pyaldvc_amd import cleanup
+from oracle import cleanup_ref as cleanup
 from pyaldvc_amd import synthetic as syn
 
 
