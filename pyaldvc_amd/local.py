@@ -39,7 +39,7 @@ import torch
 
 from . import kernels as K
 from .noise import STENCIL_NOISE_GAIN, noise_hessian_pattern
-from .tiling import KERNEL_MAX_VOXELS, local_split, plan_tiles, whole_box_tile
+from .tiling import KERNEL_MAX_VOXELS, Box, local_split, plan_tiles
 
 logger = logging.getLogger(__name__)
 
@@ -246,14 +246,14 @@ def precompute_local_context(mesh, ref, para) -> LocalContext:
     stride = int(getattr(para, "subset_stride", 1))
     shape = tuple(int(v) for v in np.shape(ref.f))
     plan = plan_tiles(mesh.grid_shape, coords_int, shape, para, half)
-    if plan.fallback:
-        logger.warning("%s", plan.fallback)
-    vols = _reference_volumes(ref, whole_gradients=plan.is_whole)
+    if plan.note:
+        logger.warning("%s", plan.note)
+    vols = _reference_volumes(ref, whole_gradients=plan.whole)
     coords_d = torch.from_numpy(coords_int).to(dev)
     node_valid = np.asarray(mesh.node_valid, dtype=bool)
     node_valid_d = torch.from_numpy(node_valid).to(dev)
     N = coords_int.shape[0]
-    if not plan.is_whole:
+    if not plan.whole:
         return _precompute_tiled(plan, vols, para, coords_int, coords_d, node_valid_d, half, stride, t0)
 
     split = _split_rows(vols, para, coords_d, node_valid_d, half, stride)
@@ -301,7 +301,7 @@ def _precompute_tiled(plan, vols, para, coords_int, coords_d, node_valid_d, half
     tiles = []
     for nodes, box in plan:
         nodes_d = torch.from_numpy(nodes).to(dev)
-        loc = torch.from_numpy(box.shift(coords_int[nodes])).to(dev)
+        loc = torch.from_numpy(box.local(coords_int[nodes])).to(dev)
         cand = None
         if split_on:  # one decision for the whole reference: count the candidates of every box first
             frac = K.subset_valid_fraction(box.crop(vols["mask"]), loc, winsize)
@@ -383,7 +383,7 @@ def _solve_tiled(ctx, mesh, para, disp, run, n_out):
                 outs = [torch.zeros((N,) + tuple(r.shape[1:]), dtype=r.dtype, device=dev) for r in res]
             for dst, src in zip(outs, res):
                 dst[nodes_d] = src
-            open_face[nodes] = box.has_open_face
+            open_face[nodes] = box.open
 
     solve(plan)
     status = outs[n_out - 2].cpu().numpy()
@@ -393,7 +393,7 @@ def _solve_tiled(ctx, mesh, para, disp, run, n_out):
                        int(escaped.size))  # fmt: skip
         open_face[:] = False
         if int(np.prod(shape)) <= KERNEL_MAX_VOXELS:
-            solve([(escaped, whole_box_tile(shape))])
+            solve([(escaped, Box.of_volume(shape))])
         else:
             d = np.abs(np.asarray(disp, dtype=np.float64)).reshape(N, 3)
             wider = np.where(np.isfinite(d), 2.0 * d, 0.0) + np.asarray(ctx.half, dtype=np.float64)[None, :]
@@ -451,7 +451,7 @@ def local_icgn(ctx: LocalContext, ref, g, U0, para, mesh, F0=None):
 
         def run(nodes_d, box):
             f_t, (gx_t, gy_t, gz_t), m_t = _tile_volumes(vols, box)
-            loc = torch.from_numpy(box.shift(ctx.coords_int[nodes_d.cpu().numpy()])).to(gd.device)
+            loc = torch.from_numpy(box.local(ctx.coords_int[nodes_d.cpu().numpy()])).to(gd.device)
             return K.icgn_12dof_parallel(
                 loc, P0[nodes_d], hx, hy, hz, f_t, gx_t, gy_t, gz_t, m_t, box.crop(gd), mode, d["L"][nodes_d],
                 d["meanf"][nodes_d], d["bottomf"][nodes_d], d["valid"][nodes_d], float(para.icgn_tol),
@@ -506,7 +506,7 @@ def subpb1_solver(ctx: LocalContext, ref, g, U_hat, F_hat, vdual, mu, para, mesh
 
         def run(nodes_d, box):
             f_t, (gx_t, gy_t, gz_t), m_t = _tile_volumes(vols, box)
-            loc = torch.from_numpy(box.shift(ctx.coords_int[nodes_d.cpu().numpy()])).to(dev)
+            loc = torch.from_numpy(box.local(ctx.coords_int[nodes_d.cpu().numpy()])).to(dev)
             return K.icgn_3dof_parallel(
                 loc, Uh_d[nodes_d], Fh_d[nodes_d], vd_d[nodes_d], hx, hy, hz, f_t, gx_t, gy_t, gz_t, m_t, box.crop(gd),
                 mode, d["H"][nodes_d], d["meanf"][nodes_d], d["bottomf"][nodes_d], d["valid"][nodes_d], float(mu),

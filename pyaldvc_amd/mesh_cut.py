@@ -20,13 +20,6 @@ import torch
 from . import kernels as K
 from ._lib import check, lib
 
-# mesh_cut.py:37-41: axis (x, y, z) -> (lower, upper) corner pairs of the four element edges along
-# it, for the hex8 corner order of hex8_elements
-EDGES = {
-    0: ((0, 1), (3, 2), (4, 5), (7, 6)),
-    1: ((0, 3), (1, 2), (4, 7), (5, 6)),
-    2: ((0, 4), (1, 5), (2, 6), (3, 7)),
-}
 MAX_ELEMENT_EXTENT = 64  # voxels per axis of an element box (one 64-bit LDS word per row)
 
 
@@ -46,15 +39,21 @@ def hex8_elements(grid_shape) -> np.ndarray:
 
 def bridging_elements(mask, coordinates, elements) -> np.ndarray:
     """``(E,)`` True for every element (rows of 8 node indices) that spans a masked boundary
-    (mesh_cut.py:98-106)."""
+    (mesh_cut.py:98-106). Element boxes, clamped to the volume, may be at most 64 voxels along
+    each axis (node steps up to 63); a wider one raises ``ValueError`` before any device work."""
     elements = np.asarray(elements, dtype=np.int64)
     E = elements.shape[0]
     if E == 0:
         return np.zeros(0, dtype=bool)
+    shape = tuple(int(v) for v in np.shape(mask))
     corners = np.round(np.asarray(coordinates, dtype=np.float64)[elements]).astype(np.int64)  # (E, 8, 3)
-    extent = int((corners.max(axis=1) - corners.min(axis=1)).max()) + 1
+    top = np.array(shape[::-1], dtype=np.int64) - 1  # x, y, z
+    lo = np.clip(corners.min(axis=1), 0, top)
+    hi = np.clip(corners.max(axis=1), 0, top)
+    extent = int((hi - lo).max()) + 1
     if extent > MAX_ELEMENT_EXTENT:
-        raise ValueError(f"element boxes up to {MAX_ELEMENT_EXTENT} voxels per axis are supported (got {extent})")
+        raise ValueError(f"element boxes of at most {MAX_ELEMENT_EXTENT} voxels per axis are supported inside the "
+                         f"volume (this mesh has {extent}: node step {extent - 1})")  # fmt: skip
     dev = K.device()
     m = K._Args.vol(mask, np.uint8)
     nz, ny, nx = (int(v) for v in m.shape)
@@ -68,32 +67,40 @@ def bridging_elements(mask, coordinates, elements) -> np.ndarray:
     return res == 1
 
 
+# For each axis (x, y, z): the corners at the low end of the element's four edges along that axis,
+# in the corner order of hex8_elements (their partners are one node step up the axis).
+EDGE_STARTS = ((0, 3, 4, 7), (0, 1, 4, 5), (0, 1, 2, 3))
+
+
 def edge_ok_from_elements(elements, bridging, n_nodes: int) -> np.ndarray:
-    """``(N, 3)`` edge flags: the +x, +y, +z edge of a node is cut when every element containing it
-    is bridging (mesh_cut.py:109-119)."""
-    elements = np.asarray(elements, dtype=np.int64)
-    b = np.asarray(bridging, dtype=np.int64)
-    total = np.zeros((n_nodes, 3), dtype=np.int64)
-    bridged = np.zeros((n_nodes, 3), dtype=np.int64)
-    for axis, pairs in EDGES.items():
-        for lo, _hi in pairs:
-            np.add.at(total[:, axis], elements[:, lo], 1)
-            np.add.at(bridged[:, axis], elements[:, lo], b)
-    return ~((total > 0) & (bridged == total))
+    """``(N, 3)`` flags of the +x, +y, +z edge of every node: False when the edge belongs to at least
+    one of ``elements`` and every element it belongs to is bridging (mesh_cut.py:109-119). Counted
+    on the device with integer scatter-adds."""
+    dev = K.device()
+    el = torch.as_tensor(np.asarray(elements, dtype=np.int64), device=dev).reshape(-1, 8)
+    br = torch.as_tensor(np.asarray(bridging, dtype=bool), device=dev).to(torch.int32)
+    members = torch.zeros(n_nodes * 3, dtype=torch.int32, device=dev)
+    bridged = torch.zeros_like(members)
+    for axis, starts in enumerate(EDGE_STARTS):
+        slot = (el[:, list(starts)] * 3 + axis).reshape(-1)
+        members.index_add_(0, slot, torch.ones_like(slot, dtype=torch.int32))
+        bridged.index_add_(0, slot, br.repeat_interleave(len(starts)))
+    return ((members == 0) | (bridged < members)).reshape(n_nodes, 3).cpu().numpy()
 
 
 def cut_mesh(elements, coordinates, mask, n_nodes: int):
-    """Drop the bridging elements among the live rows (``elements[:, 0] >= 0``): ``(elements,
-    edge_ok, n_dropped)`` (mesh_cut.py:122-138)."""
-    elements = np.array(elements, dtype=np.int64, copy=True)
-    rows = np.flatnonzero(elements[:, 0] >= 0) if elements.size else np.zeros(0, dtype=np.int64)
-    if rows.size == 0:
-        return elements, np.ones((n_nodes, 3), dtype=bool), 0
-    live = elements[rows]
-    bridging = bridging_elements(mask, coordinates, live)
-    edge_ok = edge_ok_from_elements(live, bridging, n_nodes)
-    elements[rows[bridging]] = -1
-    return elements, edge_ok, int(bridging.sum())
+    """``(elements, edge_ok, n_dropped)``: the live elements (first corner >= 0) that span a masked
+    boundary are dropped (rows set to -1) and the edges only they held are cut
+    (mesh_cut.py:122-138). Rows dropped before stay dropped and do not count against an edge."""
+    out = np.array(elements, dtype=np.int64, copy=True).reshape(-1, 8)
+    live = out[:, 0] >= 0
+    if not live.any():
+        return out, np.ones((n_nodes, 3), dtype=bool), 0
+    drop = np.zeros(out.shape[0], dtype=bool)
+    drop[live] = bridging_elements(mask, coordinates, out[live])
+    edge_ok = edge_ok_from_elements(out[live], drop[live], n_nodes)
+    out[drop] = -1
+    return out, edge_ok, int(drop.sum())
 
 
 def grid_surface_nodes(grid_shape) -> np.ndarray:
@@ -106,16 +113,20 @@ def grid_surface_nodes(grid_shape) -> np.ndarray:
 
 
 def cut_edge_nodes(edge_ok, grid_shape) -> np.ndarray:
-    """The nodes at either end of a cut edge (mesh_cut.py:154-165)."""
-    nz, ny, nx = (int(v) for v in grid_shape)
-    strides = (1, nx, nx * ny)
-    e = np.asarray(edge_ok, dtype=bool).reshape(-1, 3)
-    nodes = []
-    for axis in range(3):
-        lo = np.flatnonzero(~e[:, axis])
-        nodes += [lo, lo + strides[axis]]
-    out = np.unique(np.concatenate(nodes)) if nodes else np.zeros(0, dtype=np.int64)
-    return out[out < nz * ny * nx]
+    """Sorted indices of the nodes at either end of a cut edge (mesh_cut.py:154-165)."""
+    grid = tuple(int(v) for v in grid_shape)
+    cut = ~np.asarray(edge_ok, dtype=bool).reshape(grid + (3,))
+    ends = np.zeros(grid, dtype=bool)
+    for comp, axis in ((0, 2), (1, 1), (2, 0)):  # edge component -> array axis of x, y, z
+        c = cut[..., comp]
+        ends |= c
+        n = grid[axis]
+        if n > 1:  # the node one step up the axis closes the edge
+            up = [slice(None)] * 3
+            down = [slice(None)] * 3
+            up[axis], down[axis] = slice(1, n), slice(0, n - 1)
+            ends[tuple(up)] |= c[tuple(down)]
+    return np.flatnonzero(ends.ravel()).astype(np.int64)
 
 
 @dataclass

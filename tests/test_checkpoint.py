@@ -3,6 +3,7 @@
 from __future__ import annotations
 
 import json
+from types import SimpleNamespace
 
 import numpy as np
 import pytest
@@ -81,3 +82,72 @@ def test_resume_checks_the_run_and_a_fresh_start_clears_frames(tmp_path):
     assert ck.completed_frames() == []
     fr, _ = ck.load(0) if ck.has(0) else (None, None)
     assert fr is None
+
+
+# ---------------------------------------------------------------------------- interop with the reference
+REF_DIR = __import__("os").path.join(__import__("os").path.dirname(__file__), "golden", "ckpt_ref")
+
+
+def _ref_arrays():
+    with np.load(f"{REF_DIR}/frame_0001.npz", allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+def test_reads_a_frame_the_reference_wrote():
+    """tests/golden/ckpt_ref was written by the reference's Checkpoint.save (make_golden_checkpoint.py)."""
+    raw = _ref_arrays()
+    fr, mesh = C.Checkpoint(REF_DIR).load(1)
+    for name in ("U", "F", "U_accum", "U_local", "F_local", "U0", "zncc", "U_std", "status", "split_fraction", "outlier"):
+        assert np.array_equal(getattr(fr, name), raw[name]), name
+    assert fr.ref_frame == 0 and fr.outlier.dtype == bool
+    for name in ("node_valid", "elements", "boundary_nodes", "edge_ok"):
+        assert np.array_equal(mesh[name], raw[name]), name
+    a = fr.admm
+    assert (a.beta, a.mu, a.n_steps, a.subpb2_time) == tuple(raw["admm_scalars"][[0, 1]]) + (4, 0.75)
+    assert a.update_global == tuple(raw["admm_update_global"]) and a.primal_residual_f == tuple(raw["admm_primal_residual_f"])
+    assert len(a.local_info) == 2
+    for i, li in enumerate(a.local_info):
+        assert np.array_equal(li.n_iter, raw[f"admm_local_{i}_n_iter"]) and np.array_equal(li.status, raw[f"admm_local_{i}_status"])
+        assert (li.solve_time, li.n_bad) == (0.125 * (i + 1), i + 2)
+    assert a.beta_sweep["k_best"] == 1 and isinstance(a.beta_sweep["k_best"], int)
+    assert np.array_equal(a.beta_sweep["betas"], raw["admm_sweep_betas"])
+
+
+def test_writes_the_reference_layout(tmp_path):
+    """A frame written here has the reference file's keys, dtypes, shapes and values."""
+    raw = _ref_arrays()
+    fr, mesh = C.Checkpoint(REF_DIR).load(1)
+    ck = C.Checkpoint(tmp_path)
+    path = ck.save(1, fr, SimpleNamespace(**mesh))
+    with np.load(path, allow_pickle=False) as z:
+        mine = {k: z[k] for k in z.files}
+    assert sorted(mine) == sorted(raw)
+    for k in raw:
+        assert (mine[k].dtype, mine[k].shape) == (raw[k].dtype, raw[k].shape), k
+        assert np.array_equal(mine[k], raw[k]), k
+    assert [p.name for p in tmp_path.iterdir()] == ["frame_0001.npz"]  # no partial file left behind
+
+
+def test_resumes_on_the_reference_meta_with_attribute_style_parameters(tmp_path):
+    import json
+    import shutil
+
+    stored = json.loads(open(f"{REF_DIR}/meta.json").read())
+    p = dict(stored["para"])
+    # the parameter object as a caller holds it: attributes, a VOI range object, a schedule object, tuples
+    p["voi"] = SimpleNamespace(**{a: tuple(v) for a, v in p["voi"].items()})
+    p["frame_schedule"] = SimpleNamespace(ref_indices=tuple(p["frame_schedule"]["ref_indices"]))
+    p = {k: tuple(v) if isinstance(v, list) else v for k, v in p.items()}
+    p["verbose"] = False  # left out of the comparison, as the reference leaves it out of meta.json
+    para = SimpleNamespace(**p)
+    mesh = SimpleNamespace(grid_shape=tuple(stored["grid_shape"]), x0=stored["x0"], y0=stored["y0"], z0=stored["z0"])
+    meta = C.run_meta(para, tuple(stored["volume_shape"]), para.frame_schedule, mesh, stored["n_frames"])
+    assert C.meta_differences(stored, meta) == []
+    assert meta["para"]["voi"] == stored["para"]["voi"] and "verbose" not in meta["para"]
+    shutil.copytree(REF_DIR, tmp_path / "run")
+    ck = C.Checkpoint(tmp_path / "run")
+    ck.prepare(meta, resume=True)
+    assert ck.completed_frames() == [1]
+    other = C.run_meta(SimpleNamespace(**{**p, "mu": 5e-3}), tuple(stored["volume_shape"]), [0, 0], mesh, 3)
+    with pytest.raises(C.CheckpointMismatch, match="para.mu"):
+        ck.prepare(other, resume=True)

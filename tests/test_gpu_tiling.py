@@ -65,7 +65,7 @@ def test_tiled_local_steps_equal_the_whole_volume(gd, p, mode):
     whole_para, tiled_para = _para(subset_split=split), _para(subset_split=split, tile_local=38)
     plan = T.plan_tiles(mesh.grid_shape, np.round(mesh.coordinates).astype(np.int64), gd["drv_f"].shape, tiled_para,
                         (6, 6, 6), disp=gd[f"{p}_U0"])  # fmt: skip
-    assert len(plan) > 1 and any(b.has_open_face for _n, b in plan)
+    assert len(plan) > 1 and any(b.open for _n, b in plan)
     c0 = LOC.precompute_local_context(mesh, ref, whole_para)
     c1 = LOC.precompute_local_context(mesh, ref, tiled_para)
     assert getattr(c1, "n_tiles", 1) > 1
@@ -92,8 +92,8 @@ def test_nodes_that_leave_a_short_halo_are_solved_again(gd, monkeypatch):
     para = _para(tile_local=24)
     c0 = LOC.precompute_local_context(mesh, ref, _para())
     r0 = LOC.local_icgn(c0, ref, gd["drv_g"], gd["drv_U0"], _para(), mesh)
-    real = T.tile_pad
-    monkeypatch.setattr(T, "tile_pad", lambda para_, half, disp=None, deformed=False: real(para_, half))
+    real = T.halo
+    monkeypatch.setattr(T, "halo", lambda para_, half, disp=None, deformed=False: real(para_, half))
     seen = {}
     orig = LOC._solve_tiled
 
@@ -101,7 +101,7 @@ def test_nodes_that_leave_a_short_halo_are_solved_again(gd, monkeypatch):
         def run_spy(nodes_d, box):
             out = run(nodes_d, box)
             seen.setdefault("oob", 0)
-            seen["oob"] += int((out[2] == K.STATUS_OUT_OF_BOUNDS).sum()) if box.has_open_face else 0
+            seen["oob"] += int((out[2] == K.STATUS_OUT_OF_BOUNDS).sum()) if box.open else 0
             return out
 
         return orig(ctx, mesh_, para_, disp, run_spy, n_out)
