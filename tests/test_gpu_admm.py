@@ -236,27 +236,46 @@ def _host_loop(ctx, ref, g, mesh, system, para, U1, F1):
     return dict(U=U2, F=F2, n=n_steps, upd_g=upd_g, upd_l=upd_l, res_u=res_u, res_f=res_f, status=status, bad=bad)
 
 
-@pytest.fixture(scope="module")
-def frame(gd):
-    """The golden driver's 40x44x48 pair: local 12-DOF solve, then the global system."""
-    coords = gd["drv_coords"]
-    grid = tuple(int(v) for v in gd["drv_grid_shape"])
-    mesh = LOC.NodeMesh(coords, grid, gd["drv_node_valid_in"].copy())
-    ref = SimpleNamespace(f=gd["drv_f"], mask=None, gradient_mode="stored")
-    para = _para()
+@pytest.fixture(scope="module", params=["drv", "drvm"])
+def frame(gd, request):
+    """A golden driver pair (40x44x48): local 12-DOF solve, then the global system. ``drv``: no mask;
+    ``drvm``: a cylinder mask with voids and a crack, split subsets and a cut mesh (the elements that
+    bridge the mask dropped, cut edges excluded from the median test and the fills)."""
+    p = request.param
+    coords = gd[f"{p}_coords"]
+    grid = tuple(int(v) for v in gd[f"{p}_grid_shape"])
+    masked = p == "drvm"
+    para = _para(subset_split=masked)
+    if masked:
+        from pyaldvc_amd.mesh_cut import apply_mask_to_mesh
+
+        mm = apply_mask_to_mesh(coords, grid, gd["drvm_mask"].astype(bool), para.winsize, para.min_valid_ratio,
+                                cut_bridging=True)  # fmt: skip
+        assert np.array_equal(mm.node_valid, gd["drvm_node_valid_in"])
+        assert np.array_equal(mm.edge_ok, gd["drvm_edge_ok"])
+        mesh = LOC.NodeMesh(coords, grid, gd["drvm_node_valid_in"].copy(), gd["drvm_edge_ok"])
+        ref = SimpleNamespace(f=gd["drv_f"], mask=gd["drvm_mask"].astype(bool), gradient_mode="stored")
+        elements = mm.elements
+    else:
+        mesh = LOC.NodeMesh(coords, grid, gd[f"{p}_node_valid_in"].copy())
+        ref = SimpleNamespace(f=gd["drv_f"], mask=None, gradient_mode="stored")
+        elements = FEM.hex8_elements(grid)
     ctx = LOC.precompute_local_context(mesh, ref, para)
     mesh.node_valid = ctx.valid.copy()
-    U1, F1, info, bad = LOC.local_icgn(ctx, ref, gd["drv_g"], gd["drv_U0"], para, mesh)
+    U1, F1, info, bad = LOC.local_icgn(ctx, ref, gd["drv_g"], gd[f"{p}_U0"], para, mesh)
     spacing = tuple(float(np.diff(np.unique(coords[:, k]))[0]) for k in range(3))
-    el = FEM.hex8_elements(grid, mesh.node_valid)
+    el = np.array(elements, copy=True)
+    el[(el < 0).any(1) | ~mesh.node_valid[np.maximum(el, 0)].all(1)] = -1
     ops = FEM.build_global_operators(SimpleNamespace(grid_shape=grid, elements=el, spacing=spacing), "fem", 2)
     system = FEM.build_global_system(ops, 1e-3 * np.mean(spacing) ** 2, para.mu, 0.0, para)
-    return SimpleNamespace(mesh=mesh, ref=ref, para=para, ctx=ctx, U1=U1, F1=F1, info=info, system=system, g=gd["drv_g"])
+    return SimpleNamespace(mesh=mesh, ref=ref, para=para, ctx=ctx, U1=U1, F1=F1, info=info, system=system, g=gd["drv_g"],
+                           name=p)  # fmt: skip
 
 
 def _stopping_tol(fr):
     """An admm_tol at which the reference loop stops at step 2 (just above the smaller update)."""
-    host = _host_loop(fr.ctx, fr.ref, fr.g, fr.mesh, fr.system, _para(admm_tol=1e-12), fr.U1, fr.F1)
+    host = _host_loop(fr.ctx, fr.ref, fr.g, fr.mesh, fr.system, _para(admm_tol=1e-12, subset_split=fr.para.subset_split),
+                      fr.U1, fr.F1)  # fmt: skip
     return 1.01 * min(host["upd_g"][0], host["upd_l"][0])
 
 
@@ -264,7 +283,7 @@ def _stopping_tol(fr):
 def test_admm_loop_matches_the_reference_loop(frame, stops):
     fr = frame
     tol = _stopping_tol(fr) if stops else 1e-12
-    para = _para(admm_tol=tol)
+    para = _para(admm_tol=tol, subset_split=fr.para.subset_split)
     host = _host_loop(fr.ctx, fr.ref, fr.g, fr.mesh, fr.system, para, fr.U1, fr.F1)
     loop = A.AdmmLoop(fr.ctx, fr.g, fr.mesh, fr.system, para)
     loop.load(fr.U1, fr.F1)
@@ -284,7 +303,7 @@ def test_admm_loop_graph_replays_equal_eager(frame, stops):
     import torch
 
     fr = frame
-    para = _para(admm_tol=_stopping_tol(fr) if stops else 1e-12)
+    para = _para(admm_tol=_stopping_tol(fr) if stops else 1e-12, subset_split=fr.para.subset_split)
     loop = A.AdmmLoop(fr.ctx, fr.g, fr.mesh, fr.system, para)
     loop.load(fr.U1, fr.F1)
     loop.run(graph=False)
@@ -307,7 +326,7 @@ def test_admm_loop_graph_budget_too_small_falls_back(frame):
     is redone eagerly with the full limits and equals the eager result, and the next call records
     the graph again with the budgets of that run."""
     fr = frame
-    para = _para(admm_tol=1e-12)
+    para = _para(admm_tol=1e-12, subset_split=fr.para.subset_split)
     loop = A.AdmmLoop(fr.ctx, fr.g, fr.mesh, fr.system, para)
     loop.load(fr.U1, fr.F1)
     loop.run(graph=False)

@@ -203,3 +203,55 @@ def test_coarse_initial_guess_starts_near_the_affine_field():
     err = np.abs(U0 - np.column_stack([u, v, w]))
     assert np.max(err[inner]) < 0.05, (info, np.median(err[inner], axis=0), coords[inner][np.argmax(err[inner].max(1))])
     assert np.max(np.abs(F0[inner] - np.asarray(F))) < 5e-3  # F is the displacement gradient, as P stores it
+
+
+# ---------------------------------------------------------------------------- reference vectors
+@pytest.fixture(scope="module")
+def gp():
+    import os
+
+    with np.load(os.path.join(os.path.dirname(__file__), "golden", "golden_pyramid.npz"), allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+@pytest.mark.parametrize("key", ["plain", "gshift"])
+def test_pyramid_matches_the_reference_vectors(gp, key):
+    """The device pyramid against the reference's own pyramid_search (make_golden_pyramid.py): level
+    count, per-level radii and solved counts, expansions and flags identical; displacements within
+    1e-3 voxel."""
+    f, g = gp["f"], gp["g"]
+    assert IG.auto_pyramid_levels(f.shape, (16, 16, 16), (4, 4, 4), f=f) == int(gp["levels_auto"])
+    gs = IG.phase_correlation_shift(f, g)
+    assert np.array_equal(gs, gp["global_shift"])
+    got = IG.pyramid_search(f, g, gp["coords"], (4, 4, 4), (16, 16, 16), (4, 4, 4), levels=0,
+                            global_shift=gs if key == "gshift" else None)  # fmt: skip
+    assert got["levels"] == int(gp[f"{key}_levels"])
+    assert [tuple(p["radius"]) for p in got["per_level"]] == [tuple(r) for r in gp[f"{key}_radii"]]
+    assert [p["n_ok"] for p in got["per_level"]] == gp[f"{key}_n_ok"].tolist()
+    assert got["expansions"] == int(gp[f"{key}_expansions"])
+    assert np.array_equal(got["ok"], gp[f"{key}_ok"]) and np.array_equal(got["clipped"], gp[f"{key}_clipped"])
+    assert np.max(np.abs(got["disp"] - gp[f"{key}_disp"])) < 1e-3
+
+
+@pytest.mark.parametrize("key", ["ncc_r3", "ncc_shift", "ncc_exp"])
+def test_ncc_search_matches_the_reference_vectors(golden, key):
+    """integer_search.ncc_search / ncc_search_expanding on the device against the reference's outputs
+    (golden_local_step.npz): flags, expansions and radius identical, integer peaks identical,
+    sub-voxel displacements within 1e-3 voxel."""
+    from pyaldvc_amd import integer_search as IS
+
+    f, g, sc = golden["f"], golden["g"], golden["ncc_coords"]
+    if key == "ncc_r3":
+        res = IS.ncc_search(f, g, sc, (8, 8, 8), (3, 3, 3))
+    elif key == "ncc_shift":
+        res = IS.ncc_search(f, g, sc, (8, 6, 8), (2, 3, 2), golden["ncc_shift0"])
+    else:
+        res = IS.ncc_search_expanding(f, g, sc, (8, 8, 8), (1, 1, 1))
+        assert res["expansions"] == int(golden["ncc_exp_expansions"])
+        assert tuple(res["radius"]) == tuple(golden["ncc_exp_radius"])
+    assert np.array_equal(res["ok"], golden[f"{key}_ok"])
+    assert np.array_equal(res["clipped"], golden[f"{key}_clipped"])
+    ok = res["ok"]
+    assert np.array_equal(np.round(res["disp"][ok]), np.round(golden[f"{key}_disp"][ok]))
+    assert np.max(np.abs(res["disp"][ok] - golden[f"{key}_disp"][ok])) < 1e-3
+    assert np.max(np.abs(res["cc"][ok] - golden[f"{key}_cc"][ok])) < 1e-4

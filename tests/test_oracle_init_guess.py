@@ -87,3 +87,30 @@ def test_pyramid_recovers_the_reference_large_shift():
     info2 = ncc_ref.pyramid_search(f, g, coords, (4, 4, 4), (16, 16, 16), (4, 4, 4), 0, _clean_host,
                                    global_shift=ncc_ref.phase_correlation_shift(f, g))  # fmt: skip
     assert np.all(np.abs(info2["disp"] - np.array(shift)) < 0.3)
+
+
+@pytest.fixture(scope="module")
+def gp():
+    import os
+
+    with np.load(os.path.join(os.path.dirname(__file__), "golden", "golden_pyramid.npz"), allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+@pytest.mark.parametrize("key", ["plain", "gshift"])
+def test_pyramid_matches_the_reference_vectors(gp, key):
+    """The reference's own pyramid_search (make_golden_pyramid.py): same level count, per-level radii
+    and solved counts, expansions, ok/clipped flags; displacements within 2e-4 voxel (its maps come
+    from float32 FFTs, the oracle's from float64 direct sums)."""
+    f, g = gp["f"], gp["g"]
+    assert ncc_ref.auto_pyramid_levels(f.shape, (16, 16, 16), (4, 4, 4), f=f) == int(gp["levels_auto"])
+    gs = ncc_ref.phase_correlation_shift(f, g)
+    assert np.array_equal(gs, gp["global_shift"])
+    info = ncc_ref.pyramid_search(f, g, gp["coords"], (4, 4, 4), (16, 16, 16), (4, 4, 4), 0, _clean_host,
+                                  global_shift=gs if key == "gshift" else None)  # fmt: skip
+    assert len(info["per_level"]) == int(gp[f"{key}_levels"]) + 1
+    assert [tuple(p["radius"]) for p in info["per_level"]] == [tuple(r) for r in gp[f"{key}_radii"]]
+    assert [p["n_ok"] for p in info["per_level"]] == gp[f"{key}_n_ok"].tolist()
+    assert np.array_equal(info["ok"], gp[f"{key}_ok"]) and np.array_equal(info["clipped"], gp[f"{key}_clipped"])
+    assert np.max(np.abs(info["disp"] - gp[f"{key}_disp"])) < 2e-4
+    assert np.max(np.abs(info["cc"] - gp[f"{key}_cc"])) < 1e-5
