@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-/* ABI version of this header (1). */
+/* ABI version of this header (2). */
 int aldvc_abi_version(void);
 
 /* Hash of the sources the library was compiled from (16 hex digits); no counterpart in the
@@ -361,13 +361,19 @@ int aldvc_warp_znssd(const float *f, const float *g, int64_t nz, int64_t ny, int
                      int gny, int gnz, double ox, double oy, double oz, double sx, double sy, double sz, double *C,
                      double *stats, hipStream_t stream);
 
+/* Workspace bytes of aldvc_warp_znssd_backward (no counterpart in the reference, which has no
+ * backward): per-voxel fields of the volume (36 B per voxel) plus small per-subset and per-cell parts. */
+size_t aldvc_warp_backward_workspace_bytes(int64_t nz, int64_t ny, int64_t nx, int64_t S, int hx, int hy, int hz,
+                                           int gnx, int gny, int gnz);
+
 /* Accumulates grad_u += dL/du (float64, node layout of node_u) and, when grad_g is not NULL,
- * grad_g += dL/dg (float32 volume) for dL/dC = grad_C (S). */
+ * grad_g += dL/dg (float32 volume) for dL/dC = grad_C (S). Deterministic: every sum runs in a fixed
+ * order (no float atomics), so two calls on the same inputs give bit-identical gradients. */
 int aldvc_warp_znssd_backward(const float *f, const float *g, int64_t nz, int64_t ny, int64_t nx, int mode,
                               const int64_t *centers, int64_t S, int hx, int hy, int hz, const float *node_u,
                               int gnx, int gny, int gnz, double ox, double oy, double oz, double sx, double sy,
                               double sz, const double *stats, const double *grad_C, double *grad_u, float *grad_g,
-                              hipStream_t stream);
+                              void *workspace, size_t workspace_bytes, hipStream_t stream);
 
 /* Bridging elements of a cut mesh: out[e] = 1 when the eight corners (voxels [x, y, z], int64,
  * corners[e * 24 + 3 * k + d]) of element e do not all lie in the 6-connected in-mask component

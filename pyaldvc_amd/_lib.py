@@ -19,7 +19,7 @@ from . import build
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libaldvc_hip.so")
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 P = ctypes.c_void_p
 I64 = ctypes.c_int64
@@ -151,10 +151,11 @@ SIGNATURES = {
         I32,
         [P, P, I64, I64, I64, I32, P, I64, I32, I32, I32, P, I32, I32, I32, F64, F64, F64, F64, F64, F64, P, P, P],
     ),
+    "aldvc_warp_backward_workspace_bytes": (SZ, [I64, I64, I64, I64, I32, I32, I32, I32, I32, I32]),
     "aldvc_warp_znssd_backward": (
         I32,
         [P, P, I64, I64, I64, I32, P, I64, I32, I32, I32, P, I32, I32, I32, F64, F64, F64, F64, F64, F64,
-         P, P, P, P, P],
+         P, P, P, P, P, SZ, P],
     ),  # fmt: skip
 }
 

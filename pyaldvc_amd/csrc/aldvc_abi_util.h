@@ -6,7 +6,7 @@
 
 #include <string>
 
-#define ALDVC_ABI_VERSION 1
+#define ALDVC_ABI_VERSION 2
 
 namespace aldvc {
 

@@ -6,7 +6,8 @@ The forward is the reference's ZNCC of a warp (solver/numba_kernels.py:1354-1387
 interpolation of node displacements (mesh/hex8.py:58-61); per subset it returns
 ``C = sum (f~ - g~)^2 = 2 (1 - ZNCC)``. The backward (the reference has none) gives ``dL/du``
 for the node displacements and ``dL/dg`` for the deformed volume, from the HIP kernels of
-``csrc/aldvc_warp.hip`` with interpolation weights recomputed rather than stored.
+``csrc/aldvc_warp.hip`` with interpolation weights recomputed rather than stored; its sums run in
+a fixed order, so the gradients are bit-identical from run to run.
 """
 
 from __future__ import annotations
@@ -71,11 +72,15 @@ class WarpResidual(torch.autograd.Function):
         gC = torch.nan_to_num(grad_C.to(torch.float64), nan=0.0).contiguous()
         grad_u = torch.zeros(node_u.shape, dtype=torch.float64, device=f.device)
         grad_g = torch.zeros_like(g) if ctx.need_g else None
+        S = int(centers.shape[0])
+        gnz, gny, gnx = (int(v) for v in node_u.shape[:3])
+        nbytes = int(lib().aldvc_warp_backward_workspace_bytes(nz, ny, nx, S, *(int(h) for h in half), gnx, gny, gnz))
+        ws = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=f.device)
         check(
             lib().aldvc_warp_znssd_backward(
-                K._ptr(f), K._ptr(g), nz, ny, nx, mode, K._ptr(centers), int(centers.shape[0]), *half, K._ptr(node_u),
+                K._ptr(f), K._ptr(g), nz, ny, nx, mode, K._ptr(centers), S, *half, K._ptr(node_u),
                 *_geometry(node_u, origin, spacing), K._ptr(stats), K._ptr(gC), K._ptr(grad_u), K._ptr(grad_g),
-                K.stream_handle(),
+                K._ptr(ws), nbytes, K.stream_handle(),
             ),
             "aldvc_warp_znssd_backward",
         )  # fmt: skip

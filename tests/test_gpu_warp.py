@@ -207,3 +207,89 @@ def test_gradient_vanishes_at_the_icgn_solution():
     g_star = grad_at(Pc[0, 9:])
     g_off = grad_at(Pc[0, 9:] + np.array([0.3, -0.2, 0.25]))
     assert np.linalg.norm(g_star) < 0.05 * np.linalg.norm(g_off)
+
+
+def _keys(t):
+    """Keys cubic (a = -0.5) weights and derivatives of the 4 taps at fraction t (M,) -> (M, 4) each."""
+    t = np.asarray(t, dtype=np.float64)
+    w = np.stack([((-0.5 * t + 1.0) * t - 0.5) * t, (1.5 * t - 2.5) * t * t + 1.0, ((-1.5 * t + 2.0) * t + 0.5) * t,
+                  (0.5 * t - 0.5) * t * t], 1)  # fmt: skip
+    d = np.stack([(-1.5 * t + 2.0) * t - 0.5, (4.5 * t - 5.0) * t, (-4.5 * t + 4.0) * t + 0.5, (1.5 * t - 1.0) * t], 1)
+    return w, d
+
+
+def backward_ref(f, g, centers, half, node_u, origin, spacing, grad_C):
+    """float64 gradients of L = sum_n grad_C[n] C_n, subset by subset: every sample's dC/dg scattered
+    into its 64 cubic taps, and N_a dC/dg grad g(phi) into the 8 nodes of its cell."""
+    off = _offsets(half)
+    gg = np.zeros(g.shape)
+    gu = np.zeros(node_u.shape)
+    gnz, gny, gnx, _ = node_u.shape
+    for n, c in enumerate(centers):
+        if grad_C[n] == 0.0:
+            continue
+        p = c[None, :] + off
+        u = _fe(node_u, origin, spacing, p.astype(np.float64))
+        w = p + u
+        b = np.floor(w).astype(np.int64)
+        t = w - b
+        (wx, dx), (wy, dy), (wz, dz) = (_keys(t[:, k]) for k in range(3))
+        val = np.zeros(len(p))
+        grad = np.zeros((len(p), 3))
+        taps = []
+        for k in range(4):
+            for j in range(4):
+                for i in range(4):
+                    idx = (b[:, 2] + k - 1, b[:, 1] + j - 1, b[:, 0] + i - 1)
+                    s = g[idx].astype(np.float64)
+                    val += wx[:, i] * wy[:, j] * wz[:, k] * s
+                    grad[:, 0] += dx[:, i] * wy[:, j] * wz[:, k] * s
+                    grad[:, 1] += wx[:, i] * dy[:, j] * wz[:, k] * s
+                    grad[:, 2] += wx[:, i] * wy[:, j] * dz[:, k] * s
+                    taps.append((idx, wx[:, i] * wy[:, j] * wz[:, k]))
+        fv = f[p[:, 2], p[:, 1], p[:, 0]].astype(np.float64)
+        fc, gc = fv - fv.mean(), val - val.mean()
+        bf, bg = np.sqrt(np.sum(fc * fc)), np.sqrt(np.sum(gc * gc))
+        z = np.sum(fc * gc) / (bf * bg)
+        dg = -2.0 * grad_C[n] * (fc / bf - z * gc / bg) / bg  # dL/dg at each sample
+        for idx, wt in taps:
+            np.add.at(gg, idx, dg * wt)
+        q = (p - np.asarray(origin)) / np.asarray(spacing)
+        cc = np.clip(np.floor(q).astype(np.int64), 0, np.array([gnx, gny, gnz]) - 2)
+        tt = q - cc
+        for k in (0, 1):
+            for j in (0, 1):
+                for i in (0, 1):
+                    N = (tt[:, 0] if i else 1 - tt[:, 0]) * (tt[:, 1] if j else 1 - tt[:, 1]) * (tt[:, 2] if k else 1 - tt[:, 2])
+                    np.add.at(gu, (cc[:, 2] + k, cc[:, 1] + j, cc[:, 0] + i), (N * dg)[:, None] * grad)
+    return gu, gg
+
+
+def _backward(p, grad_C):
+    nu = _t(p["node_u"]).requires_grad_(True)
+    g = _t(p["g"]).requires_grad_(True)
+    C = W.warp_residual(nu, g, _t(p["f"]), _t(p["centers"], torch.int64), HALF, p["origin"], p["spacing"], 1)
+    (C * _t(grad_C, torch.float64)).nansum().backward()
+    return nu.grad.cpu().numpy(), g.grad.cpu().numpy()
+
+
+def test_backward_is_bit_identical_run_to_run(pair):
+    rng = np.random.default_rng(5)
+    wts = rng.uniform(0.5, 1.5, len(pair["centers"]))
+    a = _backward(pair, wts)
+    b = _backward(pair, wts)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_backward_matches_a_dense_float64_scatter(pair):
+    """Every node and every voxel of both gradients against the float64 per-subset scatter, on eight
+    overlapping subsets (the others get no upstream gradient)."""
+    p = pair
+    grad_C = np.zeros(len(p["centers"]))
+    pick = [0, 1, 5, 6, 25, 26, 30, 31]  # neighbours along x, y and z: their windows overlap
+    grad_C[pick] = np.random.default_rng(6).uniform(0.5, 1.5, len(pick))
+    gu, gg = _backward(p, grad_C)
+    ru, rg = backward_ref(p["f"], p["g"], p["centers"], HALF, p["node_u"], p["origin"], p["spacing"], grad_C)
+    assert np.max(np.abs(gu - ru)) <= 2e-4 * np.max(np.abs(ru))
+    assert np.max(np.abs(gg - rg)) <= 2e-4 * np.max(np.abs(rg))
+    assert np.array_equal(gg != 0, rg != 0)  # the same voxels receive a gradient
