@@ -706,44 +706,38 @@ __device__ inline int plan_chunk(const float* sw, const Subset& s, const Domain&
 // Stage planes [0, ez) of a brick of `rows` rows of BRK floats (row stride BRK, plane stride
 // `bplane` floats) whose origin is the anchor-relative voxel (bx, by, bz): 16-byte buffer loads of
 // the deformed volume (out-of-range offsets read zero; brick voxels outside the volume are never
-// read by an in-domain sample), eight loads in flight per lane, 4-byte LDS stores (the plane
-// stride need not keep rows 16-byte aligned).
+// read by an in-domain sample), 4-byte LDS stores (the plane stride need not keep rows 16-byte
+// aligned). A thread owns fixed (row, 16-byte group) positions and walks them through the planes,
+// so a load's address is one add from the previous plane's, with up to eight loads in flight.
 template <int BRK>
 __device__ inline void stage_brick(float* brick, int bplane, int cols, int rows, const BufVol& gb, int anchor,
                                    int rowp, int planep, int bx, int by, int bz, int ez) {
     static_assert(BRK % 4 == 0, "brick rows are whole 16-byte groups");
     const int Q = (cols + 3) >> 2;  // 16-byte groups per row: the columns the chunk's stencils reach
     const int RQ = rows * Q;        // 16-byte groups per plane
-    const int total = RQ * ez;
     const int org = anchor + bz * planep + by * rowp + bx;
-    // rem / Q as a float product: rem < 40 rows x 10 groups, so (rem + 0.5) / Q is at least 0.05
-    // from an integer and the float rounding (< 1e-4) cannot change the floor
+    // p / Q as a float product: p < 40 rows x 10 groups, so (p + 0.5) / Q is at least 0.05 from an
+    // integer and the float rounding (< 1e-4) cannot change the floor
     const float inv_q = 1.0f / (float)Q;
-    constexpr int U = 8;
-    for (int base = threadIdx.x; base < total; base += U * BLOCK) {
-        float4 r[U];
-        int dst[U];
+    constexpr int U = 4;
+    for (int p = threadIdx.x; p < RQ; p += BLOCK) {
+        const int yy = (int)(((float)p + 0.5f) * inv_q), qx = p - yy * Q;
+        const int src = org + yy * rowp + 4 * qx;
+        float* dst = brick + yy * BRK + 4 * qx;
+        for (int z0 = 0; z0 < ez; z0 += U) {
+            float4 r[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int i = base + u * BLOCK;
-            if (i < total) {
-                int zz = 0;
-                for (int t = RQ; t <= i; t += RQ) ++zz;  // ez is a handful of planes
-                const int rem = i - zz * RQ, yy = (int)(((float)rem + 0.5f) * inv_q), qx = rem - yy * Q;
-                r[u] = buf_load4(gb, 4u * (uint32_t)(org + zz * planep + yy * rowp + 4 * qx), 0);
-                dst[u] = zz * bplane + yy * BRK + 4 * qx;
-            }
-        }
+            for (int u = 0; u < U; ++u)
+                if (z0 + u < ez) r[u] = buf_load4(gb, 4u * (uint32_t)(src + (z0 + u) * planep), 0);
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int i = base + u * BLOCK;
-            if (i < total) {
-                float* d = brick + dst[u];
-                d[0] = r[u].x;
-                d[1] = r[u].y;
-                d[2] = r[u].z;
-                d[3] = r[u].w;
-            }
+            for (int u = 0; u < U; ++u)
+                if (z0 + u < ez) {
+                    float* d = dst + (z0 + u) * bplane;
+                    d[0] = r[u].x;
+                    d[1] = r[u].y;
+                    d[2] = r[u].z;
+                    d[3] = r[u].w;
+                }
         }
     }
 }
@@ -800,6 +794,18 @@ __global__ __launch_bounds__(BLOCK, 3) void k_icgn12_fused(IcgnArgs a, const int
         if (!isfinite(pf)) pf = 0.0;
         sw[12] = (float)pf;
         pg_next = pf;
+    }
+    __syncthreads();
+    // without noise correction the node's Cholesky factor is read every iteration by the control
+    // lane: a copy in LDS (the corrected factor's slot, unused then) replaces its global loads
+    const double* Lctl = nullptr;
+    if constexpr (!T3) {
+        if (!(a.noise_gain > 0.0 && a.H_all && a.pattern)) {
+            for (int i = tid; i < 144; i += BLOCK) c.L0[i] = a.L_all[144 * n + i];
+            Lctl = c.L0;
+        } else {
+            Lctl = a.L_all + 144 * n;
+        }
     }
     __syncthreads();
     const double* mn = mom + n * MOM_STRIDE;
@@ -869,8 +875,6 @@ __global__ __launch_bounds__(BLOCK, 3) void k_icgn12_fused(IcgnArgs a, const int
             const bool ok = val == val;
             const float fv = fraw - pf, gv = ok ? val - pg : 0.0f;
             const float q[4] = {X, Y, Z, 1.0f};
-            m[0] += ok ? 1.0f : 0.0f;
-            m[2] += ok ? 0.0f : 1.0f;
             m[3] += gv;
             m[4] += gv * gv;
             m[5] += fv * gv;
@@ -885,7 +889,10 @@ __global__ __launch_bounds__(BLOCK, 3) void k_icgn12_fused(IcgnArgs a, const int
                     for (int j = 0; j < 4; ++j) m[6 + 4 * cc + j] += tg * q[j];
                 }
             }
-            if (!ok) atomicOr(&nanbits[w.v >> 5], 1u << (w.v & 31));
+            if (!ok) {  // rare: the valid count is the kept count less these
+                m[2] += 1.0f;
+                atomicOr(&nanbits[w.v >> 5], 1u << (w.v & 31));
+            }
         };
         using Gather = std::integral_constant<bool, false>;
         if constexpr (BRK > 0) {
@@ -972,14 +979,19 @@ __global__ __launch_bounds__(BLOCK, 3) void k_icgn12_fused(IcgnArgs a, const int
                                     }
                                 }
                                 const int tx = kx - 8 * pxn, wxp = 8 * pxn, per_plane = tx * ky + (ky - 8 * pyn) * wxp;
+                                // integer quotients as float products: (n + 0.5) / d lies at least 0.5 / d
+                                // from an integer, and for n below 2^22 its float rounding (under
+                                // n 2^-23 / d) stays inside that margin, so the floor is exact
+                                const float inv_pp = 1.0f / (float)max(per_plane, 1), inv_tx = 1.0f / (float)max(tx, 1),
+                                            inv_wx = 1.0f / (float)max(wxp, 1);
                                 for (int t = tid; t < kz * per_plane; t += BLOCK) {
-                                    const int pz = t / per_plane, r = t - pz * per_plane;
+                                    const int pz = (int)(((float)t + 0.5f) * inv_pp), r = t - pz * per_plane;
                                     int x, y;
                                     if (r < tx * ky) {
-                                        y = r / tx;
+                                        y = (int)(((float)r + 0.5f) * inv_tx);
                                         x = wxp + (r - y * tx);
                                     } else {
-                                        const int r2 = r - tx * ky, yy = r2 / wxp;
+                                        const int r2 = r - tx * ky, yy = (int)(((float)r2 + 0.5f) * inv_wx);
                                         y = 8 * pyn + yy;
                                         x = r2 - yy * wxp;
                                     }
@@ -1086,7 +1098,7 @@ __global__ __launch_bounds__(BLOCK, 3) void k_icgn12_fused(IcgnArgs a, const int
                 exc, red);
         }
         if (tid == 0) {
-            const double n_valid = res[0], n_ref = mn[0];
+            const double n_ref = mn[0], n_valid = n_ref - res[2];  // every kept voxel was sampled once
             int st = -1;
             if (res[1] > 0.0) {
                 st = ST_OUT_OF_BOUNDS;
@@ -1119,7 +1131,7 @@ __global__ __launch_bounds__(BLOCK, 3) void k_icgn12_fused(IcgnArgs a, const int
                         c.b[j < 3 ? 3 * cc + j : 9 + cc] = (A - mf * C) - ratio * (B - mg * C);
                     }
                     const double* Hn = a.H_all ? a.H_all + 144 * n : nullptr;
-                    ctrl = ctl12_step(c, zncc, bf, n_valid, a.L_all + 144 * n, Hn, a.pattern, a.noise_gain, a.tol,
+                    ctrl = ctl12_step(c, zncc, bf, n_valid, Lctl, Hn, a.pattern, a.noise_gain, a.tol,
                                       a.dp_tol, a.patience, a.predictive, half_scale);
                 }
                 pg_next = (double)pg + mg;
