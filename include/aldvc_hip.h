@@ -299,7 +299,7 @@ size_t aldvc_admm_global_workspace_bytes(int nx, int ny, int nz);
  * sync_every = 0: all max_iter CG slots are enqueued (graph capture); > 0: the host reads the stop
  * flag every sync_every iterations and enqueues no more after convergence (eager callers).
  * capped (device int, nullable) is set to 1 when the solve used all max_iter iterations, so a
- * graph recorded with a smaller iteration budget can be checked after replay. */
+ * graph recorded with a smaller iteration budget can be checked after replay.
  * Replaces solver/subpb2_solver.py:108-155 (global_rhs, solve_subpb2) followed by
  * solver/global_operators.py:147-170 (nodal_gradient). */
 int aldvc_admm_global_step(const double *mats, const double *Ae, int nx, int ny, int nz, const uint8_t *elem_active,
@@ -307,6 +307,49 @@ int aldvc_admm_global_step(const double *mats, const double *Ae, int nx, int ny,
                            const double *F, const double *W, const double *v, double tol, int max_iter, double *U2,
                            double *F2, int *iterations_dev, const int *gate, int sync_every, int *capped,
                            void *workspace, size_t workspace_bytes, hipStream_t stream);
+
+/* ---- The finite-difference discretisation of the global step (subpb2_method "fd"). Node grid
+ * (nx, ny, nz) with spacing (hx, hy, hz); the operators are described by one byte per node, its
+ * code (aldvc_fd_codes), and applied matrix-free: D_j (central differences, one-sided where one
+ * neighbour is missing, zero rows at invalid nodes), Kg = sum_j D_j^T D_j, M = diag(valid),
+ * G_j = D_j^T; the active nodes are the valid ones. Grids of 1 node along an axis are allowed. */
+
+/* codes (N) from node_valid (N, 0/1) and edge_ok ((N, 3) 0/1, the +x, +y, +z edge of every node of
+ * a cut mesh, or NULL). Replaces solver/global_operators.py:74-137 (_difference_operator,
+ * build_fd_operators). */
+int aldvc_fd_codes(int nx, int ny, int nz, const uint8_t *node_valid, const uint8_t *edge_ok, uint8_t *codes,
+                   hipStream_t stream);
+
+/* y = cK Kg x + cM M x on the valid nodes, 0 elsewhere; x, y (N, 3). The operator of
+ * build_global_system (solver/subpb2_solver.py:45-74) with cK = beta + alpha, cM = mu. */
+int aldvc_fd_apply(const uint8_t *codes, int nx, int ny, int nz, double hx, double hy, double hz, double cK,
+                   double cM, const double *x, double *y, hipStream_t stream);
+
+/* b = mu M (U - v) + beta sum_j G_j (F - W)[:, :, j] + alpha Kg U (N, 3), zero rows at invalid
+ * nodes; U, v (N, 3), F, W (N, 3, 3). Replaces solver/subpb2_solver.py:108-127 (global_rhs). */
+int aldvc_fd_rhs(const uint8_t *codes, int nx, int ny, int nz, double hx, double hy, double hz, double beta,
+                 double mu, double alpha, const double *U, const double *F, const double *W, const double *v,
+                 double *b, hipStream_t stream);
+
+/* F[a, i, j] = (D_j U[:, i])(a) (N, 3, 3), NaN at invalid nodes. Replaces the FD branch of
+ * solver/global_operators.py:151-170 (nodal_gradient). */
+int aldvc_fd_nodal_gradient(const uint8_t *codes, int nx, int ny, int nz, double hx, double hy, double hz,
+                            const double *U, double *F, hipStream_t stream);
+
+/* Jacobi-preconditioned CG for (cK Kg + cM M) X = B on the valid nodes, as aldvc_pcg_fem (same
+ * workspace, aldvc_pcg_workspace_bytes). Replaces solver/subpb2_solver.py:77-105 (pcg_multi). */
+int aldvc_pcg_fd(const uint8_t *codes, int nx, int ny, int nz, double hx, double hy, double hz, double cK, double cM,
+                 const double *B, double *X, double tol, int max_iter, int *iterations, void *workspace,
+                 size_t workspace_bytes, hipStream_t stream);
+
+/* aldvc_admm_global_step with the FD operators (same arguments otherwise, same workspace,
+ * aldvc_admm_global_workspace_bytes). Replaces solver/subpb2_solver.py:108-155 followed by
+ * solver/global_operators.py:151-170 for subpb2_method "fd" (core/pipeline.py:216, :305). */
+int aldvc_admm_global_step_fd(const uint8_t *codes, int nx, int ny, int nz, double hx, double hy, double hz,
+                              double beta, double mu, double alpha, const double *U, const double *F,
+                              const double *W, const double *v, double tol, int max_iter, double *U2, double *F2,
+                              int *iterations_dev, const int *gate, int sync_every, int *capped, void *workspace,
+                              size_t workspace_bytes, hipStream_t stream);
 
 /* Workspace bytes of aldvc_subpb1_cleanup. */
 size_t aldvc_subpb1_cleanup_workspace_bytes(int nx, int ny, int nz);

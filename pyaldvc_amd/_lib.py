@@ -141,6 +141,15 @@ SIGNATURES = {
         I32,
         [P, P, I32, I32, I32, P, P, F64, F64, F64, P, P, P, P, F64, I32, P, P, P, P, I32, P, P, SZ, P],
     ),
+    "aldvc_fd_codes": (I32, [I32, I32, I32, P, P, P, P]),
+    "aldvc_fd_apply": (I32, [P, I32, I32, I32, F64, F64, F64, F64, F64, P, P, P]),
+    "aldvc_fd_rhs": (I32, [P, I32, I32, I32, F64, F64, F64, F64, F64, F64, P, P, P, P, P, P]),
+    "aldvc_fd_nodal_gradient": (I32, [P, I32, I32, I32, F64, F64, F64, P, P, P]),
+    "aldvc_pcg_fd": (I32, [P, I32, I32, I32, F64, F64, F64, F64, F64, P, P, F64, I32, ctypes.POINTER(I32), P, SZ, P]),
+    "aldvc_admm_global_step_fd": (
+        I32,
+        [P, I32, I32, I32, F64, F64, F64, F64, F64, F64, P, P, P, P, F64, I32, P, P, P, P, I32, P, P, SZ, P],
+    ),
     "aldvc_subpb1_cleanup_workspace_bytes": (SZ, [I32, I32, I32]),
     "aldvc_subpb1_cleanup": (I32, [I32, I32, I32, P, P, P, P, F64, F64, F64, I32, P, P, P, I32, P, P, P, SZ, P]),
     "aldvc_spring_fill_workspace_bytes": (SZ, [I32, I32, I32]),

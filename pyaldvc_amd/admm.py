@@ -8,8 +8,9 @@ One iteration (steps 2..``admm_max_iter`` of the reference loop) is:
 2. its cleanup: bad = not converged, or flagged by the universal median test (unknown nodes
    spring-filled first); bad nodes keep ``U2`` (``aldvc_subpb1_cleanup``,
    solver/subpb1_solver.py:140-164);
-3. subproblem 2: the FEM right-hand side, the float64 PCG solve and the nodal gradient
-   (``aldvc_admm_global_step``, solver/subpb2_solver.py:108-155, global_operators.py:147-170);
+3. subproblem 2: the right-hand side, the float64 PCG solve and the nodal gradient with the FEM or
+   the FD operators (``aldvc_admm_global_step`` / ``aldvc_admm_global_step_fd``,
+   solver/subpb2_solver.py:108-155, global_operators.py:147-170);
 4. the dual update, the RMS updates and the stopping rule (``aldvc_dual_update``).
 
 The first pass after the local 12-DOF solve (the global step from ``W = 0``, ``v = 0``) is the same
@@ -204,6 +205,18 @@ class AdmmLoop:
     def _global(self, step):
         s, ops = self.system, self.system.ops
         nz, ny, nx = self.grid
+        if ops.method == "fd":
+            check(
+                lib().aldvc_admm_global_step_fd(
+                    K._ptr(ops.codes), nx, ny, nz, *ops.spacing, s.beta, s.mu, s.alpha, K._ptr(self.U1n),
+                    K._ptr(self.F2), K._ptr(self.W), K._ptr(self.v), self.pcg_tol, int(self._limits[0]),
+                    K._ptr(self.U2n), K._ptr(self.F2n), K._ptr(self.pcg_iters[step:]), K._ptr(self.stop), self._sync,
+                    K._ptr(self.capped) if self._limits[0] < self.pcg_max else None, K._ptr(self.ws_global),
+                    int(self.ws_global.numel()), K.stream_handle(),
+                ),
+                "aldvc_admm_global_step_fd",
+            )  # fmt: skip
+            return
         check(
             lib().aldvc_admm_global_step(
                 K._ptr(ops.mats), K._ptr(s.Ae), nx, ny, nz, K._ptr(ops.elem_active), K._ptr(ops.node_active),

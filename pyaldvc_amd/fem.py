@@ -1,10 +1,12 @@
-"""ADMM subproblem 2 on the GPU: the hex8 FEM operators of the node grid, the right-hand side,
-the float64 Jacobi-PCG solve and the nodal gradient (solver/global_operators.py,
-solver/subpb2_solver.py, mesh/hex8.py).
+"""ADMM subproblem 2 on the GPU: the operators of the node grid (hex8 FEM or finite
+differences), the right-hand side, the float64 Jacobi-PCG solve and the nodal gradient
+(solver/global_operators.py, solver/subpb2_solver.py, mesh/hex8.py).
 
-The grid is uniform, so every element is the same box and an operator is one 8x8 element matrix
-applied matrix-free by the HIP library (``csrc/aldvc_global.hip``) over the active elements.
-The API mirrors the reference: ``build_global_operators(mesh, "fem", order)``,
+Both discretisations are applied matrix-free by the HIP library (``csrc/aldvc_global.hip``). The
+grid is uniform, so for FEM every element is the same box and an operator is one 8x8 element matrix
+summed over the active elements; for FD (``subpb2_method="fd"``) a node's rows are fixed by one
+byte of neighbour flags (valid neighbours, cut edges) and the spacing. The API mirrors the
+reference: ``build_global_operators(mesh, method, order)``,
 ``build_global_system(ops, beta, mu, alpha, para)``, ``global_rhs(...)``,
 ``solve_subpb2(system, U1, F1, W, v)`` and ``nodal_gradient(ops, U)``, host arrays in and out.
 """
@@ -87,26 +89,66 @@ def hex8_elements(grid_shape, node_valid=None) -> np.ndarray:
 
 @dataclass
 class GlobalOperators:
-    """The FEM operators of a node grid: element matrices and activity on the GPU."""
+    """The operators of a node grid on the GPU: for FEM the element matrices and the element and
+    node activity, for FD the per-node codes and the spacing."""
 
-    mats: torch.Tensor  # (5, 8, 8) float64: K, M, G_x, G_y, G_z
-    elem_active: torch.Tensor  # (nz-1, ny-1, nx-1) uint8
+    mats: torch.Tensor  # FEM: (5, 8, 8) float64: K, M, G_x, G_y, G_z
+    elem_active: torch.Tensor  # FEM: (nz-1, ny-1, nx-1) uint8
     node_active: torch.Tensor  # (nz, ny, nx) uint8
     grid_shape: tuple
     active: np.ndarray  # (N,) bool, nodes that carry equations
     method: str
     n_nodes: int
+    codes: torch.Tensor = None  # FD: (nz, ny, nx) uint8 neighbour flags (aldvc_fd_codes)
+    spacing: tuple = None  # FD: (hx, hy, hz)
 
     @property
     def n_active(self) -> int:
         return int(self.active.sum())
 
 
+def _fd_edges(mesh, n: int):
+    """The +x, +y, +z edge flags of a cut mesh, or ``None`` when nothing was cut (the reference
+    takes them only from a cut mesh, global_operators.py:126)."""
+    e = getattr(mesh, "edge_ok", None)
+    if e is None:
+        return None
+    e = np.asarray(e, dtype=bool)
+    return e.reshape(n, 3) if e.size == 3 * n and not e.all() else None
+
+
+def build_fd_operators(mesh) -> GlobalOperators:
+    """Finite-difference operators of the node grid (global_operators.py:74-137): ``mesh`` gives
+    ``grid_shape``, ``spacing`` (x, y, z), ``node_valid`` (all valid when it does not hold one flag
+    per node) and, for a cut mesh, ``edge_ok`` (N, 3)."""
+    nz, ny, nx = (int(v) for v in mesh.grid_shape)
+    n = nz * ny * nx
+    nv = getattr(mesh, "node_valid", None)
+    valid = np.asarray(nv, dtype=bool).ravel() if nv is not None and np.size(nv) == n else np.ones(n, dtype=bool)
+    edges = _fd_edges(mesh, n)
+    dev = K.device()
+    valid_d = torch.from_numpy(valid.astype(np.uint8)).to(dev)
+    edges_d = None if edges is None else torch.from_numpy(np.ascontiguousarray(edges.astype(np.uint8))).to(dev)
+    codes = torch.empty(n, dtype=torch.uint8, device=dev)
+    check(lib().aldvc_fd_codes(nx, ny, nz, K._ptr(valid_d), K._ptr(edges_d), K._ptr(codes), K.stream_handle()),
+          "aldvc_fd_codes")  # fmt: skip
+    spacing = tuple(float(h) for h in mesh.spacing)
+    if min(spacing) <= 0.0:
+        raise ValueError("the grid spacing must be positive")
+    return GlobalOperators(
+        mats=None, elem_active=None, node_active=valid_d.reshape(nz, ny, nx), grid_shape=(nz, ny, nx),
+        active=valid.copy(), method="fd", n_nodes=n, codes=codes.reshape(nz, ny, nx), spacing=spacing,
+    )  # fmt: skip
+
+
 def build_global_operators(mesh, method: str = "fem", gauss_pt_order: int = 2) -> GlobalOperators:
-    """FEM operators over the active elements of ``mesh`` (global_operators.py:58-70, :140-144):
-    ``mesh.elements`` (E, 8) with rows of -1 for dropped elements, in the grid's element order."""
+    """The operators of ``subpb2_method`` ``method`` (global_operators.py:140-144): FEM over the
+    active elements of ``mesh`` (``mesh.elements`` (E, 8) with rows of -1 for dropped elements, in
+    the grid's element order; :58-70), or finite differences (``build_fd_operators``)."""
+    if method == "fd":
+        return build_fd_operators(mesh)
     if method != "fem":
-        raise NotImplementedError("the GPU global step implements the hex8 FEM discretisation (subpb2_method='fem')")
+        raise ValueError(f"unknown subpb2 method {method!r}")
     nz, ny, nx = (int(v) for v in mesh.grid_shape)
     elements = np.asarray(mesh.elements)
     if elements.shape != ((nz - 1) * (ny - 1) * (nx - 1), 8):
@@ -136,7 +178,7 @@ class GlobalSystem:
     beta: float
     mu: float
     alpha: float
-    Ae: torch.Tensor  # (8, 8) element matrix of (beta + alpha) K + mu M
+    Ae: torch.Tensor  # FEM: (8, 8) element matrix of (beta + alpha) K + mu M; None for FD
     solver: str
     pcg_tol: float
     pcg_max_iter: int
@@ -145,7 +187,7 @@ class GlobalSystem:
 def build_global_system(ops: GlobalOperators, beta: float, mu: float, alpha: float, para) -> GlobalSystem:
     """``A = (beta + alpha) K + mu M`` on the active nodes (subpb2_solver.py:45-74). The
     reference's sparse LU for small systems ("direct") is conjugate gradients to round-off here."""
-    Ae = ((beta + alpha) * ops.mats[0] + mu * ops.mats[1]).contiguous()
+    Ae = None if ops.method == "fd" else ((beta + alpha) * ops.mats[0] + mu * ops.mats[1]).contiguous()
     solver = getattr(para, "global_solver", "auto")
     if solver == "auto":
         solver = "direct" if ops.n_active <= DIRECT_MAX_ACTIVE else "pcg"
@@ -161,8 +203,20 @@ def _dev(a, shape):
     return torch.from_numpy(np.ascontiguousarray(np.asarray(a, dtype=np.float64).reshape(shape))).to(K.device())
 
 
+def _fd(ops):
+    """The FD grid arguments of the C ABI: codes, nx, ny, nz, hx, hy, hz."""
+    nz, ny, nx = ops.grid_shape
+    return (K._ptr(ops.codes), nx, ny, nz, *ops.spacing)
+
+
 def _rhs_device(ops, beta, mu, alpha, U1, F1, W, v) -> torch.Tensor:
     n = ops.n_nodes
+    if ops.method == "fd":
+        args = [_dev(a, s) for a, s in ((U1, (n, 3)), (F1, (n, 3, 3)), (W, (n, 3, 3)), (v, (n, 3)))]
+        b = torch.empty((n, 3), dtype=torch.float64, device=args[0].device)
+        check(lib().aldvc_fd_rhs(*_fd(ops), float(beta), float(mu), float(alpha), *(K._ptr(a) for a in args), K._ptr(b),
+                                 K.stream_handle()), "aldvc_fd_rhs")  # fmt: skip
+        return b
     FmW = np.asarray(F1, dtype=np.float64).reshape(n, 3, 3) - np.asarray(W, dtype=np.float64).reshape(n, 3, 3)
     FmW_j = _dev(np.transpose(FmW, (2, 0, 1)), (3, n, 3))  # [j][node][i] = (F - W)[node, i, j]
     Umv = _dev(np.asarray(U1, dtype=np.float64).reshape(n, 3) - np.asarray(v, dtype=np.float64).reshape(n, 3), (n, 3))
@@ -184,12 +238,17 @@ def global_rhs(ops, beta, mu, alpha, U1, F1, W, v) -> np.ndarray:
 
 def pcg_fem(system: GlobalSystem, B: torch.Tensor, X0: torch.Tensor, tol: float, max_iter: int):
     """Jacobi-preconditioned CG on the active nodes for the three columns of ``B`` at once
-    (subpb2_solver.py:77-105): ``(X, iterations)``."""
+    (subpb2_solver.py:77-105), for either discretisation: ``(X, iterations)``."""
     ops = system.ops
     nz, ny, nx = ops.grid_shape
     X = X0.clone()
     it = ctypes.c_int(0)
     ws = torch.empty(int(lib().aldvc_pcg_workspace_bytes(nx, ny, nz)), dtype=torch.uint8, device=X.device)
+    if ops.method == "fd":
+        check(lib().aldvc_pcg_fd(*_fd(ops), system.beta + system.alpha, system.mu, K._ptr(B), K._ptr(X), float(tol),
+                                 int(max_iter), ctypes.byref(it), K._ptr(ws), int(ws.numel()), K.stream_handle()),
+              "aldvc_pcg_fd")  # fmt: skip
+        return X, int(it.value)
     check(
         lib().aldvc_pcg_fem(K._ptr(system.Ae), *_grid(ops), K._ptr(B), K._ptr(X), float(tol), int(max_iter),
                             ctypes.byref(it), K._ptr(ws), int(ws.numel()), K.stream_handle()),
@@ -218,11 +277,14 @@ def solve_subpb2(system: GlobalSystem, U1, F1, W, v):
 
 
 def nodal_gradient(ops: GlobalOperators, U) -> np.ndarray:
-    """``(N, 3, 3)`` lumped-L2 gradient ``F_ij = (G_j^T u_i) / m_L``, NaN at inactive nodes
-    (global_operators.py:147-170)."""
+    """``(N, 3, 3)`` displacement gradient, NaN at inactive nodes (global_operators.py:151-170):
+    FEM the lumped-L2 gradient ``F_ij = (G_j^T u_i) / m_L``, FD ``F_ij = D_j u_i``."""
     n = ops.n_nodes
     Ud = _dev(U, (n, 3))
     F = torch.empty((n, 3, 3), dtype=torch.float64, device=Ud.device)
+    if ops.method == "fd":
+        check(lib().aldvc_fd_nodal_gradient(*_fd(ops), K._ptr(Ud), K._ptr(F), K.stream_handle()), "aldvc_fd_nodal_gradient")
+        return F.cpu().numpy()
     check(lib().aldvc_fem_nodal_gradient(K._ptr(ops.mats), *_grid(ops), K._ptr(Ud), K._ptr(F), K.stream_handle()),
           "aldvc_fem_nodal_gradient")  # fmt: skip
     return F.cpu().numpy()

@@ -37,6 +37,18 @@ def test_header_declares_the_binding_table():
     assert declared() == sorted(_lib.SIGNATURES)
 
 
+@pytest.mark.parametrize("lang,compiler", [("c", "gcc"), ("c++", "g++")])
+def test_header_compiles_as_c_and_cxx(lang, compiler):
+    import shutil
+    import subprocess
+
+    if shutil.which(compiler) is None or not os.path.isdir("/opt/rocm/include"):
+        pytest.skip(f"{compiler} or the ROCm headers are missing")
+    res = subprocess.run([compiler, "-fsyntax-only", "-Wall", "-Werror", "-x", lang, "-D__HIP_PLATFORM_AMD__",
+                          "-I/opt/rocm/include", HEADER], capture_output=True, text=True)  # fmt: skip
+    assert res.returncode == 0, res.stderr[-2000:]
+
+
 def test_library_exports_every_declared_symbol(lib):
     for name in declared():
         assert hasattr(lib, name), name

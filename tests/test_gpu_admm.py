@@ -236,12 +236,15 @@ def _host_loop(ctx, ref, g, mesh, system, para, U1, F1):
     return dict(U=U2, F=F2, n=n_steps, upd_g=upd_g, upd_l=upd_l, res_u=res_u, res_f=res_f, status=status, bad=bad)
 
 
-@pytest.fixture(scope="module", params=["drv", "drvm"])
+@pytest.fixture(scope="module", params=["drv", "drvm", "drvm-fd"])
 def frame(gd, request):
     """A golden driver pair (40x44x48): local 12-DOF solve, then the global system. ``drv``: no mask;
     ``drvm``: a cylinder mask with voids and a crack, split subsets and a cut mesh (the elements that
-    bridge the mask dropped, cut edges excluded from the median test and the fills)."""
-    p = request.param
+    bridge the mask dropped, cut edges excluded from the median test and the fills); ``drvm-fd``:
+    the same frame with the finite-difference global step (valid nodes, one-sided differences at
+    the cut edges)."""
+    p, _, method = request.param.partition("-")
+    method = method or "fem"
     coords = gd[f"{p}_coords"]
     grid = tuple(int(v) for v in gd[f"{p}_grid_shape"])
     masked = p == "drvm"
@@ -266,10 +269,11 @@ def frame(gd, request):
     spacing = tuple(float(np.diff(np.unique(coords[:, k]))[0]) for k in range(3))
     el = np.array(elements, copy=True)
     el[(el < 0).any(1) | ~mesh.node_valid[np.maximum(el, 0)].all(1)] = -1
-    ops = FEM.build_global_operators(SimpleNamespace(grid_shape=grid, elements=el, spacing=spacing), "fem", 2)
+    gm = SimpleNamespace(grid_shape=grid, elements=el, spacing=spacing, node_valid=mesh.node_valid, edge_ok=mesh.edge_ok)
+    ops = FEM.build_global_operators(gm, method, 2)
     system = FEM.build_global_system(ops, 1e-3 * np.mean(spacing) ** 2, para.mu, 0.0, para)
     return SimpleNamespace(mesh=mesh, ref=ref, para=para, ctx=ctx, U1=U1, F1=F1, info=info, system=system, g=gd["drv_g"],
-                           name=p)  # fmt: skip
+                           name=request.param)  # fmt: skip
 
 
 def _stopping_tol(fr):
