@@ -391,6 +391,86 @@ int aldvc_dual_update(int64_t N, const double *U1n, const double *U2n, const dou
                       double *history, int *stop, int *steps, void *workspace, size_t workspace_bytes,
                       hipStream_t stream);
 
+/* ---- The whole ADMM loop of a frame in one call, and as one HIP graph (core/pipeline.py:286-366).
+ * AldvcAdmmLoop names every device buffer and setting of the loop; its fields are the arguments of
+ * the four entries above that one iteration calls. Sequence: U1n = U1; the first global pass
+ * (step 1: global step, dual update without the stopping rule); then for step = 2 .. max_steps:
+ * aldvc_icgn3_tuned (U_old = U2, F_fixed = F2, vdual = v, out U1raw; from step 3 on with flag 16,
+ * the reference moments of step 2), aldvc_subpb1_cleanup (U = U1raw, U_hat = U2, out U1n),
+ * the global step (U = U1n, F = F2, W, v, out U2n, F2n) and aldvc_dual_update with the stopping
+ * rule. Every call of step k writes row k of the per-step records; after the stop flag is set the
+ * later calls change nothing. */
+typedef struct AldvcAdmmLoop {
+    size_t struct_bytes; /* sizeof(AldvcAdmmLoop): checked, so a caller built against another layout is refused */
+    /* local 3-DOF step (aldvc_icgn3_tuned) */
+    const int64_t *coords;
+    int64_t n_nodes; /* N = nx * ny * nz of the node grid */
+    int hx, hy, hz, stride;
+    const float *f, *gx, *gy, *gz; /* gx..gz NULL: gradients on the fly */
+    const uint8_t *mask;
+    const float *g;
+    int64_t vol_nz, vol_ny, vol_nx;
+    int mode;
+    const double *H_all, *meanf, *bottomf;
+    const uint8_t *valid;
+    double icgn_tol, icgn_dp_tol;
+    int icgn_max_iter, icgn_patience;
+    double noise_gain;
+    int predictive;
+    const int64_t *split_index;
+    const uint8_t *split_keep;
+    int64_t keep_bytes;
+    int variant, lds_budget, flags;
+    /* node grid, cleanup (aldvc_subpb1_cleanup) */
+    int nx, ny, nz;
+    const uint8_t *edge_ok; /* (N, 3) or NULL */
+    double outlier_threshold, median_eps, fill_tol;
+    int fill_budget, fill_limit; /* CG iterations per fill; a budget below the limit sets *capped when used up */
+    /* global step: method 0 = FEM (mats, Ae, elem_active, node_active), 1 = FD (codes, spacing) */
+    int method;
+    const double *mats, *Ae;
+    const uint8_t *elem_active, *node_active;
+    const uint8_t *codes;
+    double spacing_x, spacing_y, spacing_z;
+    double beta, mu, alpha, pcg_tol;
+    int pcg_budget, pcg_limit;
+    /* dual update */
+    int accumulate;
+    double admm_tol;
+    int max_steps;
+    /* state: (N, 3) and (N, 9) float64 */
+    double *U1, *U2, *v, *U1raw, *U1n, *U2n;
+    double *F2, *W, *F2n;
+    /* per-step records, rows 0 .. max_steps: status, zncc, n_iter, bad (N per row); history (4 per
+     * row); pcg_iterations, fill_iterations (1 per row); stop, steps, capped (one int each) */
+    int8_t *status;
+    double *zncc;
+    int32_t *n_iter;
+    uint8_t *bad;
+    double *history;
+    int *pcg_iterations, *fill_iterations, *stop, *steps, *capped;
+    /* workspaces of the four entries (their *_workspace_bytes) */
+    void *ws_icgn3, *ws_cleanup, *ws_global, *ws_dual;
+    size_t ws_icgn3_bytes, ws_cleanup_bytes, ws_global_bytes, ws_dual_bytes;
+} AldvcAdmmLoop;
+
+/* Enqueue the whole loop on `stream`. sync_every > 0: the CG solves are polled by the host every
+ * sync_every iterations (eager callers; the call then waits on the stream); 0: nothing is read back
+ * and every CG slot of the budgets is enqueued, so the sequence can be captured. */
+int aldvc_admm_loop_enqueue(const AldvcAdmmLoop *loop, int sync_every, hipStream_t stream);
+
+/* Record aldvc_admm_loop_enqueue(loop, 0, .) into a HIP graph by capturing `capture_stream` (a
+ * stream the caller created, not the null stream, with no other work being captured) and
+ * instantiate it: *graph receives an executable graph for aldvc_graph_launch. A replay runs the
+ * loop on whatever the loop's buffers hold at launch time; reset stop, steps, capped, the records
+ * and the state (U1, U2, F2 = the local result, W = v = 0) before each. For callers without a
+ * graph API of their own (C, Fortran, Java over JNI). */
+int aldvc_admm_loop_graph(const AldvcAdmmLoop *loop, hipStream_t capture_stream, void **graph);
+
+/* Launch / free an executable graph made by aldvc_admm_loop_graph. */
+int aldvc_graph_launch(void *graph, hipStream_t stream);
+int aldvc_graph_destroy(void *graph);
+
 /* ---- Differentiable warp residual (no counterpart in the reference's backward; the forward is
  * solver/numba_kernels.py:1354-1387, evaluate_zncc_parallel, with the subset's affine warp replaced
  * by the trilinear FE interpolation of mesh/hex8.py:58-61). Node displacements node_u

@@ -27,6 +27,35 @@ I32 = ctypes.c_int
 F64 = ctypes.c_double
 SZ = ctypes.c_size_t
 
+_D, _I, _P, _Z, _L = ctypes.c_double, ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int64
+
+
+class AdmmLoopDesc(ctypes.Structure):
+    """``AldvcAdmmLoop`` of include/aldvc_hip.h, field for field (``struct_bytes`` lets the library
+    refuse a layout that differs from its own)."""
+
+    _fields_ = [
+        ("struct_bytes", _Z), ("coords", _P), ("n_nodes", _L), ("hx", _I), ("hy", _I), ("hz", _I), ("stride", _I),
+        ("f", _P), ("gx", _P), ("gy", _P), ("gz", _P), ("mask", _P), ("g", _P),
+        ("vol_nz", _L), ("vol_ny", _L), ("vol_nx", _L), ("mode", _I),
+        ("H_all", _P), ("meanf", _P), ("bottomf", _P), ("valid", _P),
+        ("icgn_tol", _D), ("icgn_dp_tol", _D), ("icgn_max_iter", _I), ("icgn_patience", _I), ("noise_gain", _D),
+        ("predictive", _I), ("split_index", _P), ("split_keep", _P), ("keep_bytes", _L),
+        ("variant", _I), ("lds_budget", _I), ("flags", _I),
+        ("nx", _I), ("ny", _I), ("nz", _I), ("edge_ok", _P),
+        ("outlier_threshold", _D), ("median_eps", _D), ("fill_tol", _D), ("fill_budget", _I), ("fill_limit", _I),
+        ("method", _I), ("mats", _P), ("Ae", _P), ("elem_active", _P), ("node_active", _P), ("codes", _P),
+        ("spacing_x", _D), ("spacing_y", _D), ("spacing_z", _D),
+        ("beta", _D), ("mu", _D), ("alpha", _D), ("pcg_tol", _D), ("pcg_budget", _I), ("pcg_limit", _I),
+        ("accumulate", _I), ("admm_tol", _D), ("max_steps", _I),
+        ("U1", _P), ("U2", _P), ("v", _P), ("U1raw", _P), ("U1n", _P), ("U2n", _P), ("F2", _P), ("W", _P), ("F2n", _P),
+        ("status", _P), ("zncc", _P), ("n_iter", _P), ("bad", _P), ("history", _P),
+        ("pcg_iterations", _P), ("fill_iterations", _P), ("stop", _P), ("steps", _P), ("capped", _P),
+        ("ws_icgn3", _P), ("ws_cleanup", _P), ("ws_global", _P), ("ws_dual", _P),
+        ("ws_icgn3_bytes", _Z), ("ws_cleanup_bytes", _Z), ("ws_global_bytes", _Z), ("ws_dual_bytes", _Z),
+    ]  # fmt: skip
+
+
 # name -> (restype, argtypes); mirrors include/aldvc_hip.h
 SIGNATURES = {
     "aldvc_abi_version": (I32, []),
@@ -150,6 +179,10 @@ SIGNATURES = {
         I32,
         [P, I32, I32, I32, F64, F64, F64, F64, F64, F64, P, P, P, P, F64, I32, P, P, P, P, I32, P, P, SZ, P],
     ),
+    "aldvc_admm_loop_enqueue": (I32, [ctypes.POINTER(AdmmLoopDesc), I32, P]),
+    "aldvc_admm_loop_graph": (I32, [ctypes.POINTER(AdmmLoopDesc), P, ctypes.POINTER(P)]),
+    "aldvc_graph_launch": (I32, [P, P]),
+    "aldvc_graph_destroy": (I32, [P]),
     "aldvc_subpb1_cleanup_workspace_bytes": (SZ, [I32, I32, I32]),
     "aldvc_subpb1_cleanup": (I32, [I32, I32, I32, P, P, P, P, F64, F64, F64, I32, P, P, P, I32, P, P, P, SZ, P]),
     "aldvc_spring_fill_workspace_bytes": (SZ, [I32, I32, I32]),

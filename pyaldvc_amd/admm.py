@@ -4,7 +4,7 @@ whole loop can be recorded into one HIP graph and replayed (core/pipeline.py:286
 One iteration (steps 2..``admm_max_iter`` of the reference loop) is:
 
 1. subproblem 1: translation IC-GN of every node with ``F`` held at ``F2`` and the proximal pull
-   towards ``U2 + v`` (``aldvc_icgn3_gated``);
+   towards ``U2 + v`` (``aldvc_icgn3_tuned``, gated by the stop flag);
 2. its cleanup: bad = not converged, or flagged by the universal median test (unknown nodes
    spring-filled first); bad nodes keep ``U2`` (``aldvc_subpb1_cleanup``,
    solver/subpb1_solver.py:140-164);
@@ -17,6 +17,9 @@ The first pass after the local 12-DOF solve (the global step from ``W = 0``, ``v
 sequence without 1-2. Every launch reads a device flag ``stop`` that the dual update sets when the
 reference loop would break (RMS update of ``U`` global or local below ``admm_tol``), so the
 iterations recorded after that point change nothing and the results equal the reference loop's.
+The whole sequence is one call of the C ABI (``aldvc_admm_loop_enqueue`` with the buffers in an
+``AldvcAdmmLoop`` descriptor), and the library records it into a graph itself
+(``aldvc_admm_loop_graph``), so callers other than Python get the same replayable loop.
 
 The spring fill and the median test are also exposed on their own (``spring_fill``,
 ``subpb1_cleanup``) with the reference's host signatures.
@@ -33,14 +36,13 @@ import torch
 
 from . import autotune as AT
 from . import kernels as K
-from ._lib import check, lib
+from ._lib import AdmmLoopDesc, check, lib
 from .local import INTERP_MODE_BY_NAME, LocalSolveInfo
 
 FILL_TOL = 1e-12  # relative residual of the spring fill's CG (the reference solves it directly up to 60k unknowns)
 FILL_MAX_ITER = 2000  # the reference's CG limit for the fill (inpaint.py:118)
 MEDIAN_EPS = 0.1  # noise level of the universal median test (outlier_detection.py:138)
 DIRECT_GRAPH_ITER = 2000  # PCG iterations recorded for the "direct" global solver (driven to round-off)
-REUSE_MOMENTS = 16  # aldvc_icgn3_tuned flag: the workspace holds the nodes' reference moments
 SYNC_EVERY = 8  # eager runs: the host reads a CG solve's stop flag every 8 iterations
 # A recorded graph enqueues a fixed number of CG slots per solve; idle slots after convergence still
 # cost a dispatch each, so a graph is recorded with a budget learned from the eager run (twice the
@@ -57,6 +59,26 @@ def _dev(a, dtype=torch.float64, shape=None):
     else:
         t = torch.from_numpy(np.ascontiguousarray(a)).to(device=K.device(), dtype=dtype)
     return (t if shape is None else t.reshape(shape)).contiguous()
+
+
+def _addr(t):
+    """Device address of a tensor for a descriptor field (None for a missing buffer)."""
+    return None if t is None else int(t.data_ptr())
+
+
+class _Graph:
+    """An executable HIP graph made by ``aldvc_admm_loop_graph``; freed with the object."""
+
+    def __init__(self, handle: int):
+        self.handle = handle
+
+    def launch(self):
+        check(lib().aldvc_graph_launch(ctypes.c_void_p(self.handle), K.stream_handle()), "aldvc_graph_launch")
+
+    def __del__(self):
+        if self.handle:
+            lib().aldvc_graph_destroy(ctypes.c_void_p(self.handle))
+            self.handle = None
 
 
 def _ws(nbytes: int) -> torch.Tensor:
@@ -135,8 +157,8 @@ class AdmmLoop:
     ``ctx`` (LocalContext), ``g`` (deformed volume), ``mesh`` (NodeMesh), ``system``
     (fem.GlobalSystem) and ``para`` as for the reference loop; ``U_local`` (N, 3) and ``F_local``
     (N, 3, 3) the output of the local 12-DOF step. ``run(graph=...)`` executes the first pass and
-    steps 2..``para.admm_max_iter``; ``graph=True`` records them once into a HIP graph (a
-    ``torch.cuda.CUDAGraph``) and replays it."""
+    steps 2..``para.admm_max_iter``; ``graph=True`` records them once into a HIP graph (made by
+    the library, ``aldvc_admm_loop_graph``) and replays it."""
 
     def __init__(self, ctx, g, mesh, system, para):
         self.ctx, self.mesh, self.system, self.para = ctx, mesh, system, para
@@ -202,89 +224,63 @@ class AdmmLoop:
         self.time_s = 0.0
 
     # -- launches (all on the current stream, no host synchronisation) -------------------------
-    def _global(self, step):
-        s, ops = self.system, self.system.ops
+    def _desc(self) -> AdmmLoopDesc:
+        """The loop's buffers and settings as the C ABI's ``AldvcAdmmLoop``, with the host polling
+        interval and the CG limits of the run being enqueued (``_sync``, ``_limits``)."""
+        ctx, p, d, vols, s = self.ctx, self.para, self.ctx.device, self.vols, self.system
+        ops = s.ops
         nz, ny, nx = self.grid
-        if ops.method == "fd":
-            check(
-                lib().aldvc_admm_global_step_fd(
-                    K._ptr(ops.codes), nx, ny, nz, *ops.spacing, s.beta, s.mu, s.alpha, K._ptr(self.U1n),
-                    K._ptr(self.F2), K._ptr(self.W), K._ptr(self.v), self.pcg_tol, int(self._limits[0]),
-                    K._ptr(self.U2n), K._ptr(self.F2n), K._ptr(self.pcg_iters[step:]), K._ptr(self.stop), self._sync,
-                    K._ptr(self.capped) if self._limits[0] < self.pcg_max else None, K._ptr(self.ws_global),
-                    int(self.ws_global.numel()), K.stream_handle(),
-                ),
-                "aldvc_admm_global_step_fd",
-            )  # fmt: skip
-            return
-        check(
-            lib().aldvc_admm_global_step(
-                K._ptr(ops.mats), K._ptr(s.Ae), nx, ny, nz, K._ptr(ops.elem_active), K._ptr(ops.node_active),
-                s.beta, s.mu, s.alpha, K._ptr(self.U1n), K._ptr(self.F2), K._ptr(self.W), K._ptr(self.v),
-                self.pcg_tol, int(self._limits[0]), K._ptr(self.U2n), K._ptr(self.F2n), K._ptr(self.pcg_iters[step:]),
-                K._ptr(self.stop), self._sync, K._ptr(self.capped) if self._limits[0] < self.pcg_max else None,
-                K._ptr(self.ws_global), int(self.ws_global.numel()), K.stream_handle(),
-            ),
-            "aldvc_admm_global_step",
-        )  # fmt: skip
-
-    def _dual(self, step, check_stop):
-        check(
-            lib().aldvc_dual_update(
-                self.N, K._ptr(self.U1n), K._ptr(self.U2n), K._ptr(self.F2n), K._ptr(self.U1), K._ptr(self.U2),
-                K._ptr(self.F2), K._ptr(self.W), K._ptr(self.v), int(self.para.dual_update == "accumulate"),
-                int(check_stop), float(self.para.admm_tol), int(step), K._ptr(self.hist), K._ptr(self.stop),
-                K._ptr(self.steps), K._ptr(self.ws_dual), int(self.ws_dual.numel()), K.stream_handle(),
-            ),
-            "aldvc_dual_update",
-        )  # fmt: skip
-
-    def _subpb1(self, step):
-        ctx, p, d, vols = self.ctx, self.para, self.ctx.device, self.vols
-        hx, hy, hz = ctx.half
-        nz, ny, nx = (int(v) for v in vols["f"].shape)
-        stored = vols["stored"]
-        gx, gy, gz = (vols[k] if stored else None for k in ("gx", "gy", "gz"))
         si, sk, kb = self.split
         t = self.tuning3 if self.tuning3 is not None else AT.DEFAULT
-        # the reference moments depend only on the frame's fixed inputs: the first 3-DOF pass
-        # (step 2) computes them into the workspace, later passes reuse them
-        reuse = REUSE_MOMENTS if step > 2 else 0
-        check(
-            lib().aldvc_icgn3_tuned(
-                K._ptr(d["coords"]), self.N, K._ptr(self.U2), K._ptr(self.U1raw), K._ptr(self.F2), K._ptr(self.v),
-                hx, hy, hz, int(ctx.stride), K._ptr(vols["f"]), K._ptr(gx), K._ptr(gy), K._ptr(gz),
-                K._ptr(vols["mask"]), K._ptr(self.gd), nz, ny, nx, self.mode, K._ptr(d["H"]), K._ptr(d["meanf"]),
-                K._ptr(d["bottomf"]), K._ptr(d["valid"]), float(self.system.mu), float(p.icgn_tol),
-                float(p.icgn_dp_tol), int(p.icgn_max_iter), int(p.icgn_patience),
-                float(self.gain) if self.gain > 0 else 0.0, int(bool(p.icgn_predictive_stop)), K._ptr(si), K._ptr(sk),
-                kb, K._ptr(self.n_iter[step]), K._ptr(self.status[step]), K._ptr(self.zncc[step]), K._ptr(self.stop),
-                int(t.variant), int(t.lds_budget), int(t.flags) | reuse, K._ptr(self.ws_icgn3), int(self.ws_icgn3.numel()),
-                K.stream_handle(),
-            ),
-            "aldvc_icgn3_tuned",
-        )  # fmt: skip
-        nz_, ny_, nx_ = self.grid
-        check(
-            lib().aldvc_subpb1_cleanup(
-                nx_, ny_, nz_, K._ptr(self.edge_ok), K._ptr(self.status[step]), K._ptr(self.U1raw), K._ptr(self.U2),
-                float(p.local_outlier_threshold), MEDIAN_EPS, FILL_TOL, int(self._limits[1]), K._ptr(self.U1n),
-                K._ptr(self.bad[step]), K._ptr(self.stop), self._sync, K._ptr(self.fill_iters[step:]),
-                K._ptr(self.capped) if self._limits[1] < FILL_MAX_ITER else None, K._ptr(self.ws_clean),
-                int(self.ws_clean.numel()), K.stream_handle(),
-            ),
-            "aldvc_subpb1_cleanup",
-        )  # fmt: skip
+        stored = vols["stored"]
+        D = AdmmLoopDesc()
+        D.struct_bytes = ctypes.sizeof(AdmmLoopDesc)
+        # local 3-DOF step (solver/subpb1_solver.py:77-133)
+        D.coords, D.n_nodes = _addr(d["coords"]), self.N
+        D.hx, D.hy, D.hz = ctx.half
+        D.stride = int(ctx.stride)
+        D.f, D.mask, D.g = _addr(vols["f"]), _addr(vols["mask"]), _addr(self.gd)
+        D.gx, D.gy, D.gz = (_addr(vols[k]) if stored else None for k in ("gx", "gy", "gz"))
+        D.vol_nz, D.vol_ny, D.vol_nx = (int(v) for v in vols["f"].shape)
+        D.mode = self.mode
+        D.H_all, D.meanf, D.bottomf, D.valid = (_addr(d[k]) for k in ("H", "meanf", "bottomf", "valid"))
+        D.icgn_tol, D.icgn_dp_tol = float(p.icgn_tol), float(p.icgn_dp_tol)
+        D.icgn_max_iter, D.icgn_patience = int(p.icgn_max_iter), int(p.icgn_patience)
+        D.noise_gain = float(self.gain) if self.gain > 0 else 0.0
+        D.predictive = int(bool(p.icgn_predictive_stop))
+        D.split_index, D.split_keep, D.keep_bytes = _addr(si), _addr(sk), int(kb)
+        D.variant, D.lds_budget, D.flags = int(t.variant), int(t.lds_budget), int(t.flags)
+        # cleanup (solver/subpb1_solver.py:140-164)
+        D.nx, D.ny, D.nz = nx, ny, nz
+        D.edge_ok = _addr(self.edge_ok)
+        D.outlier_threshold, D.median_eps, D.fill_tol = float(p.local_outlier_threshold), MEDIAN_EPS, FILL_TOL
+        D.fill_budget, D.fill_limit = int(self._limits[1]), FILL_MAX_ITER
+        # global step (solver/subpb2_solver.py:108-155)
+        if ops.method == "fd":
+            D.method, D.codes = 1, _addr(ops.codes)
+            D.spacing_x, D.spacing_y, D.spacing_z = ops.spacing
+        else:
+            D.method, D.mats, D.Ae = 0, _addr(ops.mats), _addr(s.Ae)
+            D.elem_active, D.node_active = _addr(ops.elem_active), _addr(ops.node_active)
+        D.beta, D.mu, D.alpha, D.pcg_tol = s.beta, s.mu, s.alpha, self.pcg_tol
+        D.pcg_budget, D.pcg_limit = int(self._limits[0]), self.pcg_max
+        # dual update and stopping rule (core/pipeline.py:300-358)
+        D.accumulate = int(p.dual_update == "accumulate")
+        D.admm_tol, D.max_steps = float(p.admm_tol), self.max_steps
+        for k in ("U1", "U2", "v", "U1raw", "U1n", "U2n", "F2", "W", "F2n", "status", "zncc", "n_iter", "bad", "stop",
+                  "steps", "capped"):  # fmt: skip
+            setattr(D, k, _addr(getattr(self, k)))
+        D.history, D.pcg_iterations, D.fill_iterations = _addr(self.hist), _addr(self.pcg_iters), _addr(self.fill_iters)
+        for k, ws in (("icgn3", self.ws_icgn3), ("cleanup", self.ws_clean), ("global", self.ws_global),
+                      ("dual", self.ws_dual)):  # fmt: skip
+            setattr(D, f"ws_{k}", _addr(ws))
+            setattr(D, f"ws_{k}_bytes", int(ws.numel()))
+        return D
 
     def _enqueue(self):
-        """The first pass and every later step, in stream order."""
-        self.U1n.copy_(self.U1)
-        self._global(1)
-        self._dual(1, False)
-        for step in range(2, self.max_steps + 1):
-            self._subpb1(step)
-            self._global(step)
-            self._dual(step, True)
+        """The first pass and every later step, in stream order (one call of the C ABI)."""
+        D = self._desc()
+        check(lib().aldvc_admm_loop_enqueue(ctypes.byref(D), self._sync, K.stream_handle()), "aldvc_admm_loop_enqueue")
 
     # -- public ----------------------------------------------------------------------------------
     def load(self, U_local, F_local, status_local=None, zncc_local=None):
@@ -339,7 +335,7 @@ class AdmmLoop:
         else:
             if self.graph is None or self.graph_budgets != self._graph_budgets():
                 self.capture()
-            self.graph.replay()
+            self.graph.launch()
             if int(self.capped.item()):
                 self.fell_back = True
                 self.load(*self._loaded)
@@ -362,18 +358,17 @@ class AdmmLoop:
         return self.budgets if self.budgets is not None else (self.pcg_max, FILL_MAX_ITER)
 
     def capture(self):
-        """Records the loop into a HIP graph (the state buffers are fixed, so a replay runs the
-        loop on whatever ``load`` put there), with the iteration budgets of the last eager run (the
-        full limits before any)."""
+        """Records the loop into a HIP graph through the C ABI (``aldvc_admm_loop_graph``; the
+        state buffers are fixed, so a replay runs the loop on whatever ``load`` put there), with
+        the iteration budgets of the last eager run (the full limits before any)."""
         self._sync, self._limits = 0, self._graph_budgets()
-        g = torch.cuda.CUDAGraph()
+        D = self._desc()
         s = torch.cuda.Stream()
         s.wait_stream(torch.cuda.current_stream())
-        with torch.cuda.stream(s):
-            with torch.cuda.graph(g, stream=s):
-                self._enqueue()
+        h = ctypes.c_void_p()
+        check(lib().aldvc_admm_loop_graph(ctypes.byref(D), K.stream_handle(s), ctypes.byref(h)), "aldvc_admm_loop_graph")
         torch.cuda.current_stream().wait_stream(s)
-        self.graph, self.graph_budgets = g, self._limits
+        self.graph, self.graph_budgets = _Graph(h.value), self._limits
 
     def results(self, info_local: LocalSolveInfo | None = None):
         """``(U_final, F_final, ADMMInfo, status, zncc, bad_last)`` on the host, as the reference

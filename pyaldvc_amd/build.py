@@ -22,6 +22,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libaldvc_hip.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "aldvc_hip.h")
 SOURCES = [
     "aldvc_icgn.hip",
     "aldvc_ncc.hip",
@@ -31,6 +32,7 @@ SOURCES = [
     "aldvc_warp.hip",
     "aldvc_admm.hip",
     "aldvc_uncertainty.hip",
+    "aldvc_admm_loop.hip",
 ]
 # compiled with the source hash defined; holds nothing else
 INFO_SOURCE = "aldvc_build_info.hip"
@@ -55,11 +57,13 @@ def hipcc() -> str:
 
 
 def source_hash(csrc: str | None = None) -> str:
-    """SHA-256 (first 16 hex digits) over the names and bytes of the sources under ``csrc``."""
+    """SHA-256 (first 16 hex digits) over the names and bytes of the sources under ``csrc`` and of
+    the public header the sources include."""
     csrc = csrc or CSRC
     h = hashlib.sha256()
-    for name in sorted(os.listdir(csrc)):
-        path = os.path.join(csrc, name)
+    paths = [os.path.join(csrc, name) for name in sorted(os.listdir(csrc))] + [HEADER]
+    for path in paths:
+        name = os.path.basename(path)
         if not os.path.isfile(path) or not name.endswith((".hip", ".h")):
             continue
         h.update(name.encode())

@@ -135,3 +135,54 @@ def test_noalloc_entry_rejects_calls_beyond_its_scratch(lib):
     assert b"254,199 nodes" in lib.aldvc_last_error()
     assert lib.aldvc_icgn12_workspace_bytes(254_199, 8, 8, 8, 1, 1) <= 64 << 20
     assert lib.aldvc_icgn12_workspace_bytes(254_200, 8, 8, 8, 1, 1) > 64 << 20
+
+
+def test_loop_descriptor_layout_matches_the_header(tmp_path):
+    """The ctypes mirror of AldvcAdmmLoop has the C compiler's size and field offsets."""
+    import shutil
+    import subprocess
+
+    from pyaldvc_amd._lib import AdmmLoopDesc
+
+    if shutil.which("gcc") is None or not os.path.isdir("/opt/rocm/include"):
+        pytest.skip("gcc or the ROCm headers are missing")
+    names = [f[0] for f in AdmmLoopDesc._fields_]
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "aldvc_hip.h"', "int main(void) {",
+             '  printf("%zu\\n", sizeof(AldvcAdmmLoop));']  # fmt: skip
+    lines += [f'  printf("%zu\\n", offsetof(AldvcAdmmLoop, {n}));' for n in names]
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    res = subprocess.run(["gcc", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", f"-I{os.path.dirname(HEADER)}",
+                          str(src), "-o", str(exe)], capture_output=True, text=True)  # fmt: skip
+    assert res.returncode == 0, res.stderr[-2000:]
+    got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    assert got[0] == ctypes.sizeof(AdmmLoopDesc)
+    assert got[1:] == [getattr(AdmmLoopDesc, n).offset for n in names]
+
+
+def test_loop_entries_refuse_bad_descriptors_without_a_gpu(lib):
+    from pyaldvc_amd._lib import AdmmLoopDesc
+
+    D = AdmmLoopDesc()
+    D.struct_bytes = ctypes.sizeof(AdmmLoopDesc) - 8  # a caller built against another layout
+    assert lib.aldvc_admm_loop_enqueue(ctypes.byref(D), 0, None) == -2 and b"layout" in lib.aldvc_last_error()
+    D.struct_bytes = ctypes.sizeof(AdmmLoopDesc)
+    D.nx, D.ny, D.nz, D.n_nodes, D.max_steps = 4, 3, 2, 24, 3
+    D.pcg_budget = D.pcg_limit = D.fill_budget = D.fill_limit = 10
+    for name, _ in AdmmLoopDesc._fields_:  # dummy addresses: every check returns before any HIP call
+        if name in ("U1", "U2", "v", "U1raw", "U1n", "U2n", "F2", "W", "F2n", "status", "zncc", "n_iter", "bad",
+                    "history", "pcg_iterations", "fill_iterations", "stop", "steps", "capped"):  # fmt: skip
+            setattr(D, name, 256)
+    D.method = 2
+    assert lib.aldvc_admm_loop_enqueue(ctypes.byref(D), 0, None) == -2 and b"method" in lib.aldvc_last_error()
+    D.method = 1
+    D.pcg_budget = 11
+    assert lib.aldvc_admm_loop_enqueue(ctypes.byref(D), 0, None) == -2 and b"budgets" in lib.aldvc_last_error()
+    D.pcg_budget = 10
+    h = ctypes.c_void_p()
+    assert lib.aldvc_admm_loop_graph(ctypes.byref(D), None, ctypes.byref(h)) == -2
+    assert b"null stream" in lib.aldvc_last_error()
+    assert lib.aldvc_graph_launch(None, None) == -2
+    assert lib.aldvc_graph_destroy(None) == 0
