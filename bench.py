@@ -214,6 +214,72 @@ def kernel_times(fn, steps):
     return float(np.mean(ms)) / 1e3
 
 
+def measure_occupancy(fn, n_slots: int):
+    """How one fused launch fills the machine, from the wall-clock ticks at which the solve of each
+    node starts and ends (``aldvc_workgroup_clock``; one extra launch, not timed): the nodes in
+    flight at most (the resident workgroups), the time until that is first reached (ramp) and after
+    it is last held (tail), and ``fill`` = sum of node durations / (peak in flight x launch span),
+    the fraction of the workgroup slots kept solving. 1 - fill bounds what any other scheduling of
+    the same node solves could save."""
+    import ctypes
+
+    import torch
+
+    from pyaldvc_amd import kernels as K
+    from pyaldvc_amd._lib import check, lib
+
+    buf = torch.zeros(2 * n_slots, dtype=torch.int64, device=K.device())
+    khz = ctypes.c_int(0)
+    check(lib().aldvc_workgroup_clock(ctypes.c_void_p(buf.data_ptr()), n_slots, ctypes.byref(khz)),
+          "aldvc_workgroup_clock")  # fmt: skip
+    try:
+        fn()
+        torch.cuda.synchronize()
+    finally:
+        lib().aldvc_workgroup_clock(None, 0, None)
+    c = buf.cpu().numpy().reshape(n_slots, 2)
+    rec = (c[:, 0] > 0) & (c[:, 1] >= c[:, 0])
+    xcd = (np.arange(n_slots)[rec] // 128) % 8  # the home XCD of the node's 128-node chunk of the list
+    c = c[rec]
+    if c.shape[0] == 0 or khz.value <= 0:
+        return None
+    s, e = c[:, 0], c[:, 1]
+    per_ms = float(khz.value)  # ticks per millisecond
+    t0, t1 = int(s.min()), int(e.max())
+    span = float(t1 - t0)
+    ev = np.concatenate([s, e]) - t0
+    d = np.concatenate([np.ones(s.size, np.int64), -np.ones(e.size, np.int64)])
+    order = np.lexsort((d, ev))  # at equal times the ends first
+    ev, live = ev[order], np.cumsum(d[order])
+    cmax = int(live.max())
+    dt = np.diff(np.append(ev, span))  # time each concurrency level is held
+    busy = float(np.sum(e - s))
+    hi = np.flatnonzero(live >= 0.9 * cmax)
+    ramp = float(ev[hi[0]]) if hi.size else 0.0
+    tail_start = float(ev[hi[-1]] + dt[hi[-1]]) if hi.size else 0.0
+    tail = np.flatnonzero(ev >= tail_start)
+    lost_tail = float(np.sum((cmax - live[tail]) * dt[tail]))
+    dur = (e - s) / per_ms
+    return {
+        "nodes": int(c.shape[0]),
+        "span_ms": span / per_ms,
+        "peak_concurrency": cmax,
+        "fill": busy / (cmax * span),
+        "ramp_ms": ramp / per_ms,
+        "tail_ms": (span - tail_start) / per_ms,
+        "tail_lost_frac": lost_tail / (cmax * span),
+        "held_95pct_frac": float(np.sum(dt[live >= 0.95 * cmax]) / span),
+        "node_ms": {"mean": float(dur.mean()), "p50": float(np.median(dur)), "p99": float(np.percentile(dur, 99)),
+                         "max": float(dur.max())},  # fmt: skip
+        "xcd_end_ms": [float((e[xcd == k].max() - t0) / per_ms) for k in range(8) if np.any(xcd == k)],
+        "xcd_busy_ms": [float(np.sum(e[xcd == k] - s[xcd == k]) / per_ms / (cmax / 8)) for k in range(8)
+                        if np.any(xcd == k)],  # fmt: skip
+        "clock_khz": int(khz.value),
+        "method": "aldvc_workgroup_clock: wall-clock ticks at the start and end of every node's solve, one extra "
+                  "launch of the same step",
+    }
+
+
 def cpu_baseline(W, pre, P0, target_s=5.0):
     """The float64 CPU restatement (OpenMP, every host thread) on a bounded sample of valid nodes:
     the sample is sized by a probe to about ``target_s`` seconds, then timed three times (warm);
@@ -461,6 +527,12 @@ def main():
     times, wall, out = timed_steps(lambda: step("fused", tuning), args.steps, args.warmup)
     t_step = float(np.mean(times))
     t_kernel = kernel_times(lambda: step("fused", tuning), args.steps)
+    occupancy = measure_occupancy(lambda: step("fused", tuning), (N + 1023) // 1024 * 1024)  # the launch's grid
+    if occupancy:
+        log(f"occupancy: {occupancy['nodes']} nodes over {occupancy['span_ms']:.1f} ms, at most "
+            f"{occupancy['peak_concurrency']} in flight, fill {occupancy['fill']:.3f}, ramp {occupancy['ramp_ms']:.2f} ms, "
+            f"tail {occupancy['tail_ms']:.2f} ms ({occupancy['tail_lost_frac']:.4f} of slot-time), node "
+            f"{occupancy['node_ms']['mean']:.2f} ms mean / {occupancy['node_ms']['max']:.2f} ms max")  # fmt: skip
     P, n_iter, status, zncc = (x.cpu().numpy() for x in out)
     conv = status == 0
     err = np.linalg.norm(P[conv, 9:] - W["U_true"][conv], axis=1)
@@ -576,6 +648,7 @@ def main():
             "algorithmic_bytes_per_launch": bytes_alg,
             "valu": valu,
             "counters": counters,
+            "occupancy": occupancy,
         },
         "cpu_baseline": cpu,
     }

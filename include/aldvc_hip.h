@@ -49,6 +49,15 @@ int aldvc_kernel_timing(int on);
 /* Duration in ms of the last timed kernel on this thread (waits for it); -1 if there is none. */
 float aldvc_last_kernel_ms(void);
 
+/* Measurement hook, no counterpart in the reference: with buf (device, 2 x capacity unsigned 64-bit
+ * values) the following fused 12-DOF and 3-DOF launches on this thread over at most capacity nodes
+ * rounded down to a multiple of 1024 (capacity >= N rounded up to a multiple of 1024) write the
+ * wall-clock tick at which the solve of the i-th entry of the launch's valid-node list starts,
+ * buf[2 i], and ends, buf[2 i + 1] (entries beyond the valid nodes are left alone); NULL switches it
+ * off. *clock_khz (host, nullable) receives the tick rate. For measuring how the launch fills the
+ * machine: nodes in flight over time, the tail, per-node durations. */
+int aldvc_workgroup_clock(void *buf, int64_t capacity, int *clock_khz);
+
 /* 7-point central-difference gradients, zero within `border` voxels of any face.
  * Replaces io/volume_ops.py:124-197 (_gradient_stencil7 / compute_gradients). */
 int aldvc_gradients(const float *f, int64_t nz, int64_t ny, int64_t nx, int border, float *gx, float *gy,

@@ -63,6 +63,7 @@ SIGNATURES = {
     "aldvc_last_error": (ctypes.c_char_p, []),
     "aldvc_kernel_timing": (I32, [I32]),
     "aldvc_last_kernel_ms": (ctypes.c_float, []),
+    "aldvc_workgroup_clock": (I32, [P, I64, ctypes.POINTER(I32)]),
     "aldvc_gradients": (I32, [P, I64, I64, I64, I32, P, P, P, P]),
     "aldvc_sample_points": (I32, [P, I64, I64, I64, P, P, P, I64, I32, P, P]),
     "aldvc_precompute": (

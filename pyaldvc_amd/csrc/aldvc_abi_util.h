@@ -39,6 +39,9 @@ inline int launch_status() {
 struct KernelTimer {
     bool on = false, have = false;
     hipEvent_t a = nullptr, b = nullptr;
+    // per-workgroup start / end wall-clock ticks of the next fused 12-DOF launches (diagnostics)
+    unsigned long long* wg_clock = nullptr;
+    int64_t wg_capacity = 0;  // workgroups the buffer holds
 };
 
 inline KernelTimer& timer() {

@@ -54,6 +54,9 @@ struct IcgnArgs {
     const double* bottomf;  // (N) reference subset norms of the precompute
     double* U;            // (N, 3) out
     double mu;
+    // diagnostics (null in normal use): wall-clock ticks at the start and end of every
+    // workgroup that takes a node, [2 b] and [2 b + 1] for workgroup b
+    unsigned long long* wg_clock;
 };
 
 // Launch choices of the fused 12-DOF kernels that leave the results' meaning unchanged (the
@@ -587,16 +590,28 @@ constexpr int MOM_STRIDE = 32;  // doubles per node in the moments workspace
 constexpr int BRICK_XY = 40;    // LDS brick rows and columns: one box for subsets up to 35 voxels wide
 constexpr int LDS_PER_WG = 54528;  // LDS budget of one fused workgroup: three per CU (160 KB)
 
-// XCD-aware slot of workgroup b over the M valid nodes: workgroup b runs on XCD b % 8, and each
-// XCD gets a contiguous run of the node list so its concurrent nodes share subset voxels in its
-// L2. Returns -1 for the workgroups beyond M (the grid is sized for every node).
+// XCD-aware slot of workgroup b over the M valid nodes: workgroup b runs on XCD b % 8, and the
+// node list is dealt to the XCDs in chunks of XCD_CHUNK consecutive nodes (two 4 x 4 x 4 tiles of
+// the tiled list at step 8), chunk q to XCD q % 8, so the nodes an XCD runs at once are neighbours
+// that share subset voxels in its L2, while every XCD samples the whole volume and ends with the
+// same share of the work (one contiguous eighth per XCD left the XCDs up to 12 % apart). Returns
+// -1 for the workgroups beyond M (the grid, xcd_grid, covers every node).
+constexpr int64_t XCD_CHUNK = 128;
 __device__ inline int64_t xcd_slot(int64_t b, int64_t M, int xcd = 1) {
     if (!xcd) return b < M ? b : -1;
-    const int64_t per = (M + 7) / 8;
-    if ((b >> 3) >= per) return -1;
-    const int64_t slot = (b & 7) * per + (b >> 3);
+    const int64_t k = b & 7, j = b >> 3;
+    const int64_t slot = ((j / XCD_CHUNK) * 8 + k) * XCD_CHUNK + (j % XCD_CHUNK);
     return slot < M ? slot : -1;
 }
+
+// list slot of the j-th node XCD k takes: its chunks are q = k, k + 8, k + 16, ...
+__device__ inline int64_t chunk_slot(int k, int64_t j) {
+    return ((j / XCD_CHUNK) * 8 + k) * XCD_CHUNK + (j % XCD_CHUNK);
+}
+constexpr int NODE_COUNTERS = 8;  // workspace header: count[0] valid nodes, count[8 + k] XCD k's next node
+
+// workgroups of a launch over N nodes: N rounded up to whole rounds of 8 chunks
+inline int64_t xcd_grid(int64_t N) { return (N + 8 * XCD_CHUNK - 1) / (8 * XCD_CHUNK) * (8 * XCD_CHUNK); }
 
 // Kept-voxel test shared by every pass: the keep row (inside the volume by construction, checked
 // anyway) or the reference mask.
@@ -747,7 +762,7 @@ __device__ inline void stage_brick(float* brick, int bplane, int cols, int rows,
 // on the same sampling paths, with the deformed-side moments restricted to q = 1.
 template <int MODE, bool STORED, bool BUF, int BRK, bool SEL, bool T3 = false>
 __global__ __launch_bounds__(BLOCK, 3) void k_icgn12_fused(IcgnArgs a, const int64_t* __restrict__ list,
-                                                        const int64_t* __restrict__ count,
+                                                        const int64_t* count,  // also the node counters
                                                         const double* __restrict__ mom, int brick_planes,
                                                         int brick_plane, int brick_rows, int lane_map, int tile_x,
                                                         int tile_y) {
@@ -755,219 +770,229 @@ __global__ __launch_bounds__(BLOCK, 3) void k_icgn12_fused(IcgnArgs a, const int
     const Volumes& V = a.V;
     const Subset& s = a.sub;
     const int tid = threadIdx.x;
-    const int64_t slot = xcd_slot(blockIdx.x, *count, a.xcd);
-    if (slot < 0) return;
-    const int64_t n = list[slot];
-    const int64_t x0 = a.coords[3 * n], y0 = a.coords[3 * n + 1], z0 = a.coords[3 * n + 2];
-    const uint8_t* keep = keep_row_of(a, n);
-
-    // dynamic LDS: one NaN bit per sampled voxel, then (brick path) the brick, 16-byte aligned
-    extern __shared__ __align__(16) unsigned nanbits[];
-    const int nwords = (s.S + 31) >> 5;
-    float* brick = reinterpret_cast<float*>(nanbits + ((nwords + 3) & ~3));
-    __shared__ float sw[14];  // warp (12), pilots pf, pg
-    __shared__ int ctrl;
-    __shared__ double red[NWAVES * NR];
-    __shared__ double res[NG];
-    __shared__ double exc[NR];
-    __shared__ std::conditional_t<T3, Ctl3, Ctl12> c;
-    constexpr int NM = T3 ? NG3 : NG;  // deformed-side moments accumulated per lane
-
-    double pg_next = 0.0;
-    if (tid == 0) {
-        const bool in_volume = keep || subset_in_volume(V, s, x0, y0, z0);
-        if constexpr (T3) {
-            ctrl = ctl3_init(c, a, n, in_volume);
-        } else {
-            bool finite = true;
-            ctl12_init(c, a.P + 12 * n);
-#pragma unroll 1
-            for (int k = 0; k < 12; ++k)
-                if (!isfinite(c.P[k])) finite = false;
-            ctrl = -1;
-            if (!finite)
-                ctrl = ST_NAN;
-            else if (!in_volume)
-                ctrl = ST_INVALID_SUBSET;
-        }
-        double pf = a.meanf ? a.meanf[n] : 0.0;
-        if (!isfinite(pf)) pf = 0.0;
-        sw[12] = (float)pf;
-        pg_next = pf;
-    }
-    __syncthreads();
-    // without noise correction the node's Cholesky factor is read every iteration by the control
-    // lane: a copy in LDS (the corrected factor's slot, unused then) replaces its global loads
-    const double* Lctl = nullptr;
-    if constexpr (!T3) {
-        if (!(a.noise_gain > 0.0 && a.H_all && a.pattern)) {
-            for (int i = tid; i < 144; i += BLOCK) c.L0[i] = a.L_all[144 * n + i];
-            Lctl = c.L0;
-        } else {
-            Lctl = a.L_all + 144 * n;
-        }
-    }
-    __syncthreads();
-    const double* mn = mom + n * MOM_STRIDE;
-    const double half_scale = (double)max(s.hx, max(s.hy, s.hz));
-    const Domain dom = make_domain(V, (int)x0, (int)y0, (int)z0);
-    const float* gbase = V.g + lin(V, z0, y0, x0);
-    const int64_t obase = lin(V, z0, y0, x0);
-    const int rowp = V.nx, planep = V.nx * V.ny;
-    const BufVol gb = make_bufvol(V.g, (int64_t)V.nz * planep, rowp, planep);
-    int it_done = 0;
-    int status = ST_MAX_ITER;
-    for (int it = 1; it <= a.max_iter; ++it) {
-        if (ctrl >= 0) break;
-        it_done = it;
+    // persistent workgroups: node slots come from per-XCD counters in the workspace header (zeroed
+    // by the compaction), own XCD first, then the others' remaining chunks
+    __shared__ int64_t s_slot;
+    __shared__ int s_victim;
+    int64_t* const ctr = const_cast<int64_t*>(count) + NODE_COUNTERS;
+    const int64_t M = *count;
+    const int home = a.xcd ? (int)(blockIdx.x & 7) : 0;
+    if (tid == 0) s_victim = 0;
+    for (;;) {
         if (tid == 0) {
-            put_warp(sw, c.P);
-            sw[13] = (float)pg_next;
+            int64_t sl = -1;
+            for (int v = s_victim; v < (a.xcd ? 8 : 1); ++v) {
+                const int k = (home + v) & 7;
+                const int64_t j = (int64_t)atomicAdd(reinterpret_cast<unsigned long long*>(ctr + k), 1ull);
+                sl = a.xcd ? chunk_slot(k, j) : j;
+                if (sl < M) {
+                    s_victim = v;
+                    break;
+                }
+                sl = -1;
+            }
+            s_slot = sl;
         }
-        for (int i = tid; i < nwords; i += BLOCK) nanbits[i] = 0u;
         __syncthreads();
-        const float pf = sw[12], pg = sw[13];
-        float m[NM];
-#pragma unroll
-        for (int q = 0; q < NM; ++q) m[q] = 0.0f;
-        // one subset voxel: reference-side values first (they do not depend on the warp, so their
-        // loads overlap the gather of the deformed sample), then the warp, the sample and the
-        // deformed-side moments. `FROM_BRICK`: every sample of the chunk is known to be inside.
-        auto voxel = [&](const auto& w, auto from_brick, int bx, int by, int bz) {
-            constexpr bool FROM_BRICK = decltype(from_brick)::value;
-            const int dx = w.dx, dy = w.dy, dz = w.dz;
-            const int64_t o = obase + w.o;
-            if constexpr (SEL)
-                if (!voxel_kept(V, keep, w.v, o, x0 + dx, y0 + dy, z0 + dz)) return;
-            float fraw, gr[3];
-            if constexpr (BUF && STORED) {
-                // a volume under 4 GB: one 32-bit byte offset serves all four reference arrays
-                const uint32_t ob = 4u * (uint32_t)((int)obase + w.o);
-                auto at = [ob](const float* vol) { return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(vol) + ob); };
-                fraw = at(V.f);
-                gr[0] = at(V.gx);
-                gr[1] = at(V.gy);
-                gr[2] = at(V.gz);
-            } else {
-                fraw = V.f[o];
-                grad_at<STORED>(V, o, (int)z0 + dz, (int)y0 + dy, (int)x0 + dx, gr[0], gr[1], gr[2]);
-            }
-            const float X = (float)dx, Y = (float)dy, Z = (float)dz;
-            const float xr = warp_coord(sw[0], sw[1], sw[2], sw[9], X, Y, Z);
-            const float yr = warp_coord(sw[3], sw[4], sw[5], sw[10], X, Y, Z);
-            const float zr = warp_coord(sw[6], sw[7], sw[8], sw[11], X, Y, Z);
-            float val;
-            if constexpr (FROM_BRICK) {
-                val = sample_brick<MODE, BRK>(brick, brick_plane, bx, by, bz, xr, yr, zr);
-            } else {
-                if (!inside(dom, xr, yr, zr)) {
-                    m[1] = 1.0f;
-                    return;
-                }
-                if constexpr (BUF)
-                    val = sample_buf<MODE>(gb, (int)obase, rowp, planep, xr, yr, zr, dom.kx, dom.ky, dom.kz);
-                else
-                    val = sample_rel<MODE>(gbase, rowp, planep, xr, yr, zr, dom.kx, dom.ky, dom.kz);
-            }
-            // a NaN sample (masked in the deformed frame) drops the voxel from this iteration; the
-            // accumulation is branch-free (gv = 0 adds nothing), so the reference-side loads above
-            // stay ahead of the sample instead of being sunk into a branch
-            const bool ok = val == val;
-            const float fv = fraw - pf, gv = ok ? val - pg : 0.0f;
-            const float q[4] = {X, Y, Z, 1.0f};
-            m[3] += gv;
-            m[4] += gv * gv;
-            m[5] += fv * gv;
+        const int64_t slot = s_slot;
+        if (slot < 0) break;
+        if (a.wg_clock && tid == 0) a.wg_clock[2 * slot] = wall_clock64();
+        const int64_t n = list[slot];
+        const int64_t x0 = a.coords[3 * n], y0 = a.coords[3 * n + 1], z0 = a.coords[3 * n + 2];
+        const uint8_t* keep = keep_row_of(a, n);
+
+        // dynamic LDS: one NaN bit per sampled voxel, then (brick path) the brick, 16-byte aligned
+        extern __shared__ __align__(16) unsigned nanbits[];
+        const int nwords = (s.S + 31) >> 5;
+        float* brick = reinterpret_cast<float*>(nanbits + ((nwords + 3) & ~3));
+        __shared__ float sw[14];  // warp (12), pilots pf, pg
+        __shared__ int ctrl;
+        __shared__ double red[NWAVES * NR];
+        __shared__ double res[NG];
+        __shared__ double exc[NR];
+        __shared__ std::conditional_t<T3, Ctl3, Ctl12> c;
+        constexpr int NM = T3 ? NG3 : NG;  // deformed-side moments accumulated per lane
+
+        double pg_next = 0.0;
+        if (tid == 0) {
+            const bool in_volume = keep || subset_in_volume(V, s, x0, y0, z0);
             if constexpr (T3) {
-#pragma unroll
-                for (int cc = 0; cc < 3; ++cc) m[6 + cc] += gr[cc] * gv;
+                ctrl = ctl3_init(c, a, n, in_volume);
             } else {
-#pragma unroll
-                for (int cc = 0; cc < 3; ++cc) {
-                    const float tg = gr[cc] * gv;
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) m[6 + 4 * cc + j] += tg * q[j];
+                bool finite = true;
+                ctl12_init(c, a.P + 12 * n);
+    #pragma unroll 1
+                for (int k = 0; k < 12; ++k)
+                    if (!isfinite(c.P[k])) finite = false;
+                ctrl = -1;
+                if (!finite)
+                    ctrl = ST_NAN;
+                else if (!in_volume)
+                    ctrl = ST_INVALID_SUBSET;
+            }
+            double pf = a.meanf ? a.meanf[n] : 0.0;
+            if (!isfinite(pf)) pf = 0.0;
+            sw[12] = (float)pf;
+            pg_next = pf;
+        }
+        __syncthreads();
+        // without noise correction the node's Cholesky factor is read every iteration by the control
+        // lane: a copy in LDS (the corrected factor's slot, unused then) replaces its global loads
+        const double* Lctl = nullptr;
+        if constexpr (!T3) {
+            if (!(a.noise_gain > 0.0 && a.H_all && a.pattern)) {
+                for (int i = tid; i < 144; i += BLOCK) c.L0[i] = a.L_all[144 * n + i];
+                Lctl = c.L0;
+            } else {
+                Lctl = a.L_all + 144 * n;
+            }
+        }
+        __syncthreads();
+        const double* mn = mom + n * MOM_STRIDE;
+        const double half_scale = (double)max(s.hx, max(s.hy, s.hz));
+        const Domain dom = make_domain(V, (int)x0, (int)y0, (int)z0);
+        const float* gbase = V.g + lin(V, z0, y0, x0);
+        const int64_t obase = lin(V, z0, y0, x0);
+        const int rowp = V.nx, planep = V.nx * V.ny;
+        const BufVol gb = make_bufvol(V.g, (int64_t)V.nz * planep, rowp, planep);
+        int it_done = 0;
+        int status = ST_MAX_ITER;
+        for (int it = 1; it <= a.max_iter; ++it) {
+            if (ctrl >= 0) break;
+            it_done = it;
+            if (tid == 0) {
+                put_warp(sw, c.P);
+                sw[13] = (float)pg_next;
+            }
+            for (int i = tid; i < nwords; i += BLOCK) nanbits[i] = 0u;
+            __syncthreads();
+            const float pf = sw[12], pg = sw[13];
+            float m[NM];
+    #pragma unroll
+            for (int q = 0; q < NM; ++q) m[q] = 0.0f;
+            // one subset voxel: reference-side values first (they do not depend on the warp, so their
+            // loads overlap the gather of the deformed sample), then the warp, the sample and the
+            // deformed-side moments. `FROM_BRICK`: every sample of the chunk is known to be inside.
+            auto voxel = [&](const auto& w, auto from_brick, int bx, int by, int bz) {
+                constexpr bool FROM_BRICK = decltype(from_brick)::value;
+                const int dx = w.dx, dy = w.dy, dz = w.dz;
+                const int64_t o = obase + w.o;
+                if constexpr (SEL)
+                    if (!voxel_kept(V, keep, w.v, o, x0 + dx, y0 + dy, z0 + dz)) return;
+                float fraw, gr[3];
+                if constexpr (BUF && STORED) {
+                    // a volume under 4 GB: one 32-bit byte offset serves all four reference arrays
+                    const uint32_t ob = 4u * (uint32_t)((int)obase + w.o);
+                    auto at = [ob](const float* vol) {
+                        return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(vol) + ob);
+                    };
+                    fraw = at(V.f);
+                    gr[0] = at(V.gx);
+                    gr[1] = at(V.gy);
+                    gr[2] = at(V.gz);
+                } else {
+                    fraw = V.f[o];
+                    grad_at<STORED>(V, o, (int)z0 + dz, (int)y0 + dy, (int)x0 + dx, gr[0], gr[1], gr[2]);
                 }
-            }
-            if (!ok) {  // rare: the valid count is the kept count less these
-                m[2] += 1.0f;
-                atomicOr(&nanbits[w.v >> 5], 1u << (w.v & 31));
-            }
-        };
-        using Gather = std::integral_constant<bool, false>;
-        if constexpr (BRK > 0) {
-            // The subset in boxes of tile_x x tile_y voxel columns (one box when its rows fit the
-            // brick), each box in chunks of planes whose warped stencils fit the brick; a chunk
-            // that does not fit (large rotation or shear, or a sample outside the domain) gathers
-            // from memory.
-            constexpr int OVER = MODE == INTERP_LINEAR ? 1 : 3;
-            const int sx = s.sx, sy = s.sy, plane_vox = sx * sy, lane = tid & 63;
-            // the voxels of box [xc, xc + kx) x [yc, yc + ky) x [zc, zc + kz), t = tid, tid + BLOCK,
-            // ... with x fastest, decoded per voxel (box chunks that gather, boxes of other lane maps)
-            auto box_voxels = [&](auto from_brick, int xc, int kx, int yc, int ky, int zc, int kz, int bx, int by,
-                                  int bz) {
-                const int per_plane = kx * ky;
-                for (int t = tid; t < kz * per_plane; t += BLOCK) {
-                    const int pz = t / per_plane, r = t - pz * per_plane, py = r / kx;
-                    const int ix = xc + r - py * kx, iy = yc + py, iz = zc + pz;
-                    Vox q;
-                    q.dx = -s.hx + s.stride * ix;
-                    q.dy = -s.hy + s.stride * iy;
-                    q.dz = -s.hz + s.stride * iz;
-                    q.v = (iz * sy + iy) * sx + ix;
-                    q.o = q.dz * planep + q.dy * rowp + q.dx;
-                    voxel(q, from_brick, bx, by, bz);
+                const float X = (float)dx, Y = (float)dy, Z = (float)dz;
+                const float xr = warp_coord(sw[0], sw[1], sw[2], sw[9], X, Y, Z);
+                const float yr = warp_coord(sw[3], sw[4], sw[5], sw[10], X, Y, Z);
+                const float zr = warp_coord(sw[6], sw[7], sw[8], sw[11], X, Y, Z);
+                float val;
+                if constexpr (FROM_BRICK) {
+                    val = sample_brick<MODE, BRK>(brick, brick_plane, bx, by, bz, xr, yr, zr);
+                } else {
+                    if (!inside(dom, xr, yr, zr)) {
+                        m[1] = 1.0f;
+                        return;
+                    }
+                    if constexpr (BUF)
+                        val = sample_buf<MODE>(gb, (int)obase, rowp, planep, xr, yr, zr, dom.kx, dom.ky, dom.kz);
+                    else
+                        val = sample_rel<MODE>(gbase, rowp, planep, xr, yr, zr, dom.kx, dom.ky, dom.kz);
+                }
+                // a NaN sample (masked in the deformed frame) drops the voxel from this iteration; the
+                // accumulation is branch-free (gv = 0 adds nothing), so the reference-side loads above
+                // stay ahead of the sample instead of being sunk into a branch
+                const bool ok = val == val;
+                const float fv = fraw - pf, gv = ok ? val - pg : 0.0f;
+                const float q[4] = {X, Y, Z, 1.0f};
+                m[3] += gv;
+                m[4] += gv * gv;
+                m[5] += fv * gv;
+                if constexpr (T3) {
+    #pragma unroll
+                    for (int cc = 0; cc < 3; ++cc) m[6 + cc] += gr[cc] * gv;
+                } else {
+    #pragma unroll
+                    for (int cc = 0; cc < 3; ++cc) {
+                        const float tg = gr[cc] * gv;
+    #pragma unroll
+                        for (int j = 0; j < 4; ++j) m[6 + 4 * cc + j] += tg * q[j];
+                    }
+                }
+                if (!ok) {  // rare: the valid count is the kept count less these
+                    m[2] += 1.0f;
+                    atomicOr(&nanbits[w.v >> 5], 1u << (w.v & 31));
                 }
             };
-            for (int yc = 0; yc < sy; yc += tile_y) {
-                for (int xc = 0; xc < sx; xc += tile_x) {
-                    const int kx = min(tile_x, sx - xc), ky = min(tile_y, sy - yc);
-                    const bool whole = kx == sx && ky == sy;
-                    for (int zc = 0; zc < s.sz;) {
-                        int kz = min(brick_planes - OVER, s.sz - zc), bx = 0, by = 0, bz = 0, ex = 0, ey = 0, ez = 0;
-                        int fit = plan_chunk<MODE, BRK>(sw, s, dom, xc, kx, yc, ky, zc, kz, brick_planes, brick_rows,
-                                                        bx, by, bz, ex, ey, ez);
-                        while (fit == 1 && kz > 1)
-                            fit = plan_chunk<MODE, BRK>(sw, s, dom, xc, kx, yc, ky, zc, --kz, brick_planes,
-                                                        brick_rows, bx, by, bz, ex, ey, ez);
-                        using Brick = std::integral_constant<bool, true>;
-                        if (fit == 0) {
-                            __syncthreads();  // the previous chunk's samples are done with the brick
-                            stage_brick<BRK>(brick, brick_plane, ex, ey, gb, (int)obase, rowp, planep, bx, by, bz, ez);
-                            __syncthreads();
-                            if (lane_map == LANES_PATCHES) {
-                                // 8 x 8 patches of one plane per wave (lane l: x + (l & 7), y + (l >> 3)).
-                                // With a brick row stride of 40 floats the 8 rows of a patch start
-                                // 40 y mod 64 = 8 different bank groups apart, so the 64 lanes read 64
-                                // different banks, and a small rotation or shear only shifts whole patch
-                                // rows by a bank or two. The patch counters are wave-uniform and advance
-                                // by NWAVES patches with carries (scalar work, no division); a lane adds
-                                // its fixed offsets within the patch. The box's columns and rows beyond
-                                // its last whole patch follow in a short tail.
-                                const int pxn = kx >> 3, pyn = ky >> 3;
-                                if (pxn > 0 && pyn > 0) {
-                                    const int lx = lane & 7, ly = lane >> 3;
-                                    const int lane_v = ly * sx + lx, lane_o = ly * rowp + lx;
-                                    int pxi = __builtin_amdgcn_readfirstlane(tid >> 6), pyi = 0, pz = 0;
-                                    while (pxi >= pxn) {
-                                        pxi -= pxn;
-                                        ++pyi;
-                                    }
-                                    while (pyi >= pyn) {
-                                        pyi -= pyn;
-                                        ++pz;
-                                    }
-                                    while (pz < kz) {
-                                        const int iz = zc + pz, x = xc + 8 * pxi, y = yc + 8 * pyi;
-                                        const int ux = -s.hx + x, uy = -s.hy + y, uz = -s.hz + iz;
-                                        Vox q;
-                                        q.dx = ux + lx;
-                                        q.dy = uy + ly;
-                                        q.dz = uz;
-                                        q.v = (iz * sy + y) * sx + x + lane_v;
-                                        q.o = uz * planep + uy * rowp + ux + lane_o;
-                                        voxel(q, Brick{}, bx, by, bz);
-                                        pxi += NWAVES;
+            using Gather = std::integral_constant<bool, false>;
+            if constexpr (BRK > 0) {
+                // The subset in boxes of tile_x x tile_y voxel columns (one box when its rows fit the
+                // brick), each box in chunks of planes whose warped stencils fit the brick; a chunk
+                // that does not fit (large rotation or shear, or a sample outside the domain) gathers
+                // from memory.
+                constexpr int OVER = MODE == INTERP_LINEAR ? 1 : 3;
+                const int sx = s.sx, sy = s.sy, plane_vox = sx * sy, lane = tid & 63;
+                // the voxels of box [xc, xc + kx) x [yc, yc + ky) x [zc, zc + kz), t = tid, tid + BLOCK,
+                // ... with x fastest, decoded per voxel (box chunks that gather, boxes of other lane maps)
+                auto box_voxels = [&](auto from_brick, int xc, int kx, int yc, int ky, int zc, int kz, int bx, int by,
+                                      int bz) {
+                    const int per_plane = kx * ky;
+                    for (int t = tid; t < kz * per_plane; t += BLOCK) {
+                        const int pz = t / per_plane, r = t - pz * per_plane, py = r / kx;
+                        const int ix = xc + r - py * kx, iy = yc + py, iz = zc + pz;
+                        Vox q;
+                        q.dx = -s.hx + s.stride * ix;
+                        q.dy = -s.hy + s.stride * iy;
+                        q.dz = -s.hz + s.stride * iz;
+                        q.v = (iz * sy + iy) * sx + ix;
+                        q.o = q.dz * planep + q.dy * rowp + q.dx;
+                        voxel(q, from_brick, bx, by, bz);
+                    }
+                };
+                for (int yc = 0; yc < sy; yc += tile_y) {
+                    for (int xc = 0; xc < sx; xc += tile_x) {
+                        const int kx = min(tile_x, sx - xc), ky = min(tile_y, sy - yc);
+                        const bool whole = kx == sx && ky == sy;
+                        for (int zc = 0; zc < s.sz;) {
+                            int kz = min(brick_planes - OVER, s.sz - zc);
+                            int bx = 0, by = 0, bz = 0, ex = 0, ey = 0, ez = 0;
+                            int fit = plan_chunk<MODE, BRK>(sw, s, dom, xc, kx, yc, ky, zc, kz, brick_planes,
+                                                            brick_rows, bx, by, bz, ex, ey, ez);
+                            while (fit == 1 && kz > 1)
+                                fit = plan_chunk<MODE, BRK>(sw, s, dom, xc, kx, yc, ky, zc, --kz, brick_planes,
+                                                            brick_rows, bx, by, bz, ex, ey, ez);
+                            using Brick = std::integral_constant<bool, true>;
+                            if (fit == 0) {
+                                __syncthreads();  // the previous chunk's samples are done with the brick
+                                stage_brick<BRK>(brick, brick_plane, ex, ey, gb, (int)obase, rowp, planep, bx, by, bz,
+                                                 ez);
+                                __syncthreads();
+                                if (lane_map == LANES_PATCHES) {
+                                    // 8 x 8 patches of one plane per wave (lane l: x + (l & 7), y + (l >> 3)).
+                                    // With a brick row stride of 40 floats the 8 rows of a patch start
+                                    // 40 y mod 64 = 8 different bank groups apart, so the 64 lanes read 64
+                                    // different banks, and a small rotation or shear only shifts whole patch
+                                    // rows by a bank or two. The patch counters are wave-uniform and advance
+                                    // by NWAVES patches with carries (scalar work, no division); a lane adds
+                                    // its fixed offsets within the patch. The box's columns and rows beyond
+                                    // its last whole patch follow in a short tail.
+                                    const int pxn = kx >> 3, pyn = ky >> 3;
+                                    if (pxn > 0 && pyn > 0) {
+                                        const int lx = lane & 7, ly = lane >> 3;
+                                        const int lane_v = ly * sx + lx, lane_o = ly * rowp + lx;
+                                        int pxi = __builtin_amdgcn_readfirstlane(tid >> 6), pyi = 0, pz = 0;
                                         while (pxi >= pxn) {
                                             pxi -= pxn;
                                             ++pyi;
@@ -976,85 +1001,93 @@ __global__ __launch_bounds__(BLOCK, 3) void k_icgn12_fused(IcgnArgs a, const int
                                             pyi -= pyn;
                                             ++pz;
                                         }
+                                        while (pz < kz) {
+                                            const int iz = zc + pz, x = xc + 8 * pxi, y = yc + 8 * pyi;
+                                            const int ux = -s.hx + x, uy = -s.hy + y, uz = -s.hz + iz;
+                                            Vox q;
+                                            q.dx = ux + lx;
+                                            q.dy = uy + ly;
+                                            q.dz = uz;
+                                            q.v = (iz * sy + y) * sx + x + lane_v;
+                                            q.o = uz * planep + uy * rowp + ux + lane_o;
+                                            voxel(q, Brick{}, bx, by, bz);
+                                            pxi += NWAVES;
+                                            while (pxi >= pxn) {
+                                                pxi -= pxn;
+                                                ++pyi;
+                                            }
+                                            while (pyi >= pyn) {
+                                                pyi -= pyn;
+                                                ++pz;
+                                            }
+                                        }
                                     }
-                                }
-                                const int tx = kx - 8 * pxn, wxp = 8 * pxn, per_plane = tx * ky + (ky - 8 * pyn) * wxp;
-                                // integer quotients as float products: (n + 0.5) / d lies at least 0.5 / d
-                                // from an integer, and for n below 2^22 its float rounding (under
-                                // n 2^-23 / d) stays inside that margin, so the floor is exact
-                                const float inv_pp = 1.0f / (float)max(per_plane, 1), inv_tx = 1.0f / (float)max(tx, 1),
-                                            inv_wx = 1.0f / (float)max(wxp, 1);
-                                for (int t = tid; t < kz * per_plane; t += BLOCK) {
-                                    const int pz = (int)(((float)t + 0.5f) * inv_pp), r = t - pz * per_plane;
-                                    int x, y;
-                                    if (r < tx * ky) {
-                                        y = (int)(((float)r + 0.5f) * inv_tx);
-                                        x = wxp + (r - y * tx);
-                                    } else {
-                                        const int r2 = r - tx * ky, yy = (int)(((float)r2 + 0.5f) * inv_wx);
-                                        y = 8 * pyn + yy;
-                                        x = r2 - yy * wxp;
+                                    const int tx = kx - 8 * pxn, wxp = 8 * pxn;
+                                    const int per_plane = tx * ky + (ky - 8 * pyn) * wxp;
+                                    // integer quotients as float products: (n + 0.5) / d lies at least 0.5 / d
+                                    // from an integer, and for n below 2^22 its float rounding (under
+                                    // n 2^-23 / d) stays inside that margin, so the floor is exact
+                                    const float inv_pp = 1.0f / (float)max(per_plane, 1);
+                                    const float inv_tx = 1.0f / (float)max(tx, 1), inv_wx = 1.0f / (float)max(wxp, 1);
+                                    for (int t = tid; t < kz * per_plane; t += BLOCK) {
+                                        const int pz = (int)(((float)t + 0.5f) * inv_pp), r = t - pz * per_plane;
+                                        int x, y;
+                                        if (r < tx * ky) {
+                                            y = (int)(((float)r + 0.5f) * inv_tx);
+                                            x = wxp + (r - y * tx);
+                                        } else {
+                                            const int r2 = r - tx * ky, yy = (int)(((float)r2 + 0.5f) * inv_wx);
+                                            y = 8 * pyn + yy;
+                                            x = r2 - yy * wxp;
+                                        }
+                                        const int ix = xc + x, iy = yc + y, iz = zc + pz;
+                                        Vox q;
+                                        q.dx = -s.hx + ix;
+                                        q.dy = -s.hy + iy;
+                                        q.dz = -s.hz + iz;
+                                        q.v = (iz * sy + iy) * sx + ix;
+                                        q.o = q.dz * planep + q.dy * rowp + q.dx;
+                                        voxel(q, Brick{}, bx, by, bz);
                                     }
-                                    const int ix = xc + x, iy = yc + y, iz = zc + pz;
-                                    Vox q;
-                                    q.dx = -s.hx + ix;
-                                    q.dy = -s.hy + iy;
-                                    q.dz = -s.hz + iz;
-                                    q.v = (iz * sy + iy) * sx + ix;
-                                    q.o = q.dz * planep + q.dy * rowp + q.dx;
-                                    voxel(q, Brick{}, bx, by, bz);
-                                }
-                            } else if (whole && lane_map == LANES_PAIRS) {
-                                // Planes in pairs: lane l < sx takes voxel x = l of row y in the first plane,
-                                // lane l >= sx voxel x = l - sx of row y in the second. The brick's plane
-                                // stride is sx modulo 64 banks, so every lane of a wave reads its own LDS bank.
-                                // The second plane's last 2 sx - 64 voxels of each row, and a last unpaired
-                                // plane, follow.
-                                                        const int inB = lane >= sx ? 1 : 0, xl = lane - inB * sx;
-                                const int npair = kz >> 1;
-                                int p = 0, y = tid >> 6;
-                                while (y >= sy) {
-                                    y -= sy;
-                                    ++p;
-                                }
-                                Vox q;
-                                q.dx = -s.hx + xl;
-                                q.dy = -s.hy + y;
-                                q.dz = -s.hz + zc + 2 * p + inB;
-                                q.v = ((zc + 2 * p + inB) * sy + y) * sx + xl;
-                                q.o = q.dz * planep + q.dy * rowp + q.dx;
-                                const int wrap_v = sy * sx, wrap_o = 2 * planep - sy * rowp;
-                                while (p < npair) {
-                                    voxel(q, Brick{}, bx, by, bz);
-                                    y += NWAVES;
-                                    q.dy += NWAVES;
-                                    q.v += NWAVES * sx;
-                                    q.o += NWAVES * rowp;
-                                    if (y >= sy) {
+                                } else if (whole && lane_map == LANES_PAIRS) {
+                                    // Planes in pairs: lane l < sx takes voxel x = l of row y in the first plane,
+                                    // lane l >= sx voxel x = l - sx of row y in the second. The brick's plane
+                                    // stride is sx modulo 64 banks, so every lane of a wave reads its own LDS bank.
+                                    // The second plane's last 2 sx - 64 voxels of each row, and a last unpaired
+                                    // plane, follow.
+                                                            const int inB = lane >= sx ? 1 : 0, xl = lane - inB * sx;
+                                    const int npair = kz >> 1;
+                                    int p = 0, y = tid >> 6;
+                                    while (y >= sy) {
                                         y -= sy;
                                         ++p;
-                                        q.dy -= sy;
-                                        q.dz += 2;
-                                        q.v += wrap_v;
-                                        q.o += wrap_o;
                                     }
-                                }
-                                const int nt = 2 * sx - 64, ntail = npair * sy * nt;
-                                for (int t = tid; t < ntail; t += BLOCK) {
-                                    const int r = t / nt, k = t - r * nt, pp = r / sy, yy = r - pp * sy;
-                                    const int iz = zc + 2 * pp + 1, xx = 64 - sx + k;
-                                    Vox e;
-                                    e.dx = -s.hx + xx;
-                                    e.dy = -s.hy + yy;
-                                    e.dz = -s.hz + iz;
-                                    e.v = (iz * sy + yy) * sx + xx;
-                                    e.o = e.dz * planep + e.dy * rowp + e.dx;
-                                    voxel(e, Brick{}, bx, by, bz);
-                                }
-                                if (kz & 1) {
-                                    const int iz = zc + kz - 1;
-                                    for (int u = tid; u < sy * sx; u += BLOCK) {
-                                        const int yy = u / sx, xx = u - yy * sx;
+                                    Vox q;
+                                    q.dx = -s.hx + xl;
+                                    q.dy = -s.hy + y;
+                                    q.dz = -s.hz + zc + 2 * p + inB;
+                                    q.v = ((zc + 2 * p + inB) * sy + y) * sx + xl;
+                                    q.o = q.dz * planep + q.dy * rowp + q.dx;
+                                    const int wrap_v = sy * sx, wrap_o = 2 * planep - sy * rowp;
+                                    while (p < npair) {
+                                        voxel(q, Brick{}, bx, by, bz);
+                                        y += NWAVES;
+                                        q.dy += NWAVES;
+                                        q.v += NWAVES * sx;
+                                        q.o += NWAVES * rowp;
+                                        if (y >= sy) {
+                                            y -= sy;
+                                            ++p;
+                                            q.dy -= sy;
+                                            q.dz += 2;
+                                            q.v += wrap_v;
+                                            q.o += wrap_o;
+                                        }
+                                    }
+                                    const int nt = 2 * sx - 64, ntail = npair * sy * nt;
+                                    for (int t = tid; t < ntail; t += BLOCK) {
+                                        const int r = t / nt, k = t - r * nt, pp = r / sy, yy = r - pp * sy;
+                                        const int iz = zc + 2 * pp + 1, xx = 64 - sx + k;
                                         Vox e;
                                         e.dx = -s.hx + xx;
                                         e.dy = -s.hy + yy;
@@ -1063,92 +1096,109 @@ __global__ __launch_bounds__(BLOCK, 3) void k_icgn12_fused(IcgnArgs a, const int
                                         e.o = e.dz * planep + e.dy * rowp + e.dx;
                                         voxel(e, Brick{}, bx, by, bz);
                                     }
+                                    if (kz & 1) {
+                                        const int iz = zc + kz - 1;
+                                        for (int u = tid; u < sy * sx; u += BLOCK) {
+                                            const int yy = u / sx, xx = u - yy * sx;
+                                            Vox e;
+                                            e.dx = -s.hx + xx;
+                                            e.dy = -s.hy + yy;
+                                            e.dz = -s.hz + iz;
+                                            e.v = (iz * sy + yy) * sx + xx;
+                                            e.o = e.dz * planep + e.dy * rowp + e.dx;
+                                            voxel(e, Brick{}, bx, by, bz);
+                                        }
+                                    }
+                                } else if (whole) {
+                                    OffsetWalk w;
+                                    w.start(s, zc * plane_vox + tid, rowp, planep);
+                                    const int vend = (zc + kz) * plane_vox;
+                                    for (; w.v < vend; w.next(s)) voxel(w, Brick{}, bx, by, bz);
+                                } else {
+                                    box_voxels(Brick{}, xc, kx, yc, ky, zc, kz, bx, by, bz);
                                 }
                             } else if (whole) {
                                 OffsetWalk w;
                                 w.start(s, zc * plane_vox + tid, rowp, planep);
                                 const int vend = (zc + kz) * plane_vox;
-                                for (; w.v < vend; w.next(s)) voxel(w, Brick{}, bx, by, bz);
+                                for (; w.v < vend; w.next(s)) voxel(w, Gather{}, 0, 0, 0);
                             } else {
-                                box_voxels(Brick{}, xc, kx, yc, ky, zc, kz, bx, by, bz);
+                                box_voxels(Gather{}, xc, kx, yc, ky, zc, kz, 0, 0, 0);
                             }
-                        } else if (whole) {
-                            OffsetWalk w;
-                            w.start(s, zc * plane_vox + tid, rowp, planep);
-                            const int vend = (zc + kz) * plane_vox;
-                            for (; w.v < vend; w.next(s)) voxel(w, Gather{}, 0, 0, 0);
-                        } else {
-                            box_voxels(Gather{}, xc, kx, yc, ky, zc, kz, 0, 0, 0);
+                            zc += kz;
                         }
-                        zc += kz;
                     }
                 }
+            } else {
+                OffsetWalk w;
+                w.start(s, tid, rowp, planep);
+                for (; w.v < s.S; w.next(s)) voxel(w, Gather{}, 0, 0, 0);
             }
-        } else {
-            OffsetWalk w;
-            w.start(s, tid, rowp, planep);
-            for (; w.v < s.S; w.next(s)) voxel(w, Gather{}, 0, 0, 0);
-        }
-        block_sum<NM, NWAVES>(m, res, red);
-        const bool correct = res[2] > 0.0 && !(res[1] > 0.0);  // block-uniform
-        if (correct) {
-            ref_moments<STORED>(
-                a, keep, x0, y0, z0, pf,
-                [&](int v, int64_t, int64_t, int64_t, int64_t) { return ((nanbits[v >> 5] >> (v & 31)) & 1u) != 0; },
-                exc, red);
+            block_sum<NM, NWAVES>(m, res, red);
+            const bool correct = res[2] > 0.0 && !(res[1] > 0.0);  // block-uniform
+            if (correct) {
+                ref_moments<STORED>(
+                    a, keep, x0, y0, z0, pf,
+                    [&](int v, int64_t, int64_t, int64_t, int64_t) {
+                        return ((nanbits[v >> 5] >> (v & 31)) & 1u) != 0;
+                    },
+                    exc, red);
+            }
+            if (tid == 0) {
+                const double n_ref = mn[0], n_valid = n_ref - res[2];  // every kept voxel was sampled once
+                int st = -1;
+                if (res[1] > 0.0) {
+                    st = ST_OUT_OF_BOUNDS;
+                } else if (n_valid < MIN_SUBSET_VOXELS || n_valid < MIN_VALID_FRACTION * n_ref) {
+                    st = ST_INVALID_SUBSET;
+                }
+                if (st >= 0) {
+                    c.zncc = NAN;
+                    ctrl = st;
+                } else {
+                    // reference-side moments of this iteration's voxel set
+                    auto R = [&](int q) { return correct ? mn[q] - exc[q] : mn[q]; };
+                    // centred statistics on the dynamic voxel set (numba_kernels.py:586-598)
+                    const double mf = R(1) / n_valid, mg = res[3] / n_valid;
+                    double ssf = R(2) - n_valid * mf * mf, ssg = res[4] - n_valid * mg * mg;
+                    ssf = ssf < 0.0 ? 0.0 : ssf;
+                    ssg = ssg < 0.0 ? 0.0 : ssg;
+                    const double bf = sqrt(ssf > 1e-30 ? ssf : 1e-30), bg = sqrt(ssg > 1e-30 ? ssg : 1e-30);
+                    if constexpr (T3) {
+                        // q = 1 columns of the reference moments: C_c = R(6 + 4c), A_c = R(18 + 4c)
+                        const double C[3] = {R(6), R(10), R(14)}, A[3] = {R(18), R(22), R(26)};
+                        ctrl = ctl3_step(c, a, n, n_valid, mf, mg, bf, bg, res[5], C, A, res + 6);
+                    } else {
+                        const double zncc = (res[5] - n_valid * mf * mg) / (bf * bg);
+                        const double ratio = bf / bg;
+    #pragma unroll 1
+                        for (int q = 0; q < 12; ++q) {
+                            const int cc = q >> 2, j = q & 3;
+                            const double C = R(3 + q), A = R(15 + q), B = res[6 + q];
+                            c.b[j < 3 ? 3 * cc + j : 9 + cc] = (A - mf * C) - ratio * (B - mg * C);
+                        }
+                        const double* Hn = a.H_all ? a.H_all + 144 * n : nullptr;
+                        ctrl = ctl12_step(c, zncc, bf, n_valid, Lctl, Hn, a.pattern, a.noise_gain, a.tol,
+                                          a.dp_tol, a.patience, a.predictive, half_scale);
+                    }
+                    pg_next = (double)pg + mg;
+                }
+            }
+            __syncthreads();
         }
         if (tid == 0) {
-            const double n_ref = mn[0], n_valid = n_ref - res[2];  // every kept voxel was sampled once
-            int st = -1;
-            if (res[1] > 0.0) {
-                st = ST_OUT_OF_BOUNDS;
-            } else if (n_valid < MIN_SUBSET_VOXELS || n_valid < MIN_VALID_FRACTION * n_ref) {
-                st = ST_INVALID_SUBSET;
-            }
-            if (st >= 0) {
-                c.zncc = NAN;
-                ctrl = st;
+            if (ctrl >= 0) status = ctrl;
+            if constexpr (T3) {
+                for (int k = 0; k < 3; ++k) a.U[3 * n + k] = c.P[9 + k];
             } else {
-                // reference-side moments of this iteration's voxel set
-                auto R = [&](int q) { return correct ? mn[q] - exc[q] : mn[q]; };
-                // centred statistics on the dynamic voxel set (numba_kernels.py:586-598)
-                const double mf = R(1) / n_valid, mg = res[3] / n_valid;
-                double ssf = R(2) - n_valid * mf * mf, ssg = res[4] - n_valid * mg * mg;
-                ssf = ssf < 0.0 ? 0.0 : ssf;
-                ssg = ssg < 0.0 ? 0.0 : ssg;
-                const double bf = sqrt(ssf > 1e-30 ? ssf : 1e-30), bg = sqrt(ssg > 1e-30 ? ssg : 1e-30);
-                if constexpr (T3) {
-                    // q = 1 columns of the reference moments: C_c = R(6 + 4c), A_c = R(18 + 4c)
-                    const double C[3] = {R(6), R(10), R(14)}, A[3] = {R(18), R(22), R(26)};
-                    ctrl = ctl3_step(c, a, n, n_valid, mf, mg, bf, bg, res[5], C, A, res + 6);
-                } else {
-                    const double zncc = (res[5] - n_valid * mf * mg) / (bf * bg);
-                    const double ratio = bf / bg;
-#pragma unroll 1
-                    for (int q = 0; q < 12; ++q) {
-                        const int cc = q >> 2, j = q & 3;
-                        const double C = R(3 + q), A = R(15 + q), B = res[6 + q];
-                        c.b[j < 3 ? 3 * cc + j : 9 + cc] = (A - mf * C) - ratio * (B - mg * C);
-                    }
-                    const double* Hn = a.H_all ? a.H_all + 144 * n : nullptr;
-                    ctrl = ctl12_step(c, zncc, bf, n_valid, Lctl, Hn, a.pattern, a.noise_gain, a.tol,
-                                      a.dp_tol, a.patience, a.predictive, half_scale);
-                }
-                pg_next = (double)pg + mg;
+                for (int k = 0; k < 12; ++k) a.P[12 * n + k] = c.P[k];
             }
+            a.n_iter[n] = it_done;
+            a.status[n] = (int8_t)status;
+            a.zncc[n] = (status == ST_CONVERGED || status == ST_STALLED || status == ST_MAX_ITER) ? c.zncc : NAN;
+            if (a.wg_clock) a.wg_clock[2 * slot + 1] = wall_clock64();
         }
-        __syncthreads();
-    }
-    if (tid == 0) {
-        if (ctrl >= 0) status = ctrl;
-        if constexpr (T3) {
-            for (int k = 0; k < 3; ++k) a.U[3 * n + k] = c.P[9 + k];
-        } else {
-            for (int k = 0; k < 12; ++k) a.P[12 * n + k] = c.P[k];
-        }
-        a.n_iter[n] = it_done;
-        a.status[n] = (int8_t)status;
-        a.zncc[n] = (status == ST_CONVERGED || status == ST_STALLED || status == ST_MAX_ITER) ? c.zncc : NAN;
+        __syncthreads();  // the next node's setup reuses the LDS state
     }
 }
 
@@ -1410,6 +1460,7 @@ __global__ __launch_bounds__(1024) void k_compact(const uint8_t* __restrict__ va
         __syncthreads();
     }
     if (threadIdx.x == 0) *count = base;
+    if (threadIdx.x < 8) count[NODE_COUNTERS + threadIdx.x] = 0;
 }
 
 // Valid nodes grouped by spatial tile (a counting sort on the tile of each node centre, tiles in
@@ -1464,6 +1515,7 @@ __global__ __launch_bounds__(1024) void k_compact_tiled(const uint8_t* __restric
         for (int k = 0; k < 16; ++k) tot += wtot[k];
         *count = tot;
     }
+    if (threadIdx.x < 8) count[NODE_COUNTERS + threadIdx.x] = 0;
 }
 
 // A launch switched off by a device flag (a stopped ADMM loop under graph replay): no node runs.
@@ -1499,6 +1551,18 @@ int brick_tile(int n, int stride, int room) {
     return t8 <= kmax ? t8 : t;
 }
 
+// Workgroups of a persistent launch of `kern` with `lds` bytes of dynamic LDS: as many as are
+// resident on the device at once (a multiple of 8, so workgroup b stays on XCD b % 8), at most nb;
+// nb when the occupancy cannot be read (the node counters make any grid correct).
+int64_t persistent_grid(const void* kern, size_t lds, int64_t nb) {
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, BLOCK, lds) != hipSuccess || per < 1 || cus < 1)
+        return nb;
+    const int64_t g = (int64_t)per * cus / 8 * 8;
+    return std::max<int64_t>(8, std::min<int64_t>(g, nb));
+}
+
 // The fused iteration kernel: LDS bricks (variant 1, when the volume is addressable with 32-bit
 // byte offsets), else memory gathers.
 template <int MODE, bool STORED, bool SEL, bool T3 = false>
@@ -1530,19 +1594,26 @@ void launch_fused(int variant, const IcgnArgs& a, const int64_t* list, const int
     }
     const size_t lds = bits + (size_t)planes * 4 * plane;
     auto kern = &k_icgn12_fused<MODE, STORED, true, BRICK_XY, SEL, T3>;
+    IcgnArgs ka = a;
+    const KernelTimer& tm = timer();
+    ka.wg_clock = (tm.wg_clock && tm.wg_capacity >= nb) ? tm.wg_clock : nullptr;
     if (planes > 0 && hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
         planes = 0;
+    auto kgb = &k_icgn12_fused<MODE, STORED, true, 0, SEL, T3>;
+    auto kgp = &k_icgn12_fused<MODE, STORED, false, 0, SEL, T3>;
+    const int64_t grid = planes > 0 ? persistent_grid(reinterpret_cast<const void*>(kern), lds, nb)
+                                    : persistent_grid(reinterpret_cast<const void*>(buf ? kgb : kgp), bits, nb);
     timer_begin(st);
     if (planes > 0) {
-        hipLaunchKernelGGL(kern, dim3((unsigned)nb), dim3(BLOCK), lds, st, a, list, count, (const double*)mom, planes,
+        hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(BLOCK), lds, st, ka, list, count, (const double*)mom, planes,
                            plane, rows, patches ? LANES_PATCHES : (pair ? LANES_PAIRS : LANES_WALK), tile_x, tile_y);
     } else if (buf) {
-        hipLaunchKernelGGL((k_icgn12_fused<MODE, STORED, true, 0, SEL, T3>), dim3((unsigned)nb), dim3(BLOCK), bits, st,
-                           a, list, count, (const double*)mom, 0, 0, 0, 0, 0, 0);
+        hipLaunchKernelGGL(kgb, dim3((unsigned)grid), dim3(BLOCK), bits, st, ka, list, count, (const double*)mom, 0, 0, 0, 0,
+                           0, 0);
     } else {
-        hipLaunchKernelGGL((k_icgn12_fused<MODE, STORED, false, 0, SEL, T3>), dim3((unsigned)nb), dim3(BLOCK), bits, st,
-                           a, list, count, (const double*)mom, 0, 0, 0, 0, 0, 0);
+        hipLaunchKernelGGL(kgp, dim3((unsigned)grid), dim3(BLOCK), bits, st, ka, list, count, (const double*)mom, 0, 0, 0, 0,
+                           0, 0);
     }
 }
 
@@ -1558,7 +1629,7 @@ void launch_icgn12(int variant, const IcgnArgs& a, const int64_t* list, const in
                                off, gbuf);
         }
     } else {
-        const int64_t nb = ((N + 7) / 8) * 8;
+        const int64_t nb = xcd_grid(N);
         double* mom = (double*)gbuf;  // variant 1 keeps the reference moments where variant 0 keeps scratch rows
         hipLaunchKernelGGL((k_ref_moments<STORED>), dim3((unsigned)nb), dim3(BLOCK), 0, st, a, list, count, mom);
         const size_t bits = sizeof(unsigned) * (size_t)(((a.sub.S + 31) / 32 + 3) & ~3);
@@ -1575,7 +1646,7 @@ void launch_icgn12(int variant, const IcgnArgs& a, const int64_t* list, const in
 template <int MODE, bool STORED>
 void launch_icgn3(int variant, const IcgnArgs& a, const int64_t* list, const int64_t* count, int64_t N, double* mom,
                   int lds_budget, int flags, hipStream_t st) {
-    const int64_t nb = ((N + 7) / 8) * 8;
+    const int64_t nb = xcd_grid(N);
     if (!(flags & ICGN3_REUSE_MOMENTS))
         hipLaunchKernelGGL((k_ref_moments<STORED>), dim3((unsigned)nb), dim3(BLOCK), 0, st, a, list, count, mom);
     const size_t bits = sizeof(unsigned) * (size_t)(((a.sub.S + 31) / 32 + 3) & ~3);
@@ -1664,6 +1735,21 @@ int aldvc_kernel_timing(int on) {
     }
     t.on = on != 0;
     t.have = false;
+    return 0;
+}
+
+int aldvc_workgroup_clock(void* buf, int64_t capacity, int* clock_khz) {
+    clear_error();
+    KernelTimer& t = timer();
+    if (buf && capacity < 1) return fail(-2, "capacity must be positive");
+    t.wg_clock = static_cast<unsigned long long*>(buf);
+    t.wg_capacity = buf ? capacity : 0;
+    if (clock_khz) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(clock_khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess)
+            return fail(-5, "the wall-clock rate could not be read");
+    }
     return 0;
 }
 

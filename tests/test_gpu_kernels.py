@@ -152,6 +152,37 @@ def test_icgn12_lattice_matches_oracle_and_truth(case, variant, mode):
     assert np.median(err) < (0.05 if mode == 1 else 0.1)
 
 
+def test_workgroup_clock_records_every_node_and_changes_nothing(case):
+    """aldvc_workgroup_clock: one start / end pair per entry of the valid-node list, ends after
+    starts, and the solve's outputs bit-identical to a run without the clock."""
+    import ctypes
+
+    import torch
+
+    coords = lattice(case["shape"], 16, 8)
+    half = (8, 8, 8)
+    f, g, gx, gy, gz = case["f"], case["g"], case["gx"], case["gy"], case["gz"]
+    H, L, mf, bf, nv, valid = cpu_ref.precompute_nodes(coords, *half, f, gx, gy, gz, MASK_NONE, 0.5, 1e12)
+    P0 = np.zeros((coords.shape[0], 12))
+    P0[:, 9:] = np.round(truth_at(case, coords))
+    args = (coords, P0, *half, f, gx, gy, gz, MASK_NONE, g, 1, L, mf, bf, valid, 1e-2, 1e-3, 100, 5)
+    plain = K.icgn_12dof_parallel(*args, variant="fused")
+    n = (coords.shape[0] + 1023) // 1024 * 1024  # the launch's grid: the node count rounded up to 1024
+    buf = torch.zeros(2 * n, dtype=torch.int64, device=K.device())
+    khz = ctypes.c_int(0)
+    K.check(K.lib().aldvc_workgroup_clock(ctypes.c_void_p(buf.data_ptr()), n, ctypes.byref(khz)), "clock")
+    try:
+        clocked = K.icgn_12dof_parallel(*args, variant="fused")
+    finally:
+        K.lib().aldvc_workgroup_clock(None, 0, None)
+    for a, b in zip(plain, clocked):
+        assert np.array_equal(a, b, equal_nan=True)
+    c = buf.cpu().numpy().reshape(n, 2)
+    rec = c[:, 0] > 0
+    assert rec.sum() == int(np.sum(valid)) and np.all(c[rec, 1] >= c[rec, 0]) and khz.value > 0
+    assert np.all(c[~rec] == 0)
+
+
 def _rot_z(deg, stretch_z=0.0):
     c, s_ = np.cos(np.radians(deg)), np.sin(np.radians(deg))
     return np.array([[c - 1.0, -s_, 0.0], [s_, c - 1.0, 0.0], [0.0, 0.0, stretch_z]])
