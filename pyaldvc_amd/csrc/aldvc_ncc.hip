@@ -350,10 +350,12 @@ struct Bf16Geom {
 
 __device__ inline float bf16_round_trip(float v) { return (float)(__bf16)v; }
 
-// Shared prologue of both kernels: whether the node has a map, and the template mean and energy
-// (float64). Returns false when the map is the flat-template value -2.
+// Shared prologue of the bfloat16 kernels (NT threads): whether the node has a map, and the
+// template mean and energy (float64). Returns false when the map is the flat-template value -2.
+template <int NT = BLOCK>
 __device__ inline bool ncc_node_stats(const NccArgs& a, const Bf16Geom& q, int64_t node, double& mf, double& ssf) {
-    __shared__ double red[NWAVES];
+    constexpr int NW = NT / 64;
+    __shared__ double red[NW];
     __shared__ double stat[2];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t x0 = a.coords[3 * node], y0 = a.coords[3 * node + 1], z0 = a.coords[3 * node + 2];
@@ -365,18 +367,22 @@ __device__ inline bool ncc_node_stats(const NccArgs& a, const Bf16Geom& q, int64
     const int n_tpl = q.sx * q.sy * q.sz;
     const int64_t tbase = ((z0 - a.hz) * a.ny + (y0 - a.hy)) * (int64_t)a.nx + (x0 - a.hx);
     double sacc = 0.0;
-    for (int v = tid; v < n_tpl; v += BLOCK) {
+    for (int v = tid; v < n_tpl; v += NT) {
         const int k = v / (q.sy * q.sx), r = v - k * q.sy * q.sx, j = r / q.sx, i = r - j * q.sx;
         sacc += a.f[tbase + ((int64_t)k * a.ny + j) * a.nx + i];
     }
     sacc = wave_sum(sacc);
     if (lane == 0) red[wave] = sacc;
     __syncthreads();
-    if (tid == 0) stat[0] = (red[0] + red[1] + red[2] + red[3]) / n_tpl;
+    if (tid == 0) {
+        double t = 0.0;
+        for (int w = 0; w < NW; ++w) t += red[w];
+        stat[0] = t / n_tpl;
+    }
     __syncthreads();
     mf = stat[0];
     double ss = 0.0;
-    for (int v = tid; v < n_tpl; v += BLOCK) {
+    for (int v = tid; v < n_tpl; v += NT) {
         const int k = v / (q.sy * q.sx), r = v - k * q.sy * q.sx, j = r / q.sx, i = r - j * q.sx;
         const double d = (double)a.f[tbase + ((int64_t)k * a.ny + j) * a.nx + i] - mf;
         ss += d * d;
@@ -385,7 +391,11 @@ __device__ inline bool ncc_node_stats(const NccArgs& a, const Bf16Geom& q, int64
     __syncthreads();
     if (lane == 0) red[wave] = ss;
     __syncthreads();
-    if (tid == 0) stat[1] = red[0] + red[1] + red[2] + red[3];
+    if (tid == 0) {
+        double t = 0.0;
+        for (int w = 0; w < NW; ++w) t += red[w];
+        stat[1] = t;
+    }
     __syncthreads();
     ssf = stat[1];
     return ssf >= 1e-20;
@@ -395,33 +405,55 @@ __device__ inline bool ncc_node_stats(const NccArgs& a, const Bf16Geom& q, int64
 // 2-D box sums (x then y, sliding in runs of 8 outputs) are added to running sums over the last sz
 // planes; the plane leaving the box (sz planes back) has its 2-D sums recomputed and subtracted,
 // and every plane from sz - 1 on completes one output plane. A thread owns the offsets
-// tid + 256 k of the plane.
-constexpr int VPT = 9;  // offsets per thread and plane: vy * vx <= 256 * VPT (radii up to 23)
+// tid + 256 k of the plane. The global loads of the next entering and leaving planes are issued
+// into registers before the current planes' sums, so their latency is hidden behind that work.
+constexpr int VPT = 9;   // offsets per thread and plane: vy * vx <= 256 * VPT (radii up to 23)
+constexpr int VPL = 16;  // window values per thread and plane: wy * wx <= 256 * VPL
 
-__device__ inline void box2d(const NccArgs& a, const Bf16Geom& q, int64_t node, int z, float c, float* Pf, float* R1,
-                             float* R2, float* P1, float* P2) {
-    const int tid = threadIdx.x;
+// plane z of the node's window, less c, into registers (element tid + 256 u, row-major)
+__device__ inline void box_load(const NccArgs& a, const Bf16Geom& q, int64_t node, int z, float (&v)[VPL]) {
     const int64_t xb = a.wx0[node], yb = a.wy0[node], zb = a.wz0[node];
-    const int nseg_x = (q.vx + 7) / 8, nseg_y = (q.vy + 7) / 8;
-    __syncthreads();  // the previous plane's readers are done
     const float* gp = a.g + ((zb + z) * a.ny + yb) * (int64_t)a.nx + xb;
-    for (int i = tid; i < q.wy * q.wx; i += BLOCK) {
-        const int y = i / q.wx, x = i - y * q.wx;
-        Pf[i] = gp[(int64_t)y * a.nx + x] - c;
+    const int wpl = q.wy * q.wx;
+    const float inv_wx = 1.0f / (float)q.wx;  // i < 2^22: the float floor of (i + 0.5) / wx is exact
+#pragma unroll
+    for (int u = 0; u < VPL; ++u) {
+        const int i = threadIdx.x + BLOCK * u;
+        if (i < wpl) {
+            const int y = (int)(((float)i + 0.5f) * inv_wx), x = i - y * q.wx;
+            v[u] = gp[(int64_t)y * a.nx + x];
+        }
+    }
+}
+
+// 2-D box sums of one plane held in registers (P1 of W - c, P2 of its squares)
+__device__ inline void box2d(const Bf16Geom& q, const float (&v)[VPL], float c, float* Pf, float* R1, float* R2,
+                             float* P1, float* P2) {
+    const int tid = threadIdx.x;
+    const int nseg_x = (q.vx + 7) / 8, nseg_y = (q.vy + 7) / 8;
+    const int wpl = q.wy * q.wx;
+    __syncthreads();  // the previous plane's readers are done
+#pragma unroll
+    for (int u = 0; u < VPL; ++u) {
+        const int i = tid + BLOCK * u;
+        if (i < wpl) Pf[i] = v[u] - c;
     }
     __syncthreads();
     for (int t = tid; t < q.wy * nseg_x; t += BLOCK) {  // row sums, 8 outputs per item
         const int y = t / nseg_x, ox0 = 8 * (t - y * nseg_x);
         const float* row = Pf + y * q.wx;
         float r1 = 0.0f, r2 = 0.0f;
+#pragma unroll 8
         for (int x = 0; x < q.sx; ++x) {
-            const float v = row[ox0 + x];
-            r1 += v;
-            r2 += v * v;
+            const float w = row[ox0 + x];
+            r1 += w;
+            r2 += w * w;
         }
         R1[y * q.vx + ox0] = r1;
         R2[y * q.vx + ox0] = r2;
-        for (int ox = ox0 + 1; ox < min(ox0 + 8, q.vx); ++ox) {
+#pragma unroll
+        for (int ox = ox0 + 1; ox < ox0 + 8; ++ox) {
+            if (ox >= q.vx) break;
             const float vin = row[ox + q.sx - 1], vout = row[ox - 1];
             r1 += vin - vout;
             r2 += vin * vin - vout * vout;
@@ -433,13 +465,16 @@ __device__ inline void box2d(const NccArgs& a, const Bf16Geom& q, int64_t node, 
     for (int t = tid; t < q.vx * nseg_y; t += BLOCK) {  // column sums, 8 outputs per item
         const int ox = t / nseg_y, oy0 = 8 * (t - ox * nseg_y);
         float r1 = 0.0f, r2 = 0.0f;
+#pragma unroll 8
         for (int y = 0; y < q.sy; ++y) {
             r1 += R1[(oy0 + y) * q.vx + ox];
             r2 += R2[(oy0 + y) * q.vx + ox];
         }
         P1[oy0 * q.vx + ox] = r1;
         P2[oy0 * q.vx + ox] = r2;
-        for (int oy = oy0 + 1; oy < min(oy0 + 8, q.vy); ++oy) {
+#pragma unroll
+        for (int oy = oy0 + 1; oy < oy0 + 8; ++oy) {
+            if (oy >= q.vy) break;
             r1 += R1[(oy + q.sy - 1) * q.vx + ox] - R1[(oy - 1) * q.vx + ox];
             r2 += R2[(oy + q.sy - 1) * q.vx + ox] - R2[(oy - 1) * q.vx + ox];
             P1[oy * q.vx + ox] = r1;
@@ -449,7 +484,7 @@ __device__ inline void box2d(const NccArgs& a, const Bf16Geom& q, int64_t node, 
     __syncthreads();
 }
 
-__global__ __launch_bounds__(BLOCK) void k_box_var(NccArgs a, Bf16Geom q) {
+__global__ __launch_bounds__(BLOCK, 3) void k_box_var(NccArgs a, Bf16Geom q) {
     extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
     const int tid = threadIdx.x;
     const int64_t node = blockIdx.x;
@@ -467,8 +502,12 @@ __global__ __launch_bounds__(BLOCK) void k_box_var(NccArgs a, Bf16Geom q) {
     float S1[VPT], S2[VPT];
 #pragma unroll
     for (int k = 0; k < VPT; ++k) S1[k] = S2[k] = 0.0f;
+    float vin[VPL], vout[VPL], nin[VPL], nout[VPL];
+    box_load(a, q, node, 0, vin);
     for (int z = 0; z < q.wz; ++z) {
-        box2d(a, q, node, z, c, Pf, R1, R2, P1, P2);
+        if (z + 1 < q.wz) box_load(a, q, node, z + 1, nin);  // next planes in flight
+        if (z + 1 >= q.sz && z + 1 < q.wz) box_load(a, q, node, z + 1 - q.sz, nout);
+        box2d(q, vin, c, Pf, R1, R2, P1, P2);
 #pragma unroll
         for (int k = 0; k < VPT; ++k) {
             const int i = tid + BLOCK * k;
@@ -478,7 +517,7 @@ __global__ __launch_bounds__(BLOCK) void k_box_var(NccArgs a, Bf16Geom q) {
             }
         }
         if (z >= q.sz) {  // the plane that left the box
-            box2d(a, q, node, z - q.sz, c, Pf, R1, R2, P1, P2);
+            box2d(q, vout, c, Pf, R1, R2, P1, P2);
 #pragma unroll
             for (int k = 0; k < VPT; ++k) {
                 const int i = tid + BLOCK * k;
@@ -495,6 +534,11 @@ __global__ __launch_bounds__(BLOCK) void k_box_var(NccArgs a, Bf16Geom q) {
                 const int i = tid + BLOCK * k;
                 if (i < nplane) op[i] = S2[k] - S1[k] * S1[k] * inv_n;
             }
+        }
+#pragma unroll
+        for (int u = 0; u < VPL; ++u) {
+            vin[u] = nin[u];
+            vout[u] = nout[u];
         }
     }
 }
@@ -684,6 +728,250 @@ __global__ __launch_bounds__(256) void k_zncc_map_bf16(NccArgs a, Bf16Geom q) {
                 }
             }
         }
+}
+
+// ---------------------------------------------------------------------------- BF16 MFMA maps, skewed planes
+// A second tiling of the same correlation for templates up to 32 voxels wide and up to 48 x
+// offsets (25^3 templates and 33^3 offsets at radius 16) that wastes less of each MFMA than the
+// banded one above (whose K slices are 64 wide for 25 taps):
+//   M = 16 x offsets ox (a column tile c), K = 32 template columns tx (zero from sx on),
+//   N = 16 = 8 output planes oz0 + j x 2 output rows 2G + d (d = 0, 1).
+// Window row y of plane z, A[m][k] = W~[z][y][16c + m + k], meets output row oy through template
+// row y - oy and output plane oz0 + j through template plane z - oz0 - j, so for the accumulator
+// tile (G, c) (rows 2G and 2G + 1) it is multiplied by
+//   B_t[k][(j, d)] = T~[z - oz0 - j][t - d][k],   t = y - 2G in [0, SY],
+// zero where that template plane or row does not exist. A wave holds the SY + 1 fragments B_t of
+// the current window plane and its accumulator tiles in registers, and reads each window row's A
+// fragment from LDS once for every tile of its column that the row reaches. The row and tile loops
+// are unrolled (SY, VY and the tile split are template parameters), so every register index is
+// static and only the (y, G) pairs with 0 <= y - 2G <= SY are issued. Work split over the NC + 1
+// waves of a workgroup: wave c < NC takes column tile c for the tiles G < GA, wave NC takes every
+// column tile for G >= GA, with GA chosen to even the MFMA counts out (13 / 13 / 13 / 12 tiles at
+// VY = 33). Window planes are double-buffered in LDS as bf16 W - c, twice (the copy shifted by
+// one element makes every lane's 8-element row start 4-byte aligned); template planes sit in a
+// ring of SK_MZ + 1 slots (the one being loaded included). Output planes in workgroups of SK_MZ.
+constexpr int SK_MZ = 8;        // output planes per workgroup
+constexpr int SK_RING = SK_MZ + 1;  // template plane slots
+constexpr int SK_PITCH = 80;    // staged window row (bf16): three column tiles + 32 taps + shift
+constexpr int SK_TP = 32;       // staged template row (bf16), zero from sx on
+
+// bf16 elements of one template slot of `rows` rows plus a zero row on either side (rows -1 and
+// `rows`, read by the B fragments that reach past the template): 4 mod 32 dwords, so the 16-byte B
+// reads of the lanes of one column group (eight consecutive template planes, two rows) fall on
+// different banks
+__host__ __device__ constexpr int sk_slot_elems(int rows) {
+    const int dw = (rows + 2) * SK_TP / 2;
+    return 2 * (dw + ((4 - dw % 32) + 32) % 32);
+}
+constexpr int SK_NT = 256;      // up to three column waves and the remainder wave
+constexpr int SK_PRE = 13;      // window values per lane prefetched for the next plane
+constexpr int SK_TPRE = 3;      // template values per lane prefetched for the next plane
+
+// bf16 elements of one copy of a staged window plane of `rows` rows: 16 mod 32 dwords, so the even-
+// and odd-column lanes of a half-wave, which read the two copies, use different banks (a half-wave
+// of one A read covers 12 consecutive dwords of each copy)
+__host__ __device__ constexpr int sk_copy_elems(int rows) {
+    const int dw = rows * SK_PITCH / 2;
+    return 2 * (dw + ((16 - dw % 32) + 32) % 32);
+}
+
+template <int SY, int VY, int NC>
+struct SkewShape {
+    static constexpr int NG = (VY + 1) / 2;               // row tiles (output rows 2G, 2G + 1)
+    static constexpr int GA = (NC * NG + NC) / (NC + 1);  // tiles of the column waves
+    static constexpr int NB = SY + 1;                     // B fragments per window plane
+    static constexpr int WROWS = 2 * (NG - 1) + SY + 1;   // window rows reached (rows from wy on are zero)
+    static constexpr int COPY = sk_copy_elems(WROWS);
+};
+
+// The MFMAs of one window plane for the tiles G0 .. G0 + NT - 1 (of NG) of NCOL column tiles.
+template <int SY, int NG, int G0, int NT, int NCOL>
+__device__ inline void skew_plane(f32x4_t (&acc)[NCOL][NT], const bf16x8_t (&bfr)[SY + 1], const uint32_t* ap) {
+    constexpr int ROWS = 2 * (NT - 1) + SY + 1;
+#pragma unroll
+    for (int yr = 0; yr < ROWS; ++yr) {  // window row 2 G0 + yr
+        bf16x8_t afr[NCOL];
+#pragma unroll
+        for (int cc = 0; cc < NCOL; ++cc) {
+            const uint32_t* p = ap + (2 * G0 + yr) * (SK_PITCH / 2) + 8 * cc;
+            const uint32_t v4[4] = {p[0], p[1], p[2], p[3]};
+            __builtin_memcpy(&afr[cc], v4, sizeof(bf16x8_t));
+        }
+#pragma unroll
+        for (int gl = 0; gl < NT; ++gl) {
+            const int t = yr - 2 * gl;
+            if (t >= 0 && t <= SY && G0 + gl < NG) {
+#pragma unroll
+                for (int cc = 0; cc < NCOL; ++cc)
+                    acc[cc][gl] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr[cc], bfr[t], acc[cc][gl], 0, 0, 0);
+            }
+        }
+    }
+}
+
+template <int SY, int VY, int NC>
+__global__ __launch_bounds__(SK_NT, 2) void k_zncc_map_bf16_skew(NccArgs a, Bf16Geom q, int nzt) {
+    using S = SkewShape<SY, VY, NC>;
+    constexpr int COPY = S::COPY;
+    extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t node = blockIdx.x / nzt;
+    const int oz0 = (int)(blockIdx.x % nzt) * SK_MZ;
+    const int nzp = min(SK_MZ, q.vz - oz0);  // output planes of this workgroup
+    const int64_t plane_out = (int64_t)q.vy * q.vx;
+    float* out = a.out + node * plane_out * q.vz + (int64_t)oz0 * plane_out;
+    double mf, ssf;
+    if (!ncc_node_stats<SK_NT>(a, q, node, mf, ssf)) {
+        for (int64_t i = tid; i < nzp * plane_out; i += SK_NT) out[i] = -2.0f;
+        return;
+    }
+    const int64_t x0 = a.coords[3 * node], y0 = a.coords[3 * node + 1], z0 = a.coords[3 * node + 2];
+    const int64_t xb = a.wx0[node], yb = a.wy0[node], zb = a.wz0[node];
+    const int64_t tbase = ((z0 - a.hz) * a.ny + (y0 - a.hy)) * (int64_t)a.nx + (x0 - a.hx);
+    const float cw = (float)mf;
+    __bf16* win = reinterpret_cast<__bf16*>(lds);  // [buffer][copy][WROWS][SK_PITCH]
+    // [slot][row -1 .. SY][SK_TP], slots SLOT elements apart; slot SK_RING stays zero (the B source
+    // of lanes whose template plane does not exist)
+    __bf16* tring = win + 4 * COPY;
+    constexpr int SLOT = sk_slot_elems(SY);
+    for (int i = tid; i < 4 * COPY + (SK_RING + 1) * SLOT; i += SK_NT) win[i] = (__bf16)0.0f;
+    __syncthreads();
+    const int wpl = q.wy * q.wx, tpl_n = SY * q.sx;
+    // row of element i of a window or template plane as a float product: (i + 0.5) / w lies at least
+    // 0.5 / w from an integer and i < 2^22, so the rounding cannot change the floor
+    const float inv_wx = 1.0f / (float)q.wx, inv_sx = 1.0f / (float)q.sx;
+    float pre[SK_PRE], tpre[SK_TPRE];
+    auto load_plane = [&](int z) {
+        const float* gp = a.g + ((zb + z) * a.ny + yb) * (int64_t)a.nx + xb;
+#pragma unroll
+        for (int u = 0; u < SK_PRE; ++u) {
+            const int i = tid + SK_NT * u;
+            if (i < wpl) {
+                const int y = (int)(((float)i + 0.5f) * inv_wx), x = i - y * q.wx;
+                pre[u] = gp[(int64_t)y * a.nx + x];
+            }
+        }
+        const int tz = z - oz0;
+        if (tz < q.sz) {
+            const float* fp = a.f + tbase + (int64_t)tz * a.ny * a.nx;
+#pragma unroll
+            for (int u = 0; u < SK_TPRE; ++u) {
+                const int i = tid + SK_NT * u;
+                if (i < tpl_n) {
+                    const int y = (int)(((float)i + 0.5f) * inv_sx), x = i - y * q.sx;
+                    tpre[u] = fp[(int64_t)y * a.nx + x];
+                }
+            }
+        }
+    };
+    auto store_plane = [&](int z) {
+        __bf16* w0 = win + (z & 1) * 2 * COPY;
+#pragma unroll
+        for (int u = 0; u < SK_PRE; ++u) {
+            const int i = tid + SK_NT * u;
+            if (i < wpl) {
+                const int y = (int)(((float)i + 0.5f) * inv_wx), x = i - y * q.wx;
+                const __bf16 v = (__bf16)(pre[u] - cw);
+                w0[y * SK_PITCH + x] = v;
+                w0[COPY + y * SK_PITCH + x + 1] = v;
+            }
+        }
+        const int tz = z - oz0;
+        if (tz < q.sz) {
+            __bf16* tp = tring + (tz % SK_RING) * SLOT + SK_TP;  // row 0 of the slot
+#pragma unroll
+            for (int u = 0; u < SK_TPRE; ++u) {
+                const int i = tid + SK_NT * u;
+                if (i < tpl_n) {
+                    const int y = (int)(((float)i + 0.5f) * inv_sx), x = i - y * q.sx;
+                    tp[y * SK_TP + x] = (__bf16)(float)((double)tpre[u] - mf);
+                }
+            }
+        }
+    };
+    const int n = lane & 15, j = n >> 1, d = n & 1, g8 = 8 * (lane >> 4);
+    const int m = lane & 15;
+    const int z_end = min(oz0 + nzp - 1 + q.sz, q.wz);
+    const float sq_ssf = (float)sqrt(ssf);
+    // the whole plane loop for one wave role: NT tiles from G0 over NCOL column tiles from c0
+    auto run = [&](auto& acc, auto g0_tag, int c0) {
+        constexpr int G0 = decltype(g0_tag)::value;
+        constexpr int NCOL = sizeof(acc) / sizeof(acc[0]);
+        constexpr int NT = sizeof(acc[0]) / sizeof(f32x4_t);
+#pragma unroll
+        for (int cc = 0; cc < NCOL; ++cc)
+#pragma unroll
+            for (int gl = 0; gl < NT; ++gl) acc[cc][gl] = (f32x4_t){0.f, 0.f, 0.f, 0.f};
+        // this lane's A source for column tile c0, row 0: columns 16 c0 + m + g8 .. + 8, from the
+        // copy that makes the start even
+        const int acol = 16 * c0 + m + g8 + (m & 1);
+        load_plane(oz0);
+        store_plane(oz0);
+        __syncthreads();
+        for (int z = oz0; z < z_end; ++z) {
+            const bool more = z + 1 < z_end;
+            if (more) load_plane(z + 1);  // in flight during this plane's MFMAs
+            const int tz = z - oz0 - j;
+            const bool zok = j < nzp && tz >= 0 && tz < q.sz;
+            // row t - d of the lane's template plane (rows -1 and SY are the slot's zero rows)
+            const __bf16* tp = tring + (zok ? tz % SK_RING : SK_RING) * SLOT + (1 - d) * SK_TP + g8;
+            bf16x8_t bfr[SY + 1];
+#pragma unroll
+            for (int t = 0; t <= SY; ++t) __builtin_memcpy(&bfr[t], tp + t * SK_TP, sizeof(bf16x8_t));
+            const uint32_t* ap =
+                reinterpret_cast<const uint32_t*>(win + (z & 1) * 2 * COPY + (m & 1) * COPY + acol);
+            skew_plane<SY, S::NG, G0, NT, NCOL>(acc, bfr, ap);
+            if (more) store_plane(z + 1);  // the other window buffer and a free template slot
+            __syncthreads();
+        }
+        // epilogue: the variance from k_box_var in place, replaced by the ZNCC
+#pragma unroll
+        for (int cc = 0; cc < NCOL; ++cc)
+#pragma unroll
+            for (int gl = 0; gl < NT; ++gl) {
+                const int oy = 2 * (G0 + gl) + d;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int ox = 16 * (c0 + cc) + 4 * (lane >> 4) + i;
+                    if (j < nzp && oy < q.vy && ox < q.vx) {
+                        float* o = out + (int64_t)j * plane_out + (int64_t)oy * q.vx + ox;
+                        const float var = *o;
+                        float val = 0.0f;
+                        if (var >= 1e-20f) {
+                            val = acc[cc][gl][i] / (sqrtf(var) * sq_ssf);
+                            val = val > 1.0f ? 1.0f : (val < -1.0f ? -1.0f : val);
+                        }
+                        *o = val;
+                    }
+                }
+            }
+    };
+    if (wave < NC) {
+        f32x4_t acc[1][S::GA];
+        run(acc, std::integral_constant<int, 0>{}, wave);
+    } else if (wave == NC) {
+        f32x4_t acc[NC][S::NG - S::GA];
+        run(acc, std::integral_constant<int, S::GA>{}, 0);
+    } else {  // no tiles: staging and barriers only
+        f32x4_t acc[1][1];
+        (void)acc;
+        load_plane(oz0);
+        store_plane(oz0);
+        __syncthreads();
+        for (int z = oz0; z < z_end; ++z) {
+            const bool more = z + 1 < z_end;
+            if (more) {
+                load_plane(z + 1);
+                store_plane(z + 1);
+            }
+            __syncthreads();
+        }
+    }
+}
+
+template <int SY, int VY, int NC>
+size_t skew_lds_bytes() {
+    return sizeof(__bf16) * (4 * (size_t)SkewShape<SY, VY, NC>::COPY + (size_t)(SK_RING + 1) * sk_slot_elems(SY));
 }
 
 size_t bf16_lds_bytes(const Bf16Geom& q) {
@@ -881,6 +1169,7 @@ int aldvc_zncc_map(const float* f, const float* g, int64_t nz, int64_t ny, int64
         if ((int64_t)q.wy * q.wx > (int64_t)BLOCK * PRE) return fail(-2, "window plane too large for the prefetch");
         if ((int64_t)q.sy * q.sx > (int64_t)BLOCK * 4) return fail(-2, "template plane too large for the prefetch");
         if ((int64_t)q.vy * q.vx > (int64_t)BLOCK * VPT) return fail(-2, "search plane too large for the box sums");
+        if ((int64_t)q.wy * q.wx > (int64_t)BLOCK * VPL) return fail(-2, "window plane too large for the box sums");
         const int64_t nblk = n * ((q.vz + MG - 1) / MG);
         const int64_t nbox = n;
         if (nblk > INT32_MAX) return fail(-2, "too many workgroups for one launch");
@@ -889,6 +1178,26 @@ int aldvc_zncc_map(const float* f, const float* g, int64_t nz, int64_t ny, int64
             hipFuncSetAttribute((const void*)k_box_var, hipFuncAttributeMaxDynamicSharedMemorySize, (int)blds) != hipSuccess)
             return fail(-5, "could not raise the LDS limit");
         hipLaunchKernelGGL(k_box_var, dim3((unsigned)nbox), dim3(BLOCK), blds, stream, a, q);
+        // the skewed-plane tiling where its geometry is instantiated (templates up to 32 voxels wide,
+        // up to 48 x offsets, the window plane within the per-lane prefetch)
+        if (q.sx <= SK_TP && (int64_t)q.wy * q.wx <= (int64_t)SK_NT * SK_PRE &&
+            (int64_t)q.sy * q.sx <= (int64_t)SK_NT * SK_TPRE) {
+            const int nzt = (q.vz + SK_MZ - 1) / SK_MZ;
+            if (n * nzt > INT32_MAX) return fail(-2, "too many workgroups for one launch");
+#define ALDVC_SKEW(SYV, VYV, NCV)                                                                           \
+    if (q.sy == SYV && q.vy == VYV && q.nxt == NCV) {                                                      \
+        const size_t sl = skew_lds_bytes<SYV, VYV, NCV>();                                                  \
+        const void* sf = (const void*)k_zncc_map_bf16_skew<SYV, VYV, NCV>;                                  \
+        if (sl > 64 * 1024 && hipFuncSetAttribute(sf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sl) != hipSuccess) \
+            return fail(-5, "could not raise the LDS limit");                                              \
+        hipLaunchKernelGGL((k_zncc_map_bf16_skew<SYV, VYV, NCV>), dim3((unsigned)(n * nzt)), dim3(SK_NT), sl, stream, a, \
+                           q, nzt);                                                                        \
+        return launch_status();                                                                            \
+    }
+            ALDVC_SKEW(25, 33, 3)
+            ALDVC_SKEW(13, 17, 2)
+#undef ALDVC_SKEW
+        }
         const void* fn = nullptr;
 #define ALDVC_BF16(NTV, KSV)                                                                                \
     if (nt <= NTV && ks == KSV) {                                                                          \

@@ -420,6 +420,27 @@ def test_ncc_search_bf16_lattice_peaks(case):
     assert np.median(d) < 0.02
 
 
+def test_zncc_map_bf16_skewed_planes_small_shape_matches_fp32():
+    """13^3 templates at radius 8 (17^3 offsets: two column tiles, a last group of one output
+    plane) through the skewed-plane tiling, and a node without a template (t_ok = 0, map -2):
+    within bf16 rounding of the float32 maps, same integer peaks."""
+    from pyaldvc_amd import synthetic as syn
+
+    shape = (96, 96, 96)
+    f_raw, g_raw = syn.bead_pair_np(shape, int(0.004 * np.prod(shape)), 1.5, syn.translation((2.3, -1.6, 3.2)), seed=5)
+    f, g = syn.normalize_np(f_raw), syn.normalize_np(g_raw)
+    coords = np.array([[48, 48, 48], [30, 40, 60], [62, 55, 35], [40, 66, 50], [50, 30, 30]], dtype=np.int64)
+    h, r = 6, 8
+    wx0, wy0, wz0 = (coords[:, k] - h - r for k in range(3))
+    ok = np.array([1, 1, 0, 1, 1], np.uint8)
+    m32 = K.zncc_direct(f, g, coords, h, h, h, r, r, r, wx0, wy0, wz0, ok)
+    m16 = K.zncc_direct(f, g, coords, h, h, h, r, r, r, wx0, wy0, wz0, ok, precision="bf16")
+    assert np.all(m16[2] == -2.0) and np.all(m32[2] == -2.0)
+    assert np.max(np.abs(m16 - m32)) < 1e-2
+    for n in (0, 1, 3, 4):
+        assert np.argmax(m16[n]) == np.argmax(m32[n])
+
+
 @pytest.mark.parametrize("radius", [(16, 16, 16), (9, 12, 5)])
 def test_zncc_map_bf16_large_template_matches_fp32(radius):
     """24^3 subsets (25^3 templates, two K slices) with 16-voxel radii, the shape of the BF16
