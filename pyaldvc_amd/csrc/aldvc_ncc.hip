@@ -969,6 +969,193 @@ __global__ __launch_bounds__(SK_NT, 2) void k_zncc_map_bf16_skew(NccArgs a, Bf16
     }
 }
 
+// The same skewed-plane tiling in float32 on v_mfma_f32_16x16x4f32 (K = 4 template columns per
+// instruction, ceil(sx / 4) slices): the float32 maps of the large-radius searches, so that the
+// bfloat16 speed-up is quoted against the same design in full precision. A and B are one float per
+// lane (A[m][k] = W~[z][y][16c + m + 4s + k], B_t[k][(j, d)] = T~[z - oz0 - j][t - d][4s + k]), read
+// as aligned dwords, so the window needs no shifted copy; the slice loop is outermost within a plane
+// so only one slice's SY + 1 B values are live.
+constexpr int SKF_PITCH = 80;  // staged window row (float): three column tiles + 32 taps
+
+template <int SY, int VY, int NC>
+struct SkewShapeF32 {
+    using S = SkewShape<SY, VY, NC>;
+    static constexpr int WIN = S::WROWS * SKF_PITCH + 16;          // floats per window buffer
+    static constexpr int SLOT = (SY + 2) * SK_TP + 4;               // floats per template slot (rows -1 .. SY)
+};
+
+template <int SY, int NG, int G0, int NT, int NCOL>
+__device__ inline void skew_plane_f32(f32x4_t (&acc)[NCOL][NT], const float (&bfr)[SY + 1], const float* ap) {
+    constexpr int ROWS = 2 * (NT - 1) + SY + 1;
+#pragma unroll
+    for (int yr = 0; yr < ROWS; ++yr) {
+        float afr[NCOL];
+#pragma unroll
+        for (int cc = 0; cc < NCOL; ++cc) afr[cc] = ap[(2 * G0 + yr) * SKF_PITCH + 16 * cc];
+#pragma unroll
+        for (int gl = 0; gl < NT; ++gl) {
+            const int t = yr - 2 * gl;
+            if (t >= 0 && t <= SY && G0 + gl < NG) {
+#pragma unroll
+                for (int cc = 0; cc < NCOL; ++cc)
+                    acc[cc][gl] = __builtin_amdgcn_mfma_f32_16x16x4f32(afr[cc], bfr[t], acc[cc][gl], 0, 0, 0);
+            }
+        }
+    }
+}
+
+template <int SY, int VY, int NC>
+__global__ __launch_bounds__(SK_NT, 2) void k_zncc_map_f32_skew(NccArgs a, Bf16Geom q, int nzt) {
+    using S = SkewShape<SY, VY, NC>;
+    using F = SkewShapeF32<SY, VY, NC>;
+    extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t node = blockIdx.x / nzt;
+    const int oz0 = (int)(blockIdx.x % nzt) * SK_MZ;
+    const int nzp = min(SK_MZ, q.vz - oz0);
+    const int64_t plane_out = (int64_t)q.vy * q.vx;
+    float* out = a.out + node * plane_out * q.vz + (int64_t)oz0 * plane_out;
+    double mf, ssf;
+    if (!ncc_node_stats<SK_NT>(a, q, node, mf, ssf)) {
+        for (int64_t i = tid; i < nzp * plane_out; i += SK_NT) out[i] = -2.0f;
+        return;
+    }
+    const int64_t x0 = a.coords[3 * node], y0 = a.coords[3 * node + 1], z0 = a.coords[3 * node + 2];
+    const int64_t xb = a.wx0[node], yb = a.wy0[node], zb = a.wz0[node];
+    const int64_t tbase = ((z0 - a.hz) * a.ny + (y0 - a.hy)) * (int64_t)a.nx + (x0 - a.hx);
+    const float cw = (float)mf;
+    float* win = reinterpret_cast<float*>(lds);  // [buffer][WROWS][SKF_PITCH]
+    float* tring = win + 2 * F::WIN;             // [slot][row -1 .. SY][SK_TP]; slot SK_RING stays zero
+    for (int i = tid; i < 2 * F::WIN + (SK_RING + 1) * F::SLOT; i += SK_NT) win[i] = 0.0f;
+    __syncthreads();
+    const int wpl = q.wy * q.wx, tpl_n = SY * q.sx;
+    const float inv_wx = 1.0f / (float)q.wx, inv_sx = 1.0f / (float)q.sx;  // exact floors for i < 2^22
+    float pre[SK_PRE], tpre[SK_TPRE];
+    auto load_plane = [&](int z) {
+        const float* gp = a.g + ((zb + z) * a.ny + yb) * (int64_t)a.nx + xb;
+#pragma unroll
+        for (int u = 0; u < SK_PRE; ++u) {
+            const int i = tid + SK_NT * u;
+            if (i < wpl) {
+                const int y = (int)(((float)i + 0.5f) * inv_wx), x = i - y * q.wx;
+                pre[u] = gp[(int64_t)y * a.nx + x];
+            }
+        }
+        const int tz = z - oz0;
+        if (tz < q.sz) {
+            const float* fp = a.f + tbase + (int64_t)tz * a.ny * a.nx;
+#pragma unroll
+            for (int u = 0; u < SK_TPRE; ++u) {
+                const int i = tid + SK_NT * u;
+                if (i < tpl_n) {
+                    const int y = (int)(((float)i + 0.5f) * inv_sx), x = i - y * q.sx;
+                    tpre[u] = fp[(int64_t)y * a.nx + x];
+                }
+            }
+        }
+    };
+    auto store_plane = [&](int z) {
+        float* w0 = win + (z & 1) * F::WIN;
+#pragma unroll
+        for (int u = 0; u < SK_PRE; ++u) {
+            const int i = tid + SK_NT * u;
+            if (i < wpl) {
+                const int y = (int)(((float)i + 0.5f) * inv_wx), x = i - y * q.wx;
+                w0[y * SKF_PITCH + x] = pre[u] - cw;
+            }
+        }
+        const int tz = z - oz0;
+        if (tz < q.sz) {
+            float* tp = tring + (tz % SK_RING) * F::SLOT + SK_TP;  // row 0 of the slot
+#pragma unroll
+            for (int u = 0; u < SK_TPRE; ++u) {
+                const int i = tid + SK_NT * u;
+                if (i < tpl_n) {
+                    const int y = (int)(((float)i + 0.5f) * inv_sx), x = i - y * q.sx;
+                    tp[y * SK_TP + x] = (float)((double)tpre[u] - mf);
+                }
+            }
+        }
+    };
+    const int n = lane & 15, j = n >> 1, d = n & 1, kk = lane >> 4, m = lane & 15;
+    const int z_end = min(oz0 + nzp - 1 + q.sz, q.wz);
+    const int nslices = (q.sx + 3) / 4;
+    const float sq_ssf = (float)sqrt(ssf);
+    auto run = [&](auto& acc, auto g0_tag, int c0) {
+        constexpr int G0 = decltype(g0_tag)::value;
+        constexpr int NCOL = sizeof(acc) / sizeof(acc[0]);
+        constexpr int NT = sizeof(acc[0]) / sizeof(f32x4_t);
+#pragma unroll
+        for (int cc = 0; cc < NCOL; ++cc)
+#pragma unroll
+            for (int gl = 0; gl < NT; ++gl) acc[cc][gl] = (f32x4_t){0.f, 0.f, 0.f, 0.f};
+        load_plane(oz0);
+        store_plane(oz0);
+        __syncthreads();
+        for (int z = oz0; z < z_end; ++z) {
+            const bool more = z + 1 < z_end;
+            if (more) load_plane(z + 1);
+            const int tz = z - oz0 - j;
+            const bool zok = j < nzp && tz >= 0 && tz < q.sz;
+            const float* tp = tring + (zok ? tz % SK_RING : SK_RING) * F::SLOT + (1 - d) * SK_TP + kk;
+            const float* ap = win + (z & 1) * F::WIN + 16 * c0 + m + kk;
+#pragma unroll 1
+            for (int sl = 0; sl < nslices; ++sl) {
+                float bfr[SY + 1];
+#pragma unroll
+                for (int t = 0; t <= SY; ++t) bfr[t] = tp[t * SK_TP + 4 * sl];
+                skew_plane_f32<SY, S::NG, G0, NT, NCOL>(acc, bfr, ap + 4 * sl);
+            }
+            if (more) store_plane(z + 1);
+            __syncthreads();
+        }
+#pragma unroll
+        for (int cc = 0; cc < NCOL; ++cc)
+#pragma unroll
+            for (int gl = 0; gl < NT; ++gl) {
+                const int oy = 2 * (G0 + gl) + d;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int ox = 16 * (c0 + cc) + 4 * (lane >> 4) + i;
+                    if (j < nzp && oy < q.vy && ox < q.vx) {
+                        float* o = out + (int64_t)j * plane_out + (int64_t)oy * q.vx + ox;
+                        const float var = *o;  // written by k_box_var
+                        float val = 0.0f;
+                        if (var >= 1e-20f) {
+                            val = acc[cc][gl][i] / (sqrtf(var) * sq_ssf);
+                            val = val > 1.0f ? 1.0f : (val < -1.0f ? -1.0f : val);
+                        }
+                        *o = val;
+                    }
+                }
+            }
+    };
+    if (wave < NC) {
+        f32x4_t acc[1][S::GA];
+        run(acc, std::integral_constant<int, 0>{}, wave);
+    } else if (wave == NC) {
+        f32x4_t acc[NC][S::NG - S::GA];
+        run(acc, std::integral_constant<int, S::GA>{}, 0);
+    } else {
+        load_plane(oz0);
+        store_plane(oz0);
+        __syncthreads();
+        for (int z = oz0; z < z_end; ++z) {
+            if (z + 1 < z_end) {
+                load_plane(z + 1);
+                store_plane(z + 1);
+            }
+            __syncthreads();
+        }
+    }
+}
+
+template <int SY, int VY, int NC>
+size_t skew_f32_lds_bytes() {
+    using F = SkewShapeF32<SY, VY, NC>;
+    return sizeof(float) * (2 * (size_t)F::WIN + (size_t)(SK_RING + 1) * F::SLOT);
+}
+
 template <int SY, int VY, int NC>
 size_t skew_lds_bytes() {
     return sizeof(__bf16) * (4 * (size_t)SkewShape<SY, VY, NC>::COPY + (size_t)(SK_RING + 1) * sk_slot_elems(SY));
@@ -1158,6 +1345,44 @@ int aldvc_zncc_map(const float* f, const float* g, int64_t nz, int64_t ny, int64
     if (n > INT32_MAX) return fail(-2, "too many nodes for one launch");
     if (precision != 0 && precision != 1) return fail(-2, "precision must be 0 (float32) or 1 (bfloat16 MFMA)");
     NccArgs a{f, g, (int)nz, (int)ny, (int)nx, coords, n, hx, hy, hz, rx, ry, rz, wx0, wy0, wz0, t_ok, out};
+    {
+        // the skewed-plane tilings (both precisions) where their geometry is instantiated: templates
+        // up to 32 voxels wide, up to 48 x offsets, planes within the per-lane prefetch and box sums;
+        // the window variance comes first, from k_box_var
+        int ks = 0;
+        const Bf16Geom q = bf16_geom(hx, hy, hz, rx, ry, rz, ks);
+        const bool fits = q.sx <= SK_TP && (int64_t)q.wy * q.wx <= (int64_t)SK_NT * SK_PRE &&
+                          (int64_t)q.sy * q.sx <= (int64_t)SK_NT * SK_TPRE && (int64_t)q.vy * q.vx <= (int64_t)BLOCK * VPT &&
+                          (int64_t)q.wy * q.wx <= (int64_t)BLOCK * VPL;
+        const bool known = (q.sy == 25 && q.vy == 33 && q.nxt == 3) || (q.sy == 13 && q.vy == 17 && q.nxt == 2);
+        if (fits && known) {
+            const int nzt = (q.vz + SK_MZ - 1) / SK_MZ;
+            if (n * nzt > INT32_MAX) return fail(-2, "too many workgroups for one launch");
+            const size_t blds = box_lds_bytes(q);
+            if (blds > 64 * 1024 &&
+                hipFuncSetAttribute((const void*)k_box_var, hipFuncAttributeMaxDynamicSharedMemorySize, (int)blds) != hipSuccess)
+                return fail(-5, "could not raise the LDS limit");
+            hipLaunchKernelGGL(k_box_var, dim3((unsigned)n), dim3(BLOCK), blds, stream, a, q);
+#define ALDVC_SKEW(KERN, LDSF, SYV, VYV, NCV)                                                               \
+    if (q.sy == SYV && q.vy == VYV && q.nxt == NCV) {                                                      \
+        const size_t sl = LDSF<SYV, VYV, NCV>();                                                            \
+        const void* sf = (const void*)KERN<SYV, VYV, NCV>;                                                  \
+        if (sl > 64 * 1024 && hipFuncSetAttribute(sf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sl) != hipSuccess) \
+            return fail(-5, "could not raise the LDS limit");                                              \
+        hipLaunchKernelGGL((KERN<SYV, VYV, NCV>), dim3((unsigned)(n * nzt)), dim3(SK_NT), sl, stream, a, q, nzt); \
+        return launch_status();                                                                            \
+    }
+            if (precision == 1) {
+                ALDVC_SKEW(k_zncc_map_bf16_skew, skew_lds_bytes, 25, 33, 3)
+                ALDVC_SKEW(k_zncc_map_bf16_skew, skew_lds_bytes, 13, 17, 2)
+            } else {
+                ALDVC_SKEW(k_zncc_map_f32_skew, skew_f32_lds_bytes, 25, 33, 3)
+                ALDVC_SKEW(k_zncc_map_f32_skew, skew_f32_lds_bytes, 13, 17, 2)
+            }
+#undef ALDVC_SKEW
+            return fail(-2, "skewed-plane shape not instantiated");
+        }
+    }
     if (precision == 1) {
         int ks = 0;
         const Bf16Geom q = bf16_geom(hx, hy, hz, rx, ry, rz, ks);
@@ -1178,26 +1403,6 @@ int aldvc_zncc_map(const float* f, const float* g, int64_t nz, int64_t ny, int64
             hipFuncSetAttribute((const void*)k_box_var, hipFuncAttributeMaxDynamicSharedMemorySize, (int)blds) != hipSuccess)
             return fail(-5, "could not raise the LDS limit");
         hipLaunchKernelGGL(k_box_var, dim3((unsigned)nbox), dim3(BLOCK), blds, stream, a, q);
-        // the skewed-plane tiling where its geometry is instantiated (templates up to 32 voxels wide,
-        // up to 48 x offsets, the window plane within the per-lane prefetch)
-        if (q.sx <= SK_TP && (int64_t)q.wy * q.wx <= (int64_t)SK_NT * SK_PRE &&
-            (int64_t)q.sy * q.sx <= (int64_t)SK_NT * SK_TPRE) {
-            const int nzt = (q.vz + SK_MZ - 1) / SK_MZ;
-            if (n * nzt > INT32_MAX) return fail(-2, "too many workgroups for one launch");
-#define ALDVC_SKEW(SYV, VYV, NCV)                                                                           \
-    if (q.sy == SYV && q.vy == VYV && q.nxt == NCV) {                                                      \
-        const size_t sl = skew_lds_bytes<SYV, VYV, NCV>();                                                  \
-        const void* sf = (const void*)k_zncc_map_bf16_skew<SYV, VYV, NCV>;                                  \
-        if (sl > 64 * 1024 && hipFuncSetAttribute(sf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sl) != hipSuccess) \
-            return fail(-5, "could not raise the LDS limit");                                              \
-        hipLaunchKernelGGL((k_zncc_map_bf16_skew<SYV, VYV, NCV>), dim3((unsigned)(n * nzt)), dim3(SK_NT), sl, stream, a, \
-                           q, nzt);                                                                        \
-        return launch_status();                                                                            \
-    }
-            ALDVC_SKEW(25, 33, 3)
-            ALDVC_SKEW(13, 17, 2)
-#undef ALDVC_SKEW
-        }
         const void* fn = nullptr;
 #define ALDVC_BF16(NTV, KSV)                                                                                \
     if (nt <= NTV && ks == KSV) {                                                                          \
