@@ -78,9 +78,8 @@ struct Ctl12 {
     int stall;
 };
 
-__device__ inline void ctl12_init(Ctl12& c, const double* P) {
-#pragma unroll 1
-    for (int k = 0; k < 12; ++k) c.P[k] = c.P_best[k] = P[k];
+template <typename C>
+__device__ inline void ctl12_reset(C& c) {
     c.norm_init = -1.0;
     c.best_zncc = -2.0;
     c.best_dp = 1e300;
@@ -88,6 +87,12 @@ __device__ inline void ctl12_init(Ctl12& c, const double* P) {
     c.dp_prev = -1.0;
     c.zncc = NAN;
     c.stall = 0;
+}
+
+__device__ inline void ctl12_init(Ctl12& c, const double* P) {
+#pragma unroll 1
+    for (int k = 0; k < 12; ++k) c.P[k] = c.P_best[k] = P[k];
+    ctl12_reset(c);
 }
 
 // L L^T x = b with every array in memory and no unrolling (numba_kernels.py:106-119).
@@ -175,6 +180,182 @@ __device__ int ctl12_step(Ctl12& c, double zncc, double bottomf_d, double n_vali
         }
     }
     return compose_warp(c.P, c.dP, c.work) ? -1 : ST_SINGULAR;
+}
+
+// ---------------------------------------------------------------------------- wave-parallel 12-DOF control
+// State of the wave-parallel control (the serial solver's b, y and dP scratch is not needed: those
+// live in the lanes' registers; `work` holds the warp composition's 3x3 intermediates).
+struct Ctl12w {
+    double P[12], P_best[12], work[27];
+    double H0[144], L0[144];
+    double norm_init, best_zncc, best_dp, last_dp, dp_prev, zncc;
+    int stall;
+};
+
+// The decisions of ctl12_step (numba_kernels.py:726-790) taken by the 64 lanes of wave 0 at once,
+// lane k < 12 owning parameter k: the two triangular solves are column sweeps (the entry solved on
+// its lane is broadcast, every later row subtracts its share), the norms are wave sums, and the
+// warp composition is formed by every lane from broadcast values, lane k keeping entry k. The
+// state stays in LDS (Ctl12), so none of it is live in the sampling loops' registers. The serial
+// form takes one dependent LDS or memory access per multiply-add of the solve; this one a few
+// dozen dependent steps per iteration.
+
+// This lane's index in its wave, computed where it is used: index arithmetic derived from it stays in
+// the control code instead of being hoisted out of the node loop and kept live (and spilled) through
+// the sampling loops.
+__device__ inline int lane_here() {
+    int l;
+    asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0" : "=v"(l));
+    return l;
+}
+
+// value of lane k (wave-uniform k) on every lane
+__device__ inline double lane_bcast(double v, int k) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, k);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), k);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// sum over lanes 0..11 in lane order (the order of the serial loops), the same value on every lane;
+// broadcasts through scalar registers need no lane-index registers
+__device__ inline double sum12(double v) {
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < 12; ++k) s += lane_bcast(v, k);
+    return s;
+}
+
+// W(P) <- W(P) W(dP)^-1 (numba_kernels.py:145-180) with dP_k on lane k. Lane q < 9 forms entry q
+// of (I + dP)^-1 (its cofactor over the determinant, as inv3), then entry q of A (I + dP)^-1, and
+// lanes 9..11 the new translation; the intermediates pass through `work` (LDS, 27 doubles). LDS
+// operations of one wave complete in order, so a lane reads what another lane wrote before.
+// Returns false when dP's 3x3 part is singular.
+__device__ inline bool compose_warp_wave(double* P, double* work, double dpk) {
+    const int lane = lane_here();
+    double* dA = work;       // I + dP (3x3)
+    double* dAi = work + 9;  // its inverse
+    double* An = work + 18;  // (I + P) (I + dP)^-1
+    if (lane < 9) dA[lane] = dpk + ((lane & 3) == 0 ? 1.0 : 0.0);  // diagonal entries 0, 4, 8
+    __builtin_amdgcn_wave_barrier();
+    const double det = dA[0] * (dA[4] * dA[8] - dA[5] * dA[7]) - dA[1] * (dA[3] * dA[8] - dA[5] * dA[6]) +
+                       dA[2] * (dA[3] * dA[7] - dA[4] * dA[6]);
+    if (fabs(det) < 1e-300) return false;
+    const int q = lane < 9 ? lane : 0, r = q / 3, cl = q - 3 * r;
+    if (lane < 9) {
+        // adj[r][c] = a[c+1][r+1] a[c+2][r+2] - a[c+1][r+2] a[c+2][r+1] (indices mod 3)
+        const int c1 = (cl + 1) % 3, c2 = (cl + 2) % 3, r1 = (r + 1) % 3, r2 = (r + 2) % 3;
+        dAi[q] = (dA[3 * c1 + r1] * dA[3 * c2 + r2] - dA[3 * c1 + r2] * dA[3 * c2 + r1]) * (1.0 / det);
+    }
+    __builtin_amdgcn_wave_barrier();
+    double mine = 0.0;
+    if (lane < 9) {
+        double acc = 0.0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) acc += (P[3 * r + k] + (r == k ? 1.0 : 0.0)) * dAi[3 * k + cl];
+        An[q] = acc;
+        mine = acc - (r == cl ? 1.0 : 0.0);
+    }
+    __builtin_amdgcn_wave_barrier();
+    const double t[3] = {lane_bcast(dpk, 9), lane_bcast(dpk, 10), lane_bcast(dpk, 11)};
+    if (lane >= 9 && lane < 12) {
+        const int i = lane - 9;
+        double acc = 0.0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) acc += An[3 * i + k] * t[k];
+        mine = P[lane] - acc;
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (lane < 12) P[lane] = mine;
+    return true;
+}
+
+// One IC-GN decision of wave 0 from the gradient entry bk (lane k, already scaled by bottomf_d)
+// and the ZNCC. Returns -1 to continue, else the final status (the same value on every lane);
+// c.P is updated.
+__device__ int ctl12_step_wave(Ctl12w& c, double bk, double zncc, double bottomf_d, double n_valid, const double* L,
+                               const double* H, const double* pattern, double noise_gain, double tol, double dp_tol,
+                               int patience, int predictive, double half_scale) {
+    const int lane = lane_here();
+    const bool own = lane < 12;
+    const double norm_abs = sqrt(sum12(bk * bk));
+    if (!isfinite(norm_abs)) {
+        if (lane == 0) c.zncc = NAN;
+        return ST_NAN;
+    }
+    const double norm_init = c.norm_init < 0.0 ? norm_abs : c.norm_init;
+    const double last_dp = c.last_dp, dp_prev = c.dp_prev, best_zncc = c.best_zncc, best_dp = c.best_dp;
+    const int stall = c.stall;
+    __builtin_amdgcn_wave_barrier();
+    if (lane == 0) {
+        c.norm_init = norm_init;
+        c.zncc = zncc;
+    }
+    const double norm_rel = norm_init > 1e-300 ? norm_abs / norm_init : 0.0;
+    if (norm_rel < tol || norm_abs < ABS_TOL) return ST_CONVERGED;
+    bool use_corr = false;
+    const double n_full = pattern ? pattern[9 * 12 + 9] : 0.0;
+    if (noise_gain > 0.0 && n_full > 0.0 && last_dp < NOISE_CORR_STEP && H) {
+        const double om = 1.0 - zncc;
+        const double s2 = (om > 0.0 ? om : 0.0) * bottomf_d * bottomf_d / n_valid;
+        const double corr = noise_gain * s2 * (n_valid / n_full);
+        int ok = 0;
+        if (lane == 0) ok = corrected_cholesky12(H, pattern, corr, c.H0, c.L0) ? 1 : 0;
+        use_corr = __builtin_amdgcn_readfirstlane(ok) != 0;  // every lane is active again: lane 0's flag
+        __builtin_amdgcn_wave_barrier();
+    }
+    const double* Lf = use_corr ? c.L0 : L;
+    // forward sweep L y = -b, then back sweep L^T x = y; lane i reads row i (forward) or column i
+    // (back) of the factor one entry per step (rows of lanes >= 12 are clamped and unused)
+    const int row = own ? lane : 11;
+    const double dii = Lf[row * 13];
+    double s = own ? -bk : 0.0, y = 0.0;
+#pragma unroll 1
+    for (int k = 0; k < 12; ++k) {
+        const double Lik = Lf[row * 12 + k];
+        const double yk = lane_bcast(s / dii, k);
+        if (lane == k) y = yk;
+        if (lane > k) s -= Lik * yk;
+    }
+    double t = y, x = 0.0;
+#pragma unroll 1
+    for (int k = 11; k >= 0; --k) {
+        const double Lki = Lf[k * 12 + row];
+        const double xk = lane_bcast(t / dii, k);
+        if (lane == k) x = xk;
+        if (lane < k) t -= Lki * xk;
+    }
+    const double v = lane < 9 ? x * half_scale : x;
+    const double dp_norm = sqrt(sum12(v * v));
+    if (!isfinite(dp_norm)) {
+        if (lane == 0) c.zncc = NAN;
+        return ST_NAN;
+    }
+    if (lane == 0) c.last_dp = dp_norm;
+    if (dp_norm < dp_tol) return ST_CONVERGED;
+    if (predictive && dp_prev > 0.0 && dp_norm < PREDICT_CONTRACTION * dp_prev && dp_norm * dp_norm < dp_tol * dp_prev)
+        return compose_warp_wave(c.P, c.work, x) ? ST_CONVERGED : ST_SINGULAR;
+    if (lane == 0) c.dp_prev = dp_norm;
+    bool improved = zncc > best_zncc + STALL_ZNCC_EPS;
+    if (improved) {
+        if (lane == 0) c.best_zncc = zncc;
+        if (own) c.P_best[lane] = c.P[lane];
+    }
+    if (dp_norm < best_dp * STALL_STEP_DECAY) {
+        if (lane == 0) c.best_dp = dp_norm;
+        improved = true;
+    }
+    if (improved) {
+        if (lane == 0) c.stall = 0;
+    } else {
+        if (lane == 0) c.stall = stall + 1;
+        if (patience > 0 && stall + 1 >= patience) {
+            if (own) c.P[lane] = c.P_best[lane];
+            if (lane == 0) c.zncc = best_zncc;
+            return ST_STALLED;
+        }
+    }
+    return compose_warp_wave(c.P, c.work, x) ? -1 : ST_SINGULAR;
 }
 
 // Per-node state of a translation-only solve (ADMM subproblem 1, numba_kernels.py:931-1082):
@@ -588,7 +769,9 @@ constexpr int NG = 18;  // 0 n_valid, 1 oob, 2 n_nan, 3 sum g', 4 sum g'^2, 5 su
 constexpr int NG3 = 9;  // translation-only: 0..5 as NG, 6 + c: sum grad_c g'
 constexpr int MOM_STRIDE = 32;  // doubles per node in the moments workspace
 constexpr int BRICK_XY = 40;    // LDS brick rows and columns: one box for subsets up to 35 voxels wide
-constexpr int LDS_PER_WG = 54528;  // LDS budget of one fused workgroup: three per CU (160 KB)
+// LDS budget of one fused workgroup, static and dynamic together: three per CU. gfx950 allocates a
+// workgroup's LDS in 1280-byte granules (160 KB in 128), so three workgroups hold 42 granules each.
+constexpr int LDS_PER_WG = 42 * 1280;
 
 // XCD-aware slot of workgroup b over the M valid nodes: workgroup b runs on XCD b % 8, and the
 // node list is dealt to the XCDs in chunks of XCD_CHUNK consecutive nodes (two 4 x 4 x 4 tiles of
@@ -659,6 +842,64 @@ __device__ inline void ref_moments(const IcgnArgs& a, const uint8_t* keep, int64
     block_sum<NR, NWAVES>(m, out, red);
 }
 
+// The same moments over every kept voxel, summed along voxel columns in y: a thread walks half of
+// one column (its X and Z fixed), keeping per-column sums of grad_c and grad_c f' that it folds into
+// the X, Z and 1 moments once at the column's end; the Y moments and 1, f', f'^2 are summed per
+// voxel. About two thirds of the per-voxel arithmetic of the per-voxel form and no index decode;
+// neighbouring lanes take neighbouring x (coalesced loads), and a column stays inside its plane.
+template <bool STORED>
+__device__ inline void ref_moments_columns(const IcgnArgs& a, const uint8_t* keep, int64_t x0, int64_t y0, int64_t z0,
+                                           float pf, double* out, double* red) {
+    const Volumes& V = a.V;
+    const Subset& s = a.sub;
+    const int64_t obase = lin(V, z0, y0, x0);
+    const int64_t rowp = V.nx, planep = (int64_t)V.nx * V.ny;
+    float m[NR];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) m[q] = 0.0f;
+    const int ncol = s.sx * s.sz, yh = (s.sy + 1) >> 1;
+    for (int t = threadIdx.x; t < 2 * ncol; t += BLOCK) {
+        const int upper = t >= ncol ? 1 : 0, col = t - upper * ncol;  // first every lower half, then every upper
+        const int iz = col / s.sx, ix = col - iz * s.sx;
+        const int iy0 = upper ? yh : 0, iy1 = upper ? s.sy : yh;
+        const int dx = -s.hx + s.stride * ix, dz = -s.hz + s.stride * iz;
+        const int64_t ostep = s.stride * rowp;
+        int64_t o = obase + (int64_t)dz * planep + (int64_t)(-s.hy + s.stride * iy0) * rowp + dx;
+        // per-column sums of grad_c and grad_c f' (folded with X and Z below); the rest goes straight
+        // into the thread's moments
+        float A[3] = {0.0f, 0.0f, 0.0f}, B[3] = {0.0f, 0.0f, 0.0f};
+        for (int iy = iy0; iy < iy1; ++iy, o += ostep) {
+            const int dy = -s.hy + s.stride * iy;
+            if (!voxel_kept(V, keep, (iz * s.sy + iy) * s.sx + ix, o, x0 + dx, y0 + dy, z0 + dz)) continue;
+            const float fv = V.f[o] - pf, Y = (float)dy;
+            float gr[3];
+            grad_at<STORED>(V, o, (int)z0 + dz, (int)y0 + dy, (int)x0 + dx, gr[0], gr[1], gr[2]);
+            m[0] += 1.0f;
+            m[1] += fv;
+            m[2] += fv * fv;
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc) {
+                const float tf = gr[cc] * fv;
+                A[cc] += gr[cc];
+                B[cc] += tf;
+                m[4 + 4 * cc] += gr[cc] * Y;
+                m[16 + 4 * cc] += tf * Y;
+            }
+        }
+        const float X = (float)dx, Z = (float)dz;
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) {
+            m[3 + 4 * cc] += A[cc] * X;
+            m[5 + 4 * cc] += A[cc] * Z;
+            m[6 + 4 * cc] += A[cc];
+            m[15 + 4 * cc] += B[cc] * X;
+            m[17 + 4 * cc] += B[cc] * Z;
+            m[18 + 4 * cc] += B[cc];
+        }
+    }
+    block_sum<NR, NWAVES>(m, out, red);
+}
+
 template <bool STORED>
 __global__ __launch_bounds__(BLOCK) void k_ref_moments(IcgnArgs a, const int64_t* __restrict__ list,
                                                        const int64_t* __restrict__ count, double* __restrict__ mom) {
@@ -672,11 +913,7 @@ __global__ __launch_bounds__(BLOCK) void k_ref_moments(IcgnArgs a, const int64_t
     if (!keep && !subset_in_volume(a.V, a.sub, x0, y0, z0)) return;  // refused by the iteration kernel
     double pfd = a.meanf ? a.meanf[n] : 0.0;
     const float pf = isfinite(pfd) ? (float)pfd : 0.0f;
-    const Volumes& V = a.V;
-    ref_moments<STORED>(
-        a, keep, x0, y0, z0, pf,
-        [&](int v, int64_t o, int64_t xx, int64_t yy, int64_t zz) { return voxel_kept(V, keep, v, o, xx, yy, zz); },
-        res, red);
+    ref_moments_columns<STORED>(a, keep, x0, y0, z0, pf, res, red);
     for (int q = threadIdx.x; q < NR; q += BLOCK) mom[n * MOM_STRIDE + q] = res[q];
 }
 
@@ -810,34 +1047,45 @@ __global__ __launch_bounds__(BLOCK, 3) void k_icgn12_fused(IcgnArgs a, const int
         __shared__ double red[NWAVES * NR];
         __shared__ double res[NG];
         __shared__ double exc[NR];
-        __shared__ std::conditional_t<T3, Ctl3, Ctl12> c;
+        __shared__ std::conditional_t<T3, Ctl3, Ctl12w> c;
         constexpr int NM = T3 ? NG3 : NG;  // deformed-side moments accumulated per lane
 
         double pg_next = 0.0;
-        if (tid == 0) {
-            const bool in_volume = keep || subset_in_volume(V, s, x0, y0, z0);
+        if (tid < 64) {
+            int init = -1;
             if constexpr (T3) {
-                ctrl = ctl3_init(c, a, n, in_volume);
+                if (tid == 0) init = ctl3_init(c, a, n, keep || subset_in_volume(V, s, x0, y0, z0));
             } else {
+                // lane k of wave 0 loads parameter k
                 bool finite = true;
-                ctl12_init(c, a.P + 12 * n);
-    #pragma unroll 1
-                for (int k = 0; k < 12; ++k)
-                    if (!isfinite(c.P[k])) finite = false;
-                ctrl = -1;
-                if (!finite)
-                    ctrl = ST_NAN;
-                else if (!in_volume)
-                    ctrl = ST_INVALID_SUBSET;
+                if (tid < 12) {
+                    const double pk = a.P[12 * n + tid];
+                    c.P[tid] = c.P_best[tid] = pk;
+                    finite = isfinite(pk);
+                }
+                const bool all_finite = __ballot(!finite) == 0;
+                if (tid == 0) {
+                    ctl12_reset(c);
+                    if (!all_finite)
+                        init = ST_NAN;
+                    else if (!(keep || subset_in_volume(V, s, x0, y0, z0)))
+                        init = ST_INVALID_SUBSET;
+                }
             }
-            double pf = a.meanf ? a.meanf[n] : 0.0;
-            if (!isfinite(pf)) pf = 0.0;
-            sw[12] = (float)pf;
-            pg_next = pf;
+            if (tid == 0) {
+                ctrl = init;
+                double pf = a.meanf ? a.meanf[n] : 0.0;
+                if (!isfinite(pf)) pf = 0.0;
+                sw[12] = (float)pf;
+                pg_next = pf;
+            }
         }
         __syncthreads();
-        // without noise correction the node's Cholesky factor is read every iteration by the control
-        // lane: a copy in LDS (the corrected factor's slot, unused then) replaces its global loads
+        // the node's reference-side moments, read by the control every iteration
+        __shared__ double s_mom[NR];
+        for (int i = tid; i < NR; i += BLOCK) s_mom[i] = mom[n * MOM_STRIDE + i];
+        // without noise correction the node's Cholesky factor is read every iteration by the control:
+        // a copy in LDS (the corrected factor's slot, unused then) replaces its global loads
         const double* Lctl = nullptr;
         if constexpr (!T3) {
             if (!(a.noise_gain > 0.0 && a.H_all && a.pattern)) {
@@ -848,7 +1096,7 @@ __global__ __launch_bounds__(BLOCK, 3) void k_icgn12_fused(IcgnArgs a, const int
             }
         }
         __syncthreads();
-        const double* mn = mom + n * MOM_STRIDE;
+        const double* mn = s_mom;
         const double half_scale = (double)max(s.hx, max(s.hy, s.hz));
         const Domain dom = make_domain(V, (int)x0, (int)y0, (int)z0);
         const float* gbase = V.g + lin(V, z0, y0, x0);
@@ -1144,7 +1392,9 @@ __global__ __launch_bounds__(BLOCK, 3) void k_icgn12_fused(IcgnArgs a, const int
                     },
                     exc, red);
             }
-            if (tid == 0) {
+            // the control: lane 0 for the translation-only solve, wave 0 for the 12-DOF one
+            if (T3 ? tid == 0 : tid < 64) {
+                const int lane = tid;
                 const double n_ref = mn[0], n_valid = n_ref - res[2];  // every kept voxel was sampled once
                 int st = -1;
                 if (res[1] > 0.0) {
@@ -1153,8 +1403,10 @@ __global__ __launch_bounds__(BLOCK, 3) void k_icgn12_fused(IcgnArgs a, const int
                     st = ST_INVALID_SUBSET;
                 }
                 if (st >= 0) {
-                    c.zncc = NAN;
-                    ctrl = st;
+                    if (lane == 0) {
+                        c.zncc = NAN;
+                        ctrl = st;
+                    }
                 } else {
                     // reference-side moments of this iteration's voxel set
                     auto R = [&](int q) { return correct ? mn[q] - exc[q] : mn[q]; };
@@ -1171,15 +1423,19 @@ __global__ __launch_bounds__(BLOCK, 3) void k_icgn12_fused(IcgnArgs a, const int
                     } else {
                         const double zncc = (res[5] - n_valid * mf * mg) / (bf * bg);
                         const double ratio = bf / bg;
-    #pragma unroll 1
-                        for (int q = 0; q < 12; ++q) {
-                            const int cc = q >> 2, j = q & 3;
+                        // lane k: gradient entry k (k < 9: component k / 3 against q_(k mod 3); k >= 9:
+                        // component k - 9 against q = 1), from moment q = 4 c + j
+                        double bk = 0.0;
+                        const int lane = lane_here();
+                        if (lane < 12) {
+                            const int q = lane < 9 ? 4 * (lane / 3) + lane % 3 : 4 * (lane - 9) + 3;
                             const double C = R(3 + q), A = R(15 + q), B = res[6 + q];
-                            c.b[j < 3 ? 3 * cc + j : 9 + cc] = (A - mf * C) - ratio * (B - mg * C);
+                            bk = (A - mf * C) - ratio * (B - mg * C);
                         }
                         const double* Hn = a.H_all ? a.H_all + 144 * n : nullptr;
-                        ctrl = ctl12_step(c, zncc, bf, n_valid, Lctl, Hn, a.pattern, a.noise_gain, a.tol,
-                                          a.dp_tol, a.patience, a.predictive, half_scale);
+                        const int r = ctl12_step_wave(c, bk, zncc, bf, n_valid, Lctl, Hn, a.pattern, a.noise_gain,
+                                                      a.tol, a.dp_tol, a.patience, a.predictive, half_scale);
+                        if (lane == 0) ctrl = r;
                     }
                     pg_next = (double)pg + mg;
                 }

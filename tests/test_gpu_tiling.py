@@ -158,11 +158,13 @@ def test_a_volume_beyond_two_to_the_31_voxels_is_solved_in_boxes():
     cd = torch.from_numpy(coords[sel]).to(dev)
     H, L, mf, bf, nv, valid = K.precompute_nodes(cd, 8, 8, 8, fc, ph, ph, ph, nomask, 0.5, 1e12)
     P, it, st, zn = K.icgn_12dof_parallel(cd, torch.from_numpy(P0).to(dev), 8, 8, 8, fc, ph, ph, ph, nomask, gc, 1, L, mf,
-                                          bf, valid, 1e-2, 1e-3, 100, 5, 1, H, variant="auto")  # fmt: skip
+                                          bf, valid, 1e-2, 1e-3, 100, 5, 1, H, variant="fused_gather")  # fmt: skip
     st = st.cpu().numpy()
     assert np.array_equal(st, info.status[sel])
     # the boxes of the big volume exceed 2^30 voxels and take the pointer-gather form of the kernel,
-    # the 4000-voxel crop the LDS-brick form: the same answer to float32 rounding, not bit for bit
+    # the 4000-voxel crop (under 2^30) the buffer-gather form: the same samples in the same order, so
+    # the boxes must reproduce the crop bit for bit. (The LDS-brick form rounds its taps differently,
+    # and a node whose last step lies at a stopping threshold may then stop one update apart.)
     ok = ~bad[sel]
     assert ok.sum() > 0.9 * sel.size
-    assert np.max(np.abs(P.cpu().numpy()[ok, 9:] - U[sel][ok])) < 1e-5
+    assert np.array_equal(P.cpu().numpy()[ok, 9:], U[sel][ok])
