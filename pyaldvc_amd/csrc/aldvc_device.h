@@ -324,11 +324,17 @@ __device__ inline float sample_brick(const float* brick, int plane, int bx, int 
     }
 }
 
-// Anchor-relative warped position with a pinned evaluation order (explicit fused multiply-adds):
-// evaluated the same way everywhere, it is monotone in each of X, Y, Z, so over a box of subset
-// voxels its extremes are taken at the box corners -- the bound the LDS bricks rely on.
+// Anchor-relative warped position with a pinned evaluation order (explicit fused multiply-adds,
+// X innermost, Z outermost): evaluated the same way everywhere, it is monotone in each of X, Y, Z,
+// so over a box of subset voxels its extremes are taken at the box corners -- the bound the LDS
+// bricks rely on. warp_xy is the part a voxel column of fixed (X, Y) shares; warp_coord =
+// warp_z(a2, Z, warp_xy(a0, a1, t, X, Y)) to the bit.
+__device__ inline float warp_xy(float a0, float a1, float t, float X, float Y) {
+    return __fmaf_rn(a1, Y, __fmaf_rn(a0, X, t));
+}
+__device__ inline float warp_z(float a2, float Z, float xy) { return __fmaf_rn(a2, Z, xy); }
 __device__ inline float warp_coord(float a0, float a1, float a2, float t, float X, float Y, float Z) {
-    return __fmaf_rn(a0, X, __fmaf_rn(a1, Y, __fmaf_rn(a2, Z, t)));
+    return warp_z(a2, Z, warp_xy(a0, a1, t, X, Y));
 }
 
 // Admissible sampling domain (interp_kernels.py:127-136) in anchor-relative coordinates:

@@ -958,38 +958,61 @@ __device__ inline int plan_chunk(const float* sw, const Subset& s, const Domain&
 // Stage planes [0, ez) of a brick of `rows` rows of BRK floats (row stride BRK, plane stride
 // `bplane` floats) whose origin is the anchor-relative voxel (bx, by, bz): 16-byte buffer loads of
 // the deformed volume (out-of-range offsets read zero; brick voxels outside the volume are never
-// read by an in-domain sample), 4-byte LDS stores (the plane stride need not keep rows 16-byte
-// aligned). A thread owns fixed (row, 16-byte group) positions and walks them through the planes,
-// so a load's address is one add from the previous plane's, with up to eight loads in flight.
+// read by an in-domain sample). A thread owns at most two fixed (row, 16-byte group) positions
+// (rows x groups <= 2 x BLOCK) and walks them through the planes, four planes at a time: the loads
+// of both positions are in flight before any is stored. Planes 16-byte aligned in LDS take one
+// 16-byte store per group, others four 4-byte stores.
 template <int BRK>
 __device__ inline void stage_brick(float* brick, int bplane, int cols, int rows, const BufVol& gb, int anchor,
                                    int rowp, int planep, int bx, int by, int bz, int ez) {
     static_assert(BRK % 4 == 0, "brick rows are whole 16-byte groups");
+    static_assert(BRK * (BRK / 4) <= 2 * BLOCK, "a brick plane has at most two 16-byte groups per thread");
     const int Q = (cols + 3) >> 2;  // 16-byte groups per row: the columns the chunk's stencils reach
     const int RQ = rows * Q;        // 16-byte groups per plane
     const int org = anchor + bz * planep + by * rowp + bx;
-    // p / Q as a float product: p < 40 rows x 10 groups, so (p + 0.5) / Q is at least 0.05 from an
-    // integer and the float rounding (< 1e-4) cannot change the floor
+    const bool wide = (bplane & 3) == 0;
+    // p / Q as a float product: p < 2 x BLOCK, so (p + 0.5) / Q is at least 0.05 from an integer and
+    // the float rounding (< 1e-4) cannot change the floor
     const float inv_q = 1.0f / (float)Q;
+    const int pa = threadIdx.x, pb = threadIdx.x + BLOCK;
+    const bool va = pa < RQ, vb = pb < RQ;
+    const int ya = (int)(((float)pa + 0.5f) * inv_q), yb = (int)(((float)pb + 0.5f) * inv_q);
+    const int qa = pa - ya * Q, qb = pb - yb * Q;
+    // a lane without a second position loads the volume's first voxels of each plane (in range) and
+    // stores nothing: the loads stay outside divergent branches, so they are all issued back to back
+    uint32_t oa = va ? 4u * (uint32_t)(org + ya * rowp + 4 * qa) : 0u;
+    uint32_t ob = vb ? 4u * (uint32_t)(org + yb * rowp + 4 * qb) : 0u;
+    float* da = brick + ya * BRK + 4 * qa;
+    float* db = brick + yb * BRK + 4 * qb;
+    const uint32_t pstep = 4u * (uint32_t)planep;
     constexpr int U = 4;
-    for (int p = threadIdx.x; p < RQ; p += BLOCK) {
-        const int yy = (int)(((float)p + 0.5f) * inv_q), qx = p - yy * Q;
-        const int src = org + yy * rowp + 4 * qx;
-        float* dst = brick + yy * BRK + 4 * qx;
-        for (int z0 = 0; z0 < ez; z0 += U) {
-            float4 r[U];
+    auto put = [&](float* d, const float4& v) {
+        if (wide) {
+            *reinterpret_cast<float4*>(d) = v;
+        } else {
+            d[0] = v.x;
+            d[1] = v.y;
+            d[2] = v.z;
+            d[3] = v.w;
+        }
+    };
+    for (int z0 = 0; z0 < ez; z0 += U) {
+        float4 ra[U], rb[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u)
-                if (z0 + u < ez) r[u] = buf_load4(gb, 4u * (uint32_t)(src + (z0 + u) * planep), 0);
+        for (int u = 0; u < U; ++u) {
+            if (z0 + u < ez) {
+                ra[u] = buf_load4(gb, oa, 0);
+                rb[u] = buf_load4(gb, ob, 0);
+            }
+            oa += pstep;
+            ob += pstep;
+        }
 #pragma unroll
-            for (int u = 0; u < U; ++u)
-                if (z0 + u < ez) {
-                    float* d = dst + (z0 + u) * bplane;
-                    d[0] = r[u].x;
-                    d[1] = r[u].y;
-                    d[2] = r[u].z;
-                    d[3] = r[u].w;
-                }
+        for (int u = 0; u < U; ++u) {
+            if (z0 + u < ez) {
+                if (va) put(da + (z0 + u) * bplane, ra[u]);
+                if (vb) put(db + (z0 + u) * bplane, rb[u]);
+            }
         }
     }
 }
@@ -1238,35 +1261,91 @@ __global__ __launch_bounds__(BLOCK, 3) void k_icgn12_fused(IcgnArgs a, const int
                                     // its last whole patch follow in a short tail.
                                     const int pxn = kx >> 3, pyn = ky >> 3;
                                     if (pxn > 0 && pyn > 0) {
+                                        // A wave takes the chunk's patch columns (pxi, pyi) in turn and walks
+                                        // each through the chunk's kz planes, so a lane's X and Y stay fixed
+                                        // along the walk: the warp's X and Y terms, the column's offsets and
+                                        // the X- and Y-weighted moments are formed once per column (the
+                                        // column's sums of grad_c g' are folded in at its end), and a sample
+                                        // adds only its Z terms.
                                         const int lx = lane & 7, ly = lane >> 3;
-                                        const int lane_v = ly * sx + lx, lane_o = ly * rowp + lx;
-                                        int pxi = __builtin_amdgcn_readfirstlane(tid >> 6), pyi = 0, pz = 0;
+                                        const int npc = pxn * pyn, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+                                        int pxi = wave, pyi = 0;
                                         while (pxi >= pxn) {
                                             pxi -= pxn;
                                             ++pyi;
                                         }
-                                        while (pyi >= pyn) {
-                                            pyi -= pyn;
-                                            ++pz;
-                                        }
-                                        while (pz < kz) {
-                                            const int iz = zc + pz, x = xc + 8 * pxi, y = yc + 8 * pyi;
-                                            const int ux = -s.hx + x, uy = -s.hy + y, uz = -s.hz + iz;
-                                            Vox q;
-                                            q.dx = ux + lx;
-                                            q.dy = uy + ly;
-                                            q.dz = uz;
-                                            q.v = (iz * sy + y) * sx + x + lane_v;
-                                            q.o = uz * planep + uy * rowp + ux + lane_o;
-                                            voxel(q, Brick{}, bx, by, bz);
+                                        for (int pc = wave; pc < npc; pc += NWAVES) {
+                                            const int ix = xc + 8 * pxi + lx, iy = yc + 8 * pyi + ly;
+                                            const int dx = -s.hx + ix, dy = -s.hy + iy, dz0 = -s.hz + zc;
+                                            const float X = (float)dx, Y = (float)dy;
+                                            const float wx = warp_xy(sw[0], sw[1], sw[9], X, Y);
+                                            const float wy = warp_xy(sw[3], sw[4], sw[10], X, Y);
+                                            const float wz = warp_xy(sw[6], sw[7], sw[11], X, Y);
+                                            const int o0 = dz0 * planep + dy * rowp + dx;
+                                            const int v0 = (zc * sy + iy) * sx + ix;
+                                            float A[3] = {0.0f, 0.0f, 0.0f};  // the column's sums of grad_c g'
+                                            for (int pz = 0; pz < kz; ++pz) {
+                                                const int dz = dz0 + pz, o = o0 + pz * planep;
+                                                if constexpr (SEL)
+                                                    if (!voxel_kept(V, keep, v0 + pz * plane_vox, obase + o, x0 + dx,
+                                                                    y0 + dy, z0 + dz))
+                                                        continue;
+                                                float fraw, gr[3];
+                                                if constexpr (STORED) {
+                                                    const uint32_t ob = 4u * (uint32_t)((int)obase + o);
+                                                    auto at = [ob](const float* vol) {
+                                                        return *reinterpret_cast<const float*>(
+                                                            reinterpret_cast<const char*>(vol) + ob);
+                                                    };
+                                                    fraw = at(V.f);
+                                                    gr[0] = at(V.gx);
+                                                    gr[1] = at(V.gy);
+                                                    gr[2] = at(V.gz);
+                                                } else {
+                                                    fraw = V.f[obase + o];
+                                                    grad_at<false>(V, obase + o, (int)z0 + dz, (int)y0 + dy, (int)x0 + dx,
+                                                                   gr[0], gr[1], gr[2]);
+                                                }
+                                                const float Z = (float)dz;
+                                                const float val = sample_brick<MODE, BRK>(
+                                                    brick, brick_plane, bx, by, bz, warp_z(sw[2], Z, wx), warp_z(sw[5], Z, wy),
+                                                    warp_z(sw[8], Z, wz));
+                                                const bool ok = val == val;
+                                                const float fv = fraw - pf, gv = ok ? val - pg : 0.0f;
+                                                m[3] += gv;
+                                                m[4] += gv * gv;
+                                                m[5] += fv * gv;
+                                                if constexpr (T3) {
+    #pragma unroll
+                                                    for (int cc = 0; cc < 3; ++cc) A[cc] = __fmaf_rn(gr[cc], gv, A[cc]);
+                                                } else {
+                                                    const float gz = gv * Z;
+    #pragma unroll
+                                                    for (int cc = 0; cc < 3; ++cc) {
+                                                        A[cc] = __fmaf_rn(gr[cc], gv, A[cc]);
+                                                        m[8 + 4 * cc] = __fmaf_rn(gr[cc], gz, m[8 + 4 * cc]);
+                                                    }
+                                                }
+                                                if (!ok) {
+                                                    const int v = v0 + pz * plane_vox;
+                                                    m[2] += 1.0f;
+                                                    atomicOr(&nanbits[v >> 5], 1u << (v & 31));
+                                                }
+                                            }
+    #pragma unroll
+                                            for (int cc = 0; cc < 3; ++cc) {
+                                                if constexpr (T3) {
+                                                    m[6 + cc] += A[cc];
+                                                } else {
+                                                    m[6 + 4 * cc] = __fmaf_rn(X, A[cc], m[6 + 4 * cc]);
+                                                    m[7 + 4 * cc] = __fmaf_rn(Y, A[cc], m[7 + 4 * cc]);
+                                                    m[9 + 4 * cc] += A[cc];
+                                                }
+                                            }
                                             pxi += NWAVES;
                                             while (pxi >= pxn) {
                                                 pxi -= pxn;
                                                 ++pyi;
-                                            }
-                                            while (pyi >= pyn) {
-                                                pyi -= pyn;
-                                                ++pz;
                                             }
                                         }
                                     }
