@@ -387,11 +387,45 @@ __device__ inline void grad_at(const Volumes& V, int64_t o, int zz, int yy, int 
 __device__ inline bool kept_bit(const uint8_t* row, int v) { return ((row[v >> 3] >> (v & 7)) & 1) != 0; }
 
 // ---------------------------------------------------------------------------- reductions
+// value of lane k (wave-uniform k) on every lane
+__device__ inline double lane_bcast(double v, int k) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, k);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), k);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+__device__ inline float lane_bcast(float v, int k) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
+}
+
+// v of lane (l ^ X) within each half-wave of 32 lanes: ds_swizzle in its bit-mask mode, the pattern
+// an immediate (no lane-index register, unlike a general shuffle)
+template <int X>
+__device__ inline int swizzle_xor(int v) {
+    static_assert(X > 0 && X < 32, "xor within 32 lanes");
+    return __builtin_amdgcn_ds_swizzle(v, 0x1F | (X << 10));
+}
+template <int X>
+__device__ inline double swizzle_xor(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = swizzle_xor<X>((int)b), hi = swizzle_xor<X>((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+template <int X>
+__device__ inline float swizzle_xor(float v) {
+    return __int_as_float(swizzle_xor<X>(__float_as_int(v)));
+}
+
+// Sum over the 64 lanes of a wave, the same value on every lane: a butterfly within each half-wave,
+// then the two halves' sums (lanes 0 and 32) added.
 template <typename T>
 __device__ inline T wave_sum(T v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
+    v += swizzle_xor<16>(v);
+    v += swizzle_xor<8>(v);
+    v += swizzle_xor<4>(v);
+    v += swizzle_xor<2>(v);
+    v += swizzle_xor<1>(v);
+    return lane_bcast(v, 0) + lane_bcast(v, 32);
 }
 
 // Block-wide sums of NQ per-thread float partials, accumulated in float64 across lanes and

@@ -209,14 +209,6 @@ __device__ inline int lane_here() {
     return l;
 }
 
-// value of lane k (wave-uniform k) on every lane
-__device__ inline double lane_bcast(double v, int k) {
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readlane((int)b, k);
-    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), k);
-    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
-}
-
 // sum over lanes 0..11 in lane order (the order of the serial loops), the same value on every lane;
 // broadcasts through scalar registers need no lane-index registers
 __device__ inline double sum12(double v) {
