@@ -839,7 +839,7 @@ __device__ inline void ref_moments(const IcgnArgs& a, const uint8_t* keep, int64
 // the X, Z and 1 moments once at the column's end; the Y moments and 1, f', f'^2 are summed per
 // voxel. About two thirds of the per-voxel arithmetic of the per-voxel form and no index decode;
 // neighbouring lanes take neighbouring x (coalesced loads), and a column stays inside its plane.
-template <bool STORED>
+template <bool STORED, bool SEL>
 __device__ inline void ref_moments_columns(const IcgnArgs& a, const uint8_t* keep, int64_t x0, int64_t y0, int64_t z0,
                                            float pf, double* out, double* red) {
     const Volumes& V = a.V;
@@ -860,9 +860,13 @@ __device__ inline void ref_moments_columns(const IcgnArgs& a, const uint8_t* kee
         // per-column sums of grad_c and grad_c f' (folded with X and Z below); the rest goes straight
         // into the thread's moments
         float A[3] = {0.0f, 0.0f, 0.0f}, B[3] = {0.0f, 0.0f, 0.0f};
+        // without a selection the body is straight-line code, unrolled so that the next voxel's loads
+        // are in flight while this one is summed
+#pragma unroll(SEL ? 1 : 2)
         for (int iy = iy0; iy < iy1; ++iy, o += ostep) {
             const int dy = -s.hy + s.stride * iy;
-            if (!voxel_kept(V, keep, (iz * s.sy + iy) * s.sx + ix, o, x0 + dx, y0 + dy, z0 + dz)) continue;
+            if constexpr (SEL)
+                if (!voxel_kept(V, keep, (iz * s.sy + iy) * s.sx + ix, o, x0 + dx, y0 + dy, z0 + dz)) continue;
             const float fv = V.f[o] - pf, Y = (float)dy;
             float gr[3];
             grad_at<STORED>(V, o, (int)z0 + dz, (int)y0 + dy, (int)x0 + dx, gr[0], gr[1], gr[2]);
@@ -892,7 +896,7 @@ __device__ inline void ref_moments_columns(const IcgnArgs& a, const uint8_t* kee
     block_sum<NR, NWAVES>(m, out, red);
 }
 
-template <bool STORED>
+template <bool STORED, bool SEL>
 __global__ __launch_bounds__(BLOCK) void k_ref_moments(IcgnArgs a, const int64_t* __restrict__ list,
                                                        const int64_t* __restrict__ count, double* __restrict__ mom) {
     const int64_t slot = xcd_slot(blockIdx.x, *count, a.xcd);
@@ -905,7 +909,7 @@ __global__ __launch_bounds__(BLOCK) void k_ref_moments(IcgnArgs a, const int64_t
     if (!keep && !subset_in_volume(a.V, a.sub, x0, y0, z0)) return;  // refused by the iteration kernel
     double pfd = a.meanf ? a.meanf[n] : 0.0;
     const float pf = isfinite(pfd) ? (float)pfd : 0.0f;
-    ref_moments_columns<STORED>(a, keep, x0, y0, z0, pf, res, red);
+    ref_moments_columns<STORED, SEL>(a, keep, x0, y0, z0, pf, res, red);
     for (int q = threadIdx.x; q < NR; q += BLOCK) mom[n * MOM_STRIDE + q] = res[q];
 }
 
@@ -1944,6 +1948,16 @@ void launch_fused(int variant, const IcgnArgs& a, const int64_t* list, const int
     }
 }
 
+// Reference moments of every listed node; `sel`: a mask or keep rows select the subset voxels.
+template <bool STORED>
+void launch_ref_moments(bool sel, const IcgnArgs& a, const int64_t* list, const int64_t* count, int64_t nb, double* mom,
+                        hipStream_t st) {
+    if (sel)
+        hipLaunchKernelGGL((k_ref_moments<STORED, true>), dim3((unsigned)nb), dim3(BLOCK), 0, st, a, list, count, mom);
+    else
+        hipLaunchKernelGGL((k_ref_moments<STORED, false>), dim3((unsigned)nb), dim3(BLOCK), 0, st, a, list, count, mom);
+}
+
 // The valid-node count lives on the device (written by k_compact); grids are sized for all N
 // nodes and the workgroups past the count exit at once, so no host round trip is needed.
 template <int MODE, bool STORED>
@@ -1958,10 +1972,10 @@ void launch_icgn12(int variant, const IcgnArgs& a, const int64_t* list, const in
     } else {
         const int64_t nb = xcd_grid(N);
         double* mom = (double*)gbuf;  // variant 1 keeps the reference moments where variant 0 keeps scratch rows
-        hipLaunchKernelGGL((k_ref_moments<STORED>), dim3((unsigned)nb), dim3(BLOCK), 0, st, a, list, count, mom);
+        const bool sel = a.V.mask != nullptr || (a.split_index != nullptr && a.split_keep != nullptr);
+        launch_ref_moments<STORED>(sel, a, list, count, nb, mom, st);
         const size_t bits = sizeof(unsigned) * (size_t)(((a.sub.S + 31) / 32 + 3) & ~3);
         const bool buf = (int64_t)a.V.nx * a.V.ny * a.V.nz < ((int64_t)1 << 30);  // 32-bit byte offsets
-        const bool sel = a.V.mask != nullptr || (a.split_index != nullptr && a.split_keep != nullptr);
         if (sel) launch_fused<MODE, STORED, true>(variant, a, list, count, nb, bits, buf, mom, lds_budget, flags, st);
         else launch_fused<MODE, STORED, false>(variant, a, list, count, nb, bits, buf, mom, lds_budget, flags, st);
         timer_end(st);
@@ -1974,11 +1988,10 @@ template <int MODE, bool STORED>
 void launch_icgn3(int variant, const IcgnArgs& a, const int64_t* list, const int64_t* count, int64_t N, double* mom,
                   int lds_budget, int flags, hipStream_t st) {
     const int64_t nb = xcd_grid(N);
-    if (!(flags & ICGN3_REUSE_MOMENTS))
-        hipLaunchKernelGGL((k_ref_moments<STORED>), dim3((unsigned)nb), dim3(BLOCK), 0, st, a, list, count, mom);
+    const bool sel = a.V.mask != nullptr || (a.split_index != nullptr && a.split_keep != nullptr);
+    if (!(flags & ICGN3_REUSE_MOMENTS)) launch_ref_moments<STORED>(sel, a, list, count, nb, mom, st);
     const size_t bits = sizeof(unsigned) * (size_t)(((a.sub.S + 31) / 32 + 3) & ~3);
     const bool buf = (int64_t)a.V.nx * a.V.ny * a.V.nz < ((int64_t)1 << 30);
-    const bool sel = a.V.mask != nullptr || (a.split_index != nullptr && a.split_keep != nullptr);
     if (sel) launch_fused<MODE, STORED, true, true>(variant, a, list, count, nb, bits, buf, mom, lds_budget, flags, st);
     else launch_fused<MODE, STORED, false, true>(variant, a, list, count, nb, bits, buf, mom, lds_budget, flags, st);
     timer_end(st);
