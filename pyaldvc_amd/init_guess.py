@@ -368,17 +368,21 @@ def compute_initial_guess(f, g, mesh, para, previous=None, split_fraction=None):
 
 
 # ---------------------------------------------------------------------------- coarse-lattice start
-def _lattice_axes(mesh):
-    """Node positions along z, y and x of a full node grid."""
-    nz, ny, nx = (int(v) for v in mesh.grid_shape)
-    c = np.asarray(mesh.coordinates, dtype=np.float64).reshape(nz, ny, nx, 3)
-    return c[:, 0, 0, 2].copy(), c[0, :, 0, 1].copy(), c[0, 0, :, 0].copy()
+# The contract (coarse_init.py:33-128): solve the initial guess and the 12-DOF local step on every
+# k-th node along each axis of the node grid only, then carry U and F to every node by linear
+# interpolation in node position -- extended linearly past the last lattice node, non-finite lattice
+# values first replaced by the nearest finite node's -- so that the fine solve starts from both; with
+# fewer than two lattice nodes along an axis the plain initial guess is used and F0 is None.
+#
+# Here the lift is separable: the coarse table is interpolated along x, then y, then z, each a
+# two-tap linear map per fine grid line on the device (the fine nodes form a full grid, so this is
+# the trilinear interpolant at every node to rounding), and U and F travel as one 12-column table.
 
 
 def coarse_lattice(mesh, factor: int):
-    """``(coarse mesh, fine index)``: the nodes whose grid index is a multiple of ``factor`` along
-    every axis, as a mesh of their own (coarse_init.py:33-48); ``None`` for ``factor`` 1 or fewer than
-    two such nodes along an axis."""
+    """``(coarse mesh, fine rows)``: the nodes whose grid index is a multiple of ``factor`` along
+    every axis, as a mesh of their own (their validity carried over), and their rows in ``mesh``;
+    ``None`` for ``factor`` 1 or when an axis would keep fewer than two nodes."""
     from .local import NodeMesh
 
     k = int(factor)
@@ -386,92 +390,95 @@ def coarse_lattice(mesh, factor: int):
     picked = np.arange(mesh.n_nodes).reshape(grid)[::k, ::k, ::k] if k > 1 else None
     if picked is None or min(picked.shape) < MIN_COARSE_NODES_PER_AXIS:
         return None
-    fine_index = picked.ravel()
-    node_valid = np.asarray(mesh.node_valid, dtype=bool)
-    sub_valid = node_valid[fine_index].copy() if node_valid.size == mesh.n_nodes else np.ones(fine_index.size, dtype=bool)
-    return NodeMesh(np.asarray(mesh.coordinates, dtype=np.float64)[fine_index], picked.shape, sub_valid), fine_index
+    rows = picked.ravel()
+    valid = np.asarray(mesh.node_valid, dtype=bool)
+    sub_valid = valid[rows].copy() if valid.size == mesh.n_nodes else np.ones(rows.size, dtype=bool)
+    return NodeMesh(np.asarray(mesh.coordinates, dtype=np.float64)[rows], picked.shape, sub_valid), rows
 
 
-def _nearest_finite(table: torch.Tensor) -> torch.Tensor:
-    """Rows of ``table`` (nz, ny, nx, C) with a non-finite entry take the row of the nearest grid node
-    whose row is finite (Euclidean distance in grid steps; among equally near nodes the first in C
-    order). By brute force on the device, in blocks of holes."""
-    nz, ny, nx, C = table.shape
-    flat = table.reshape(-1, C)
-    finite = torch.isfinite(flat).all(1)
-    holes = torch.nonzero(~finite).squeeze(1)
-    if holes.numel() == 0:
+def _grid_axes(mesh):
+    """Node positions along z, y and x of a full node grid."""
+    nz, ny, nx = (int(v) for v in mesh.grid_shape)
+    c = np.asarray(mesh.coordinates, dtype=np.float64).reshape(nz, ny, nx, 3)
+    return c[:, 0, 0, 2].copy(), c[0, :, 0, 1].copy(), c[0, 0, :, 0].copy()
+
+
+def _fill_holes_from_nearest(table: torch.Tensor) -> torch.Tensor:
+    """``table`` (gz, gy, gx, C) with every row holding a non-finite entry replaced by the row of the
+    nearest grid node whose row is finite (Euclidean distance in grid steps; all zeros when no row
+    is finite). The reference takes the node from scipy's Euclidean distance transform
+    (coarse_init.py:70-78); this is the exact transform as three separable passes -- along z, then
+    y, then x, each position taking the position q along the line that minimises (p - q)^2 plus the
+    squared distance found so far at q, the lower q on a tie -- which also picks the transform's
+    node where several are equally near."""
+    shape = table.shape
+    rows = table.reshape(-1, shape[-1])
+    good = torch.isfinite(rows).all(1)
+    if bool(good.all()):
         return table
-    donors = torch.nonzero(finite).squeeze(1)
-    if donors.numel() == 0:
-        return torch.nan_to_num(table)
+    if not bool(good.any()):
+        return torch.zeros_like(table)
+    dev = table.device
+    d2 = torch.where(good, 0.0, float("inf")).to(torch.float64).reshape(shape[:3])
+    donor = torch.arange(rows.shape[0], device=dev).reshape(shape[:3])
+    for axis in range(3):
+        n = shape[axis]
+        pos = torch.arange(n, device=dev, dtype=torch.float64)
+        dm, om = d2.movedim(axis, -1), donor.movedim(axis, -1)  # (..., n)
+        cost = dm.unsqueeze(-2) + (pos[:, None] - pos[None, :]) ** 2  # (..., p, q)
+        q = torch.argmin(cost, -1)  # the first (lowest) q of equal cost
+        d2 = torch.gather(cost, -1, q.unsqueeze(-1)).squeeze(-1).movedim(-1, axis)
+        donor = torch.gather(om, -1, q).movedim(-1, axis)
+    return rows[donor.reshape(-1)].reshape(shape)
 
-    def ijk(idx):
-        return torch.stack([idx // (ny * nx), (idx // nx) % ny, idx % nx], 1).double()
 
-    pd = ijk(donors)
-    out = flat.clone()
-    step = max(1, (1 << 26) // max(int(donors.numel()), 1))
-    for a in range(0, int(holes.numel()), step):
-        h = holes[a : a + step]
-        dist = torch.cdist(ijk(h), pd)  # argmin returns the first of equal minima
-        out[h] = flat[donors[torch.argmin(dist, 1)]]
-    return out.reshape(table.shape)
+def _axis_lift(coarse_pos: torch.Tensor, fine_pos: torch.Tensor):
+    """``(lo, w)`` per fine position: the lattice interval [lo, lo + 1] it is interpolated on (the
+    first or last one beyond the lattice's ends) and its weight on lo + 1 (outside [0, 1] beyond the
+    ends: linear extrapolation)."""
+    lo = torch.clamp(torch.searchsorted(coarse_pos, fine_pos, right=True) - 1, 0, coarse_pos.numel() - 2)
+    return lo, (fine_pos - coarse_pos[lo]) / (coarse_pos[lo + 1] - coarse_pos[lo])
 
 
 def interpolate_to_mesh(coarse, values, mesh) -> np.ndarray:
-    """Trilinear interpolation of per-node values ``(Nc, ...)`` from the coarse lattice to every node
-    of ``mesh``, linear extrapolation beyond the lattice, non-finite coarse values replaced by the
-    nearest finite node's first (coarse_init.py:51-67). On the device in float64."""
+    """Per-node values ``(Nc, ...)`` on the lattice ``coarse`` carried to every node of ``mesh`` (the
+    contract above; float64 on the device)."""
     v = np.asarray(values, dtype=np.float64)
-    gz, gy, gx = (int(n) for n in coarse.grid_shape)
     dev = K.device()
-    table = _nearest_finite(torch.from_numpy(np.ascontiguousarray(v.reshape(gz, gy, gx, -1))).to(dev))
-    pts = torch.from_numpy(np.asarray(mesh.coordinates, dtype=np.float64)).to(dev)
-    idx, frac = [], []
-    for axis, col in zip(_lattice_axes(coarse), (2, 1, 0)):
-        a = torch.from_numpy(axis).to(dev)
-        x = pts[:, col].contiguous()
-        i = torch.clamp(torch.searchsorted(a, x, right=True) - 1, 0, a.numel() - 2)
-        idx.append(i)
-        frac.append((x - a[i]) / (a[i + 1] - a[i]))  # outside [0, 1] beyond the lattice: extrapolation
-    (iz, iy, ix), (tz, ty, tx) = idx, frac
-    out = torch.zeros((pts.shape[0], table.shape[-1]), dtype=torch.float64, device=dev)
-    for dz in (0, 1):
-        wz = tz if dz else 1.0 - tz
-        for dy in (0, 1):
-            wy = ty if dy else 1.0 - ty
-            for dx in (0, 1):
-                wx = tx if dx else 1.0 - tx
-                out += (wz * wy * wx)[:, None] * table[iz + dz, iy + dy, ix + dx]
-    return out.cpu().numpy().reshape((mesh.n_nodes,) + v.shape[1:])
+    table = torch.from_numpy(np.ascontiguousarray(v.reshape(tuple(int(n) for n in coarse.grid_shape) + (-1,)))).to(dev)
+    table = _fill_holes_from_nearest(table)
+    for dim, (cpos, fpos) in zip((2, 1, 0), zip(_grid_axes(coarse)[::-1], _grid_axes(mesh)[::-1])):
+        lo, w = _axis_lift(torch.from_numpy(cpos).to(dev), torch.from_numpy(fpos).to(dev))
+        shape = [1] * table.dim()
+        shape[dim] = -1
+        w = w.reshape(shape)
+        table = table.index_select(dim, lo) * (1.0 - w) + table.index_select(dim, lo + 1) * w
+    return table.reshape((mesh.n_nodes,) + v.shape[1:]).cpu().numpy()
 
 
 def coarse_initial_guess(ref, g_norm, g_prep, mesh, para):
-    """``(U0, F0, info)``: the initial guess and the 12-DOF local step on the coarse lattice of every
-    ``para.init_coarse_factor``-th node, U and F interpolated to all nodes (coarse_init.py:81-128);
-    with too small a lattice, the plain initial guess and ``F0 = None``."""
+    """``(U0, F0, info)`` from the coarse lattice of every ``para.init_coarse_factor``-th node
+    (coarse_init.py:81-128; the contract above). ``info`` holds the initial guess's own entries plus
+    ``method``, ``coarse_factor``, ``coarse_nodes``, ``coarse_converged``, ``coarse_bad`` and the
+    ``coarse_ncc_time``, ``coarse_icgn_time`` and ``coarse_total_time`` the reference reports."""
     from .local import local_icgn, precompute_local_context
 
-    t_start = time.perf_counter()
+    clock = time.perf_counter
+    t0 = clock()
     k = int(para.init_coarse_factor)
     lattice = coarse_lattice(mesh, k)
     if lattice is None:
         U0, info = compute_initial_guess(ref.f, g_norm, mesh, para)
-        info["coarse_factor"] = 1
-        return U0, None, info
-    coarse, _ = lattice
-    U0c, info = compute_initial_guess(ref.f, g_norm, coarse, para)
-    t_search = time.perf_counter() - t_start
-    ctx = precompute_local_context(coarse, ref, para)
-    Uc, Fc, solve, bad = local_icgn(ctx, ref, g_prep, U0c, para, coarse)
-    U0 = interpolate_to_mesh(coarse, Uc, mesh)
-    F0 = interpolate_to_mesh(coarse, np.asarray(Fc).reshape(coarse.n_nodes, 9), mesh).reshape(mesh.n_nodes, 3, 3)
+        return U0, None, {**info, "coarse_factor": 1}
+    sub = lattice[0]
+    U_sub, info = compute_initial_guess(ref.f, g_norm, sub, para)
+    t_guess = clock() - t0
+    U, F, solve, bad = local_icgn(precompute_local_context(sub, ref, para), ref, g_prep, U_sub, para, sub)
+    lifted = interpolate_to_mesh(sub, np.concatenate([np.asarray(U), np.asarray(F).reshape(sub.n_nodes, 9)], 1), mesh)
     converged = float(np.mean(solve.status == 0))
-    info.update({"method": f"coarse x{k} + {info.get('method', 'ncc')}", "coarse_factor": k,
-                 "coarse_nodes": int(coarse.n_nodes), "coarse_converged": converged, "coarse_bad": int(bad.sum()),
-                 "coarse_ncc_time": t_search, "coarse_icgn_time": float(solve.solve_time),
-                 "coarse_total_time": time.perf_counter() - t_start})  # fmt: skip
-    logger.info("Coarse start: %d nodes of %d solved on the x%d lattice, %.1f %% converged, %d inpainted",
-                coarse.n_nodes, mesh.n_nodes, k, 100.0 * converged, int(bad.sum()))  # fmt: skip
-    return U0, F0, info
+    info.update(method=f"coarse x{k} + {info.get('method', 'ncc')}", coarse_factor=k, coarse_nodes=int(sub.n_nodes),
+                coarse_converged=converged, coarse_bad=int(bad.sum()), coarse_ncc_time=t_guess,
+                coarse_icgn_time=float(solve.solve_time), coarse_total_time=clock() - t0)  # fmt: skip
+    logger.info("Coarse start x%d: %d lattice nodes for %d, %.1f %% converged, %d filled; guess %.2f s, IC-GN %.2f s",
+                k, sub.n_nodes, mesh.n_nodes, 100.0 * converged, int(bad.sum()), t_guess, solve.solve_time)  # fmt: skip
+    return lifted[:, :3].copy(), lifted[:, 3:].reshape(mesh.n_nodes, 3, 3), info
