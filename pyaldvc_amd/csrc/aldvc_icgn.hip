@@ -1210,6 +1210,11 @@ __global__ __launch_bounds__(BLOCK, 3) void k_icgn12_fused(IcgnArgs a, const int
                 // that does not fit (large rotation or shear, or a sample outside the domain) gathers
                 // from memory.
                 constexpr int OVER = MODE == INTERP_LINEAR ? 1 : 3;
+                // chunk plans (kz, fit, brick origin, brick extents), double-buffered by chunk parity: wave 0
+                // writes chunk c + 1's slot after the barrier that starts chunk c, when every wave has
+                // read chunk c - 1's
+                __shared__ int s_plan[2][8];
+                int chunk = 0;
                 const int sx = s.sx, sy = s.sy, plane_vox = sx * sy, lane = tid & 63;
                 // the voxels of box [xc, xc + kx) x [yc, yc + ky) x [zc, zc + kz), t = tid, tid + BLOCK,
                 // ... with x fastest, decoded per voxel (box chunks that gather, boxes of other lane maps)
@@ -1233,16 +1238,36 @@ __global__ __launch_bounds__(BLOCK, 3) void k_icgn12_fused(IcgnArgs a, const int
                         const int kx = min(tile_x, sx - xc), ky = min(tile_y, sy - yc);
                         const bool whole = kx == sx && ky == sy;
                         for (int zc = 0; zc < s.sz;) {
-                            int kz = min(brick_planes - OVER, s.sz - zc);
-                            int bx = 0, by = 0, bz = 0, ex = 0, ey = 0, ez = 0;
-                            int fit = plan_chunk<MODE, BRK>(sw, s, dom, xc, kx, yc, ky, zc, kz, brick_planes,
-                                                            brick_rows, bx, by, bz, ex, ey, ez);
-                            while (fit == 1 && kz > 1)
-                                fit = plan_chunk<MODE, BRK>(sw, s, dom, xc, kx, yc, ky, zc, --kz, brick_planes,
-                                                            brick_rows, bx, by, bz, ex, ey, ez);
+                            // wave 0 plans the chunk (the plan is uniform; the other waves do not repeat
+                            // it) and publishes it through LDS at the barrier that also ends the previous
+                            // chunk's use of the brick
+                            if (tid < 64) {
+                                int kz = min(brick_planes - OVER, s.sz - zc);
+                                int bx = 0, by = 0, bz = 0, ex = 0, ey = 0, ez = 0;
+                                int fit = plan_chunk<MODE, BRK>(sw, s, dom, xc, kx, yc, ky, zc, kz, brick_planes,
+                                                                brick_rows, bx, by, bz, ex, ey, ez);
+                                while (fit == 1 && kz > 1)
+                                    fit = plan_chunk<MODE, BRK>(sw, s, dom, xc, kx, yc, ky, zc, --kz, brick_planes,
+                                                                brick_rows, bx, by, bz, ex, ey, ez);
+                                if (tid == 0) {
+                                    int* pl = s_plan[chunk & 1];
+                                    pl[0] = kz;
+                                    pl[1] = fit;
+                                    pl[2] = bx;
+                                    pl[3] = by;
+                                    pl[4] = bz;
+                                    pl[5] = ex;
+                                    pl[6] = ey;
+                                    pl[7] = ez;
+                                }
+                            }
+                            __syncthreads();
+                            const int* pl = s_plan[chunk & 1];
+                            const int kz = pl[0], fit = pl[1], bx = pl[2], by = pl[3], bz = pl[4];
+                            const int ex = pl[5], ey = pl[6], ez = pl[7];
+                            ++chunk;
                             using Brick = std::integral_constant<bool, true>;
                             if (fit == 0) {
-                                __syncthreads();  // the previous chunk's samples are done with the brick
                                 stage_brick<BRK>(brick, brick_plane, ex, ey, gb, (int)obase, rowp, planep, bx, by, bz,
                                                  ez);
                                 __syncthreads();
