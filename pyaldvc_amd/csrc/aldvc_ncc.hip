@@ -411,13 +411,14 @@ constexpr int VPT = 9;   // offsets per thread and plane: vy * vx <= 256 * VPT (
 constexpr int VPL = 16;  // window values per thread and plane: wy * wx <= 256 * VPL
 
 // plane z of the node's window, less c, into registers (element tid + 256 u, row-major)
-__device__ inline void box_load(const NccArgs& a, const Bf16Geom& q, int64_t node, int z, float (&v)[VPL]) {
+template <int NV>
+__device__ inline void box_load(const NccArgs& a, const Bf16Geom& q, int64_t node, int z, float (&v)[NV]) {
     const int64_t xb = a.wx0[node], yb = a.wy0[node], zb = a.wz0[node];
     const float* gp = a.g + ((zb + z) * a.ny + yb) * (int64_t)a.nx + xb;
     const int wpl = q.wy * q.wx;
     const float inv_wx = 1.0f / (float)q.wx;  // i < 2^22: the float floor of (i + 0.5) / wx is exact
 #pragma unroll
-    for (int u = 0; u < VPL; ++u) {
+    for (int u = 0; u < NV; ++u) {
         const int i = threadIdx.x + BLOCK * u;
         if (i < wpl) {
             const int y = (int)(((float)i + 0.5f) * inv_wx), x = i - y * q.wx;
@@ -426,64 +427,15 @@ __device__ inline void box_load(const NccArgs& a, const Bf16Geom& q, int64_t nod
     }
 }
 
-// 2-D box sums of one plane held in registers (P1 of W - c, P2 of its squares)
-__device__ inline void box2d(const Bf16Geom& q, const float (&v)[VPL], float c, float* Pf, float* R1, float* R2,
-                             float* P1, float* P2) {
-    const int tid = threadIdx.x;
-    const int nseg_x = (q.vx + 7) / 8, nseg_y = (q.vy + 7) / 8;
-    const int wpl = q.wy * q.wx;
-    __syncthreads();  // the previous plane's readers are done
-#pragma unroll
-    for (int u = 0; u < VPL; ++u) {
-        const int i = tid + BLOCK * u;
-        if (i < wpl) Pf[i] = v[u] - c;
-    }
-    __syncthreads();
-    for (int t = tid; t < q.wy * nseg_x; t += BLOCK) {  // row sums, 8 outputs per item
-        const int y = t / nseg_x, ox0 = 8 * (t - y * nseg_x);
-        const float* row = Pf + y * q.wx;
-        float r1 = 0.0f, r2 = 0.0f;
-#pragma unroll 8
-        for (int x = 0; x < q.sx; ++x) {
-            const float w = row[ox0 + x];
-            r1 += w;
-            r2 += w * w;
-        }
-        R1[y * q.vx + ox0] = r1;
-        R2[y * q.vx + ox0] = r2;
-#pragma unroll
-        for (int ox = ox0 + 1; ox < ox0 + 8; ++ox) {
-            if (ox >= q.vx) break;
-            const float vin = row[ox + q.sx - 1], vout = row[ox - 1];
-            r1 += vin - vout;
-            r2 += vin * vin - vout * vout;
-            R1[y * q.vx + ox] = r1;
-            R2[y * q.vx + ox] = r2;
-        }
-    }
-    __syncthreads();
-    for (int t = tid; t < q.vx * nseg_y; t += BLOCK) {  // column sums, 8 outputs per item
-        const int ox = t / nseg_y, oy0 = 8 * (t - ox * nseg_y);
-        float r1 = 0.0f, r2 = 0.0f;
-#pragma unroll 8
-        for (int y = 0; y < q.sy; ++y) {
-            r1 += R1[(oy0 + y) * q.vx + ox];
-            r2 += R2[(oy0 + y) * q.vx + ox];
-        }
-        P1[oy0 * q.vx + ox] = r1;
-        P2[oy0 * q.vx + ox] = r2;
-#pragma unroll
-        for (int oy = oy0 + 1; oy < oy0 + 8; ++oy) {
-            if (oy >= q.vy) break;
-            r1 += R1[(oy + q.sy - 1) * q.vx + ox] - R1[(oy - 1) * q.vx + ox];
-            r2 += R2[(oy + q.sy - 1) * q.vx + ox] - R2[(oy - 1) * q.vx + ox];
-            P1[oy * q.vx + ox] = r1;
-            P2[oy * q.vx + ox] = r2;
-        }
-    }
-    __syncthreads();
-}
-
+// Window variance of every offset, z first: each thread keeps, for its elements of the window plane,
+// the running sums over the last sz planes of W - c and of its square (registers; the plane that
+// leaves is re-read and subtracted), and each output plane is then the 2-D box sum of those two
+// z-summed planes in LDS -- one 2-D pass per output plane rather than one per entering and per leaving
+// window plane. The 2-D pass: row sums over sx columns (running, 8 outputs per item), then column
+// sums over sy rows, written out as S2 - S1^2 / n.
+// NV: window elements per thread and plane (wy * wx <= BLOCK * NV), sized to the window so that the
+// six per-thread arrays stay in registers.
+template <int NV>
 __global__ __launch_bounds__(BLOCK, 3) void k_box_var(NccArgs a, Bf16Geom q) {
     extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
     const int tid = threadIdx.x;
@@ -491,52 +443,93 @@ __global__ __launch_bounds__(BLOCK, 3) void k_box_var(NccArgs a, Bf16Geom q) {
     double mf, ssf;
     if (!ncc_node_stats(a, q, node, mf, ssf)) return;  // the MFMA kernel writes the -2 map
     const float c = (float)mf;
-    float* Pf = reinterpret_cast<float*>(lds);  // wy x wx centred plane
-    float* R1 = Pf + q.wy * q.wx;                // wy x vx row sums
-    float* R2 = R1 + q.wy * q.vx;
-    float* P1 = R2 + q.wy * q.vx;  // vy x vx box sums
-    float* P2 = P1 + q.vy * q.vx;
-    const int nplane = q.vy * q.vx;
+    const int wpl = q.wy * q.wx, nplane = q.vy * q.vx;
+    float* Z1 = reinterpret_cast<float*>(lds);  // wy x wx: sums over the last sz planes of W - c
+    float* Z2 = Z1 + wpl;                       // ... and of (W - c)^2
+    float* R1 = Z2 + wpl;                       // wy x vx row sums of Z1
+    float* R2 = R1 + q.wy * q.vx;               // ... of Z2
     const float inv_n = 1.0f / (float)(q.sx * q.sy * q.sz);
+    const int nseg_x = (q.vx + 7) / 8, nseg_y = (q.vy + 7) / 8;
     float* o = a.out + node * (int64_t)nplane * q.vz;
-    float S1[VPT], S2[VPT];
+    float z1[NV], z2[NV];
 #pragma unroll
-    for (int k = 0; k < VPT; ++k) S1[k] = S2[k] = 0.0f;
-    float vin[VPL], vout[VPL], nin[VPL], nout[VPL];
+    for (int u = 0; u < NV; ++u) z1[u] = z2[u] = 0.0f;
+    float vin[NV], vout[NV], nin[NV], nout[NV];
     box_load(a, q, node, 0, vin);
     for (int z = 0; z < q.wz; ++z) {
         if (z + 1 < q.wz) box_load(a, q, node, z + 1, nin);  // next planes in flight
         if (z + 1 >= q.sz && z + 1 < q.wz) box_load(a, q, node, z + 1 - q.sz, nout);
-        box2d(q, vin, c, Pf, R1, R2, P1, P2);
 #pragma unroll
-        for (int k = 0; k < VPT; ++k) {
-            const int i = tid + BLOCK * k;
-            if (i < nplane) {
-                S1[k] += P1[i];
-                S2[k] += P2[i];
-            }
+        for (int u = 0; u < NV; ++u) {
+            const float w = vin[u] - c;
+            z1[u] += w;
+            z2[u] += w * w;
         }
         if (z >= q.sz) {  // the plane that left the box
-            box2d(q, vout, c, Pf, R1, R2, P1, P2);
 #pragma unroll
-            for (int k = 0; k < VPT; ++k) {
-                const int i = tid + BLOCK * k;
-                if (i < nplane) {
-                    S1[k] -= P1[i];
-                    S2[k] -= P2[i];
-                }
+            for (int u = 0; u < NV; ++u) {
+                const float w = vout[u] - c;
+                z1[u] -= w;
+                z2[u] -= w * w;
             }
         }
         if (z >= q.sz - 1) {
-            float* op = o + (int64_t)(z - q.sz + 1) * nplane;
+            // the z-summed planes into LDS (the previous output plane's readers passed the barrier
+            // after its row sums, so Z1/Z2 are free)
 #pragma unroll
-            for (int k = 0; k < VPT; ++k) {
-                const int i = tid + BLOCK * k;
-                if (i < nplane) op[i] = S2[k] - S1[k] * S1[k] * inv_n;
+            for (int u = 0; u < NV; ++u) {
+                const int i = tid + BLOCK * u;
+                if (i < wpl) {
+                    Z1[i] = z1[u];
+                    Z2[i] = z2[u];
+                }
             }
+            __syncthreads();
+            for (int t = tid; t < q.wy * nseg_x; t += BLOCK) {  // row sums, 8 outputs per item
+                const int y = t / nseg_x, ox0 = 8 * (t - y * nseg_x);
+                const float* r1p = Z1 + y * q.wx;
+                const float* r2p = Z2 + y * q.wx;
+                float r1 = 0.0f, r2 = 0.0f;
+#pragma unroll 8
+                for (int x = 0; x < q.sx; ++x) {
+                    r1 += r1p[ox0 + x];
+                    r2 += r2p[ox0 + x];
+                }
+                R1[y * q.vx + ox0] = r1;
+                R2[y * q.vx + ox0] = r2;
+#pragma unroll
+                for (int ox = ox0 + 1; ox < ox0 + 8; ++ox) {
+                    if (ox >= q.vx) break;
+                    r1 += r1p[ox + q.sx - 1] - r1p[ox - 1];
+                    r2 += r2p[ox + q.sx - 1] - r2p[ox - 1];
+                    R1[y * q.vx + ox] = r1;
+                    R2[y * q.vx + ox] = r2;
+                }
+            }
+            __syncthreads();
+            float* op = o + (int64_t)(z - q.sz + 1) * nplane;
+            for (int t = tid; t < q.vx * nseg_y; t += BLOCK) {  // column sums, 8 outputs per item
+                const int ox = t / nseg_y, oy0 = 8 * (t - ox * nseg_y);
+                float r1 = 0.0f, r2 = 0.0f;
+#pragma unroll 8
+                for (int y = 0; y < q.sy; ++y) {
+                    r1 += R1[(oy0 + y) * q.vx + ox];
+                    r2 += R2[(oy0 + y) * q.vx + ox];
+                }
+                op[oy0 * q.vx + ox] = r2 - r1 * r1 * inv_n;
+#pragma unroll
+                for (int oy = oy0 + 1; oy < oy0 + 8; ++oy) {
+                    if (oy >= q.vy) break;
+                    r1 += R1[(oy + q.sy - 1) * q.vx + ox] - R1[(oy - 1) * q.vx + ox];
+                    r2 += R2[(oy + q.sy - 1) * q.vx + ox] - R2[(oy - 1) * q.vx + ox];
+                    op[oy * q.vx + ox] = r2 - r1 * r1 * inv_n;
+                }
+            }
+            // the next output plane's Z1/Z2 stores wait for this plane's row sums at the barrier
+            // above; its row sums overwrite R1/R2 only after its own first barrier
         }
 #pragma unroll
-        for (int u = 0; u < VPL; ++u) {
+        for (int u = 0; u < NV; ++u) {
             vin[u] = nin[u];
             vout[u] = nout[u];
         }
@@ -1166,7 +1159,25 @@ size_t bf16_lds_bytes(const Bf16Geom& q) {
 }
 
 size_t box_lds_bytes(const Bf16Geom& q) {
-    return sizeof(float) * ((size_t)q.wy * q.wx + 2 * (size_t)q.wy * q.vx + 2 * (size_t)q.vy * q.vx);
+    return sizeof(float) * (2 * (size_t)q.wy * q.wx + 2 * (size_t)q.wy * q.vx);
+}
+
+// k_box_var over n nodes with the smallest instance whose per-thread arrays hold a window plane
+// (the caller has checked wy * wx <= BLOCK * VPL).
+int launch_box_var(const NccArgs& a, const Bf16Geom& q, int64_t n, hipStream_t stream) {
+    const size_t blds = box_lds_bytes(q);
+    const int64_t wpl = (int64_t)q.wy * q.wx;
+    auto go = [&](auto kern) -> int {
+        if (blds > 64 * 1024 &&
+            hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)blds) != hipSuccess)
+            return fail(-5, "could not raise the LDS limit");
+        hipLaunchKernelGGL(kern, dim3((unsigned)n), dim3(BLOCK), blds, stream, a, q);
+        return 0;
+    };
+    if (wpl <= (int64_t)BLOCK * 4) return go(k_box_var<4>);
+    if (wpl <= (int64_t)BLOCK * 8) return go(k_box_var<8>);
+    if (wpl <= (int64_t)BLOCK * 13) return go(k_box_var<13>);
+    return go(k_box_var<VPL>);
 }
 
 Bf16Geom bf16_geom(int hx, int hy, int hz, int rx, int ry, int rz, int& ks) {
@@ -1358,11 +1369,7 @@ int aldvc_zncc_map(const float* f, const float* g, int64_t nz, int64_t ny, int64
         if (fits && known) {
             const int nzt = (q.vz + SK_MZ - 1) / SK_MZ;
             if (n * nzt > INT32_MAX) return fail(-2, "too many workgroups for one launch");
-            const size_t blds = box_lds_bytes(q);
-            if (blds > 64 * 1024 &&
-                hipFuncSetAttribute((const void*)k_box_var, hipFuncAttributeMaxDynamicSharedMemorySize, (int)blds) != hipSuccess)
-                return fail(-5, "could not raise the LDS limit");
-            hipLaunchKernelGGL(k_box_var, dim3((unsigned)n), dim3(BLOCK), blds, stream, a, q);
+            if (int e = launch_box_var(a, q, n, stream)) return e;
 #define ALDVC_SKEW(KERN, LDSF, SYV, VYV, NCV)                                                               \
     if (q.sy == SYV && q.vy == VYV && q.nxt == NCV) {                                                      \
         const size_t sl = LDSF<SYV, VYV, NCV>();                                                            \
@@ -1398,11 +1405,7 @@ int aldvc_zncc_map(const float* f, const float* g, int64_t nz, int64_t ny, int64
         const int64_t nblk = n * ((q.vz + MG - 1) / MG);
         const int64_t nbox = n;
         if (nblk > INT32_MAX) return fail(-2, "too many workgroups for one launch");
-        const size_t blds = box_lds_bytes(q);
-        if (blds > 64 * 1024 &&
-            hipFuncSetAttribute((const void*)k_box_var, hipFuncAttributeMaxDynamicSharedMemorySize, (int)blds) != hipSuccess)
-            return fail(-5, "could not raise the LDS limit");
-        hipLaunchKernelGGL(k_box_var, dim3((unsigned)nbox), dim3(BLOCK), blds, stream, a, q);
+        if (int e = launch_box_var(a, q, nbox, stream)) return e;
         const void* fn = nullptr;
 #define ALDVC_BF16(NTV, KSV)                                                                                \
     if (nt <= NTV && ks == KSV) {                                                                          \
