@@ -20,7 +20,7 @@ import torch
 from . import kernels as K
 from ._lib import check, lib
 
-MAX_ELEMENT_EXTENT = 64  # voxels per axis of an element box (one 64-bit LDS word per row)
+MAX_ELEMENT_EXTENT = 64  # voxels per axis of an element box the LDS kernel takes (one 64-bit word per row)
 
 
 def hex8_elements(grid_shape) -> np.ndarray:
@@ -39,8 +39,10 @@ def hex8_elements(grid_shape) -> np.ndarray:
 
 def bridging_elements(mask, coordinates, elements) -> np.ndarray:
     """``(E,)`` True for every element (rows of 8 node indices) that spans a masked boundary
-    (mesh_cut.py:98-106). Element boxes, clamped to the volume, may be at most 64 voxels along
-    each axis (node steps up to 63); a wider one raises ``ValueError`` before any device work."""
+    (mesh_cut.py:98-106): its first corner is masked out, or some corner lies outside the first
+    corner's 6-connected in-mask component within the element's box (clamped to the volume).
+    Boxes of at most 64 voxels per axis run in the LDS kernel (one 64-bit word per box row); wider
+    ones (node steps of 64 and more) in :func:`_bridging_wide`."""
     elements = np.asarray(elements, dtype=np.int64)
     E = elements.shape[0]
     if E == 0:
@@ -48,23 +50,65 @@ def bridging_elements(mask, coordinates, elements) -> np.ndarray:
     shape = tuple(int(v) for v in np.shape(mask))
     corners = np.round(np.asarray(coordinates, dtype=np.float64)[elements]).astype(np.int64)  # (E, 8, 3)
     top = np.array(shape[::-1], dtype=np.int64) - 1  # x, y, z
-    lo = np.clip(corners.min(axis=1), 0, top)
-    hi = np.clip(corners.max(axis=1), 0, top)
-    extent = int((hi - lo).max()) + 1
-    if extent > MAX_ELEMENT_EXTENT:
-        raise ValueError(f"element boxes of at most {MAX_ELEMENT_EXTENT} voxels per axis are supported inside the "
-                         f"volume (this mesh has {extent}: node step {extent - 1})")  # fmt: skip
+    extent = (np.clip(corners.max(axis=1), 0, top) - np.clip(corners.min(axis=1), 0, top)).max(axis=1) + 1
     dev = K.device()
     m = K._Args.vol(mask, np.uint8)
     nz, ny, nx = (int(v) for v in m.shape)
-    cd = torch.from_numpy(np.ascontiguousarray(corners)).to(dev)
-    out = torch.zeros(E, dtype=torch.uint8, device=dev)
-    check(lib().aldvc_bridging_elements(K._ptr(m), nz, ny, nx, K._ptr(cd), E, extent, K._ptr(out), K.stream_handle()),
-          "aldvc_bridging_elements")  # fmt: skip
-    res = out.cpu().numpy()
-    if (res == 2).any():
-        raise RuntimeError("an element box exceeded the declared extent")
-    return res == 1
+    res = np.zeros(E, dtype=bool)
+    narrow = np.flatnonzero(extent <= MAX_ELEMENT_EXTENT)
+    if narrow.size:
+        cd = torch.from_numpy(np.ascontiguousarray(corners[narrow])).to(dev)
+        out = torch.zeros(narrow.size, dtype=torch.uint8, device=dev)
+        max_ext = int(extent[narrow].max())
+        check(lib().aldvc_bridging_elements(K._ptr(m), nz, ny, nx, K._ptr(cd), int(narrow.size), max_ext, K._ptr(out),
+                                            K.stream_handle()), "aldvc_bridging_elements")  # fmt: skip
+        got = out.cpu().numpy()
+        if (got == 2).any():
+            raise RuntimeError("an element box exceeded the declared extent")
+        res[narrow] = got == 1
+    wide = np.flatnonzero(extent > MAX_ELEMENT_EXTENT)
+    if wide.size:
+        res[wide] = _bridging_wide(m, corners[wide])
+    return res
+
+
+def _bridging_wide(m: torch.Tensor, corners: np.ndarray, budget: int = 1 << 28) -> np.ndarray:
+    """The bridging test of :func:`bridging_elements` for element boxes of any size, on the device:
+    the boxes of a batch are cut out of the mask into one zero-padded boolean tensor, and the first
+    corner's component grows by one face step per round (the union of the six shifted copies, kept
+    inside the mask) until a round adds nothing. ``budget``: voxels per batch."""
+    nz, ny, nx = (int(v) for v in m.shape)
+    top = np.array([nx - 1, ny - 1, nz - 1], dtype=np.int64)
+    lo = np.clip(corners.min(axis=1), 0, top)
+    hi = np.clip(corners.max(axis=1), 0, top)
+    local = np.clip(corners - lo[:, None, :], 0, (hi - lo)[:, None, :])  # (E, 8, 3) x, y, z in the box
+    Sx, Sy, Sz = (int(v) for v in (hi - lo).max(axis=0) + 1)
+    per = max(1, budget // (Sx * Sy * Sz))
+    out = np.zeros(corners.shape[0], dtype=bool)
+    for b0 in range(0, corners.shape[0], per):
+        idx = range(b0, min(corners.shape[0], b0 + per))
+        sub = torch.zeros((len(idx), Sz, Sy, Sx), dtype=torch.bool, device=m.device)
+        for j, e in enumerate(idx):
+            (x0, y0, z0), (x1, y1, z1) = lo[e], hi[e]
+            sub[j, : z1 - z0 + 1, : y1 - y0 + 1, : x1 - x0 + 1] = m[z0 : z1 + 1, y0 : y1 + 1, x0 : x1 + 1] != 0
+        lz, ly, lx = (torch.from_numpy(local[list(idx), :, a]).to(m.device) for a in (2, 1, 0))  # (B, 8)
+        rows = torch.arange(len(idx), device=m.device)
+        comp = torch.zeros_like(sub)
+        comp[rows, lz[:, 0], ly[:, 0], lx[:, 0]] = sub[rows, lz[:, 0], ly[:, 0], lx[:, 0]]
+        while True:
+            grown = comp.clone()
+            for d in (1, 2, 3):  # z, y, x
+                n = comp.shape[d]
+                if n > 1:
+                    grown.narrow(d, 1, n - 1).logical_or_(comp.narrow(d, 0, n - 1))
+                    grown.narrow(d, 0, n - 1).logical_or_(comp.narrow(d, 1, n - 1))
+            grown.logical_and_(sub)
+            if torch.equal(grown, comp):
+                break
+            comp = grown
+        reached = comp[rows[:, None], lz, ly, lx]  # (B, 8)
+        out[list(idx)] = (~reached.all(dim=1)).cpu().numpy()
+    return out
 
 
 # For each axis (x, y, z): the corners at the low end of the element's four edges along that axis,

@@ -54,6 +54,26 @@ def test_bridging_elements_match_connected_components(step, seed):
     assert np.array_equal(got, ref)
 
 
+def test_wide_and_narrow_element_boxes_together():
+    """Node steps of 70 voxels along x and y (boxes wider than the LDS kernel's 64) with 12 along z:
+    every element through the device flood fill, against the brute-force labelling."""
+    shape = (50, 160, 180)
+    rng = np.random.default_rng(9)
+    mask = np.ones(shape, np.uint8)
+    mask[:, :, 100:102] = 0
+    mask[24:26, :, :] = 0
+    mask[24:26, 50:60, 20:30] = 1
+    mask &= (rng.random(shape) > 0.02).astype(np.uint8)
+    ax = [np.array([8.0, 78.0, 148.0, 171.0]), np.array([6.0, 76.0, 146.0]), np.array([6.0, 18.0, 30.0, 42.0])]
+    Z, Y, X = np.meshgrid(ax[2], ax[1], ax[0], indexing="ij")
+    coords = np.column_stack([X.ravel(), Y.ravel(), Z.ravel()])
+    elements = MC.hex8_elements((ax[2].size, ax[1].size, ax[0].size))
+    got = MC.bridging_elements(mask, coords, elements)
+    ref = _brute_bridging(mask, coords, elements)
+    assert ref.any() and not ref.all()
+    assert np.array_equal(got, ref)
+
+
 def test_the_mesh_is_cut_at_a_wall():
     """tests/test_subset_split.py:250-275: a wall three voxels wide through the volume; the dropped
     elements are exactly the ones that cross it, and the cut edges follow from them."""
