@@ -106,7 +106,11 @@ int aldvc_icgn12(const int64_t *coords, int64_t N, double *P, int hx, int hy, in
  * per workgroup for the brick (0: three workgroups per CU; at most 160 KiB), flags = 1: no
  * plane-pair lane mapping, 2: node list in index order instead of spatial tiles, 4: workgroup b
  * takes list slot b instead of the XCD-aware mapping, 8: no 8 x 8 lane patches (the default
- * mapping of the brick path). Same reference function as aldvc_icgn12. */
+ * mapping of the brick path), 32: reference moments by the separable pass over the distinct node
+ * coordinates when the gradients are stored, no mask or keep rows select voxels and stride is 1
+ * (double sums; the workspace then also holds aldvc_icgn12_row_moments_bytes past
+ * aldvc_icgn12_workspace_bytes, else, or when the node set needs more, the per-node pass runs).
+ * Same reference function as aldvc_icgn12. */
 int aldvc_icgn12_tuned(const int64_t *coords, int64_t N, double *P, int hx, int hy, int hz, int stride,
                        const float *f, const float *gx, const float *gy, const float *gz, const uint8_t *mask,
                        const float *g, int64_t nz, int64_t ny, int64_t nx, int mode, const double *L_all,
@@ -115,6 +119,12 @@ int aldvc_icgn12_tuned(const int64_t *coords, int64_t N, double *P, int hx, int 
                        const int64_t *split_index, const uint8_t *split_keep, int64_t keep_bytes, int32_t *n_iter,
                        int8_t *status, double *zncc, int variant, int lds_budget, int flags, void *workspace,
                        size_t workspace_bytes, hipStream_t stream);
+
+/* Extra workspace of flag 32 of aldvc_icgn12_tuned / aldvc_icgn3_tuned for nodes with ncx distinct x,
+ * ncy distinct y and z within a range of nplanes planes including the subset half-widths
+ * (max z - min z + 2 hz + 1), over those whose subsets lie inside the volume. 0 when a count is
+ * not positive. No counterpart in the reference (a workspace size). */
+size_t aldvc_icgn12_row_moments_bytes(int64_t ncx, int64_t ncy, int64_t nplanes, int64_t ny, int64_t nx);
 
 /* aldvc_icgn12 that allocates nothing. No counterpart in the reference: for callers that cannot
  * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
@@ -163,7 +173,8 @@ int aldvc_icgn3_gated(const int64_t *coords, int64_t N, const double *U_old, dou
 
 /* aldvc_icgn3_gated (gate may be NULL) with the launch choices made explicit, as aldvc_icgn12_tuned:
  * variant = 1: LDS bricks where a chunk's warped stencils fit one (subsets wider than a brick in
- * x/y boxes), 2: memory gathers only; lds_budget and flags as aldvc_icgn12_tuned, plus flag 16:
+ * x/y boxes), 2: memory gathers only; lds_budget and flags as aldvc_icgn12_tuned (flag 32 with its
+ * workspace past aldvc_icgn3_workspace_bytes), plus flag 16:
  * the workspace already holds these nodes' reference moments from a previous call with the same
  * nodes, reference volume, gradients, mask / keep rows and meanf (they are then not recomputed;
  * an ADMM loop's later 3-DOF passes). The other two 3-DOF entries run variant 1 with lds_budget 0

@@ -209,6 +209,13 @@ class AdmmLoop:
         sp = ctx.split_args()
         self.split_args = sp
         self.split = K._split(K._Args(self.gd), N, sp.get("split_index"), sp.get("split_keep"))
+        # the first 3-DOF pass's reference moments by the separable pass, where it applies
+        d = ctx.device
+        extra = K.row_moments_bytes(d["coords"], d["valid"], *ctx.half, ctx.stride, tuple(vols["f"].shape),
+                                    vols["stored"], vols["mask"] is not None, self.split[0] is not None)  # fmt: skip
+        self.row_flag = K.TUNE_ROW_MOMENTS if extra else 0
+        if extra:
+            self.ws_icgn3 = _ws(L.aldvc_icgn3_workspace_bytes(N) + extra)
         self.tuning3 = None  # launch shape of the 3-DOF solves, chosen by the autotuner in load()
         if system.solver == "direct":
             self.pcg_tol, self.pcg_max = 1e-13, int(min(max(system.pcg_max_iter, 10 * ops.n_active), DIRECT_GRAPH_ITER))
@@ -249,7 +256,7 @@ class AdmmLoop:
         D.noise_gain = float(self.gain) if self.gain > 0 else 0.0
         D.predictive = int(bool(p.icgn_predictive_stop))
         D.split_index, D.split_keep, D.keep_bytes = _addr(si), _addr(sk), int(kb)
-        D.variant, D.lds_budget, D.flags = int(t.variant), int(t.lds_budget), int(t.flags)
+        D.variant, D.lds_budget, D.flags = int(t.variant), int(t.lds_budget), int(t.flags) | self.row_flag
         # cleanup (solver/subpb1_solver.py:140-164)
         D.nx, D.ny, D.nz = nx, ny, nz
         D.edge_ok = _addr(self.edge_ok)

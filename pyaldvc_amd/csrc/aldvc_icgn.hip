@@ -896,9 +896,13 @@ __device__ inline void ref_moments_columns(const IcgnArgs& a, const uint8_t* kee
     block_sum<NR, NWAVES>(m, out, red);
 }
 
+// skip: non-null when the separable pass below may have taken every node (its flag, set on the
+// device, says whether it did)
 template <bool STORED, bool SEL>
 __global__ __launch_bounds__(BLOCK) void k_ref_moments(IcgnArgs a, const int64_t* __restrict__ list,
-                                                       const int64_t* __restrict__ count, double* __restrict__ mom) {
+                                                       const int64_t* __restrict__ count, double* __restrict__ mom,
+                                                       const int* __restrict__ skip) {
+    if (skip && *skip) return;
     const int64_t slot = xcd_slot(blockIdx.x, *count, a.xcd);
     if (slot < 0) return;
     const int64_t n = list[slot];
@@ -911,6 +915,250 @@ __global__ __launch_bounds__(BLOCK) void k_ref_moments(IcgnArgs a, const int64_t
     const float pf = isfinite(pfd) ? (float)pfd : 0.0f;
     ref_moments_columns<STORED, SEL>(a, keep, x0, y0, z0, pf, res, red);
     for (int q = threadIdx.x; q < NR; q += BLOCK) mom[n * MOM_STRIDE + q] = res[q];
+}
+
+// ---------------------------------------------------------------------------- separable reference moments
+// With every subset voxel kept (no mask, no keep rows, stride 1), a node's reference moments are
+// box sums over its subset cube of per-voxel terms: f, f^2, grad_c, grad_c f, and the grad terms
+// times the voxel's offset X, Y or Z from the node, with the pilot pf folded in afterwards
+// (sum f' = sum f - n pf, ...). Nodes on a lattice share most of these sums: the x-sums of one
+// voxel row serve every node whose centre has that x and whose subset holds the row, and so on
+// along y and z. So the sums are taken axis by axis at the distinct node coordinates only:
+//   k_rs_xy: per plane z, distinct centre x (cx) and distinct centre y (cy), the 20 sums over the
+//            sx x sy rectangle around (cx, cy): x-row sums in float (four lanes a row, a quarter
+//            each), accumulated along y as running prefix sums in double, whose differences
+//            between a window's end and start are the rectangle sums;
+//   k_rs_z:  per node, its sz planes' rectangle sums, with the Z weights, in the layout of
+//            k_ref_moments.
+// A voxel is then read by about sx / (x spacing) lanes instead of once per node whose subset holds
+// it (27 times on a lattice of step hx / 2). The distinct coordinates come from marks over the
+// node list (k_rs_mark, k_rs_index). Rectangle sums: R[((p * RS_Q + q) * ncy + cyi) * ncx + cxi]
+// for plane p = z - zbase, with q = 0 f, 1 f^2, 2 + c grad_c, 5 + c grad_c X, 8 + c grad_c f,
+// 11 + c grad_c f X, 14 + c grad_c Y, 17 + c grad_c f Y.
+constexpr int RS_Q = 20;
+constexpr int RS_SEG = 16;   // centres y per work item of k_rs_xy
+constexpr int RS_TILE = 16;  // centres x per work item (four lanes each)
+enum : int { RS_NCX, RS_NCY, RS_ZLO, RS_ZHI, RS_FITS, RS_NPLANES, RS_ZBASE, RS_HDR = 64 };
+enum : int { TUNE_ROW_MOMENTS = 32 };
+
+// Device buffers of the separable pass, carved from the workspace after the node list and moments.
+struct RowMoments {
+    int* hdr;   // RS_HDR ints: the counts above; set on the device
+    int* xidx;  // nx: mark of each x, then the index of the centre x it is (-1: none)
+    int* yidx;  // ny
+    int* cxs;   // the distinct centre x, ascending
+    int* cys;   // the distinct centre y
+    double* R;  // rectangle sums
+    int64_t r_doubles;  // room for R
+};
+
+__host__ __device__ inline size_t row_moments_fixed_bytes(int64_t ny, int64_t nx) {
+    return sizeof(int) * RS_HDR + (sizeof(int) * 2 * (size_t)(nx + ny) + 255) / 256 * 256;
+}
+
+__global__ void k_rs_clear(RowMoments r, int nx, int ny) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nx) r.xidx[i] = 0;
+    if (i < ny) r.yidx[i] = 0;
+    if (i < RS_HDR) r.hdr[i] = i == RS_ZLO ? INT_MAX : (i == RS_ZHI ? -1 : 0);
+}
+
+// Marks the centres of the listed nodes whose subsets lie inside the volume (the others are
+// refused by the iteration kernel and need no moments).
+__global__ void k_rs_mark(IcgnArgs a, const int64_t* __restrict__ list, const int64_t* __restrict__ count,
+                          RowMoments r) {
+    const int64_t M = *count;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < M; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t n = list[i];
+        const int64_t x0 = a.coords[3 * n], y0 = a.coords[3 * n + 1], z0 = a.coords[3 * n + 2];
+        if (!subset_in_volume(a.V, a.sub, x0, y0, z0)) continue;
+        r.xidx[x0] = 1;
+        r.yidx[y0] = 1;
+        atomicMin(&r.hdr[RS_ZLO], (int)z0);
+        atomicMax(&r.hdr[RS_ZHI], (int)z0);
+    }
+}
+
+// marks -> indices and the ascending list of marked positions; returns the count (one block of
+// 1024 threads, every thread calls it)
+__device__ inline int rs_index_axis(int* idx, int* out, int n, int* wsum) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int base = 0;
+    for (int c0 = 0; c0 < n; c0 += 1024) {
+        const int i = c0 + threadIdx.x;
+        const bool m = i < n && idx[i] != 0;
+        const unsigned long long b = __ballot(m);
+        if (lane == 0) wsum[wave] = __popcll(b);
+        __syncthreads();
+        int before = base;
+        for (int w = 0; w < wave; ++w) before += wsum[w];
+        int total = base;
+        for (int w = 0; w < 16; ++w) total += wsum[w];
+        const int at = before + (int)__popcll(b & ((1ull << lane) - 1));
+        if (i < n) {
+            idx[i] = m ? at : -1;
+            if (m) out[at] = i;
+        }
+        base = total;
+        __syncthreads();  // wsum is rewritten by the next chunk
+    }
+    return base;
+}
+
+// Distinct centres and whether the rectangle sums fit the room given (else every node takes the
+// per-node pass, k_ref_moments).
+__global__ __launch_bounds__(1024) void k_rs_index(RowMoments r, int nx, int ny, int hz) {
+    __shared__ int wsum[16];
+    const int ncx = rs_index_axis(r.xidx, r.cxs, nx, wsum);
+    const int ncy = rs_index_axis(r.yidx, r.cys, ny, wsum);
+    if (threadIdx.x == 0) {
+        const int zlo = r.hdr[RS_ZLO], zhi = r.hdr[RS_ZHI];
+        const int64_t planes = zhi >= zlo ? (int64_t)(zhi - zlo) + 2 * hz + 1 : 0;
+        r.hdr[RS_NCX] = ncx;
+        r.hdr[RS_NCY] = ncy;
+        r.hdr[RS_NPLANES] = (int)planes;
+        r.hdr[RS_ZBASE] = zlo - hz;
+        r.hdr[RS_FITS] = planes > 0 && ncx > 0 && ncy > 0 && planes * RS_Q * ncy * (int64_t)ncx <= r.r_doubles;
+    }
+}
+
+// sum over the four lanes of a quad
+__device__ inline double quad_sum(double v) {
+    v += swizzle_xor<1>(v);
+    return v + swizzle_xor<2>(v);
+}
+
+// One wave per work item (plane, RS_SEG centres y, RS_TILE centres x); lane 4 j + k takes centre x
+// j of the tile and the k-th quarter of its rows. Stored gradients only.
+__global__ __launch_bounds__(BLOCK) void k_rs_xy(Volumes V, Subset s, RowMoments r) {
+    if (!r.hdr[RS_FITS]) return;
+    const int ncx = r.hdr[RS_NCX], ncy = r.hdr[RS_NCY], zbase = r.hdr[RS_ZBASE];
+    const int nseg = (ncy + RS_SEG - 1) / RS_SEG, ntile = (ncx + RS_TILE - 1) / RS_TILE;
+    const int64_t items = (int64_t)r.hdr[RS_NPLANES] * nseg * ntile;
+    const int lane = threadIdx.x & 63, j = lane >> 2, part = lane & 3;
+    const int kper = (s.sx + 3) >> 2, k0 = part * kper, k1 = min(s.sx, k0 + kper);
+    const int64_t plane = (int64_t)V.nx * V.ny, cstride = (int64_t)ncy * ncx;
+    for (int64_t it = blockIdx.x * (int64_t)NWAVES + (threadIdx.x >> 6); it < items;
+         it += (int64_t)gridDim.x * NWAVES) {
+        const int p = (int)(it / ((int64_t)nseg * ntile));
+        const int rem = (int)(it - (int64_t)p * nseg * ntile), seg = rem / ntile, tile = rem - seg * ntile;
+        const int cxi = tile * RS_TILE + j;
+        const bool act = cxi < ncx;
+        const int cx = act ? r.cxs[cxi] : r.cxs[0];
+        const int i0 = seg * RS_SEG, i1 = min(ncy, i0 + RS_SEG);
+        const int ylo = r.cys[i0] - s.hy, yhi = r.cys[i1 - 1] + s.hy;
+        const int64_t obase = (int64_t)(zbase + p) * plane + (cx - s.hx);
+        double* Rp = r.R + (int64_t)p * RS_Q * cstride + cxi;
+        double P[RS_Q];
+#pragma unroll
+        for (int q = 0; q < RS_Q; ++q) P[q] = 0.0;
+        int si = i0, ei = i0;
+        for (int y = ylo; y <= yhi; ++y) {
+            while (si < i1 && r.cys[si] - s.hy == y) {  // a window starts: keep -prefix
+#pragma unroll
+                for (int q = 0; q < RS_Q; ++q) {
+                    const double v = quad_sum(P[q]);
+                    if (act && part == 0) Rp[(int64_t)q * cstride + (int64_t)si * ncx] = -v;
+                }
+                ++si;
+            }
+            const int64_t o = obase + (int64_t)y * V.nx;
+            float m[14];
+#pragma unroll
+            for (int q = 0; q < 14; ++q) m[q] = 0.0f;
+            for (int k = k0; k < k1; ++k) {
+                const float fv = V.f[o + k], X = (float)(k - s.hx);
+                const float g[3] = {V.gx[o + k], V.gy[o + k], V.gz[o + k]};
+                m[0] += fv;
+                m[1] = __fmaf_rn(fv, fv, m[1]);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const float tf = g[c] * fv;
+                    m[2 + c] += g[c];
+                    m[5 + c] = __fmaf_rn(g[c], X, m[5 + c]);
+                    m[8 + c] += tf;
+                    m[11 + c] = __fmaf_rn(tf, X, m[11 + c]);
+                }
+            }
+            const double Y = (double)y;
+#pragma unroll
+            for (int q = 0; q < 14; ++q) P[q] += (double)m[q];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                P[14 + c] = fma(Y, (double)m[2 + c], P[14 + c]);
+                P[17 + c] = fma(Y, (double)m[8 + c], P[17 + c]);
+            }
+            while (ei < i1 && r.cys[ei] + s.hy == y) {  // a window ends: prefix now minus at its start
+                double D[RS_Q];
+#pragma unroll
+                for (int q = 0; q < RS_Q; ++q) {
+                    const double v = quad_sum(P[q]);
+                    D[q] = act && part == 0 ? Rp[(int64_t)q * cstride + (int64_t)ei * ncx] + v : 0.0;
+                }
+                const double cy = (double)r.cys[ei];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {  // absolute y weights -> offsets from the centre
+                    D[14 + c] -= cy * D[2 + c];
+                    D[17 + c] -= cy * D[8 + c];
+                }
+                if (act && part == 0)
+#pragma unroll
+                    for (int q = 0; q < RS_Q; ++q) Rp[(int64_t)q * cstride + (int64_t)ei * ncx] = D[q];
+                ++ei;
+            }
+        }
+    }
+}
+
+// Per node: the sz planes' rectangle sums with the Z weights, the pilot folded in, written where
+// k_ref_moments writes (same layout, double sums throughout).
+__global__ __launch_bounds__(BLOCK) void k_rs_z(IcgnArgs a, const int64_t* __restrict__ list,
+                                                const int64_t* __restrict__ count, RowMoments r,
+                                                double* __restrict__ mom) {
+    if (!r.hdr[RS_FITS]) return;
+    const int64_t M = *count;
+    const int ncx = r.hdr[RS_NCX], ncy = r.hdr[RS_NCY], zbase = r.hdr[RS_ZBASE];
+    const int64_t cstride = (int64_t)ncy * ncx;
+    const Subset& s = a.sub;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < M; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t n = list[i];
+        const int64_t x0 = a.coords[3 * n], y0 = a.coords[3 * n + 1], z0 = a.coords[3 * n + 2];
+        if (!subset_in_volume(a.V, s, x0, y0, z0)) continue;
+        const double* Rn = r.R + (int64_t)r.yidx[y0] * ncx + r.xidx[x0];
+        double S[RS_Q], SZ[6];
+#pragma unroll
+        for (int q = 0; q < RS_Q; ++q) S[q] = 0.0;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) SZ[q] = 0.0;
+        for (int dz = -s.hz; dz <= s.hz; ++dz) {
+            const double* Rz = Rn + (int64_t)(z0 + dz - zbase) * RS_Q * cstride;
+            const double Z = (double)dz;
+#pragma unroll
+            for (int q = 0; q < RS_Q; ++q) {
+                const double v = Rz[q * cstride];
+                S[q] += v;
+                if (q >= 2 && q < 5) SZ[q - 2] = fma(Z, v, SZ[q - 2]);
+                if (q >= 8 && q < 11) SZ[q - 5] = fma(Z, v, SZ[q - 5]);
+            }
+        }
+        const double pfd = a.meanf ? a.meanf[n] : 0.0;
+        const double pf = isfinite(pfd) ? (double)(float)pfd : 0.0;  // the per-node pass's float pilot
+        const double nv = (double)s.S;
+        double* out = mom + n * MOM_STRIDE;
+        out[0] = nv;
+        out[1] = S[0] - nv * pf;
+        out[2] = S[1] - 2.0 * pf * S[0] + nv * pf * pf;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const double gq[4] = {S[5 + c], S[14 + c], SZ[c], S[2 + c]};
+            const double fq[4] = {S[11 + c], S[17 + c], SZ[3 + c], S[8 + c]};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                out[3 + 4 * c + q] = gq[q];
+                out[15 + 4 * c + q] = fq[q] - pf * gq[q];
+            }
+        }
+    }
 }
 
 // Chunk plan of the LDS-brick path. The box of subset voxels [xc, xc + kx) x [yc, yc + ky) x
@@ -1974,20 +2222,36 @@ void launch_fused(int variant, const IcgnArgs& a, const int64_t* list, const int
 }
 
 // Reference moments of every listed node; `sel`: a mask or keep rows select the subset voxels.
+// `rm`: buffers of the separable pass (null: per node only); it runs for stored gradients with every
+// subset voxel kept, and the per-node pass then exits unless the rectangle sums did not fit.
 template <bool STORED>
-void launch_ref_moments(bool sel, const IcgnArgs& a, const int64_t* list, const int64_t* count, int64_t nb, double* mom,
-                        hipStream_t st) {
+void launch_ref_moments(bool sel, const IcgnArgs& a, const int64_t* list, const int64_t* count, int64_t N, int64_t nb,
+                        double* mom, const RowMoments* rm, hipStream_t st) {
+    const int* skip = nullptr;
+    if (STORED && !sel && rm && a.sub.stride == 1) {
+        const int nx = a.V.nx, ny = a.V.ny;
+        const unsigned nodes = (unsigned)std::min<int64_t>((N + 255) / 256, 4096);
+        hipLaunchKernelGGL(k_rs_clear, dim3((unsigned)((std::max(std::max(nx, ny), (int)RS_HDR) + 255) / 256)),
+                           dim3(256), 0, st, *rm, nx, ny);
+        hipLaunchKernelGGL(k_rs_mark, dim3(nodes), dim3(256), 0, st, a, list, count, *rm);
+        hipLaunchKernelGGL(k_rs_index, dim3(1), dim3(1024), 0, st, *rm, nx, ny, a.sub.hz);
+        hipLaunchKernelGGL(k_rs_xy, dim3(2048), dim3(BLOCK), 0, st, a.V, a.sub, *rm);
+        hipLaunchKernelGGL(k_rs_z, dim3(nodes), dim3(BLOCK), 0, st, a, list, count, *rm, mom);
+        skip = rm->hdr + RS_FITS;
+    }
     if (sel)
-        hipLaunchKernelGGL((k_ref_moments<STORED, true>), dim3((unsigned)nb), dim3(BLOCK), 0, st, a, list, count, mom);
+        hipLaunchKernelGGL((k_ref_moments<STORED, true>), dim3((unsigned)nb), dim3(BLOCK), 0, st, a, list, count, mom,
+                           skip);
     else
-        hipLaunchKernelGGL((k_ref_moments<STORED, false>), dim3((unsigned)nb), dim3(BLOCK), 0, st, a, list, count, mom);
+        hipLaunchKernelGGL((k_ref_moments<STORED, false>), dim3((unsigned)nb), dim3(BLOCK), 0, st, a, list, count, mom,
+                           skip);
 }
 
 // The valid-node count lives on the device (written by k_compact); grids are sized for all N
 // nodes and the workgroups past the count exit at once, so no host round trip is needed.
 template <int MODE, bool STORED>
 void launch_icgn12(int variant, const IcgnArgs& a, const int64_t* list, const int64_t* count, int64_t N, float* gbuf,
-                   int lds_budget, int flags, hipStream_t st) {
+                   int lds_budget, int flags, const RowMoments* rm, hipStream_t st) {
     if (variant == 0) {
         for (int64_t off = 0; off < N; off += PLAIN_CHUNK) {
             const int64_t cnt = std::min<int64_t>(PLAIN_CHUNK, N - off);
@@ -1998,7 +2262,7 @@ void launch_icgn12(int variant, const IcgnArgs& a, const int64_t* list, const in
         const int64_t nb = xcd_grid(N);
         double* mom = (double*)gbuf;  // variant 1 keeps the reference moments where variant 0 keeps scratch rows
         const bool sel = a.V.mask != nullptr || (a.split_index != nullptr && a.split_keep != nullptr);
-        launch_ref_moments<STORED>(sel, a, list, count, nb, mom, st);
+        launch_ref_moments<STORED>(sel, a, list, count, N, nb, mom, rm, st);
         const size_t bits = sizeof(unsigned) * (size_t)(((a.sub.S + 31) / 32 + 3) & ~3);
         const bool buf = (int64_t)a.V.nx * a.V.ny * a.V.nz < ((int64_t)1 << 30);  // 32-bit byte offsets
         if (sel) launch_fused<MODE, STORED, true>(variant, a, list, count, nb, bits, buf, mom, lds_budget, flags, st);
@@ -2011,10 +2275,10 @@ void launch_icgn12(int variant, const IcgnArgs& a, const int64_t* list, const in
 // the subset's warped rows fit one, else buffer or pointer gathers).
 template <int MODE, bool STORED>
 void launch_icgn3(int variant, const IcgnArgs& a, const int64_t* list, const int64_t* count, int64_t N, double* mom,
-                  int lds_budget, int flags, hipStream_t st) {
+                  int lds_budget, int flags, const RowMoments* rm, hipStream_t st) {
     const int64_t nb = xcd_grid(N);
     const bool sel = a.V.mask != nullptr || (a.split_index != nullptr && a.split_keep != nullptr);
-    if (!(flags & ICGN3_REUSE_MOMENTS)) launch_ref_moments<STORED>(sel, a, list, count, nb, mom, st);
+    if (!(flags & ICGN3_REUSE_MOMENTS)) launch_ref_moments<STORED>(sel, a, list, count, N, nb, mom, rm, st);
     const size_t bits = sizeof(unsigned) * (size_t)(((a.sub.S + 31) / 32 + 3) & ~3);
     const bool buf = (int64_t)a.V.nx * a.V.ny * a.V.nz < ((int64_t)1 << 30);
     if (sel) launch_fused<MODE, STORED, true, true>(variant, a, list, count, nb, bits, buf, mom, lds_budget, flags, st);
@@ -2177,6 +2441,36 @@ int aldvc_precompute(const int64_t* coords, int64_t N, int hx, int hy, int hz, i
     return launch_status();
 }
 
+}  // extern "C"
+
+namespace {
+// Buffers of the separable reference-moment pass in the workspace past the `base` bytes every call
+// needs, when the call asks for it (TUNE_ROW_MOMENTS), the fused path runs on stored gradients with
+// every subset voxel kept, and the workspace holds more than the fixed part.
+const RowMoments* row_moments_at(RowMoments& rm, int flags, int variant, const Volumes& V, const Subset& s,
+                                 bool keep_rows, void* ws, size_t ws_bytes, size_t base) {
+    if (!(flags & TUNE_ROW_MOMENTS) || variant == 0 || !V.gx || V.mask || keep_rows || s.stride != 1) return nullptr;
+    const size_t fixed = row_moments_fixed_bytes(V.ny, V.nx);
+    if (ws_bytes < base + fixed + RS_Q * sizeof(double)) return nullptr;
+    char* p = (char*)ws + base;
+    rm.hdr = (int*)p;
+    rm.xidx = rm.hdr + RS_HDR;
+    rm.yidx = rm.xidx + V.nx;
+    rm.cxs = rm.yidx + V.ny;
+    rm.cys = rm.cxs + V.nx;
+    rm.R = (double*)(p + fixed);
+    rm.r_doubles = (int64_t)((ws_bytes - base - fixed) / sizeof(double));
+    return &rm;
+}
+}  // namespace
+
+extern "C" {
+
+size_t aldvc_icgn12_row_moments_bytes(int64_t ncx, int64_t ncy, int64_t nplanes, int64_t ny, int64_t nx) {
+    if (ncx <= 0 || ncy <= 0 || nplanes <= 0) return 0;
+    return row_moments_fixed_bytes(ny, nx) + sizeof(double) * RS_Q * (size_t)nplanes * (size_t)ncy * (size_t)ncx;
+}
+
 size_t aldvc_icgn12_workspace_bytes(int64_t N, int hx, int hy, int hz, int stride, int variant) {
     const Subset s = make_subset(hx, hy, hz, stride);
     size_t bytes = 256 + sizeof(int64_t) * (size_t)(N > 0 ? N : 1);  // count + node list
@@ -2196,7 +2490,7 @@ int aldvc_icgn12_tuned(const int64_t* coords, int64_t N, double* P, int hx, int 
                        size_t workspace_bytes, hipStream_t stream) {
     clear_error();
     if (lds_budget < 0 || lds_budget > 160 * 1024) return fail(-2, "lds_budget must be 0 or at most 160 KiB");
-    if (flags & ~(TUNE_NO_PAIRS | TUNE_INDEX_ORDER | TUNE_NO_XCD | TUNE_NO_PATCHES))
+    if (flags & ~(TUNE_NO_PAIRS | TUNE_INDEX_ORDER | TUNE_NO_XCD | TUNE_NO_PATCHES | TUNE_ROW_MOMENTS))
         return fail(-2, "unknown tuning flags");
     Volumes V{f, gx, gy, gz, g, mask, (int)nz, (int)ny, (int)nx};
     const Subset s = make_subset(hx, hy, hz, stride);
@@ -2227,10 +2521,13 @@ int aldvc_icgn12_tuned(const int64_t* coords, int64_t N, double* P, int hx, int 
     IcgnArgs a{V, s, coords, P, L_all, H_all, pattern, meanf, split_index, split_keep, keep_bytes,
                noise_gain, tol, dp_tol, max_iter, patience, predictive, n_iter, status, zncc,
                (flags & TUNE_NO_XCD) ? 0 : 1};
+    RowMoments rm;
+    const RowMoments* rmp = row_moments_at(rm, flags, variant, V, s, split_index && split_keep, workspace,
+                                           workspace_bytes, need);
     const bool stored = gx != nullptr;
 #define ALDVC_DISPATCH(MD)                                                                                \
-    if (stored) launch_icgn12<MD, true>(variant, a, list, count, N, gbuf, lds_budget, flags, stream);    \
-    else launch_icgn12<MD, false>(variant, a, list, count, N, gbuf, lds_budget, flags, stream);
+    if (stored) launch_icgn12<MD, true>(variant, a, list, count, N, gbuf, lds_budget, flags, rmp, stream);    \
+    else launch_icgn12<MD, false>(variant, a, list, count, N, gbuf, lds_budget, flags, rmp, stream);
     switch (mode) {
         case 0: ALDVC_DISPATCH(0) break;
         case 1: ALDVC_DISPATCH(1) break;
@@ -2292,7 +2589,8 @@ int aldvc_icgn3_tuned(const int64_t* coords, int64_t N, const double* U_old, dou
     clear_error();
     if (variant != 1 && variant != 2) return fail(-2, "variant must be 1 (LDS bricks where they fit) or 2 (gathers only)");
     if (lds_budget < 0 || lds_budget > 160 * 1024) return fail(-2, "lds_budget must be 0 or at most 160 KiB");
-    if (flags & ~(TUNE_NO_PAIRS | TUNE_INDEX_ORDER | TUNE_NO_XCD | TUNE_NO_PATCHES | ICGN3_REUSE_MOMENTS))
+    if (flags & ~(TUNE_NO_PAIRS | TUNE_INDEX_ORDER | TUNE_NO_XCD | TUNE_NO_PATCHES | ICGN3_REUSE_MOMENTS |
+                  TUNE_ROW_MOMENTS))
         return fail(-2, "unknown tuning flags");
     Volumes V{f, gx, gy, gz, g, mask, (int)nz, (int)ny, (int)nx};
     const Subset s = make_subset(hx, hy, hz, stride);
@@ -2346,10 +2644,13 @@ int aldvc_icgn3_tuned(const int64_t* coords, int64_t N, const double* U_old, dou
     a.bottomf = bottomf;
     a.U = U;
     a.mu = mu;
+    RowMoments rm;
+    const RowMoments* rmp = row_moments_at(rm, flags, variant, V, s, split_index && split_keep, workspace,
+                                           workspace_bytes, need);
     const bool stored = gx != nullptr;
 #define ALDVC_DISPATCH3(MD)                                                                           \
-    if (stored) launch_icgn3<MD, true>(variant, a, list, count, N, mom, lds_budget, flags, stream);    \
-    else launch_icgn3<MD, false>(variant, a, list, count, N, mom, lds_budget, flags, stream);
+    if (stored) launch_icgn3<MD, true>(variant, a, list, count, N, mom, lds_budget, flags, rmp, stream);    \
+    else launch_icgn3<MD, false>(variant, a, list, count, N, mom, lds_budget, flags, rmp, stream);
     switch (mode) {
         case 0: ALDVC_DISPATCH3(0) break;
         case 1: ALDVC_DISPATCH3(1) break;

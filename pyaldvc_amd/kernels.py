@@ -293,6 +293,32 @@ def build_split_rows(coords, cand, hx, hy, hz, stride, mask, margin=0):
 
 # --------------------------------------------------------------------------- 12-DOF IC-GN
 
+TUNE_ROW_MOMENTS = 32  # aldvc_icgn12_tuned / aldvc_icgn3_tuned: separable reference moments
+
+
+def row_moments_bytes(c, vd, hx, hy, hz, stride, dims, stored, masked, split) -> int:
+    """Extra workspace of the separable reference-moment pass (flag ``TUNE_ROW_MOMENTS``) for the
+    nodes ``c`` (device ``(N, 3)`` int64, x y z), 0 when they take the per-node pass: sized from the
+    distinct x and y and the z range of the valid nodes whose subsets lie inside the volume (one
+    device-to-host read of four counts). The pass needs stored gradients, every subset voxel kept
+    (no mask, no keep rows) and stride 1."""
+    if not stored or masked or split or int(stride) != 1 or c.shape[0] == 0:
+        return 0
+    nz, ny, nx = dims
+    x, y, z = c[:, 0], c[:, 1], c[:, 2]
+    ok = (vd != 0) & (x >= hx) & (x < nx - hx) & (y >= hy) & (y < ny - hy) & (z >= hz) & (z < nz - hz)
+    mx = torch.zeros(nx + 1, dtype=torch.int64, device=c.device)
+    my = torch.zeros(ny + 1, dtype=torch.int64, device=c.device)
+    mx[torch.where(ok, x, nx)] = 1
+    my[torch.where(ok, y, ny)] = 1
+    zlo = torch.where(ok, z, nz).min()
+    zhi = torch.where(ok, z, -1).max()
+    ncx, ncy, zlo, zhi = (int(v) for v in torch.stack([mx[:nx].sum(), my[:ny].sum(), zlo, zhi]).tolist())
+    if zhi < zlo:
+        return 0
+    return int(lib().aldvc_icgn12_row_moments_bytes(ncx, ncy, zhi - zlo + 2 * int(hz) + 1, ny, nx))
+
+
 
 def icgn_12dof_parallel(
     coords, P0, hx, hy, hz, f, gx, gy, gz, mask, g, mode, L_all, meanf_all, bottomf_all, valid,
@@ -347,6 +373,11 @@ def icgn_12dof_parallel(
     zncc = torch.zeros(N, dtype=torch.float64, device=dev)
     var = VARIANTS[variant]
     ws_bytes = int(lib().aldvc_icgn12_workspace_bytes(N, int(hx), int(hy), int(hz), int(stride), var))
+    extra = row_moments_bytes(c, vd, hx, hy, hz, stride, (nz, ny, nx), gxd is not None, md is not None,
+                              si is not None) if var != 0 else 0  # fmt: skip
+    if extra:
+        flags |= TUNE_ROW_MOMENTS
+        ws_bytes += extra
     ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
     check(
         lib().aldvc_icgn12_tuned(
@@ -415,6 +446,8 @@ def icgn_3dof_parallel(
     status = torch.zeros(N, dtype=torch.int8, device=dev)
     zncc = torch.zeros(N, dtype=torch.float64, device=dev)
     ws_bytes = int(lib().aldvc_icgn3_workspace_bytes(N))
+    extra = row_moments_bytes(c, vd, hx, hy, hz, stride, (nz, ny, nx), gxd is not None, md is not None, si is not None)
+    ws_bytes += extra
     ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
     check(
         lib().aldvc_icgn3_tuned(
@@ -423,7 +456,8 @@ def icgn_3dof_parallel(
             _ptr(Hd), _ptr(mf), _ptr(bf), _ptr(vd), float(mu), float(tol), float(dp_tol), int(max_iter),
             int(patience), float(noise_gain) if noise_gain > 0.0 else 0.0, int(bool(predictive)),
             _ptr(si), _ptr(sk), kb, _ptr(n_iter), _ptr(status), _ptr(zncc), None, int(tuning.variant),
-            int(tuning.lds_budget), int(tuning.flags), _ptr(ws), ws_bytes, stream_handle(stream),
+            int(tuning.lds_budget), int(tuning.flags) | (TUNE_ROW_MOMENTS if extra else 0), _ptr(ws), ws_bytes,
+            stream_handle(stream),
         ),
         "aldvc_icgn3_tuned",
     )  # fmt: skip

@@ -1,0 +1,142 @@
+"""Reference moments of the fused 12-DOF kernel: the separable pass over the distinct node
+coordinates (tuning flag 32) and the per-node pass, read back from the workspace and checked
+against float64 sums over each subset computed here from their definition
+(0 n, 1 sum f', 2 sum f'^2, 3 + 4c + j sum grad_c q_j, 15 + 4c + j sum grad_c q_j f', with
+q = (X, Y, Z, 1) the voxel's offset from the node and f' = f - pilot). Nodes: a lattice, irregular
+points, duplicates, nodes whose subsets leave the volume and invalid ones."""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+HALF = (5, 6, 4)  # hx, hy, hz
+ROW_FLAG = 32
+
+
+def _p(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+@pytest.fixture(scope="module")
+def case():
+    import torch
+
+    from pyaldvc_amd import kernels as K
+
+    assert torch.cuda.is_available()
+    nz, ny, nx = 40, 44, 52
+    rng = np.random.default_rng(3)
+    f_np = (rng.standard_normal((nz, ny, nx)) * 2.0 + 7.0).astype(np.float32)
+    f = torch.from_numpy(f_np).cuda()
+    gx, gy, gz = K.compute_gradients(f)
+    hx, hy, hz = HALF
+    lat = np.stack(np.meshgrid(np.arange(hx, nx - hx, 7), np.arange(hy, ny - hy, 5), np.arange(hz, nz - hz, 6),
+                               indexing="ij"), -1).reshape(-1, 3)  # fmt: skip
+    odd = np.column_stack([rng.integers(hx, nx - hx, 40), rng.integers(hy, ny - hy, 40), rng.integers(hz, nz - hz, 40)])
+    edge = np.array([[hx - 1, 20, 20], [20, ny - hy, 20], [20, 20, 0], [nx - 1, ny - 1, nz - 1]])
+    coords = np.concatenate([lat, odd, lat[:9], edge]).astype(np.int64)
+    rng.shuffle(coords)
+    N = coords.shape[0]
+    valid = (rng.random(N) > 0.1).astype(np.uint8)
+    meanf = f_np.mean() + rng.standard_normal(N) * 0.5
+    meanf[:5] = np.nan  # no pilot
+    return dict(f=f, gx=gx, gy=gy, gz=gz, f_np=f_np, g_np=[t.cpu().numpy() for t in (gx, gy, gz)], coords=coords,
+                valid=valid, meanf=meanf)  # fmt: skip
+
+
+def _moments(c, flags, extra):
+    """Runs aldvc_icgn12_tuned (one iteration) and returns the moments of every node (N, 27)."""
+    import torch
+
+    from pyaldvc_amd import _lib
+
+    lib = _lib.lib()
+    nz, ny, nx = c["f"].shape
+    N = c["coords"].shape[0]
+    hx, hy, hz = HALF
+    cd = torch.from_numpy(c["coords"]).cuda()
+    P = torch.zeros((N, 12), dtype=torch.float64, device="cuda")
+    L = torch.eye(12, dtype=torch.float64).reshape(1, 144).repeat(N, 1).cuda()
+    meanf = torch.from_numpy(c["meanf"]).cuda()
+    valid = torch.from_numpy(c["valid"]).cuda()
+    n_iter = torch.zeros(N, dtype=torch.int32, device="cuda")
+    status = torch.zeros(N, dtype=torch.int8, device="cuda")
+    zncc = torch.zeros(N, dtype=torch.float64, device="cuda")
+    base = int(lib.aldvc_icgn12_workspace_bytes(N, hx, hy, hz, 1, 1))
+    ws = torch.full((base + extra,), 0xFF, dtype=torch.uint8, device="cuda")
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    rc = lib.aldvc_icgn12_tuned(
+        _p(cd), N, _p(P), hx, hy, hz, 1, _p(c["f"]), _p(c["gx"]), _p(c["gy"]), _p(c["gz"]), None, _p(c["f"]),
+        nz, ny, nx, 1, _p(L), None, None, 0.0, _p(meanf), _p(valid), 1e-2, 1e-3, 1, 5, 1, None, None, 0,
+        _p(n_iter), _p(status), _p(zncc), 1, 0, flags, _p(ws), base + extra, stream,
+    )  # fmt: skip
+    _lib.check(rc, "aldvc_icgn12_tuned")
+    torch.cuda.synchronize()
+    list_bytes = (256 + 8 * N + 255) // 256 * 256
+    mom = ws[list_bytes : list_bytes + 8 * 32 * N].cpu().numpy().view(np.float64).reshape(N, 32)
+    return mom[:, :27].copy()
+
+
+def _expected(c):
+    """float64 moments and the sums of the terms' magnitudes, per node whose subset is inside."""
+    f, gs = c["f_np"].astype(np.float64), [g.astype(np.float64) for g in c["g_np"]]
+    hx, hy, hz = HALF
+    nz, ny, nx = f.shape
+    Z, Y, X = np.meshgrid(np.arange(-hz, hz + 1), np.arange(-hy, hy + 1), np.arange(-hx, hx + 1), indexing="ij")
+    q = [X.astype(float), Y.astype(float), Z.astype(float), np.ones(X.shape)]
+    out, mag, inside = {}, {}, []
+    for n, (x0, y0, z0) in enumerate(c["coords"]):
+        if not (c["valid"][n] and hx <= x0 < nx - hx and hy <= y0 < ny - hy and hz <= z0 < nz - hz):
+            continue
+        inside.append(n)
+        sl = (slice(z0 - hz, z0 + hz + 1), slice(y0 - hy, y0 + hy + 1), slice(x0 - hx, x0 + hx + 1))
+        pf = float(np.float32(c["meanf"][n])) if np.isfinite(c["meanf"][n]) else 0.0
+        fp = f[sl] - pf
+        terms = [np.ones(fp.shape), fp, fp * fp] + [None] * 24
+        for cc in range(3):
+            g = gs[cc][sl]
+            for j in range(4):
+                terms[3 + 4 * cc + j] = g * q[j]
+                terms[15 + 4 * cc + j] = g * q[j] * fp
+        out[n] = np.array([t.sum() for t in terms])
+        mag[n] = np.array([np.abs(t).sum() for t in terms])
+    return out, mag, inside
+
+
+def _check(got, exp, mag, inside, rel):
+    for n in inside:
+        err = np.abs(got[n] - exp[n])
+        assert np.all(err <= rel * mag[n] + 1e-9), (n, np.max(err / (mag[n] + 1e-30)))
+
+
+def test_row_moments_match_definition(case):
+    from pyaldvc_amd import kernels as K
+    from pyaldvc_amd import _lib
+
+    exp, mag, inside = _expected(case)
+    assert len(inside) > 100
+    import torch
+
+    nz, ny, nx = case["f"].shape
+    cd = torch.from_numpy(case["coords"]).cuda()
+    vd = torch.from_numpy(case["valid"]).cuda()
+    extra = K.row_moments_bytes(cd, vd, *HALF, 1, (nz, ny, nx), True, False, False)
+    sel = case["coords"][case["valid"] != 0]
+    hx, hy, hz = HALF
+    ins = sel[(sel[:, 0] >= hx) & (sel[:, 0] < nx - hx) & (sel[:, 1] >= hy) & (sel[:, 1] < ny - hy)
+              & (sel[:, 2] >= hz) & (sel[:, 2] < nz - hz)]  # fmt: skip
+    planes = int(ins[:, 2].max() - ins[:, 2].min()) + 2 * hz + 1
+    assert extra == _lib.lib().aldvc_icgn12_row_moments_bytes(
+        len(np.unique(ins[:, 0])), len(np.unique(ins[:, 1])), planes, ny, nx)  # fmt: skip
+    rows = _moments(case, ROW_FLAG, extra)
+    _check(rows, exp, mag, inside, 1e-6)
+    node = _moments(case, 0, 0)
+    _check(node, exp, mag, inside, 1e-5)
+    # the flag with too little room for the rectangle sums: the per-node pass, bit for bit
+    small = _moments(case, ROW_FLAG, 4096 + 64)
+    assert np.array_equal(small[inside], node[inside])
