@@ -926,8 +926,8 @@ __global__ __launch_bounds__(BLOCK) void k_ref_moments(IcgnArgs a, const int64_t
 // along y and z. So the sums are taken axis by axis at the distinct node coordinates only:
 //   k_rs_xy: per plane z, distinct centre x (cx) and distinct centre y (cy), the 20 sums over the
 //            sx x sy rectangle around (cx, cy): x-row sums in float (four lanes a row, a quarter
-//            each), accumulated along y as running prefix sums in double, whose differences
-//            between a window's end and start are the rectangle sums;
+//            each, combined), accumulated along y as running prefix sums in double, whose
+//            differences between a window's end and start are the rectangle sums;
 //   k_rs_z:  per node, its sz planes' rectangle sums, with the Z weights, in the layout of
 //            k_ref_moments.
 // A voxel is then read by about sx / (x spacing) lanes instead of once per node whose subset holds
@@ -938,6 +938,7 @@ __global__ __launch_bounds__(BLOCK) void k_ref_moments(IcgnArgs a, const int64_t
 constexpr int RS_Q = 20;
 constexpr int RS_SEG = 16;   // centres y per work item of k_rs_xy
 constexpr int RS_TILE = 16;  // centres x per work item (four lanes each)
+constexpr int RS_KC = 9;     // voxels of a quarter row loaded at once
 enum : int { RS_NCX, RS_NCY, RS_ZLO, RS_ZHI, RS_FITS, RS_NPLANES, RS_ZBASE, RS_HDR = 64 };
 enum : int { TUNE_ROW_MOMENTS = 32 };
 
@@ -1022,15 +1023,22 @@ __global__ __launch_bounds__(1024) void k_rs_index(RowMoments r, int nx, int ny,
     }
 }
 
-// sum over the four lanes of a quad
-__device__ inline double quad_sum(double v) {
-    v += swizzle_xor<1>(v);
-    return v + swizzle_xor<2>(v);
+// sum over the four lanes of a quad (DPP quad permutations: lanes xor 1, then xor 2)
+__device__ inline float quad_sum(float v) {
+    v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, true));
+    return v + __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, true));
 }
 
+// Rectangle sums a quad lane keeps, five per lane: RS_DST[part][i] is the sum q of slot i, taken
+// from the row sum RS_SRC[part][i] (times y for slots 1 and 3 of parts 0-2: the y-weighted grad
+// sums sit beside their unweighted ones, 0 and 2, which the window's y offsets need).
+__constant__ const int8_t RS_DST[4][5] = {{2, 14, 8, 17, 0}, {3, 15, 9, 18, 1}, {4, 16, 10, 19, 5}, {6, 7, 11, 12, 13}};
+constexpr int8_t RS_SRC[4][5] = {{2, 2, 8, 8, 0}, {3, 3, 9, 9, 1}, {4, 4, 10, 10, 5}, {6, 7, 11, 12, 13}};
+
 // One wave per work item (plane, RS_SEG centres y, RS_TILE centres x); lane 4 j + k takes centre x
-// j of the tile and the k-th quarter of its rows. Stored gradients only.
-__global__ __launch_bounds__(BLOCK) void k_rs_xy(Volumes V, Subset s, RowMoments r) {
+// j of the tile and the k-th quarter of each row. The quad's row sums are combined every row and
+// each lane keeps the prefix sums of five of the 20 rectangle sums. Stored gradients only.
+__global__ __launch_bounds__(BLOCK, 4) void k_rs_xy(Volumes V, Subset s, RowMoments r) {
     if (!r.hdr[RS_FITS]) return;
     const int ncx = r.hdr[RS_NCX], ncy = r.hdr[RS_NCY], zbase = r.hdr[RS_ZBASE];
     const int nseg = (ncy + RS_SEG - 1) / RS_SEG, ntile = (ncx + RS_TILE - 1) / RS_TILE;
@@ -1038,8 +1046,13 @@ __global__ __launch_bounds__(BLOCK) void k_rs_xy(Volumes V, Subset s, RowMoments
     const int lane = threadIdx.x & 63, j = lane >> 2, part = lane & 3;
     const int kper = (s.sx + 3) >> 2, k0 = part * kper, k1 = min(s.sx, k0 + kper);
     const int64_t plane = (int64_t)V.nx * V.ny, cstride = (int64_t)ncy * ncx;
-    for (int64_t it = blockIdx.x * (int64_t)NWAVES + (threadIdx.x >> 6); it < items;
-         it += (int64_t)gridDim.x * NWAVES) {
+    const bool ywt = part < 3;  // slots 1 and 3 are y-weighted
+    int64_t qoff[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) qoff[i] = (int64_t)RS_DST[part][i] * cstride;
+    // the work item is the same on every lane of a wave: in scalar registers
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (int64_t it = blockIdx.x * (int64_t)NWAVES + wave; it < items; it += (int64_t)gridDim.x * NWAVES) {
         const int p = (int)(it / ((int64_t)nseg * ntile));
         const int rem = (int)(it - (int64_t)p * nseg * ntile), seg = rem / ntile, tile = rem - seg * ntile;
         const int cxi = tile * RS_TILE + j;
@@ -1047,63 +1060,76 @@ __global__ __launch_bounds__(BLOCK) void k_rs_xy(Volumes V, Subset s, RowMoments
         const int cx = act ? r.cxs[cxi] : r.cxs[0];
         const int i0 = seg * RS_SEG, i1 = min(ncy, i0 + RS_SEG);
         const int ylo = r.cys[i0] - s.hy, yhi = r.cys[i1 - 1] + s.hy;
-        const int64_t obase = (int64_t)(zbase + p) * plane + (cx - s.hx);
+        const int64_t pbase = (int64_t)(zbase + p) * plane;
+        const unsigned xoff = (unsigned)(cx - s.hx);  // lane's first voxel in the row
         double* Rp = r.R + (int64_t)p * RS_Q * cstride + cxi;
-        double P[RS_Q];
-#pragma unroll
-        for (int q = 0; q < RS_Q; ++q) P[q] = 0.0;
+        double P[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
         int si = i0, ei = i0;
         for (int y = ylo; y <= yhi; ++y) {
-            while (si < i1 && r.cys[si] - s.hy == y) {  // a window starts: keep -prefix
+            if (si < i1 && r.cys[si] - s.hy == y) {  // a window starts: keep -prefix (centres are distinct)
+                if (act)
 #pragma unroll
-                for (int q = 0; q < RS_Q; ++q) {
-                    const double v = quad_sum(P[q]);
-                    if (act && part == 0) Rp[(int64_t)q * cstride + (int64_t)si * ncx] = -v;
-                }
+                    for (int i = 0; i < 5; ++i) Rp[qoff[i] + (int64_t)si * ncx] = -P[i];
                 ++si;
             }
-            const int64_t o = obase + (int64_t)y * V.nx;
+            // the row's base is the same on every lane; lanes differ by 32-bit byte offsets
+            const int64_t orow = pbase + (int64_t)y * V.nx;
+            const char *fr = (const char*)(V.f + orow), *g0r = (const char*)(V.gx + orow),
+                       *g1r = (const char*)(V.gy + orow), *g2r = (const char*)(V.gz + orow);
             float m[14];
 #pragma unroll
             for (int q = 0; q < 14; ++q) m[q] = 0.0f;
-            for (int k = k0; k < k1; ++k) {
-                const float fv = V.f[o + k], X = (float)(k - s.hx);
-                const float g[3] = {V.gx[o + k], V.gy[o + k], V.gz[o + k]};
-                m[0] += fv;
-                m[1] = __fmaf_rn(fv, fv, m[1]);
+            // RS_KC voxels at a time, every load issued before the first is used (the lane's last
+            // voxel stands in for those past its quarter and counts zero)
+            for (int kc = k0; kc < k1; kc += RS_KC) {
+                float fv[RS_KC], g[3][RS_KC];
 #pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    const float tf = g[c] * fv;
-                    m[2 + c] += g[c];
-                    m[5 + c] = __fmaf_rn(g[c], X, m[5 + c]);
-                    m[8 + c] += tf;
-                    m[11 + c] = __fmaf_rn(tf, X, m[11 + c]);
+                for (int t = 0; t < RS_KC; ++t) {
+                    const unsigned b = 4u * (xoff + (unsigned)min(kc + t, k1 - 1));
+                    fv[t] = *(const float*)(fr + b);
+                    g[0][t] = *(const float*)(g0r + b);
+                    g[1][t] = *(const float*)(g1r + b);
+                    g[2][t] = *(const float*)(g2r + b);
+                }
+#pragma unroll
+                for (int t = 0; t < RS_KC; ++t) {
+                    const bool in = kc + t < k1;
+                    const float f1 = in ? fv[t] : 0.0f, X = (float)(kc + t - s.hx);
+                    m[0] += f1;
+                    m[1] = __fmaf_rn(f1, f1, m[1]);
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        const float gc = in ? g[c][t] : 0.0f, tf = gc * f1;
+                        m[2 + c] += gc;
+                        m[5 + c] = __fmaf_rn(gc, X, m[5 + c]);
+                        m[8 + c] += tf;
+                        m[11 + c] = __fmaf_rn(tf, X, m[11 + c]);
+                    }
                 }
             }
+#pragma unroll
+            for (int q = 0; q < 14; ++q) m[q] = quad_sum(m[q]);
             const double Y = (double)y;
 #pragma unroll
-            for (int q = 0; q < 14; ++q) P[q] += (double)m[q];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                P[14 + c] = fma(Y, (double)m[2 + c], P[14 + c]);
-                P[17 + c] = fma(Y, (double)m[8 + c], P[17 + c]);
+            for (int i = 0; i < 5; ++i) {
+                const float v = part == 0 ? m[RS_SRC[0][i]]
+                                          : (part == 1 ? m[RS_SRC[1][i]] : (part == 2 ? m[RS_SRC[2][i]] : m[RS_SRC[3][i]]));
+                P[i] = fma((i == 1 || i == 3) && ywt ? Y : 1.0, (double)v, P[i]);
             }
-            while (ei < i1 && r.cys[ei] + s.hy == y) {  // a window ends: prefix now minus at its start
-                double D[RS_Q];
+            if (ei < i1 && r.cys[ei] + s.hy == y) {  // a window ends: prefix now minus at its start
+                if (act) {
+                    double* Re = Rp + (int64_t)ei * ncx;
+                    double D[5];
 #pragma unroll
-                for (int q = 0; q < RS_Q; ++q) {
-                    const double v = quad_sum(P[q]);
-                    D[q] = act && part == 0 ? Rp[(int64_t)q * cstride + (int64_t)ei * ncx] + v : 0.0;
+                    for (int i = 0; i < 5; ++i) D[i] = Re[qoff[i]] + P[i];
+                    if (ywt) {  // absolute y weights -> offsets from the centre
+                        const double cy = (double)r.cys[ei];
+                        D[1] -= cy * D[0];
+                        D[3] -= cy * D[2];
+                    }
+#pragma unroll
+                    for (int i = 0; i < 5; ++i) Re[qoff[i]] = D[i];
                 }
-                const double cy = (double)r.cys[ei];
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {  // absolute y weights -> offsets from the centre
-                    D[14 + c] -= cy * D[2 + c];
-                    D[17 + c] -= cy * D[8 + c];
-                }
-                if (act && part == 0)
-#pragma unroll
-                    for (int q = 0; q < RS_Q; ++q) Rp[(int64_t)q * cstride + (int64_t)ei * ncx] = D[q];
                 ++ei;
             }
         }
