@@ -294,6 +294,7 @@ def build_split_rows(coords, cand, hx, hy, hz, stride, mask, margin=0):
 # --------------------------------------------------------------------------- 12-DOF IC-GN
 
 TUNE_ROW_MOMENTS = 32  # aldvc_icgn12_tuned / aldvc_icgn3_tuned: separable reference moments
+ROW_MOMENTS_MIN_READS = 500_000_000  # subset-voxel reads of the per-node pass (~0.8 ms of it) below which it stays
 
 
 def row_moments_bytes(c, vd, hx, hy, hz, stride, dims, stored, masked, split) -> int:
@@ -301,10 +302,16 @@ def row_moments_bytes(c, vd, hx, hy, hz, stride, dims, stored, masked, split) ->
     nodes ``c`` (device ``(N, 3)`` int64, x y z), 0 when they take the per-node pass: sized from the
     distinct x and y and the z range of the valid nodes whose subsets lie inside the volume (one
     device-to-host read of four counts). The pass needs stored gradients, every subset voxel kept
-    (no mask, no keep rows) and stride 1."""
+    (no mask, no keep rows) and stride 1. It is taken only where the per-node pass would read the
+    volume many times over (``N * S >= ROW_MOMENTS_MIN_READS`` and at least four times the voxel
+    count): its cost follows the volume, the per-node pass's the node count, and below that the
+    separable pass's fixed cost (six launches and the count read) is not repaid."""
     if not stored or masked or split or int(stride) != 1 or c.shape[0] == 0:
         return 0
     nz, ny, nx = dims
+    reads = int(c.shape[0]) * (2 * int(hx) + 1) * (2 * int(hy) + 1) * (2 * int(hz) + 1)
+    if reads < ROW_MOMENTS_MIN_READS or reads < 4 * nz * ny * nx:
+        return 0
     x, y, z = c[:, 0], c[:, 1], c[:, 2]
     ok = (vd != 0) & (x >= hx) & (x < nx - hx) & (y >= hy) & (y < ny - hy) & (z >= hz) & (z < nz - hz)
     mx = torch.zeros(nx + 1, dtype=torch.int64, device=c.device)

@@ -114,10 +114,11 @@ def _check(got, exp, mag, inside, rel):
         assert np.all(err <= rel * mag[n] + 1e-9), (n, np.max(err / (mag[n] + 1e-30)))
 
 
-def test_row_moments_match_definition(case):
+def test_row_moments_match_definition(case, monkeypatch):
     from pyaldvc_amd import kernels as K
     from pyaldvc_amd import _lib
 
+    monkeypatch.setattr(K, "ROW_MOMENTS_MIN_READS", 0)  # this node set is below the size that takes it
     exp, mag, inside = _expected(case)
     assert len(inside) > 100
     import torch
@@ -140,3 +141,47 @@ def test_row_moments_match_definition(case):
     # the flag with too little room for the rectangle sums: the per-node pass, bit for bit
     small = _moments(case, ROW_FLAG, 4096 + 64)
     assert np.array_equal(small[inside], node[inside])
+
+
+def test_row_moments_solve_matches_per_node(monkeypatch):
+    """The whole 12-DOF solve with the separable moments against the per-node ones, on a bead
+    pair under strain and rotation: same statuses and iterations, parameters within the update
+    tolerance."""
+    import torch
+
+    from pyaldvc_amd import kernels as K
+    from pyaldvc_amd import synthetic as syn
+
+    shape = (72, 80, 88)
+    F = np.array([[0.015, 0.01, 0.0], [0.0, -0.01, 0.005], [0.004, 0.0, 0.012]])
+    disp = syn.affine(F, tuple((s - 1) / 2 for s in shape[::-1]), (1.4, -0.6, 0.8))
+    f_raw, g_raw = syn.bead_pair_np(shape, int(0.008 * np.prod(shape)), 1.5, disp, seed=5)
+    f = torch.from_numpy(syn.normalize_np(f_raw)).cuda()
+    g = torch.from_numpy(syn.normalize_np(g_raw)).cuda()
+    gx, gy, gz = K.compute_gradients(f)
+    coords = syn.node_lattice(shape, (16, 16, 16), (4, 4, 4))
+    cd = torch.from_numpy(coords).cuda()
+    nomask = np.ones((1, 1, 1), np.uint8)
+    H, L, mf, bf, nv, valid = K.precompute_nodes(cd, 8, 8, 8, f, gx, gy, gz, nomask, 0.5, 1e12)
+    u, v, w = disp(*(coords[:, i].astype(float) for i in range(3)))
+    P0 = np.zeros((coords.shape[0], 12))
+    P0[:, 9:] = np.round(np.column_stack([u, v, w]))
+    P0d = torch.from_numpy(P0).cuda()
+
+    def solve():
+        out = K.icgn_12dof_parallel(cd, P0d, 8, 8, 8, f, gx, gy, gz, nomask, g, 1, L, mf, bf, valid, 1e-4, 1e-5,
+                                    50, 5, variant="fused")  # fmt: skip
+        return [t.cpu().numpy() for t in out]
+
+    P1, it1, st1, z1 = solve()
+    monkeypatch.setattr(K, "ROW_MOMENTS_MIN_READS", 0)
+    assert K.row_moments_bytes(cd, valid, 8, 8, 8, 1, shape, True, False, False) > 0
+    P2, it2, st2, z2 = solve()
+    assert np.array_equal(st1, st2)
+    assert np.array_equal(it1, it2)
+    ok = st1 == 0
+    assert ok.sum() > 0.5 * len(st1)
+    # the two moment sums round differently (double rectangle sums against float per-thread sums):
+    # the solves agree to well within the stopping tolerance of the update (1e-4)
+    assert np.max(np.abs(P1[ok] - P2[ok])) < 1e-4
+    assert np.median(np.abs(P1[ok, 9:] - P2[ok, 9:])) < 1e-7
