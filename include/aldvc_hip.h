@@ -107,9 +107,10 @@ int aldvc_icgn12(const int64_t *coords, int64_t N, double *P, int hx, int hy, in
  * plane-pair lane mapping, 2: node list in index order instead of spatial tiles, 4: workgroup b
  * takes list slot b instead of the XCD-aware mapping, 8: no 8 x 8 lane patches (the default
  * mapping of the brick path), 32: reference moments by the separable pass over the distinct node
- * coordinates when the gradients are stored, no mask or keep rows select voxels and stride is 1
- * (double sums; the workspace then also holds aldvc_icgn12_row_moments_bytes past
- * aldvc_icgn12_workspace_bytes, else, or when the node set needs more, the per-node pass runs).
+ * coordinates when the gradients are stored and stride is 1, for the nodes without keep rows
+ * (double sums, weighted by the mask; the workspace then also holds
+ * aldvc_icgn12_row_moments_bytes past aldvc_icgn12_workspace_bytes, else, or when the node set
+ * needs more, the per-node pass runs).
  * Same reference function as aldvc_icgn12. */
 int aldvc_icgn12_tuned(const int64_t *coords, int64_t N, double *P, int hx, int hy, int hz, int stride,
                        const float *f, const float *gx, const float *gy, const float *gz, const uint8_t *mask,
@@ -122,8 +123,9 @@ int aldvc_icgn12_tuned(const int64_t *coords, int64_t N, double *P, int hx, int 
 
 /* Extra workspace of flag 32 of aldvc_icgn12_tuned / aldvc_icgn3_tuned for nodes with ncx distinct x,
  * ncy distinct y and z within a range of nplanes planes including the subset half-widths
- * (max z - min z + 2 hz + 1), over those whose subsets lie inside the volume. 0 when a count is
- * not positive. No counterpart in the reference (a workspace size). */
+ * (max z - min z + 2 hz + 1), over those without keep rows whose subsets lie inside the volume (an
+ * upper bound is fine). 0 when a count is not positive. No counterpart in the reference (a
+ * workspace size). */
 size_t aldvc_icgn12_row_moments_bytes(int64_t ncx, int64_t ncy, int64_t nplanes, int64_t ny, int64_t nx);
 
 /* aldvc_icgn12 that allocates nothing. No counterpart in the reference: for callers that cannot

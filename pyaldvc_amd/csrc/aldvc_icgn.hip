@@ -896,18 +896,18 @@ __device__ inline void ref_moments_columns(const IcgnArgs& a, const uint8_t* kee
     block_sum<NR, NWAVES>(m, out, red);
 }
 
-// skip: non-null when the separable pass below may have taken every node (its flag, set on the
-// device, says whether it did)
+// skip: non-null when the separable pass below may have taken the nodes without keep rows (its
+// flag, set on the device, says whether it did)
 template <bool STORED, bool SEL>
 __global__ __launch_bounds__(BLOCK) void k_ref_moments(IcgnArgs a, const int64_t* __restrict__ list,
                                                        const int64_t* __restrict__ count, double* __restrict__ mom,
                                                        const int* __restrict__ skip) {
-    if (skip && *skip) return;
     const int64_t slot = xcd_slot(blockIdx.x, *count, a.xcd);
     if (slot < 0) return;
     const int64_t n = list[slot];
     const int64_t x0 = a.coords[3 * n], y0 = a.coords[3 * n + 1], z0 = a.coords[3 * n + 2];
     const uint8_t* keep = keep_row_of(a, n);
+    if (skip && *skip && !keep) return;  // taken by the separable pass
     __shared__ double red[NWAVES * NR];
     __shared__ double res[NR];
     if (!keep && !subset_in_volume(a.V, a.sub, x0, y0, z0)) return;  // refused by the iteration kernel
@@ -918,10 +918,10 @@ __global__ __launch_bounds__(BLOCK) void k_ref_moments(IcgnArgs a, const int64_t
 }
 
 // ---------------------------------------------------------------------------- separable reference moments
-// With every subset voxel kept (no mask, no keep rows, stride 1), a node's reference moments are
-// box sums over its subset cube of per-voxel terms: f, f^2, grad_c, grad_c f, and the grad terms
-// times the voxel's offset X, Y or Z from the node, with the pilot pf folded in afterwards
-// (sum f' = sum f - n pf, ...). Nodes on a lattice share most of these sums: the x-sums of one
+// For a node without keep rows (stride 1), the reference moments are box sums over its subset cube
+// of per-voxel terms weighted by the reference mask w (1 everywhere without one): w, w f, w f^2,
+// w grad_c, w grad_c f, and the grad terms times the voxel's offset X, Y or Z from the node, with
+// the pilot pf folded in afterwards (sum w f' = sum w f - n pf, ...). Nodes on a lattice share most of these sums: the x-sums of one
 // voxel row serve every node whose centre has that x and whose subset holds the row, and so on
 // along y and z. So the sums are taken axis by axis at the distinct node coordinates only:
 //   k_rs_xy: per plane z, distinct centre x (cx) and distinct centre y (cy), the 20 sums over the
@@ -934,8 +934,9 @@ __global__ __launch_bounds__(BLOCK) void k_ref_moments(IcgnArgs a, const int64_t
 // it (27 times on a lattice of step hx / 2). The distinct coordinates come from marks over the
 // node list (k_rs_mark, k_rs_index). Rectangle sums: R[((p * RS_Q + q) * ncy + cyi) * ncx + cxi]
 // for plane p = z - zbase, with q = 0 f, 1 f^2, 2 + c grad_c, 5 + c grad_c X, 8 + c grad_c f,
-// 11 + c grad_c f X, 14 + c grad_c Y, 17 + c grad_c f Y.
-constexpr int RS_Q = 20;
+// 11 + c grad_c f X, 14 + c grad_c Y, 17 + c grad_c f Y, 20 the kept count (with a mask only), each
+// weighted by w. Nodes with keep rows take the per-node pass.
+constexpr int RS_Q = 21;
 constexpr int RS_SEG = 16;   // centres y per work item of k_rs_xy
 constexpr int RS_TILE = 16;  // centres x per work item (four lanes each)
 constexpr int RS_KC = 9;     // voxels of a quarter row loaded at once
@@ -972,7 +973,7 @@ __global__ void k_rs_mark(IcgnArgs a, const int64_t* __restrict__ list, const in
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < M; i += (int64_t)gridDim.x * blockDim.x) {
         const int64_t n = list[i];
         const int64_t x0 = a.coords[3 * n], y0 = a.coords[3 * n + 1], z0 = a.coords[3 * n + 2];
-        if (!subset_in_volume(a.V, a.sub, x0, y0, z0)) continue;
+        if (keep_row_of(a, n) || !subset_in_volume(a.V, a.sub, x0, y0, z0)) continue;
         r.xidx[x0] = 1;
         r.yidx[y0] = 1;
         atomicMin(&r.hdr[RS_ZLO], (int)z0);
@@ -1029,17 +1030,24 @@ __device__ inline float quad_sum(float v) {
     return v + __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, true));
 }
 
-// Rectangle sums a quad lane keeps, five per lane: RS_DST[part][i] is the sum q of slot i, taken
-// from the row sum RS_SRC[part][i] (times y for slots 1 and 3 of parts 0-2: the y-weighted grad
-// sums sit beside their unweighted ones, 0 and 2, which the window's y offsets need).
-__constant__ const int8_t RS_DST[4][5] = {{2, 14, 8, 17, 0}, {3, 15, 9, 18, 1}, {4, 16, 10, 19, 5}, {6, 7, 11, 12, 13}};
-constexpr int8_t RS_SRC[4][5] = {{2, 2, 8, 8, 0}, {3, 3, 9, 9, 1}, {4, 4, 10, 10, 5}, {6, 7, 11, 12, 13}};
+// Rectangle sums a quad lane keeps, five per lane and the kept count on part 3 with a mask:
+// RS_DST[part][i] is the sum q of slot i, taken from the row sum RS_SRC[part][i] (times y for slots
+// 1 and 3 of parts 0-2: the y-weighted grad sums sit beside their unweighted ones, 0 and 2, which
+// the window's y offsets need); -1: no slot.
+constexpr int RS_SLOTS = 6;
+__constant__ const int8_t RS_DST[4][RS_SLOTS] = {
+    {2, 14, 8, 17, 0, -1}, {3, 15, 9, 18, 1, -1}, {4, 16, 10, 19, 5, -1}, {6, 7, 11, 12, 13, 20}};
+constexpr int8_t RS_SRC[4][RS_SLOTS] = {
+    {2, 2, 8, 8, 0, 14}, {3, 3, 9, 9, 1, 14}, {4, 4, 10, 10, 5, 14}, {6, 7, 11, 12, 13, 14}};
 
 // One wave per work item (plane, RS_SEG centres y, RS_TILE centres x); lane 4 j + k takes centre x
 // j of the tile and the k-th quarter of each row. The quad's row sums are combined every row and
-// each lane keeps the prefix sums of five of the 20 rectangle sums. Stored gradients only.
+// each lane keeps the prefix sums of its slots of the rectangle sums. MASKED: every term weighted by
+// the reference mask, and the kept count summed. Stored gradients only.
+template <bool MASKED>
 __global__ __launch_bounds__(BLOCK, 4) void k_rs_xy(Volumes V, Subset s, RowMoments r) {
     if (!r.hdr[RS_FITS]) return;
+    constexpr int NM = MASKED ? 15 : 14;  // row sums: the 14 terms, and the kept count
     const int ncx = r.hdr[RS_NCX], ncy = r.hdr[RS_NCY], zbase = r.hdr[RS_ZBASE];
     const int nseg = (ncy + RS_SEG - 1) / RS_SEG, ntile = (ncx + RS_TILE - 1) / RS_TILE;
     const int64_t items = (int64_t)r.hdr[RS_NPLANES] * nseg * ntile;
@@ -1047,9 +1055,14 @@ __global__ __launch_bounds__(BLOCK, 4) void k_rs_xy(Volumes V, Subset s, RowMome
     const int kper = (s.sx + 3) >> 2, k0 = part * kper, k1 = min(s.sx, k0 + kper);
     const int64_t plane = (int64_t)V.nx * V.ny, cstride = (int64_t)ncy * ncx;
     const bool ywt = part < 3;  // slots 1 and 3 are y-weighted
-    int64_t qoff[5];
+    constexpr int NSL = MASKED ? RS_SLOTS : RS_SLOTS - 1;
+    int64_t qoff[NSL];
+    bool slot[NSL];
 #pragma unroll
-    for (int i = 0; i < 5; ++i) qoff[i] = (int64_t)RS_DST[part][i] * cstride;
+    for (int i = 0; i < NSL; ++i) {
+        slot[i] = RS_DST[part][i] >= 0;
+        qoff[i] = (int64_t)max((int)RS_DST[part][i], 0) * cstride;
+    }
     // the work item is the same on every lane of a wave: in scalar registers
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     for (int64_t it = blockIdx.x * (int64_t)NWAVES + wave; it < items; it += (int64_t)gridDim.x * NWAVES) {
@@ -1063,37 +1076,44 @@ __global__ __launch_bounds__(BLOCK, 4) void k_rs_xy(Volumes V, Subset s, RowMome
         const int64_t pbase = (int64_t)(zbase + p) * plane;
         const unsigned xoff = (unsigned)(cx - s.hx);  // lane's first voxel in the row
         double* Rp = r.R + (int64_t)p * RS_Q * cstride + cxi;
-        double P[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        double P[NSL];
+#pragma unroll
+        for (int i = 0; i < NSL; ++i) P[i] = 0.0;
         int si = i0, ei = i0;
         for (int y = ylo; y <= yhi; ++y) {
             if (si < i1 && r.cys[si] - s.hy == y) {  // a window starts: keep -prefix (centres are distinct)
                 if (act)
 #pragma unroll
-                    for (int i = 0; i < 5; ++i) Rp[qoff[i] + (int64_t)si * ncx] = -P[i];
+                    for (int i = 0; i < NSL; ++i)
+                        if (slot[i]) Rp[qoff[i] + (int64_t)si * ncx] = -P[i];
                 ++si;
             }
             // the row's base is the same on every lane; lanes differ by 32-bit byte offsets
             const int64_t orow = pbase + (int64_t)y * V.nx;
             const char *fr = (const char*)(V.f + orow), *g0r = (const char*)(V.gx + orow),
                        *g1r = (const char*)(V.gy + orow), *g2r = (const char*)(V.gz + orow);
-            float m[14];
+            const uint8_t* mr = MASKED ? V.mask + orow : nullptr;
+            float m[NM];
 #pragma unroll
-            for (int q = 0; q < 14; ++q) m[q] = 0.0f;
+            for (int q = 0; q < NM; ++q) m[q] = 0.0f;
             // RS_KC voxels at a time, every load issued before the first is used (the lane's last
             // voxel stands in for those past its quarter and counts zero)
             for (int kc = k0; kc < k1; kc += RS_KC) {
                 float fv[RS_KC], g[3][RS_KC];
+                uint8_t mk[RS_KC];
 #pragma unroll
                 for (int t = 0; t < RS_KC; ++t) {
-                    const unsigned b = 4u * (xoff + (unsigned)min(kc + t, k1 - 1));
+                    const unsigned e = xoff + (unsigned)min(kc + t, k1 - 1), b = 4u * e;
                     fv[t] = *(const float*)(fr + b);
                     g[0][t] = *(const float*)(g0r + b);
                     g[1][t] = *(const float*)(g1r + b);
                     g[2][t] = *(const float*)(g2r + b);
+                    if constexpr (MASKED) mk[t] = mr[e];
                 }
 #pragma unroll
                 for (int t = 0; t < RS_KC; ++t) {
-                    const bool in = kc + t < k1;
+                    bool in = kc + t < k1;
+                    if constexpr (MASKED) in = in && mk[t] != 0;
                     const float f1 = in ? fv[t] : 0.0f, X = (float)(kc + t - s.hx);
                     m[0] += f1;
                     m[1] = __fmaf_rn(f1, f1, m[1]);
@@ -1105,13 +1125,14 @@ __global__ __launch_bounds__(BLOCK, 4) void k_rs_xy(Volumes V, Subset s, RowMome
                         m[8 + c] += tf;
                         m[11 + c] = __fmaf_rn(tf, X, m[11 + c]);
                     }
+                    if constexpr (MASKED) m[NM - 1] += in ? 1.0f : 0.0f;
                 }
             }
 #pragma unroll
-            for (int q = 0; q < 14; ++q) m[q] = quad_sum(m[q]);
+            for (int q = 0; q < NM; ++q) m[q] = quad_sum(m[q]);
             const double Y = (double)y;
 #pragma unroll
-            for (int i = 0; i < 5; ++i) {
+            for (int i = 0; i < NSL; ++i) {
                 const float v = part == 0 ? m[RS_SRC[0][i]]
                                           : (part == 1 ? m[RS_SRC[1][i]] : (part == 2 ? m[RS_SRC[2][i]] : m[RS_SRC[3][i]]));
                 P[i] = fma((i == 1 || i == 3) && ywt ? Y : 1.0, (double)v, P[i]);
@@ -1119,16 +1140,17 @@ __global__ __launch_bounds__(BLOCK, 4) void k_rs_xy(Volumes V, Subset s, RowMome
             if (ei < i1 && r.cys[ei] + s.hy == y) {  // a window ends: prefix now minus at its start
                 if (act) {
                     double* Re = Rp + (int64_t)ei * ncx;
-                    double D[5];
+                    double D[NSL];
 #pragma unroll
-                    for (int i = 0; i < 5; ++i) D[i] = Re[qoff[i]] + P[i];
+                    for (int i = 0; i < NSL; ++i) D[i] = slot[i] ? Re[qoff[i]] + P[i] : 0.0;
                     if (ywt) {  // absolute y weights -> offsets from the centre
                         const double cy = (double)r.cys[ei];
                         D[1] -= cy * D[0];
                         D[3] -= cy * D[2];
                     }
 #pragma unroll
-                    for (int i = 0; i < 5; ++i) Re[qoff[i]] = D[i];
+                    for (int i = 0; i < NSL; ++i)
+                        if (slot[i]) Re[qoff[i]] = D[i];
                 }
                 ++ei;
             }
@@ -1136,8 +1158,8 @@ __global__ __launch_bounds__(BLOCK, 4) void k_rs_xy(Volumes V, Subset s, RowMome
     }
 }
 
-// Per node: the sz planes' rectangle sums with the Z weights, the pilot folded in, written where
-// k_ref_moments writes (same layout, double sums throughout).
+// Per node without keep rows: the sz planes' rectangle sums with the Z weights, the pilot folded
+// in, written where k_ref_moments writes (same layout, double sums throughout).
 __global__ __launch_bounds__(BLOCK) void k_rs_z(IcgnArgs a, const int64_t* __restrict__ list,
                                                 const int64_t* __restrict__ count, RowMoments r,
                                                 double* __restrict__ mom) {
@@ -1146,10 +1168,11 @@ __global__ __launch_bounds__(BLOCK) void k_rs_z(IcgnArgs a, const int64_t* __res
     const int ncx = r.hdr[RS_NCX], ncy = r.hdr[RS_NCY], zbase = r.hdr[RS_ZBASE];
     const int64_t cstride = (int64_t)ncy * ncx;
     const Subset& s = a.sub;
+    const bool masked = a.V.mask != nullptr;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < M; i += (int64_t)gridDim.x * blockDim.x) {
         const int64_t n = list[i];
         const int64_t x0 = a.coords[3 * n], y0 = a.coords[3 * n + 1], z0 = a.coords[3 * n + 2];
-        if (!subset_in_volume(a.V, s, x0, y0, z0)) continue;
+        if (keep_row_of(a, n) || !subset_in_volume(a.V, s, x0, y0, z0)) continue;
         const double* Rn = r.R + (int64_t)r.yidx[y0] * ncx + r.xidx[x0];
         double S[RS_Q], SZ[6];
 #pragma unroll
@@ -1160,16 +1183,17 @@ __global__ __launch_bounds__(BLOCK) void k_rs_z(IcgnArgs a, const int64_t* __res
             const double* Rz = Rn + (int64_t)(z0 + dz - zbase) * RS_Q * cstride;
             const double Z = (double)dz;
 #pragma unroll
-            for (int q = 0; q < RS_Q; ++q) {
+            for (int q = 0; q < RS_Q - 1; ++q) {
                 const double v = Rz[q * cstride];
                 S[q] += v;
                 if (q >= 2 && q < 5) SZ[q - 2] = fma(Z, v, SZ[q - 2]);
                 if (q >= 8 && q < 11) SZ[q - 5] = fma(Z, v, SZ[q - 5]);
             }
+            if (masked) S[RS_Q - 1] += Rz[(RS_Q - 1) * cstride];
         }
         const double pfd = a.meanf ? a.meanf[n] : 0.0;
         const double pf = isfinite(pfd) ? (double)(float)pfd : 0.0;  // the per-node pass's float pilot
-        const double nv = (double)s.S;
+        const double nv = masked ? S[RS_Q - 1] : (double)s.S;
         double* out = mom + n * MOM_STRIDE;
         out[0] = nv;
         out[1] = S[0] - nv * pf;
@@ -2248,20 +2272,22 @@ void launch_fused(int variant, const IcgnArgs& a, const int64_t* list, const int
 }
 
 // Reference moments of every listed node; `sel`: a mask or keep rows select the subset voxels.
-// `rm`: buffers of the separable pass (null: per node only); it runs for stored gradients with every
-// subset voxel kept, and the per-node pass then exits unless the rectangle sums did not fit.
+// `rm`: buffers of the separable pass (null: per node only); it runs for stored gradients at stride
+// 1 and takes the nodes without keep rows, and the per-node pass then takes only those with keep
+// rows, or every node when the rectangle sums did not fit.
 template <bool STORED>
 void launch_ref_moments(bool sel, const IcgnArgs& a, const int64_t* list, const int64_t* count, int64_t N, int64_t nb,
                         double* mom, const RowMoments* rm, hipStream_t st) {
     const int* skip = nullptr;
-    if (STORED && !sel && rm && a.sub.stride == 1) {
+    if (STORED && rm && a.sub.stride == 1) {
         const int nx = a.V.nx, ny = a.V.ny;
         const unsigned nodes = (unsigned)std::min<int64_t>((N + 255) / 256, 4096);
         hipLaunchKernelGGL(k_rs_clear, dim3((unsigned)((std::max(std::max(nx, ny), (int)RS_HDR) + 255) / 256)),
                            dim3(256), 0, st, *rm, nx, ny);
         hipLaunchKernelGGL(k_rs_mark, dim3(nodes), dim3(256), 0, st, a, list, count, *rm);
         hipLaunchKernelGGL(k_rs_index, dim3(1), dim3(1024), 0, st, *rm, nx, ny, a.sub.hz);
-        hipLaunchKernelGGL(k_rs_xy, dim3(2048), dim3(BLOCK), 0, st, a.V, a.sub, *rm);
+        if (a.V.mask) hipLaunchKernelGGL(k_rs_xy<true>, dim3(2048), dim3(BLOCK), 0, st, a.V, a.sub, *rm);
+        else hipLaunchKernelGGL(k_rs_xy<false>, dim3(2048), dim3(BLOCK), 0, st, a.V, a.sub, *rm);
         hipLaunchKernelGGL(k_rs_z, dim3(nodes), dim3(BLOCK), 0, st, a, list, count, *rm, mom);
         skip = rm->hdr + RS_FITS;
     }
@@ -2471,11 +2497,11 @@ int aldvc_precompute(const int64_t* coords, int64_t N, int hx, int hy, int hz, i
 
 namespace {
 // Buffers of the separable reference-moment pass in the workspace past the `base` bytes every call
-// needs, when the call asks for it (TUNE_ROW_MOMENTS), the fused path runs on stored gradients with
-// every subset voxel kept, and the workspace holds more than the fixed part.
+// needs, when the call asks for it (TUNE_ROW_MOMENTS), the fused path runs on stored gradients at
+// stride 1, and the workspace holds more than the fixed part.
 const RowMoments* row_moments_at(RowMoments& rm, int flags, int variant, const Volumes& V, const Subset& s,
-                                 bool keep_rows, void* ws, size_t ws_bytes, size_t base) {
-    if (!(flags & TUNE_ROW_MOMENTS) || variant == 0 || !V.gx || V.mask || keep_rows || s.stride != 1) return nullptr;
+                                 void* ws, size_t ws_bytes, size_t base) {
+    if (!(flags & TUNE_ROW_MOMENTS) || variant == 0 || !V.gx || s.stride != 1) return nullptr;
     const size_t fixed = row_moments_fixed_bytes(V.ny, V.nx);
     if (ws_bytes < base + fixed + RS_Q * sizeof(double)) return nullptr;
     char* p = (char*)ws + base;
@@ -2548,8 +2574,7 @@ int aldvc_icgn12_tuned(const int64_t* coords, int64_t N, double* P, int hx, int 
                noise_gain, tol, dp_tol, max_iter, patience, predictive, n_iter, status, zncc,
                (flags & TUNE_NO_XCD) ? 0 : 1};
     RowMoments rm;
-    const RowMoments* rmp = row_moments_at(rm, flags, variant, V, s, split_index && split_keep, workspace,
-                                           workspace_bytes, need);
+    const RowMoments* rmp = row_moments_at(rm, flags, variant, V, s, workspace, workspace_bytes, need);
     const bool stored = gx != nullptr;
 #define ALDVC_DISPATCH(MD)                                                                                \
     if (stored) launch_icgn12<MD, true>(variant, a, list, count, N, gbuf, lds_budget, flags, rmp, stream);    \
@@ -2671,8 +2696,7 @@ int aldvc_icgn3_tuned(const int64_t* coords, int64_t N, const double* U_old, dou
     a.U = U;
     a.mu = mu;
     RowMoments rm;
-    const RowMoments* rmp = row_moments_at(rm, flags, variant, V, s, split_index && split_keep, workspace,
-                                           workspace_bytes, need);
+    const RowMoments* rmp = row_moments_at(rm, flags, variant, V, s, workspace, workspace_bytes, need);
     const bool stored = gx != nullptr;
 #define ALDVC_DISPATCH3(MD)                                                                           \
     if (stored) launch_icgn3<MD, true>(variant, a, list, count, N, mom, lds_budget, flags, rmp, stream);    \

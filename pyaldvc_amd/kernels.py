@@ -297,16 +297,16 @@ TUNE_ROW_MOMENTS = 32  # aldvc_icgn12_tuned / aldvc_icgn3_tuned: separable refer
 ROW_MOMENTS_MIN_READS = 500_000_000  # subset-voxel reads of the per-node pass (~0.8 ms of it) below which it stays
 
 
-def row_moments_bytes(c, vd, hx, hy, hz, stride, dims, stored, masked, split) -> int:
+def row_moments_bytes(c, vd, hx, hy, hz, stride, dims, stored) -> int:
     """Extra workspace of the separable reference-moment pass (flag ``TUNE_ROW_MOMENTS``) for the
     nodes ``c`` (device ``(N, 3)`` int64, x y z), 0 when they take the per-node pass: sized from the
     distinct x and y and the z range of the valid nodes whose subsets lie inside the volume (one
-    device-to-host read of four counts). The pass needs stored gradients, every subset voxel kept
-    (no mask, no keep rows) and stride 1. It is taken only where the per-node pass would read the
+    device-to-host read of four counts). The pass needs stored gradients and stride 1; it weights
+    the sums by the reference mask, and nodes with keep rows take the per-node pass. It is taken only where the per-node pass would read the
     volume many times over (``N * S >= ROW_MOMENTS_MIN_READS`` and at least four times the voxel
     count): its cost follows the volume, the per-node pass's the node count, and below that the
     separable pass's fixed cost (six launches and the count read) is not repaid."""
-    if not stored or masked or split or int(stride) != 1 or c.shape[0] == 0:
+    if not stored or int(stride) != 1 or c.shape[0] == 0:
         return 0
     nz, ny, nx = dims
     reads = int(c.shape[0]) * (2 * int(hx) + 1) * (2 * int(hy) + 1) * (2 * int(hz) + 1)
@@ -380,8 +380,7 @@ def icgn_12dof_parallel(
     zncc = torch.zeros(N, dtype=torch.float64, device=dev)
     var = VARIANTS[variant]
     ws_bytes = int(lib().aldvc_icgn12_workspace_bytes(N, int(hx), int(hy), int(hz), int(stride), var))
-    extra = row_moments_bytes(c, vd, hx, hy, hz, stride, (nz, ny, nx), gxd is not None, md is not None,
-                              si is not None) if var != 0 else 0  # fmt: skip
+    extra = row_moments_bytes(c, vd, hx, hy, hz, stride, (nz, ny, nx), gxd is not None) if var != 0 else 0
     if extra:
         flags |= TUNE_ROW_MOMENTS
         ws_bytes += extra
@@ -453,7 +452,7 @@ def icgn_3dof_parallel(
     status = torch.zeros(N, dtype=torch.int8, device=dev)
     zncc = torch.zeros(N, dtype=torch.float64, device=dev)
     ws_bytes = int(lib().aldvc_icgn3_workspace_bytes(N))
-    extra = row_moments_bytes(c, vd, hx, hy, hz, stride, (nz, ny, nx), gxd is not None, md is not None, si is not None)
+    extra = row_moments_bytes(c, vd, hx, hy, hz, stride, (nz, ny, nx), gxd is not None)
     ws_bytes += extra
     ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
     check(

@@ -45,12 +45,23 @@ def case():
     valid = (rng.random(N) > 0.1).astype(np.uint8)
     meanf = f_np.mean() + rng.standard_normal(N) * 0.5
     meanf[:5] = np.nan  # no pilot
+    # a reference mask of 4-voxel blocks (about 70 % kept) and keep rows for a fifth of the nodes
+    blocks = rng.random((nz // 4 + 1, ny // 4 + 1, nx // 4 + 1)) < 0.7
+    mask_np = np.repeat(np.repeat(np.repeat(blocks, 4, 0), 4, 1), 4, 2)[:nz, :ny, :nx].astype(np.uint8)
+    S = (2 * hx + 1) * (2 * hy + 1) * (2 * hz + 1)
+    rows = rng.random((N // 5, S)) < 0.6
+    split_index = np.full(N, -1, np.int64)
+    split_index[rng.choice(N, N // 5, replace=False)] = np.arange(N // 5)
+    keep = np.packbits(rows, axis=1, bitorder="little")
     return dict(f=f, gx=gx, gy=gy, gz=gz, f_np=f_np, g_np=[t.cpu().numpy() for t in (gx, gy, gz)], coords=coords,
-                valid=valid, meanf=meanf)  # fmt: skip
+                valid=valid, meanf=meanf, mask_np=mask_np, mask=torch.from_numpy(mask_np).cuda(), rows=rows,
+                split_index_np=split_index, split_index=torch.from_numpy(split_index).cuda(),
+                split_keep=torch.from_numpy(keep).cuda())  # fmt: skip
 
 
-def _moments(c, flags, extra):
-    """Runs aldvc_icgn12_tuned (one iteration) and returns the moments of every node (N, 27)."""
+def _moments(c, flags, extra, masked=False, split=False):
+    """Runs aldvc_icgn12_tuned (one iteration) and returns the moments of every node (N, 27);
+    ``masked``: with the case's reference mask, ``split``: with its keep rows."""
     import torch
 
     from pyaldvc_amd import _lib
@@ -71,8 +82,10 @@ def _moments(c, flags, extra):
     ws = torch.full((base + extra,), 0xFF, dtype=torch.uint8, device="cuda")
     stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
     rc = lib.aldvc_icgn12_tuned(
-        _p(cd), N, _p(P), hx, hy, hz, 1, _p(c["f"]), _p(c["gx"]), _p(c["gy"]), _p(c["gz"]), None, _p(c["f"]),
-        nz, ny, nx, 1, _p(L), None, None, 0.0, _p(meanf), _p(valid), 1e-2, 1e-3, 1, 5, 1, None, None, 0,
+        _p(cd), N, _p(P), hx, hy, hz, 1, _p(c["f"]), _p(c["gx"]), _p(c["gy"]), _p(c["gz"]),
+        _p(c["mask"]) if masked else None, _p(c["f"]), nz, ny, nx, 1, _p(L), None, None, 0.0, _p(meanf), _p(valid),
+        1e-2, 1e-3, 1, 5, 1, _p(c["split_index"]) if split else None, _p(c["split_keep"]) if split else None,
+        int(c["split_keep"].shape[1]) if split else 0,
         _p(n_iter), _p(status), _p(zncc), 1, 0, flags, _p(ws), base + extra, stream,
     )  # fmt: skip
     _lib.check(rc, "aldvc_icgn12_tuned")
@@ -82,8 +95,9 @@ def _moments(c, flags, extra):
     return mom[:, :27].copy()
 
 
-def _expected(c):
-    """float64 moments and the sums of the terms' magnitudes, per node whose subset is inside."""
+def _expected(c, masked=False, split=False):
+    """float64 moments and the sums of the terms' magnitudes, per node whose subset is inside, over
+    its kept voxels (its keep row, else the mask, else every voxel)."""
     f, gs = c["f_np"].astype(np.float64), [g.astype(np.float64) for g in c["g_np"]]
     hx, hy, hz = HALF
     nz, ny, nx = f.shape
@@ -96,10 +110,15 @@ def _expected(c):
         inside.append(n)
         sl = (slice(z0 - hz, z0 + hz + 1), slice(y0 - hy, y0 + hy + 1), slice(x0 - hx, x0 + hx + 1))
         pf = float(np.float32(c["meanf"][n])) if np.isfinite(c["meanf"][n]) else 0.0
+        w = np.ones(X.shape)
+        if split and c["split_index_np"][n] >= 0:
+            w = c["rows"][c["split_index_np"][n]].reshape(X.shape).astype(float)
+        elif masked:
+            w = (c["mask_np"][sl] != 0).astype(float)
         fp = f[sl] - pf
-        terms = [np.ones(fp.shape), fp, fp * fp] + [None] * 24
+        terms = [w, w * fp, w * fp * fp] + [None] * 24
         for cc in range(3):
-            g = gs[cc][sl]
+            g = gs[cc][sl] * w
             for j in range(4):
                 terms[3 + 4 * cc + j] = g * q[j]
                 terms[15 + 4 * cc + j] = g * q[j] * fp
@@ -126,7 +145,7 @@ def test_row_moments_match_definition(case, monkeypatch):
     nz, ny, nx = case["f"].shape
     cd = torch.from_numpy(case["coords"]).cuda()
     vd = torch.from_numpy(case["valid"]).cuda()
-    extra = K.row_moments_bytes(cd, vd, *HALF, 1, (nz, ny, nx), True, False, False)
+    extra = K.row_moments_bytes(cd, vd, *HALF, 1, (nz, ny, nx), True)
     sel = case["coords"][case["valid"] != 0]
     hx, hy, hz = HALF
     ins = sel[(sel[:, 0] >= hx) & (sel[:, 0] < nx - hx) & (sel[:, 1] >= hy) & (sel[:, 1] < ny - hy)
@@ -175,7 +194,7 @@ def test_row_moments_solve_matches_per_node(monkeypatch):
 
     P1, it1, st1, z1 = solve()
     monkeypatch.setattr(K, "ROW_MOMENTS_MIN_READS", 0)
-    assert K.row_moments_bytes(cd, valid, 8, 8, 8, 1, shape, True, False, False) > 0
+    assert K.row_moments_bytes(cd, valid, 8, 8, 8, 1, shape, True) > 0
     P2, it2, st2, z2 = solve()
     assert np.array_equal(st1, st2)
     assert np.array_equal(it1, it2)
@@ -185,3 +204,29 @@ def test_row_moments_solve_matches_per_node(monkeypatch):
     # the solves agree to well within the stopping tolerance of the update (1e-4)
     assert np.max(np.abs(P1[ok] - P2[ok])) < 1e-4
     assert np.median(np.abs(P1[ok, 9:] - P2[ok, 9:])) < 1e-7
+
+
+@pytest.mark.parametrize("masked,split", [(True, False), (False, True), (True, True)])
+def test_row_moments_mask_and_keep_rows(case, masked, split):
+    """With a reference mask the separable sums are mask-weighted (and count the kept voxels);
+    nodes with keep rows take the per-node pass in the same call."""
+    import torch
+
+    from pyaldvc_amd import _lib
+
+    exp, mag, inside = _expected(case, masked, split)
+    assert len(inside) > 100
+    nz, ny, nx = case["f"].shape
+    hx, hy, hz = HALF
+    ins = case["coords"][inside]
+    planes = int(ins[:, 2].max() - ins[:, 2].min()) + 2 * hz + 1
+    extra = int(_lib.lib().aldvc_icgn12_row_moments_bytes(len(np.unique(ins[:, 0])), len(np.unique(ins[:, 1])),
+                                                          planes, ny, nx))  # fmt: skip
+    rows = _moments(case, ROW_FLAG, extra, masked, split)
+    _check(rows, exp, mag, inside, 1e-6)
+    node = _moments(case, 0, 0, masked, split)
+    _check(node, exp, mag, inside, 1e-5)
+    if split:  # the nodes with keep rows take the per-node pass either way: bit for bit
+        kr = [n for n in inside if case["split_index_np"][n] >= 0]
+        assert len(kr) > 10
+        assert np.array_equal(rows[kr], node[kr])
