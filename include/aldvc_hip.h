@@ -80,6 +80,25 @@ int aldvc_precompute(const int64_t *coords, int64_t N, int hx, int hy, int hz, i
                      double *L, double *meanf, double *bottomf, int64_t *n_valid, uint8_t *valid,
                      hipStream_t stream);
 
+/* aldvc_precompute with a workspace: flags = 32 takes the sums of the nodes without keep rows
+ * (stored gradients, stride 1) by a separable pass over the distinct node coordinates (double
+ * sums, weighted by the mask) when the workspace holds aldvc_precompute_row_bytes for them (else,
+ * or when the node set needs more, the per-node pass runs, as with flags 0). Same reference
+ * function as aldvc_precompute. */
+int aldvc_precompute_tuned(const int64_t *coords, int64_t N, int hx, int hy, int hz, int stride,
+                           const float *f, const float *gx, const float *gy, const float *gz,
+                           const uint8_t *mask, int64_t nz, int64_t ny, int64_t nx, double min_valid_ratio,
+                           double cond_max, const int64_t *split_index, const uint8_t *split_keep,
+                           int64_t keep_bytes, double *H, double *L, double *meanf, double *bottomf,
+                           int64_t *n_valid, uint8_t *valid, int flags, void *workspace,
+                           size_t workspace_bytes, hipStream_t stream);
+
+/* Workspace of flag 32 of aldvc_precompute_tuned for nodes with ncx distinct x, ncy distinct y and
+ * z within a range of nplanes planes including the subset half-widths, over those without keep rows
+ * whose subsets lie inside the volume (an upper bound is fine); 0 when a count is not positive. No
+ * counterpart in the reference (a workspace size). */
+size_t aldvc_precompute_row_bytes(int64_t ncx, int64_t ncy, int64_t nplanes, int64_t ny, int64_t nx);
+
 /* Workspace bytes aldvc_icgn12 needs for N nodes (variant 0 adds the 2048-node scratch rows). */
 size_t aldvc_icgn12_workspace_bytes(int64_t N, int hx, int hy, int hz, int stride, int variant);
 

@@ -70,6 +70,12 @@ SIGNATURES = {
         I32,
         [P, I64, I32, I32, I32, I32, P, P, P, P, P, I64, I64, I64, F64, F64, P, P, I64, P, P, P, P, P, P, P],
     ),
+    "aldvc_precompute_tuned": (
+        I32,
+        [P, I64, I32, I32, I32, I32, P, P, P, P, P, I64, I64, I64, F64, F64, P, P, I64, P, P, P, P, P, P,
+         I32, P, SZ, P],
+    ),  # fmt: skip
+    "aldvc_precompute_row_bytes": (SZ, [I64, I64, I64, I64, I64]),
     "aldvc_icgn12_workspace_bytes": (SZ, [I64, I32, I32, I32, I32, I32]),
     "aldvc_icgn12_row_moments_bytes": (SZ, [I64, I64, I64, I64, I64]),
     "aldvc_icgn12": (

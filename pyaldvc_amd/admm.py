@@ -212,7 +212,7 @@ class AdmmLoop:
         # the first 3-DOF pass's reference moments by the separable pass, where it applies
         d = ctx.device
         extra = K.row_moments_bytes(d["coords"], d["valid"], *ctx.half, ctx.stride, tuple(vols["f"].shape),
-                                    vols["stored"])  # fmt: skip
+                                    vols["stored"], getattr(ctx, "separable", None))  # fmt: skip
         self.row_flag = K.TUNE_ROW_MOMENTS if extra else 0
         if extra:
             self.ws_icgn3 = _ws(L.aldvc_icgn3_workspace_bytes(N) + extra)

@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <mutex>
 #include <string>
+#include <type_traits>
 
 #include "aldvc_abi_util.h"
 
@@ -926,20 +927,21 @@ __global__ __launch_bounds__(BLOCK) void k_ref_moments(IcgnArgs a, const int64_t
 // along y and z. So the sums are taken axis by axis at the distinct node coordinates only:
 //   k_rs_xy: per plane z, distinct centre x (cx) and distinct centre y (cy), the 20 sums over the
 //            sx x sy rectangle around (cx, cy): x-row sums in float (four lanes a row, a quarter
-//            each, combined), accumulated along y as running prefix sums in double, whose
-//            differences between a window's end and start are the rectangle sums;
+//            each, combined), added row by row in double with the row's offset Y from cy;
 //   k_rs_z:  per node, its sz planes' rectangle sums, with the Z weights, in the layout of
 //            k_ref_moments.
-// A voxel is then read by about sx / (x spacing) lanes instead of once per node whose subset holds
-// it (27 times on a lattice of step hx / 2). The distinct coordinates come from marks over the
+// A voxel is then read by about (sx / x spacing) (sy / y spacing) lanes instead of once per node
+// whose subset holds it. Every sum is formed from the node's own subset voxels in a fixed order
+// (quarter rows, rows, planes), so a node's moments do not depend on the other nodes of the call:
+// a volume cut into boxes gives the same bits as the whole. The distinct coordinates come from marks over the
 // node list (k_rs_mark, k_rs_index). Rectangle sums: R[((p * RS_Q + q) * ncy + cyi) * ncx + cxi]
 // for plane p = z - zbase, with q = 0 f, 1 f^2, 2 + c grad_c, 5 + c grad_c X, 8 + c grad_c f,
 // 11 + c grad_c f X, 14 + c grad_c Y, 17 + c grad_c f Y, 20 the kept count (with a mask only), each
 // weighted by w. Nodes with keep rows take the per-node pass.
 constexpr int RS_Q = 21;
-constexpr int RS_SEG = 16;   // centres y per work item of k_rs_xy
 constexpr int RS_TILE = 16;  // centres x per work item (four lanes each)
-constexpr int RS_KC = 9;     // voxels of a quarter row loaded at once
+constexpr int RS_KC = 8;     // voxels of a quarter row loaded at once
+constexpr int RS_G = 4;      // consecutive centres y per work item: their windows share rows
 enum : int { RS_NCX, RS_NCY, RS_ZLO, RS_ZHI, RS_FITS, RS_NPLANES, RS_ZBASE, RS_HDR = 64 };
 enum : int { TUNE_ROW_MOMENTS = 32 };
 
@@ -965,13 +967,14 @@ __global__ void k_rs_clear(RowMoments r, int nx, int ny) {
     if (i < RS_HDR) r.hdr[i] = i == RS_ZLO ? INT_MAX : (i == RS_ZHI ? -1 : 0);
 }
 
-// Marks the centres of the listed nodes whose subsets lie inside the volume (the others are
-// refused by the iteration kernel and need no moments).
+// Marks the centres of the listed nodes (every node 0..n_all-1 when there is no list) without keep
+// rows whose subsets lie inside the volume (the others take the per-node pass, or are refused by
+// the iteration kernel and need no moments).
 __global__ void k_rs_mark(IcgnArgs a, const int64_t* __restrict__ list, const int64_t* __restrict__ count,
-                          RowMoments r) {
-    const int64_t M = *count;
+                          int64_t n_all, RowMoments r) {
+    const int64_t M = list ? *count : n_all;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < M; i += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t n = list[i];
+        const int64_t n = list ? list[i] : i;
         const int64_t x0 = a.coords[3 * n], y0 = a.coords[3 * n + 1], z0 = a.coords[3 * n + 2];
         if (keep_row_of(a, n) || !subset_in_volume(a.V, a.sub, x0, y0, z0)) continue;
         r.xidx[x0] = 1;
@@ -1007,9 +1010,9 @@ __device__ inline int rs_index_axis(int* idx, int* out, int n, int* wsum) {
     return base;
 }
 
-// Distinct centres and whether the rectangle sums fit the room given (else every node takes the
-// per-node pass, k_ref_moments).
-__global__ __launch_bounds__(1024) void k_rs_index(RowMoments r, int nx, int ny, int hz) {
+// Distinct centres and whether the rectangle sums (nq per plane and centre pair) fit the room given
+// (else every node takes the per-node pass).
+__global__ __launch_bounds__(1024) void k_rs_index(RowMoments r, int nx, int ny, int hz, int nq) {
     __shared__ int wsum[16];
     const int ncx = rs_index_axis(r.xidx, r.cxs, nx, wsum);
     const int ncy = rs_index_axis(r.yidx, r.cys, ny, wsum);
@@ -1020,7 +1023,7 @@ __global__ __launch_bounds__(1024) void k_rs_index(RowMoments r, int nx, int ny,
         r.hdr[RS_NCY] = ncy;
         r.hdr[RS_NPLANES] = (int)planes;
         r.hdr[RS_ZBASE] = zlo - hz;
-        r.hdr[RS_FITS] = planes > 0 && ncx > 0 && ncy > 0 && planes * RS_Q * ncy * (int64_t)ncx <= r.r_doubles;
+        r.hdr[RS_FITS] = planes > 0 && ncx > 0 && ncy > 0 && planes * nq * ncy * (int64_t)ncx <= r.r_doubles;
     }
 }
 
@@ -1040,119 +1043,136 @@ __constant__ const int8_t RS_DST[4][RS_SLOTS] = {
 constexpr int8_t RS_SRC[4][RS_SLOTS] = {
     {2, 2, 8, 8, 0, 14}, {3, 3, 9, 9, 1, 14}, {4, 4, 10, 10, 5, 14}, {6, 7, 11, 12, 13, 14}};
 
-// One wave per work item (plane, RS_SEG centres y, RS_TILE centres x); lane 4 j + k takes centre x
-// j of the tile and the k-th quarter of each row. The quad's row sums are combined every row and
-// each lane keeps the prefix sums of its slots of the rectangle sums. MASKED: every term weighted by
-// the reference mask, and the kept count summed. Stored gradients only.
+// One wave per work item (plane, centre y, RS_TILE centres x); lane 4 j + k takes centre x j of the
+// tile and the k-th quarter of each of the window's sy rows. The quad's row sums are combined every
+// row and each lane adds its slots of the rectangle sums. MASKED: every term weighted by the
+// reference mask, and the kept count summed. Stored gradients only.
 template <bool MASKED>
-__global__ __launch_bounds__(BLOCK, 4) void k_rs_xy(Volumes V, Subset s, RowMoments r) {
+__global__ __launch_bounds__(BLOCK, 3) void k_rs_xy(Volumes V, Subset s, RowMoments r) {
     if (!r.hdr[RS_FITS]) return;
     constexpr int NM = MASKED ? 15 : 14;  // row sums: the 14 terms, and the kept count
     const int ncx = r.hdr[RS_NCX], ncy = r.hdr[RS_NCY], zbase = r.hdr[RS_ZBASE];
-    const int nseg = (ncy + RS_SEG - 1) / RS_SEG, ntile = (ncx + RS_TILE - 1) / RS_TILE;
-    const int64_t items = (int64_t)r.hdr[RS_NPLANES] * nseg * ntile;
+    const int ntile = (ncx + RS_TILE - 1) / RS_TILE;
     const int lane = threadIdx.x & 63, j = lane >> 2, part = lane & 3;
-    const int kper = (s.sx + 3) >> 2, k0 = part * kper, k1 = min(s.sx, k0 + kper);
+    // a quad's lanes split a row: lane part takes base = sx / 4 voxels from k0, and one more (at kx)
+    // when part < sx % 4; a lane without one reads a valid voxel there and counts it zero
+    const int base = s.sx >> 2, extra = s.sx & 3, k0 = part * base + min(part, extra);
+    const int kx = min(k0 + base, s.sx - 1);
+    const bool has_extra = part < extra;
     const int64_t plane = (int64_t)V.nx * V.ny, cstride = (int64_t)ncy * ncx;
     const bool ywt = part < 3;  // slots 1 and 3 are y-weighted
+    // slot sources picked by 0/1 weights (a product with 0 and three exact additions of 0: the
+    // lane's own row sum, bit for bit, without per-use compares)
+    const float w0 = part == 0 ? 1.0f : 0.0f, w1 = part == 1 ? 1.0f : 0.0f, w2 = part == 2 ? 1.0f : 0.0f,
+                w3 = part == 3 ? 1.0f : 0.0f;
     constexpr int NSL = MASKED ? RS_SLOTS : RS_SLOTS - 1;
-    int64_t qoff[NSL];
-    bool slot[NSL];
-#pragma unroll
-    for (int i = 0; i < NSL; ++i) {
-        slot[i] = RS_DST[part][i] >= 0;
-        qoff[i] = (int64_t)max((int)RS_DST[part][i], 0) * cstride;
-    }
     // the work item is the same on every lane of a wave: in scalar registers
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    for (int64_t it = blockIdx.x * (int64_t)NWAVES + wave; it < items; it += (int64_t)gridDim.x * NWAVES) {
-        const int p = (int)(it / ((int64_t)nseg * ntile));
-        const int rem = (int)(it - (int64_t)p * nseg * ntile), seg = rem / ntile, tile = rem - seg * ntile;
+    // XCD-aware items: workgroup b runs on XCD b % 8 (the grid is a multiple of 8) and XCD k takes
+    // the k-th eighth of the (plane, group of RS_G centres y) pairs in order, so the windows an XCD
+    // sums at once are neighbours whose rows its L2 holds
+    const int ngroup = (ncy + RS_G - 1) / RS_G;
+    const int64_t pairs = (int64_t)r.hdr[RS_NPLANES] * ngroup, xcd = blockIdx.x & 7;
+    const int64_t lo = pairs * xcd / 8, items = (pairs * (xcd + 1) / 8 - lo) * ntile;
+    for (int64_t it = (blockIdx.x >> 3) * (int64_t)NWAVES + wave; it < items;
+         it += (int64_t)(gridDim.x >> 3) * NWAVES) {
+        const int64_t pair = lo + it / ntile;
+        const int tile = (int)(it - (it / ntile) * ntile), p = (int)(pair / ngroup);
+        const int c0 = (int)(pair - (int64_t)p * ngroup) * RS_G, gn = min(RS_G, ncy - c0);
         const int cxi = tile * RS_TILE + j;
         const bool act = cxi < ncx;
         const int cx = act ? r.cxs[cxi] : r.cxs[0];
-        const int i0 = seg * RS_SEG, i1 = min(ncy, i0 + RS_SEG);
-        const int ylo = r.cys[i0] - s.hy, yhi = r.cys[i1 - 1] + s.hy;
+        int cyg[RS_G];
+#pragma unroll
+        for (int g = 0; g < RS_G; ++g) cyg[g] = r.cys[c0 + min(g, gn - 1)];
         const int64_t pbase = (int64_t)(zbase + p) * plane;
-        const unsigned xoff = (unsigned)(cx - s.hx);  // lane's first voxel in the row
-        double* Rp = r.R + (int64_t)p * RS_Q * cstride + cxi;
-        double P[NSL];
+        const unsigned xoff = (unsigned)(cx - s.hx);  // the subset row's first voxel
+        double P[RS_G][NSL];
 #pragma unroll
-        for (int i = 0; i < NSL; ++i) P[i] = 0.0;
-        int si = i0, ei = i0;
-        for (int y = ylo; y <= yhi; ++y) {
-            if (si < i1 && r.cys[si] - s.hy == y) {  // a window starts: keep -prefix (centres are distinct)
-                if (act)
+        for (int g = 0; g < RS_G; ++g)
 #pragma unroll
-                    for (int i = 0; i < NSL; ++i)
-                        if (slot[i]) Rp[qoff[i] + (int64_t)si * ncx] = -P[i];
-                ++si;
-            }
+            for (int i = 0; i < NSL; ++i) P[g][i] = 0.0;
+        for (int y = cyg[0] - s.hy; y <= cyg[gn - 1] + s.hy; ++y) {
             // the row's base is the same on every lane; lanes differ by 32-bit byte offsets
             const int64_t orow = pbase + (int64_t)y * V.nx;
             const char *fr = (const char*)(V.f + orow), *g0r = (const char*)(V.gx + orow),
                        *g1r = (const char*)(V.gy + orow), *g2r = (const char*)(V.gz + orow);
-            const uint8_t* mr = MASKED ? V.mask + orow : nullptr;
             float m[NM];
 #pragma unroll
             for (int q = 0; q < NM; ++q) m[q] = 0.0f;
-            // RS_KC voxels at a time, every load issued before the first is used (the lane's last
-            // voxel stands in for those past its quarter and counts zero)
-            for (int kc = k0; kc < k1; kc += RS_KC) {
-                float fv[RS_KC], g[3][RS_KC];
-                uint8_t mk[RS_KC];
+            auto add = [&](bool in, float fv, float g0, float g1, float g2, int k) {
+                const float f1 = in ? fv : 0.0f, X = (float)(k - s.hx);
+                const float g[3] = {in ? g0 : 0.0f, in ? g1 : 0.0f, in ? g2 : 0.0f};
+                m[0] += f1;
+                m[1] = __fmaf_rn(f1, f1, m[1]);
 #pragma unroll
-                for (int t = 0; t < RS_KC; ++t) {
-                    const unsigned e = xoff + (unsigned)min(kc + t, k1 - 1), b = 4u * e;
+                for (int c = 0; c < 3; ++c) {
+                    const float tf = g[c] * f1;
+                    m[2 + c] += g[c];
+                    m[5 + c] = __fmaf_rn(g[c], X, m[5 + c]);
+                    m[8 + c] += tf;
+                    m[11 + c] = __fmaf_rn(tf, X, m[11 + c]);
+                }
+                if constexpr (MASKED) m[NM - 1] += in ? 1.0f : 0.0f;
+            };
+            // RS_KC voxels at a time, every load issued before the first is used; the loads of the
+            // last, partial chunk stay on the lane's last voxel and count zero
+            const uint8_t* mr = MASKED ? V.mask + orow : nullptr;
+            for (int kc = 0; kc < max(base, 1); kc += RS_KC) {
+                float fv[RS_KC + 1], gv[3][RS_KC + 1];
+                uint8_t mk[RS_KC + 1];
+#pragma unroll
+                for (int t = 0; t <= RS_KC; ++t) {  // t = RS_KC: the extra voxel, fetched with the first chunk
+                    const int k = t < RS_KC ? min(k0 + max(min(kc + t, base - 1), 0), s.sx - 1) : kx;
+                    const unsigned e = xoff + (unsigned)k, b = 4u * e;
                     fv[t] = *(const float*)(fr + b);
-                    g[0][t] = *(const float*)(g0r + b);
-                    g[1][t] = *(const float*)(g1r + b);
-                    g[2][t] = *(const float*)(g2r + b);
+                    gv[0][t] = *(const float*)(g0r + b);
+                    gv[1][t] = *(const float*)(g1r + b);
+                    gv[2][t] = *(const float*)(g2r + b);
                     if constexpr (MASKED) mk[t] = mr[e];
                 }
+                auto chunk = [&](auto full) {  // full: every voxel of the chunk is the lane's
 #pragma unroll
-                for (int t = 0; t < RS_KC; ++t) {
-                    bool in = kc + t < k1;
-                    if constexpr (MASKED) in = in && mk[t] != 0;
-                    const float f1 = in ? fv[t] : 0.0f, X = (float)(kc + t - s.hx);
-                    m[0] += f1;
-                    m[1] = __fmaf_rn(f1, f1, m[1]);
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) {
-                        const float gc = in ? g[c][t] : 0.0f, tf = gc * f1;
-                        m[2 + c] += gc;
-                        m[5 + c] = __fmaf_rn(gc, X, m[5 + c]);
-                        m[8 + c] += tf;
-                        m[11 + c] = __fmaf_rn(tf, X, m[11 + c]);
+                    for (int t = 0; t < RS_KC; ++t) {
+                        bool in = decltype(full)::value || kc + t < base;
+                        if constexpr (MASKED) in = in && mk[t] != 0;
+                        add(in, fv[t], gv[0][t], gv[1][t], gv[2][t], k0 + kc + t);
                     }
-                    if constexpr (MASKED) m[NM - 1] += in ? 1.0f : 0.0f;
+                };
+                if (kc + RS_KC <= base) chunk(std::true_type{});  // uniform
+                else chunk(std::false_type{});
+                if (kc == 0) {
+                    bool in = has_extra;
+                    if constexpr (MASKED) in = in && mk[RS_KC] != 0;
+                    add(in, fv[RS_KC], gv[0][RS_KC], gv[1][RS_KC], gv[2][RS_KC], kx);
                 }
             }
 #pragma unroll
             for (int q = 0; q < NM; ++q) m[q] = quad_sum(m[q]);
-            const double Y = (double)y;
+            double v[NSL];
 #pragma unroll
-            for (int i = 0; i < NSL; ++i) {
-                const float v = part == 0 ? m[RS_SRC[0][i]]
-                                          : (part == 1 ? m[RS_SRC[1][i]] : (part == 2 ? m[RS_SRC[2][i]] : m[RS_SRC[3][i]]));
-                P[i] = fma((i == 1 || i == 3) && ywt ? Y : 1.0, (double)v, P[i]);
-            }
-            if (ei < i1 && r.cys[ei] + s.hy == y) {  // a window ends: prefix now minus at its start
-                if (act) {
-                    double* Re = Rp + (int64_t)ei * ncx;
-                    double D[NSL];
+            for (int i = 0; i < NSL; ++i)
+                v[i] = (double)__fmaf_rn(m[RS_SRC[3][i]], w3, __fmaf_rn(m[RS_SRC[2][i]], w2,
+                                         __fmaf_rn(m[RS_SRC[1][i]], w1, m[RS_SRC[0][i]] * w0)));
+            // the row into every window of the group that holds it (each window's rows in order)
 #pragma unroll
-                    for (int i = 0; i < NSL; ++i) D[i] = slot[i] ? Re[qoff[i]] + P[i] : 0.0;
-                    if (ywt) {  // absolute y weights -> offsets from the centre
-                        const double cy = (double)r.cys[ei];
-                        D[1] -= cy * D[0];
-                        D[3] -= cy * D[2];
-                    }
+            for (int g = 0; g < RS_G; ++g) {
+                const int dy = y - cyg[g];
+                if (g < gn && dy >= -s.hy && dy <= s.hy) {  // uniform
+                    const double Y = (double)dy;
 #pragma unroll
-                    for (int i = 0; i < NSL; ++i)
-                        if (slot[i]) Re[qoff[i]] = D[i];
+                    for (int i = 0; i < NSL; ++i) P[g][i] = fma((i == 1 || i == 3) && ywt ? Y : 1.0, v[i], P[g][i]);
                 }
-                ++ei;
+            }
+        }
+        if (act) {
+#pragma unroll
+            for (int g = 0; g < RS_G; ++g) {
+                if (g >= gn) break;
+                double* Rc = r.R + (int64_t)p * RS_Q * cstride + (int64_t)(c0 + g) * ncx + cxi;
+#pragma unroll
+                for (int i = 0; i < NSL; ++i)
+                    if (RS_DST[part][i] >= 0) Rc[(int64_t)RS_DST[part][i] * cstride] = P[g][i];
             }
         }
     }
@@ -1861,19 +1881,73 @@ __global__ __launch_bounds__(BLOCK, 3) void k_icgn12_fused(IcgnArgs a, const int
 // float64 cross-lane sums), then meanf/bottomf, the float64 Cholesky factor and the validity
 // rules of numba_kernels.py:391-417. Subsets that leave the volume without a keep row are
 // rejected before any read (numba_kernels.py:342-343).
+// Shared tail of the precompute: the 81 sums r (the Hessian's upper triangle row by row, sum f,
+// sum f^2, the kept count) -> H, the Cholesky factor L, mean, norm, count and validity of one node
+// (thread 0 decides and factors in LDS, Hs and Ls: 144 doubles each; every thread of the block
+// calls it and copies H and L out).
+__device__ inline void precompute_finish(const double* r, int S, double min_valid_ratio, double cond_max, double* Hs,
+                                         double* Ls, double* H, double* L, double* meanf, double* bottomf,
+                                         int64_t* nvalid, uint8_t* valid) {
+    if (threadIdx.x == 0) {
+        int q = 0;
+        for (int i = 0; i < 12; ++i)
+            for (int j = i; j < 12; ++j) {
+                Hs[i * 12 + j] = r[q];
+                Hs[j * 12 + i] = r[q];
+                ++q;
+            }
+        const double nv = r[80];
+        const int64_t nvi = (int64_t)(nv + 0.5);
+        double mf = 0.0, bf = 1.0;
+        bool ok = true;
+        if (nvi < 27 || (double)nvi < min_valid_ratio * (double)S) {
+            ok = false;
+        } else {
+            mf = r[78] / nv;
+            double ssf = r[79] - nv * mf * mf;
+            ssf = ssf < 0.0 ? 0.0 : ssf;
+            bf = sqrt(ssf > 1e-30 ? ssf : 1e-30);
+            if (bf < 1e-10) ok = false;
+        }
+        *nvalid = nvi;
+        *meanf = mf;
+        *bottomf = bf;
+        if (ok) ok = cholesky12(Hs, Ls);
+        else
+            for (int i = 0; i < 144; ++i) Ls[i] = 0.0;
+        if (ok) {
+            double dmin = Ls[0], dmax = Ls[0];
+            for (int i = 1; i < 12; ++i) {
+                dmin = Ls[i * 13] < dmin ? Ls[i * 13] : dmin;
+                dmax = Ls[i * 13] > dmax ? Ls[i * 13] : dmax;
+            }
+            if (dmin <= 0.0 || (dmax / dmin) * (dmax / dmin) > cond_max) ok = false;
+        }
+        *valid = ok ? 1 : 0;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < 144; i += blockDim.x) {
+        H[i] = Hs[i];
+        L[i] = Ls[i];
+    }
+}
+
+// skip: non-null when the separable precompute may have taken the nodes without keep rows whose
+// subsets lie inside the volume (its flag, set on the device, says whether it did)
 template <bool STORED>
 __global__ __launch_bounds__(BLOCK) void k_precompute(Volumes V, Subset s, const int64_t* __restrict__ coords,
                                                       int64_t N, double min_valid_ratio, double cond_max,
                                                       const int64_t* split_index, const uint8_t* split_keep,
                                                       int64_t keep_bytes, double* H_all, double* L_all,
                                                       double* meanf, double* bottomf, int64_t* nvalid,
-                                                      uint8_t* valid) {
+                                                      uint8_t* valid, const int* __restrict__ skip) {
     const int64_t n = blockIdx.x;
     if (n >= N) return;
     const int tid = threadIdx.x;
     const int64_t x0 = coords[3 * n], y0 = coords[3 * n + 1], z0 = coords[3 * n + 2];
     const uint8_t* keep =
         (split_index && split_keep && split_index[n] >= 0) ? split_keep + split_index[n] * keep_bytes : nullptr;
+    if (skip && *skip && !keep && subset_in_volume(V, s, x0, y0, z0)) return;
     double* H = H_all + 144 * n;
     double* L = L_all + 144 * n;
     if (!keep && !subset_in_volume(V, s, x0, y0, z0)) {
@@ -1893,6 +1967,7 @@ __global__ __launch_bounds__(BLOCK) void k_precompute(Volumes V, Subset s, const
     __shared__ double red[NWAVES * NQ];
     __shared__ double res[NQ];
     __shared__ double Hs[144];
+    __shared__ double Ls[144];
     float acc[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) acc[q] = 0.0f;
@@ -1928,46 +2003,233 @@ __global__ __launch_bounds__(BLOCK) void k_precompute(Volumes V, Subset s, const
         acc[80] += 1.0f;
     }
     block_sum<NQ, NWAVES>(acc, res, red);
-    const double* r = res;
-    if (tid == 0) {
-        int q = 0;
-        for (int i = 0; i < 12; ++i)
-            for (int j = i; j < 12; ++j) {
-                Hs[i * 12 + j] = r[q];
-                Hs[j * 12 + i] = r[q];
-                ++q;
+    precompute_finish(res, s.S, min_valid_ratio, cond_max, Hs, Ls, H, L, meanf + n, bottomf + n, nvalid + n,
+                      valid + n);
+}
+
+// ---------------------------------------------------------------------------- separable precompute
+// The precompute's sums are box sums too: H[(c,i),(d,j)] = sum w g_c g_d q_i q_j with q = (X, Y, Z, 1)
+// is, per gradient product p = (c, d) (six of them), a sum of g_c g_d times a monomial of degree
+// <= 2 in the voxel's offsets; with sum w f, sum w f^2 and the kept count that is 63 box sums per
+// node, taken axis by axis at the distinct node coordinates as the reference moments are (same
+// marks, index and buffers):
+//   k_pc_xy:   per plane and centre pair, per product p: g g {1, Y, Y^2, X, X Y, X^2} (x-row sums of
+//              g g {1, X, X^2} in float, added row by row in double times {1, Y, Y^2}), and f, f^2
+//              and the kept count -- PC_Q = 39 rectangle sums;
+//   k_pc_zfin: per node, one wave: the sz planes' rectangle sums times {1, Z, Z^2}, gathered into
+//              the 81 sums of the per-node kernel, and its tail (validity, Cholesky).
+// Rectangle sums R[((p * PC_Q + q) * ncy + cyi) * ncx + cxi], q = 6 p + k for product p and
+// k = 0 g g, 1 g g Y, 2 g g Y^2, 3 g g X, 4 g g X Y, 5 g g X^2; 36 f, 37 f^2, 38 the kept count.
+constexpr int PC_Q = 39;
+constexpr int PC_G = 2;  // consecutive centres y per work item of k_pc_xy
+constexpr int PC_SLOTS = 12;  // rectangle sums per quad lane: products {0, 1}, {2, 3}, {4} + f, f^2, n, {5}
+// x-row sums: 3 p + {0, 1, 2} = g g {1, X, X^2} of product p, 18 f, 19 f^2, 20 the kept count
+__constant__ const int8_t PC_DST[4][PC_SLOTS] = {{0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11},
+                                                 {12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 22, 23},
+                                                 {24, 25, 26, 27, 28, 29, 36, 37, 38, -1, -1, -1},
+                                                 {30, 31, 32, 33, 34, 35, -1, -1, -1, -1, -1, -1}};
+constexpr int8_t PC_SRC[4][PC_SLOTS] = {{0, 0, 0, 1, 1, 2, 3, 3, 3, 4, 4, 5},
+                                        {6, 6, 6, 7, 7, 8, 9, 9, 9, 10, 10, 11},
+                                        {12, 12, 12, 13, 13, 14, 18, 19, 20, 20, 20, 20},
+                                        {15, 15, 15, 16, 16, 17, 17, 17, 17, 17, 17, 17}};
+constexpr int8_t PC_YPOW[4][PC_SLOTS] = {{0, 1, 2, 0, 1, 0, 0, 1, 2, 0, 1, 0},
+                                         {0, 1, 2, 0, 1, 0, 0, 1, 2, 0, 1, 0},
+                                         {0, 1, 2, 0, 1, 0, 0, 0, 0, 0, 0, 0},
+                                         {0, 1, 2, 0, 1, 0, 0, 0, 0, 0, 0, 0}};
+constexpr int8_t PC_VALID[4] = {12, 12, 9, 6};  // slots in use per part (with the kept count)
+
+// One wave per work item, as k_rs_xy (PC_G centres y per item); lane 4 j + k adds its PC_SLOTS
+// rectangle sums of every window of the item.
+template <bool MASKED>
+__global__ __launch_bounds__(BLOCK, 2) void k_pc_xy(Volumes V, Subset s, RowMoments r) {
+    if (!r.hdr[RS_FITS]) return;
+    constexpr int NM = MASKED ? 21 : 20;
+    const int ncx = r.hdr[RS_NCX], ncy = r.hdr[RS_NCY], zbase = r.hdr[RS_ZBASE];
+    const int ntile = (ncx + RS_TILE - 1) / RS_TILE;
+    const int lane = threadIdx.x & 63, j = lane >> 2, part = lane & 3;
+    const int base = s.sx >> 2, extra = s.sx & 3, k0 = part * base + min(part, extra);
+    const int kx = min(k0 + base, s.sx - 1);
+    const bool has_extra = part < extra;
+    const int64_t plane = (int64_t)V.nx * V.ny, cstride = (int64_t)ncy * ncx;
+    const int nslot = PC_VALID[part] - (MASKED || part != 2 ? 0 : 1);
+    const float w0 = part == 0 ? 1.0f : 0.0f, w1 = part == 1 ? 1.0f : 0.0f, w2 = part == 2 ? 1.0f : 0.0f,
+                w3 = part == 3 ? 1.0f : 0.0f;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int ngroup = (ncy + PC_G - 1) / PC_G;
+    const int64_t pairs = (int64_t)r.hdr[RS_NPLANES] * ngroup, xcd = blockIdx.x & 7;
+    const int64_t lo = pairs * xcd / 8, items = (pairs * (xcd + 1) / 8 - lo) * ntile;
+    for (int64_t it = (blockIdx.x >> 3) * (int64_t)NWAVES + wave; it < items;
+         it += (int64_t)(gridDim.x >> 3) * NWAVES) {
+        const int64_t pair = lo + it / ntile;
+        const int tile = (int)(it - (it / ntile) * ntile), p = (int)(pair / ngroup);
+        const int c0 = (int)(pair - (int64_t)p * ngroup) * PC_G, gn = min(PC_G, ncy - c0);
+        const int cxi = tile * RS_TILE + j;
+        const bool act = cxi < ncx;
+        const int cx = act ? r.cxs[cxi] : r.cxs[0];
+        int cyg[PC_G];
+#pragma unroll
+        for (int g = 0; g < PC_G; ++g) cyg[g] = r.cys[c0 + min(g, gn - 1)];
+        const int64_t pbase = (int64_t)(zbase + p) * plane;
+        const unsigned xoff = (unsigned)(cx - s.hx);
+        double P[PC_G][PC_SLOTS];
+#pragma unroll
+        for (int g = 0; g < PC_G; ++g)
+#pragma unroll
+            for (int i = 0; i < PC_SLOTS; ++i) P[g][i] = 0.0;
+        for (int y = cyg[0] - s.hy; y <= cyg[gn - 1] + s.hy; ++y) {
+            const int64_t orow = pbase + (int64_t)y * V.nx;
+            const char *fr = (const char*)(V.f + orow), *g0r = (const char*)(V.gx + orow),
+                       *g1r = (const char*)(V.gy + orow), *g2r = (const char*)(V.gz + orow);
+            const uint8_t* mr = MASKED ? V.mask + orow : nullptr;
+            float m[NM];
+#pragma unroll
+            for (int q = 0; q < NM; ++q) m[q] = 0.0f;
+            auto add = [&](bool in, float fv, float g0, float g1, float g2, int k) {
+                const float f1 = in ? fv : 0.0f, X = (float)(k - s.hx);
+                const float gc[3] = {in ? g0 : 0.0f, in ? g1 : 0.0f, in ? g2 : 0.0f};
+                const float gg[6] = {gc[0] * gc[0], gc[0] * gc[1], gc[0] * gc[2],
+                                     gc[1] * gc[1], gc[1] * gc[2], gc[2] * gc[2]};
+#pragma unroll
+                for (int q = 0; q < 6; ++q) {
+                    const float gx = gg[q] * X;
+                    m[3 * q] += gg[q];
+                    m[3 * q + 1] += gx;
+                    m[3 * q + 2] = __fmaf_rn(gx, X, m[3 * q + 2]);
+                }
+                m[18] += f1;
+                m[19] = __fmaf_rn(f1, f1, m[19]);
+                if constexpr (MASKED) m[20] += in ? 1.0f : 0.0f;
+            };
+            for (int kc = 0; kc < max(base, 1); kc += RS_KC) {
+                float fv[RS_KC + 1], gv[3][RS_KC + 1];
+                uint8_t mk[RS_KC + 1];
+#pragma unroll
+                for (int t = 0; t <= RS_KC; ++t) {  // t = RS_KC: the extra voxel, fetched with the first chunk
+                    const int k = t < RS_KC ? min(k0 + max(min(kc + t, base - 1), 0), s.sx - 1) : kx;
+                    const unsigned e = xoff + (unsigned)k, b = 4u * e;
+                    fv[t] = *(const float*)(fr + b);
+                    gv[0][t] = *(const float*)(g0r + b);
+                    gv[1][t] = *(const float*)(g1r + b);
+                    gv[2][t] = *(const float*)(g2r + b);
+                    if constexpr (MASKED) mk[t] = mr[e];
+                }
+                auto chunk = [&](auto full) {
+#pragma unroll
+                    for (int t = 0; t < RS_KC; ++t) {
+                        bool in = decltype(full)::value || kc + t < base;
+                        if constexpr (MASKED) in = in && mk[t] != 0;
+                        add(in, fv[t], gv[0][t], gv[1][t], gv[2][t], k0 + kc + t);
+                    }
+                };
+                if (kc + RS_KC <= base) chunk(std::true_type{});  // uniform
+                else chunk(std::false_type{});
+                if (kc == 0) {
+                    bool in = has_extra;
+                    if constexpr (MASKED) in = in && mk[RS_KC] != 0;
+                    add(in, fv[RS_KC], gv[0][RS_KC], gv[1][RS_KC], gv[2][RS_KC], kx);
+                }
             }
-        const double nv = r[80];
-        const int64_t nvi = (int64_t)(nv + 0.5);
-        double mf = 0.0, bf = 1.0;
-        bool ok = true;
-        if (nvi < 27 || (double)nvi < min_valid_ratio * (double)s.S) {
-            ok = false;
-        } else {
-            mf = r[78] / nv;
-            double ssf = r[79] - nv * mf * mf;
-            ssf = ssf < 0.0 ? 0.0 : ssf;
-            bf = sqrt(ssf > 1e-30 ? ssf : 1e-30);
-            if (bf < 1e-10) ok = false;
-        }
-        nvalid[n] = nvi;
-        meanf[n] = mf;
-        bottomf[n] = bf;
-        if (ok) ok = cholesky12(Hs, L);
-        else
-            for (int i = 0; i < 144; ++i) L[i] = 0.0;
-        if (ok) {
-            double dmin = L[0], dmax = L[0];
-            for (int i = 1; i < 12; ++i) {
-                dmin = L[i * 13] < dmin ? L[i * 13] : dmin;
-                dmax = L[i * 13] > dmax ? L[i * 13] : dmax;
+#pragma unroll
+            for (int q = 0; q < NM; ++q) m[q] = quad_sum(m[q]);
+            float v[PC_SLOTS];
+#pragma unroll
+            for (int i = 0; i < PC_SLOTS; ++i) {
+                const int a0 = min((int)PC_SRC[0][i], NM - 1), a1 = min((int)PC_SRC[1][i], NM - 1);
+                const int a2 = min((int)PC_SRC[2][i], NM - 1), a3 = min((int)PC_SRC[3][i], NM - 1);
+                v[i] = __fmaf_rn(m[a3], w3, __fmaf_rn(m[a2], w2, __fmaf_rn(m[a1], w1, m[a0] * w0)));
             }
-            if (dmin <= 0.0 || (dmax / dmin) * (dmax / dmin) > cond_max) ok = false;
+#pragma unroll
+            for (int g = 0; g < PC_G; ++g) {
+                const int dy = y - cyg[g];
+                if (g < gn && dy >= -s.hy && dy <= s.hy) {  // uniform
+                    const double Y = (double)dy, Y2 = Y * Y;
+#pragma unroll
+                    for (int i = 0; i < PC_SLOTS; ++i) {
+                        const int yp = part < 2 ? PC_YPOW[0][i] : PC_YPOW[2][i];
+                        P[g][i] = fma(yp == 0 ? 1.0 : (yp == 1 ? Y : Y2), (double)v[i], P[g][i]);
+                    }
+                }
+            }
         }
-        valid[n] = ok ? 1 : 0;
+        if (act) {
+#pragma unroll
+            for (int g = 0; g < PC_G; ++g) {
+                if (g >= gn) break;
+                double* Rc = r.R + (int64_t)p * PC_Q * cstride + (int64_t)(c0 + g) * ncx + cxi;
+#pragma unroll
+                for (int i = 0; i < PC_SLOTS; ++i)
+                    if (i < nslot) Rc[(int64_t)PC_DST[part][i] * cstride] = P[g][i];
+            }
+        }
+    }
+}
+
+// Index into the per-node z sums (3 q + z power) of each of the 81 precompute sums: the Hessian's
+// upper triangle row by row over SD = (g_x X, g_x Y, g_x Z, g_y X, ..., g_z Z, g_x, g_y, g_z), then
+// f (q 36), f^2 (37) and the kept count (38).
+struct PcGather {
+    uint8_t at[81];
+};
+constexpr PcGather make_pc_gather() {
+    PcGather g{};
+    // q = (X, Y, Z, 1) pairs -> (k, z power): 1 (0,0), Z (0,1), ZZ (0,2), Y (1,0), YZ (1,1), YY (2,0),
+    // X (3,0), XZ (3,1), XY (4,0), XX (5,0)
+    const int mk[4][4] = {{5, 4, 3, 3}, {4, 2, 1, 1}, {3, 1, 0, 0}, {3, 1, 0, 0}};
+    const int mz[4][4] = {{0, 0, 1, 0}, {0, 0, 1, 0}, {1, 1, 2, 1}, {0, 0, 1, 0}};
+    const int pp[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+    int t = 0;
+    for (int i = 0; i < 12; ++i)
+        for (int j = i; j < 12; ++j) {
+            const int ci = i < 9 ? i / 3 : i - 9, qi = i < 9 ? i % 3 : 3;
+            const int cj = j < 9 ? j / 3 : j - 9, qj = j < 9 ? j % 3 : 3;
+            g.at[t++] = (uint8_t)(3 * (6 * pp[ci][cj] + mk[qi][qj]) + mz[qi][qj]);
+        }
+    g.at[78] = 3 * 36;
+    g.at[79] = 3 * 37;
+    g.at[80] = 3 * 38;
+    return g;
+}
+__constant__ const PcGather PC_GATHER = make_pc_gather();
+
+// One 64-lane workgroup per node without keep rows inside the volume: lane q < PC_Q sums rectangle
+// sum q over the node's sz planes times {1, Z, Z^2}; the 81 sums are gathered and finished as by the
+// per-node kernel.
+__global__ __launch_bounds__(64) void k_pc_zfin(Volumes V, Subset s, const int64_t* __restrict__ coords, int64_t N,
+                                                const int64_t* __restrict__ split_index,
+                                                const uint8_t* __restrict__ split_keep, RowMoments r,
+                                                double min_valid_ratio, double cond_max, double* H_all,
+                                                double* L_all, double* meanf, double* bottomf, int64_t* nvalid,
+                                                uint8_t* valid) {
+    if (!r.hdr[RS_FITS]) return;
+    const int64_t n = blockIdx.x;
+    if (n >= N || (split_index && split_keep && split_index[n] >= 0)) return;  // keep rows: per node
+    const int64_t x0 = coords[3 * n], y0 = coords[3 * n + 1], z0 = coords[3 * n + 2];
+    if (!subset_in_volume(V, s, x0, y0, z0)) return;
+    __shared__ double zs[3 * PC_Q];
+    __shared__ double res[81];
+    __shared__ double Hs[144];
+    __shared__ double Ls[144];
+    const int ncx = r.hdr[RS_NCX], ncy = r.hdr[RS_NCY], zbase = r.hdr[RS_ZBASE];
+    const int64_t cstride = (int64_t)ncy * ncx;
+    const int q = threadIdx.x;
+    if (q < PC_Q) {
+        const double* Rq = r.R + (int64_t)q * cstride + (int64_t)r.yidx[y0] * ncx + r.xidx[x0];
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+        for (int dz = -s.hz; dz <= s.hz; ++dz) {
+            const double v = Rq[(int64_t)(z0 + dz - zbase) * PC_Q * cstride], Z = (double)dz;
+            a0 += v;
+            a1 = fma(Z, v, a1);
+            a2 = fma(Z * Z, v, a2);
+        }
+        zs[3 * q] = a0;
+        zs[3 * q + 1] = a1;
+        zs[3 * q + 2] = a2;
     }
     __syncthreads();
-    for (int i = tid; i < 144; i += BLOCK) H[i] = Hs[i];
+    for (int t = threadIdx.x; t < 81; t += 64) res[t] = t == 80 && !V.mask ? (double)s.S : zs[PC_GATHER.at[t]];
+    __syncthreads();
+    precompute_finish(res, s.S, min_valid_ratio, cond_max, Hs, Ls, H_all + 144 * n, L_all + 144 * n, meanf + n,
+                      bottomf + n, nvalid + n, valid + n);
 }
 
 // ---------------------------------------------------------------------------- ZNCC of given warps
@@ -2284,8 +2546,8 @@ void launch_ref_moments(bool sel, const IcgnArgs& a, const int64_t* list, const 
         const unsigned nodes = (unsigned)std::min<int64_t>((N + 255) / 256, 4096);
         hipLaunchKernelGGL(k_rs_clear, dim3((unsigned)((std::max(std::max(nx, ny), (int)RS_HDR) + 255) / 256)),
                            dim3(256), 0, st, *rm, nx, ny);
-        hipLaunchKernelGGL(k_rs_mark, dim3(nodes), dim3(256), 0, st, a, list, count, *rm);
-        hipLaunchKernelGGL(k_rs_index, dim3(1), dim3(1024), 0, st, *rm, nx, ny, a.sub.hz);
+        hipLaunchKernelGGL(k_rs_mark, dim3(nodes), dim3(256), 0, st, a, list, count, (int64_t)0, *rm);
+        hipLaunchKernelGGL(k_rs_index, dim3(1), dim3(1024), 0, st, *rm, nx, ny, a.sub.hz, RS_Q);
         if (a.V.mask) hipLaunchKernelGGL(k_rs_xy<true>, dim3(2048), dim3(BLOCK), 0, st, a.V, a.sub, *rm);
         else hipLaunchKernelGGL(k_rs_xy<false>, dim3(2048), dim3(BLOCK), 0, st, a.V, a.sub, *rm);
         hipLaunchKernelGGL(k_rs_z, dim3(nodes), dim3(BLOCK), 0, st, a, list, count, *rm, mom);
@@ -2468,29 +2730,79 @@ int aldvc_sample_points(const float* vol, int64_t nz, int64_t ny, int64_t nx, co
     return launch_status();
 }
 
-int aldvc_precompute(const int64_t* coords, int64_t N, int hx, int hy, int hz, int stride, const float* f,
-                     const float* gx, const float* gy, const float* gz, const uint8_t* mask, int64_t nz, int64_t ny,
-                     int64_t nx, double min_valid_ratio, double cond_max, const int64_t* split_index,
-                     const uint8_t* split_keep, int64_t keep_bytes, double* H_all, double* L_all, double* meanf,
-                     double* bottomf, int64_t* n_valid, uint8_t* valid, hipStream_t stream) {
+size_t aldvc_precompute_row_bytes(int64_t ncx, int64_t ncy, int64_t nplanes, int64_t ny, int64_t nx) {
+    if (ncx <= 0 || ncy <= 0 || nplanes <= 0) return 0;
+    return row_moments_fixed_bytes(ny, nx) + sizeof(double) * PC_Q * (size_t)nplanes * (size_t)ncy * (size_t)ncx;
+}
+
+int aldvc_precompute_tuned(const int64_t* coords, int64_t N, int hx, int hy, int hz, int stride, const float* f,
+                           const float* gx, const float* gy, const float* gz, const uint8_t* mask, int64_t nz,
+                           int64_t ny, int64_t nx, double min_valid_ratio, double cond_max, const int64_t* split_index,
+                           const uint8_t* split_keep, int64_t keep_bytes, double* H_all, double* L_all, double* meanf,
+                           double* bottomf, int64_t* n_valid, uint8_t* valid, int flags, void* workspace,
+                           size_t workspace_bytes, hipStream_t stream) {
     clear_error();
     Volumes V{f, gx, gy, gz, f, mask, (int)nz, (int)ny, (int)nx};
     const Subset s = make_subset(hx, hy, hz, stride);
     if (int e = check_common(V, s, 1)) return e;
+    if (flags & ~TUNE_ROW_MOMENTS) return fail(-2, "unknown tuning flags");
     if (N < 0) return fail(-2, "negative node count");
     if (N == 0) return 0;
     if (!coords || !H_all || !L_all || !meanf || !bottomf || !n_valid || !valid) return fail(-2, "null pointer");
     if (split_keep && keep_bytes < (s.S + 7) / 8) return fail(-2, "keep rows shorter than ceil(S/8) bytes");
     if (N > INT32_MAX) return fail(-2, "too many nodes for one launch");
+    const int* skip = nullptr;
+    const size_t fixed = row_moments_fixed_bytes(ny, nx);
+    if ((flags & TUNE_ROW_MOMENTS) && gx && stride == 1 && workspace && workspace_bytes >= fixed + PC_Q * sizeof(double)) {
+        // the separable pass for the nodes without keep rows inside the volume (the per-node kernel
+        // then takes the others, or every node when the rectangle sums do not fit)
+        RowMoments rm;
+        char* p = (char*)workspace;
+        rm.hdr = (int*)p;
+        rm.xidx = rm.hdr + RS_HDR;
+        rm.yidx = rm.xidx + nx;
+        rm.cxs = rm.yidx + ny;
+        rm.cys = rm.cxs + nx;
+        rm.R = (double*)(p + fixed);
+        rm.r_doubles = (int64_t)((workspace_bytes - fixed) / sizeof(double));
+        IcgnArgs a{};
+        a.V = V;
+        a.sub = s;
+        a.coords = coords;
+        a.split_index = split_index;
+        a.split_keep = split_keep;
+        a.keep_bytes = keep_bytes;
+        const unsigned nodes = (unsigned)std::min<int64_t>((N + 255) / 256, 4096);
+        hipLaunchKernelGGL(k_rs_clear, dim3((unsigned)((std::max(std::max<int64_t>(nx, ny), (int64_t)RS_HDR) + 255) / 256)),
+                           dim3(256), 0, stream, rm, (int)nx, (int)ny);
+        hipLaunchKernelGGL(k_rs_mark, dim3(nodes), dim3(256), 0, stream, a, (const int64_t*)nullptr,
+                           (const int64_t*)nullptr, N, rm);
+        hipLaunchKernelGGL(k_rs_index, dim3(1), dim3(1024), 0, stream, rm, (int)nx, (int)ny, hz, PC_Q);
+        if (mask) hipLaunchKernelGGL(k_pc_xy<true>, dim3(2048), dim3(BLOCK), 0, stream, V, s, rm);
+        else hipLaunchKernelGGL(k_pc_xy<false>, dim3(2048), dim3(BLOCK), 0, stream, V, s, rm);
+        hipLaunchKernelGGL(k_pc_zfin, dim3((unsigned)N), dim3(64), 0, stream, V, s, coords, N, split_index, split_keep,
+                           rm, min_valid_ratio, cond_max, H_all, L_all, meanf, bottomf, n_valid, valid);
+        skip = rm.hdr + RS_FITS;
+    }
     if (gx)
         hipLaunchKernelGGL(k_precompute<true>, dim3((unsigned)N), dim3(BLOCK), 0, stream, V, s, coords, N,
                            min_valid_ratio, cond_max, split_index, split_keep, keep_bytes, H_all, L_all, meanf,
-                           bottomf, n_valid, valid);
+                           bottomf, n_valid, valid, skip);
     else
         hipLaunchKernelGGL(k_precompute<false>, dim3((unsigned)N), dim3(BLOCK), 0, stream, V, s, coords, N,
                            min_valid_ratio, cond_max, split_index, split_keep, keep_bytes, H_all, L_all, meanf,
-                           bottomf, n_valid, valid);
+                           bottomf, n_valid, valid, skip);
     return launch_status();
+}
+
+int aldvc_precompute(const int64_t* coords, int64_t N, int hx, int hy, int hz, int stride, const float* f,
+                     const float* gx, const float* gy, const float* gz, const uint8_t* mask, int64_t nz, int64_t ny,
+                     int64_t nx, double min_valid_ratio, double cond_max, const int64_t* split_index,
+                     const uint8_t* split_keep, int64_t keep_bytes, double* H_all, double* L_all, double* meanf,
+                     double* bottomf, int64_t* n_valid, uint8_t* valid, hipStream_t stream) {
+    return aldvc_precompute_tuned(coords, N, hx, hy, hz, stride, f, gx, gy, gz, mask, nz, ny, nx, min_valid_ratio,
+                                  cond_max, split_index, split_keep, keep_bytes, H_all, L_all, meanf, bottomf,
+                                  n_valid, valid, 0, nullptr, 0, stream);
 }
 
 }  // extern "C"

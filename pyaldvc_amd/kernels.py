@@ -212,8 +212,11 @@ def sample_points(vol, zs, ys, xs, mode):
 # --------------------------------------------------------------------------- precompute
 
 
-def precompute_nodes(coords, hx, hy, hz, f, gx, gy, gz, mask, min_valid_ratio, cond_max, stride=1, split_index=None, split_keep=None):
-    """``(H_all (N,12,12), L_all (N,12,12), meanf (N,), bottomf (N,), n_valid (N,), valid (N,))``."""
+def precompute_nodes(coords, hx, hy, hz, f, gx, gy, gz, mask, min_valid_ratio, cond_max, stride=1, split_index=None,
+                     split_keep=None, separable=None):  # fmt: skip
+    """``(H_all (N,12,12), L_all (N,12,12), meanf (N,), bottomf (N,), n_valid (N,), valid (N,))``.
+    ``separable``: the separable pass for the nodes without keep rows (None: by
+    ``separable_wanted``)."""
     A = _Args(f)
     c = A.arr(coords, np.int64)
     N = int(c.shape[0])
@@ -223,6 +226,9 @@ def precompute_nodes(coords, hx, hy, hz, f, gx, gy, gz, mask, min_valid_ratio, c
     md = _mask(A, f, mask)
     si, sk, kb = _split(A, N, split_index, split_keep)
     dev = fd.device
+    ext = _separable_extent(c, None, hx, hy, hz, stride, (nz, ny, nx), gxd is not None, separable)
+    ws_bytes = 0 if ext is None else int(lib().aldvc_precompute_row_bytes(*ext, ny, nx))
+    ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
     H = torch.zeros((N, 12, 12), dtype=torch.float64, device=dev)
     L = torch.zeros((N, 12, 12), dtype=torch.float64, device=dev)
     meanf = torch.zeros(N, dtype=torch.float64, device=dev)
@@ -230,12 +236,13 @@ def precompute_nodes(coords, hx, hy, hz, f, gx, gy, gz, mask, min_valid_ratio, c
     nvalid = torch.zeros(N, dtype=torch.int64, device=dev)
     valid = torch.zeros(N, dtype=torch.uint8, device=dev)
     check(
-        lib().aldvc_precompute(
+        lib().aldvc_precompute_tuned(
             _ptr(c), N, int(hx), int(hy), int(hz), int(stride), _ptr(fd), _ptr(gxd), _ptr(gyd), _ptr(gzd), _ptr(md),
             nz, ny, nx, float(min_valid_ratio), float(cond_max), _ptr(si), _ptr(sk), kb,
-            _ptr(H), _ptr(L), _ptr(meanf), _ptr(bottomf), _ptr(nvalid), _ptr(valid), stream_handle(),
+            _ptr(H), _ptr(L), _ptr(meanf), _ptr(bottomf), _ptr(nvalid), _ptr(valid),
+            TUNE_ROW_MOMENTS if ws_bytes else 0, _ptr(ws), ws_bytes, stream_handle(),
         ),
-        "aldvc_precompute",
+        "aldvc_precompute_tuned",
     )  # fmt: skip
     return A.out(H), A.out(L), A.out(meanf), A.out(bottomf), A.out(nvalid), A.out(valid.bool())
 
@@ -293,27 +300,40 @@ def build_split_rows(coords, cand, hx, hy, hz, stride, mask, margin=0):
 
 # --------------------------------------------------------------------------- 12-DOF IC-GN
 
-TUNE_ROW_MOMENTS = 32  # aldvc_icgn12_tuned / aldvc_icgn3_tuned: separable reference moments
+TUNE_ROW_MOMENTS = 32  # aldvc_icgn12_tuned, aldvc_icgn3_tuned, aldvc_precompute_tuned: the separable passes
 ROW_MOMENTS_MIN_READS = 500_000_000  # subset-voxel reads of the per-node pass (~0.8 ms of it) below which it stays
 
 
-def row_moments_bytes(c, vd, hx, hy, hz, stride, dims, stored) -> int:
-    """Extra workspace of the separable reference-moment pass (flag ``TUNE_ROW_MOMENTS``) for the
-    nodes ``c`` (device ``(N, 3)`` int64, x y z), 0 when they take the per-node pass: sized from the
-    distinct x and y and the z range of the valid nodes whose subsets lie inside the volume (one
-    device-to-host read of four counts). The pass needs stored gradients and stride 1; it weights
-    the sums by the reference mask, and nodes with keep rows take the per-node pass. It is taken only where the per-node pass would read the
-    volume many times over (``N * S >= ROW_MOMENTS_MIN_READS`` and at least four times the voxel
-    count): its cost follows the volume, the per-node pass's the node count, and below that the
-    separable pass's fixed cost (six launches and the count read) is not repaid."""
-    if not stored or int(stride) != 1 or c.shape[0] == 0:
-        return 0
+def separable_wanted(n_nodes, half, stride, dims, stored) -> bool:
+    """Whether the separable passes (reference moments, precompute) pay for ``n_nodes`` nodes of
+    half-widths ``half`` in a volume of shape ``dims`` (z, y, x): stored gradients, stride 1, and at
+    least ``ROW_MOMENTS_MIN_READS`` subset-voxel reads of the per-node pass and four times the voxel
+    count (the separable pass's cost follows the volume, the per-node pass's the node count; below
+    that, its extra launches and the count read are not repaid). A caller that cuts a volume into
+    boxes decides once for the whole and passes the answer to every box (``separable=``), so that
+    boxes and whole take the same pass."""
+    if not stored or int(stride) != 1 or n_nodes <= 0:
+        return False
     nz, ny, nx = dims
-    reads = int(c.shape[0]) * (2 * int(hx) + 1) * (2 * int(hy) + 1) * (2 * int(hz) + 1)
-    if reads < ROW_MOMENTS_MIN_READS or reads < 4 * nz * ny * nx:
-        return 0
+    hx, hy, hz = (int(h) for h in half)
+    reads = int(n_nodes) * (2 * hx + 1) * (2 * hy + 1) * (2 * hz + 1)
+    return reads >= ROW_MOMENTS_MIN_READS and reads >= 4 * nz * ny * nx
+
+
+def _separable_extent(c, vd, hx, hy, hz, stride, dims, stored, separable=None):
+    """``(ncx, ncy, nplanes)`` of the separable passes over the nodes ``c`` (device ``(N, 3)``
+    int64, x y z) whose subsets lie inside the volume (and are valid, when ``vd`` is given), or None
+    where the per-node pass runs: on-the-fly gradients, stride > 1, ``separable`` False, or (None)
+    ``separable_wanted`` says no. One device-to-host read of four counts."""
+    if not stored or int(stride) != 1 or c.shape[0] == 0 or separable is False:
+        return None
+    nz, ny, nx = dims
+    if separable is None and not separable_wanted(int(c.shape[0]), (hx, hy, hz), stride, dims, stored):
+        return None
     x, y, z = c[:, 0], c[:, 1], c[:, 2]
-    ok = (vd != 0) & (x >= hx) & (x < nx - hx) & (y >= hy) & (y < ny - hy) & (z >= hz) & (z < nz - hz)
+    ok = (x >= hx) & (x < nx - hx) & (y >= hy) & (y < ny - hy) & (z >= hz) & (z < nz - hz)
+    if vd is not None:
+        ok &= vd != 0
     mx = torch.zeros(nx + 1, dtype=torch.int64, device=c.device)
     my = torch.zeros(ny + 1, dtype=torch.int64, device=c.device)
     mx[torch.where(ok, x, nx)] = 1
@@ -322,8 +342,16 @@ def row_moments_bytes(c, vd, hx, hy, hz, stride, dims, stored) -> int:
     zhi = torch.where(ok, z, -1).max()
     ncx, ncy, zlo, zhi = (int(v) for v in torch.stack([mx[:nx].sum(), my[:ny].sum(), zlo, zhi]).tolist())
     if zhi < zlo:
-        return 0
-    return int(lib().aldvc_icgn12_row_moments_bytes(ncx, ncy, zhi - zlo + 2 * int(hz) + 1, ny, nx))
+        return None
+    return ncx, ncy, zhi - zlo + 2 * int(hz) + 1
+
+
+def row_moments_bytes(c, vd, hx, hy, hz, stride, dims, stored, separable=None) -> int:
+    """Extra workspace of the separable reference-moment pass (flag ``TUNE_ROW_MOMENTS``) for the
+    nodes ``c``, 0 when they take the per-node pass (``_separable_extent``). The pass weights the
+    sums by the reference mask; nodes with keep rows take the per-node pass."""
+    ext = _separable_extent(c, vd, hx, hy, hz, stride, dims, stored, separable)
+    return 0 if ext is None else int(lib().aldvc_icgn12_row_moments_bytes(*ext, dims[1], dims[2]))
 
 
 
@@ -331,6 +359,7 @@ def icgn_12dof_parallel(
     coords, P0, hx, hy, hz, f, gx, gy, gz, mask, g, mode, L_all, meanf_all, bottomf_all, valid,
     tol, dp_tol, max_iter, patience, stride=1, H_all=None, pattern=None, noise_gain=0.0,
     predictive=True, split_index=None, split_keep=None, variant: str = "fused", stream=None, tuning=None,
+    separable=None,
 ):  # fmt: skip
     """12-DOF IC-GN over all nodes: ``(P_out (N,12), n_iter (N,) int32, status (N,) int8, zncc (N,))``.
 
@@ -339,9 +368,11 @@ def icgn_12dof_parallel(
     volume in LDS bricks where the warped subset fits one; ``"fused_gather"`` is the one-pass
     kernel with memory gathers only; ``"auto"`` lets the autotuner pick the fused launch shape for
     this subset shape on this device (timed once on a node sample, then cached). ``tuning`` (an
-    ``autotune.Tuning``) sets the launch shape explicitly. ``meanf_all`` feeds the fused kernel's
-    accumulation pilot only; ``bottomf_all`` is accepted for signature parity (the 12-DOF solver
-    renormalises on the dynamic voxel set, numba_kernels.py:680-708).
+    ``autotune.Tuning``) sets the launch shape explicitly. ``separable``: the reference moments of
+    the nodes without keep rows by the separable pass (None: by ``separable_wanted``).
+    ``meanf_all`` feeds the fused kernel's accumulation pilot only; ``bottomf_all`` is accepted
+    for signature parity (the 12-DOF solver renormalises on the dynamic voxel set,
+    numba_kernels.py:680-708).
     """
     if variant == "auto" and tuning is None:
         from . import autotune
@@ -380,7 +411,7 @@ def icgn_12dof_parallel(
     zncc = torch.zeros(N, dtype=torch.float64, device=dev)
     var = VARIANTS[variant]
     ws_bytes = int(lib().aldvc_icgn12_workspace_bytes(N, int(hx), int(hy), int(hz), int(stride), var))
-    extra = row_moments_bytes(c, vd, hx, hy, hz, stride, (nz, ny, nx), gxd is not None) if var != 0 else 0
+    extra = 0 if var == 0 else row_moments_bytes(c, vd, hx, hy, hz, stride, (nz, ny, nx), gxd is not None, separable)
     if extra:
         flags |= TUNE_ROW_MOMENTS
         ws_bytes += extra
@@ -406,13 +437,15 @@ def icgn_3dof_parallel(
     coords, U_old, F_fixed, vdual, hx, hy, hz, f, gx, gy, gz, mask, g, mode, H_all, meanf_all,
     bottomf_all, valid, mu, tol, dp_tol, max_iter, patience, stride=1, n_full=0.0, noise_gain=0.0,
     predictive=True, split_index=None, split_keep=None, stream=None, variant: str = "auto", tuning=None,
+    separable=None,
 ):  # fmt: skip
     """ADMM subproblem-1 translation IC-GN: ``(U_out (N,3), n_iter, status, zncc)``.
 
     The fused kernel's translation-only form. ``variant="auto"`` (default) takes the autotuner's
     launch shape for this subset shape on this device (timed once on a node sample, then cached);
     ``"fused"``: LDS bricks where a chunk fits one; ``"fused_gather"``: memory gathers only.
-    ``tuning`` (an ``autotune.Tuning``) sets the launch shape explicitly."""
+    ``tuning`` (an ``autotune.Tuning``) sets the launch shape explicitly; ``separable`` as for
+    ``icgn_12dof_parallel``."""
     if tuning is None:
         if variant == "auto":
             from . import autotune
@@ -452,7 +485,7 @@ def icgn_3dof_parallel(
     status = torch.zeros(N, dtype=torch.int8, device=dev)
     zncc = torch.zeros(N, dtype=torch.float64, device=dev)
     ws_bytes = int(lib().aldvc_icgn3_workspace_bytes(N))
-    extra = row_moments_bytes(c, vd, hx, hy, hz, stride, (nz, ny, nx), gxd is not None)
+    extra = row_moments_bytes(c, vd, hx, hy, hz, stride, (nz, ny, nx), gxd is not None, separable)
     ws_bytes += extra
     ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
     check(

@@ -115,6 +115,8 @@ class LocalContext:
         self.split_fraction = split_fraction
         self._volumes = volumes  # device copies of f, gx, gy, gz, mask for the solves
         self._host: dict = {}
+        # separable passes for the whole reference and every box of it (None: each call decides)
+        self.separable = None
 
     def _h(self, name):
         if name not in self._host:
@@ -253,8 +255,10 @@ def precompute_local_context(mesh, ref, para) -> LocalContext:
     node_valid = np.asarray(mesh.node_valid, dtype=bool)
     node_valid_d = torch.from_numpy(node_valid).to(dev)
     N = coords_int.shape[0]
+    # one choice of pass for the whole reference, boxes included, so that they give the same bits
+    sep = K.separable_wanted(N, half, stride, shape, vols["stored"])
     if not plan.whole:
-        return _precompute_tiled(plan, vols, para, coords_int, coords_d, node_valid_d, half, stride, t0)
+        return _precompute_tiled(plan, vols, para, coords_int, coords_d, node_valid_d, half, stride, t0, sep)
 
     split = _split_rows(vols, para, coords_d, node_valid_d, half, stride)
     split_kw = {} if split is None else {"split_index": split[0], "split_keep": split[1]}
@@ -262,7 +266,7 @@ def precompute_local_context(mesh, ref, para) -> LocalContext:
     mask = vols["mask"] if vols["mask"] is not None else _nomask()
     H, L, meanf, bottomf, n_valid, valid = K.precompute_nodes(
         coords_d, hx, hy, hz, vols["f"], gx, gy, gz, mask, float(para.min_valid_ratio), float(para.hessian_cond_max),
-        stride, **split_kw,
+        stride, **split_kw, separable=sep,
     )  # fmt: skip
     valid = valid & node_valid_d
     device = {"coords": coords_d, "H": H, "L": L, "meanf": meanf, "bottomf": bottomf, "n_valid": n_valid,
@@ -280,10 +284,12 @@ def precompute_local_context(mesh, ref, para) -> LocalContext:
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     logger.info("Local precompute: %d nodes, %d valid, %.2fs", N, int(valid.sum()), dt)
-    return LocalContext(coords_int, half, device, dt, stride, noise_hessian_pattern(half, stride), split_fraction, vols)
+    ctx = LocalContext(coords_int, half, device, dt, stride, noise_hessian_pattern(half, stride), split_fraction, vols)
+    ctx.separable = sep
+    return ctx
 
 
-def _precompute_tiled(plan, vols, para, coords_int, coords_d, node_valid_d, half, stride, t0) -> LocalContext:
+def _precompute_tiled(plan, vols, para, coords_int, coords_d, node_valid_d, half, stride, t0, sep) -> LocalContext:
     """precompute_local_context over the boxes of ``plan`` (local_icgn.py:250-300): per box the
     crops of f, mask and gradients, the split rows of its nodes, the precompute kernel on the
     shifted coordinates; the per-node results and split rows scattered back into whole arrays."""
@@ -329,7 +335,7 @@ def _precompute_tiled(plan, vols, para, coords_int, coords_d, node_valid_d, half
             kw = {"split_index": si, "split_keep": rows}
             parts.append((nodes_d, cand, rows, n_keep, n_inmask))
         out = K.precompute_nodes(loc, hx, hy, hz, f_t, gx, gy, gz, m_t, float(para.min_valid_ratio),
-                                 float(para.hessian_cond_max), stride, **kw)  # fmt: skip
+                                 float(para.hessian_cond_max), stride, **kw, separable=sep)  # fmt: skip
         for dst, src in zip((H, L, meanf, bottomf, n_valid), out[:5]):
             dst[nodes_d] = src
         valid[nodes_d] = out[5].bool()
@@ -358,6 +364,7 @@ def _precompute_tiled(plan, vols, para, coords_int, coords_d, node_valid_d, half
     dt = time.perf_counter() - t0
     logger.info("Local precompute: %d nodes, %d valid, %d tiles, %.2fs", N, int(valid.sum()), len(plan), dt)
     ctx = LocalContext(coords_int, half, device, dt, stride, noise_hessian_pattern(half, stride), split_fraction, vols)
+    ctx.separable = sep
     ctx.n_tiles = len(plan)
     return ctx
 
@@ -458,6 +465,7 @@ def local_icgn(ctx: LocalContext, ref, g, U0, para, mesh, F0=None):
                 float(para.icgn_dp_tol), int(para.icgn_max_iter), int(para.icgn_patience), ctx.stride, d["H"][nodes_d],
                 pattern, gain, bool(para.icgn_predictive_stop),
                 **local_split(d.get("split_index"), d.get("split_keep"), nodes_d), variant=ICGN12_VARIANT,
+                separable=ctx.separable,
             )  # fmt: skip
 
         P, n_iter, status, zncc = _solve_tiled(ctx, mesh, para, U0, run, 4)
@@ -466,6 +474,7 @@ def local_icgn(ctx: LocalContext, ref, g, U0, para, mesh, F0=None):
             d["coords"], P0, hx, hy, hz, vols["f"], gx, gy, gz, mask, gd, mode, d["L"], d["meanf"], d["bottomf"],
             d["valid"], float(para.icgn_tol), float(para.icgn_dp_tol), int(para.icgn_max_iter), int(para.icgn_patience),
             ctx.stride, d["H"], pattern, gain, bool(para.icgn_predictive_stop), **ctx.split_args(), variant=ICGN12_VARIANT,
+            separable=ctx.separable,
         )  # fmt: skip
     P = P.cpu().numpy()
     n_iter = n_iter.cpu().numpy().astype(np.int32)
@@ -512,7 +521,7 @@ def subpb1_solver(ctx: LocalContext, ref, g, U_hat, F_hat, vdual, mu, para, mesh
                 mode, d["H"][nodes_d], d["meanf"][nodes_d], d["bottomf"][nodes_d], d["valid"][nodes_d], float(mu),
                 float(para.icgn_tol), float(para.icgn_dp_tol), int(para.icgn_max_iter), int(para.icgn_patience),
                 ctx.stride, float(pattern[9, 9]), gain, bool(para.icgn_predictive_stop),
-                **local_split(d.get("split_index"), d.get("split_keep"), nodes_d),
+                **local_split(d.get("split_index"), d.get("split_keep"), nodes_d), separable=ctx.separable,
             )  # fmt: skip
 
         U, n_iter, status, zncc = _solve_tiled(ctx, mesh, para, U_hat + vdual, run, 4)
@@ -521,7 +530,7 @@ def subpb1_solver(ctx: LocalContext, ref, g, U_hat, F_hat, vdual, mu, para, mesh
             d["coords"], Uh_d, Fh_d, vd_d, hx, hy, hz, vols["f"], gx, gy, gz, mask, gd, mode, d["H"], d["meanf"],
             d["bottomf"], d["valid"], float(mu), float(para.icgn_tol), float(para.icgn_dp_tol), int(para.icgn_max_iter),
             int(para.icgn_patience), ctx.stride, float(pattern[9, 9]), gain, bool(para.icgn_predictive_stop),
-            **ctx.split_args(),
+            **ctx.split_args(), separable=ctx.separable,
         )  # fmt: skip
     # the cleanup on the device: median test (unknown nodes spring-filled first) and
     # U_filled = bad ? U_hat : U (nodes the local solver could not handle keep the global estimate)

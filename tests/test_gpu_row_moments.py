@@ -230,3 +230,87 @@ def test_row_moments_mask_and_keep_rows(case, masked, split):
         kr = [n for n in inside if case["split_index_np"][n] >= 0]
         assert len(kr) > 10
         assert np.array_equal(rows[kr], node[kr])
+
+
+def _precompute(c, flags, extra, masked, split):
+    """aldvc_precompute_tuned on the case's nodes: (H, L, meanf, bottomf, n_valid, valid) on the host."""
+    import torch
+
+    from pyaldvc_amd import _lib
+
+    lib = _lib.lib()
+    nz, ny, nx = c["f"].shape
+    N = c["coords"].shape[0]
+    hx, hy, hz = HALF
+    cd = torch.from_numpy(c["coords"]).cuda()
+    H = torch.full((N, 144), 7.0, dtype=torch.float64, device="cuda")
+    L = torch.full((N, 144), 7.0, dtype=torch.float64, device="cuda")
+    mf = torch.full((N,), 7.0, dtype=torch.float64, device="cuda")
+    bf = torch.full((N,), 7.0, dtype=torch.float64, device="cuda")
+    nv = torch.full((N,), -7, dtype=torch.int64, device="cuda")
+    va = torch.full((N,), 7, dtype=torch.uint8, device="cuda")
+    ws = torch.full((max(extra, 1),), 0xFF, dtype=torch.uint8, device="cuda")
+    rc = lib.aldvc_precompute_tuned(
+        _p(cd), N, hx, hy, hz, 1, _p(c["f"]), _p(c["gx"]), _p(c["gy"]), _p(c["gz"]),
+        _p(c["mask"]) if masked else None, nz, ny, nx, 0.5, 1e12,
+        _p(c["split_index"]) if split else None, _p(c["split_keep"]) if split else None,
+        int(c["split_keep"].shape[1]) if split else 0, _p(H), _p(L), _p(mf), _p(bf), _p(nv), _p(va),
+        flags, _p(ws), extra, ctypes.c_void_p(torch.cuda.current_stream().cuda_stream),
+    )  # fmt: skip
+    _lib.check(rc, "aldvc_precompute_tuned")
+    torch.cuda.synchronize()
+    return [t.cpu().numpy() for t in (H, L, mf, bf, nv, va)]
+
+
+@pytest.mark.parametrize("masked,split", [(False, False), (True, False), (False, True), (True, True)])
+def test_separable_precompute(case, masked, split):
+    """The precompute's Hessian, mean, norm, count, validity and Cholesky factor from the separable
+    pass (flag 32) against float64 sums over each node's kept voxels computed here, and against the
+    per-node pass; nodes with keep rows take the per-node pass in the same call (bit for bit)."""
+    from pyaldvc_amd import _lib
+
+    f, gs = case["f_np"].astype(np.float64), [g.astype(np.float64) for g in case["g_np"]]
+    hx, hy, hz = HALF
+    nz, ny, nx = f.shape
+    Z, Y, X = np.meshgrid(np.arange(-hz, hz + 1), np.arange(-hy, hy + 1), np.arange(-hx, hx + 1), indexing="ij")
+    q = [X.astype(float), Y.astype(float), Z.astype(float), np.ones(X.shape)]
+    N = case["coords"].shape[0]
+    keep_node = (case["split_index_np"] >= 0) if split else np.zeros(N, bool)
+    inside = [n for n, (x0, y0, z0) in enumerate(case["coords"])
+              if hx <= x0 < nx - hx and hy <= y0 < ny - hy and hz <= z0 < nz - hz]  # fmt: skip
+    sep = [n for n in inside if not keep_node[n]]
+    ins = case["coords"][sep]
+    planes = int(ins[:, 2].max() - ins[:, 2].min()) + 2 * hz + 1
+    extra = int(_lib.lib().aldvc_precompute_row_bytes(len(np.unique(ins[:, 0])), len(np.unique(ins[:, 1])), planes,
+                                                      ny, nx))  # fmt: skip
+    rows = _precompute(case, 32, extra, masked, split)
+    node = _precompute(case, 0, 0, masked, split)
+    for n in inside:
+        x0, y0, z0 = case["coords"][n]
+        sl = (slice(z0 - hz, z0 + hz + 1), slice(y0 - hy, y0 + hy + 1), slice(x0 - hx, x0 + hx + 1))
+        w = np.ones(X.shape)
+        if keep_node[n]:
+            w = case["rows"][case["split_index_np"][n]].reshape(X.shape).astype(float)
+        elif masked:
+            w = (case["mask_np"][sl] != 0).astype(float)
+        g = [gs[c][sl] for c in range(3)]
+        sd = [g[c] * q[j] for c in range(3) for j in range(3)] + [g[0], g[1], g[2]]
+        Hd = np.array([[np.sum(w * a * b) for b in sd] for a in sd])
+        Hm = np.array([[np.sum(np.abs(w * a * b)) for b in sd] for a in sd])
+        cnt = int(w.sum())
+        for got, rel in ((rows, 1e-6), (node, 1e-5)):
+            Hg, Lg, mfg, bfg, nvg, vag = (a[n] for a in got)
+            assert nvg == cnt
+            assert np.all(np.abs(Hg.reshape(12, 12) - Hd) <= rel * Hm + 1e-9), n
+            if cnt >= 27 and cnt >= 0.5 * X.size:
+                mfd = np.sum(w * f[sl]) / cnt
+                bfd = np.sqrt(max(np.sum(w * f[sl] ** 2) - cnt * mfd * mfd, 1e-30))
+                assert abs(mfg - mfd) <= 1e-6 * abs(mfd) + 1e-9
+                assert abs(bfg - bfd) <= 1e-5 * bfd
+        assert rows[5][n] == node[5][n], n
+        if rows[5][n]:
+            Ld = np.linalg.cholesky(Hd)
+            assert np.allclose(rows[1][n].reshape(12, 12), Ld, rtol=1e-5, atol=1e-5 * np.abs(Ld).max())
+        if keep_node[n]:
+            assert all(np.array_equal(a[n], b[n]) for a, b in zip(rows, node)), n
+    assert sum(1 for n in inside if rows[5][n]) > 50
