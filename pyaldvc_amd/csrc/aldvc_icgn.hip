@@ -2393,8 +2393,9 @@ __global__ __launch_bounds__(1024) void k_compact_tiled(const uint8_t* __restric
     const int T = ntx * nty * ntz;
     for (int t = threadIdx.x; t < T; t += 1024) hist[t] = 0u;
     __syncthreads();
-    auto tile_of = [&](int64_t n) {
-        int cx = (int)(coords[3 * n] / tx), cy = (int)(coords[3 * n + 1] / ty), cz = (int)(coords[3 * n + 2] / tz);
+    auto tile_of = [&](int64_t n) {  // coordinates clamped to int first: 32-bit quotients
+        auto c32 = [](int64_t v) { return (int)(v < INT_MIN ? INT_MIN : (v > INT_MAX ? INT_MAX : v)); };
+        int cx = c32(coords[3 * n]) / tx, cy = c32(coords[3 * n + 1]) / ty, cz = c32(coords[3 * n + 2]) / tz;
         cx = cx < 0 ? 0 : (cx >= ntx ? ntx - 1 : cx);
         cy = cy < 0 ? 0 : (cy >= nty ? nty - 1 : cy);
         cz = cz < 0 ? 0 : (cz >= ntz ? ntz - 1 : cz);
