@@ -973,14 +973,25 @@ __global__ void k_rs_clear(RowMoments r, int nx, int ny) {
 __global__ void k_rs_mark(IcgnArgs a, const int64_t* __restrict__ list, const int64_t* __restrict__ count,
                           int64_t n_all, RowMoments r) {
     const int64_t M = list ? *count : n_all;
+    int zlo = INT_MAX, zhi = -1;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < M; i += (int64_t)gridDim.x * blockDim.x) {
         const int64_t n = list ? list[i] : i;
         const int64_t x0 = a.coords[3 * n], y0 = a.coords[3 * n + 1], z0 = a.coords[3 * n + 2];
         if (keep_row_of(a, n) || !subset_in_volume(a.V, a.sub, x0, y0, z0)) continue;
         r.xidx[x0] = 1;
         r.yidx[y0] = 1;
-        atomicMin(&r.hdr[RS_ZLO], (int)z0);
-        atomicMax(&r.hdr[RS_ZHI], (int)z0);
+        zlo = min(zlo, (int)z0);
+        zhi = max(zhi, (int)z0);
+    }
+    // the z range: one atomic pair per wave
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        zlo = min(zlo, __shfl_xor(zlo, off, 64));
+        zhi = max(zhi, __shfl_xor(zhi, off, 64));
+    }
+    if ((threadIdx.x & 63) == 0 && zhi >= 0) {
+        atomicMin(&r.hdr[RS_ZLO], zlo);
+        atomicMax(&r.hdr[RS_ZHI], zhi);
     }
 }
 
