@@ -922,19 +922,22 @@ __global__ __launch_bounds__(BLOCK) void k_ref_moments(IcgnArgs a, const int64_t
 // For a node without keep rows (stride 1), the reference moments are box sums over its subset cube
 // of per-voxel terms weighted by the reference mask w (1 everywhere without one): w, w f, w f^2,
 // w grad_c, w grad_c f, and the grad terms times the voxel's offset X, Y or Z from the node, with
-// the pilot pf folded in afterwards (sum w f' = sum w f - n pf, ...). Nodes on a lattice share most of these sums: the x-sums of one
-// voxel row serve every node whose centre has that x and whose subset holds the row, and so on
-// along y and z. So the sums are taken axis by axis at the distinct node coordinates only:
+// the pilot pf folded in afterwards (sum w f' = sum w f - n pf, ...). Nodes on a lattice share
+// most of these sums: the x-sums of one voxel row serve every node whose centre has that x and
+// whose subset holds the row, and so on along y and z. So the sums are taken axis by axis at the
+// distinct node coordinates only:
 //   k_rs_xy: per plane z, distinct centre x (cx) and distinct centre y (cy), the 20 sums over the
 //            sx x sy rectangle around (cx, cy): x-row sums in float (four lanes a row, a quarter
 //            each, combined), added row by row in double with the row's offset Y from cy;
 //   k_rs_z:  per node, its sz planes' rectangle sums, with the Z weights, in the layout of
 //            k_ref_moments.
-// A voxel is then read by about (sx / x spacing) (sy / y spacing) lanes instead of once per node
-// whose subset holds it. Every sum is formed from the node's own subset voxels in a fixed order
+// A voxel is then read by about sx / (x spacing) lanes for each work item whose rows hold it (four
+// consecutive windows per item: about 1.8 items at a lattice step of hy / 2) instead of once per
+// node whose subset holds it. Every sum is formed from the node's own subset voxels in a fixed order
 // (quarter rows, rows, planes), so a node's moments do not depend on the other nodes of the call:
-// a volume cut into boxes gives the same bits as the whole. The distinct coordinates come from marks over the
-// node list (k_rs_mark, k_rs_index). Rectangle sums: R[((p * RS_Q + q) * ncy + cyi) * ncx + cxi]
+// a volume cut into boxes gives the same bits as the whole. The distinct coordinates come from
+// marks over the node list (k_rs_mark, k_rs_index). Rectangle sums:
+// R[((p * RS_Q + q) * ncy + cyi) * ncx + cxi]
 // for plane p = z - zbase, with q = 0 f, 1 f^2, 2 + c grad_c, 5 + c grad_c X, 8 + c grad_c f,
 // 11 + c grad_c f X, 14 + c grad_c Y, 17 + c grad_c f Y, 20 the kept count (with a mask only), each
 // weighted by w. Nodes with keep rows take the per-node pass.
