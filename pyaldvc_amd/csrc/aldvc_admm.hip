@@ -57,30 +57,45 @@ __device__ inline bool gated(const int* stop) { return stop && *stop; }
 // ---------------------------------------------------------------------------- spring fill
 // Unknown nodes of column c: bit c of unknown[a]. Springs join 6-neighbours inside the grid, except
 // across cut edges (edge_ok (N, 3): the +x, +y, +z edge of every node; null for an uncut mesh).
+//
+// The springs of node a, as 6 bits in the order -x, +x, -y, +y, -z, +z. Worked out once per solve,
+// so the operator's stencil is a byte and three strides (no index decode on the solve's path).
+__device__ inline unsigned spring_code(const NodeGrid& g, const uint8_t* edge_ok, int64_t a) {
+    int p[3];
+    g.pos(a, p);
+    const int64_t stride[3] = {1, (int64_t)g.nx, (int64_t)g.nx * g.ny};
+    unsigned m = 0;
+    for (int ax = 0; ax < 3; ++ax) {
+        if (p[ax] > 0 && (!edge_ok || edge_ok[3 * (a - stride[ax]) + ax])) m |= 1u << (2 * ax);
+        if (p[ax] < g.dim(ax) - 1 && (!edge_ok || edge_ok[3 * a + ax])) m |= 2u << (2 * ax);
+    }
+    return m;
+}
+
 struct SpringOp {
     NodeGrid g;
     const uint8_t* unknown;
-    const uint8_t* edge_ok;
+    const uint8_t* code;  // spring_code of every node
 
     template <class F>
     __device__ void for_springs(int64_t a, F f) const {
-        int p[3];
-        g.pos(a, p);
-        const int64_t stride[3] = {1, (int64_t)g.nx, (int64_t)g.nx * g.ny};
-        for (int ax = 0; ax < 3; ++ax) {
-            if (p[ax] > 0 && (!edge_ok || edge_ok[3 * (a - stride[ax]) + ax])) f(a - stride[ax]);
-            if (p[ax] < g.dim(ax) - 1 && (!edge_ok || edge_ok[3 * a + ax])) f(a + stride[ax]);
-        }
+        const unsigned m = code[a];
+        const int64_t sy = g.nx, sz = (int64_t)g.nx * g.ny;
+        if (m & 1u) f(a - 1);
+        if (m & 2u) f(a + 1);
+        if (m & 4u) f(a - sy);
+        if (m & 8u) f(a + sy);
+        if (m & 16u) f(a - sz);
+        if (m & 32u) f(a + sz);
     }
-    __device__ double diag(int64_t a) const {
-        int deg = 0;
-        for_springs(a, [&](int64_t) { ++deg; });
-        return (double)deg + SPRING_TIKHONOV;
-    }
+    __device__ double diag(int64_t a) const { return (double)__popc(code[a]) + SPRING_TIKHONOV; }
     __device__ void inv_diag(int64_t a, double d[3]) const {
         const double dg = diag(a);
         for (int c = 0; c < 3; ++c) d[c] = (unknown[a] >> c) & 1 ? 1.0 / dg : 0.0;
     }
+    // A node known in column c holds zero in that column of every vector the solve applies this to:
+    // the right-hand side and the start are zero there, so the residual, the search direction and
+    // the iterate stay zero. The coupling to a neighbour therefore needs no test of its own bits.
     template <class X>
     __device__ void apply(int64_t a, const X& x, double y[3]) const {
         const unsigned u = unknown[a];
@@ -91,12 +106,125 @@ struct SpringOp {
         const double dg = diag(a);
         for (int c = 0; c < 3; ++c) y[c] = (u >> c) & 1 ? dg * x(a, c) : 0.0;
         for_springs(a, [&](int64_t q) {
-            const unsigned uq = unknown[q] & u;
             for (int c = 0; c < 3; ++c)
-                if ((uq >> c) & 1) y[c] -= x(q, c);
+                if ((u >> c) & 1) y[c] -= x(q, c);
         });
     }
 };
+
+__global__ void k_spring_codes(NodeGrid g, const uint8_t* __restrict__ edge_ok, uint8_t* __restrict__ code,
+                               const int* __restrict__ skip) {
+    if (skip && *skip) return;
+    const int64_t N = g.n();
+    for (int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; a < N; a += (int64_t)gridDim.x * blockDim.x)
+        code[a] = (uint8_t)spring_code(g, edge_ok, a);
+}
+
+// ---------------------------------------------------------------------------- compact fill
+// The fill solved over its unknown nodes only: entry i of the compact vectors is node list[i] (the
+// nodes with any unknown bit, ascending), nbr[6 i + k] the entry of its k-th spring neighbour in the
+// order of spring_code (-1: no spring, or a neighbour known in every column) and meta[i] its
+// unknown bits | spring count << 3. The iteration is the one over all nodes: a neighbour known in
+// a column holds zero there, so leaving it out changes no sum.
+struct CompactSpringOp {
+    const uint8_t* meta;
+    const int* nbr;
+
+    __device__ void inv_diag(int64_t i, double d[3]) const {
+        const unsigned m = meta[i];
+        const double dg = (double)(m >> 3) + SPRING_TIKHONOV;
+        for (int c = 0; c < 3; ++c) d[c] = (m >> c) & 1 ? 1.0 / dg : 0.0;
+    }
+    template <class X>
+    __device__ void apply(int64_t i, const X& x, double y[3]) const {
+        const unsigned m = meta[i];
+        const double dg = (double)(m >> 3) + SPRING_TIKHONOV;
+        for (int c = 0; c < 3; ++c) y[c] = (m >> c) & 1 ? dg * x(i, c) : 0.0;
+        for (int k = 0; k < 6; ++k) {
+            const int j = nbr[6 * i + k];
+            if (j >= 0)
+                for (int c = 0; c < 3; ++c)
+                    if ((m >> c) & 1) y[c] -= x(j, c);
+        }
+    }
+};
+
+constexpr int LIST_THREADS = 1024;
+
+// list, pos_of (entry of each listed node) and *count in one workgroup: each thread scans a
+// contiguous run of nodes (4-byte loads of the unknown bytes: the run starts on a multiple of 4
+// and the unknown array is padded to 256 bytes), the runs' counts are scanned in the block, then
+// each thread writes its run's entries in order.
+__global__ __launch_bounds__(LIST_THREADS) void k_fill_list(int64_t N, const uint8_t* __restrict__ unknown,
+                                                            int* __restrict__ list, int* __restrict__ pos_of,
+                                                            int* __restrict__ count, const int* __restrict__ skip) {
+    if (skip && *skip) return;
+    __shared__ int scan[LIST_THREADS];
+    const int t = threadIdx.x;
+    const int64_t run = ((N + LIST_THREADS - 1) / LIST_THREADS + 3) / 4 * 4;
+    const int64_t lo = (int64_t)t * run, hi = lo + run < N ? lo + run : N;
+    int n = 0;
+    for (int64_t a = lo; a < hi; a += 4) {
+        const uint32_t w = *reinterpret_cast<const uint32_t*>(unknown + a);
+        for (int k = 0; k < 4; ++k) n += (a + k < hi && ((w >> (8 * k)) & 0xffu)) ? 1 : 0;
+    }
+    scan[t] = n;
+    __syncthreads();
+    for (int off = 1; off < LIST_THREADS; off <<= 1) {  // inclusive scan (Hillis-Steele)
+        const int v = t >= off ? scan[t - off] : 0;
+        __syncthreads();
+        scan[t] += v;
+        __syncthreads();
+    }
+    int pos = scan[t] - n;
+    for (int64_t a = lo; a < hi; a += 4) {
+        const uint32_t w = *reinterpret_cast<const uint32_t*>(unknown + a);
+        for (int k = 0; k < 4; ++k)
+            if (a + k < hi && ((w >> (8 * k)) & 0xffu)) {
+                list[pos] = (int)(a + k);
+                pos_of[a + k] = pos;
+                ++pos;
+            }
+    }
+    if (t == LIST_THREADS - 1) *count = scan[t];
+}
+
+// the compact operator's tables and right-hand side (b of the listed nodes)
+__global__ void k_fill_tables(SpringOp op, const int* __restrict__ list, const int* __restrict__ pos_of,
+                              const int* __restrict__ count, const double* __restrict__ b_full,
+                              uint8_t* __restrict__ meta, int* __restrict__ nbr, double* __restrict__ bc,
+                              double* __restrict__ Xc, const int* __restrict__ skip) {
+    if (skip && *skip) return;
+    const int M = *count;
+    const int64_t sy = op.g.nx, sz = (int64_t)op.g.nx * op.g.ny;
+    const int64_t step[6] = {-1, 1, -sy, sy, -sz, sz};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < M; i += gridDim.x * blockDim.x) {
+        const int64_t a = list[i];
+        const unsigned code = op.code[a];
+        meta[i] = (uint8_t)(op.unknown[a] | (__popc(code) << 3));
+        for (int k = 0; k < 6; ++k) {
+            int j = -1;
+            if ((code >> k) & 1) {
+                const int64_t q = a + step[k];
+                if (op.unknown[q]) j = pos_of[q];
+            }
+            nbr[6 * i + k] = j;
+        }
+        for (int c = 0; c < 3; ++c) {
+            bc[3 * i + c] = b_full[3 * a + c];
+            Xc[3 * i + c] = 0.0;
+        }
+    }
+}
+
+// the compact solution back into the all-node iterate (zero at the nodes not listed)
+__global__ void k_fill_scatter(const int* __restrict__ list, const int* __restrict__ count,
+                               const double* __restrict__ Xc, double* __restrict__ X, const int* __restrict__ skip) {
+    if (skip && *skip) return;
+    const int M = *count;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < M; i += gridDim.x * blockDim.x)
+        for (int c = 0; c < 3; ++c) X[3 * (int64_t)list[i] + c] = Xc[3 * i + c];
+}
 
 // bad = status != CONVERGED; unknown bits (not good, or NaN); per-block counts of good nodes and
 // of known nodes per column
@@ -161,18 +289,36 @@ __global__ void k_spring_rhs(SpringOp op, const double* __restrict__ vals, doubl
 }
 
 // ---------------------------------------------------------------------------- median test
-// Values of ranks r0 and r1 (0-based) among the entries of v with ok set; registers only.
+// The entries of v with ok set in ascending order, the others (+inf) after them: a bitonic network
+// over 32 slots with every index fixed at compile time, so the values stay in registers.
 template <int M>
-__device__ inline void select2(const double (&v)[M], const bool (&ok)[M], int r0, int r1, double& o0, double& o1) {
-    o0 = o1 = __builtin_nan("");
+__device__ inline void sort32(const double (&v)[M], const bool (&ok)[M], double (&s)[32]) {
+    static_assert(M <= 32, "at most 32 values");
 #pragma unroll
-    for (int i = 0; i < M; ++i) {
-        int rank = 0;
+    for (int k = 0; k < 32; ++k) s[k] = (k < M && ok[k < M ? k : 0]) ? v[k < M ? k : 0] : __builtin_inf();
 #pragma unroll
-        for (int j = 0; j < M; ++j) rank += (ok[j] && (v[j] < v[i] || (v[j] == v[i] && j < i))) ? 1 : 0;
-        if (ok[i] && rank == r0) o0 = v[i];
-        if (ok[i] && rank == r1) o1 = v[i];
-    }
+    for (int size = 2; size <= 32; size <<= 1)
+#pragma unroll
+        for (int stride = size / 2; stride > 0; stride >>= 1)
+#pragma unroll
+            for (int i = 0; i < 32; ++i) {
+                const int j = i ^ stride;
+                if (j > i) {
+                    const double lo = fmin(s[i], s[j]), hi = fmax(s[i], s[j]);
+                    const bool up = (i & size) == 0;
+                    s[i] = up ? lo : hi;
+                    s[j] = up ? hi : lo;
+                }
+            }
+}
+
+// s[r] (NaN when r is not below n)
+__device__ inline double pick32(const double (&s)[32], int r, int n) {
+    double o = __builtin_nan("");
+#pragma unroll
+    for (int k = 0; k < 32; ++k)
+        if (k == r && r < n) o = s[k];
+    return o;
 }
 
 // Move by d (+1 or -1) along axis ax from p through an ok edge, inside the grid.
@@ -241,13 +387,11 @@ __device__ inline double median26(const double (&v)[26], const bool (&ok)[26], i
     n = 0;
 #pragma unroll
     for (int k = 0; k < 26; ++k) n += ok[k] ? 1 : 0;
-    double lo, hi;
-    if (!CUT) {
-        select2<26>(v, ok, 13, 13, lo, hi);
-        return lo;
-    }
+    double s[32];
+    sort32<26>(v, ok, s);
+    if (!CUT) return pick32(s, 13, n);
     if (n == 0) return __builtin_nan("");
-    select2<26>(v, ok, (n - 1) / 2, n / 2, lo, hi);
+    const double lo = pick32(s, (n - 1) / 2, n), hi = pick32(s, n / 2, n);
     return (n & 1) ? lo : (lo + hi) / 2.0;
 }
 
@@ -409,36 +553,58 @@ struct Carver {
     }
 };
 
+// The compact fill's tables and vectors, sized for every node unknown.
+struct CompactFill {
+    int *list, *pos_of, *nbr, *count;
+    uint8_t* meta;
+    double *bc, *Xc;
+    void take(Carver& c, int64_t N) {
+        list = c.take<int>(N);
+        pos_of = c.take<int>(N);
+        nbr = c.take<int>(6 * N);
+        meta = c.take<uint8_t>(N);
+        bc = c.take<double>(3 * N);
+        Xc = c.take<double>(3 * N);
+        count = c.take<int>(1);
+    }
+};
+
 struct CleanupWs {
-    uint8_t *unknown, *cnt;
+    uint8_t *unknown, *code, *cnt;
     double *absf, *b, *X, *part;
     MedCtl* ctl;
+    CompactFill cf;
     void* pcg_ws;
     size_t bytes;
     CleanupWs(void* ws, int64_t N) {
         Carver c{static_cast<char*>(ws)};
         unknown = c.take<uint8_t>(N);
+        code = c.take<uint8_t>(N);
         cnt = c.take<uint8_t>(3 * N);
         absf = c.take<double>(3 * N);
         b = c.take<double>(3 * N);
         X = c.take<double>(3 * N);
         part = c.take<double>(4 * pcg::MAX_BLOCKS);
         ctl = c.take<MedCtl>(1);
+        cf.take(c, N);
         pcg_ws = c.take<char>(pcg::workspace_bytes(N));
         bytes = c.used;
     }
 };
 
 struct FillWs {
-    uint8_t* unknown;
+    uint8_t *unknown, *code;
     double *b, *X;
+    CompactFill cf;
     void* pcg_ws;
     size_t bytes;
     FillWs(void* ws, int64_t N) {
         Carver c{static_cast<char*>(ws)};
         unknown = c.take<uint8_t>(N);
+        code = c.take<uint8_t>(N);
         b = c.take<double>(3 * N);
         X = c.take<double>(3 * N);
+        cf.take(c, N);
         pcg_ws = c.take<char>(pcg::workspace_bytes(N));
         bytes = c.used;
     }
@@ -446,7 +612,29 @@ struct FillWs {
 
 int check_node_grid(int nx, int ny, int nz) {
     if (nx < 1 || ny < 1 || nz < 1) return fail(-2, "empty node grid");
+    if ((int64_t)nx * ny * nz > INT32_MAX) return fail(-2, "more than 2^31 - 1 nodes");
     return 0;
+}
+
+// Spring fill of b (all nodes) into X (all nodes, zero on entry) over the unknown nodes only:
+// list them, build the compact tables, solve, scatter back. `skip` (nullable, device) switches the
+// whole fill off; the solve's arguments are those of pcg::solve.
+hipError_t fill_solve(const SpringOp& op, int64_t N, const double* b, double* X, const CompactFill& cf, double tol,
+                      int max_iter, void* pcg_ws, const int* skip, int sync_every, int* iterations_host,
+                      int* iterations_dev, int* capped, hipStream_t st) {
+    constexpr unsigned COMPACT_BLOCKS = 128;
+    const unsigned nbc = pcg::blocks_for(N) < COMPACT_BLOCKS ? pcg::blocks_for(N) : COMPACT_BLOCKS;
+    hipLaunchKernelGGL(k_fill_list, dim3(1), dim3(LIST_THREADS), 0, st, N, op.unknown, cf.list, cf.pos_of, cf.count,
+                       skip);
+    hipLaunchKernelGGL(k_fill_tables, dim3(nbc), dim3(ABLOCK), 0, st, op, (const int*)cf.list,
+                       (const int*)cf.pos_of, (const int*)cf.count, b, cf.meta, cf.nbr, cf.bc, cf.Xc, skip);
+    const CompactSpringOp cop{cf.meta, cf.nbr};
+    hipError_t e = pcg::solve(cop, N, (const double*)cf.bc, cf.Xc, tol, max_iter, pcg_ws, skip, sync_every,
+                              iterations_host, iterations_dev, st, capped, (const int*)cf.count, nbc);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_fill_scatter, dim3(nbc), dim3(ABLOCK), 0, st, (const int*)cf.list, (const int*)cf.count,
+                       (const double*)cf.Xc, X, skip);
+    return hipGetLastError();
 }
 
 }  // namespace
@@ -476,15 +664,17 @@ int aldvc_subpb1_cleanup(int nx, int ny, int nz, const uint8_t* edge_ok, const i
     MedCtl* ctl = w.ctl;
     void* pcg_ws = w.pcg_ws;
     const unsigned nb = pcg::blocks_for(N);
-    const SpringOp op{g, unknown, nullptr};  // the median test fills without cut edges (outlier_detection.py:127)
+    const SpringOp op{g, unknown, w.code};  // the median test fills without cut edges (outlier_detection.py:127)
 
     hipLaunchKernelGGL(k_cleanup_setup, dim3(nb), dim3(ABLOCK), 0, stream, g, status, U, unknown, part, stop);
     hipLaunchKernelGGL(k_cleanup_switches, dim3(1), dim3(ABLOCK), 0, stream, (const double*)part, (int)nb, threshold,
                        ctl, stop);
     if (!edge_ok) {
+        hipLaunchKernelGGL(k_spring_codes, dim3(nb), dim3(ABLOCK), 0, stream, g, (const uint8_t*)nullptr, w.code,
+                           (const int*)&ctl->skip);
         hipLaunchKernelGGL(k_spring_rhs, dim3(nb), dim3(ABLOCK), 0, stream, op, U, b, X, (const int*)&ctl->skip);
-        if (pcg::solve(op, N, (const double*)b, X, fill_tol, fill_max_iter, pcg_ws, &ctl->skip, sync_every, nullptr,
-                       fill_iterations, stream, capped) != hipSuccess)
+        if (fill_solve(op, N, b, X, w.cf, fill_tol, fill_max_iter, pcg_ws, &ctl->skip, sync_every, nullptr,
+                       fill_iterations, capped, stream) != hipSuccess)
             return fail(-5, "the cleanup's spring-fill solve failed");
         hipLaunchKernelGGL(k_median_pass1<false>, dim3(nb), dim3(ABLOCK), 0, stream, g, edge_ok,
                            (const uint8_t*)unknown, U, (const double*)X, absf, cnt, (const MedCtl*)ctl);
@@ -519,12 +709,13 @@ int aldvc_spring_fill(int nx, int ny, int nz, const uint8_t* edge_ok, double* va
     double *b = w.b, *X = w.X;
     void* pcg_ws = w.pcg_ws;
     const unsigned nb = pcg::blocks_for(N);
-    const SpringOp op{g, unknown, edge_ok};
+    const SpringOp op{g, unknown, w.code};
+    hipLaunchKernelGGL(k_spring_codes, dim3(nb), dim3(ABLOCK), 0, stream, g, edge_ok, w.code, (const int*)nullptr);
     hipLaunchKernelGGL(k_unknown_from_nan, dim3(nb), dim3(ABLOCK), 0, stream, N, (const double*)vals, unknown);
     hipLaunchKernelGGL(k_spring_rhs, dim3(nb), dim3(ABLOCK), 0, stream, op, (const double*)vals, b, X,
                        (const int*)nullptr);
     *iterations = 0;
-    if (pcg::solve(op, N, (const double*)b, X, tol, max_iter, pcg_ws, nullptr, 8, iterations, nullptr, stream) !=
+    if (fill_solve(op, N, b, X, w.cf, tol, max_iter, pcg_ws, nullptr, 8, iterations, nullptr, nullptr, stream) !=
         hipSuccess)
         return fail(-5, "the spring-fill solve failed");
     hipLaunchKernelGGL(k_fill_merge, dim3(nb), dim3(ABLOCK), 0, stream, N, (const uint8_t*)unknown, (const double*)X,

@@ -48,16 +48,30 @@ inline unsigned blocks_for(int64_t n) {
     return (unsigned)(b < MAX_BLOCKS ? (b > 0 ? b : 1) : MAX_BLOCKS);
 }
 
-// Sum of K doubles per thread over the block, in a fixed tree order; the total is in red[k][0].
+// Sum over the 64 lanes of a wave by an xor butterfly: partners add the same two values, so every
+// lane ends with the same bits.
+__device__ inline double wave_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Sum of K doubles per thread over the block in a fixed order (each wave by the butterfly, then the
+// waves in order); the total is in red[k][0].
 template <int K>
 __device__ inline void block_sum(const double v[K], double (*red)[BLOCK]) {
-    for (int k = 0; k < K; ++k) red[k][threadIdx.x] = v[k];
+    constexpr int NW = BLOCK / 64;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double w[K];
+    for (int k = 0; k < K; ++k) w[k] = wave_sum(v[k]);
+    if (lane == 0)
+        for (int k = 0; k < K; ++k) red[k][1 + wave] = w[k];
     __syncthreads();
-    for (int off = BLOCK / 2; off > 0; off >>= 1) {
-        if ((int)threadIdx.x < off)
-            for (int k = 0; k < K; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + off];
-        __syncthreads();
+    if ((int)threadIdx.x < K) {
+        double t = red[threadIdx.x][1];
+        for (int i = 1; i < NW; ++i) t += red[threadIdx.x][1 + i];
+        red[threadIdx.x][0] = t;
     }
+    __syncthreads();
 }
 
 // Per-block partials of K sums: block `blk` writes part[K * blk + k].
@@ -90,12 +104,14 @@ static __global__ void k_reset(Ctl* ctl, const int* gate) {
 
 // R = B - A X, Z = D^-1 R, P_old = 0; partials of <R, Z> and <B, B>
 template <class Op>
-__global__ __launch_bounds__(BLOCK) void k_setup(Op op, int64_t N, const double* __restrict__ B,
+__global__ __launch_bounds__(BLOCK) void k_setup(Op op, int64_t N, const int* __restrict__ n_dev,
+                                                 const double* __restrict__ B,
                                                  const double* __restrict__ X, double* __restrict__ R,
                                                  double* __restrict__ Z, double* __restrict__ Pold,
                                                  double* __restrict__ inv_diag, double* __restrict__ part2,
                                                  const Ctl* ctl) {
     if (ctl->done) return;
+    if (n_dev) N = *n_dev;
     __shared__ double red[6][BLOCK];
     double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     auto xv = [X](int64_t b, int c) { return X[3 * b + c]; };
@@ -127,11 +143,13 @@ __device__ inline bool converged(const double t[6], const Ctl* ctl, double tol) 
 
 // iteration slot k: the stopping test of iteration k, then P = Z + beta P_old, AP = A P
 template <class Op>
-__global__ __launch_bounds__(BLOCK) void k_iter1(Op op, int64_t N, int k, const double* __restrict__ Z,
+__global__ __launch_bounds__(BLOCK) void k_iter1(Op op, int64_t N, const int* __restrict__ n_dev, int k,
+                                                 const double* __restrict__ Z,
                                                  const double* __restrict__ Pold, double* __restrict__ Pnew,
                                                  double* __restrict__ AP, const double* __restrict__ part2,
                                                  double* __restrict__ part1, Ctl* ctl, double tol) {
     if (ctl->done) return;
+    if (n_dev) N = *n_dev;
     __shared__ double red[6][BLOCK];
     double t[6];
     total_partials<6>(part2, gridDim.x, t, red);
@@ -173,13 +191,15 @@ __global__ __launch_bounds__(BLOCK) void k_iter1(Op op, int64_t N, int k, const 
 }
 
 // iteration slot k: alpha = <r, z> / <P, AP>; X += alpha P, R -= alpha AP, Z = D^-1 R
-static __global__ __launch_bounds__(BLOCK) void k_iter2(int64_t N, int k, const double* __restrict__ P,
+static __global__ __launch_bounds__(BLOCK) void k_iter2(int64_t N, const int* __restrict__ n_dev, int k,
+                                                        const double* __restrict__ P,
                                                         const double* __restrict__ AP,
                                                         const double* __restrict__ inv_diag, double* __restrict__ X,
                                                         double* __restrict__ R, double* __restrict__ Z,
                                                         const double* __restrict__ part1, double* __restrict__ part2,
                                                         const Ctl* ctl) {
     if (ctl->done) return;
+    if (n_dev) N = *n_dev;
     __shared__ double red[6][BLOCK];
     double t[3];
     total_partials<3>(part1, gridDim.x, t, red);
@@ -248,22 +268,24 @@ inline Workspace carve(void* ws, int64_t N) {
 // iteration count is returned in *iterations_host); with 0 all `max_iter` iteration slots are
 // enqueued (those after convergence return at once) and nothing is read back, so the sequence can
 // be captured into a graph; *iterations_dev, when given, receives the count on the device.
-// `gate` (nullable, device) switches the whole solve off when nonzero.
+// `gate` (nullable, device) switches the whole solve off when nonzero. `n_dev` (nullable, device)
+// holds the number of nodes when only the device knows it (at most N, which sizes the workspace);
+// `blocks` (0: as many as N calls for) sets the grid of such a solve.
 template <class Op>
 inline hipError_t solve(const Op& op, int64_t N, const double* B, double* X, double tol, int max_iter, void* ws,
                         const int* gate, int sync_every, int* iterations_host, int* iterations_dev,
-                        hipStream_t st, int* capped = nullptr) {
+                        hipStream_t st, int* capped = nullptr, const int* n_dev = nullptr, unsigned blocks = 0) {
     Workspace w = carve(ws, N);
-    const unsigned nb = blocks_for(N);
+    const unsigned nb = blocks ? (blocks < (unsigned)MAX_BLOCKS ? blocks : (unsigned)MAX_BLOCKS) : blocks_for(N);
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, w.ctl, gate);
-    hipLaunchKernelGGL(k_setup<Op>, dim3(nb), dim3(BLOCK), 0, st, op, N, B, (const double*)X, w.R, w.Z, w.P[1],
+    hipLaunchKernelGGL(k_setup<Op>, dim3(nb), dim3(BLOCK), 0, st, op, N, n_dev, B, (const double*)X, w.R, w.Z, w.P[1],
                        w.inv_diag, w.part2, (const Ctl*)w.ctl);
     for (int k = 0; k < max_iter; ++k) {
         double* Pold = w.P[(k + 1) & 1];
         double* Pnew = w.P[k & 1];
-        hipLaunchKernelGGL(k_iter1<Op>, dim3(nb), dim3(BLOCK), 0, st, op, N, k, (const double*)w.Z,
+        hipLaunchKernelGGL(k_iter1<Op>, dim3(nb), dim3(BLOCK), 0, st, op, N, n_dev, k, (const double*)w.Z,
                            (const double*)Pold, Pnew, w.AP, (const double*)w.part2, w.part1, w.ctl, tol);
-        hipLaunchKernelGGL(k_iter2, dim3(nb), dim3(BLOCK), 0, st, N, k, (const double*)Pnew, (const double*)w.AP,
+        hipLaunchKernelGGL(k_iter2, dim3(nb), dim3(BLOCK), 0, st, N, n_dev, k, (const double*)Pnew, (const double*)w.AP,
                            (const double*)w.inv_diag, X, w.R, w.Z, (const double*)w.part1, w.part2,
                            (const Ctl*)w.ctl);
         if (sync_every > 0 && (k + 1) % sync_every == 0 && k + 1 < max_iter) {

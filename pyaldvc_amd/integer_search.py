@@ -98,18 +98,16 @@ def search(fd: torch.Tensor, gd: torch.Tensor, coords: torch.Tensor, half, radiu
     extent = [2 * int(q) + 1 + 2 * int(p) for q, p in zip(half, res.radius)]  # window extent (x, y, z)
     if n == 0 or any(e > d for e, d in zip(extent, (nx, ny, nz))):
         return res
-    h = torch.tensor(half, dtype=torch.int64, device=dev)
-    r = torch.tensor(res.radius, dtype=torch.int64, device=dev)
-    span = torch.tensor(extent, dtype=torch.int64, device=dev)
-    dims = torch.tensor([nx, ny, nz], dtype=torch.int64, device=dev)
     vx, vy, vz = (2 * int(q) + 1 for q in res.radius)
+    # the per-axis constants (x, y, z) in one upload
+    consts = [int(q) for q in half] + list(res.radius) + extent + [nx, ny, nz, vx - 1, vy - 1, vz - 1]
+    h, r, span, dims, edge_hi = torch.tensor(consts, dtype=torch.int64, device=dev).reshape(5, 3)
     start = coords - h - r + (0 if shift is None else shift)
     win0 = torch.minimum(torch.clamp(start, min=0), dims - span)
     # faces the clamp left alone: a peak there may lie beyond the window
     free = torch.cat([start >= 0, start <= dims - span], 1)
     t_ok = ((coords - h >= 0) & (coords + h < dims)).all(1).to(torch.uint8)
     rows = max(1, min(n, MAP_BYTES_PER_CHUNK // (4 * vx * vy * vz)))
-    edge_hi = torch.tensor([vx - 1, vy - 1, vz - 1], dtype=torch.int64, device=dev)
     for a in range(0, n, rows):
         b = min(n, a + rows)
         c = coords[a:b].contiguous()
