@@ -152,7 +152,7 @@ struct CompactSpringOp {
 constexpr int LIST_THREADS = 1024;
 
 // list, pos_of (entry of each listed node) and *count in one workgroup: each thread scans a
-// contiguous run of nodes (4-byte loads of the unknown bytes: the run starts on a multiple of 4
+// contiguous run of nodes (16-byte loads of the unknown bytes: the run starts on a multiple of 16
 // and the unknown array is padded to 256 bytes), the runs' counts are scanned in the block, then
 // each thread writes its run's entries in order.
 __global__ __launch_bounds__(LIST_THREADS) void k_fill_list(int64_t N, const uint8_t* __restrict__ unknown,
@@ -161,12 +161,14 @@ __global__ __launch_bounds__(LIST_THREADS) void k_fill_list(int64_t N, const uin
     if (skip && *skip) return;
     __shared__ int scan[LIST_THREADS];
     const int t = threadIdx.x;
-    const int64_t run = ((N + LIST_THREADS - 1) / LIST_THREADS + 3) / 4 * 4;
+    const int64_t run = ((N + LIST_THREADS - 1) / LIST_THREADS + 15) / 16 * 16;
     const int64_t lo = (int64_t)t * run, hi = lo + run < N ? lo + run : N;
     int n = 0;
-    for (int64_t a = lo; a < hi; a += 4) {
-        const uint32_t w = *reinterpret_cast<const uint32_t*>(unknown + a);
-        for (int k = 0; k < 4; ++k) n += (a + k < hi && ((w >> (8 * k)) & 0xffu)) ? 1 : 0;
+    for (int64_t a = lo; a < hi; a += 16) {
+        const uint4 w = *reinterpret_cast<const uint4*>(unknown + a);
+        const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+        for (int k = 0; k < 16; ++k) n += (a + k < hi && ((ws[k >> 2] >> (8 * (k & 3))) & 0xffu)) ? 1 : 0;
     }
     scan[t] = n;
     __syncthreads();
@@ -177,10 +179,13 @@ __global__ __launch_bounds__(LIST_THREADS) void k_fill_list(int64_t N, const uin
         __syncthreads();
     }
     int pos = scan[t] - n;
-    for (int64_t a = lo; a < hi; a += 4) {
-        const uint32_t w = *reinterpret_cast<const uint32_t*>(unknown + a);
-        for (int k = 0; k < 4; ++k)
-            if (a + k < hi && ((w >> (8 * k)) & 0xffu)) {
+    for (int64_t a = lo; a < hi; a += 16) {
+        const uint4 w = *reinterpret_cast<const uint4*>(unknown + a);
+        const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+        if (!(ws[0] | ws[1] | ws[2] | ws[3])) continue;
+#pragma unroll
+        for (int k = 0; k < 16; ++k)
+            if (a + k < hi && ((ws[k >> 2] >> (8 * (k & 3))) & 0xffu)) {
                 list[pos] = (int)(a + k);
                 pos_of[a + k] = pos;
                 ++pos;
@@ -214,6 +219,195 @@ __global__ void k_fill_tables(SpringOp op, const int* __restrict__ list, const i
             bc[3 * i + c] = b_full[3 * a + c];
             Xc[3 * i + c] = 0.0;
         }
+    }
+}
+
+// ---------------------------------------------------------------------------- one-workgroup fill
+// A fill of at most SMALL_MAX unknown nodes is solved in one launch of three workgroups, one per
+// column, each an independent Jacobi-preconditioned CG over the compact operator's column (the
+// reference fills every column by a solve of its own: utils/inpaint.py:35-121). The column's
+// search direction lives in LDS; each thread keeps the iterate, residual, A P and the tables of its
+// SMALL_EPT entries (entry i = thread + SMALL_THREADS k) in registers. Scalars and the stopping
+// rule (relative residual below tol, tested after every update but the last) are pcg::solve's,
+// per column; the block sums run in a fixed order, so every thread derives the same scalars.
+constexpr int SMALL_THREADS = 1024;
+constexpr int SMALL_EPT = 7;
+constexpr int SMALL_MAX = SMALL_THREADS * SMALL_EPT;  // 7168 entries: 56 KiB of LDS per vector
+constexpr int SMALL_WAVES = SMALL_THREADS / 64;
+constexpr unsigned SMALL_NONE = 0xffffu;  // no neighbour entry
+
+struct SmallCtl {
+    int gate;           // 1: the all-node solve is not needed (fill skipped, or solved here)
+    int solved;         // 1: solved here
+    int iterations[3];  // per column, of the solve made here
+    int capped[3];      // per column: it took max_iter iterations
+};
+
+// totals of K sums over the workgroup: each wave by the butterfly, then the waves in order
+template <int K>
+__device__ inline void small_sum(const double v[K], double (*red)[SMALL_WAVES], double out[K]) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double w[K];
+    for (int k = 0; k < K; ++k) w[k] = pcg::wave_sum(v[k]);
+    if (lane == 0)
+        for (int k = 0; k < K; ++k) red[k][wave] = w[k];
+    __syncthreads();
+    for (int k = 0; k < K; ++k) {
+        double t = red[k][0];
+        for (int i = 1; i < SMALL_WAVES; ++i) t += red[k][i];
+        out[k] = t;
+    }
+}
+
+__global__ __launch_bounds__(SMALL_THREADS) void k_fill_small(const int* __restrict__ count,
+                                                              const uint8_t* __restrict__ meta,
+                                                              const int* __restrict__ nbr,
+                                                              const double* __restrict__ bc, double* __restrict__ Xc,
+                                                              double tol, int max_iter, const int* __restrict__ skip,
+                                                              SmallCtl* sc) {
+    const int t = threadIdx.x, col = blockIdx.x;
+    const bool lead = t == 0 && col == 0;
+    if (skip && *skip) {
+        if (lead) {
+            sc->gate = 1;
+            sc->solved = 0;
+        }
+        return;
+    }
+    const int M = *count;
+    if (M > SMALL_MAX) {
+        if (lead) {
+            sc->gate = 0;
+            sc->solved = 0;
+        }
+        return;
+    }
+    __shared__ double Ps[SMALL_MAX], Xs[SMALL_MAX];  // search direction and iterate of the column
+    __shared__ double redA[1][SMALL_WAVES], redB[2][SMALL_WAVES];
+    __shared__ double dg_of[8], dinv_of[8];  // diagonal and its inverse by spring count
+    unsigned nb[SMALL_EPT][3], mt[SMALL_EPT];  // neighbour entries (16 bits each), meta
+    double R[SMALL_EPT], AP[SMALL_EPT];
+    if (t < 8) {
+        dg_of[t] = (double)t + SPRING_TIKHONOV;
+        dinv_of[t] = 1.0 / dg_of[t];
+    }
+    __syncthreads();
+    // the Jacobi inverse of entry e: zero where the column is known
+    auto dinv = [&](int e) { return (mt[e] >> col) & 1 ? dinv_of[mt[e] >> 3] : 0.0; };
+    double s2[2] = {0.0, 0.0};
+#pragma unroll
+    for (int e = 0; e < SMALL_EPT; ++e) {
+        const int i = t + SMALL_THREADS * e;
+        nb[e][0] = nb[e][1] = nb[e][2] = 0xffffffffu;
+        mt[e] = 0;
+        R[e] = AP[e] = 0.0;
+        if (i < M) {
+            mt[e] = meta[i];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                const int j = nbr[6 * i + k];
+                const unsigned v = j >= 0 ? (unsigned)j : SMALL_NONE;
+                nb[e][k >> 1] = (nb[e][k >> 1] & ~(0xffffu << (16 * (k & 1)))) | (v << (16 * (k & 1)));
+            }
+            const double b = bc[3 * i + col];
+            const double z = dinv(e) * b;
+            R[e] = b;
+            Ps[i] = z;
+            Xs[i] = 0.0;
+            s2[0] += b * z;
+            s2[1] += b * b;
+        }
+    }
+    double tot[2];
+    small_sum<2>(s2, redB, tot);  // its barrier also publishes P
+    double rz = tot[0];
+    const double bnorm = tot[1] > 0.0 ? sqrt(tot[1]) : 1.0;  // a zero column counts as 1
+    int iterations = max_iter, capped = 1;
+    for (int k = 0; k < max_iter; ++k) {
+        // the tables count as rewritten here, so the compiler derives the LDS addresses inside the
+        // loop instead of holding them in registers across it
+#pragma unroll
+        for (int e = 0; e < SMALL_EPT; ++e)
+            asm volatile("" : "+v"(nb[e][0]), "+v"(nb[e][1]), "+v"(nb[e][2]), "+v"(mt[e]));
+        // A P (the compact operator's terms in its order) and <P, A P>
+        double s1[1] = {0.0};
+#pragma unroll
+        for (int e = 0; e < SMALL_EPT; ++e) {
+            const int i = t + SMALL_THREADS * e;
+            if (i < M) {
+                const bool on = (mt[e] >> col) & 1;
+                const double p = Ps[i];
+                double y = on ? dg_of[mt[e] >> 3] * p : 0.0;
+#pragma unroll
+                for (int q = 0; q < 6; ++q) {
+                    const unsigned j = (nb[e][q >> 1] >> (16 * (q & 1))) & 0xffffu;
+                    if (j != SMALL_NONE && on) y -= Ps[j];
+                }
+                AP[e] = y;
+                s1[0] += p * y;
+            }
+        }
+        double pap[1];
+        small_sum<1>(s1, redA, pap);
+        const double al = pap[0] == 0.0 ? 0.0 : rz / pap[0];
+        s2[0] = s2[1] = 0.0;
+#pragma unroll
+        for (int e = 0; e < SMALL_EPT; ++e) {
+            const int i = t + SMALL_THREADS * e;
+            if (i < M) {
+                Xs[i] += Ps[i] * al;
+                const double r = R[e] - AP[e] * al;
+                R[e] = r;
+                const double z = dinv(e) * r;
+                s2[0] += r * z;
+                s2[1] += r * r;
+            }
+        }
+        small_sum<2>(s2, redB, tot);
+        if (k + 1 == max_iter) break;  // the last slot: max_iter iterations, whatever the residual
+        if (sqrt(tot[1]) / bnorm < tol) {
+            iterations = k + 1;
+            capped = 0;
+            break;
+        }
+        const double be = rz == 0.0 ? 0.0 : tot[0] / rz;
+        rz = tot[0];
+#pragma unroll
+        for (int e = 0; e < SMALL_EPT; ++e) {
+            const int i = t + SMALL_THREADS * e;
+            if (i < M) Ps[i] = dinv(e) * R[e] + Ps[i] * be;
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int e = 0; e < SMALL_EPT; ++e) {
+        const int i = t + SMALL_THREADS * e;
+        if (i < M) Xc[3 * i + col] = Xs[i];
+    }
+    if (t == 0) {
+        sc->iterations[col] = iterations;
+        sc->capped[col] = capped;
+    }
+    if (lead) {
+        sc->gate = 1;
+        sc->solved = 1;
+    }
+}
+
+// the fill's iteration count (the largest over the columns) and capped flag from whichever solve ran
+__global__ void k_fill_result(const SmallCtl* sc, const int* __restrict__ all_node_iterations, int* __restrict__ out,
+                              int* __restrict__ capped) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (sc->solved) {
+        int it = 0, cap = 0;
+        for (int c = 0; c < 3; ++c) {
+            it = sc->iterations[c] > it ? sc->iterations[c] : it;
+            cap |= sc->capped[c];
+        }
+        *out = it;
+        if (capped && cap) *capped = 1;
+    } else {
+        *out = *all_node_iterations;
     }
 }
 
@@ -542,9 +736,15 @@ __global__ void k_fill_merge(int64_t N, const uint8_t* __restrict__ unknown, con
 
 // Carves consecutive 256-byte aligned regions out of a workspace; with a null base it only
 // measures, so the size query and the carving follow the same sequence.
+//
+// The first region starts at the first 256-byte boundary of the workspace (the measured size
+// counts the 255 bytes that may take), so every region is aligned for 16-byte loads.
 struct Carver {
     char* base;
-    size_t used = 0;
+    size_t used;
+    explicit Carver(char* b)
+        : base(b ? reinterpret_cast<char*>((reinterpret_cast<uintptr_t>(b) + 255) & ~uintptr_t(255)) : nullptr),
+          used(b ? 0 : 255) {}
     template <class T>
     T* take(size_t count) {
         T* p = base ? reinterpret_cast<T*>(base + used) : nullptr;
@@ -555,9 +755,10 @@ struct Carver {
 
 // The compact fill's tables and vectors, sized for every node unknown.
 struct CompactFill {
-    int *list, *pos_of, *nbr, *count;
+    int *list, *pos_of, *nbr, *count, *iterations;
     uint8_t* meta;
     double *bc, *Xc;
+    SmallCtl* small;
     void take(Carver& c, int64_t N) {
         list = c.take<int>(N);
         pos_of = c.take<int>(N);
@@ -566,6 +767,8 @@ struct CompactFill {
         bc = c.take<double>(3 * N);
         Xc = c.take<double>(3 * N);
         count = c.take<int>(1);
+        iterations = c.take<int>(2);
+        small = c.take<SmallCtl>(1);
     }
 };
 
@@ -617,8 +820,9 @@ int check_node_grid(int nx, int ny, int nz) {
 }
 
 // Spring fill of b (all nodes) into X (all nodes, zero on entry) over the unknown nodes only:
-// list them, build the compact tables, solve, scatter back. `skip` (nullable, device) switches the
-// whole fill off; the solve's arguments are those of pcg::solve.
+// list them, build the compact tables, solve (in one workgroup when they are few, else by
+// pcg::solve, which the one-workgroup kernel gates off when it has solved), scatter back. `skip`
+// (nullable, device) switches the whole fill off; the other arguments are those of pcg::solve.
 hipError_t fill_solve(const SpringOp& op, int64_t N, const double* b, double* X, const CompactFill& cf, double tol,
                       int max_iter, void* pcg_ws, const int* skip, int sync_every, int* iterations_host,
                       int* iterations_dev, int* capped, hipStream_t st) {
@@ -628,12 +832,23 @@ hipError_t fill_solve(const SpringOp& op, int64_t N, const double* b, double* X,
                        skip);
     hipLaunchKernelGGL(k_fill_tables, dim3(nbc), dim3(ABLOCK), 0, st, op, (const int*)cf.list,
                        (const int*)cf.pos_of, (const int*)cf.count, b, cf.meta, cf.nbr, cf.bc, cf.Xc, skip);
+    hipLaunchKernelGGL(k_fill_small, dim3(3), dim3(SMALL_THREADS), 0, st, (const int*)cf.count,
+                       (const uint8_t*)cf.meta, (const int*)cf.nbr, (const double*)cf.bc, cf.Xc, tol, max_iter, skip,
+                       cf.small);
     const CompactSpringOp cop{cf.meta, cf.nbr};
-    hipError_t e = pcg::solve(cop, N, (const double*)cf.bc, cf.Xc, tol, max_iter, pcg_ws, skip, sync_every,
-                              iterations_host, iterations_dev, st, capped, (const int*)cf.count, nbc);
+    hipError_t e = pcg::solve(cop, N, (const double*)cf.bc, cf.Xc, tol, max_iter, pcg_ws, &cf.small->gate,
+                              sync_every, nullptr, cf.iterations, st, capped, (const int*)cf.count, nbc);
     if (e != hipSuccess) return e;
+    int* out = iterations_dev ? iterations_dev : cf.iterations + 1;
+    hipLaunchKernelGGL(k_fill_result, dim3(1), dim3(64), 0, st, (const SmallCtl*)cf.small, (const int*)cf.iterations,
+                       out, capped);
     hipLaunchKernelGGL(k_fill_scatter, dim3(nbc), dim3(ABLOCK), 0, st, (const int*)cf.list, (const int*)cf.count,
                        (const double*)cf.Xc, X, skip);
+    if (iterations_host) {
+        e = hipMemcpyAsync(iterations_host, out, sizeof(int), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) return e;
+    }
     return hipGetLastError();
 }
 

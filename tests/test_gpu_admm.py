@@ -46,7 +46,7 @@ def _edges(shape, seed, frac=0.15):
 
 
 @pytest.mark.parametrize("cut", [False, True])
-@pytest.mark.parametrize("shape,holes", [((9, 11, 13), 0.3), ((6, 7, 5), 0.8), ((1, 12, 17), 0.25)])
+@pytest.mark.parametrize("shape,holes", [((9, 11, 13), 0.3), ((6, 7, 5), 0.8), ((1, 12, 17), 0.25), ((24, 30, 36), 0.5)])
 def test_spring_fill_matches_host(shape, holes, cut):
     rng = np.random.default_rng(3)
     vals = _field(shape, 1, 0.0)
@@ -57,6 +57,23 @@ def test_spring_fill_matches_host(shape, holes, cut):
     got = A.spring_fill(vals, shape, edge_ok=edge)
     for c in range(3):
         ref = C.fill_nan_grid(vals[:, c].reshape(shape), edge_ok=edge).ravel()
+        assert np.max(np.abs(got[:, c] - ref)) < 1e-8 * max(1.0, np.max(np.abs(ref))), c
+    known = ~np.isnan(vals)
+    assert np.array_equal(got[known], vals[known])
+
+
+@pytest.mark.parametrize("n_unknown", [7168, 7169])
+def test_spring_fill_either_side_of_the_one_workgroup_limit(n_unknown):
+    """Up to 7,168 unknown nodes the fill runs in one workgroup per column, beyond that over the
+    whole grid's launch shape: both against the oracle."""
+    shape = (20, 24, 30)
+    N = int(np.prod(shape))
+    rng = np.random.default_rng(n_unknown)
+    vals = _field(shape, 2, 0.0)
+    vals[rng.choice(N, n_unknown, replace=False)] = np.nan
+    got = A.spring_fill(vals, shape)
+    for c in range(3):
+        ref = C.fill_nan_grid(vals[:, c].reshape(shape)).ravel()
         assert np.max(np.abs(got[:, c] - ref)) < 1e-8 * max(1.0, np.max(np.abs(ref))), c
     known = ~np.isnan(vals)
     assert np.array_equal(got[known], vals[known])
@@ -73,9 +90,8 @@ def _host_cleanup(U, U_hat, status, shape, threshold, edge):
 
 
 @pytest.mark.parametrize("cut", [False, True])
-@pytest.mark.parametrize("seed", [0, 1, 2])
-def test_cleanup_matches_host_median_test(seed, cut):
-    shape = (10, 12, 14)
+@pytest.mark.parametrize("seed,shape", [(0, (10, 12, 14)), (1, (10, 12, 14)), (2, (10, 12, 14)), (3, (24, 30, 36))])
+def test_cleanup_matches_host_median_test(seed, shape, cut):
     N = int(np.prod(shape))
     rng = np.random.default_rng(seed)
     U = _field(shape, seed)
