@@ -94,6 +94,23 @@ __device__ inline void total_partials(const double* part, int nparts, double out
     __syncthreads();  // red is reused by the caller
 }
 
+// This thread's share of nparts partials (K sums each) written by an earlier kernel, added in block
+// order; the loads are issued at once, so a kernel can start them before anything it waits for.
+template <int K>
+__device__ inline void load_partials(const double* part, int nparts, double s[K]) {
+    for (int k = 0; k < K; ++k) s[k] = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += BLOCK)
+        for (int k = 0; k < K; ++k) s[k] += part[K * i + k];
+}
+
+// The totals of the shares: every block gets the same bits.
+template <int K>
+__device__ inline void sum_shares(const double s[K], double out[K], double (*red)[BLOCK]) {
+    block_sum<K>(s, red);
+    for (int k = 0; k < K; ++k) out[k] = red[k][0];
+    __syncthreads();  // red is reused by the caller
+}
+
 // (kernels that are not templates have internal linkage: every source including this has its own)
 static __global__ void k_reset(Ctl* ctl, const int* gate) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
@@ -134,13 +151,6 @@ __global__ __launch_bounds__(BLOCK) void k_setup(Op op, int64_t N, const int* __
     write_partials<6>(s, part2, red);
 }
 
-// Convergence of iteration k (k >= 1) from iter2(k-1)'s partials of <R, R>.
-__device__ inline bool converged(const double t[6], const Ctl* ctl, double tol) {
-    bool done = true;
-    for (int c = 0; c < 3; ++c) done = done && (sqrt(t[3 + c]) / ctl->bnorm[c] < tol);
-    return done;
-}
-
 // iteration slot k: the stopping test of iteration k, then P = Z + beta P_old, AP = A P
 template <class Op>
 __global__ __launch_bounds__(BLOCK) void k_iter1(Op op, int64_t N, const int* __restrict__ n_dev, int k,
@@ -148,11 +158,20 @@ __global__ __launch_bounds__(BLOCK) void k_iter1(Op op, int64_t N, const int* __
                                                  const double* __restrict__ Pold, double* __restrict__ Pnew,
                                                  double* __restrict__ AP, const double* __restrict__ part2,
                                                  double* __restrict__ part1, Ctl* ctl, double tol) {
-    if (ctl->done) return;
+    // the reads that depend on nothing of this kernel first, so that their latencies overlap
+    const int done = ctl->done;
     if (n_dev) N = *n_dev;
+    double sh[6];
+    load_partials<6>(part2, gridDim.x, sh);
+    double rz_prev[3], bn[3];
+    for (int c = 0; c < 3; ++c) {
+        rz_prev[c] = ctl->rz[(k - 1) & 1][c];  // the slot of iteration k - 1 (unused when k = 0)
+        bn[c] = ctl->bnorm[c];
+    }
+    if (done) return;
     __shared__ double red[6][BLOCK];
     double t[6];
-    total_partials<6>(part2, gridDim.x, t, red);
+    sum_shares<6>(sh, t, red);
     double be[3] = {0.0, 0.0, 0.0};
     if (k == 0) {  // t = <R, Z>, <B, B> of the setup
         if (blockIdx.x == 0 && threadIdx.x == 0)
@@ -161,17 +180,16 @@ __global__ __launch_bounds__(BLOCK) void k_iter1(Op op, int64_t N, const int* __
                 ctl->bnorm[c] = t[3 + c] > 0.0 ? sqrt(t[3 + c]) : 1.0;  // a zero column counts as 1
             }
     } else {  // t = <R, Z>, <R, R> after iteration k
-        if (converged(t, ctl, tol)) {
+        bool conv = true;
+        for (int c = 0; c < 3; ++c) conv = conv && (sqrt(t[3 + c]) / bn[c] < tol);
+        if (conv) {
             if (blockIdx.x == 0 && threadIdx.x == 0) {
                 ctl->iterations = k;
                 ctl->done = 1;
             }
             return;
         }
-        for (int c = 0; c < 3; ++c) {
-            const double rz = ctl->rz[(k - 1) & 1][c];
-            be[c] = rz == 0.0 ? 0.0 : t[c] / rz;
-        }
+        for (int c = 0; c < 3; ++c) be[c] = rz_prev[c] == 0.0 ? 0.0 : t[c] / rz_prev[c];
         if (blockIdx.x == 0 && threadIdx.x == 0)
             for (int c = 0; c < 3; ++c) ctl->rz[k & 1][c] = t[c];
     }
@@ -198,24 +216,51 @@ static __global__ __launch_bounds__(BLOCK) void k_iter2(int64_t N, const int* __
                                                         double* __restrict__ R, double* __restrict__ Z,
                                                         const double* __restrict__ part1, double* __restrict__ part2,
                                                         const Ctl* ctl) {
-    if (ctl->done) return;
-    if (n_dev) N = *n_dev;
+    // the reads that depend on nothing of this kernel first (the partials, the control block and
+    // this thread's first node, in memory for every node below the host's N), so that their
+    // latencies overlap; each later node is read while the one before is updated
+    const int done = ctl->done;
+    const int64_t n = n_dev ? *n_dev : N;
+    double sh[3];
+    load_partials<3>(part1, gridDim.x, sh);
+    double rz[3];
+    for (int c = 0; c < 3; ++c) rz[c] = ctl->rz[k & 1][c];
+    const int64_t stride = (int64_t)gridDim.x * BLOCK;
+    int64_t a = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    double vp[3], vap[3], vd[3], vx[3], vr[3];
+    auto fetch = [&](int64_t b) {
+        for (int c = 0; c < 3; ++c) {
+            const int64_t i = 3 * b + c;
+            vp[c] = P[i];
+            vap[c] = AP[i];
+            vd[c] = inv_diag[i];
+            vx[c] = X[i];
+            vr[c] = R[i];
+        }
+    };
+    if (a < N) fetch(a);
+    if (done) return;
     __shared__ double red[6][BLOCK];
     double t[3];
-    total_partials<3>(part1, gridDim.x, t, red);
+    sum_shares<3>(sh, t, red);
     double al[3];
-    for (int c = 0; c < 3; ++c) al[c] = t[c] == 0.0 ? 0.0 : ctl->rz[k & 1][c] / t[c];
+    for (int c = 0; c < 3; ++c) al[c] = t[c] == 0.0 ? 0.0 : rz[c] / t[c];
     double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    for (int64_t a = (int64_t)blockIdx.x * BLOCK + threadIdx.x; a < N; a += (int64_t)gridDim.x * BLOCK) {
+    for (; a < n; a += stride) {
+        double xo[3], ro[3], zo[3];
+        for (int c = 0; c < 3; ++c) {
+            xo[c] = vx[c] + vp[c] * al[c];
+            ro[c] = vr[c] - vap[c] * al[c];
+            zo[c] = vd[c] * ro[c];
+            s[c] += ro[c] * zo[c];
+            s[3 + c] += ro[c] * ro[c];
+        }
+        if (a + stride < n) fetch(a + stride);
         for (int c = 0; c < 3; ++c) {
             const int64_t i = 3 * a + c;
-            X[i] += P[i] * al[c];
-            const double r = R[i] - AP[i] * al[c];
-            R[i] = r;
-            const double z = inv_diag[i] * r;
-            Z[i] = z;
-            s[c] += r * z;
-            s[3 + c] += r * r;
+            X[i] = xo[c];
+            R[i] = ro[c];
+            Z[i] = zo[c];
         }
     }
     write_partials<6>(s, part2, red);
