@@ -182,7 +182,10 @@ __host__ __device__ inline int64_t small_lds_floats(int sx, int sy, int sz, int 
     return (int64_t)sx * sy * sz + wx * wy * wz + (xs > part ? xs : part) + 2 * wz * vy * vx;
 }
 
-template <int RX, int NT>
+// SX > 0: the template width known at compile time (it must equal 2 hx + 1), so the sliding
+// register window of a template row unrolls into plain reuse of one row of loads, summed in the
+// same order as the general loop.
+template <int RX, int NT, int SX = 0>
 __global__ __launch_bounds__(NT) void k_zncc_map_small(NccArgs a) {
     extern __shared__ float sm[];
     const int64_t node = blockIdx.x;
@@ -260,16 +263,28 @@ __global__ __launch_bounds__(NT) void k_zncc_map_small(NccArgs a) {
         for (int ty = 0; ty < sy; ++ty) {
             const float* wrow = win + ((oz + tz) * wy + oy + ty) * wx;
             const float* trow = tpl + (tz * sy + ty) * sx;
-            float w[RX];
+            if constexpr (SX > 0) {
+                float w[SX + RX - 1];
 #pragma unroll
-            for (int q = 0; q < RX - 1; ++q) w[q] = wrow[q];
-            for (int tx = 0; tx < sx; ++tx) {
-                w[RX - 1] = wrow[tx + RX - 1];
-                const float t = trow[tx];
+                for (int i = 0; i < SX + RX - 1; ++i) w[i] = wrow[i];
 #pragma unroll
-                for (int q = 0; q < RX; ++q) acc[q] += t * w[q];
+                for (int tx = 0; tx < SX; ++tx) {
+                    const float t = trow[tx];
 #pragma unroll
-                for (int q = 0; q < RX - 1; ++q) w[q] = w[q + 1];
+                    for (int q = 0; q < RX; ++q) acc[q] += t * w[tx + q];
+                }
+            } else {
+                float w[RX];
+#pragma unroll
+                for (int q = 0; q < RX - 1; ++q) w[q] = wrow[q];
+                for (int tx = 0; tx < sx; ++tx) {
+                    w[RX - 1] = wrow[tx + RX - 1];
+                    const float t = trow[tx];
+#pragma unroll
+                    for (int q = 0; q < RX; ++q) acc[q] += t * w[q];
+#pragma unroll
+                    for (int q = 0; q < RX - 1; ++q) w[q] = w[q + 1];
+                }
             }
         }
         float* dst = part + tz * n_off + (oz * vy + oy) * RX;
@@ -1430,23 +1445,35 @@ int aldvc_zncc_map(const float* f, const float* g, int64_t nz, int64_t ny, int64
         // two workgroups per CU at the most (LDS): 512 lanes each when the (template plane, output
         // row) items fit one round of them, 256 otherwise
         const bool wide = sz * (2 * rz + 1) * (2 * ry + 1) <= 512 && sz * (2 * rz + 1) * (2 * ry + 1) > 256;
-#define ALDVC_SMALL_NT(RXV, NTV)                                                                          \
+#define ALDVC_SMALL_NT(RXV, NTV, SXV)                                                                     \
         {                                                                                                    \
-            const void* fn = (const void*)k_zncc_map_small<RXV, NTV>;                                        \
+            const void* fn = (const void*)k_zncc_map_small<RXV, NTV, SXV>;                                   \
             if (slds > 64 * 1024 && hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)slds) != hipSuccess) \
                 return fail(-5, "could not raise the LDS limit for the window");                             \
-            hipLaunchKernelGGL((k_zncc_map_small<RXV, NTV>), dim3((unsigned)n), dim3(NTV), slds, stream, a); \
+            hipLaunchKernelGGL((k_zncc_map_small<RXV, NTV, SXV>), dim3((unsigned)n), dim3(NTV), slds, stream, a); \
             return launch_status();                                                                          \
         }
 #define ALDVC_SMALL(RXV)                                                                                  \
         if (vx == RXV && slds <= SMALL_LDS_MAX) {                                                             \
-            if (wide) ALDVC_SMALL_NT(RXV, 512)                                                                \
-            ALDVC_SMALL_NT(RXV, 256)                                                                          \
+            if (wide) ALDVC_SMALL_NT(RXV, 512, 0)                                                             \
+            ALDVC_SMALL_NT(RXV, 256, 0)                                                                       \
         }
+        // the pyramid's templates (16^3 at full resolution, 8^3 on the coarse levels: 17 and 9 wide)
+        // with its radii (2, doubled to 4)
+#define ALDVC_SMALL_SX(RXV, SXV)                                                                          \
+        if (vx == RXV && sx == SXV && slds <= SMALL_LDS_MAX) {                                                \
+            if (wide) ALDVC_SMALL_NT(RXV, 512, SXV)                                                           \
+            ALDVC_SMALL_NT(RXV, 256, SXV)                                                                     \
+        }
+        ALDVC_SMALL_SX(5, 17)
+        ALDVC_SMALL_SX(5, 9)
+        ALDVC_SMALL_SX(9, 17)
+        ALDVC_SMALL_SX(9, 9)
         ALDVC_SMALL(3)
         ALDVC_SMALL(5)
         ALDVC_SMALL(7)
         ALDVC_SMALL(9)
+#undef ALDVC_SMALL_SX
 #undef ALDVC_SMALL
 #undef ALDVC_SMALL_NT
     }
