@@ -413,8 +413,10 @@ size_t aldvc_spring_fill_workspace_bytes(int nx, int ny, int nz);
 
 /* Spring-model fill of the NaN entries of vals (N, 3) on the node grid, column by column, in place:
  * the discrete Laplace equation over 6-neighbour springs (none across cut edges when edge_ok is
- * given), known nodes as data, a 1e-10 Tikhonov term. Jacobi-CG to tol (relative), at most
- * max_iter iterations; *iterations (host) receives the count. Replaces utils/inpaint.py:35-121
+ * given), known nodes as data, a 1e-10 Tikhonov term. Jacobi-CG over the unknown nodes to tol
+ * (relative), at most max_iter iterations: with at most 7,168 unknown nodes each column is its own
+ * solve in one launch, else the three columns run together until all meet tol; *iterations (host)
+ * receives the count (the largest over the columns). Replaces utils/inpaint.py:35-121
  * (fill_nan_grid, "spring"). */
 int aldvc_spring_fill(int nx, int ny, int nz, const uint8_t *edge_ok, double *vals, double tol, int max_iter,
                       int *iterations, void *workspace, size_t workspace_bytes, hipStream_t stream);
