@@ -41,6 +41,11 @@ __device__ inline void fill_map(float* o, int64_t n_off, float v) {
     for (int64_t i = threadIdx.x; i < n_off; i += NT) o[i] = v;
 }
 
+// floor(n / d) for 0 <= n < 2^22 from inv = 1.0f / d: (n + 0.5) / d lies at least 0.5 / d from an
+// integer, and the float rounding of the product (under (n + 0.5) 2^-23 / d) stays inside that
+// margin, so the truncation is exact.
+__device__ inline int qdiv(int n, float inv) { return (int)(((float)n + 0.5f) * inv); }
+
 // The node's template into LDS, centred by its mean: float64 mean and energy (sum T~^2). Returns
 // false for a flat template (energy below 1e-20). Block-uniform; every thread must call it.
 template <int NT = BLOCK>
@@ -50,8 +55,9 @@ __device__ inline bool centred_template(const NccArgs& a, float* tpl, int64_t x0
     __shared__ double bc[2];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     double s = 0.0;
+    const float inv_plane = 1.0f / (float)(sy * sx), inv_sx = 1.0f / (float)sx;
     for (int v = tid; v < n_tpl; v += NT) {
-        const int k = v / (sy * sx), r = v - k * sy * sx, j = r / sx, i = r - j * sx;
+        const int k = qdiv(v, inv_plane), r = v - k * sy * sx, j = qdiv(r, inv_sx), i = r - j * sx;
         const float val = a.f[((z0 - a.hz + k) * a.ny + (y0 - a.hy + j)) * (int64_t)a.nx + (x0 - a.hx + i)];
         tpl[v] = val;
         s += val;
@@ -221,8 +227,9 @@ __global__ __launch_bounds__(NT) void k_zncc_map_small(NccArgs a) {
     }
     const float c = mf;  // window pilot: sum T~ (g - c) = sum T~ g because sum T~ = 0
     const int n_win = wz * wy * wx;
+    const float inv_wplane = 1.0f / (float)(wy * wx), inv_wx = 1.0f / (float)wx;
     for (int v = tid; v < n_win; v += NT) {
-        const int k = v / (wy * wx), r = v - k * wy * wx, j = r / wx, i = r - j * wx;
+        const int k = qdiv(v, inv_wplane), r = v - k * wy * wx, j = qdiv(r, inv_wx), i = r - j * wx;
         win[v] = a.g[((zb + k) * a.ny + (yb + j)) * (int64_t)a.nx + (xb + i)] - c;
     }
     __syncthreads();
@@ -241,8 +248,9 @@ __global__ __launch_bounds__(NT) void k_zncc_map_small(NccArgs a) {
     __syncthreads();
     // y pass: sums over sy consecutive rows
     const int n_y = wz * vy * RX;
+    const float inv_yrow = 1.0f / (float)(vy * RX);
     for (int v = tid; v < n_y; v += NT) {
-        const int z = v / (vy * RX), r = v - z * vy * RX, oy = r / RX, ox = r - oy * RX;
+        const int z = qdiv(v, inv_yrow), r = v - z * vy * RX, oy = r / RX, ox = r - oy * RX;
         const int b = (z * wy + oy) * RX + ox;
         float p1 = 0.0f, p2 = 0.0f;
         for (int ty = 0; ty < sy; ++ty) {
@@ -255,8 +263,9 @@ __global__ __launch_bounds__(NT) void k_zncc_map_small(NccArgs a) {
     __syncthreads();
     // numerators: item (tz, oz, oy), RX outputs along x
     const int items = sz * vz * vy;
+    const float inv_zy = 1.0f / (float)(vz * vy), inv_vy = 1.0f / (float)vy;
     for (int it = tid; it < items; it += NT) {
-        const int tz = it / (vz * vy), r = it - tz * vz * vy, oz = r / vy, oy = r - oz * vy;
+        const int tz = qdiv(it, inv_zy), r = it - tz * vz * vy, oz = qdiv(r, inv_vy), oy = r - oz * vy;
         float acc[RX];
 #pragma unroll
         for (int q = 0; q < RX; ++q) acc[q] = 0.0f;
@@ -295,7 +304,7 @@ __global__ __launch_bounds__(NT) void k_zncc_map_small(NccArgs a) {
     const float inv_n = 1.0f / (float)n_tpl;
     const float sqrt_ssf = (float)sqrt(ssf);
     for (int off = tid; off < n_off; off += NT) {
-        const int oz = off / (vy * RX), r = off - oz * vy * RX;  // r = oy * RX + ox
+        const int oz = qdiv(off, inv_yrow), r = off - oz * vy * RX;  // r = oy * RX + ox
         float num = 0.0f, s1 = 0.0f, s2 = 0.0f;
         for (int tz = 0; tz < sz; ++tz) {
             num += part[tz * n_off + off];
