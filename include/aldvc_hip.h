@@ -338,7 +338,8 @@ size_t aldvc_admm_global_workspace_bytes(int nx, int ny, int nz);
  * device-controlled PCG (inactive nodes keep U), F2 = nodal gradient of U2 (NaN at inactive
  * nodes). U, v, U2 (N, 3); F, W, F2 (N, 3, 3); Ae = (beta + alpha) K + mu M (8 x 8).
  * sync_every = 0: all max_iter CG slots are enqueued (graph capture); > 0: the host reads the stop
- * flag every sync_every iterations and enqueues no more after convergence (eager callers).
+ * flag after sync_every iterations, then at intervals doubling up to 4 sync_every, and enqueues no
+ * more after convergence (eager callers).
  * capped (device int, nullable) is set to 1 when the solve used all max_iter iterations, so a
  * graph recorded with a smaller iteration budget can be checked after replay.
  * Replaces solver/subpb2_solver.py:108-155 (global_rhs, solve_subpb2) followed by
@@ -497,9 +498,9 @@ typedef struct AldvcAdmmLoop {
     size_t ws_icgn3_bytes, ws_cleanup_bytes, ws_global_bytes, ws_dual_bytes;
 } AldvcAdmmLoop;
 
-/* Enqueue the whole loop on `stream`. sync_every > 0: the CG solves are polled by the host every
- * sync_every iterations (eager callers; the call then waits on the stream); 0: nothing is read back
- * and every CG slot of the budgets is enqueued, so the sequence can be captured. */
+/* Enqueue the whole loop on `stream`. sync_every > 0: the CG solves are polled by the host as
+ * aldvc_admm_global_step polls them (eager callers; the call then waits on the stream); 0: nothing
+ * is read back and every CG slot of the budgets is enqueued, so the sequence can be captured. */
 int aldvc_admm_loop_enqueue(const AldvcAdmmLoop *loop, int sync_every, hipStream_t stream);
 
 /* Record aldvc_admm_loop_enqueue(loop, 0, .) into a HIP graph by capturing `capture_stream` (a

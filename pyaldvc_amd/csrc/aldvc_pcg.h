@@ -309,7 +309,8 @@ inline Workspace carve(void* ws, int64_t N) {
 }
 
 // Enqueue a whole solve of A X = B from the X given. With `sync_every` > 0 the host reads the
-// control block after every `sync_every` iterations and stops enqueueing once it is done (the
+// control block after `sync_every` iterations, then at intervals that double up to 4 `sync_every`,
+// and stops enqueueing once it is done (the
 // iteration count is returned in *iterations_host); with 0 all `max_iter` iteration slots are
 // enqueued (those after convergence return at once) and nothing is read back, so the sequence can
 // be captured into a graph; *iterations_dev, when given, receives the count on the device.
@@ -325,6 +326,7 @@ inline hipError_t solve(const Op& op, int64_t N, const double* B, double* X, dou
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, w.ctl, gate);
     hipLaunchKernelGGL(k_setup<Op>, dim3(nb), dim3(BLOCK), 0, st, op, N, n_dev, B, (const double*)X, w.R, w.Z, w.P[1],
                        w.inv_diag, w.part2, (const Ctl*)w.ctl);
+    int next_poll = sync_every, interval = sync_every;
     for (int k = 0; k < max_iter; ++k) {
         double* Pold = w.P[(k + 1) & 1];
         double* Pnew = w.P[k & 1];
@@ -333,7 +335,9 @@ inline hipError_t solve(const Op& op, int64_t N, const double* B, double* X, dou
         hipLaunchKernelGGL(k_iter2, dim3(nb), dim3(BLOCK), 0, st, N, n_dev, k, (const double*)Pnew, (const double*)w.AP,
                            (const double*)w.inv_diag, X, w.R, w.Z, (const double*)w.part1, w.part2,
                            (const Ctl*)w.ctl);
-        if (sync_every > 0 && (k + 1) % sync_every == 0 && k + 1 < max_iter) {
+        if (sync_every > 0 && k + 1 == next_poll && k + 1 < max_iter) {
+            interval = interval < 4 * sync_every ? 2 * interval : interval;
+            next_poll += interval;
             int done = 0;
             hipError_t e = hipMemcpyAsync(&done, &w.ctl->done, sizeof(int), hipMemcpyDeviceToHost, st);
             if (e == hipSuccess) e = hipStreamSynchronize(st);
