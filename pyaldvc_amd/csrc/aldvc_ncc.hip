@@ -219,6 +219,18 @@ __global__ __launch_bounds__(NT) void k_zncc_map_small(NccArgs a) {
     const int n_x = wz * wy * RX, n_part = sz * n_off;
     float* ys1 = xs1 + (2 * n_x > n_part ? 2 * n_x : n_part);  // wz x vy x RX box sums over (x, y)
     float* ys2 = ys1 + wz * vy * RX;
+    // the window's first WPF values per lane are read before the template's reductions, so their
+    // latency overlaps them; they are centred and stored once the template mean is known
+    constexpr int WPF = 20;
+    const int n_win = wz * wy * wx;
+    const float inv_wplane = 1.0f / (float)(wy * wx), inv_wx = 1.0f / (float)wx;
+    auto gwin = [&](int v) {
+        const int k = qdiv(v, inv_wplane), r = v - k * wy * wx, j = qdiv(r, inv_wx), i = r - j * wx;
+        return a.g[((zb + k) * a.ny + (yb + j)) * (int64_t)a.nx + (xb + i)];
+    };
+    float pre[WPF];
+#pragma unroll
+    for (int q = 0; q < WPF; ++q) pre[q] = tid + q * NT < n_win ? gwin(tid + q * NT) : 0.0f;
     double ssf;
     float mf;
     if (!centred_template<NT>(a, tpl, x0, y0, z0, sx, sy, n_tpl, mf, ssf)) {
@@ -226,12 +238,10 @@ __global__ __launch_bounds__(NT) void k_zncc_map_small(NccArgs a) {
         return;
     }
     const float c = mf;  // window pilot: sum T~ (g - c) = sum T~ g because sum T~ = 0
-    const int n_win = wz * wy * wx;
-    const float inv_wplane = 1.0f / (float)(wy * wx), inv_wx = 1.0f / (float)wx;
-    for (int v = tid; v < n_win; v += NT) {
-        const int k = qdiv(v, inv_wplane), r = v - k * wy * wx, j = qdiv(r, inv_wx), i = r - j * wx;
-        win[v] = a.g[((zb + k) * a.ny + (yb + j)) * (int64_t)a.nx + (xb + i)] - c;
-    }
+#pragma unroll
+    for (int q = 0; q < WPF; ++q)
+        if (tid + q * NT < n_win) win[tid + q * NT] = pre[q] - c;
+    for (int v = tid + WPF * NT; v < n_win; v += NT) win[v] = gwin(v) - c;
     __syncthreads();
     // x pass: sums over sx consecutive columns of every window row
     for (int v = tid; v < n_x; v += NT) {
